@@ -1,0 +1,167 @@
+/* krome.h -- the C ABI of a network library built by this project
+ * (libkrome_<network>.so): the batched per-cell chemistry + Tgas solve on
+ * an AMD MI355X (gfx950).  The signatures, flag values and option keys are
+ * those of KROME-X's generated C interface, so a caller of the reference's
+ * libkrome links against this one unchanged; each declaration cites the
+ * reference declaration it replaces (runtime/interfaces/C/krome.h) and the
+ * implementation (runtime/interfaces/C/krome.cpp).
+ *
+ * Differences of this build (not of the signatures):
+ *  - every solve runs on the GPU: option path 0 (auto) and 2 (device) take
+ *    the device for any number of cells, path 1 (per-cell host loop)
+ *    returns KROME_FLAG_UNSUPPORTED; a cell that fails on the device keeps
+ *    its input and its flag (no host re-solve);
+ *  - conserve mode 2, linsol 2 (LAPACK) and 3 (sparse) return
+ *    KROME_FLAG_UNSUPPORTED; networks without photo or dust bins only;
+ *  - extra option keys select this build's kernels: device_kernel (0 the
+ *    plain FP64 kernel, 1 the persistent batched kernel), mixed_precision
+ *    (persistent kernel: 1 = FP32 Jacobian, LU and triangular solves with
+ *    the FP64 Newton residual, state and error test), queue_chunk and
+ *    waves_per_cu (persistent kernel tuning; 0 = automatic).
+ */
+#ifndef KROME_MI355X_H
+#define KROME_MI355X_H
+
+/* the integrator's failures (CVODE's values; krome.h:62-75) */
+#define KROME_FLAG_TOO_MUCH_WORK (-1)
+#define KROME_FLAG_TOO_MUCH_ACC (-2)
+#define KROME_FLAG_ERR_FAILURE (-3)
+#define KROME_FLAG_CONV_FAILURE (-4)
+#define KROME_FLAG_LSETUP_FAIL (-6)
+#define KROME_FLAG_LSOLVE_FAIL (-7)
+#define KROME_FLAG_RHSFUNC_FAIL (-8)
+#define KROME_FLAG_FIRST_RHSFUNC_ERR (-9)
+#define KROME_FLAG_REPTD_RHSFUNC_ERR (-10)
+#define KROME_FLAG_UNREC_RHSFUNC_ERR (-11)
+#define KROME_FLAG_NLS_FAIL (-16)
+#define KROME_FLAG_ILL_INPUT (-22)
+#define KROME_FLAG_TOO_CLOSE (-27)
+/* KROME's flags (krome.h:76-89) */
+#define KROME_FLAG_OK 0
+#define KROME_FLAG_NO_DUST (-102)
+#define KROME_FLAG_NO_FIELD (-103)
+#define KROME_FLAG_UNSUPPORTED (-104)
+#define KROME_FLAG_PARAM_UNSET (-105)
+#define KROME_FLAG_CONSERVE_GAS_TOTAL (-106)
+#define KROME_FLAG_SURFACE_BINS (-107)
+#define KROME_FLAG_NO_LAPACK (-108)
+#define KROME_FLAG_NO_KOKKOS (-109)
+#define KROME_FLAG_NONPOSITIVE_T (-110)
+#define KROME_FLAG_DEVICE_MEMORY (-111)
+#define KROME_FLAG_CONSERVE_UNREACHED (-112)
+#define KROME_FLAG_CONSERVE_LARGE 2
+#define KROME_FLAG_HOST_RECOVERED 1
+
+/* species-array layouts of the device-resident calls (krome.h:326-327) */
+#define KROME_LAYOUT_SPECIES_MAJOR 0
+#define KROME_LAYOUT_CELL_MAJOR 1
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* a one-line description of a flag, never NULL (krome.h:95;
+ * krome.cpp krome_flag_message) */
+const char* krome_flag_message(int flag);
+
+/* network description (krome.h:98-113) */
+const char* krome_version(void);
+const char* krome_network_name(void);
+int krome_nspec(void);
+int krome_nrea(void);
+int krome_npar(void);
+const char* krome_species_name(int i);
+int krome_species_index(const char* name);
+const char* krome_param_name(int i);
+int krome_param_index(const char* name);
+void krome_params_default(double* par);
+const char* krome_param_required_by(int i);
+
+/* solver options, an opaque handle set by key (krome.h:116-173): the keys
+ * of krome.h plus this build's device_kernel, mixed_precision,
+ * queue_chunk, waves_per_cu; -1 for an unknown key or a bad value */
+typedef struct krome_opts krome_opts;
+krome_opts* krome_opts_new(void);
+void krome_opts_free(krome_opts* h);
+int krome_opts_set(krome_opts* h, const char* key, double value);
+int krome_opts_get(const krome_opts* h, const char* key, double* value);
+
+/* the device backend (krome.h:193-201): compiled in (1); initialize
+ * selects the HIP device (device_id < 0: the current one) */
+int krome_has_device(void);
+int krome_device_initialize(void);
+int krome_device_finalize(void);
+int krome_device_initialize_id(int device_id);
+
+/* the optional per-cell arrays of a multi-cell call (krome.h:275-279);
+ * unused by this build's networks (no photon or dust bins) */
+typedef struct {
+  double* Phi;
+  double* Td;
+  double* adust;
+} krome_cells;
+
+/* integrate ncell independent cells over their own dt[c] [s]: n[ncell]
+ * [nspec] and Tgas (with evolve_T) updated in place, per-cell parameter
+ * rows par (par_stride npar, or 0: one row for every cell; NULL: the
+ * defaults), flags[ncell] optional.  Returns 0 or the flag of the lowest
+ * failing cell; -105 (every cell flagged) when a REQUIRED parameter is
+ * unset, -1 on bad arguments (krome.h:291-294, krome.cpp:396-436) */
+int krome_solve_cells(int ncell, double* n, double* Tgas, const double* dt,
+                      const double* par, int par_stride,
+                      const krome_cells* extra, int* flags,
+                      const krome_opts* h);
+
+/* the same with mass fractions x[ncell][nspec] and rho[ncell] [g cm^-3]
+ * (krome.h:302-305, krome.cpp:438-470) */
+int krome_solve_cells_x(int ncell, double* x, const double* rho,
+                        double* Tgas, const double* dt, const double* par,
+                        int par_stride, const krome_cells* extra,
+                        int* flags, const krome_opts* h);
+
+/* device-resident arrays (krome.h:328-350, krome.cpp:1237-1293): n in
+ * the species-major (n[i ld + c], ld >= ncell, 0: ncell) or cell-major
+ * layout, Tgas, dt, par and flags in device memory; the device is
+ * synchronized before and after.  summary (host, may be NULL) counts the
+ * cells solved and failed.  The mass-fraction variant returns
+ * KROME_FLAG_UNSUPPORTED in this build. */
+typedef struct {
+  long device;
+  long host;
+  long failed;
+} krome_device_summary;
+int krome_solve_cells_device(int ncell, double* n, int layout, long ld,
+                             double* Tgas, const double* dt,
+                             const double* par, int par_stride,
+                             const krome_cells* extra, int* flags,
+                             krome_device_summary* summary,
+                             const krome_opts* h);
+int krome_solve_cells_x_device(int ncell, double* x, const double* rho,
+                               int layout, long ld, double* Tgas,
+                               const double* dt, const double* par,
+                               int par_stride, const krome_cells* extra,
+                               int* flags, krome_device_summary* summary,
+                               const krome_opts* h);
+
+/* This build's extension, no counterpart in the reference's C ABI: the
+ * device-resident solve with the per-cell counters of krome::SolveStats
+ * (runtime/krome.hpp:257-268; 9 per cell: steps, rhs_evals, jac_evals,
+ * rhs_evals_jac, lin_setups, newton_iters, newton_fails, err_fails,
+ * restarts) into the device array stats[ncell][9] (may be NULL), for
+ * benchmarks and tests.  par is required; the call is asynchronous on the
+ * null stream (the caller synchronizes). */
+int krome_solve_cells_device_stats(int ncell, double* n, int layout, long ld,
+                                   double* Tgas, const double* dt,
+                                   const double* par, int par_stride,
+                                   int* flags, long* stats,
+                                   const krome_opts* h);
+
+/* the name of a flag, e.g. "FLAG_TOO_MUCH_WORK" (the reference's
+ * krome::flag_name, runtime/core/solve_cell.hpp:106-137) */
+const char* krome_flag_name(int flag);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

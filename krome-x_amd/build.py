@@ -1,0 +1,64 @@
+"""Build the network libraries: one libkrome_<network>.so per configured
+network, hand-written HIP for gfx950 compiled with hipcc (in-tree).
+
+    python3 krome-x_amd/build.py [--regen]
+
+--regen regenerates the network headers from the .knet files first (needs
+the KROME-X network folder, KROME_NETWORKS or /root/reference/networks).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+CSRC = HERE / "csrc"
+GEN = CSRC / "generated"
+NETWORKS = ("primordial", "primordial3")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+         # no packed FP32 instructions (v_pk_*_f32) in any kernel
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-Wno-unused-result"]
+
+
+def lib_path(net):
+    return HERE / ("libkrome_%s.so" % net)
+
+
+def regen(networks_dir=None):
+    nd = networks_dir or os.environ.get("KROME_NETWORKS",
+                                        "/root/reference/networks")
+    if not Path(nd).is_dir():
+        return False
+    sys.path.insert(0, str(HERE))
+    import netgen
+    netgen.main(["--networks", str(nd), "--out", str(GEN)])
+    return True
+
+
+def build(net, verbose=False):
+    out = lib_path(net)
+    tmp = out.with_suffix(".so.tmp")
+    cmd = [HIPCC, *FLAGS, "-I", str(CSRC), "-I", str(GEN),
+           "-DKX_NET=net_%s" % net,
+           '-DKX_NET_HEADER="net_%s.h"' % net,
+           '-DKX_META_HEADER="net_%s_meta.h"' % net,
+           str(CSRC / "kx_abi.hip"), "-o", str(tmp)]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, out)
+    return out
+
+
+def build_all(verbose=False, regen_headers=False):
+    if regen_headers:
+        regen()
+    return [build(n, verbose) for n in NETWORKS]
+
+
+if __name__ == "__main__":
+    build_all(verbose=True, regen_headers="--regen" in sys.argv)

@@ -1,0 +1,114 @@
+"""Seeded synthetic cells of the benchmark workloads.
+
+A counter-based generator (splitmix64 of seed * 2^40 + 16 c + k) gives
+draw k of cell c as a uniform in [0, 1) independent of how many cells are
+made or in which order, so a sample of cells (the CPU baseline, the
+fixtures) is the same cells as in the full batch.  Log-uniform draws are
+10^(a + (b - a) u).
+
+Workloads (the species order of the generated networks):
+
+  primordial    4096 cells, seed 0: nH in 1..1e6 cm^-3, Tgas 10..1e4 K,
+                x = n(H+)/nH = n(e-)/nH in 1e-6..1e-2, y = n(H2)/nH in
+                1e-8..1e-3, He = 0.0775 nH, the other species 1e-20 nH,
+                H = nH (1 - x - 2y); dt = 3.15e10 s; fixed Tgas
+  primordial3   2^20 cells, seed 1: nH in 1..1e16, Tgas 10..3e3 K,
+                x in 1e-8..1e-3, y in 1e-8..0.5, He = 0.0775 nH, the other
+                species 1e-20 nH, H = nH max(1 - x - 2y, 1e-20);
+                dt = 0.01 tff, tff = sqrt(3 pi / (32 G rho)) per cell
+                (the compression time scale parameter), Tcmb 2.725,
+                opratio 3
+"""
+from __future__ import annotations
+
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+_META = _HERE / "csrc" / "generated" / "networks.json"
+GRAVITY = 6.6743e-08
+
+
+def network_meta(name):
+    return json.loads(_META.read_text())[name]
+
+
+def splitmix64(x):
+    x = (x + np.uint64(0x9E3779B97F4A7C15)).astype(np.uint64)
+    x = ((x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9))
+    x = ((x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB))
+    return x ^ (x >> np.uint64(31))
+
+
+def uniform(seed, cells, k):
+    """draw k of the given cells: uniform in [0, 1), 53 bits"""
+    c = np.asarray(cells, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        x = splitmix64(np.uint64(seed) * np.uint64(1 << 40)
+                       + c * np.uint64(16) + np.uint64(k))
+    return (x >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def log_uniform(seed, cells, k, a, b):
+    la, lb = math.log10(a), math.log10(b)
+    return 10.0 ** (la + (lb - la) * uniform(seed, cells, k))
+
+
+def _species_rows(meta, nH, x, y):
+    sp = meta["species"]
+    n = np.empty((nH.size, len(sp)))
+    n[:] = 1e-20 * nH[:, None]
+    col = {s: i for i, s in enumerate(sp)}
+    n[:, col["H+"]] = x * nH
+    n[:, col["e-"]] = x * nH
+    n[:, col["H2"]] = y * nH
+    n[:, col["He"]] = 0.0775 * nH
+    n[:, col["H"]] = nH * np.maximum(1.0 - x - 2.0 * y, 1e-20)
+    return n
+
+
+def primordial_cells(ncell=4096, seed=0, first=0):
+    """the fixed-Tgas primordial workload (cells first .. first+ncell-1)"""
+    meta = network_meta("primordial")
+    c = np.arange(first, first + ncell)
+    nH = log_uniform(seed, c, 0, 1.0, 1e6)
+    T = log_uniform(seed, c, 1, 10.0, 1e4)
+    x = log_uniform(seed, c, 2, 1e-6, 1e-2)
+    y = log_uniform(seed, c, 3, 1e-8, 1e-3)
+    n = _species_rows(meta, nH, x, y)
+    dt = np.full(ncell, 3.15e10)
+    par = np.tile(np.array(meta["param_defaults"], dtype=np.float64),
+                  (ncell, 1))
+    return {"n": n, "T": T, "dt": dt, "par": par,
+            "opts": {"evolve_T": 0, "rtol": 1e-6, "atol_n": 1e-20,
+                     "atol_T": 1e-6, "linsol": 1, "jac": 0,
+                     "max_steps": 100000}}
+
+
+def primordial3_cells(ncell=1 << 20, seed=1, first=0):
+    """the primordial3 collapse workload with Tgas evolved"""
+    meta = network_meta("primordial3")
+    c = np.arange(first, first + ncell)
+    nH = log_uniform(seed, c, 0, 1.0, 1e16)
+    T = log_uniform(seed, c, 1, 10.0, 3e3)
+    x = log_uniform(seed, c, 2, 1e-8, 1e-3)
+    y = log_uniform(seed, c, 3, 1e-8, 0.5)
+    n = _species_rows(meta, nH, x, y)
+    rho = n @ np.array(meta["mass_g"])
+    tff = np.sqrt(3.0 * math.pi / (32.0 * GRAVITY * np.maximum(rho, 1e-40)))
+    dt = 0.01 * tff
+    pn = meta["params"]
+    par = np.empty((ncell, len(pn)))
+    for j, p in enumerate(pn):
+        par[:, j] = meta["param_defaults"][j]
+    par[:, pn.index("tff")] = tff
+    return {"n": n, "T": T, "dt": dt, "par": par,
+            "opts": {"evolve_T": 1, "rtol": 1e-8, "atol_n": 1e-20,
+                     "atol_T": 1e-6, "linsol": 1, "jac": 0,
+                     "max_steps": 100000}}
+
+
+WORKLOADS = {"primordial": primordial_cells, "primordial3": primordial3_cells}

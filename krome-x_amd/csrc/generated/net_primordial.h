@@ -1,0 +1,853 @@
+// generated by netgen.py from primordial.knet -- do not edit
+#pragma once
+#include "kx_physics.h"
+
+namespace kx {
+struct net_primordial {
+  static constexpr const char* NAME = "primordial";
+  static constexpr int NSPEC = 12;
+  static constexpr int NREA = 38;
+  static constexpr int NPAR = 2;
+  static constexpr int IDX_D = 0;
+  static constexpr int IDX_Dj = 1;
+  static constexpr int IDX_H = 2;
+  static constexpr int IDX_Hj = 3;
+  static constexpr int IDX_Hk = 4;
+  static constexpr int IDX_H2 = 5;
+  static constexpr int IDX_H2j = 6;
+  static constexpr int IDX_HD = 7;
+  static constexpr int IDX_He = 8;
+  static constexpr int IDX_Hej = 9;
+  static constexpr int IDX_Hejj = 10;
+  static constexpr int IDX_E = 11;
+  static constexpr bool HAS_E = true;
+  static constexpr int I_E = 11;
+  static constexpr bool THERMAL = false;
+  static constexpr bool RATES_DENSITY = false;
+  // parameter row (param_names order)
+  struct Params {
+    double gamma_ad;
+    double Tcmb;
+  };
+  KX_HD static Params params_from_row(const double* row) {
+    Params p;
+    p.gamma_ad = row[0];
+    p.Tcmb = row[1];
+    return p;
+  }
+  // index of the first REQUIRED parameter still NaN, or -1
+  KX_HD static int params_missing(const double* row) {
+    (void)row;
+    return -1;
+  }
+  KX_HD static int charge(int i) {
+    switch (i) {
+      case 1: return 1;
+      case 3: return 1;
+      case 4: return -1;
+      case 6: return 1;
+      case 9: return 1;
+      case 10: return 2;
+      case 11: return -1;
+      default: return 0;
+    }
+  }
+  KX_HD static double mass_g(int i) {
+    switch (i) {
+      case 0: return 3.3443256801323995e-24;
+      case 1: return 3.343414741762353e-24;
+      case 2: return 1.6738233791328e-24;
+      case 3: return 1.6729124407627538e-24;
+      case 4: return 1.6747343175028463e-24;
+      case 5: return 3.3476467582656e-24;
+      case 6: return 3.346735819895553e-24;
+      case 7: return 5.0181490592651994e-24;
+      case 8: return 6.64647366797316e-24;
+      case 9: return 6.645562729603113e-24;
+      case 10: return 6.644651791233067e-24;
+      case 11: return 9.10938370046373e-28;
+      default: return 0.0;
+    }
+  }
+  template <class R> KX_HD static R ntot(const R* n) {
+    R s = R(0);
+    s += n[0];
+    s += n[1];
+    s += n[2];
+    s += n[3];
+    s += n[4];
+    s += n[5];
+    s += n[6];
+    s += n[7];
+    s += n[8];
+    s += n[9];
+    s += n[10];
+    s += n[11];
+    return s;
+  }
+  template <class R> KX_HD static R Hnuclei(const R* n) {
+    R s = R(0);
+    s += n[0] * R(0.0);
+    s += n[1] * R(0.0);
+    s += n[2] * R(1.0);
+    s += n[3] * R(1.0);
+    s += n[4] * R(1.0);
+    s += n[5] * R(2.0);
+    s += n[6] * R(2.0);
+    s += n[7] * R(1.0);
+    s += n[8] * R(0.0);
+    s += n[9] * R(0.0);
+    s += n[10] * R(0.0);
+    s += n[11] * R(0.0);
+    return s;
+  }
+  template <class R> KX_HD static R rho(const R* n) {
+    R s = R(0);
+    s += n[0] * R(3.3443256801323995e-24);
+    s += n[1] * R(3.343414741762353e-24);
+    s += n[2] * R(1.6738233791328e-24);
+    s += n[3] * R(1.6729124407627538e-24);
+    s += n[4] * R(1.6747343175028463e-24);
+    s += n[5] * R(3.3476467582656e-24);
+    s += n[6] * R(3.346735819895553e-24);
+    s += n[7] * R(5.0181490592651994e-24);
+    s += n[8] * R(6.64647366797316e-24);
+    s += n[9] * R(6.645562729603113e-24);
+    s += n[10] * R(6.644651791233067e-24);
+    s += n[11] * R(9.10938370046373e-28);
+    return s;
+  }
+  // rates of the rate temperature Trate (and parameters); the
+  // caller zeroes k
+  template <class R>
+  KX_HD static void rates_T(R Trate, const Params& p, R* k) {
+    (void)p;
+    const R Tgas = Trate;
+    (void)Tgas;
+    {
+      const R v_lnTe = kx::log((Tgas * R(8.617333262145179e-05)));
+      k[0] = kx::exp((((((((((-R(32.71396786)) + (R(13.536556) * v_lnTe)) - (R(5.73932875) * kx::ipow<2>(v_lnTe))) + (R(1.56315498) * kx::ipow<3>(v_lnTe))) - (R(0.2877056) * kx::ipow<4>(v_lnTe))) + (R(0.0348255977) * kx::ipow<5>(v_lnTe))) - (R(0.00263197617) * kx::ipow<6>(v_lnTe))) + (R(0.000111954395) * kx::ipow<7>(v_lnTe))) - (R(2.03914985e-06) * kx::ipow<8>(v_lnTe))));  // H + e- -> H+ + e- + e-
+    }
+    {  // H+ + e- -> H: 2 piece(s), zero below 0.0, clamped above INFINITY
+      R Tgas = Trate;
+      const R v_invTe = (R(1.0) / (Tgas * R(8.617333262145179e-05)));
+      const R v_lnTe = kx::log((Tgas * R(8.617333262145179e-05)));
+      if (Tgas < R(5500.0)) k[1] = (R(3.92e-13) * kx::pow(v_invTe, R(0.6353)));  // H+ + e- -> H
+      if (Tgas >= R(5500.0)) k[2] = kx::exp(((((((((((-R(28.61303380689232)) - (R(0.7241125657826851) * v_lnTe)) - (R(0.02026044731984691) * kx::ipow<2>(v_lnTe))) - (R(0.002380861877349834) * kx::ipow<3>(v_lnTe))) - (R(0.0003212605213188796) * kx::ipow<4>(v_lnTe))) - (R(1.421502914054107e-05) * kx::ipow<5>(v_lnTe))) + (R(4.989108920299513e-06) * kx::ipow<6>(v_lnTe))) + (R(5.755614137575758e-07) * kx::ipow<7>(v_lnTe))) - (R(1.856767039775261e-08) * kx::ipow<8>(v_lnTe))) - (R(3.071135243196595e-09) * kx::ipow<9>(v_lnTe))));  // H+ + e- -> H
+    }
+    {
+      const R v_lnTe = kx::log((Tgas * R(8.617333262145179e-05)));
+      k[3] = kx::exp((((((((((-R(44.09864886)) + (R(23.91596563) * v_lnTe)) - (R(10.7532302) * kx::ipow<2>(v_lnTe))) + (R(3.05803875) * kx::ipow<3>(v_lnTe))) - (R(0.56851189) * kx::ipow<4>(v_lnTe))) + (R(0.0679539123) * kx::ipow<5>(v_lnTe))) - (R(0.0050090561) * kx::ipow<6>(v_lnTe))) + (R(0.000206723616) * kx::ipow<7>(v_lnTe))) - (R(3.64916141e-06) * kx::ipow<8>(v_lnTe))));  // He + e- -> He+ + e- + e-
+    }
+    {  // He+ + e- -> He: 2 piece(s), zero below 0.0, clamped above INFINITY
+      R Tgas = Trate;
+      const R v_invTe = (R(1.0) / (Tgas * R(8.617333262145179e-05)));
+      const R v_Te = (Tgas * R(8.617333262145179e-05));
+      if (Tgas < R(9280.0)) k[4] = (R(3.92e-13) * kx::pow(v_invTe, R(0.6353)));  // He+ + e- -> He
+      if (Tgas >= R(9280.0)) k[5] = (((R(1.54e-09) * (R(1.0) + (R(0.3) / kx::exp((R(8.099328789667) * v_invTe))))) / (kx::exp((R(40.49664394833662) * v_invTe)) * kx::pow(v_Te, R(1.5)))) + (R(3.92e-13) / kx::pow(v_Te, R(0.6353))));  // He+ + e- -> He
+    }
+    {
+      const R v_lnTe = kx::log((Tgas * R(8.617333262145179e-05)));
+      k[6] = kx::exp((((((((((-R(68.71040990212)) + (R(43.93347632635) * v_lnTe)) - (R(18.48066993568) * kx::ipow<2>(v_lnTe))) + (R(4.701626486759002) * kx::ipow<3>(v_lnTe))) - (R(0.7692466334492) * kx::ipow<4>(v_lnTe))) + (R(0.08113042097303) * kx::ipow<5>(v_lnTe))) - (R(0.005324020628287001) * kx::ipow<6>(v_lnTe))) + (R(0.0001975705312221) * kx::ipow<7>(v_lnTe))) - (R(3.165581065665e-06) * kx::ipow<8>(v_lnTe))));  // He+ + e- -> He++ + e- + e-
+    }
+    {
+      const R v_sqrTgas = kx::sqrt(Tgas);
+      k[7] = (((R(3.36e-10) / v_sqrTgas) / kx::pow((Tgas / R(1000.0)), R(0.2))) / (R(1.0) + kx::pow((Tgas / R(1000000.0)), R(0.7))));  // He++ + e- -> He+
+    }
+    {
+      const R v_Te = (Tgas * R(8.617333262145179e-05));
+      k[8] = (R(6.77e-15) * kx::pow(v_Te, R(0.8779)));  // H + e- -> H-
+    }
+    {  // H- + H -> H2 + e-: 2 piece(s), zero below 0.0, clamped above INFINITY
+      R Tgas = Trate;
+      const R v_lnTe = kx::log((Tgas * R(8.617333262145179e-05)));
+      if (Tgas < R(1160.0)) k[9] = R(1.43e-09);  // H- + H -> H2 + e-
+      if (Tgas >= R(1160.0)) k[10] = kx::exp(((((((((-R(20.06913897587003)) + (R(0.2289800603272916) * v_lnTe)) + (R(0.03599837721023835) * kx::ipow<2>(v_lnTe))) - (R(0.004555120027032095) * kx::ipow<3>(v_lnTe))) - (R(0.0003105115447124016) * kx::ipow<4>(v_lnTe))) + (R(0.0001073294010367247) * kx::ipow<5>(v_lnTe))) - (R(8.36671960467864e-06) * kx::ipow<6>(v_lnTe))) + (R(2.238306228891639e-07) * kx::ipow<7>(v_lnTe))));  // H- + H -> H2 + e-
+    }
+    {  // H + H+ -> H2+: 2 piece(s), zero below 0.0, clamped above INFINITY
+      R Tgas = Trate;
+      if (Tgas < R(6700.0)) k[11] = (R(1.85e-23) * kx::pow(Tgas, R(1.8)));  // H + H+ -> H2+
+      if (Tgas >= R(6700.0)) k[12] = (R(5.81e-16) * kx::pow((Tgas / R(56200.0)), ((-R(0.6657)) * kx::log10((Tgas / R(56200.0))))));  // H + H+ -> H2+
+    }
+    k[13] = R(6e-10);  // H2+ + H -> H2 + H+
+    {  // H2 + H+ -> H2+ + H: 1 piece(s), zero below 3480.0, clamped above INFINITY
+      R Tgas = Trate;
+      const R v_lnTe = kx::log((Tgas * R(8.617333262145179e-05)));
+      if (Tgas >= R(3480.0)) k[14] = kx::exp((((((((((-R(24.24914687731536)) + (R(3.400824447095291) * v_lnTe)) - (R(3.898003964650152) * kx::ipow<2>(v_lnTe))) + (R(2.045587822403071) * kx::ipow<3>(v_lnTe))) - (R(0.5416182856220388) * kx::ipow<4>(v_lnTe))) + (R(0.0841077503763412) * kx::ipow<5>(v_lnTe))) - (R(0.007879026154483455) * kx::ipow<6>(v_lnTe))) + (R(0.0004138398421504563) * kx::ipow<7>(v_lnTe))) - (R(9.36345888928611e-06) * kx::ipow<8>(v_lnTe))));  // H2 + H+ -> H2+ + H
+    }
+    {
+      const R v_invT = (R(1.0) / Tgas);
+      k[15] = ((R(5.6e-11) * kx::exp(((-R(102124.0)) * v_invT))) * kx::sqrt(Tgas));  // H2 + e- -> H + H + e-
+    }
+    {
+      const R v_Te = (Tgas * R(8.617333262145179e-05));
+      const R v_invTe = (R(1.0) / (Tgas * R(8.617333262145179e-05)));
+      k[16] = (((R(1.0670825e-10) * kx::pow(v_Te, R(2.012))) * kx::exp(((-R(4.463)) * v_invTe))) / kx::pow((R(1.0) + (R(0.2472) * v_Te)), R(3.512)));  // H2 + H -> H + H + H
+    }
+    {
+      const R v_lnTe = kx::log((Tgas * R(8.617333262145179e-05)));
+      k[17] = kx::exp((((((((((-R(18.01849334273)) + (R(2.360852208681) * v_lnTe)) - (R(0.2827443061704) * kx::ipow<2>(v_lnTe))) + (R(0.01623316639567) * kx::ipow<3>(v_lnTe))) - (R(0.03365012031362999) * kx::ipow<4>(v_lnTe))) + (R(0.01178329782711) * kx::ipow<5>(v_lnTe))) - (R(0.001656194699504) * kx::ipow<6>(v_lnTe))) + (R(0.0001068275202678) * kx::ipow<7>(v_lnTe))) - (R(2.631285809207e-06) * kx::ipow<8>(v_lnTe))));  // H- + e- -> H + e- + e-
+    }
+    {  // H- + H -> H + H + e-: 2 piece(s), zero below 0.0, clamped above INFINITY
+      R Tgas = Trate;
+      const R v_Te = (Tgas * R(8.617333262145179e-05));
+      const R v_lnTe = kx::log((Tgas * R(8.617333262145179e-05)));
+      if (Tgas < R(1160.0)) k[18] = (R(2.56e-09) * kx::pow(v_Te, R(1.78186)));  // H- + H -> H + H + e-
+      if (Tgas >= R(1160.0)) k[19] = kx::exp(((((((((((-R(20.37260896533324)) + (R(1.139449335841631) * v_lnTe)) - (R(0.1421013521554148) * kx::ipow<2>(v_lnTe))) + (R(0.00846445538663) * kx::ipow<3>(v_lnTe))) - (R(0.0014327641212992) * kx::ipow<4>(v_lnTe))) + (R(0.0002012250284791) * kx::ipow<5>(v_lnTe))) + (R(8.66396324309e-05) * kx::ipow<6>(v_lnTe))) - (R(2.585009680264e-05) * kx::ipow<7>(v_lnTe))) + (R(2.4555011970392e-06) * kx::ipow<8>(v_lnTe))) - (R(8.06838246118e-08) * kx::ipow<9>(v_lnTe))));  // H- + H -> H + H + e-
+    }
+    {
+      const R v_Te = (Tgas * R(8.617333262145179e-05));
+      k[20] = (R(6.5e-09) / kx::sqrt(v_Te));  // H- + H+ -> H + H
+    }
+    k[21] = (R(1e-08) * kx::pow(Tgas, (-R(0.4))));  // H- + H+ -> H2+ + e-
+    {  // H2+ + e- -> H + H: 2 piece(s), zero below 0.0, clamped above INFINITY
+      R Tgas = Trate;
+      if (Tgas < R(617.0)) k[22] = R(1e-08);  // H2+ + e- -> H + H
+      if (Tgas >= R(617.0)) k[23] = (R(1.32e-06) * kx::pow(Tgas, (-R(0.76))));  // H2+ + e- -> H + H
+    }
+    {
+      const R v_invT = (R(1.0) / Tgas);
+      k[24] = (R(5e-07) * kx::sqrt((R(100.0) * v_invT)));  // H2+ + H- -> H + H2
+    }
+    {  // H + H + H -> H2 + H: 2 piece(s), zero below 0.0, clamped above INFINITY
+      R Tgas = Trate;
+      const R v_T32 = (Tgas / R(300.0));
+      if (Tgas < R(300.0)) k[25] = (R(1.3e-32) * kx::pow(v_T32, (-R(0.38))));  // H + H + H -> H2 + H
+      if (Tgas >= R(300.0)) k[26] = (R(1.3e-32) * kx::ipow<-1>(v_T32));  // H + H + H -> H2 + H
+    }
+    {  // H2 + H + H -> H2 + H2: 2 piece(s), zero below 0.0, clamped above INFINITY
+      R Tgas = Trate;
+      const R v_T32 = (Tgas / R(300.0));
+      if (Tgas < R(300.0)) k[27] = ((R(1.3e-32) * kx::pow(v_T32, (-R(0.38)))) / R(8.0));  // H2 + H + H -> H2 + H2
+      if (Tgas >= R(300.0)) k[28] = ((R(1.3e-32) * kx::ipow<-1>(v_T32)) / R(8.0));  // H2 + H + H -> H2 + H2
+    }
+    {  // H+ + D -> H + D+: 1 piece(s), zero below 50.0, clamped above INFINITY
+      R Tgas = Trate;
+      const R v_invT = (R(1.0) / Tgas);
+      if (Tgas >= R(50.0)) k[29] = (((R(2e-10) * kx::pow(Tgas, R(0.402))) * kx::exp(((-R(37.1)) * v_invT))) - (R(3.31e-17) * kx::pow(Tgas, R(1.48))));  // H+ + D -> H + D+
+    }
+    {  // H + D+ -> H+ + D: 1 piece(s), zero below 50.0, clamped above INFINITY
+      R Tgas = Trate;
+      const R v_invT = (R(1.0) / Tgas);
+      if (Tgas >= R(50.0)) k[30] = (((R(2.06e-10) * kx::pow(Tgas, R(0.396))) * kx::exp(((-R(33.0)) * v_invT))) + (R(2.03e-09) * kx::pow(Tgas, (-R(0.332)))));  // H + D+ -> H+ + D
+    }
+    k[31] = (R(1e-09) * ((R(0.417) + (R(0.846) * kx::log10(Tgas))) - (R(0.137) * kx::ipow<2>(kx::log10(Tgas)))));  // H2 + D+ -> HD + H+
+    {
+      const R v_invT = (R(1.0) / Tgas);
+      k[32] = (R(1e-09) * kx::exp(((-R(457.0)) * v_invT)));  // HD + H+ -> H2 + D+
+    }
+    {  // H2 + D -> HD + H: 2 piece(s), zero below 0.0, clamped above INFINITY
+      R Tgas = Trate;
+      const R v_invT = (R(1.0) / Tgas);
+      if (Tgas < R(2000.0)) k[33] = kx::pow(R(10.0), ((((((-R(56.4737)) + (R(5.88886) * kx::log10(Tgas))) + (R(7.19692) * kx::ipow<2>(kx::log10(Tgas)))) + (R(2.25069) * kx::ipow<3>(kx::log10(Tgas)))) - (R(2.16903) * kx::ipow<4>(kx::log10(Tgas)))) + (R(0.317887) * kx::ipow<5>(kx::log10(Tgas)))));  // H2 + D -> HD + H
+      if (Tgas >= R(2000.0)) k[34] = (R(3.17e-10) * kx::exp(((-R(5207.0)) * v_invT)));  // H2 + D -> HD + H
+    }
+    {  // HD + H -> H2 + D: 1 piece(s), zero below 200.0, clamped above INFINITY
+      R Tgas = Trate;
+      const R v_invT = (R(1.0) / Tgas);
+      if (Tgas >= R(200.0)) k[35] = (R(5.25e-11) * kx::exp((((-R(4430.0)) * v_invT) + (R(173900.0) * kx::ipow<2>(v_invT)))));  // HD + H -> H2 + D
+    }
+    {
+      const R v_T32 = (Tgas / R(300.0));
+      k[36] = (R(1.5e-09) * kx::pow(v_T32, (-R(0.1))));  // D + H- -> HD + e-
+    }
+    k[37] = (R(3.6e-12) * kx::pow((Tgas / R(300.0)), (-R(0.75))));  // D+ + e- -> D
+  }
+  // density-dependent rates
+  template <class R>
+  KX_HD static void rates_n(R Trate, const R* n, const Params& p, R* k) {
+    (void)p; (void)n;
+    const R Tgas = Trate;
+    (void)Tgas;
+  }
+  // every reaction: the T-only ones in k (zeroed) then the rest
+  template <class R>
+  KX_HD static void rates(R Trate, const R* n, const Params& p, R* k) {
+    for (int i = 0; i < NREA; ++i) k[i] = R(0);
+    rates_T(Trate, p, k);
+    rates_n(Trate, n, p, k);
+  }
+  static constexpr int NREA_T = 38;
+  // dn/dt = sum over reactions of S k prod(n), in reaction order
+  template <class R>
+  KX_HD static void rhs(const R* n, const R* k, R* dn) {
+    for (int i = 0; i < NSPEC; ++i) dn[i] = R(0);
+    R fl;
+    fl = k[0]*n[2]*n[11];  // H + e- -> H+ + e- + e-
+    dn[2] -= fl;
+    dn[11] -= fl;
+    dn[3] += fl;
+    dn[11] += fl;
+    dn[11] += fl;
+    fl = k[1]*n[3]*n[11];  // H+ + e- -> H
+    dn[3] -= fl;
+    dn[11] -= fl;
+    dn[2] += fl;
+    fl = k[2]*n[3]*n[11];  // H+ + e- -> H
+    dn[3] -= fl;
+    dn[11] -= fl;
+    dn[2] += fl;
+    fl = k[3]*n[8]*n[11];  // He + e- -> He+ + e- + e-
+    dn[8] -= fl;
+    dn[11] -= fl;
+    dn[9] += fl;
+    dn[11] += fl;
+    dn[11] += fl;
+    fl = k[4]*n[9]*n[11];  // He+ + e- -> He
+    dn[9] -= fl;
+    dn[11] -= fl;
+    dn[8] += fl;
+    fl = k[5]*n[9]*n[11];  // He+ + e- -> He
+    dn[9] -= fl;
+    dn[11] -= fl;
+    dn[8] += fl;
+    fl = k[6]*n[9]*n[11];  // He+ + e- -> He++ + e- + e-
+    dn[9] -= fl;
+    dn[11] -= fl;
+    dn[10] += fl;
+    dn[11] += fl;
+    dn[11] += fl;
+    fl = k[7]*n[10]*n[11];  // He++ + e- -> He+
+    dn[10] -= fl;
+    dn[11] -= fl;
+    dn[9] += fl;
+    fl = k[8]*n[2]*n[11];  // H + e- -> H-
+    dn[2] -= fl;
+    dn[11] -= fl;
+    dn[4] += fl;
+    fl = k[9]*n[4]*n[2];  // H- + H -> H2 + e-
+    dn[4] -= fl;
+    dn[2] -= fl;
+    dn[5] += fl;
+    dn[11] += fl;
+    fl = k[10]*n[4]*n[2];  // H- + H -> H2 + e-
+    dn[4] -= fl;
+    dn[2] -= fl;
+    dn[5] += fl;
+    dn[11] += fl;
+    fl = k[11]*n[2]*n[3];  // H + H+ -> H2+
+    dn[2] -= fl;
+    dn[3] -= fl;
+    dn[6] += fl;
+    fl = k[12]*n[2]*n[3];  // H + H+ -> H2+
+    dn[2] -= fl;
+    dn[3] -= fl;
+    dn[6] += fl;
+    fl = k[13]*n[6]*n[2];  // H2+ + H -> H2 + H+
+    dn[6] -= fl;
+    dn[2] -= fl;
+    dn[5] += fl;
+    dn[3] += fl;
+    fl = k[14]*n[5]*n[3];  // H2 + H+ -> H2+ + H
+    dn[5] -= fl;
+    dn[3] -= fl;
+    dn[6] += fl;
+    dn[2] += fl;
+    fl = k[15]*n[5]*n[11];  // H2 + e- -> H + H + e-
+    dn[5] -= fl;
+    dn[11] -= fl;
+    dn[2] += fl;
+    dn[2] += fl;
+    dn[11] += fl;
+    fl = k[16]*n[5]*n[2];  // H2 + H -> H + H + H
+    dn[5] -= fl;
+    dn[2] -= fl;
+    dn[2] += fl;
+    dn[2] += fl;
+    dn[2] += fl;
+    fl = k[17]*n[4]*n[11];  // H- + e- -> H + e- + e-
+    dn[4] -= fl;
+    dn[11] -= fl;
+    dn[2] += fl;
+    dn[11] += fl;
+    dn[11] += fl;
+    fl = k[18]*n[4]*n[2];  // H- + H -> H + H + e-
+    dn[4] -= fl;
+    dn[2] -= fl;
+    dn[2] += fl;
+    dn[2] += fl;
+    dn[11] += fl;
+    fl = k[19]*n[4]*n[2];  // H- + H -> H + H + e-
+    dn[4] -= fl;
+    dn[2] -= fl;
+    dn[2] += fl;
+    dn[2] += fl;
+    dn[11] += fl;
+    fl = k[20]*n[4]*n[3];  // H- + H+ -> H + H
+    dn[4] -= fl;
+    dn[3] -= fl;
+    dn[2] += fl;
+    dn[2] += fl;
+    fl = k[21]*n[4]*n[3];  // H- + H+ -> H2+ + e-
+    dn[4] -= fl;
+    dn[3] -= fl;
+    dn[6] += fl;
+    dn[11] += fl;
+    fl = k[22]*n[6]*n[11];  // H2+ + e- -> H + H
+    dn[6] -= fl;
+    dn[11] -= fl;
+    dn[2] += fl;
+    dn[2] += fl;
+    fl = k[23]*n[6]*n[11];  // H2+ + e- -> H + H
+    dn[6] -= fl;
+    dn[11] -= fl;
+    dn[2] += fl;
+    dn[2] += fl;
+    fl = k[24]*n[6]*n[4];  // H2+ + H- -> H + H2
+    dn[6] -= fl;
+    dn[4] -= fl;
+    dn[2] += fl;
+    dn[5] += fl;
+    fl = k[25]*n[2]*n[2]*n[2];  // H + H + H -> H2 + H
+    dn[2] -= fl;
+    dn[2] -= fl;
+    dn[2] -= fl;
+    dn[5] += fl;
+    dn[2] += fl;
+    fl = k[26]*n[2]*n[2]*n[2];  // H + H + H -> H2 + H
+    dn[2] -= fl;
+    dn[2] -= fl;
+    dn[2] -= fl;
+    dn[5] += fl;
+    dn[2] += fl;
+    fl = k[27]*n[5]*n[2]*n[2];  // H2 + H + H -> H2 + H2
+    dn[5] -= fl;
+    dn[2] -= fl;
+    dn[2] -= fl;
+    dn[5] += fl;
+    dn[5] += fl;
+    fl = k[28]*n[5]*n[2]*n[2];  // H2 + H + H -> H2 + H2
+    dn[5] -= fl;
+    dn[2] -= fl;
+    dn[2] -= fl;
+    dn[5] += fl;
+    dn[5] += fl;
+    fl = k[29]*n[3]*n[0];  // H+ + D -> H + D+
+    dn[3] -= fl;
+    dn[0] -= fl;
+    dn[2] += fl;
+    dn[1] += fl;
+    fl = k[30]*n[2]*n[1];  // H + D+ -> H+ + D
+    dn[2] -= fl;
+    dn[1] -= fl;
+    dn[3] += fl;
+    dn[0] += fl;
+    fl = k[31]*n[5]*n[1];  // H2 + D+ -> HD + H+
+    dn[5] -= fl;
+    dn[1] -= fl;
+    dn[7] += fl;
+    dn[3] += fl;
+    fl = k[32]*n[7]*n[3];  // HD + H+ -> H2 + D+
+    dn[7] -= fl;
+    dn[3] -= fl;
+    dn[5] += fl;
+    dn[1] += fl;
+    fl = k[33]*n[5]*n[0];  // H2 + D -> HD + H
+    dn[5] -= fl;
+    dn[0] -= fl;
+    dn[7] += fl;
+    dn[2] += fl;
+    fl = k[34]*n[5]*n[0];  // H2 + D -> HD + H
+    dn[5] -= fl;
+    dn[0] -= fl;
+    dn[7] += fl;
+    dn[2] += fl;
+    fl = k[35]*n[7]*n[2];  // HD + H -> H2 + D
+    dn[7] -= fl;
+    dn[2] -= fl;
+    dn[5] += fl;
+    dn[0] += fl;
+    fl = k[36]*n[0]*n[4];  // D + H- -> HD + e-
+    dn[0] -= fl;
+    dn[4] -= fl;
+    dn[7] += fl;
+    dn[11] += fl;
+    fl = k[37]*n[1]*n[11];  // D+ + e- -> D
+    dn[1] -= fl;
+    dn[11] -= fl;
+    dn[0] += fl;
+    (void)fl;
+  }
+  // d(dn_i)/d(n_j) at fixed rates into J (column-major, element
+  // (i, j) at J[i + j ld]; added to, the caller zeroes it)
+  template <class R>
+  KX_HD static void jac_species(const R* n, const R* k, R* J, int ld) {
+    R d;
+    d = k[0]*n[11];  // H + e- -> H+ + e- + e-, d/d n[2]
+    J[2 + 2 * ld] -= d;
+    J[11 + 2 * ld] -= d;
+    J[3 + 2 * ld] += d;
+    J[11 + 2 * ld] += d;
+    J[11 + 2 * ld] += d;
+    d = k[0]*n[2];  // H + e- -> H+ + e- + e-, d/d n[11]
+    J[2 + 11 * ld] -= d;
+    J[11 + 11 * ld] -= d;
+    J[3 + 11 * ld] += d;
+    J[11 + 11 * ld] += d;
+    J[11 + 11 * ld] += d;
+    d = k[1]*n[11];  // H+ + e- -> H, d/d n[3]
+    J[3 + 3 * ld] -= d;
+    J[11 + 3 * ld] -= d;
+    J[2 + 3 * ld] += d;
+    d = k[1]*n[3];  // H+ + e- -> H, d/d n[11]
+    J[3 + 11 * ld] -= d;
+    J[11 + 11 * ld] -= d;
+    J[2 + 11 * ld] += d;
+    d = k[2]*n[11];  // H+ + e- -> H, d/d n[3]
+    J[3 + 3 * ld] -= d;
+    J[11 + 3 * ld] -= d;
+    J[2 + 3 * ld] += d;
+    d = k[2]*n[3];  // H+ + e- -> H, d/d n[11]
+    J[3 + 11 * ld] -= d;
+    J[11 + 11 * ld] -= d;
+    J[2 + 11 * ld] += d;
+    d = k[3]*n[11];  // He + e- -> He+ + e- + e-, d/d n[8]
+    J[8 + 8 * ld] -= d;
+    J[11 + 8 * ld] -= d;
+    J[9 + 8 * ld] += d;
+    J[11 + 8 * ld] += d;
+    J[11 + 8 * ld] += d;
+    d = k[3]*n[8];  // He + e- -> He+ + e- + e-, d/d n[11]
+    J[8 + 11 * ld] -= d;
+    J[11 + 11 * ld] -= d;
+    J[9 + 11 * ld] += d;
+    J[11 + 11 * ld] += d;
+    J[11 + 11 * ld] += d;
+    d = k[4]*n[11];  // He+ + e- -> He, d/d n[9]
+    J[9 + 9 * ld] -= d;
+    J[11 + 9 * ld] -= d;
+    J[8 + 9 * ld] += d;
+    d = k[4]*n[9];  // He+ + e- -> He, d/d n[11]
+    J[9 + 11 * ld] -= d;
+    J[11 + 11 * ld] -= d;
+    J[8 + 11 * ld] += d;
+    d = k[5]*n[11];  // He+ + e- -> He, d/d n[9]
+    J[9 + 9 * ld] -= d;
+    J[11 + 9 * ld] -= d;
+    J[8 + 9 * ld] += d;
+    d = k[5]*n[9];  // He+ + e- -> He, d/d n[11]
+    J[9 + 11 * ld] -= d;
+    J[11 + 11 * ld] -= d;
+    J[8 + 11 * ld] += d;
+    d = k[6]*n[11];  // He+ + e- -> He++ + e- + e-, d/d n[9]
+    J[9 + 9 * ld] -= d;
+    J[11 + 9 * ld] -= d;
+    J[10 + 9 * ld] += d;
+    J[11 + 9 * ld] += d;
+    J[11 + 9 * ld] += d;
+    d = k[6]*n[9];  // He+ + e- -> He++ + e- + e-, d/d n[11]
+    J[9 + 11 * ld] -= d;
+    J[11 + 11 * ld] -= d;
+    J[10 + 11 * ld] += d;
+    J[11 + 11 * ld] += d;
+    J[11 + 11 * ld] += d;
+    d = k[7]*n[11];  // He++ + e- -> He+, d/d n[10]
+    J[10 + 10 * ld] -= d;
+    J[11 + 10 * ld] -= d;
+    J[9 + 10 * ld] += d;
+    d = k[7]*n[10];  // He++ + e- -> He+, d/d n[11]
+    J[10 + 11 * ld] -= d;
+    J[11 + 11 * ld] -= d;
+    J[9 + 11 * ld] += d;
+    d = k[8]*n[11];  // H + e- -> H-, d/d n[2]
+    J[2 + 2 * ld] -= d;
+    J[11 + 2 * ld] -= d;
+    J[4 + 2 * ld] += d;
+    d = k[8]*n[2];  // H + e- -> H-, d/d n[11]
+    J[2 + 11 * ld] -= d;
+    J[11 + 11 * ld] -= d;
+    J[4 + 11 * ld] += d;
+    d = k[9]*n[4];  // H- + H -> H2 + e-, d/d n[2]
+    J[4 + 2 * ld] -= d;
+    J[2 + 2 * ld] -= d;
+    J[5 + 2 * ld] += d;
+    J[11 + 2 * ld] += d;
+    d = k[9]*n[2];  // H- + H -> H2 + e-, d/d n[4]
+    J[4 + 4 * ld] -= d;
+    J[2 + 4 * ld] -= d;
+    J[5 + 4 * ld] += d;
+    J[11 + 4 * ld] += d;
+    d = k[10]*n[4];  // H- + H -> H2 + e-, d/d n[2]
+    J[4 + 2 * ld] -= d;
+    J[2 + 2 * ld] -= d;
+    J[5 + 2 * ld] += d;
+    J[11 + 2 * ld] += d;
+    d = k[10]*n[2];  // H- + H -> H2 + e-, d/d n[4]
+    J[4 + 4 * ld] -= d;
+    J[2 + 4 * ld] -= d;
+    J[5 + 4 * ld] += d;
+    J[11 + 4 * ld] += d;
+    d = k[11]*n[3];  // H + H+ -> H2+, d/d n[2]
+    J[2 + 2 * ld] -= d;
+    J[3 + 2 * ld] -= d;
+    J[6 + 2 * ld] += d;
+    d = k[11]*n[2];  // H + H+ -> H2+, d/d n[3]
+    J[2 + 3 * ld] -= d;
+    J[3 + 3 * ld] -= d;
+    J[6 + 3 * ld] += d;
+    d = k[12]*n[3];  // H + H+ -> H2+, d/d n[2]
+    J[2 + 2 * ld] -= d;
+    J[3 + 2 * ld] -= d;
+    J[6 + 2 * ld] += d;
+    d = k[12]*n[2];  // H + H+ -> H2+, d/d n[3]
+    J[2 + 3 * ld] -= d;
+    J[3 + 3 * ld] -= d;
+    J[6 + 3 * ld] += d;
+    d = k[13]*n[6];  // H2+ + H -> H2 + H+, d/d n[2]
+    J[6 + 2 * ld] -= d;
+    J[2 + 2 * ld] -= d;
+    J[5 + 2 * ld] += d;
+    J[3 + 2 * ld] += d;
+    d = k[13]*n[2];  // H2+ + H -> H2 + H+, d/d n[6]
+    J[6 + 6 * ld] -= d;
+    J[2 + 6 * ld] -= d;
+    J[5 + 6 * ld] += d;
+    J[3 + 6 * ld] += d;
+    d = k[14]*n[5];  // H2 + H+ -> H2+ + H, d/d n[3]
+    J[5 + 3 * ld] -= d;
+    J[3 + 3 * ld] -= d;
+    J[6 + 3 * ld] += d;
+    J[2 + 3 * ld] += d;
+    d = k[14]*n[3];  // H2 + H+ -> H2+ + H, d/d n[5]
+    J[5 + 5 * ld] -= d;
+    J[3 + 5 * ld] -= d;
+    J[6 + 5 * ld] += d;
+    J[2 + 5 * ld] += d;
+    d = k[15]*n[11];  // H2 + e- -> H + H + e-, d/d n[5]
+    J[5 + 5 * ld] -= d;
+    J[11 + 5 * ld] -= d;
+    J[2 + 5 * ld] += d;
+    J[2 + 5 * ld] += d;
+    J[11 + 5 * ld] += d;
+    d = k[15]*n[5];  // H2 + e- -> H + H + e-, d/d n[11]
+    J[5 + 11 * ld] -= d;
+    J[11 + 11 * ld] -= d;
+    J[2 + 11 * ld] += d;
+    J[2 + 11 * ld] += d;
+    J[11 + 11 * ld] += d;
+    d = k[16]*n[5];  // H2 + H -> H + H + H, d/d n[2]
+    J[5 + 2 * ld] -= d;
+    J[2 + 2 * ld] -= d;
+    J[2 + 2 * ld] += d;
+    J[2 + 2 * ld] += d;
+    J[2 + 2 * ld] += d;
+    d = k[16]*n[2];  // H2 + H -> H + H + H, d/d n[5]
+    J[5 + 5 * ld] -= d;
+    J[2 + 5 * ld] -= d;
+    J[2 + 5 * ld] += d;
+    J[2 + 5 * ld] += d;
+    J[2 + 5 * ld] += d;
+    d = k[17]*n[11];  // H- + e- -> H + e- + e-, d/d n[4]
+    J[4 + 4 * ld] -= d;
+    J[11 + 4 * ld] -= d;
+    J[2 + 4 * ld] += d;
+    J[11 + 4 * ld] += d;
+    J[11 + 4 * ld] += d;
+    d = k[17]*n[4];  // H- + e- -> H + e- + e-, d/d n[11]
+    J[4 + 11 * ld] -= d;
+    J[11 + 11 * ld] -= d;
+    J[2 + 11 * ld] += d;
+    J[11 + 11 * ld] += d;
+    J[11 + 11 * ld] += d;
+    d = k[18]*n[4];  // H- + H -> H + H + e-, d/d n[2]
+    J[4 + 2 * ld] -= d;
+    J[2 + 2 * ld] -= d;
+    J[2 + 2 * ld] += d;
+    J[2 + 2 * ld] += d;
+    J[11 + 2 * ld] += d;
+    d = k[18]*n[2];  // H- + H -> H + H + e-, d/d n[4]
+    J[4 + 4 * ld] -= d;
+    J[2 + 4 * ld] -= d;
+    J[2 + 4 * ld] += d;
+    J[2 + 4 * ld] += d;
+    J[11 + 4 * ld] += d;
+    d = k[19]*n[4];  // H- + H -> H + H + e-, d/d n[2]
+    J[4 + 2 * ld] -= d;
+    J[2 + 2 * ld] -= d;
+    J[2 + 2 * ld] += d;
+    J[2 + 2 * ld] += d;
+    J[11 + 2 * ld] += d;
+    d = k[19]*n[2];  // H- + H -> H + H + e-, d/d n[4]
+    J[4 + 4 * ld] -= d;
+    J[2 + 4 * ld] -= d;
+    J[2 + 4 * ld] += d;
+    J[2 + 4 * ld] += d;
+    J[11 + 4 * ld] += d;
+    d = k[20]*n[4];  // H- + H+ -> H + H, d/d n[3]
+    J[4 + 3 * ld] -= d;
+    J[3 + 3 * ld] -= d;
+    J[2 + 3 * ld] += d;
+    J[2 + 3 * ld] += d;
+    d = k[20]*n[3];  // H- + H+ -> H + H, d/d n[4]
+    J[4 + 4 * ld] -= d;
+    J[3 + 4 * ld] -= d;
+    J[2 + 4 * ld] += d;
+    J[2 + 4 * ld] += d;
+    d = k[21]*n[4];  // H- + H+ -> H2+ + e-, d/d n[3]
+    J[4 + 3 * ld] -= d;
+    J[3 + 3 * ld] -= d;
+    J[6 + 3 * ld] += d;
+    J[11 + 3 * ld] += d;
+    d = k[21]*n[3];  // H- + H+ -> H2+ + e-, d/d n[4]
+    J[4 + 4 * ld] -= d;
+    J[3 + 4 * ld] -= d;
+    J[6 + 4 * ld] += d;
+    J[11 + 4 * ld] += d;
+    d = k[22]*n[11];  // H2+ + e- -> H + H, d/d n[6]
+    J[6 + 6 * ld] -= d;
+    J[11 + 6 * ld] -= d;
+    J[2 + 6 * ld] += d;
+    J[2 + 6 * ld] += d;
+    d = k[22]*n[6];  // H2+ + e- -> H + H, d/d n[11]
+    J[6 + 11 * ld] -= d;
+    J[11 + 11 * ld] -= d;
+    J[2 + 11 * ld] += d;
+    J[2 + 11 * ld] += d;
+    d = k[23]*n[11];  // H2+ + e- -> H + H, d/d n[6]
+    J[6 + 6 * ld] -= d;
+    J[11 + 6 * ld] -= d;
+    J[2 + 6 * ld] += d;
+    J[2 + 6 * ld] += d;
+    d = k[23]*n[6];  // H2+ + e- -> H + H, d/d n[11]
+    J[6 + 11 * ld] -= d;
+    J[11 + 11 * ld] -= d;
+    J[2 + 11 * ld] += d;
+    J[2 + 11 * ld] += d;
+    d = k[24]*n[6];  // H2+ + H- -> H + H2, d/d n[4]
+    J[6 + 4 * ld] -= d;
+    J[4 + 4 * ld] -= d;
+    J[2 + 4 * ld] += d;
+    J[5 + 4 * ld] += d;
+    d = k[24]*n[4];  // H2+ + H- -> H + H2, d/d n[6]
+    J[6 + 6 * ld] -= d;
+    J[4 + 6 * ld] -= d;
+    J[2 + 6 * ld] += d;
+    J[5 + 6 * ld] += d;
+    d = k[25]*n[2]*n[2]*R(3.0);  // H + H + H -> H2 + H, d/d n[2]
+    J[2 + 2 * ld] -= d;
+    J[2 + 2 * ld] -= d;
+    J[2 + 2 * ld] -= d;
+    J[5 + 2 * ld] += d;
+    J[2 + 2 * ld] += d;
+    d = k[26]*n[2]*n[2]*R(3.0);  // H + H + H -> H2 + H, d/d n[2]
+    J[2 + 2 * ld] -= d;
+    J[2 + 2 * ld] -= d;
+    J[2 + 2 * ld] -= d;
+    J[5 + 2 * ld] += d;
+    J[2 + 2 * ld] += d;
+    d = k[27]*n[5]*n[2]*R(2.0);  // H2 + H + H -> H2 + H2, d/d n[2]
+    J[5 + 2 * ld] -= d;
+    J[2 + 2 * ld] -= d;
+    J[2 + 2 * ld] -= d;
+    J[5 + 2 * ld] += d;
+    J[5 + 2 * ld] += d;
+    d = k[27]*n[2]*n[2];  // H2 + H + H -> H2 + H2, d/d n[5]
+    J[5 + 5 * ld] -= d;
+    J[2 + 5 * ld] -= d;
+    J[2 + 5 * ld] -= d;
+    J[5 + 5 * ld] += d;
+    J[5 + 5 * ld] += d;
+    d = k[28]*n[5]*n[2]*R(2.0);  // H2 + H + H -> H2 + H2, d/d n[2]
+    J[5 + 2 * ld] -= d;
+    J[2 + 2 * ld] -= d;
+    J[2 + 2 * ld] -= d;
+    J[5 + 2 * ld] += d;
+    J[5 + 2 * ld] += d;
+    d = k[28]*n[2]*n[2];  // H2 + H + H -> H2 + H2, d/d n[5]
+    J[5 + 5 * ld] -= d;
+    J[2 + 5 * ld] -= d;
+    J[2 + 5 * ld] -= d;
+    J[5 + 5 * ld] += d;
+    J[5 + 5 * ld] += d;
+    d = k[29]*n[3];  // H+ + D -> H + D+, d/d n[0]
+    J[3 + 0 * ld] -= d;
+    J[0 + 0 * ld] -= d;
+    J[2 + 0 * ld] += d;
+    J[1 + 0 * ld] += d;
+    d = k[29]*n[0];  // H+ + D -> H + D+, d/d n[3]
+    J[3 + 3 * ld] -= d;
+    J[0 + 3 * ld] -= d;
+    J[2 + 3 * ld] += d;
+    J[1 + 3 * ld] += d;
+    d = k[30]*n[2];  // H + D+ -> H+ + D, d/d n[1]
+    J[2 + 1 * ld] -= d;
+    J[1 + 1 * ld] -= d;
+    J[3 + 1 * ld] += d;
+    J[0 + 1 * ld] += d;
+    d = k[30]*n[1];  // H + D+ -> H+ + D, d/d n[2]
+    J[2 + 2 * ld] -= d;
+    J[1 + 2 * ld] -= d;
+    J[3 + 2 * ld] += d;
+    J[0 + 2 * ld] += d;
+    d = k[31]*n[5];  // H2 + D+ -> HD + H+, d/d n[1]
+    J[5 + 1 * ld] -= d;
+    J[1 + 1 * ld] -= d;
+    J[7 + 1 * ld] += d;
+    J[3 + 1 * ld] += d;
+    d = k[31]*n[1];  // H2 + D+ -> HD + H+, d/d n[5]
+    J[5 + 5 * ld] -= d;
+    J[1 + 5 * ld] -= d;
+    J[7 + 5 * ld] += d;
+    J[3 + 5 * ld] += d;
+    d = k[32]*n[7];  // HD + H+ -> H2 + D+, d/d n[3]
+    J[7 + 3 * ld] -= d;
+    J[3 + 3 * ld] -= d;
+    J[5 + 3 * ld] += d;
+    J[1 + 3 * ld] += d;
+    d = k[32]*n[3];  // HD + H+ -> H2 + D+, d/d n[7]
+    J[7 + 7 * ld] -= d;
+    J[3 + 7 * ld] -= d;
+    J[5 + 7 * ld] += d;
+    J[1 + 7 * ld] += d;
+    d = k[33]*n[5];  // H2 + D -> HD + H, d/d n[0]
+    J[5 + 0 * ld] -= d;
+    J[0 + 0 * ld] -= d;
+    J[7 + 0 * ld] += d;
+    J[2 + 0 * ld] += d;
+    d = k[33]*n[0];  // H2 + D -> HD + H, d/d n[5]
+    J[5 + 5 * ld] -= d;
+    J[0 + 5 * ld] -= d;
+    J[7 + 5 * ld] += d;
+    J[2 + 5 * ld] += d;
+    d = k[34]*n[5];  // H2 + D -> HD + H, d/d n[0]
+    J[5 + 0 * ld] -= d;
+    J[0 + 0 * ld] -= d;
+    J[7 + 0 * ld] += d;
+    J[2 + 0 * ld] += d;
+    d = k[34]*n[0];  // H2 + D -> HD + H, d/d n[5]
+    J[5 + 5 * ld] -= d;
+    J[0 + 5 * ld] -= d;
+    J[7 + 5 * ld] += d;
+    J[2 + 5 * ld] += d;
+    d = k[35]*n[7];  // HD + H -> H2 + D, d/d n[2]
+    J[7 + 2 * ld] -= d;
+    J[2 + 2 * ld] -= d;
+    J[5 + 2 * ld] += d;
+    J[0 + 2 * ld] += d;
+    d = k[35]*n[2];  // HD + H -> H2 + D, d/d n[7]
+    J[7 + 7 * ld] -= d;
+    J[2 + 7 * ld] -= d;
+    J[5 + 7 * ld] += d;
+    J[0 + 7 * ld] += d;
+    d = k[36]*n[4];  // D + H- -> HD + e-, d/d n[0]
+    J[0 + 0 * ld] -= d;
+    J[4 + 0 * ld] -= d;
+    J[7 + 0 * ld] += d;
+    J[11 + 0 * ld] += d;
+    d = k[36]*n[0];  // D + H- -> HD + e-, d/d n[4]
+    J[0 + 4 * ld] -= d;
+    J[4 + 4 * ld] -= d;
+    J[7 + 4 * ld] += d;
+    J[11 + 4 * ld] += d;
+    d = k[37]*n[11];  // D+ + e- -> D, d/d n[1]
+    J[1 + 1 * ld] -= d;
+    J[11 + 1 * ld] -= d;
+    J[0 + 1 * ld] += d;
+    d = k[37]*n[1];  // D+ + e- -> D, d/d n[11]
+    J[1 + 11 * ld] -= d;
+    J[11 + 11 * ld] -= d;
+    J[0 + 11 * ld] += d;
+    (void)d; (void)n; (void)k; (void)J; (void)ld;
+  }
+};
+}  // namespace kx

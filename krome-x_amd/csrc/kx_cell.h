@@ -1,0 +1,900 @@
+// The per-cell BDF integrator of the plain FP64 drop-in kernel: one GPU
+// thread per cell, every vector and matrix of the cell in a workspace of
+// interleaved cells (element i of thread slot c at c + i S: a wave's 64
+// lanes read 64 consecutive doubles).  It is KROME-X's integrator
+// (runtime/core/cvode/bdf_cell.hpp:491-1547, CVODE 7.9.0's BDF in
+// Nordsieck form, orders 1-5, modified Newton, dense LU with partial
+// pivoting, local error test, order/step selection, restarts), with the
+// same steps, constants and operation order, so that a cell reproduces the
+// CPU solve up to the last bits of the math library.  N, the number of
+// equations, is a compile-time constant: the loops unroll and the
+// problem's rhs works on register arrays.
+#pragma once
+#include <cfloat>
+
+#include "kx_physics.h"
+
+namespace kx {
+
+constexpr int QMAX = 5;
+constexpr int LMAX = QMAX + 1;
+
+// CVODE's flags (bdf_cell.hpp:107-122)
+enum : int {
+  BDF_OK = 0, TOO_MUCH_WORK = -1, TOO_MUCH_ACC = -2, ERR_FAILURE = -3,
+  CONV_FAILURE = -4, LSETUP_FAIL = -6, LSOLVE_FAIL = -7, RHSFUNC_FAIL = -8,
+  FIRST_RHSFUNC_ERR = -9, REPTD_RHSFUNC_ERR = -10, UNREC_RHSFUNC_ERR = -11,
+  NLS_FAIL = -16, ILL_INPUT = -22, TOO_CLOSE = -27
+};
+
+// KROME flags (runtime/krome.hpp:95-145)
+enum : int {
+  FLAG_UNSUPPORTED = -104, FLAG_PARAM_UNSET = -105, FLAG_NONPOSITIVE_T = -110,
+  FLAG_DEVICE_MEMORY = -111
+};
+
+namespace bdfc {
+// cvode.c / cvode_ls.c / cvode_nls.c constants (bdf_cell.hpp:241-272)
+constexpr double HLB_FACTOR = 100.0, HUB_FACTOR = 0.1, H_BIAS = 0.5;
+constexpr int MAX_ITERS = 4;
+constexpr double CORTES = 0.1, ETA_MIN_FX = 0.0, ETA_MAX_FX = 1.5,
+                 ETA_MAX_FS = 10000.0, ETA_MAX_ES = 10.0, ETA_MAX_GS = 10.0,
+                 ETA_MIN = 0.1, ETA_MAX_EF = 0.2, ETA_MIN_EF = 0.1,
+                 ETA_CF = 0.25;
+constexpr long SMALL_NST = 10;
+constexpr int SMALL_NEF = 2;
+constexpr double ADDON = 1e-6, BIAS1 = 6.0, BIAS2 = 6.0, BIAS3 = 10.0;
+constexpr int LONG_WAIT = 10, MXNEF1 = 3;
+constexpr long MSBP = 20, MSBJ = 51;
+constexpr double DGMAX_LSETUP = 0.3, DGMAX_JBAD = 0.2, MIN_INC_MULT = 1000.0;
+constexpr int NLS_MAXCOR = 3;
+constexpr double CRDOWN = 0.3, RDIV = 2.0;
+enum : int {
+  DO_ERROR_TEST = 2, PREDICT_AGAIN = 3, TRY_AGAIN = 5, FIRST_CALL = 6,
+  PREV_CONV_FAIL = 7, PREV_ERR_FAIL = 9, RHSFUNC_RECVR = 10,
+  NLS_CONTINUE = 901, NLS_CONV_RECVR = 902
+};
+enum : int { NO_FAILURES = 0, FAIL_BAD_J = 1, FAIL_OTHER = 2 };
+KX_HD inline double mx(double a, double b) { return a > b ? a : b; }
+KX_HD inline double mn(double a, double b) { return a < b ? a : b; }
+KX_HD inline double rt(double x) { return x <= 0.0 ? 0.0 : ::sqrt(x); }
+}  // namespace bdfc
+
+// options of a cell's integration (bdf_cell.hpp:133-170): the scalar
+// rtol, per-equation absolute tolerances, the step budget and failure
+// limits, the Jacobian (difference quotients or the problem's) and the
+// charge invariant of the Newton corrections (inv_row >= 0: on)
+template <int N>
+struct CellOpts {
+  double rtol;
+  double atol[N];
+  long max_steps;
+  int max_err_fails, max_conv_fails, max_restarts;
+  bool dq_jac;
+  int inv_row, inv_n;
+  int inv_c[N];
+};
+
+// CVODE's counters (bdf_cell.hpp:177-191)
+struct CellStats {
+  long nst, nfe, nfeDQ, nje, nsetups, nni, nnf, ncfn, netf;
+  KX_HD void zero() {
+    nst = nfe = nfeDQ = nje = nsetups = nni = nnf = ncfn = netf = 0;
+  }
+  KX_HD void add(const CellStats& s) {
+    nst += s.nst; nfe += s.nfe; nfeDQ += s.nfeDQ; nje += s.nje;
+    nsetups += s.nsetups; nni += s.nni; nnf += s.nnf; ncfn += s.ncfn;
+    netf += s.netf;
+  }
+};
+
+// doubles of a cell's workspace: Nordsieck array (6 N), 9 vectors, the
+// saved Jacobian and the Newton matrix (2 N^2); pivots are N ints
+template <int N>
+constexpr long cell_doubles() { return (LMAX + 9) * N + 2 * N * N; }
+
+// the integrator of one cell on strided storage; Prob provides
+//   int rhs(const double* y, double* f)   (registers, N values)
+//   int jac(const double* y, double* J)   (registers, N x N col-major,
+//                                          zeroed by the caller)
+template <int N, class Prob>
+class Cell {
+ public:
+  CellStats st;
+
+  KX_HD Cell(Prob& prob, double* work, int* iwork, long stride,
+             const CellOpts<N>& opts)
+      : P(prob), S(stride), o(opts) {
+    double* w = work;
+    for (int j = 0; j < LMAX; j++, w += N * S) zn[j] = w;
+    ewt = w; w += N * S;
+    y = w; w += N * S;
+    acor = w; w += N * S;
+    tempv = w; w += N * S;
+    ftemp = w; w += N * S;
+    delta = w; w += N * S;
+    vt1 = w; w += N * S;
+    vt2 = w; w += N * S;
+    vt3 = w; w += N * S;
+    A = w; w += N * N * S;
+    J = w;
+    piv = iwork;
+    mxstep = o.max_steps == 0 ? 500 : o.max_steps;
+    double m = o.atol[0];
+    for (int i = 1; i < N; i++) m = o.atol[i] < m ? o.atol[i] : m;
+    atolmin0 = m == 0.0;
+  }
+
+  KX_HD double& at(double* v, int i) const { return v[(long)i * S]; }
+
+  KX_HD void reinit(double t0, const double* y0, double h_init) {
+    tn = t0;
+    for (int i = 0; i < N; i++) at(zn[0], i) = y0[i];
+    q = 1; L = 2; qwait = L;
+    etamax = bdfc::ETA_MAX_FS;
+    qu = 0; hu = 0.0; tolsf = 1.0;
+    st.zero();
+    nstlp = nscon = nstlj = 0;
+    h0u = next_h = 0.0;
+    next_q = 0;
+    hin = h_init;
+    acnrmcur = nls_jcur = jcur = false;
+    saved_tq5 = 0.0;
+    for (int i = 0; i <= LMAX; i++) tau[i] = 0.0;
+    eta = 1.0;
+  }
+
+  // CVode(tout, CV_NORMAL) (bdf_cell.hpp:566-638); yout registers
+  KX_HD int solve(double tout, double* yout, double* tret) {
+    using namespace bdfc;
+    *tret = tn;
+    const double tdist = ::fabs(tout - tn);
+    const double tround = DBL_EPSILON * mx(::fabs(tn), ::fabs(tout));
+    if (tdist == 0.0 || tdist < 2.0 * tround) return TOO_CLOSE;
+    if (ewt_set(zn[0], ewt) != 0) return ILL_INPUT;
+    const int r0 = rhs_v(zn[0], zn[1]);
+    st.nfe++;
+    if (r0 < 0) return RHSFUNC_FAIL;
+    if (r0 > 0) return FIRST_RHSFUNC_ERR;
+    h = hin;
+    if (h != 0.0 && (tout - tn) * h < 0.0) return ILL_INPUT;
+    if (h == 0.0) {
+      const int hf = hin_est(tout);
+      if (hf != BDF_OK) return hf;
+    }
+    hscale = h; h0u = h; hprime = h;
+    scal(h, zn[1], zn[1]);
+    long nloc = 0;
+    int istate = BDF_OK;
+    for (;;) {
+      next_h = h;
+      next_q = q;
+      if (st.nst > 0 && ewt_set(zn[0], ewt) != 0) { istate = ILL_INPUT; break; }
+      if (mxstep > 0 && nloc >= mxstep) { istate = TOO_MUCH_WORK; break; }
+      tolsf = DBL_EPSILON * wrms(zn[0], ewt);
+      if (tolsf > 1.0) { istate = TOO_MUCH_ACC; tolsf *= 2.0; break; }
+      tolsf = 1.0;
+      const int kf = step();
+      if (kf != BDF_OK) { istate = kf; break; }
+      nloc++;
+      if ((tn - tout) * h >= 0.0) {
+        *tret = tout;
+        dky(tout, yout);
+        next_q = qprime;
+        next_h = hprime;
+        return BDF_OK;
+      }
+    }
+    *tret = tn;
+    for (int i = 0; i < N; i++) yout[i] = at(zn[0], i);
+    return istate;
+  }
+
+  KX_HD double current_step() const { return next_h; }
+
+ private:
+  Prob& P;
+  const long S;
+  const CellOpts<N>& o;
+  long mxstep;
+  bool atolmin0;
+  double* zn[LMAX];
+  double *ewt, *y, *acor, *tempv, *ftemp, *delta, *vt1, *vt2, *vt3, *A, *J;
+  int* piv;
+  double tn = 0, h = 0, hin = 0, hprime = 0, hscale = 0, h0u = 0, next_h = 0,
+         hu = 0, eta = 1, etamax = 0, etaq = 0, etaqm1 = 0, etaqp1 = 0,
+         tolsf = 1, saved_tq5 = 0;
+  double tau[LMAX + 1], tq[6], l[LMAX];
+  double rl1 = 0, gamma = 0, gammap = 0, gamrat = 0, crate = 0, delp = 0,
+         delnrm = 0, acnrm = 0;
+  int q = 1, qprime = 1, next_q = 0, qu = 0, L = 2, qwait = 2, convfail = 0;
+  long nscon = 0, nstlp = 0, nstlj = 0;
+  bool acnrmcur = false, jcur = false, nls_jcur = false;
+
+  // ---- vector kernels (the serial N_Vector, bdf_cell.hpp:293-400) ----
+  KX_HD int rhs_v(const double* yv, double* fv) {
+    double yr[N], fr[N];
+    for (int i = 0; i < N; i++) yr[i] = at(const_cast<double*>(yv), i);
+    const int r = P.rhs(yr, fr);
+    for (int i = 0; i < N; i++) at(fv, i) = fr[i];
+    return r;
+  }
+  KX_HD void axpy(double a, const double* x, double* yv) {
+    if (a == 1.0) {
+      for (int i = 0; i < N; i++) at(yv, i) += at(const_cast<double*>(x), i);
+    } else if (a == -1.0) {
+      for (int i = 0; i < N; i++) at(yv, i) -= at(const_cast<double*>(x), i);
+    } else {
+      for (int i = 0; i < N; i++)
+        at(yv, i) += a * at(const_cast<double*>(x), i);
+    }
+  }
+  KX_HD void lsum(double a, const double* x_, double b, const double* y_,
+                  double* z) {
+    double* x = const_cast<double*>(x_);
+    double* yv = const_cast<double*>(y_);
+    if (b == 1.0 && z == yv) return axpy(a, x, z);
+    if (a == 1.0 && z == x) return axpy(b, yv, z);
+    if (a == 1.0 && b == 1.0) {
+      for (int i = 0; i < N; i++) at(z, i) = at(x, i) + at(yv, i);
+      return;
+    }
+    if ((a == 1.0 && b == -1.0) || (a == -1.0 && b == 1.0)) {
+      const bool t = a == 1.0;
+      double* p = t ? x : yv;
+      double* m = t ? yv : x;
+      for (int i = 0; i < N; i++) at(z, i) = at(p, i) - at(m, i);
+      return;
+    }
+    if (a == 1.0 || b == 1.0) {
+      const bool t = a == 1.0;
+      const double c = t ? b : a;
+      double* v1 = t ? yv : x;
+      double* v2 = t ? x : yv;
+      for (int i = 0; i < N; i++) at(z, i) = (c * at(v1, i)) + at(v2, i);
+      return;
+    }
+    if (a == -1.0 || b == -1.0) {
+      const bool t = a == -1.0;
+      const double c = t ? b : a;
+      double* v1 = t ? yv : x;
+      double* v2 = t ? x : yv;
+      for (int i = 0; i < N; i++) at(z, i) = (c * at(v1, i)) - at(v2, i);
+      return;
+    }
+    if (a == b) {
+      for (int i = 0; i < N; i++) at(z, i) = a * (at(x, i) + at(yv, i));
+      return;
+    }
+    if (a == -b) {
+      for (int i = 0; i < N; i++) at(z, i) = a * (at(x, i) - at(yv, i));
+      return;
+    }
+    for (int i = 0; i < N; i++) at(z, i) = (a * at(x, i)) + (b * at(yv, i));
+  }
+  KX_HD void scal(double c, const double* x, double* z) {
+    for (int i = 0; i < N; i++) at(z, i) = c * at(const_cast<double*>(x), i);
+  }
+  KX_HD double wrms(const double* x, const double* w) {
+    double s = 0.0;
+    for (int i = 0; i < N; i++) {
+      const double p = at(const_cast<double*>(x), i) *
+                       at(const_cast<double*>(w), i);
+      s += p * p;
+    }
+    return bdfc::rt(s / N);
+  }
+
+  // cvEwtSetSV (bdf_cell.hpp:689-713)
+  KX_HD int ewt_set(const double* yc, double* w) {
+    double v[N];
+    for (int i = 0; i < N; i++) v[i] = ::fabs(at(const_cast<double*>(yc), i));
+    for (int i = 0; i < N; i++) v[i] = (o.rtol * v[i]) + o.atol[i];
+    if (atolmin0) {
+      double m = v[0];
+      for (int i = 1; i < N; i++) m = v[i] < m ? v[i] : m;
+      if (m <= 0.0) return -1;
+    }
+    for (int i = 0; i < N; i++) at(w, i) = 1.0 / v[i];
+    return 0;
+  }
+
+  // cvHin, cvUpperBoundH0, cvYddNorm (bdf_cell.hpp:717-794)
+  KX_HD int hin_est(double tout) {
+    using namespace bdfc;
+    const double tdiff = tout - tn;
+    const int sign = tdiff > 0.0 ? 1 : -1;
+    const double tdist = ::fabs(tdiff);
+    const double tround = DBL_EPSILON * mx(::fabs(tn), ::fabs(tout));
+    const double hlb = HLB_FACTOR * tround;
+    const double hub = hub_est(tdist);
+    double hg = rt(hlb * hub);
+    if (hub < hlb) {
+      h = sign == -1 ? -hg : hg;
+      return BDF_OK;
+    }
+    double hs = hg, hnew = hg, ydd = 0.0;
+    for (int c1 = 1; c1 <= MAX_ITERS; c1++) {
+      bool ok = false;
+      for (int c2 = 1; c2 <= MAX_ITERS; c2++) {
+        const int r = ydd_norm(hg * sign, &ydd);
+        if (r < 0) return RHSFUNC_FAIL;
+        if (r == BDF_OK) { ok = true; break; }
+        hg *= 0.2;
+      }
+      if (!ok) {
+        if (c1 <= 2) return REPTD_RHSFUNC_ERR;
+        hnew = hs;
+        break;
+      }
+      hs = hg;
+      hnew = (ydd * hub * hub > 2.0) ? rt(2.0 / ydd) : rt(hg * hub);
+      if (c1 == MAX_ITERS) break;
+      const double hrat = hnew / hg;
+      if (hrat > 0.5 && hrat < 2.0) break;
+      if (c1 > 1 && hrat > 2.0) { hnew = hg; break; }
+      hg = hnew;
+    }
+    double h0 = H_BIAS * hnew;
+    if (h0 < hlb) h0 = hlb;
+    if (h0 > hub) h0 = hub;
+    h = sign == -1 ? -h0 : h0;
+    return BDF_OK;
+  }
+  KX_HD double hub_est(double tdist) {
+    double* t1 = tempv;
+    double* t2 = acor;
+    for (int i = 0; i < N; i++) at(t2, i) = ::fabs(at(zn[0], i));
+    ewt_set(zn[0], t1);
+    for (int i = 0; i < N; i++) at(t1, i) = 1.0 / at(t1, i);
+    lsum(bdfc::HUB_FACTOR, t2, 1.0, t1, t1);
+    for (int i = 0; i < N; i++) at(t2, i) = ::fabs(at(zn[1], i));
+    for (int i = 0; i < N; i++) at(t1, i) = at(t2, i) / at(t1, i);
+    double inv = 0.0;
+    for (int i = 0; i < N; i++)
+      if (::fabs(at(t1, i)) > inv) inv = ::fabs(at(t1, i));
+    double hub = bdfc::HUB_FACTOR * tdist;
+    if (hub * inv > 1.0) hub = 1.0 / inv;
+    return hub;
+  }
+  KX_HD int ydd_norm(double hg, double* out) {
+    lsum(hg, zn[1], 1.0, zn[0], y);
+    const int r = rhs_v(y, tempv);
+    st.nfe++;
+    if (r < 0) return RHSFUNC_FAIL;
+    if (r > 0) return bdfc::RHSFUNC_RECVR;
+    lsum(1.0 / hg, tempv, -1.0 / hg, zn[1], tempv);
+    *out = wrms(tempv, ewt);
+    return BDF_OK;
+  }
+
+  // cvStep (bdf_cell.hpp:798-822)
+  KX_HD int step() {
+    using namespace bdfc;
+    int ncf = 0, nef = 0;
+    double dsm = 0.0;
+    if (st.nst > 0 && hprime != h) adjust();
+    const double saved_t = tn;
+    int nflag = FIRST_CALL;
+    for (;;) {
+      predict();
+      set_coeffs();
+      nflag = nls(nflag);
+      const int kf = handle_nflag(&nflag, saved_t, &ncf);
+      if (kf == PREDICT_AGAIN) continue;
+      if (kf != DO_ERROR_TEST) return kf;
+      const int ef = error_test(&nflag, saved_t, &nef, &dsm);
+      if (ef == TRY_AGAIN) continue;
+      if (ef != BDF_OK) return ef;
+      break;
+    }
+    complete();
+    next_step(dsm);
+    etamax = st.nst <= SMALL_NST ? ETA_MAX_ES : ETA_MAX_GS;
+    scal(tq[2], acor, acor);
+    return BDF_OK;
+  }
+
+  // order and step size changes (bdf_cell.hpp:825-887)
+  KX_HD void adjust() {
+    if (qprime != q) {
+      order_change(qprime - q);
+      q = qprime;
+      L = q + 1;
+      qwait = L;
+    }
+    rescale();
+  }
+  KX_HD void order_change(int dq) {
+    if (q == 2 && dq != 1) return;
+    if (dq == 1) order_up();
+    else if (dq == -1) order_down();
+  }
+  KX_HD void order_up() {
+    for (int i = 0; i <= QMAX; i++) l[i] = 0.0;
+    double a1 = 1.0, prod = 1.0, xiold = 1.0, a0 = -1.0;
+    l[2] = 1.0;
+    double hsum = hscale;
+    for (int j = 1; j < q; j++) {
+      hsum += tau[j + 1];
+      const double xi = hsum / hscale;
+      prod *= xi;
+      a0 -= 1.0 / (j + 1);
+      a1 += 1.0 / xi;
+      for (int i = j + 2; i >= 2; i--) l[i] = l[i] * xiold + l[i - 1];
+      xiold = xi;
+    }
+    const double A1 = (-a0 - a1) / prod;
+    scal(A1, zn[QMAX], zn[L]);
+    for (int j = 2; j <= q; j++) axpy(l[j], zn[L], zn[j]);
+  }
+  KX_HD void order_down() {
+    for (int i = 0; i <= QMAX; i++) l[i] = 0.0;
+    l[2] = 1.0;
+    double hsum = 0.0;
+    for (int j = 1; j <= q - 2; j++) {
+      hsum += tau[j];
+      const double xi = hsum / hscale;
+      for (int i = j + 2; i >= 2; i--) l[i] = l[i] * xi + l[i - 1];
+    }
+    for (int j = 2; j < q; j++) axpy(-l[j], zn[q], zn[j]);
+  }
+  KX_HD void rescale() {
+    double c = eta;
+    for (int j = 1; j <= q; j++) {
+      scal(c, zn[j], zn[j]);
+      c = eta * c;
+    }
+    h = hscale * eta;
+    next_h = h;
+    hscale = h;
+    nscon = 0;
+  }
+  KX_HD void predict() {
+    tn += h;
+    for (int k = 1; k <= q; k++)
+      for (int j = q; j >= k; j--) axpy(1.0, zn[j], zn[j - 1]);
+  }
+  KX_HD void restore(double saved_t) {
+    tn = saved_t;
+    for (int k = 1; k <= q; k++)
+      for (int j = q; j >= k; j--) axpy(-1.0, zn[j], zn[j - 1]);
+  }
+
+  // cvSet, cvSetBDF, cvSetTqBDF (bdf_cell.hpp:903-954)
+  KX_HD void set_coeffs() {
+    double xi_inv = 1.0, xistar_inv = 1.0, a0 = -1.0, a0h = -1.0;
+    l[0] = l[1] = 1.0;
+    for (int i = 2; i <= q; i++) l[i] = 0.0;
+    double hsum = h;
+    if (q > 1) {
+      for (int j = 2; j < q; j++) {
+        hsum += tau[j - 1];
+        xi_inv = h / hsum;
+        a0 -= 1.0 / j;
+        for (int i = j; i >= 1; i--) l[i] += l[i - 1] * xi_inv;
+      }
+      a0 -= 1.0 / q;
+      xistar_inv = -l[1] - a0;
+      hsum += tau[q - 1];
+      xi_inv = h / hsum;
+      a0h = -l[1] - xi_inv;
+      for (int i = q; i >= 1; i--) l[i] += l[i - 1] * xistar_inv;
+    }
+    const double A1 = 1.0 - a0h + a0;
+    const double A2 = 1.0 + q * A1;
+    tq[2] = ::fabs(A1 / (a0 * A2));
+    tq[5] = ::fabs(A2 * xistar_inv / (l[q] * xi_inv));
+    if (qwait == 1) {
+      if (q > 1) {
+        const double C = xistar_inv / l[q];
+        const double A3 = a0 + 1.0 / q;
+        const double A4 = a0h + xi_inv;
+        const double Cpinv = (1.0 - A4 + A3) / A3;
+        tq[1] = ::fabs(C * Cpinv);
+      } else {
+        tq[1] = 1.0;
+      }
+      hsum += tau[q];
+      xi_inv = h / hsum;
+      const double A5 = a0 - (1.0 / (q + 1));
+      const double A6 = a0h - xi_inv;
+      const double Cppinv = (1.0 - A6 + A5) / A2;
+      tq[3] = ::fabs(Cppinv / (xi_inv * (q + 2) * A5));
+    }
+    tq[4] = bdfc::CORTES / tq[2];
+    rl1 = 1.0 / l[1];
+    gamma = h * rl1;
+    if (st.nst == 0) gammap = gamma;
+    gamrat = st.nst > 0 ? gamma / gammap : 1.0;
+  }
+
+  // the Newton iteration (bdf_cell.hpp:958-1065)
+  KX_HD int nls(int nflag) {
+    using namespace bdfc;
+    convfail = (nflag == FIRST_CALL || nflag == PREV_ERR_FAIL) ? NO_FAILURES
+                                                               : FAIL_OTHER;
+    bool setup = nflag == PREV_CONV_FAIL || nflag == PREV_ERR_FAIL ||
+                 st.nst == 0 || st.nst >= nstlp + MSBP ||
+                 ::fabs(gamrat - 1.0) > DGMAX_LSETUP;
+    for (int i = 0; i < N; i++) at(acor, i) = 0.0;
+    const double tol = tq[4];
+    bool jbad = false;
+    long niters = 0, nfails = 0;
+    int r;
+    for (;;) {
+      int it = 0;
+      r = residual();
+      if (r != BDF_OK) break;
+      if (setup) {
+        r = lsetup(jbad, &nls_jcur);
+        if (r != BDF_OK) break;
+      }
+      for (;;) {
+        niters++;
+        const bool inv = o.dq_jac && o.inv_row >= 0;
+        const double cg = inv ? inv_dot(delta) : 0.0;
+        scal(-1.0, delta, delta);
+        lsolve(delta);
+        if (inv) inv_set(delta, -cg);
+        lsum(1.0, acor, 1.0, delta, acor);
+        r = conv_test(it, tol);
+        it++;
+        if (r == BDF_OK) { nls_jcur = false; break; }
+        if (r != NLS_CONTINUE) break;
+        if (it >= NLS_MAXCOR) { r = NLS_CONV_RECVR; break; }
+        r = residual();
+        if (r != BDF_OK) break;
+      }
+      if (r == BDF_OK) break;
+      if (r > 0 && !nls_jcur) {
+        nfails++;
+        setup = true;
+        jbad = true;
+        for (int i = 0; i < N; i++) at(acor, i) = 0.0;
+        continue;
+      }
+      break;
+    }
+    if (r != BDF_OK) nfails++;
+    st.nni += niters;
+    st.nnf += nfails;
+    if (r != BDF_OK) return r;
+    lsum(1.0, zn[0], 1.0, acor, y);
+    if (!acnrmcur) acnrm = wrms(acor, ewt);
+    jcur = false;
+    return BDF_OK;
+  }
+  KX_HD int residual() {
+    lsum(1.0, zn[0], 1.0, acor, y);
+    const int r = rhs_v(y, ftemp);
+    st.nfe++;
+    if (r < 0) return RHSFUNC_FAIL;
+    if (r > 0) return bdfc::RHSFUNC_RECVR;
+    lsum(rl1, zn[1], 1.0, acor, delta);
+    lsum(-gamma, ftemp, 1.0, delta, delta);
+    return BDF_OK;
+  }
+  KX_HD int conv_test(int m, double tol) {
+    using namespace bdfc;
+    delnrm = wrms(delta, ewt);
+    if (m > 0) crate = mx(CRDOWN * crate, delnrm / delp);
+    const double dcon = delnrm * mn(1.0, crate) / tol;
+    if (dcon <= 1.0) {
+      acnrm = m == 0 ? delnrm : wrms(acor, ewt);
+      acnrmcur = true;
+      return BDF_OK;
+    }
+    if (m >= 1 && delnrm > RDIV * delp) return NLS_CONV_RECVR;
+    delp = delnrm;
+    return NLS_CONTINUE;
+  }
+  KX_HD int lsetup(bool jbad, bool* jcur_out) {
+    if (jbad) convfail = bdfc::FAIL_BAD_J;
+    const int r = ls_setup();
+    st.nsetups++;
+    *jcur_out = jcur;
+    gamrat = 1.0;
+    gammap = gamma;
+    crate = 1.0;
+    delnrm = 0.0;
+    nstlp = st.nst;
+    if (r < 0) return LSETUP_FAIL;
+    if (r > 0) return bdfc::NLS_CONV_RECVR;
+    return BDF_OK;
+  }
+  // J reused or re-evaluated, A = I - gamma J, LU (bdf_cell.hpp:1069-1130)
+  KX_HD int ls_setup() {
+    using namespace bdfc;
+    const double dg = ::fabs((gamma / gammap) - 1.0);
+    const bool jbad = st.nst == 0 || st.nst >= nstlj + MSBJ ||
+                      (convfail == FAIL_BAD_J && dg < DGMAX_JBAD) ||
+                      convfail == FAIL_OTHER;
+    int r = 0;
+    if (!jbad) {
+      jcur = false;
+    } else {
+      jcur = true;
+      const int jr = o.dq_jac ? dq_jac() : an_jac();
+      if (jr < 0) r = -1;
+      else if (jr > 0) r = 1;
+    }
+    if (r == 0) {
+      const double c = -gamma;
+      for (int j = 0; j < N; j++) {
+        for (int i = 0; i < N; i++) at(A, i + j * N) = at(J, i + j * N) * c;
+        at(A, j + j * N) += 1.0;
+      }
+    }
+    if (jcur) {
+      st.nje++;
+      nstlj = st.nst;
+    }
+    if (r != 0) return r;
+    return lu() > 0 ? 1 : 0;
+  }
+  KX_HD int an_jac() {
+    double yr[N], Jr[N * N];
+    for (int i = 0; i < N; i++) yr[i] = at(y, i);
+    for (int k = 0; k < N * N; k++) Jr[k] = 0.0;
+    const int r = P.jac(yr, Jr);
+    for (int k = 0; k < N * N; k++) at(J, k) = Jr[k];
+    return r;
+  }
+  // cvLsDenseDQJac (bdf_cell.hpp:1290-1311)
+  KX_HD int dq_jac() {
+    const double srur = bdfc::rt(DBL_EPSILON);
+    const double fn = wrms(ftemp, ewt);
+    const double minInc =
+        fn != 0.0 ? (bdfc::MIN_INC_MULT * ::fabs(h) * DBL_EPSILON * (double)N *
+                     fn)
+                  : 1.0;
+    int r = 0;
+    for (int j = 0; j < N; j++) {
+      const double ys = at(y, j);
+      const double inc = bdfc::mx(srur * ::fabs(ys), minInc / at(ewt, j));
+      at(y, j) += inc;
+      r = rhs_v(y, vt1);
+      st.nfeDQ++;
+      if (r != 0) break;
+      at(y, j) = ys;
+      const double ii = 1.0 / inc;
+      lsum(ii, vt1, -ii, ftemp, J + (long)j * N * S);
+    }
+    return r;
+  }
+  // dense LU with partial pivoting (bdf_cell.hpp:402-433)
+  KX_HD int lu() {
+    for (int k = 0; k < N; k++) {
+      int l_ = k;
+      for (int i = k + 1; i < N; i++)
+        if (::fabs(at(A, i + k * N)) > ::fabs(at(A, l_ + k * N))) l_ = i;
+      piv[k * S] = l_;
+      if (at(A, l_ + k * N) == 0.0) return k + 1;
+      if (l_ != k)
+        for (int j = 0; j < N; j++) {
+          const double t = at(A, l_ + j * N);
+          at(A, l_ + j * N) = at(A, k + j * N);
+          at(A, k + j * N) = t;
+        }
+      const double rr = 1.0 / at(A, k + k * N);
+      for (int i = k + 1; i < N; i++) at(A, i + k * N) *= rr;
+      for (int j = k + 1; j < N; j++) {
+        const double akj = at(A, k + j * N);
+        if (akj != 0.0)
+          for (int i = k + 1; i < N; i++)
+            at(A, i + j * N) -= akj * at(A, i + k * N);
+      }
+    }
+    return 0;
+  }
+  // triangular solves and the gamma-ratio scaling (bdf_cell.hpp:435-457,
+  // 1134-1153)
+  KX_HD void lsolve(double* bv) {
+    double b[N];
+    for (int i = 0; i < N; i++) b[i] = at(bv, i);
+    for (int k = 0; k < N; k++) {
+      const int pk = piv[k * S];
+      if (pk != k) {
+        double t = b[k], u = 0.0;
+        for (int i = 0; i < N; i++) u = i == pk ? b[i] : u;
+        for (int i = 0; i < N; i++) b[i] = i == pk ? t : b[i];
+        b[k] = u;
+      }
+    }
+    for (int k = 0; k < N - 1; k++)
+      for (int i = k + 1; i < N; i++) b[i] -= at(A, i + k * N) * b[k];
+    for (int k = N - 1; k > 0; k--) {
+      b[k] /= at(A, k + k * N);
+      for (int i = 0; i < k; i++) b[i] -= at(A, i + k * N) * b[k];
+    }
+    b[0] /= at(A, 0);
+    if (gamrat != 1.0) {
+      const double c = 2.0 / (1.0 + gamrat);
+      for (int i = 0; i < N; i++) b[i] = c * b[i];
+    }
+    for (int i = 0; i < N; i++) at(bv, i) = b[i];
+  }
+  // the charge invariant (bdf_cell.hpp:1267-1287)
+  KX_HD double inv_dot(double* x) {
+    double s = 0.0;
+    for (int i = 0; i < N; i++)
+      if (i < o.inv_n && o.inv_c[i] != 0) s += o.inv_c[i] * at(x, i);
+    return s;
+  }
+  KX_HD void inv_set(double* x, double target) {
+    const int r = o.inv_row;
+    double s = 0.0, sa = 0.0, yr = 0.0, cr = 0.0;
+    for (int i = 0; i < N; i++) {
+      if (i < o.inv_n && i != r && o.inv_c[i] != 0) {
+        s += o.inv_c[i] * at(x, i);
+        sa += ::fabs(o.inv_c[i] * (at(zn[0], i) + at(acor, i)));
+      }
+      if (i == r) {
+        yr = at(zn[0], i) + at(acor, i);
+        cr = o.inv_c[i];
+      }
+    }
+    const double xr = (target - s) / cr;
+    if (::fabs(cr * yr) >= 1e-6 * sa && yr + xr >= 0.0) at(x, r) = xr;
+  }
+  // cvHandleNFlag (bdf_cell.hpp:1315-1336)
+  KX_HD int handle_nflag(int* nf, double saved_t, int* ncf) {
+    using namespace bdfc;
+    const int f = *nf;
+    if (f == BDF_OK) return DO_ERROR_TEST;
+    st.ncfn++;
+    restore(saved_t);
+    if (f < 0) {
+      if (f == LSETUP_FAIL || f == LSOLVE_FAIL || f == RHSFUNC_FAIL) return f;
+      return NLS_FAIL;
+    }
+    (*ncf)++;
+    etamax = 1.0;
+    if (::fabs(h) <= 0.0 || *ncf == o.max_conv_fails) {
+      if (f == NLS_CONV_RECVR) return CONV_FAILURE;
+      if (f == RHSFUNC_RECVR) return REPTD_RHSFUNC_ERR;
+    }
+    eta = mx(ETA_CF, 0.0);
+    *nf = PREV_CONV_FAIL;
+    rescale();
+    return PREDICT_AGAIN;
+  }
+  // cvDoErrorTest (bdf_cell.hpp:1339-1381)
+  KX_HD int error_test(int* nf, double saved_t, int* nef, double* dsmp) {
+    using namespace bdfc;
+    const double dsm = acnrm * tq[2];
+    *dsmp = dsm;
+    if (dsm <= 1.0) return BDF_OK;
+    (*nef)++;
+    st.netf++;
+    *nf = PREV_ERR_FAIL;
+    restore(saved_t);
+    if (::fabs(h) <= 0.0 || *nef == o.max_err_fails) return ERR_FAILURE;
+    etamax = 1.0;
+    if (*nef <= MXNEF1) {
+      eta = 1.0 / (::pow(BIAS2 * dsm, 1.0 / L) + ADDON);
+      eta = mx(ETA_MIN_EF, mx(eta, 0.0));
+      if (*nef >= SMALL_NEF) eta = mn(eta, ETA_MAX_EF);
+      rescale();
+      return TRY_AGAIN;
+    }
+    if (q > 1) {
+      eta = mx(ETA_MIN_EF, 0.0);
+      order_change(-1);
+      L = q;
+      q--;
+      qwait = L;
+      rescale();
+      return TRY_AGAIN;
+    }
+    eta = mx(ETA_MIN_EF, 0.0);
+    h *= eta;
+    next_h = h;
+    hscale = h;
+    qwait = LONG_WAIT;
+    nscon = 0;
+    const int r = rhs_v(zn[0], tempv);
+    st.nfe++;
+    if (r < 0) return RHSFUNC_FAIL;
+    if (r > 0) return UNREC_RHSFUNC_ERR;
+    scal(h, tempv, zn[1]);
+    return TRY_AGAIN;
+  }
+  // cvCompleteStep (bdf_cell.hpp:1386-1400)
+  KX_HD void complete() {
+    st.nst++;
+    nscon++;
+    hu = h;
+    qu = q;
+    for (int i = q; i >= 2; i--) tau[i] = tau[i - 1];
+    if (q == 1 && st.nst > 1) tau[2] = tau[1];
+    tau[1] = h;
+    for (int j = 0; j <= q; j++) axpy(l[j], acor, zn[j]);
+    qwait--;
+    if (qwait == 1 && q != QMAX) {
+      for (int i = 0; i < N; i++) at(zn[QMAX], i) = at(acor, i);
+      saved_tq5 = tq[5];
+    }
+  }
+  // cvPrepareNextStep and the order / step choice (bdf_cell.hpp:1403-1488)
+  KX_HD void next_step(double dsm) {
+    using namespace bdfc;
+    if (etamax == 1.0) {
+      qwait = qwait > 2 ? qwait : 2;
+      qprime = q;
+      hprime = h;
+      eta = 1.0;
+      return;
+    }
+    etaq = 1.0 / (::pow(BIAS2 * dsm, 1.0 / L) + ADDON);
+    if (qwait != 0) {
+      eta = etaq;
+      qprime = q;
+      set_eta();
+      return;
+    }
+    qwait = 2;
+    etaqm1 = 0.0;
+    if (q > 1) {
+      const double ddn = wrms(zn[q], ewt) * tq[1];
+      etaqm1 = 1.0 / (::pow(BIAS1 * ddn, 1.0 / q) + ADDON);
+    }
+    etaqp1 = 0.0;
+    if (q != QMAX && saved_tq5 != 0.0) {
+      double p = 1.0;
+      const double b = h / tau[2];
+      for (int i = 1; i <= L; i++) p *= b;
+      const double cquot = (tq[5] / saved_tq5) * p;
+      lsum(-cquot, zn[QMAX], 1.0, acor, tempv);
+      const double dup = wrms(tempv, ewt) * tq[3];
+      etaqp1 = 1.0 / (::pow(BIAS3 * dup, 1.0 / (L + 1)) + ADDON);
+    }
+    const double etam = mx(etaqm1, mx(etaq, etaqp1));
+    if (etam > ETA_MIN_FX && etam < ETA_MAX_FX) {
+      eta = 1.0;
+      qprime = q;
+    } else if (etam == etaq) {
+      eta = etaq;
+      qprime = q;
+    } else if (etam == etaqm1) {
+      eta = etaqm1;
+      qprime = q - 1;
+    } else {
+      eta = etaqp1;
+      qprime = q + 1;
+      for (int i = 0; i < N; i++) at(zn[QMAX], i) = at(acor, i);
+    }
+    set_eta();
+  }
+  KX_HD void set_eta() {
+    using namespace bdfc;
+    if (eta > ETA_MIN_FX && eta < ETA_MAX_FX) {
+      eta = 1.0;
+      hprime = h;
+      return;
+    }
+    if (eta >= ETA_MAX_FX) {
+      eta = mn(eta, etamax);
+    } else {
+      eta = mx(eta, ETA_MIN);
+    }
+    hprime = h * eta;
+    if (qprime < q) nscon = 0;
+  }
+  // CVodeGetDky(t, 0) (bdf_cell.hpp:1492-1500)
+  KX_HD void dky(double t, double* out) {
+    const double s = (t - tn) / h;
+    for (int i = 0; i < N; i++) out[i] = 0.0;
+    for (int j = q; j >= 0; j--) {
+      double c = 1.0;
+      for (int i = 0; i < j; i++) c *= s;
+      if (j == q)
+        for (int i = 0; i < N; i++) out[i] = c * at(zn[j], i);
+      else
+        for (int i = 0; i < N; i++) out[i] += c * at(zn[j], i);
+    }
+  }
+};
+
+}  // namespace kx

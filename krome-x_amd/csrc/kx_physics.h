@@ -1,0 +1,300 @@
+// Runtime physics of the per-cell right-hand side, for host (the CPU
+// oracle, g++) and gfx950 device code (hipcc), each a template on the real
+// type R (double: the FP64 path; float: the FP32 Jacobian of the mixed
+// path).  The formulas are the published fits KROME-X evaluates; their
+// coefficients are physical data and are the same numbers:
+//   cooling_H2          Glover & Abel 2008 + HM79/Glover 2015 LTE limit
+//                       (runtime/core/cooling.hpp:74-177)
+//   cooling_continuum   Lenzuni+96 Planck opacity with a Jeans-length
+//                       escape fraction (cooling.hpp:283-303)
+//   dissH2_Martin96     Martin, Schwarz & Mandy 1996 H2 collisional
+//                       dissociation (runtime/core/special_rates.hpp:20-66)
+//   h2_heat_fraction    HM79 thermalized fraction of the H2 formation
+//                       heat (kromex/codegen.py:5005-5011)
+//   h2_opacity_ripamonti  Ripamonti & Abel 2004 (codegen.py:3657-3660)
+//   gamma_rot, gamma_rot_ortho_para  exact rotational heat capacities
+//                       (codegen.py:3745-3779)
+#pragma once
+#include <cmath>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define KX_HD __host__ __device__
+#else
+#define KX_HD
+#endif
+
+namespace kx {
+
+// the math of both precisions under one name
+KX_HD inline double exp(double x) { return ::exp(x); }
+KX_HD inline float exp(float x) { return ::expf(x); }
+KX_HD inline double log(double x) { return ::log(x); }
+KX_HD inline float log(float x) { return ::logf(x); }
+KX_HD inline double log10(double x) { return ::log10(x); }
+KX_HD inline float log10(float x) { return ::log10f(x); }
+KX_HD inline double sqrt(double x) { return ::sqrt(x); }
+KX_HD inline float sqrt(float x) { return ::sqrtf(x); }
+KX_HD inline double pow(double x, double y) { return ::pow(x, y); }
+KX_HD inline float pow(float x, float y) { return ::powf(x, y); }
+KX_HD inline double tanh(double x) { return ::tanh(x); }
+KX_HD inline float tanh(float x) { return ::tanhf(x); }
+KX_HD inline double fabs(double x) { return ::fabs(x); }
+KX_HD inline float fabs(float x) { return ::fabsf(x); }
+KX_HD inline double fmin(double x, double y) { return ::fmin(x, y); }
+KX_HD inline float fmin(float x, float y) { return ::fminf(x, y); }
+KX_HD inline double fmax(double x, double y) { return ::fmax(x, y); }
+KX_HD inline float fmax(float x, float y) { return ::fmaxf(x, y); }
+
+// x^N for an integer constant N by binary powering
+template <int N, class R>
+KX_HD inline R ipow(R x) {
+  if constexpr (N < 0) {
+    return R(1) / ipow<-N>(x);
+  } else if constexpr (N == 0) {
+    (void)x;
+    return R(1);
+  } else if constexpr (N == 1) {
+    return x;
+  } else {
+    const R h = ipow<N / 2>(x);
+    if constexpr (N % 2 == 1) return h * h * x;
+    else return h * h;
+  }
+}
+
+namespace phys {
+constexpr double kB = 1.380649e-16;           // erg / K
+constexpr double p_mass = 1.67262192369e-24;  // g
+constexpr double gravity = 6.6743e-08;        // cm^3 g^-1 s^-2
+constexpr double stefboltz = 5.6703744191844286e-05;
+constexpr double pi = 3.141592653589793;
+}  // namespace phys
+
+// smooth window in log T: 10^(200 (s(x, -0.2) s(-x, -1.2) - 1)) with
+// s(x, x0) = 10 / (10 + exp(-50 (x - x0))), x the position of log T in
+// [lmin, lmax]; 0 below 1e-199
+template <class R>
+KX_HD inline R cool_window(R logT, R lmin, R lmax) {
+  const R x = (logT - lmin) / (lmax - lmin);
+  const R s1 = R(10) / (R(10) + kx::exp(R(-50) * (x - R(-0.2))));
+  const R s2 = R(10) / (R(10) + kx::exp(R(-50) * (-x - R(-1.2))));
+  const R w = kx::pow(R(10), R(200) * (s1 * s2 - R(1)));
+  return w < R(1e-199) ? R(0) : w;
+}
+
+// H2 line cooling [erg cm^-3 s^-1]: the low-density rates of the
+// colliders H, H+, H2, e-, He combined with the LTE limit HDL as
+// n(H2) / (1/HDL + 1/L_low)
+template <class R>
+KX_HD inline R cooling_H2(R T, R nH, R nHj, R nH2, R ne, R nHe) {
+  const R T3 = T * R(1e-3);
+  const R l3 = kx::log10(T3);
+  const R lT = kx::log10(T);
+  const R l2 = l3 * l3, l3_ = l2 * l3, l4 = l3_ * l3, l5 = l4 * l3,
+          l6 = l5 * l3, l7 = l6 * l3, l8 = l7 * l3;
+  const R w14 = cool_window(lT, R(1), R(4));
+  const R w24 = cool_window(lT, R(2), R(4));
+  R cool = R(0);
+  // H2-H
+  R f;
+  if (T <= R(1e2))
+    f = kx::pow(R(10), R(-16.818342) + R(37.383713) * l3 + R(58.145166) * l2
+                + R(48.656103) * l3_ + R(20.159831) * l4 + R(3.8479610) * l5);
+  else if (T <= R(1e3))
+    f = kx::pow(R(10), R(-24.311209) + R(3.5692468) * l3 - R(11.332860) * l2
+                - R(27.850082) * l3_ - R(21.328264) * l4 - R(4.2519023) * l5);
+  else if (T <= R(6e3))
+    f = kx::pow(R(10), R(-24.311209) + R(4.6450521) * l3 - R(3.7209846) * l2
+                + R(5.9369081) * l3_ - R(5.5108049) * l4 + R(1.5538288) * l5);
+  else
+    f = R(1.862314467912518e-22) * cool_window(lT, R(1), kx::log10(R(6e3)));
+  cool += f * nH;
+  // H2-H+
+  if (T > R(1e1) && T <= R(1e4))
+    f = kx::pow(R(10), R(-22.089523) + R(1.5714711) * l3 + R(0.015391166) * l2
+                - R(0.23619985) * l3_ - R(0.51002221) * l4
+                + R(0.32168730) * l5) * nHj;
+  else
+    f = R(1.182509139382060e-21) * nHj * w14;
+  cool += f;
+  // H2-H2
+  cool += w24 * kx::pow(R(10), R(-23.962112) + R(2.09433740) * l3
+                        - R(0.77151436) * l2 + R(0.43693353) * l3_
+                        - R(0.14913216) * l4 - R(0.033638326) * l5) * nH2;
+  // H2-e
+  if (T <= R(5e2))
+    f = kx::pow(R(10), kx::fmin(R(-21.928796) + R(16.815730) * l3
+                                + R(96.743155) * l2 + R(343.19180) * l3_
+                                + R(734.71651) * l4 + R(983.67576) * l5
+                                + R(801.81247) * l6 + R(364.14446) * l7
+                                + R(70.609154) * l8, R(30))) * ne;
+  else
+    f = kx::pow(R(10), R(-22.921189) + R(1.6802758) * l3 + R(0.93310622) * l2
+                + R(4.0406627) * l3_ - R(4.7274036) * l4 - R(8.8077017) * l5
+                + R(8.9167183) * l6 + R(6.4380698) * l7
+                - R(6.3701156) * l8) * ne;
+  cool += f * w24;
+  // H2-He
+  if (T > R(1e1) && T <= R(1e4))
+    f = kx::pow(R(10), R(-23.689237) + R(2.1892372) * l3 - R(0.81520438) * l2
+                + R(0.29036281) * l3_ - R(0.16596184) * l4
+                + R(0.19191375) * l5) * nHe;
+  else
+    f = R(1.002560385050777e-22) * nHe * w14;
+  cool += f;
+  if (cool <= R(0)) return R(0);
+  // LTE limit
+  R HDL;
+  if (T < R(2e3)) {
+    const R HDLR = R(9.5e-22) * kx::pow(T3, R(3.76))
+                       / (R(1) + R(0.12) * kx::pow(T3, R(2.1)))
+                       * kx::exp(-kx::pow(R(0.13) / T3, R(3)))
+                   + R(3e-24) * kx::exp(R(-0.51) / T3);
+    const R HDLV = R(6.7e-19) * kx::exp(R(-5.86) / T3)
+                   + R(1.6e-18) * kx::exp(R(-11.7) / T3);
+    HDL = HDLR + HDLV;
+  } else if (T <= R(1e4)) {
+    HDL = kx::pow(R(10), R(-20.584225) + R(5.0194035) * l3 - R(1.5738805) * l2
+                  - R(4.7155769) * l3_ + R(2.4714161) * l4
+                  + R(5.4710750) * l5 - R(3.9467356) * l6
+                  - R(2.2148338) * l7 + R(1.8161874) * l8);
+  } else {
+    const R d = R(1) / (R(1) + kx::exp(kx::fmin((T - R(3e4)) * R(2e-4),
+                                                   R(3e2))));
+    HDL = R(5.531333679406485e-19) * d;
+  }
+  if (HDL == R(0)) return R(0);
+  return nH2 / (R(1) / HDL + R(1) / cool);
+}
+
+// optically thick H2 line cooling factor (Ripamonti & Abel 2004)
+template <class R>
+KX_HD inline R h2_opacity_ripamonti(R ntot) {
+  return kx::fmin(R(1), kx::pow(kx::fmax(R(1.25e-10) * ntot, R(1e-40)),
+                                R(-0.45)));
+}
+
+// continuum cooling [erg cm^-3 s^-1]: 4 sigma T^4 kappa_P rho, times the
+// escape fraction min(1, tau^-2) over a Jeans length
+template <class R>
+KX_HD inline R cooling_continuum(R T, R rho_in, R ntot) {
+  R kgas = R(0);
+  if (!(rho_in < R(1e-12)))
+    kgas = kx::pow(R(10), R(1.000042) * kx::log10(kx::fmin(rho_in, R(0.5)))
+                              + R(2.14989));
+  const R rho = kx::fmax(rho_in, R(1e-40));
+  const R mu = rho / kx::fmax(ntot, R(1e-40)) / R(phys::p_mass);
+  const R lj = kx::sqrt(R(phys::pi) * R(phys::kB) * T / rho / R(phys::p_mass)
+                        / R(phys::gravity) / mu);
+  const R tau = lj * kgas * rho + R(1e-40);
+  const R beta = kx::fmin(R(1), R(1) / (tau * tau));
+  return R(4) * R(phys::stefboltz) * T * T * T * T * kgas * rho * beta;
+}
+
+// thermalized fraction 1 / (1 + ncr / n_H) of the H2 formation heat (HM79
+// critical density; dd the H nuclei density)
+template <class R>
+KX_HD inline R h2_heat_fraction(R T, R nH, R nH2, R dd) {
+  const R a = R(4e2) / T;
+  const R ncr = R(1e6) / kx::sqrt(T)
+                / (R(1.6) * kx::exp(-(a * a)) * nH / dd
+                   + R(1.4) * kx::exp(R(-1.2e4) / (T + R(1.2e3))) * nH2 / dd);
+  return R(1) / (R(1) + ncr / dd);
+}
+
+// rotational heat capacity C_rot / k of a diatomic with rotational
+// constant brot [K]: the variance of E_j / T over the 11 lowest levels
+template <class R>
+KX_HD inline R gamma_rot(R T, R brot) {
+  const R ib = brot / kx::fmax(T, R(10));
+  R z = R(0), e1 = R(0), e2 = R(0);
+  for (int j = 0; j <= 10; ++j) {
+    const R E = R(j * (j + 1)) * ib;
+    const R g = R(2 * j + 1) * kx::exp(-E);
+    z += g;
+    e1 += g * E;
+    e2 += g * E * E;
+  }
+  const R m = e1 / z;
+  return e2 / z - m * m;
+}
+
+// the same for H2 as a mixture of para (even J) and ortho (odd J, weight
+// 3) with ortho fraction a, rotational constant 85.4 K
+template <class R>
+KX_HD inline R gamma_rot_ortho_para(R T, R a) {
+  const R ib = R(85.4) / kx::fmax(T, R(10));
+  R zp = R(0), p1 = R(0), p2 = R(0), zo = R(0), o1 = R(0), o2 = R(0);
+  for (int j = 0; j <= 10; j += 2) {
+    const int j1 = j + 1;
+    const R Ep = R(j * (j + 1)) * ib;
+    const R Eo = R(j1 * (j1 + 1)) * ib;
+    const R gp = R(2 * j + 1) * kx::exp(-Ep);
+    const R go = R(3 * (2 * j1 + 1)) * kx::exp(-Eo);
+    zp += gp;
+    p1 += gp * Ep;
+    p2 += gp * Ep * Ep;
+    zo += go;
+    o1 += go * Eo;
+    o2 += go * Eo * Eo;
+  }
+  const R b = R(1) - a;
+  const R mp = p1 / zp, mo = o1 / zo;
+  return b * (p2 / zp - mp * mp) + a * (o2 / zo - mo * mo);
+}
+
+// H2 collisional dissociation by H (Martin, Schwarz & Mandy 1996):
+// collision-induced dissociation plus dissociative tunnelling, each a
+// density interpolation between low- and high-density limits; n_H the H
+// nuclei density, the fit evaluated at min(T, 4.5e4 K)
+template <class R>
+KX_HD inline R dissH2_Martin96(R Tin, R n_H) {
+  const R T = kx::fmin(Tin, R(4.5e4));
+  const R lT = kx::log10(T);
+  const R iT = R(1) / T;
+  const R lT2 = lT * lT, lT3 = lT2 * lT;
+  R k = R(0);
+#pragma unroll
+  for (int ch = 0; ch < 2; ++ch) {
+    // the fit coefficients of the two channels
+    const R c0 = ch == 0 ? R(-178.4239) : R(-142.7664);
+    const R c1 = ch == 0 ? R(-68.42243) : R(42.70741);
+    const R c2 = ch == 0 ? R(43.20243) : R(-2.027365);
+    const R c3 = ch == 0 ? R(-4.633167) : R(-0.2582097);
+    const R c4 = ch == 0 ? R(69.70086) : R(21.36094);
+    const R c5 = ch == 0 ? R(40870.38) : R(27535.31);
+    const R c6 = ch == 0 ? R(-23705.70) : R(-21467.79);
+    const R c7 = ch == 0 ? R(128.8953) : R(60.34928);
+    const R c8 = ch == 0 ? R(-53.91334) : R(-27.43096);
+    const R c9 = ch == 0 ? R(5.315517) : R(2.676150);
+    const R c10 = ch == 0 ? R(-19.73427) : R(-11.28215);
+    const R c11 = ch == 0 ? R(16780.95) : R(14254.55);
+    const R c12 = ch == 0 ? R(-25786.11) : R(-23125.20);
+    const R c13 = ch == 0 ? R(14.82123) : R(9.305564);
+    const R c14 = ch == 0 ? R(-4.890915) : R(-2.464009);
+    const R c15 = ch == 0 ? R(0.4749030) : R(0.1985955);
+    const R c16 = ch == 0 ? R(-133.8283) : R(743.0600);
+    const R c17 = ch == 0 ? R(-1.164408) : R(-1.174242);
+    const R c18 = ch == 0 ? R(0.8227443) : R(0.7502286);
+    const R c19 = ch == 0 ? R(0.5864073) : R(0.2358848);
+    const R c20 = ch == 0 ? R(-2.056313) : R(2.937507);
+    const R kh1 = c0 + c1 * lT + c2 * lT2 + c3 * lT3
+                  + c4 * kx::log10(R(1) + c5 * iT);
+    const R kh2 = c6 * iT;
+    const R kl1 = c7 + c8 * lT + c9 * lT2 + c10 * kx::log10(R(1) + c11 * iT);
+    const R kl2 = c12 * iT;
+    const R nc1 = c13 + c14 * lT + c15 * lT2 + c16 * iT;
+    const R nc2 = c17 + nc1;
+    const R pp = c18 + c19 * kx::exp(-T / R(1.850e3))
+                 + c20 * kx::exp(-T / R(4.40e2));
+    const R n_c1 = kx::pow(R(10), nc1);
+    const R n_c2 = kx::pow(R(10), nc2);
+    const R lk = kh1 - (kh1 - kl1) / (R(1) + kx::pow(n_H / n_c1, pp)) + kh2
+                 - (kh2 - kl2) / (R(1) + kx::pow(n_H / n_c2, pp));
+    k += kx::pow(R(10), lk);
+  }
+  return k;
+}
+
+}  // namespace kx

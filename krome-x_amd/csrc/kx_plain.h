@@ -1,0 +1,141 @@
+// The plain FP64 drop-in kernel: one thread per cell, each running the
+// whole per-cell solve (KROME-X solve(), runtime/core/solve_cell.hpp:
+// 2094-2203, over the integrator of kx_cell.h), the counterpart of the
+// reference's Kokkos kernel "krome_bdf_cells" (runtime/core/
+// solve_kokkos.hpp:972-1039) with its interleaved workspace layout.
+#pragma once
+#include "kx_cell.h"
+#include "kx_problem.h"
+
+namespace kx {
+
+// where a call's arrays live and how the species array is laid out:
+// element (c, i) of n at n[c * n_cs + i * n_ss] (cell-major: n_cs = NSPEC,
+// n_ss = 1; species-major: n_cs = 1, n_ss = ld)
+struct CellsArgs {
+  int ncell;
+  double* n;
+  long n_cs, n_ss;
+  double* T;
+  const double* dt;
+  const double* par;
+  int par_stride;     // 0: one row for every cell
+  int* flags;         // ncell, always written
+  long* stats;        // ncell x 9 or nullptr
+  SolveOpts o;
+};
+
+// the integrator options of a cell (solve_cell.hpp:1662-1705: the
+// tolerances per slot, the DQ Jacobian with evolved Tgas, the charge
+// invariant with the DQ Jacobian and electrons, charge_newton_ 1631-1642)
+template <class Net, bool ET>
+KX_HD inline bool cell_opts(const SolveOpts& o,
+                            CellOpts<Problem<Net, ET>::N>& co) {
+  constexpr int NS = Net::NSPEC;
+  constexpr int N = Problem<Net, ET>::N;
+  co.rtol = o.rtol;
+  for (int i = 0; i < NS; ++i) co.atol[i] = o.atol_n;
+  if (ET) co.atol[N - 1] = o.atol_T;
+  co.max_steps = o.max_steps;
+  co.max_err_fails = o.max_err_fails;
+  co.max_conv_fails = o.max_conv_fails;
+  co.max_restarts = o.max_restarts;
+  co.dq_jac = o.jac == 2 || (o.jac == 0 && ET);
+  co.inv_row = -1;
+  co.inv_n = 0;
+  for (int i = 0; i < N; ++i) co.inv_c[i] = i < NS ? Net::charge(i) : 0;
+  if (co.dq_jac && Net::HAS_E) {
+    co.inv_row = Net::I_E;
+    co.inv_n = NS;
+  }
+  // the analytic Jacobian here is the mass-action block at fixed Tgas
+  return co.dq_jac || !(ET || Net::RATES_DENSITY);
+}
+
+// the post-solve of a successful cell: clip, conservation (solve_cell.hpp:
+// 1458-1490, modes 1 and 3), write back
+template <class Net, bool ET>
+KX_HD inline void cell_finish(const CellsArgs& a, int c, const double* y) {
+  constexpr int NS = Net::NSPEC;
+  double n0[NS], n[NS];
+  for (int i = 0; i < NS; ++i) {
+    n0[i] = a.n[c * a.n_cs + i * a.n_ss];
+    n[i] = y[i];
+    if (a.o.clip_negative && n[i] < 0.0) n[i] = 0.0;
+  }
+  if (Net::HAS_E && a.o.conserve == 1) {
+    double q0 = 0.0, q = 0.0;
+    for (int i = 0; i < NS; ++i) {
+      q0 += Net::charge(i) * n0[i];
+      q += Net::charge(i) * n[i];
+    }
+    n[Net::I_E] = ::fmax(n[Net::I_E] + (q - q0), 0.0);
+  } else if (Net::HAS_E && a.o.conserve == 3) {
+    double qi = 0.0;
+    for (int i = 0; i < NS; ++i)
+      if (i != Net::I_E) qi += Net::charge(i) * n[i];
+    n[Net::I_E] = ::fmax(qi, 0.0);
+  }
+  for (int i = 0; i < NS; ++i) a.n[c * a.n_cs + i * a.n_ss] = n[i];
+  if (ET) a.T[c] = y[NS];
+}
+
+template <class Net, bool ET>
+__global__ void __launch_bounds__(128)
+plain_cells_kernel(CellsArgs a, double* work, int* iwork, long S) {
+  using Prob = Problem<Net, ET>;
+  constexpr int NS = Net::NSPEC;
+  constexpr int N = Prob::N;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.ncell) return;
+  long* sc = a.stats ? a.stats + (long)c * 9 : nullptr;
+  if (!(a.dt[c] > 0.0)) {  // dt <= 0: untouched (solve_cells.hpp:96-101)
+    a.flags[c] = 0;
+    if (sc)
+      for (int i = 0; i < 9; ++i) sc[i] = 0;
+    return;
+  }
+  Prob P;
+  P.p = Net::params_from_row(a.par + (long)c * a.par_stride);
+  P.Tfix = a.T[c];
+  P.hit_T0 = false;
+  CellOpts<N> co;
+  if (!cell_opts<Net, ET>(a.o, co)) {
+    a.flags[c] = FLAG_UNSUPPORTED;
+    return;
+  }
+  double y[N];
+  for (int i = 0; i < NS; ++i) y[i] = a.n[c * a.n_cs + i * a.n_ss];
+  if (ET) y[N - 1] = a.T[c];
+  Cell<N, Prob> cv(P, work + c, iwork + c, S, co);
+  cv.reinit(0.0, y, 0.0);
+  CellStats tot;
+  tot.zero();
+  double t = 0.0;
+  int restarts = 0, flag = 0;
+  for (;;) {  // bdf::integrate's restart loop (bdf_cell.hpp:1513-1547)
+    const int f = cv.solve(a.dt[c], y, &t);
+    tot.add(cv.st);
+    if (f >= 0) { flag = 0; break; }
+    const bool again = f == ERR_FAILURE || f == CONV_FAILURE ||
+                       f == REPTD_RHSFUNC_ERR;
+    if (!again || restarts >= co.max_restarts) { flag = f; break; }
+    ++restarts;
+    const double hc = cv.current_step();
+    cv.reinit(t, y, hc > 0.0 ? 0.1 * hc : 0.0);
+  }
+  if (flag == 0 && ET && !(y[N - 1] > 0.0 && y[N - 1] - y[N - 1] == 0.0))
+    flag = FLAG_NONPOSITIVE_T;
+  if (P.hit_T0 && (flag == REPTD_RHSFUNC_ERR || flag == RHSFUNC_FAIL ||
+                   flag == UNREC_RHSFUNC_ERR || flag == FIRST_RHSFUNC_ERR))
+    flag = FLAG_NONPOSITIVE_T;
+  a.flags[c] = flag;
+  if (sc) {
+    sc[0] = tot.nst; sc[1] = tot.nfe; sc[2] = tot.nje; sc[3] = tot.nfeDQ;
+    sc[4] = tot.nsetups; sc[5] = tot.nni; sc[6] = tot.nnf; sc[7] = tot.netf;
+    sc[8] = restarts;
+  }
+  if (flag == 0) cell_finish<Net, ET>(a, c, y);
+}
+
+}  // namespace kx

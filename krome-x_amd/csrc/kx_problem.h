@@ -1,0 +1,71 @@
+// One cell's problem on the device: the right-hand side and the analytic
+// Jacobian of KROME-X's per-cell solver (runtime/core/solve_cell.hpp:
+// state_T_ / thermal_T_ 540-557, full_rhs_ 750-789, rhs_host_ 804-813,
+// jac_cell_ 1122-1132) over a generated network Net, with the state
+// y = [n_0 .. n_{NSPEC-1} | Tgas (when evolved)] in registers.
+#pragma once
+#include "kx_physics.h"
+
+namespace kx {
+
+// the solve's options as the kernels see them (SolverOpts, krome.hpp:
+// 187-255, the keys of this path)
+struct SolveOpts {
+  double rtol, atol_n, atol_T;
+  long max_steps;
+  int max_err_fails, max_conv_fails, max_restarts;
+  int evolve_T;      // Tgas evolved (a thermal network)
+  int jac;           // 0 auto (analytic at fixed T, else DQ), 1 analytic, 2 DQ
+  int clip_negative;
+  int conserve;      // 0 off, 1 electrons restore the charge, 3 neutralize
+};
+
+template <class Net, bool ET>
+struct Problem {
+  static constexpr int NS = Net::NSPEC;
+  static constexpr int N = NS + (ET ? 1 : 0);
+  typename Net::Params p;
+  double Tfix;
+  bool hit_T0;
+
+  KX_HD double rate_T(const double* y) const {
+    const double T = ET ? y[NS] : Tfix;
+    return ::fmin(::fmax(T, p.Tcmb), 1e9);
+  }
+  // the rhs at y; 1 (recoverable) for a non-positive Tgas or a non-finite
+  // result, the integrator then retries with a smaller step
+  template <class R>
+  KX_HD int rhs_r(const R* y, R* f) {
+    if (ET && !(y[N - 1] > R(0))) {
+      hit_T0 = true;
+      return 1;
+    }
+    const R T = ET ? y[N - 1] : R(Tfix);
+    const R Tr = kx::fmin(kx::fmax(T, R(p.Tcmb)), R(1e9));
+    R k[Net::NREA];
+    Net::rates(Tr, y, p, k);
+    Net::rhs(y, k, f);
+    if constexpr (ET) {
+      const R Tth = kx::fmin(kx::fmax(y[NS], R(2.73)), R(1e9));
+      f[NS] = Net::thermal(Tth, y, k, p);
+    }
+    bool fin = true;
+    for (int i = 0; i < N; ++i) fin = fin && kx_isfinite(f[i]);
+    return fin ? 0 : 1;
+  }
+  KX_HD int rhs(const double* y, double* f) { return rhs_r<double>(y, f); }
+  // the analytic Jacobian at fixed Tgas (mass action at the rates of the
+  // state; J zeroed by the caller)
+  KX_HD int jac(const double* y, double* J) {
+    double k[Net::NREA];
+    Net::rates(rate_T(y), y, p, k);
+    Net::jac_species(y, k, J, N);
+    return 0;
+  }
+  template <class R>
+  KX_HD static bool kx_isfinite(R x) {
+    return x == x && x - x == R(0);
+  }
+};
+
+}  // namespace kx

@@ -1,0 +1,338 @@
+"""Python interface of this project's network libraries: KROME-X's
+`Network.solve` (runtime/interfaces/python/krome.py:136-542) over the C
+ABI of libkrome_<network>.so (include/krome.h), whose batched solves run on
+the GPU (hand-written HIP kernels for gfx950).
+
+    import krome
+    net = krome.load("primordial3")
+    opts = net.options(evolve_T=True, rtol=1e-8)
+    res = net.solve(n, Tgas=100.0, dt=3.15e13, par={"tff": 1e12}, opts=opts)
+
+Array layouts follow the C interface in numpy's row-major order: n[ncell,
+nspec], par[ncell, npar]; a single cell may be 1-D arrays and scalars and
+comes back the same way.  solve() returns NEW arrays, the inputs are not
+modified, and raises KromeError when a cell fails (check=False: inspect
+result.flags).  solve_device() works on torch tensors resident on the GPU
+(species-major n[nspec, ncell], the coalesced layout).
+
+There is no CPU path: loading a library that was not built, or a machine
+without a HIP device, fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+__all__ = ["load", "Network", "Options", "Result", "KromeError"]
+
+HERE = Path(__file__).resolve().parent
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+_L = ctypes.POINTER(ctypes.c_long)
+_ENUMS = {"jac": ("auto", "analytic", "fd"),
+          "linsol": ("auto", "dense", "lapack", "sparse"),
+          "path": ("auto", "percell", "device"),
+          "device_kernel": ("plain", "persistent")}
+_KEYS = ("evolve_T", "jac", "linsol", "rtol", "atol_n", "atol_T", "atol_u",
+         "max_steps", "max_err_fails", "max_conv_fails", "max_restarts",
+         "clip_negative", "error_report", "renormalize_x", "conserve",
+         "path", "device_cache", "device_memory", "min_device",
+         "min_interleave", "device_kernel", "mixed_precision",
+         "queue_chunk", "waves_per_cu")
+LAYOUT_SPECIES_MAJOR = 0
+LAYOUT_CELL_MAJOR = 1
+
+
+class KromeError(RuntimeError):
+    pass
+
+
+def _dptr(a):
+    return None if a is None else a.ctypes.data_as(_D)
+
+
+def lib_path(network):
+    return HERE / ("libkrome_%s.so" % network)
+
+
+def load(network="primordial3", path=None):
+    """the Network of a built library (by network name, or its path)"""
+    p = Path(path) if path else lib_path(network)
+    if not p.exists():
+        raise KromeError("%s is not built (python3 krome-x_amd/build.py)" % p)
+    # one HIP runtime per process: when torch is importable its runtime
+    # (soname libamdhip64.so.7) is loaded first and serves this library
+    # too, so torch tensors and this library's kernels share the device
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    return Network(ctypes.CDLL(str(p)))
+
+
+class Network:
+    """one network's library: its species, parameters and solves"""
+
+    def __init__(self, lib):
+        self._lib = lib
+        L = lib
+        for name, res, args in (
+                ("krome_nspec", ctypes.c_int, []),
+                ("krome_nrea", ctypes.c_int, []),
+                ("krome_npar", ctypes.c_int, []),
+                ("krome_network_name", ctypes.c_char_p, []),
+                ("krome_version", ctypes.c_char_p, []),
+                ("krome_species_name", ctypes.c_char_p, [ctypes.c_int]),
+                ("krome_param_name", ctypes.c_char_p, [ctypes.c_int]),
+                ("krome_param_required_by", ctypes.c_char_p, [ctypes.c_int]),
+                ("krome_flag_message", ctypes.c_char_p, [ctypes.c_int]),
+                ("krome_flag_name", ctypes.c_char_p, [ctypes.c_int]),
+                ("krome_params_default", None, [_D]),
+                ("krome_opts_new", ctypes.c_void_p, []),
+                ("krome_opts_free", None, [ctypes.c_void_p]),
+                ("krome_opts_set", ctypes.c_int,
+                 [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_double]),
+                ("krome_opts_get", ctypes.c_int,
+                 [ctypes.c_void_p, ctypes.c_char_p, _D]),
+                ("krome_has_device", ctypes.c_int, []),
+                ("krome_device_initialize", ctypes.c_int, []),
+                ("krome_device_finalize", ctypes.c_int, []),
+                ("krome_solve_cells", ctypes.c_int,
+                 [ctypes.c_int, _D, _D, _D, _D, ctypes.c_int,
+                  ctypes.c_void_p, _I, ctypes.c_void_p]),
+                ("krome_solve_cells_x", ctypes.c_int,
+                 [ctypes.c_int, _D, _D, _D, _D, _D, ctypes.c_int,
+                  ctypes.c_void_p, _I, ctypes.c_void_p]),
+                ("krome_solve_cells_device", ctypes.c_int,
+                 [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_long,
+                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                  ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                  ctypes.c_void_p, ctypes.c_void_p]),
+                ("krome_solve_cells_device_stats", ctypes.c_int,
+                 [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_long,
+                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                  ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                  ctypes.c_void_p])):
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        self.nspec = L.krome_nspec()
+        self.nrea = L.krome_nrea()
+        self.npar = L.krome_npar()
+        self.name = L.krome_network_name().decode()
+        self.species = [L.krome_species_name(i).decode()
+                        for i in range(self.nspec)]
+        self.params = [L.krome_param_name(i).decode()
+                       for i in range(self.npar)]
+        self.required = {p: L.krome_param_required_by(i).decode()
+                         for i, p in enumerate(self.params)
+                         if L.krome_param_required_by(i)}
+        self._pidx = {p: i for i, p in enumerate(self.params)}
+        self._opts = None
+
+    def idx(self, name):
+        return self.species.index(name)
+
+    def params_default(self):
+        p = np.zeros(self.npar)
+        self._lib.krome_params_default(_dptr(p))
+        return p
+
+    def param_rows(self, ncell, **values):
+        rows = np.tile(self.params_default(), (ncell, 1))
+        for k, v in values.items():
+            rows[:, self._pidx[k]] = np.broadcast_to(np.asarray(v, float),
+                                                     (ncell,))
+        return rows
+
+    def options(self, **values):
+        return Options(self, **values)
+
+    def flag_message(self, flag):
+        return self._lib.krome_flag_message(int(flag)).decode()
+
+    def flag_name(self, flag):
+        return self._lib.krome_flag_name(int(flag)).decode()
+
+    # -- solves ----------------------------------------------------------
+
+    def solve(self, n, Tgas, dt, par=None, Phi=None, opts=None, check=True):
+        """integrate every cell over its own dt [s] (krome_solve_cells)"""
+        if Phi is not None:
+            raise KromeError("this network has no photon bins")
+        single, n, ncell = self._cells(n)
+        T = self._per_cell(Tgas, ncell)
+        dtv = self._per_cell(dt, ncell)
+        opts = self._options(opts)
+        P, stride = self._params(par, ncell)
+        flags = np.zeros(ncell, dtype=np.intc)
+        ierr = self._lib.krome_solve_cells(
+            ncell, _dptr(n), _dptr(T), _dptr(dtv), _dptr(P), stride, None,
+            flags.ctypes.data_as(_I), opts._h)
+        res = Result(n[0] if single else n, T[0] if single else T,
+                     flags[0] if single else flags, ierr)
+        self._check(check, ierr, flags, "krome_solve_cells")
+        return res
+
+    def solve_x(self, x, rho, Tgas, dt, par=None, opts=None, check=True):
+        """solve() with mass fractions x and the gas density rho [g cm^-3]
+        (krome_solve_cells_x)"""
+        single, x, ncell = self._cells(x)
+        rh = self._per_cell(rho, ncell)
+        T = self._per_cell(Tgas, ncell)
+        dtv = self._per_cell(dt, ncell)
+        opts = self._options(opts)
+        P, stride = self._params(par, ncell)
+        flags = np.zeros(ncell, dtype=np.intc)
+        ierr = self._lib.krome_solve_cells_x(
+            ncell, _dptr(x), _dptr(rh), _dptr(T), _dptr(dtv), _dptr(P),
+            stride, None, flags.ctypes.data_as(_I), opts._h)
+        res = Result(x[0] if single else x, T[0] if single else T,
+                     flags[0] if single else flags, ierr)
+        self._check(check, ierr, flags, "krome_solve_cells_x")
+        return res
+
+    def solve_device(self, n, Tgas, dt, par, opts=None, flags=None,
+                     stats=None, sync=True):
+        """device-resident solve on torch tensors: n[nspec, ncell]
+        (species-major, float64), Tgas, dt [ncell], par[ncell, npar], all
+        on the GPU; updated in place.  flags (int32 [ncell]) and stats
+        (int64 [ncell, 9]) optional device tensors.  Returns the C
+        function's flag."""
+        ncell = int(Tgas.shape[0])
+        for t in (n, Tgas, dt, par):
+            if not t.is_cuda or not t.is_contiguous():
+                raise KromeError("solve_device needs contiguous GPU tensors")
+        if tuple(n.shape) != (self.nspec, ncell):
+            raise KromeError("n must be [nspec, ncell]")
+        if tuple(par.shape) != (ncell, self.npar):
+            raise KromeError("par must be [ncell, npar]")
+        opts = self._options(opts)
+        if flags is None or stats is not None:
+            import torch
+            if flags is None:
+                flags = torch.zeros(ncell, dtype=torch.int32,
+                                    device=Tgas.device)
+        r = self._lib.krome_solve_cells_device_stats(
+            ncell, n.data_ptr(), LAYOUT_SPECIES_MAJOR, ncell, Tgas.data_ptr(),
+            dt.data_ptr(), par.data_ptr(), self.npar, flags.data_ptr(),
+            stats.data_ptr() if stats is not None else None, opts._h)
+        if sync:
+            import torch
+            torch.cuda.synchronize()
+        return r
+
+    # -- inputs (the krome_solve_cells conventions) -----------------------
+
+    def _cells(self, n):
+        n = np.array(n, dtype=float, order="C", copy=True)
+        if n.ndim == 0 or n.ndim > 2 or n.shape[-1] != self.nspec:
+            raise KromeError("n must have shape (%d,) or (ncell, %d), got %s"
+                             % (self.nspec, self.nspec, n.shape))
+        single = n.ndim == 1
+        n = n.reshape(-1, self.nspec)
+        return single, n, n.shape[0]
+
+    @staticmethod
+    def _per_cell(v, ncell):
+        return np.array(np.broadcast_to(np.asarray(v, dtype=float),
+                                        (ncell,)), order="C", copy=True)
+
+    def _options(self, opts):
+        if opts is not None:
+            return opts
+        if self._opts is None:
+            self._opts = Options(self)
+        return self._opts
+
+    def _params(self, par, ncell):
+        stride, P = 0, None
+        if isinstance(par, dict):
+            P = self.param_rows(ncell, **par)
+            stride = self.npar
+        elif par is not None:
+            P = np.ascontiguousarray(par, dtype=float)
+            if P.shape == (self.npar,):
+                stride = 0
+            elif P.shape == (ncell, self.npar):
+                stride = self.npar
+            else:
+                raise KromeError("par must have shape (%d,) or (%d, %d), "
+                                 "got %s" % (self.npar, ncell, self.npar,
+                                             P.shape))
+        rows = (self.params_default()[None, :] if P is None
+                else P.reshape(-1, self.npar))
+        unset = [s for s in self.required
+                 if np.isnan(rows[:, self._pidx[s]]).any()]
+        if unset:
+            raise KromeError("required parameter(s) not set: %s" % "; ".join(
+                "%s (for %s)" % (s, self.required[s]) for s in unset))
+        return P, stride
+
+    def _check(self, check, ierr, flags, what):
+        if check and ierr != 0:
+            bad = np.flatnonzero(np.atleast_1d(flags))
+            raise KromeError("%s returned %d: %s (failed cells: %s); pass "
+                             "check=False to get the result anyway" % (
+                                 what, ierr, self.flag_message(ierr),
+                                 bad.tolist() or "n/a"))
+
+
+class Options:
+    """solver options (a krome_opts handle).  Keys: those of KROME-X's
+    krome_opts_set (evolve_T, jac, linsol, rtol, atol_n, atol_T,
+    max_steps, max_err_fails, max_conv_fails, max_restarts,
+    clip_negative, error_report, renormalize_x, conserve, path,
+    device_cache, device_memory, min_device, min_interleave) and this
+    build's device_kernel ("plain" / "persistent"), mixed_precision,
+    queue_chunk, waves_per_cu."""
+
+    def __init__(self, net, **values):
+        self._net = net
+        self._h = net._lib.krome_opts_new()
+        for k, v in values.items():
+            self[k] = v
+
+    def __setitem__(self, key, value):
+        if key not in _KEYS:
+            raise KromeError("unknown option %r" % key)
+        if key in _ENUMS and isinstance(value, str):
+            value = _ENUMS[key].index(value)
+        if self._net._lib.krome_opts_set(self._h, key.encode(),
+                                         float(value)) != 0:
+            raise KromeError("invalid value %r for option %s" % (value, key))
+
+    def __getitem__(self, key):
+        v = ctypes.c_double()
+        if self._net._lib.krome_opts_get(self._h, key.encode(),
+                                         ctypes.byref(v)) != 0:
+            raise KromeError("unknown option %r" % key)
+        return v.value
+
+    def close(self):
+        if self._h is not None:
+            self._net._lib.krome_opts_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Result:
+    """solve() output: n, Tgas, per-cell flags (0 = success), ierr"""
+
+    def __init__(self, n, Tgas, flags, ierr):
+        self.n, self.Tgas, self.flags, self.ierr = n, Tgas, flags, ierr
+
+    @property
+    def ok(self):
+        return self.ierr == 0
+
+    def __repr__(self):
+        return "<krome.Result ierr=%d, %d cell(s)>" % (
+            self.ierr, np.atleast_1d(self.flags).size)

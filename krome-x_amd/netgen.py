@@ -1,0 +1,914 @@
+"""Network kernel generator: a `.knet` network -> one C++/HIP header.
+
+The header holds, for one network and one thermal configuration, the
+functions the per-cell integrator evaluates, each a template on the real
+type R (double for the FP64 path, float for the FP32 Jacobian of the
+mixed-precision path) and callable from host code (the CPU oracle, g++) and
+from gfx950 kernels (hipcc):
+
+  rates_T(Tgas, p, k)         rates of the rate temperature and parameters
+                              (KROME-X `rates_fixed`, codegen.py:4760-4793)
+  rates_n(Tgas, n, p, k)      density-dependent rates (`rates_var`)
+  rhs(n, k, dn)               mass-action dn/dt in reaction order
+                              (codegen.py:4289-4327)
+  jac_species(n, k, J)        analytic dn/dt Jacobian at fixed rates
+                              (mass action; codegen.py:5180-5225 without
+                              the dk/dn aggregate term, which the networks
+                              generated here with an analytic Jacobian do
+                              not have: every rate is a function of T)
+  thermal(Tth, n, k, p)       dTgas/dt = (gamma-1)(heat-cool)/(kB ntot)
+                              (solve_cell.hpp:305-429, the configured
+                              terms only)
+
+The semantics follow the KROME-X network format (docs/network-format.md):
+species in code-point order (section 2.1), half-open T pieces tiled into
+piecewise rates with the `asym` outside rule (zero below, clamped above;
+section 3.1), the rate language of section 4 (`var`, `extern`, built-in
+functions, `n(X)`, `Tgas_eV`, `ntot`, `Hnuclei`).  The parser and the
+emitter are this project's own; only the format, the element masses
+(kromex/knet.py:64-77) and the H2 chemistry energy ledger
+(codegen.py:4953-4961) are data that must agree with the reference.
+"""
+from __future__ import annotations
+
+import ast
+import math
+import re
+from dataclasses import dataclass, field
+from pathlib import Path
+
+# element masses [amu] and the electron mass: the reference's table
+# (kromex/knet.py:64-77), so species masses (rho, tff, x <-> n) agree
+MASS_AMU = {
+    "H": 1.008, "D": 2.014, "He": 4.0026, "Li": 6.94, "Be": 9.012,
+    "B": 10.81, "C": 12.011, "N": 14.007, "O": 15.999, "F": 18.998,
+    "Ne": 20.180, "Na": 22.990, "Mg": 24.305, "Al": 26.982, "Si": 28.085,
+    "P": 30.974, "S": 32.06, "Cl": 35.45, "Ar": 39.948, "K": 39.098,
+    "Ca": 40.078, "Sc": 44.956, "Ti": 47.867, "V": 50.942, "Cr": 51.996,
+    "Mn": 54.938, "Fe": 55.845, "Co": 58.933, "Ni": 58.693, "Cu": 63.546,
+    "Zn": 65.38,
+}
+ELECTRON_MASS_AMU = 5.48579909e-4
+AMU_G = 1.6605390666e-24          # CODATA 2018
+BOLTZMANN_EV = 8.617333262145179e-05
+BOLTZMANN_ERG = 1.380649e-16
+EV_TO_ERG = 1.602176634e-12
+CONSTANTS = {"boltzmann_eV": BOLTZMANN_EV, "boltzmann_erg": BOLTZMANN_ERG,
+             "eV_to_erg": EV_TO_ERG, "pi": math.pi,
+             "N_avogadro": 6.02214076e+23, "planck_eV": 4.135667696923859e-15,
+             "Rgas_kJ": 0.00831446261815324}
+# default values of the predefined parameters (kromex/knet.py:53-58)
+PARAM_DEFAULTS = {"gamma_ad": 5.0 / 3.0, "Tcmb": 2.725, "tff": math.nan,
+                  "opratio": 3.0, "G0": 0.0, "d2g": 0.01, "Av": 0.0,
+                  "zredshift": 0.0}
+_ELEMENTS_BY_LEN = sorted(MASS_AMU, key=len, reverse=True)
+_SPECIES_RE = re.compile(r"^(?P<curly>\{)?(?P<core>[A-Za-z][A-Za-z0-9]*)"
+                         r"(?P<labels>(?:_[A-Za-z0-9]+)*)(?P<charge>\++|-+)?"
+                         r"(?P<curly2>\})?$")
+# the extern rate functions this build provides (kx_physics.h) and
+# whether they read the densities
+EXTERNS = {"dissH2_Martin96": True}
+
+
+class KnetError(Exception):
+    pass
+
+
+@dataclass
+class Species:
+    name: str
+    comp: dict
+    charge: int
+    mass_amu: float
+
+    @property
+    def cid(self):
+        """C identifier: + -> j, - -> k (KROME's IDX_ names)"""
+        if self.name == "e-":
+            return "E"
+        return self.name.replace("+", "j").replace("-", "k")
+
+
+def parse_species(tok):
+    """a species token -> (Species, kinetic)"""
+    tok = tok.strip()
+    if tok == "e-":
+        return Species("e-", {}, -1, ELECTRON_MASS_AMU), True
+    m = _SPECIES_RE.match(tok)
+    if not m or bool(m.group("curly")) != bool(m.group("curly2")):
+        raise KnetError("malformed species %r" % tok)
+    core, comp, i = m.group("core"), {}, 0
+    while i < len(core):
+        for el in _ELEMENTS_BY_LEN:
+            if core.startswith(el, i):
+                i += len(el)
+                digits = ""
+                while i < len(core) and core[i].isdigit():
+                    digits += core[i]
+                    i += 1
+                comp[el] = comp.get(el, 0) + int(digits or 1)
+                break
+        else:
+            raise KnetError("species %r is not an element composition" % tok)
+    ch = m.group("charge") or ""
+    charge = len(ch) * (1 if ch.startswith("+") else -1) if ch else 0
+    mass = sum(MASS_AMU[e] * c for e, c in comp.items())
+    name = core + (m.group("labels") or "") + ch
+    return (Species(name, comp, charge, mass - charge * ELECTRON_MASS_AMU),
+            not m.group("curly"))
+
+
+@dataclass
+class Reaction:
+    reactants: list          # [(name, kinetic)]
+    products: list           # [name]
+    tlo: float
+    thi: float
+    ranged: bool
+    rate: str
+    line: int
+    index: int = -1          # k[] slot, in file order
+    group: int = -1          # piecewise group id
+
+    @property
+    def eq(self):
+        lhs = " + ".join(("{%s}" % s) if not k else s
+                         for s, k in self.reactants)
+        return "%s -> %s" % (lhs, " + ".join(self.products))
+
+
+@dataclass
+class Network:
+    name: str
+    species: list = field(default_factory=list)   # Species, code-point order
+    reactions: list = field(default_factory=list)
+    vars: dict = field(default_factory=dict)      # name -> expr text
+    params: dict = field(default_factory=dict)    # network params
+    externs: list = field(default_factory=list)
+
+    def idx(self, name):
+        for i, s in enumerate(self.species):
+            if s.name == name:
+                return i
+        raise KnetError("unknown species %r" % name)
+
+    @property
+    def nspec(self):
+        return len(self.species)
+
+    @property
+    def nrea(self):
+        return len(self.reactions)
+
+
+def _number(tok):
+    t = tok.strip().lower()
+    if t in ("inf", "+inf", "infinity"):
+        return math.inf
+    return float(t)
+
+
+def parse_knet(path):
+    """parse a `.knet` file into a Network (docs/network-format.md 1-4)"""
+    path = Path(path)
+    text = path.read_text()
+    # continuation lines, comments
+    lines, buf, start = [], "", None
+    for i, raw in enumerate(text.splitlines(), 1):
+        s = raw.split("#", 1)[0].rstrip()
+        if start is None:
+            start = i
+        if s.endswith("\\"):
+            buf += s[:-1] + " "
+            continue
+        buf += s
+        if buf.strip():
+            lines.append((start, buf.strip()))
+        buf, start = "", None
+    if not lines or not lines[0][1].startswith("krome_format"):
+        raise KnetError("%s: no krome_format header" % path)
+    net = Network(name=path.stem)
+    spec = {}
+    rx = []
+    for ln, s in lines[1:]:
+        head = s.split(None, 1)[0]
+        if head == "var":
+            name, expr = s[3:].split("=", 1)
+            net.vars[name.strip()] = expr.strip()   # later definitions win
+            continue
+        if head == "param":
+            name, val = s[5:].split("=", 1)
+            net.params[name.strip()] = float(val)
+            continue
+        if head == "extern":
+            for e in s[6:].replace(",", " ").split():
+                if e not in EXTERNS:
+                    raise KnetError("line %d: extern %s is not provided"
+                                    % (ln, e))
+                net.externs.append(e)
+            continue
+        if head == "species":
+            for t in s[7:].replace(",", " ").split():
+                sp, _ = parse_species(t)
+                spec[sp.name] = sp
+            continue
+        if head in ("ref",):
+            continue
+        if head in ("table", "cooling", "heating", "rate_scale", "freeze"):
+            raise KnetError("line %d: directive %r is not supported by this "
+                            "generator" % (ln, head))
+        if "->" not in s:
+            raise KnetError("line %d: not a reaction: %r" % (ln, s))
+        lhs, rest = s.split("->", 1)
+        b = rest.index("[")
+        e = rest.index("]", b)
+        prods = [t for t in rest[:b].split() if t != "+"]
+        rng = rest[b + 1:e].strip()
+        rate = rest[e + 1:].split(";", 1)[0].strip()
+        if rng:
+            lo, hi = (x.strip() for x in rng.split(","))
+            tlo = _number(lo) if lo else 0.0
+            thi = _number(hi) if hi else math.inf
+            ranged = True
+        else:
+            tlo, thi, ranged = 0.0, math.inf, False
+        reac = []
+        for t in lhs.split():
+            if t == "+":
+                continue
+            sp, kin = parse_species(t)
+            spec.setdefault(sp.name, sp)
+            reac.append((sp.name, kin))
+        pl = []
+        for t in prods:
+            sp, _ = parse_species(t)
+            spec.setdefault(sp.name, sp)
+            pl.append(sp.name)
+        rx.append(Reaction(reac, pl, tlo, thi, ranged, rate, ln))
+    # code-point order (network-format.md 2.1)
+    net.species = [spec[k] for k in sorted(spec)]
+    for i, r in enumerate(rx):
+        r.index = i
+    net.reactions = rx
+    _group(net)
+    _check_balance(net)
+    return net
+
+
+def _check_balance(net):
+    for r in net.reactions:
+        sp = {s.name: s for s in net.species}
+        q0 = sum(sp[s].charge for s, _ in r.reactants)
+        q1 = sum(sp[s].charge for s in r.products)
+        if q0 != q1:
+            raise KnetError("line %d: charge not conserved in %s"
+                            % (r.line, r.eq))
+
+
+def _group(net):
+    """tile the ranged lines of identical stoichiometry into piecewise
+    groups (network-format.md 3.1); an unranged line is its own group"""
+    gid = 0
+    by_key = {}
+    for r in net.reactions:
+        if not r.ranged:
+            r.group = gid
+            gid += 1
+            continue
+        key = (tuple(sorted(s for s, _ in r.reactants)),
+               tuple(sorted(r.products)))
+        by_key.setdefault(key, []).append(r)
+    for key, rs in by_key.items():
+        rs = sorted(rs, key=lambda r: r.tlo)
+        chains = []
+        for r in rs:
+            for ch in chains:
+                if ch[-1].thi == r.tlo:
+                    ch.append(r)
+                    break
+            else:
+                chains.append([r])
+        for ch in chains:
+            for r in ch:
+                r.group = gid
+            gid += 1
+
+
+def groups(net):
+    """the piecewise groups in the order of their first reaction"""
+    out = {}
+    for r in net.reactions:
+        out.setdefault(r.group, []).append(r)
+    return sorted((sorted(g, key=lambda r: r.tlo) for g in out.values()),
+                  key=lambda g: min(r.index for r in g))
+
+
+# ---------------------------------------------------------------------------
+# expressions
+
+_N_CALL = re.compile(r"\bn\(\s*([^()\s]+)\s*\)")
+
+
+class Expr:
+    """one rate / var expression, parsed with Python's grammar (the rate
+    language is a subset of it once `n(X)` species literals are named)"""
+
+    def __init__(self, text, net):
+        self.text = text
+        self.net = net
+        nsub = {}
+
+        def repl(m):
+            i = net.idx(m.group(1))
+            nsub["__n%d" % i] = i
+            return "__n%d" % i
+        t = _N_CALL.sub(repl, text)
+        try:
+            self.tree = ast.parse(t, mode="eval").body
+        except SyntaxError as exc:
+            raise KnetError("bad expression %r: %s" % (text, exc))
+        self.names = {n.id for n in ast.walk(self.tree)
+                      if isinstance(n, ast.Name)}
+        self.calls = {n.func.id for n in ast.walk(self.tree)
+                      if isinstance(n, ast.Call)}
+
+
+class Emitter:
+    """C++ for expressions of one network: every symbol resolves to a var
+    (evaluated in the enclosing block), a state symbol, a parameter, a
+    constant or a species density"""
+
+    def __init__(self, net, param_names):
+        self.net = net
+        self.params = param_names
+        self.exprs = {v: Expr(t, net) for v, t in net.vars.items()}
+
+    def deps(self, expr):
+        """the vars an expression needs, transitively, in dependency order"""
+        out, seen = [], set()
+
+        def visit(e, stack):
+            for nm in sorted(e.names):
+                if nm in self.exprs and nm not in seen:
+                    if nm in stack:
+                        raise KnetError("cyclic var %s" % nm)
+                    visit(self.exprs[nm], stack | {nm})
+                    seen.add(nm)
+                    out.append(nm)
+        visit(expr, set())
+        return out
+
+    def density_dependent(self, expr):
+        """whether an expression (with its vars) reads the densities"""
+        es = [expr] + [self.exprs[v] for v in self.deps(expr)]
+        for e in es:
+            if e.names & {"ntot", "Hnuclei"}:
+                return True
+            if any(nm.startswith("__n") for nm in e.names):
+                return True
+            if any(EXTERNS.get(c) for c in e.calls):
+                return True
+        return False
+
+    def c(self, node):
+        """C++ of an AST node (type R)"""
+        if isinstance(node, ast.Constant):
+            return "R(%r)" % float(node.value)
+        if isinstance(node, ast.Name):
+            nm = node.id
+            if nm in self.exprs:
+                return "v_" + nm
+            if nm == "Tgas":
+                return "Tgas"
+            if nm == "Tgas_eV":
+                return "(Tgas * R(%r))" % BOLTZMANN_EV
+            if nm == "ntot":
+                return "ntot(n)"
+            if nm == "Hnuclei":
+                return "Hnuclei(n)"
+            if nm.startswith("__n"):
+                return "n[%d]" % int(nm[3:])
+            if nm in self.params:
+                return "R(p.%s)" % nm
+            if nm in CONSTANTS:
+                return "R(%r)" % CONSTANTS[nm]
+            raise KnetError("unknown symbol %r" % nm)
+        if isinstance(node, ast.UnaryOp):
+            if isinstance(node.op, ast.USub):
+                return "(-%s)" % self.c(node.operand)
+            if isinstance(node.op, ast.UAdd):
+                return self.c(node.operand)
+        if isinstance(node, ast.BinOp):
+            a = self.c(node.left)
+            if isinstance(node.op, ast.Pow):
+                ex = node.right
+                val = None
+                if isinstance(ex, ast.Constant):
+                    val = float(ex.value)
+                elif (isinstance(ex, ast.UnaryOp)
+                      and isinstance(ex.op, ast.USub)
+                      and isinstance(ex.operand, ast.Constant)):
+                    val = -float(ex.operand.value)
+                if val is not None and val == int(val) and abs(val) <= 16:
+                    return "kx::ipow<%d>(%s)" % (int(val), a)
+                if val is not None and val == 0.5:
+                    return "kx::sqrt(%s)" % a
+                return "kx::pow(%s, %s)" % (a, self.c(ex))
+            b = self.c(node.right)
+            op = {ast.Add: "+", ast.Sub: "-", ast.Mult: "*",
+                  ast.Div: "/"}[type(node.op)]
+            return "(%s %s %s)" % (a, op, b)
+        if isinstance(node, ast.Call):
+            f = node.func.id
+            args = node.args
+            if f in ("exp", "log", "log10", "sqrt", "tanh"):
+                return "kx::%s(%s)" % (f, self.c(args[0]))
+            if f == "abs":
+                return "kx::fabs(%s)" % self.c(args[0])
+            if f in ("min", "max"):
+                return "kx::f%s(%s, %s)" % (f, self.c(args[0]),
+                                           self.c(args[1]))
+            if f == "where":
+                cnd = args[0]
+                if not isinstance(cnd, ast.Compare):
+                    raise KnetError("where() needs a comparison")
+                op = {ast.Lt: "<", ast.LtE: "<=", ast.Gt: ">",
+                      ast.GtE: ">="}[type(cnd.ops[0])]
+                return "((%s %s %s) ? %s : %s)" % (
+                    self.c(cnd.left), op, self.c(cnd.comparators[0]),
+                    self.c(args[1]), self.c(args[2]))
+            if f in EXTERNS:
+                return "kx::%s(Tgas, Hnuclei(n))" % f
+            raise KnetError("unknown function %r" % f)
+        raise KnetError("unsupported syntax %s" % ast.dump(node))
+
+    def block(self, exprs, indent):
+        """the var definitions the expressions need, as C++ lines"""
+        need = []
+        for e in exprs:
+            for v in self.deps(e):
+                if v not in need:
+                    need.append(v)
+        # dependency order over the union
+        order = []
+        for v in need:
+            for d in self.deps(self.exprs[v]) + [v]:
+                if d not in order:
+                    order.append(d)
+        return ["%sconst R v_%s = %s;" % (indent, v, self.c(self.exprs[v].tree))
+                for v in order]
+
+
+# ---------------------------------------------------------------------------
+# thermal configuration
+
+# H2 chemistry energy ledger [eV]: (reactants, products, energy, thermalized)
+# (codegen.py:4953-4961)
+CHEM_LEDGER = [
+    (["H", "H", "H"], ["H", "H2"], 4.48, True),
+    (["H", "H", "H2"], ["H2", "H2"], 4.48, True),
+    (["H", "H-"], ["H2", "e-"], 3.53, True),
+    (["H", "H2+"], ["H+", "H2"], 1.83, True),
+    (["H", "H2"], ["H", "H", "H"], -4.48, False),
+    (["H2", "e-"], ["H", "H", "e-"], -4.48, False),
+    (["H2", "H2"], ["H", "H", "H2"], -4.48, False),
+]
+# diatomic constants [K] for thermal.gamma = EXACT: vibrational and
+# rotational wavenumbers [cm^-1] times hc/k (codegen.py:318-325)
+HC_K = 1.4387768775039336
+GAMMA_VIB_K = {s: v * HC_K for s, v in {
+    "H2": 4401.213, "HD": 3813.15, "D2": 3115.5, "CO": 2169.75589,
+    "OH": 3737.761, "N2": 2358.57, "O2": 1580.161}.items()}
+GAMMA_ROT_K = {s: v * HC_K for s, v in {
+    "H2": 60.853, "H2+": 42.9, "HD": 45.644, "D2": 30.443, "N2": 1.9982,
+    "O2": 1.4264, "CO": 2.78}.items()}
+
+COOLING = ("H2", "CONT", "CHEM")
+HEATING = ("COMPRESS", "CHEM")
+
+
+@dataclass
+class Thermal:
+    cooling: tuple = ()
+    heating: tuple = ()
+    gamma: str = "CONST"
+    h2_opacity: str = ""
+
+    def __post_init__(self):
+        for c in self.cooling:
+            if c not in COOLING:
+                raise KnetError("cooling %s is not supported" % c)
+        for h in self.heating:
+            if h not in HEATING:
+                raise KnetError("heating %s is not supported" % h)
+        if self.gamma not in ("CONST", "EXACT"):
+            raise KnetError("gamma %s is not supported" % self.gamma)
+        if self.h2_opacity not in ("", "RIPAMONTI"):
+            raise KnetError("h2_opacity %s is not supported" % self.h2_opacity)
+
+
+def param_list(net, th):
+    """the parameter rows' order: gamma_ad (CONST gamma), Tcmb, the
+    configured processes' (tff for COMPRESS), opratio (EXACT gamma with
+    H2), then the network's own params"""
+    out = []
+    if th.gamma == "CONST":
+        out.append("gamma_ad")
+    out.append("Tcmb")
+    if "COMPRESS" in th.heating:
+        out.append("tff")
+    names = {s.name for s in net.species}
+    if th.gamma == "EXACT" and "H2" in names:
+        out.append("opratio")
+    for p in net.params:
+        if p not in out:
+            out.append(p)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# the header
+
+def _num(x):
+    if math.isinf(x):
+        return "INFINITY"
+    if math.isnan(x):
+        return "NAN"
+    return repr(float(x))
+
+
+def generate(knet_path, thermal=None, name=None):
+    """C++ header text of a network (+ its thermal configuration) and a
+    metadata dict (species, masses, params) for the host side"""
+    net = parse_knet(knet_path)
+    th = thermal or Thermal()
+    pnames = param_list(net, th)
+    em = Emitter(net, pnames)
+    ns = name or net.name
+    NS, NR = net.nspec, net.nrea
+    L = []
+    w = L.append
+    w("// generated by netgen.py from %s -- do not edit" % Path(knet_path).name)
+    w("#pragma once")
+    w('#include "kx_physics.h"')
+    w("")
+    w("namespace kx {")
+    w("struct net_%s {" % ns)
+    w('  static constexpr const char* NAME = "%s";' % ns)
+    w("  static constexpr int NSPEC = %d;" % NS)
+    w("  static constexpr int NREA = %d;" % NR)
+    w("  static constexpr int NPAR = %d;" % len(pnames))
+    for i, s in enumerate(net.species):
+        w("  static constexpr int IDX_%s = %d;" % (s.cid, i))
+    has_e = any(s.name == "e-" for s in net.species)
+    w("  static constexpr bool HAS_E = %s;" % ("true" if has_e else "false"))
+    w("  static constexpr int I_E = %d;" % (net.idx("e-") if has_e else -1))
+    thermal_on = bool(th.cooling or th.heating)
+    w("  static constexpr bool THERMAL = %s;"
+      % ("true" if thermal_on else "false"))
+    # n-dependent rates -> no analytic Jacobian
+    grp = groups(net)
+    dens_groups = [g for g in grp
+                   if any(em.density_dependent(Expr(r.rate, net)) for r in g)]
+    w("  static constexpr bool RATES_DENSITY = %s;"
+      % ("true" if dens_groups else "false"))
+    w("  // parameter row (param_names order)")
+    w("  struct Params {")
+    for p in pnames:
+        w("    double %s;" % p)
+    w("  };")
+    w("  KX_HD static Params params_from_row(const double* row) {")
+    w("    Params p;")
+    for i, p in enumerate(pnames):
+        w("    p.%s = row[%d];" % (p, i))
+    w("    return p;")
+    w("  }")
+    req = [i for i, p in enumerate(pnames)
+           if math.isnan(net.params.get(p, PARAM_DEFAULTS.get(p, 0.0)))]
+    w("  // index of the first REQUIRED parameter still NaN, or -1")
+    w("  KX_HD static int params_missing(const double* row) {")
+    for i in req:
+        w("    if (row[%d] != row[%d]) return %d;" % (i, i, i))
+    w("    (void)row;")
+    w("    return -1;")
+    w("  }")
+    # species data
+    charges = [s.charge for s in net.species]
+    masses = [s.mass_amu * AMU_G for s in net.species]
+    natH = [float(s.comp.get("H", 0)) for s in net.species]
+    w("  KX_HD static int charge(int i) {")
+    w("    switch (i) {")
+    for i, q in enumerate(charges):
+        if q:
+            w("      case %d: return %d;" % (i, q))
+    w("      default: return 0;")
+    w("    }")
+    w("  }")
+    w("  KX_HD static double mass_g(int i) {")
+    w("    switch (i) {")
+    for i, m in enumerate(masses):
+        w("      case %d: return %r;" % (i, m))
+    w("      default: return 0.0;")
+    w("    }")
+    w("  }")
+    w("  template <class R> KX_HD static R ntot(const R* n) {")
+    w("    R s = R(0);")
+    for i in range(NS):
+        w("    s += n[%d];" % i)
+    w("    return s;")
+    w("  }")
+    w("  template <class R> KX_HD static R Hnuclei(const R* n) {")
+    w("    R s = R(0);")
+    for i in range(NS):
+        if natH[i]:
+            w("    s += n[%d] * R(%r);" % (i, natH[i]))
+        else:
+            w("    s += n[%d] * R(0.0);" % i)
+    w("    return s;")
+    w("  }")
+    w("  template <class R> KX_HD static R rho(const R* n) {")
+    w("    R s = R(0);")
+    for i in range(NS):
+        w("    s += n[%d] * R(%r);" % (i, masses[i]))
+    w("    return s;")
+    w("  }")
+
+    # rates
+    def emit_group(g, indent):
+        out = []
+        ex = [Expr(r.rate, net) for r in g]
+        if not g[0].ranged:
+            r = g[0]
+            blk = em.block(ex, indent + "  ")
+            line = "k[%d] = %s;  // %s" % (r.index, em.c(ex[0].tree), r.eq)
+            if not blk:
+                return [indent + line]
+            return ([indent + "{"] + blk + [indent + "  " + line]
+                    + [indent + "}"])
+        lo, hi = g[0].tlo, g[-1].thi
+        out.append("%s{  // %s: %d piece(s), zero below %s, clamped above %s"
+                   % (indent, g[0].eq, len(g), _num(lo), _num(hi)))
+        ind = indent + "  "
+        out.append("%sR Tgas = Trate;" % ind)
+        if not math.isinf(hi):
+            out.append("%sif (Tgas > R(%r)) Tgas = R(%r);" % (ind, hi, hi))
+        for ln in em.block(ex, ind):
+            out.append(ln)
+        for j, (r, e) in enumerate(zip(g, ex)):
+            conds = []
+            if r.tlo > 0.0:
+                conds.append("Tgas >= R(%r)" % r.tlo)
+            if j < len(g) - 1:
+                conds.append("Tgas < R(%r)" % r.thi)
+            cnd = " && ".join(conds) or "true"
+            out.append("%sif (%s) k[%d] = %s;  // %s" % (
+                ind, cnd, r.index, em.c(e.tree), r.eq))
+        out.append("%s}" % indent)
+        return out
+
+    w("  // rates of the rate temperature Trate (and parameters); the")
+    w("  // caller zeroes k")
+    w("  template <class R>")
+    w("  KX_HD static void rates_T(R Trate, const Params& p, R* k) {")
+    w("    (void)p;")
+    w("    const R Tgas = Trate;")
+    w("    (void)Tgas;")
+    tgroups = [g for g in grp if g not in dens_groups]
+    for g in tgroups:
+        L.extend(emit_group(g, "    "))
+    w("  }")
+    w("  // density-dependent rates")
+    w("  template <class R>")
+    w("  KX_HD static void rates_n(R Trate, const R* n, const Params& p, R* k) {")
+    w("    (void)p; (void)n;")
+    w("    const R Tgas = Trate;")
+    w("    (void)Tgas;")
+    for g in dens_groups:
+        L.extend(emit_group(g, "    "))
+    w("  }")
+    w("  // every reaction: the T-only ones in k (zeroed) then the rest")
+    w("  template <class R>")
+    w("  KX_HD static void rates(R Trate, const R* n, const Params& p, R* k) {")
+    w("    for (int i = 0; i < NREA; ++i) k[i] = R(0);")
+    w("    rates_T(Trate, p, k);")
+    w("    rates_n(Trate, n, p, k);")
+    w("  }")
+    w("  static constexpr int NREA_T = %d;"
+      % sum(len(g) for g in tgroups))
+
+    # rhs
+    w("  // dn/dt = sum over reactions of S k prod(n), in reaction order")
+    w("  template <class R>")
+    w("  KX_HD static void rhs(const R* n, const R* k, R* dn) {")
+    w("    for (int i = 0; i < NSPEC; ++i) dn[i] = R(0);")
+    w("    R fl;")
+    for r in net.reactions:
+        prod = "*".join(["k[%d]" % r.index] + ["n[%d]" % net.idx(s)
+                                                for s, kin in r.reactants
+                                                if kin])
+        w("    fl = %s;  // %s" % (prod, r.eq))
+        for s, _ in r.reactants:
+            w("    dn[%d] -= fl;" % net.idx(s))
+        for s in r.products:
+            w("    dn[%d] += fl;" % net.idx(s))
+    w("    (void)fl;")
+    w("  }")
+    # analytic species Jacobian (fixed rates)
+    w("  // d(dn_i)/d(n_j) at fixed rates into J (column-major, element")
+    w("  // (i, j) at J[i + j ld]; added to, the caller zeroes it)")
+    w("  template <class R>")
+    w("  KX_HD static void jac_species(const R* n, const R* k, R* J, int ld) {")
+    w("    R d;")
+    for r in net.reactions:
+        kin = [net.idx(s) for s, k_ in r.reactants if k_]
+        for j in sorted(set(kin)):
+            mult = kin.count(j)
+            others = list(kin)
+            others.remove(j)
+            terms = ["k[%d]" % r.index] + ["n[%d]" % o for o in others]
+            if mult > 1:
+                terms.append("R(%r)" % float(mult))
+            w("    d = %s;  // %s, d/d n[%d]" % ("*".join(terms), r.eq, j))
+            for s, _ in r.reactants:
+                w("    J[%d + %d * ld] -= d;" % (net.idx(s), j))
+            for s in r.products:
+                w("    J[%d + %d * ld] += d;" % (net.idx(s), j))
+    w("    (void)d; (void)n; (void)k; (void)J; (void)ld;")
+    w("  }")
+
+    # thermal
+    if thermal_on:
+        _emit_thermal(w, net, th, pnames)
+    w("};")
+    w("}  // namespace kx")
+    meta = {"name": ns, "species": [s.name for s in net.species],
+            "charge": charges, "mass_g": masses, "params": pnames,
+            "param_defaults": [net.params.get(p, PARAM_DEFAULTS.get(p))
+                               for p in pnames],
+            "nrea": NR, "thermal": thermal_on,
+            "reactions": [r.eq for r in net.reactions],
+            "rates_density": bool(dens_groups)}
+    return "\n".join(L) + "\n", meta
+
+
+def _emit_thermal(w, net, th, pnames):
+    names = {s.name for s in net.species}
+    idx = net.idx
+    w("  // thermal terms at the thermal temperature Tth (capped to")
+    w("  // [2.73, 1e9] K by the caller) with the rates k of the state")
+    if "CHEM" in th.cooling or "CHEM" in th.heating:
+        heat, cool = [], []
+        for r in net.reactions:
+            R_ = sorted(s for s, _ in r.reactants)
+            P_ = sorted(r.products)
+            for rr, pp, q, ff in CHEM_LEDGER:
+                if R_ == rr and P_ == pp:
+                    (cool if q < 0 else heat).append((r, abs(q), ff))
+                    break
+    if "CHEM" in th.cooling:
+        w("  template <class R>")
+        w("  KX_HD static R cooling_chem(const R* n, const R* k) {")
+        w("    R c = R(0);")
+        for r, q, _ in cool:
+            prod = "*".join("n[%d]" % idx(s) for s, _ in r.reactants)
+            w("    c += k[%d] * R(%r) * %s;  // %s" % (r.index, q, prod, r.eq))
+        w("    return c * R(%r);" % EV_TO_ERG)
+        w("  }")
+    if "CHEM" in th.heating:
+        w("  template <class R>")
+        w("  KX_HD static R heating_chem(R Tth, const R* n, const R* k) {")
+        w("    R h = R(0);")
+        if heat:
+            w("    const R ffac = kx::h2_heat_fraction(Tth, n[%d], n[%d], "
+              "Hnuclei(n));" % (idx("H"), idx("H2")))
+            for r, q, ff in heat:
+                prod = "*".join("n[%d]" % idx(s) for s, _ in r.reactants)
+                w("    h += k[%d] * (R(%r) * ffac) * %s;  // %s"
+                  % (r.index, q, prod, r.eq))
+        w("    (void)Tth;")
+        w("    return h * R(%r);" % EV_TO_ERG)
+        w("  }")
+    # gamma
+    w("  template <class R>")
+    w("  KX_HD static R gamma_index(R Tth, const R* n, const Params& p) {")
+    if th.gamma == "CONST":
+        w("    (void)Tth; (void)n;")
+        w("    return R(p.gamma_ad);")
+    else:
+        mono = [s for s in net.species
+                if (s.name == "e-") or sum(s.comp.values()) == 1]
+        dis = [s for s in net.species if s.name != "e-"
+               and sum(s.comp.values()) == 2
+               and (s.name in GAMMA_VIB_K or s.name in GAMMA_ROT_K)]
+        ves = [GAMMA_VIB_K[s.name] for s in dis if s.name in GAMMA_VIB_K]
+        if ves:
+            w("    const R iTv = R(1) / kx::fmax(Tth, R(%r));"
+              % (min(ves) * 1e-2))
+        terms = []
+        for s in dis:
+            vp = "R(0)"
+            if s.name in GAMMA_VIB_K:
+                w("    const R x_%s = R(%r) * iTv, e_%s = kx::exp(x_%s);"
+                  % (s.cid, GAMMA_VIB_K[s.name], s.cid, s.cid))
+                vp = ("R(2) * x_%s * x_%s * e_%s / ((e_%s - R(1)) * "
+                      "(e_%s - R(1)))" % ((s.cid,) * 5))
+            rp = "R(2)"
+            if s.name == "H2":
+                rp = ("R(2) * kx::gamma_rot_ortho_para(Tth, R(p.opratio) / "
+                      "(R(p.opratio) + R(1)))")
+            elif s.name in GAMMA_ROT_K:
+                rp = "R(2) * kx::gamma_rot(Tth, R(%r))" % GAMMA_ROT_K[s.name]
+            w("    const R gi_%s = R(0.5) * (R(3) + %s + %s);"
+              % (s.cid, rp, vp))
+            terms.append(s)
+        w("    const R mosum = %s;" % (" + ".join("n[%d]" % idx(s.name)
+                                              for s in mono) or "R(0)"))
+        w("    const R ysum = mosum%s;"
+          % "".join(" + n[%d]" % idx(s.name) for s in terms))
+        w("    const R gsum = R(1.5) * mosum%s;"
+          % "".join(" + n[%d] * gi_%s" % (idx(s.name), s.cid)
+                    for s in terms))
+        w("    (void)p;")
+        w("    return gsum > R(0) ? R(1) + ysum / gsum : R(5) / R(3);")
+    w("  }")
+    # dT/dt
+    w("  // dTgas/dt: heating minus cooling (the reference's term order:")
+    w("  // heat CHEM, COMPRESS; cool H2, CHEM, CONT)")
+    w("  template <class R>")
+    w("  KX_HD static R thermal(R Tth, const R* n, const R* k, const Params& p) {")
+    w("    R heat = R(0), cool = R(0);")
+    if "CHEM" in th.heating:
+        w("    heat += heating_chem(Tth, n, k);")
+    if "COMPRESS" in th.heating:
+        w("    heat += ntot(n) * R(%r) * Tth / R(p.tff);" % BOLTZMANN_ERG)
+    if "H2" in th.cooling:
+        if "H2" not in names:
+            raise KnetError("H2 cooling without H2")
+        args = ", ".join("n[%d]" % idx(x) if x in names else "R(0)"
+                         for x in ("H", "H+", "H2", "e-", "He"))
+        op = ("kx::h2_opacity_ripamonti(ntot(n))"
+              if th.h2_opacity == "RIPAMONTI" else "R(1)")
+        w("    cool += kx::cooling_H2(Tth, %s) * %s;" % (args, op))
+    if "CHEM" in th.cooling:
+        w("    cool += cooling_chem(n, k);")
+    if "CONT" in th.cooling:
+        w("    cool += kx::cooling_continuum(Tth, rho(n), ntot(n));")
+    w("    return (gamma_index(Tth, n, p) - R(1)) * (heat - cool)")
+    w("        / (R(%r) * ntot(n));" % BOLTZMANN_ERG)
+    w("  }")
+
+
+def meta_header(meta, th):
+    """the names and defaults a network's C ABI reports (krome_species_name,
+    krome_param_name, krome_params_default, krome_param_required_by)"""
+    def cstr(xs):
+        return ", ".join('"%s"' % x for x in xs)
+    req = []
+    for p, d in zip(meta["params"], meta["param_defaults"]):
+        if d is not None and isinstance(d, float) and math.isnan(d):
+            req.append("thermal.heating = ['COMPRESS']" if p == "tff"
+                       else "the network")
+        else:
+            req.append("")
+    L = ["// generated by netgen.py -- do not edit",
+         "// the species and parameter names of network %s" % meta["name"],
+         "const char* const kSpecies[] = {%s};" % cstr(meta["species"]),
+         "const char* const kParams[] = {%s};" % cstr(meta["params"]),
+         "const double kParamDefaults[] = {%s};" % ", ".join(
+             "NAN" if (isinstance(d, float) and math.isnan(d)) else repr(float(d))
+             for d in meta["param_defaults"]),
+         "const char* const kParamRequiredBy[] = {%s};" % cstr(req)]
+    return "\n".join(L) + "\n"
+
+
+# the configurations this project builds: (network file, thermal options)
+CONFIGS = {
+    "primordial": ("primordial.knet", Thermal()),
+    "primordial3": ("primordial3.knet",
+                    Thermal(cooling=("H2", "CONT", "CHEM"),
+                            heating=("COMPRESS", "CHEM"), gamma="EXACT",
+                            h2_opacity="RIPAMONTI")),
+}
+
+
+def main(argv=None):
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--networks", required=True,
+                    help="folder of the .knet files")
+    ap.add_argument("--out", required=True, help="output folder")
+    a = ap.parse_args(argv)
+    out = Path(a.out)
+    out.mkdir(parents=True, exist_ok=True)
+    metas = {}
+    for name, (fn, th) in CONFIGS.items():
+        text, meta = generate(Path(a.networks) / fn, th, name)
+        (out / ("net_%s.h" % name)).write_text(text)
+        (out / ("net_%s_meta.h" % name)).write_text(meta_header(meta, th))
+        metas[name] = meta
+    (out / "networks.json").write_text(json.dumps(metas, indent=1))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,285 @@
+// ORACLE (test infrastructure only; tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg use it as the checker or the CPU baseline,
+// never the product path): a CPU restatement of KROME-X's per-cell solve
+// and its OpenMP batch loop,
+//   solve()               runtime/core/solve_cell.hpp:2094-2203
+//   integrate_ + charge   solve_cell.hpp:1631-1705
+//   full_rhs_, rhs_host_  solve_cell.hpp:540-557, 750-813 (rates at
+//                         min(max(T, Tcmb), 1e9), thermal terms at
+//                         min(max(T, 2.73), 1e9), T <= 0 or a non-finite
+//                         rhs a recoverable failure)
+//   jac_cell_ (fixed T)   solve_cell.hpp:883-1132 (the analytic species
+//                         block; networks with density-dependent rates
+//                         are refused at fixed T)
+//   solve_cells_percell   runtime/core/solve_cells.hpp:47-118, 130-161
+// over the networks of krome-x_amd/csrc/generated, with the integrator of
+// oracle/kx_bdf.hpp, or with -DKX_REF_BDF the reference's own
+// runtime/core/cvode/bdf_cell.hpp (oracle/Makefile target ref).
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "net_primordial.h"
+#include "net_primordial3.h"
+
+#ifdef KX_REF_BDF
+#include "bdf_cell.hpp"
+namespace bdf_ = krome::bdf;
+#else
+#include "kx_bdf.hpp"
+namespace bdf_ = kxo;
+#endif
+
+namespace {
+
+// KROME flags (runtime/krome.hpp:95-145)
+constexpr int FLAG_UNSUPPORTED = -104;
+constexpr int FLAG_PARAM_UNSET = -105;
+constexpr int FLAG_NONPOSITIVE_T = -110;
+
+// the options row of the C entry points
+enum Opt {
+  O_EVOLVE_T, O_RTOL, O_ATOL_N, O_ATOL_T, O_LINSOL, O_JAC, O_MAX_STEPS,
+  O_MAX_ERR_FAILS, O_MAX_CONV_FAILS, O_MAX_RESTARTS, O_CLIP, O_CONSERVE,
+  O_COUNT
+};
+
+template <class Net>
+struct Problem {
+  typename Net::Params p;
+  bool evolve_T = false;
+  double Tfix = 0.0;
+  bool hit_T0 = false;
+  double kT[Net::NREA + 1];
+  double kT_T = -1.0;
+
+  double rate_T(const double* y) const {
+    const double T = evolve_T ? y[Net::NSPEC] : Tfix;
+    return std::fmin(std::fmax(T, p.Tcmb), 1e9);
+  }
+  void rates(double Tr, const double* y, double* k) {
+    if (Tr != kT_T) {  // the T-only rates, cached per rate temperature
+      for (int i = 0; i < Net::NREA; ++i) kT[i] = 0.0;
+      Net::rates_T(Tr, p, kT);
+      kT_T = Tr;
+    }
+    for (int i = 0; i < Net::NREA; ++i) k[i] = kT[i];
+    Net::rates_n(Tr, y, p, k);
+  }
+  int rhs(double, const double* y, double* f) {
+    constexpr int NS = Net::NSPEC;
+    if (evolve_T && !(y[NS] > 0.0)) {
+      hit_T0 = true;
+      return 1;
+    }
+    double k[Net::NREA + 1];
+    rates(rate_T(y), y, k);
+    Net::rhs(y, k, f);
+    int neq = NS;
+    if constexpr (Net::THERMAL) {
+      if (evolve_T) {
+        const double Tth = std::fmin(std::fmax(y[NS], 2.73), 1e9);
+        f[NS] = Net::thermal(Tth, y, k, p);
+        neq = NS + 1;
+      }
+    }
+    for (int i = 0; i < neq; ++i)
+      if (!std::isfinite(f[i])) return 1;
+    return 0;
+  }
+  int jac(double, const double* y, const double*, double* J, double*,
+          double*, double*) {
+    double k[Net::NREA + 1];
+    rates(rate_T(y), y, k);
+    Net::jac_species(y, k, J, Net::NSPEC);
+    return 0;
+  }
+};
+
+template <class Net>
+int solve_one(double* n, double& T, const double* row, double dt,
+              const double* o, long* stats) {
+  constexpr int NS = Net::NSPEC;
+  if (Net::params_missing(row) >= 0) return FLAG_PARAM_UNSET;
+  const bool evolve_T = o[O_EVOLVE_T] != 0.0;
+  if (evolve_T && !Net::THERMAL) return FLAG_UNSUPPORTED;
+  const int jac = static_cast<int>(o[O_JAC]);
+  const bool dq = jac == 2 || (jac == 0 && evolve_T);
+  if (!dq && (evolve_T || Net::RATES_DENSITY)) return FLAG_UNSUPPORTED;
+  Problem<Net> P;
+  P.p = Net::params_from_row(row);
+  P.evolve_T = evolve_T;
+  P.Tfix = T;
+  const int N = NS + (evolve_T ? 1 : 0);
+  double y[NS + 1], atol[NS + 1];
+  for (int i = 0; i < NS; ++i) {
+    y[i] = n[i];
+    atol[i] = o[O_ATOL_N];
+  }
+  if (evolve_T) {
+    y[NS] = T;
+    atol[NS] = o[O_ATOL_T];
+  }
+  int charge[NS];
+  for (int i = 0; i < NS; ++i) charge[i] = Net::charge(i);
+  bdf_::Opts bo;
+  bo.rtol = o[O_RTOL];
+  bo.atol = atol;
+  bo.max_steps = static_cast<long>(o[O_MAX_STEPS]);
+  bo.max_err_fails = static_cast<int>(o[O_MAX_ERR_FAILS]);
+  bo.max_conv_fails = static_cast<int>(o[O_MAX_CONV_FAILS]);
+  bo.dq_jac = dq;
+  if (Net::HAS_E) {  // the charge invariant (solve_cell.hpp:1631-1642)
+    bo.inv_row = Net::I_E;
+    bo.inv_n = NS;
+    bo.inv_c = charge;
+  }
+  std::vector<double> work(bdf_::work_doubles(N));
+  int iwork[NS + 1];
+  bdf_::Stats bs;
+  int nr = 0;
+#ifdef KX_REF_BDF
+  int flag = bdf_::integrate<Problem<Net>, true>(
+      P, N, y, dt, bo, static_cast<int>(o[O_MAX_RESTARTS]), work.data(),
+      iwork, &bs, &nr, 1, nullptr);
+#else
+  int flag = bdf_::integrate(P, N, y, dt, bo,
+                             static_cast<int>(o[O_MAX_RESTARTS]),
+                             work.data(), iwork, &bs, &nr);
+#endif
+  if (flag == 0 && evolve_T && !(y[NS] > 0.0 && std::isfinite(y[NS])))
+    flag = FLAG_NONPOSITIVE_T;
+  if (P.hit_T0 && (flag == -10 || flag == -8 || flag == -11 || flag == -9))
+    flag = FLAG_NONPOSITIVE_T;
+  if (stats != nullptr) {
+    const long v[9] = {bs.nst, bs.nfe, bs.nje, bs.nfeDQ, bs.nsetups,
+                       bs.nni, bs.nnf, bs.netf, (long)nr};
+    for (int i = 0; i < 9; ++i) stats[i] = v[i];
+  }
+  if (flag != 0) return flag;
+  double n0[NS];
+  for (int i = 0; i < NS; ++i) {
+    n0[i] = n[i];
+    n[i] = y[i];
+    if (o[O_CLIP] != 0.0 && n[i] < 0.0) n[i] = 0.0;
+  }
+  const int mode = static_cast<int>(o[O_CONSERVE]);
+  if (Net::HAS_E && mode == 1) {  // solve_cell.hpp:1478-1487
+    double q0 = 0.0, q = 0.0;
+    for (int i = 0; i < NS; ++i) {
+      q0 += Net::charge(i) * n0[i];
+      q += Net::charge(i) * n[i];
+    }
+    n[Net::I_E] = std::fmax(n[Net::I_E] + (q - q0), 0.0);
+  } else if (Net::HAS_E && mode == 3) {  // solve_cell.hpp:1463-1471
+    double qi = 0.0;
+    for (int i = 0; i < NS; ++i)
+      if (i != Net::I_E) qi += Net::charge(i) * n[i];
+    n[Net::I_E] = std::fmax(qi, 0.0);
+  }
+  if (evolve_T) T = y[NS];
+  return 0;
+}
+
+template <class Net>
+int solve_cells(int ncell, double* n, double* T, const double* dt,
+                const double* par, int par_stride, const double* o,
+                int* flags, long* stats) {
+  constexpr int NS = Net::NSPEC;
+  if (static_cast<int>(o[O_CONSERVE]) == 2) return FLAG_UNSUPPORTED;
+  std::vector<double> dflt(Net::NPAR, 0.0);
+  bool missing = false;
+  for (int c = 0; c < ncell; ++c) {
+    const double* row = par + static_cast<size_t>(c) * par_stride;
+    const bool m = Net::params_missing(row) >= 0;
+    if (flags) flags[c] = m ? FLAG_PARAM_UNSET : 0;
+    missing |= m;
+  }
+  if (missing) return FLAG_PARAM_UNSET;
+  std::vector<int> fl(ncell, 0);
+#pragma omp parallel for schedule(dynamic)
+  for (int c = 0; c < ncell; ++c) {
+    long* sc = stats ? stats + static_cast<size_t>(c) * 9 : nullptr;
+    if (dt[c] > 0.0) {
+      fl[c] = solve_one<Net>(n + static_cast<size_t>(c) * NS, T[c],
+                             par + static_cast<size_t>(c) * par_stride,
+                             dt[c], o, sc);
+    } else if (sc) {
+      for (int i = 0; i < 9; ++i) sc[i] = 0;
+    }
+  }
+  int first = 0;
+  for (int c = 0; c < ncell; ++c) {
+    if (flags) flags[c] = fl[c];
+    if (first == 0) first = fl[c];
+  }
+  return first;
+}
+
+template <class Net>
+void funcs(int ncell, const double* n, const double* T, const double* par,
+           int evolve_T, double* k, double* f) {
+  constexpr int NS = Net::NSPEC;
+  for (int c = 0; c < ncell; ++c) {
+    Problem<Net> P;
+    P.p = Net::params_from_row(par + static_cast<size_t>(c) * Net::NPAR);
+    P.evolve_T = evolve_T != 0;
+    P.Tfix = T[c];
+    double y[NS + 1];
+    for (int i = 0; i < NS; ++i) y[i] = n[c * NS + i];
+    y[NS] = T[c];
+    P.rates(P.rate_T(y), y, k + static_cast<size_t>(c) * Net::NREA);
+    P.rhs(0.0, y, f + static_cast<size_t>(c) * (NS + 1));
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+#ifdef KX_REF_BDF
+#define KXO_(name) kxr_##name
+#else
+#define KXO_(name) kxo_##name
+#endif
+
+// a network of krome-x_amd/csrc/generated by name: 0 primordial,
+// 1 primordial3, -1 unknown
+int KXO_(network)(const char* name) {
+  if (!std::strcmp(name, "primordial")) return 0;
+  if (!std::strcmp(name, "primordial3")) return 1;
+  return -1;
+}
+
+// krome_solve_cells semantics over per-cell parameter rows (par_stride 0:
+// one row for all); opts[O_COUNT] as enum Opt; stats[ncell 9] optional
+int KXO_(solve_cells)(int net, int ncell, double* n, double* T,
+                      const double* dt, const double* par, int par_stride,
+                      const double* opts, int* flags, long* stats) {
+  switch (net) {
+    case 0:
+      return solve_cells<kx::net_primordial>(ncell, n, T, dt, par,
+                                             par_stride, opts, flags, stats);
+    case 1:
+      return solve_cells<kx::net_primordial3>(ncell, n, T, dt, par,
+                                              par_stride, opts, flags, stats);
+  }
+  return -1;
+}
+
+// rates k[ncell NREA] and the full rhs f[ncell (NSPEC + 1)] at states
+int KXO_(funcs)(int net, int ncell, const double* n, const double* T,
+                const double* par, int evolve_T, double* k, double* f) {
+  switch (net) {
+    case 0:
+      funcs<kx::net_primordial>(ncell, n, T, par, evolve_T, k, f);
+      return 0;
+    case 1:
+      funcs<kx::net_primordial3>(ncell, n, T, par, evolve_T, k, f);
+      return 0;
+  }
+  return -1;
+}
+
+}  // extern "C"

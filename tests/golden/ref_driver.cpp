@@ -1,0 +1,187 @@
+// Fixture writer over a KROME-X project generated by the reference's own
+// code generator (tests/golden/make_golden.py runs the generator in this
+// container and compiles this file against its output with g++).  It calls
+// the reference's public C++ API only: krome::solve_cells (the PER-CELL
+// OpenMP path, runtime/core/solve_cells.hpp:47-118) for whole solves, and
+// the per-cell building blocks of solve_cell.hpp (init_ctx_, full_rhs_,
+// jac_host_) plus the generated rates / thermal functions for the
+// function-level vectors.  Nothing of it is part of the product.
+//
+//   ref_driver solve IN OUT     IN: i64 ncell, nspec, npar, nopt; f64
+//                               opts[nopt]; n0[ncell nspec]; T[ncell];
+//                               dt[ncell]; par[ncell npar].  OUT: f64
+//                               n[ncell nspec], T[ncell], flag[ncell],
+//                               stats[ncell 9], seconds
+//   ref_driver funcs IN OUT     IN as above (dt unused).  OUT per sample:
+//                               k[NREA], f[NEQ] (full rhs), J[NEQ^2]
+//                               (column-major, the solver's analytic
+//                               Jacobian), and the thermal terms
+//                               (8 values, 0 when not configured)
+#include "krome.hpp"
+
+#include <chrono>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+struct In {
+  int64_t ncell = 0, nspec = 0, npar = 0, nopt = 0;
+  std::vector<double> opts, n0, T, dt, par;
+};
+
+bool read_in(const char* path, In& in) {
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return false;
+  int64_t h[4];
+  if (std::fread(h, sizeof(int64_t), 4, f) != 4) return false;
+  in.ncell = h[0];
+  in.nspec = h[1];
+  in.npar = h[2];
+  in.nopt = h[3];
+  auto rd = [&](std::vector<double>& v, size_t m) {
+    v.resize(m);
+    return std::fread(v.data(), sizeof(double), m, f) == m;
+  };
+  bool ok = rd(in.opts, in.nopt) && rd(in.n0, in.ncell * in.nspec) &&
+            rd(in.T, in.ncell) && rd(in.dt, in.ncell) &&
+            rd(in.par, in.ncell * in.npar);
+  std::fclose(f);
+  return ok;
+}
+
+// opts: evolve_T, rtol, atol_n, atol_T, linsol, jac, max_steps
+krome::SolverOpts solver_opts(const In& in) {
+  krome::SolverOpts so;
+  so.evolve_T = in.opts[0] != 0.0;
+  so.rtol = in.opts[1];
+  so.atol_n = in.opts[2];
+  so.atol_T = in.opts[3];
+  const int ls = static_cast<int>(in.opts[4]);
+  so.linsol = ls == 1 ? krome::LINSOL_DENSE : krome::LINSOL_AUTO;
+  const int jc = static_cast<int>(in.opts[5]);
+  so.jac = jc == 1 ? krome::JAC_ANALYTIC
+                   : (jc == 2 ? krome::JAC_FD : krome::JAC_AUTO);
+  so.max_steps = static_cast<long>(in.opts[6]);
+  so.error_report = false;
+  return so;
+}
+
+int do_solve(const In& in, const char* out) {
+  if (in.nspec != krome::NSPEC || in.npar != krome::NPAR) return 2;
+  krome::SolverOpts so = solver_opts(in);
+  krome::CellsOpts co;
+  co.path = krome::PATH_PERCELL;
+  std::vector<int> flags(in.ncell, 0);
+  std::vector<krome::SolveStats> st(in.ncell);
+  co.flags = flags.data();
+  co.stats = st.data();
+  std::vector<krome::Params> P(in.ncell);
+  for (int64_t c = 0; c < in.ncell; ++c)
+    krome::params_from_row(in.par.data() + c * in.npar, P[c]);
+  std::vector<double> n = in.n0, T = in.T;
+  const auto t0 = std::chrono::steady_clock::now();
+  krome::solve_cells(static_cast<int>(in.ncell), n.data(), T.data(),
+                     in.dt.data(), P.data(), so, co);
+  const double secs = std::chrono::duration<double>(
+                          std::chrono::steady_clock::now() - t0).count();
+  FILE* f = std::fopen(out, "wb");
+  if (!f) return 3;
+  std::fwrite(n.data(), sizeof(double), n.size(), f);
+  std::fwrite(T.data(), sizeof(double), T.size(), f);
+  for (int64_t c = 0; c < in.ncell; ++c) {
+    const double fl = flags[c];
+    std::fwrite(&fl, sizeof(double), 1, f);
+  }
+  for (int64_t c = 0; c < in.ncell; ++c) {
+    const krome::SolveStats& s = st[c];
+    const double v[9] = {double(s.steps), double(s.rhs_evals),
+                         double(s.jac_evals), double(s.rhs_evals_jac),
+                         double(s.lin_setups), double(s.newton_iters),
+                         double(s.newton_fails), double(s.err_fails),
+                         double(s.restarts)};
+    std::fwrite(v, sizeof(double), 9, f);
+  }
+  std::fwrite(&secs, sizeof(double), 1, f);
+  std::fclose(f);
+  return 0;
+}
+
+// the configured thermal terms at the thermal temperature: H2 cooling
+// (with its opacity factor), continuum, chemical cooling, chemical
+// heating, compression heating, gamma, dT/dt, the H2 collisional
+// dissociation rate at the rate temperature
+template <class P>
+void thermal_terms(double Tth, double Trate, const double* n, const double* k,
+                   const P& p, krome::ThermCtx* tc, double* o) {
+  using TC = krome::ThermalConfig;
+  for (int i = 0; i < 8; ++i) o[i] = 0.0;
+  if constexpr (TC::cool_H2)
+    o[0] = krome::cooling_H2(Tth, n) * krome::h2_opacity_(Tth, n, p);
+  if constexpr (TC::cool_cont) o[1] = krome::cooling_Continuum(Tth, n);
+  if constexpr (TC::cool_chem) o[2] = krome::cooling_chem(Tth, n, k, p);
+  if constexpr (TC::heat_chem) o[3] = krome::heating_chem(Tth, n, k, p);
+  if constexpr (TC::heat_compress) o[4] = krome::heat_compress_(Tth, n, p);
+  o[5] = krome::gamma_index(Tth, n, p);
+  double dnx[krome::NSPEC];
+  o[6] = krome::thermal_aux(Tth, n, k, tc, dnx);
+#ifdef KROME_EXTERN_dissH2_Martin96
+  o[7] = krome::dissH2_Martin96(Trate, n, p);
+#else
+  (void)Trate;
+#endif
+}
+
+int do_funcs(const In& in, const char* out) {
+  if (in.nspec != krome::NSPEC || in.npar != krome::NPAR) return 2;
+  krome::SolverOpts so = solver_opts(in);
+  FILE* f = std::fopen(out, "wb");
+  if (!f) return 3;
+  for (int64_t c = 0; c < in.ncell; ++c) {
+    krome::Params p;
+    krome::params_from_row(in.par.data() + c * in.npar, p);
+    const double* n = in.n0.data() + c * in.nspec;
+    krome::SolveCtx ctx;
+    if (krome::init_ctx_(ctx, in.T[c], p, so) != 0) return 4;
+    krome::install_hooks_(ctx, n, so);
+    const int NEQ = ctx.NEQ;
+    std::vector<double> y(NEQ), fv(NEQ), J(NEQ * NEQ), t1(NEQ), t2(NEQ);
+    krome::pack_state_(ctx, n, in.T[c], so, y.data());
+    double k[krome::NREA + 1];
+    const double Trate = krome::state_T_(y.data(), &ctx);
+    krome::rates(Trate, y.data(), ctx.tc.p, k);
+    krome::full_rhs_(y.data(), fv.data(), &ctx);
+    krome::jac_host_(y.data(), krome::DenseMat_(J.data(), NEQ), &ctx,
+                     t1.data(), t2.data());
+    double th[8];
+    krome::rates(Trate, y.data(), ctx.tc.p, k);
+    thermal_terms(krome::thermal_T_(in.T[c]), Trate, y.data(), k, ctx.tc.p,
+                  &ctx.tc, th);
+    std::fwrite(k, sizeof(double), krome::NREA, f);
+    std::fwrite(fv.data(), sizeof(double), NEQ, f);
+    std::fwrite(J.data(), sizeof(double), NEQ * NEQ, f);
+    std::fwrite(th, sizeof(double), 8, f);
+  }
+  std::fclose(f);
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc != 4) {
+    std::fprintf(stderr, "usage: %s solve|funcs IN OUT\n", argv[0]);
+    return 1;
+  }
+  In in;
+  if (!read_in(argv[2], in)) {
+    std::fprintf(stderr, "cannot read %s\n", argv[2]);
+    return 1;
+  }
+  if (!std::strcmp(argv[1], "solve")) return do_solve(in, argv[3]);
+  if (!std::strcmp(argv[1], "funcs")) return do_funcs(in, argv[3]);
+  return 1;
+}
