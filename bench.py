@@ -1,0 +1,253 @@
+"""Benchmark: cells/s of the batched per-cell stiff-chemistry solve on one
+MI355X, primordial3 collapse cells (2^20 cells, seed 1: nH 1..1e16 cm^-3,
+Tgas 10..3e3 K, x_ion 1e-8..1e-3, H2/nH 1e-8..0.5, He 0.0775 nH, dt = 0.01
+tff, Tgas evolved, cooling H2 + CONT + CHEM, heating COMPRESS + CHEM,
+gamma EXACT, H2 opacity RIPAMONTI; rtol 1e-8, atol_n 1e-20, atol_T 1e-6).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--ncell N]
+                    [--kernel persistent|plain] [--fp64]
+
+A step is one solve of the whole batch from its initial state: the inputs
+are resident in HBM (species-major n[nspec, ncell]); each step first
+restores the working arrays from a resident pristine copy (a
+device-to-device copy inside the step), then runs the solve kernel.  The
+headline is the persistent batched kernel with the mixed-precision Newton
+(FP32 Jacobian / LU / triangular solves, FP64 state, residual and error
+test); beside it, in the same run: the plain FP64 one-thread-per-cell
+drop-in kernel on the same cells (ratio), the agreement of the two, and
+the CPU restatement of the reference solver on a bounded sample of the
+same cells on the host's cores (cpu_baseline).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO / "krome-x_amd"))
+
+METRIC = ("cells/sec to t_end, mixed-prec persistent vs plain FP64 GPU, "
+          "@rtol 1e-4 vs FP64")
+PEAK_FP64 = 78.6e12       # MI355X FP64 vector (= FP64 matrix), FLOP/s
+PEAK_FP32 = 157.3e12      # MI355X FP32 vector
+PEAK_HBM = 8.0e12         # bytes/s
+# algorithmic bytes per cell: read n (9), Tgas, dt, 3 parameters; write n
+# (9), Tgas (8 B each) and the flag (4 B)
+BYTES_PER_CELL = 8 * (9 + 1 + 1 + 3) + 8 * (9 + 1) + 4
+
+
+def flops_per_cell(stats, opc, N=10, q_avg=3.0):
+    """algorithmic FLOPs of each cell from its counters: rhs evaluations
+    (Newton residuals + DQ Jacobian columns) x the generated rhs's
+    operation count, the DQ column updates, the Newton matrix setups (A =
+    I - gamma J and the LU), the triangular solves with the Newton vector
+    updates, and the per-step Nordsieck / error-test vector work (at the
+    mean order q_avg)"""
+    f_rhs = opc["full_rhs"]
+    f_getrf = N * (N - 1) / 2 + (N - 1) * N * (2 * N - 1) / 3 + N
+    f_getrs = 2 * N * N - N
+    step = (q_avg * (q_avg + 1) / 2 + 2 * (q_avg + 1) + 12) * N
+    st = stats.astype(np.float64)
+    nst, nfe, nje, nfeDQ, nset, nni = (st[:, 0], st[:, 1], st[:, 2],
+                                       st[:, 3], st[:, 4], st[:, 5])
+    fp_rhs64 = nfe * f_rhs
+    fp_dq = nfeDQ * f_rhs + nje * N * (2 * N + 4)
+    fp_lu = nset * (N * N + N + f_getrf) + nni * f_getrs
+    fp_vec = nni * 14 * N + nst * step
+    return fp_rhs64, fp_dq, fp_lu, fp_vec
+
+
+def cpu_baseline(w, nsample):
+    """the oracle (CPU restatement of the reference solver, OpenMP over
+    cells) on the first nsample cells of the workload"""
+    lib = ctypes.CDLL(str(REPO / "oracle" / "_build" / "libkx_oracle.so"))
+    D = ctypes.POINTER(ctypes.c_double)
+    cores = min(16, os.cpu_count() or 1)
+    lib.kxo_set_threads(cores)
+    cores = lib.kxo_get_threads()
+    n = np.ascontiguousarray(w["n"][:nsample])
+    T = np.ascontiguousarray(w["T"][:nsample])
+    dt = np.ascontiguousarray(w["dt"][:nsample])
+    par = np.ascontiguousarray(w["par"][:nsample])
+    o = w["opts"]
+    opts = np.array([o["evolve_T"], o["rtol"], o["atol_n"], o["atol_T"],
+                     1, 0, o["max_steps"], 7, 10, 5, 1, 0], dtype=np.float64)
+    flags = np.zeros(nsample, dtype=np.intc)
+    t0 = time.perf_counter()
+    r = lib.kxo_solve_cells(1, nsample, n.ctypes.data_as(D),
+                            T.ctypes.data_as(D), dt.ctypes.data_as(D),
+                            par.ctypes.data_as(D), par.shape[1],
+                            opts.ctypes.data_as(D),
+                            flags.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                            None)
+    el = time.perf_counter() - t0
+    return {"value": nsample / el, "unit": "cells/s", "cores": int(cores),
+            "kind": "port",
+            "sample": "the first %d of the same cells, CPU restatement of "
+                      "the reference per-cell solve (OpenMP dynamic), %.1f s "
+                      "wall, return %d" % (nsample, el, r)}, n, T
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--ncell", type=int, default=1 << 20)
+    ap.add_argument("--kernel", default="persistent",
+                    choices=("persistent", "plain"))
+    ap.add_argument("--fp64", action="store_true",
+                    help="the persistent kernel with the FP64 Newton")
+    ap.add_argument("--cpu-sample", type=int, default=8192)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU only: the cells are independent and "
+              "this build has no multi-GPU path (--gpus %d refused)" % a.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import cells
+    import krome
+
+    dev = torch.device("cuda:0")
+    net = krome.load("primordial3")
+    w = cells.primordial3_cells(a.ncell)
+    opc = json.loads((REPO / "krome-x_amd" / "csrc" / "generated" /
+                      "opcount.json").read_text())["primordial3"]
+    n0 = torch.tensor(np.ascontiguousarray(w["n"].T), device=dev)
+    T0 = torch.tensor(w["T"], device=dev)
+    dt = torch.tensor(w["dt"], device=dev)
+    par = torch.tensor(np.ascontiguousarray(w["par"]), device=dev)
+    n = n0.clone()
+    T = T0.clone()
+    flags = torch.zeros(a.ncell, dtype=torch.int32, device=dev)
+    stats = torch.zeros((a.ncell, 9), dtype=torch.int64, device=dev)
+    base = dict(evolve_T=True, rtol=w["opts"]["rtol"],
+                atol_n=w["opts"]["atol_n"], atol_T=w["opts"]["atol_T"],
+                error_report=False)
+    mixed = a.kernel == "persistent" and not a.fp64
+    head = net.options(**base, device_kernel=a.kernel,
+                       mixed_precision=1 if mixed else 0)
+    plain = net.options(**base, device_kernel="plain")
+
+    def run(opts, st=None):
+        n.copy_(n0)
+        T.copy_(T0)
+        return net.solve_device(n, T, dt, par, opts=opts, flags=flags,
+                                stats=st, sync=False)
+
+    # warmup (the first also collects the counters for the FLOP count)
+    for i in range(a.warmup):
+        r = run(head, stats if i == 0 else None)
+        torch.cuda.synchronize()
+        if r != 0:
+            raise SystemExit("warmup solve returned %d" % r)
+    if a.warmup == 0:
+        run(head, stats)
+        torch.cuda.synchronize()
+    st_head = stats.cpu().numpy()
+    fl_head = flags.cpu().numpy()
+    n_head = n.cpu().numpy().copy()
+    T_head = T.cpu().numpy().copy()
+    # timed steps
+    ev = [(torch.cuda.Event(enable_timing=True),
+           torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        n.copy_(n0)
+        T.copy_(T0)
+        ev[k][0].record()
+        net.solve_device(n, T, dt, par, opts=head, flags=flags, sync=False)
+        ev[k][1].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = np.mean([e0.elapsed_time(e1) for e0, e1 in ev])
+    ms_per_step = 1e3 * wall / a.steps
+    value = a.ncell * a.steps / wall
+
+    # the plain FP64 drop-in on the same cells (ratio, agreement)
+    plain_res = None
+    if not a.no_plain and a.kernel != "plain":
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        run(plain)
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - t1
+        n_pl = n.cpu().numpy()
+        T_pl = T.cpu().numpy()
+        fl_pl = flags.cpu().numpy()
+        ok = (fl_pl == 0) & (fl_head == 0)
+        ntot = n_pl.sum(axis=0)
+        big = np.abs(n_pl) > 1e-12 * ntot[None, :]
+        rel = np.abs(n_head - n_pl) / np.maximum(np.abs(n_pl), 1e-300)
+        rel = np.where(big & ok[None, :], rel, 0.0)
+        Trel = np.where(ok, np.abs(T_head - T_pl) / T_pl, 0.0)
+        plain_res = {"value": a.ncell / tp, "seconds": tp,
+                     "max_rel_species_vs_fp64": float(rel.max()),
+                     "max_rel_species_vs_fp64_by_species": {
+                         s: float(rel[i].max())
+                         for i, s in enumerate(net.species)},
+                     "max_rel_Tgas_vs_fp64": float(Trel.max()),
+                     "flags_equal": bool(np.array_equal(fl_pl, fl_head))}
+
+    # roofline of the solve kernel: algorithmic FLOPs per launch from the
+    # cells' counters over the kernel's event-timed duration
+    fr, fdq, flu, fvec = flops_per_cell(st_head, opc)
+    tot = float((fr + fdq + flu + fvec).sum())
+    fp32 = float((fdq + flu).sum()) if mixed else 0.0
+    achieved = tot / (kern_ms * 1e-3)
+    traffic = None
+    tf = REPO / "profiles" / "traffic.json"
+    if tf.exists():
+        traffic = json.loads(tf.read_text()).get(
+            "mixed" if mixed else ("persistent_fp64" if a.kernel ==
+                                   "persistent" else "plain"))
+    roof = {"bound": "mfma", "achieved": achieved / 1e12,
+            "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP64, "traffic": traffic,
+            "compute": "FP64 vector ALU (MI355X FP64 vector peak = FP64 "
+                       "matrix peak, 78.6 TFLOP/s); transcendentals count "
+                       "one op each",
+            "flop_per_launch": tot, "fp32_share": fp32 / tot,
+            "kernel_ms": float(kern_ms),
+            "algorithmic_bytes_per_launch": BYTES_PER_CELL * a.ncell,
+            "hbm_frac": BYTES_PER_CELL * a.ncell / (kern_ms * 1e-3)
+            / PEAK_HBM}
+    cpu = None
+    if not a.no_cpu:
+        cpu, _, _ = cpu_baseline(w, min(a.cpu_sample, a.ncell))
+    steps = st_head[:, 0]
+    out = {
+        "metric": METRIC, "value": value, "unit": "cells/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64+f32" if mixed else "f64", "data": "synthetic",
+        "config": {"workload": "primordial3 collapse cells, Tgas evolved, "
+                               "seed 1, dt = 0.01 tff, rtol 1e-8",
+                   "ncell": a.ncell, "kernel": a.kernel,
+                   "mixed_precision": mixed, "parallelism": "1 GPU"},
+        "roofline": roof, "cpu_baseline": cpu,
+        "flags_nonzero": int((fl_head != 0).sum()),
+        "steps_per_cell": {"p50": float(np.percentile(steps, 50)),
+                           "p99": float(np.percentile(steps, 99)),
+                           "max": int(steps.max())},
+        "plain_fp64": plain_res,
+        "ratio_vs_plain_fp64": (value / plain_res["value"]
+                                if plain_res else None),
+        "ratio_vs_cpu": value / cpu["value"] if cpu else None,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -672,8 +672,8 @@ struct net_primordial3 {
   template <class R>
   KX_HD static R gamma_index(R Tth, const R* n, const Params& p) {
     const R iTv = R(1) / kx::fmax(Tth, R(63.3236349736972));
-    const R x_H2 = R(6332.36349736972) * iTv, e_H2 = kx::exp(x_H2);
-    const R gi_H2 = R(0.5) * (R(3) + R(2) * kx::gamma_rot_ortho_para(Tth, R(p.opratio) / (R(p.opratio) + R(1))) + R(2) * x_H2 * x_H2 * e_H2 / ((e_H2 - R(1)) * (e_H2 - R(1))));
+    const R x_H2 = R(6332.36349736972) * iTv;
+    const R gi_H2 = R(0.5) * (R(3) + R(2) * kx::gamma_rot_ortho_para(Tth, R(p.opratio) / (R(p.opratio) + R(1))) + R(2) * kx::vib_heat_capacity(x_H2));
     const R gi_H2j = R(0.5) * (R(3) + R(2) * kx::gamma_rot(Tth, R(61.72352804491875)) + R(0));
     const R mosum = n[0] + n[1] + n[2] + n[5] + n[6] + n[7] + n[8];
     const R ysum = mosum + n[3] + n[4];

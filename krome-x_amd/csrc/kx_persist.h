@@ -1,16 +1,1326 @@
-// The persistent batched kernel (placeholder until the lockstep kernel
-// lands): not available yet.
+// The persistent batched kernel: KROME-X's per-cell integrator
+// (runtime/core/cvode/bdf_cell.hpp, CVODE's BDF; the per-cell solve of
+// runtime/core/solve_cell.hpp:2094-2203) restated as a state machine that
+// suspends at every right-hand-side evaluation.  A wave's 64 lanes each
+// integrate a cell; every iteration of the kernel's loop evaluates ONE rhs
+// per lane, at one program point, for all lanes together (lanes in any
+// phase of the method: a Newton residual, a difference-quotient Jacobian
+// column, the initial-step estimate), then advances each lane's machine
+// to its next rhs request.  A lane whose cell is done takes the next cell
+// from a device work queue (one atomic per wave), so the waves stay full
+// while cells of very different stiffness finish at different times.
+//
+// The per-lane workspace (Nordsieck array, the method's vectors, the
+// saved Jacobian and the factorized Newton matrix) lives in HBM,
+// interleaved across the kernel's lanes (element i of lane s at i S + s:
+// a wave's accesses are coalesced); the method's scalars live in
+// registers.
+//
+// Precision: FP64 throughout, or (mixed) the Jacobian (difference
+// quotients of an FP32 rhs with FP32-sized increments), the Newton matrix
+// I - gamma J, its LU factorization and the triangular solves in FP32,
+// while the state, the Nordsieck history, the Newton residual (an FP64
+// rhs), the WRMS norms and the local error test stay FP64.  A Jacobian
+// whose FP32 evaluation fails (a non-finite FP32 rhs) is re-evaluated in
+// FP64.
 #pragma once
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "kx_cell.h"
 #include "kx_plain.h"
+#include "kx_problem.h"
 #include "kx_util.h"
 
+// the waves per SIMD the persistent kernel is compiled for (its register
+// budget: 2 -> 256 VGPRs + AGPRs per lane)
+#ifndef KX_PERSIST_WAVES_PER_EU
+#define KX_PERSIST_WAVES_PER_EU 2
+#endif
+
 namespace kx {
+
+// program counters of a lane's machine
+enum Pc : int {
+  P_FETCH = 0, P_SOLVE_BEGIN, P_FIRST_RHS, P_HIN_BEGIN, P_HIN_REQ, P_HIN_RET,
+  P_HIN_END, P_AFTER_HIN, P_LOOP_TOP, P_STEP_BEGIN, P_ATTEMPT, P_NLS_BEGIN,
+  P_NLS_OUTER, P_RESID_REQ, P_RESID_RET, P_LSETUP_BEGIN, P_DQ_BEGIN,
+  P_DQ_BASE_RET, P_DQ_REQ, P_DQ_RET, P_LSETUP_FORM, P_INNER_ITER,
+  P_INNER_EXIT, P_NLS_EXIT, P_HANDLE, P_ERRTEST_RHS, P_STEP_DONE,
+  P_FAIL_END, P_SOLVE_END, P_CELL_DONE, P_IDLE
+};
+// vectors of a lane's workspace (index into the interleaved arrays)
+enum Vec : int {
+  V_ZN0 = 0, V_EWT = 6, V_Y, V_ACOR, V_TEMPV, V_FTEMP, V_DELTA, V_VT1,
+  V_COUNT
+};
+
+template <int N, class MT>
+struct PersistWork {
+  double* v;     // V_COUNT * N * S doubles
+  MT* J;         // N * N * S
+  MT* A;         // N * N * S
+  int* piv;      // N * S
+  double* st;    // LANE_SLOTS * S: the lanes' scalars (LaneState)
+  long S;
+  KX_HD double& at(int vec, int i, long s) const {
+    return v[((long)vec * N + i) * S + s];
+  }
+  KX_HD MT& Jat(int k, long s) const { return J[(long)k * S + s]; }
+  KX_HD MT& Aat(int k, long s) const { return A[(long)k * S + s]; }
+};
+
+// the scalars of a lane's machine (CVodeMem, the loop variables of the
+// suspended functions, and the cell bookkeeping)
+struct LaneState {
+  // CVodeMem
+  double tn, h, hin, hprime, hscale, next_h, eta, etamax, etaq, etaqm1,
+      etaqp1, saved_tq5;
+  double tau[7], tq[6], l[6];
+  double rl1, gamma, gammap, gamrat, crate, delp, delnrm, acnrm;
+  long q, qprime, qwait, L, convfail;
+  long nscon, nstlp, nstlj;
+  long acnrmcur, jcur, nls_jcur;
+  // loop variables of cvStep / cvNls / cvHin / the DQ Jacobian
+  double saved_t, dsm, tol, cg, ysaved, inc, minInc, tret;
+  double hin_hg, hin_hs, hin_hnew, hin_hub, hin_hlb;
+  long ncf, nef, nflag, niters, nconvfails, curiter, first_resid, callSetup,
+      jbad, jcol, jr, count1, count2, nls_r, dq32;
+  long nstloc;
+  // counters (this CVode call; the cell's totals)
+  long nst, nfe, nfeDQ, nje, nsetups, nni, nnf, ncfn, netf;
+  long tot[9];
+  long restarts, flag;
+  // the cell
+  long cell;
+  double tout;
+};
+// every member is one 8-byte slot: member k of lane s at slot k S + s
+constexpr int LANE_SLOTS = sizeof(LaneState) / 8;
+static_assert(sizeof(LaneState) % 8 == 0, "8-byte slots");
+#define KX_SLOT(f) ((long)(offsetof(LaneState, f) / 8))
+// a member of this lane (st), an element of an array member (sta)
+#define ST(f) (*reinterpret_cast<decltype(LaneState::f)*>( \
+    &w.st[KX_SLOT(f) * w.S + s]))
+#define STA(f, j) (w.st[(KX_SLOT(f) + (j)) * w.S + s])
+
+template <class Net, bool ET, bool MIXED>
+struct Persist {
+  using Prob = Problem<Net, ET>;
+  static constexpr int NS = Net::NSPEC;
+  static constexpr int N = Prob::N;
+  using MT = typename std::conditional<MIXED, float, double>::type;
+  using W = PersistWork<N, MT>;
+
+  // ---- vector kernels over one lane's slot ------------------------------
+  static KX_HD void axpy(const W& w, long s, double a, int x, int y) {
+    if (a == 1.0) {
+      for (int i = 0; i < N; i++) w.at(y, i, s) += w.at(x, i, s);
+    } else if (a == -1.0) {
+      for (int i = 0; i < N; i++) w.at(y, i, s) -= w.at(x, i, s);
+    } else {
+      for (int i = 0; i < N; i++) w.at(y, i, s) += a * w.at(x, i, s);
+    }
+  }
+  // z = a x + b y with N_VLinearSum's special cases (bdf_cell.hpp:308-374)
+  static KX_HD void lsum(const W& w, long s, double a, int x, double b, int y,
+                         int z) {
+    if (b == 1.0 && z == y) return axpy(w, s, a, x, z);
+    if (a == 1.0 && z == x) return axpy(w, s, b, y, z);
+    double xv[N], yv[N], zv[N];
+    for (int i = 0; i < N; i++) {
+      xv[i] = w.at(x, i, s);
+      yv[i] = w.at(y, i, s);
+    }
+    if (a == 1.0 && b == 1.0) {
+      for (int i = 0; i < N; i++) zv[i] = xv[i] + yv[i];
+    } else if ((a == 1.0 && b == -1.0) || (a == -1.0 && b == 1.0)) {
+      const bool t = a == 1.0;
+      for (int i = 0; i < N; i++) zv[i] = t ? xv[i] - yv[i] : yv[i] - xv[i];
+    } else if (a == 1.0 || b == 1.0) {
+      const bool t = a == 1.0;
+      const double c = t ? b : a;
+      for (int i = 0; i < N; i++)
+        zv[i] = t ? (c * yv[i]) + xv[i] : (c * xv[i]) + yv[i];
+    } else if (a == -1.0 || b == -1.0) {
+      const bool t = a == -1.0;
+      const double c = t ? b : a;
+      for (int i = 0; i < N; i++)
+        zv[i] = t ? (c * yv[i]) - xv[i] : (c * xv[i]) - yv[i];
+    } else if (a == b) {
+      for (int i = 0; i < N; i++) zv[i] = a * (xv[i] + yv[i]);
+    } else if (a == -b) {
+      for (int i = 0; i < N; i++) zv[i] = a * (xv[i] - yv[i]);
+    } else {
+      for (int i = 0; i < N; i++) zv[i] = (a * xv[i]) + (b * yv[i]);
+    }
+    for (int i = 0; i < N; i++) w.at(z, i, s) = zv[i];
+  }
+  static KX_HD void scal(const W& w, long s, double c, int x, int z) {
+    for (int i = 0; i < N; i++) w.at(z, i, s) = c * w.at(x, i, s);
+  }
+  static KX_HD void zero(const W& w, long s, int z) {
+    for (int i = 0; i < N; i++) w.at(z, i, s) = 0.0;
+  }
+  static KX_HD double wrms(const W& w, long s, int x) {
+    double sum = 0.0;
+    for (int i = 0; i < N; i++) {
+      const double p = w.at(x, i, s) * w.at(V_EWT, i, s);
+      sum += p * p;
+    }
+    return bdfc::rt(sum / N);
+  }
+  // cvEwtSetSV into vector dst (bdf_cell.hpp:689-713)
+  static KX_HD double atol(const SolveOpts& o, int i) {
+    return i < NS ? o.atol_n : o.atol_T;
+  }
+  static KX_HD int ewt_set(const W& w, long s, const SolveOpts& o,
+                           bool atolmin0, int src, int dst) {
+    double v[N];
+    for (int i = 0; i < N; i++) v[i] = ::fabs(w.at(src, i, s));
+    for (int i = 0; i < N; i++) v[i] = (o.rtol * v[i]) + atol(o, i);
+    if (atolmin0) {
+      double m = v[0];
+      for (int i = 1; i < N; i++) m = v[i] < m ? v[i] : m;
+      if (m <= 0.0) return -1;
+    }
+    for (int i = 0; i < N; i++) w.at(dst, i, s) = 1.0 / v[i];
+    return 0;
+  }
+
+  // ---- the BDF bookkeeping (bdf_cell.hpp:825-954, 1386-1488) ------------
+  static KX_HD void rescale(const W& w, long s) {
+    double c = ST(eta);
+#pragma unroll
+    for (int j = 1; j <= QMAX; j++)
+      if (j <= ST(q)) {
+        scal(w, s, c, j, j);
+        c = ST(eta) * c;
+      }
+    ST(h) = ST(hscale) * ST(eta);
+    ST(next_h) = ST(h);
+    ST(hscale) = ST(h);
+    ST(nscon) = 0;
+  }
+  static KX_HD void predict(const W& w, long s) {
+    ST(tn) += ST(h);
+    for (int k = 1; k <= QMAX; k++)
+      for (int j = QMAX; j >= k; j--)
+        if (k <= ST(q) && j <= ST(q)) axpy(w, s, 1.0, j, j - 1);
+  }
+  static KX_HD void restore(const W& w, long s) {
+    ST(tn) = ST(saved_t);
+    for (int k = 1; k <= QMAX; k++)
+      for (int j = QMAX; j >= k; j--)
+        if (k <= ST(q) && j <= ST(q)) axpy(w, s, -1.0, j, j - 1);
+  }
+  static KX_HD void order_up(const W& w, long s) {
+#pragma unroll
+    for (int i = 0; i <= QMAX; i++) STA(l, i) = 0.0;
+    double a1 = 1.0, prod = 1.0, xiold = 1.0, a0 = -1.0;
+    STA(l, 2) = 1.0;
+    double hsum = ST(hscale);
+#pragma unroll
+    for (int j = 1; j < QMAX; j++)
+      if (j < ST(q)) {
+        hsum += STA(tau, j + 1);
+        const double xi = hsum / ST(hscale);
+        prod *= xi;
+        a0 -= 1.0 / (j + 1);
+        a1 += 1.0 / xi;
+#pragma unroll
+        for (int i = QMAX; i >= 2; i--)
+          if (i <= j + 2) STA(l, i) = STA(l, i) * xiold + STA(l, i - 1);
+        xiold = xi;
+      }
+    const double A1 = (-a0 - a1) / prod;
+    scal(w, s, A1, QMAX, ST(L));
+#pragma unroll
+    for (int j = 2; j <= QMAX; j++)
+      if (j <= ST(q)) axpy(w, s, STA(l, j), ST(L), j);
+  }
+  static KX_HD void order_down(const W& w, long s) {
+#pragma unroll
+    for (int i = 0; i <= QMAX; i++) STA(l, i) = 0.0;
+    STA(l, 2) = 1.0;
+    double hsum = 0.0;
+#pragma unroll
+    for (int j = 1; j <= QMAX - 2; j++)
+      if (j <= ST(q) - 2) {
+        hsum += STA(tau, j);
+        const double xi = hsum / ST(hscale);
+#pragma unroll
+        for (int i = QMAX; i >= 2; i--)
+          if (i <= j + 2) STA(l, i) = STA(l, i) * xi + STA(l, i - 1);
+      }
+#pragma unroll
+    for (int j = 2; j < QMAX; j++)
+      if (j < ST(q)) axpy(w, s, -STA(l, j), ST(q), j);
+  }
+  static KX_HD void order_change(const W& w, long s, int dq) {
+    if (ST(q) == 2 && dq != 1) return;
+    if (dq == 1) order_up(w, s);
+    else if (dq == -1) order_down(w, s);
+  }
+  static KX_HD void adjust(const W& w, long s) {
+    if (ST(qprime) != ST(q)) {
+      order_change(w, s, ST(qprime) - ST(q));
+      ST(q) = ST(qprime);
+      ST(L) = ST(q) + 1;
+      ST(qwait) = ST(L);
+    }
+    rescale(w, s);
+  }
+  static KX_HD void set_coeffs(const W& w, long s) {
+    const int q = ST(q);
+    double xi_inv = 1.0, xistar_inv = 1.0, a0 = -1.0, a0h = -1.0;
+    STA(l, 0) = STA(l, 1) = 1.0;
+#pragma unroll
+    for (int i = 2; i <= QMAX; i++)
+      if (i <= q) STA(l, i) = 0.0;
+    double hsum = ST(h);
+    if (q > 1) {
+#pragma unroll
+      for (int j = 2; j < QMAX; j++)
+        if (j < q) {
+          hsum += STA(tau, j - 1);
+          xi_inv = ST(h) / hsum;
+          a0 -= 1.0 / j;
+#pragma unroll
+          for (int i = QMAX; i >= 1; i--)
+            if (i <= j) STA(l, i) += STA(l, i - 1) * xi_inv;
+        }
+      a0 -= 1.0 / q;
+      xistar_inv = -STA(l, 1) - a0;
+      double tq1 = 0.0;
+#pragma unroll
+      for (int j = 1; j <= QMAX; j++)
+        if (j == q - 1) tq1 = STA(tau, j);
+      hsum += tq1;
+      xi_inv = ST(h) / hsum;
+      a0h = -STA(l, 1) - xi_inv;
+#pragma unroll
+      for (int i = QMAX; i >= 1; i--)
+        if (i <= q) STA(l, i) += STA(l, i - 1) * xistar_inv;
+    }
+    double lq = 0.0;
+#pragma unroll
+    for (int i = 1; i <= QMAX; i++)
+      if (i == q) lq = STA(l, i);
+    const double A1 = 1.0 - a0h + a0;
+    const double A2 = 1.0 + q * A1;
+    STA(tq, 2) = ::fabs(A1 / (a0 * A2));
+    STA(tq, 5) = ::fabs(A2 * xistar_inv / (lq * xi_inv));
+    if (ST(qwait) == 1) {
+      if (q > 1) {
+        const double C = xistar_inv / lq;
+        const double A3 = a0 + 1.0 / q;
+        const double A4 = a0h + xi_inv;
+        const double Cpinv = (1.0 - A4 + A3) / A3;
+        STA(tq, 1) = ::fabs(C * Cpinv);
+      } else {
+        STA(tq, 1) = 1.0;
+      }
+      double tqq = 0.0;
+#pragma unroll
+      for (int j = 1; j <= QMAX; j++)
+        if (j == q) tqq = STA(tau, j);
+      hsum += tqq;
+      xi_inv = ST(h) / hsum;
+      const double A5 = a0 - (1.0 / (q + 1));
+      const double A6 = a0h - xi_inv;
+      const double Cppinv = (1.0 - A6 + A5) / A2;
+      STA(tq, 3) = ::fabs(Cppinv / (xi_inv * (q + 2) * A5));
+    }
+    STA(tq, 4) = bdfc::CORTES / STA(tq, 2);
+    ST(rl1) = 1.0 / STA(l, 1);
+    ST(gamma) = ST(h) * ST(rl1);
+    if (ST(nst) == 0) ST(gammap) = ST(gamma);
+    ST(gamrat) = ST(nst) > 0 ? ST(gamma) / ST(gammap) : 1.0;
+  }
+  static KX_HD void complete(const W& w, long s) {
+    ST(nst)++;
+    ST(nscon)++;
+#pragma unroll
+    for (int i = QMAX; i >= 2; i--)
+      if (i <= ST(q)) STA(tau, i) = STA(tau, i - 1);
+    if (ST(q) == 1 && ST(nst) > 1) STA(tau, 2) = STA(tau, 1);
+    STA(tau, 1) = ST(h);
+#pragma unroll
+    for (int j = 0; j <= QMAX; j++)
+      if (j <= ST(q)) axpy(w, s, STA(l, j), V_ACOR, j);
+    ST(qwait)--;
+    if (ST(qwait) == 1 && ST(q) != QMAX) {
+      for (int i = 0; i < N; i++) w.at(QMAX, i, s) = w.at(V_ACOR, i, s);
+      ST(saved_tq5) = STA(tq, 5);
+    }
+  }
+  static KX_HD void set_eta(const W& w, long s) {
+    using namespace bdfc;
+    if (ST(eta) > ETA_MIN_FX && ST(eta) < ETA_MAX_FX) {
+      ST(eta) = 1.0;
+      ST(hprime) = ST(h);
+      return;
+    }
+    if (ST(eta) >= ETA_MAX_FX) ST(eta) = mn(ST(eta), ST(etamax));
+    else ST(eta) = mx(ST(eta), ETA_MIN);
+    ST(hprime) = ST(h) * ST(eta);
+    if (ST(qprime) < ST(q)) ST(nscon) = 0;
+  }
+  static KX_HD void next_step(const W& w, long s) {
+    using namespace bdfc;
+    if (ST(etamax) == 1.0) {
+      ST(qwait) = ST(qwait) > 2 ? ST(qwait) : 2;
+      ST(qprime) = ST(q);
+      ST(hprime) = ST(h);
+      ST(eta) = 1.0;
+      return;
+    }
+    ST(etaq) = 1.0 / (::pow(BIAS2 * ST(dsm), 1.0 / ST(L)) + ADDON);
+    if (ST(qwait) != 0) {
+      ST(eta) = ST(etaq);
+      ST(qprime) = ST(q);
+      set_eta(w, s);
+      return;
+    }
+    ST(qwait) = 2;
+    ST(etaqm1) = 0.0;
+    if (ST(q) > 1) {
+      const double ddn = wrms(w, s, ST(q)) * STA(tq, 1);
+      ST(etaqm1) = 1.0 / (::pow(BIAS1 * ddn, 1.0 / ST(q)) + ADDON);
+    }
+    ST(etaqp1) = 0.0;
+    if (ST(q) != QMAX && ST(saved_tq5) != 0.0) {
+      double p = 1.0;
+      const double b = ST(h) / STA(tau, 2);
+#pragma unroll
+      for (int i = 1; i <= LMAX; i++)
+        if (i <= ST(L)) p *= b;
+      const double cquot = (STA(tq, 5) / ST(saved_tq5)) * p;
+      lsum(w, s, -cquot, QMAX, 1.0, V_ACOR, V_TEMPV);
+      const double dup = wrms(w, s, V_TEMPV) * STA(tq, 3);
+      ST(etaqp1) = 1.0 / (::pow(BIAS3 * dup, 1.0 / (ST(L) + 1)) + ADDON);
+    }
+    const double etam = mx(ST(etaqm1), mx(ST(etaq), ST(etaqp1)));
+    if (etam > ETA_MIN_FX && etam < ETA_MAX_FX) {
+      ST(eta) = 1.0;
+      ST(qprime) = ST(q);
+    } else if (etam == ST(etaq)) {
+      ST(eta) = ST(etaq);
+      ST(qprime) = ST(q);
+    } else if (etam == ST(etaqm1)) {
+      ST(eta) = ST(etaqm1);
+      ST(qprime) = ST(q) - 1;
+    } else {
+      ST(eta) = ST(etaqp1);
+      ST(qprime) = ST(q) + 1;
+      for (int i = 0; i < N; i++) w.at(QMAX, i, s) = w.at(V_ACOR, i, s);
+    }
+    set_eta(w, s);
+  }
+
+  // ---- the Newton matrix -------------------------------------------------
+  // A = I - gamma J and its LU with partial pivoting (bdf_cell.hpp:
+  // 1099-1108, 402-433), in the matrix precision
+  static KX_HD int form_and_factor(const W& w, long s, double gamma) {
+    const MT c = MT(-gamma);
+    for (int j = 0; j < N; j++) {
+      for (int i = 0; i < N; i++)
+        w.Aat(i + j * N, s) = w.Jat(i + j * N, s) * c;
+      w.Aat(j + j * N, s) += MT(1);
+    }
+    for (int k = 0; k < N; k++) {
+      int l_ = k;
+      MT best = kx::fabs(w.Aat(k + k * N, s));
+      for (int i = k + 1; i < N; i++) {
+        const MT v = kx::fabs(w.Aat(i + k * N, s));
+        if (v > best) {
+          best = v;
+          l_ = i;
+        }
+      }
+      w.piv[(long)k * w.S + s] = l_;
+      if (w.Aat(l_ + k * N, s) == MT(0)) return k + 1;
+      if (l_ != k)
+        for (int j = 0; j < N; j++) {
+          const MT t = w.Aat(l_ + j * N, s);
+          w.Aat(l_ + j * N, s) = w.Aat(k + j * N, s);
+          w.Aat(k + j * N, s) = t;
+        }
+      const MT r = MT(1) / w.Aat(k + k * N, s);
+      for (int i = k + 1; i < N; i++) w.Aat(i + k * N, s) *= r;
+      for (int j = k + 1; j < N; j++) {
+        const MT akj = w.Aat(k + j * N, s);
+        if (akj != MT(0))
+          for (int i = k + 1; i < N; i++)
+            w.Aat(i + j * N, s) -= akj * w.Aat(i + k * N, s);
+      }
+    }
+    return 0;
+  }
+  // delta := A^-1 delta, times 2 / (1 + gamrat) when gamma changed since
+  // the setup (bdf_cell.hpp:435-457, 1134-1153).  Mixed: the right-hand
+  // side scaled by its largest magnitude into FP32, solved, scaled back.
+  static KX_HD void lsolve(const W& w, long s, double gamrat) {
+    MT b[N];
+    double sc = 1.0;
+    if (MIXED) {
+      double m = 0.0;
+      for (int i = 0; i < N; i++) m = ::fmax(m, ::fabs(w.at(V_DELTA, i, s)));
+      sc = m > 0.0 ? m : 1.0;
+    }
+    for (int i = 0; i < N; i++) b[i] = MT(w.at(V_DELTA, i, s) / sc);
+    for (int k = 0; k < N; k++) {
+      const int pk = w.piv[(long)k * w.S + s];
+      if (pk != k) {
+        MT u = MT(0);
+        for (int i = 0; i < N; i++) u = i == pk ? b[i] : u;
+        const MT t = b[k];
+        for (int i = 0; i < N; i++) b[i] = i == pk ? t : b[i];
+        b[k] = u;
+      }
+    }
+    for (int k = 0; k < N - 1; k++)
+      for (int i = k + 1; i < N; i++) b[i] -= w.Aat(i + k * N, s) * b[k];
+    for (int k = N - 1; k > 0; k--) {
+      b[k] /= w.Aat(k + k * N, s);
+      for (int i = 0; i < k; i++) b[i] -= w.Aat(i + k * N, s) * b[k];
+    }
+    b[0] /= w.Aat(0, s);
+    if (gamrat != 1.0) {
+      const double c = 2.0 / (1.0 + gamrat);
+      for (int i = 0; i < N; i++) w.at(V_DELTA, i, s) = c * (sc * double(b[i]));
+    } else {
+      for (int i = 0; i < N; i++) w.at(V_DELTA, i, s) = sc * double(b[i]);
+    }
+  }
+
+  // the charge invariant of the corrections (bdf_cell.hpp:1267-1287)
+  // (the charges c over the species; the electrons' row r)
+  static KX_HD double inv_dot(const W& w, long s) {
+    double sum = 0.0;
+    for (int i = 0; i < NS; i++)
+      if (Net::charge(i) != 0) sum += Net::charge(i) * w.at(V_DELTA, i, s);
+    return sum;
+  }
+  static KX_HD void inv_set(const W& w, long s, double target) {
+    constexpr int r = Net::I_E;
+    double sum = 0.0, sa = 0.0, yr = 0.0, cr = 0.0;
+    for (int i = 0; i < NS; i++) {
+      if (i != r && Net::charge(i) != 0) {
+        sum += Net::charge(i) * w.at(V_DELTA, i, s);
+        sa += ::fabs(Net::charge(i) * (w.at(0, i, s) + w.at(V_ACOR, i, s)));
+      }
+      if (i == r) {
+        yr = w.at(0, i, s) + w.at(V_ACOR, i, s);
+        cr = Net::charge(i);
+      }
+    }
+    const double xr = (target - sum) / cr;
+    if (::fabs(cr * yr) >= 1e-6 * sa && yr + xr >= 0.0) w.at(V_DELTA, r, s) = xr;
+  }
+  // cvNlsConvTest (bdf_cell.hpp:1035-1048)
+  static KX_HD int conv_test(const W& w, long s, int m) {
+    using namespace bdfc;
+    ST(delnrm) = wrms(w, s, V_DELTA);
+    if (m > 0) ST(crate) = mx(CRDOWN * ST(crate), ST(delnrm) / ST(delp));
+    const double dcon = ST(delnrm) * mn(1.0, ST(crate)) / ST(tol);
+    if (dcon <= 1.0) {
+      ST(acnrm) = m == 0 ? ST(delnrm) : wrms(w, s, V_ACOR);
+      ST(acnrmcur) = true;
+      return BDF_OK;
+    }
+    if (m >= 1 && ST(delnrm) > RDIV * ST(delp)) return NLS_CONV_RECVR;
+    ST(delp) = ST(delnrm);
+    return NLS_CONTINUE;
+  }
+  // CVodeGetDky(t, 0) into V_Y (bdf_cell.hpp:1492-1500)
+  static KX_HD void dky(const W& w, long s, double t) {
+    const double sv = (t - ST(tn)) / ST(h);
+    double out[N];
+    for (int i = 0; i < N; i++) out[i] = 0.0;
+#pragma unroll
+    for (int j = QMAX; j >= 0; j--)
+      if (j <= ST(q)) {
+        double c = 1.0;
+#pragma unroll
+        for (int i = 0; i < QMAX; i++)
+          if (i < j) c *= sv;
+        if (j == ST(q))
+          for (int i = 0; i < N; i++) out[i] = c * w.at(j, i, s);
+        else
+          for (int i = 0; i < N; i++) out[i] += c * w.at(j, i, s);
+      }
+    for (int i = 0; i < N; i++) w.at(V_Y, i, s) = out[i];
+  }
+  static KX_HD void reinit(const W& w, long s, double t0,
+                           double h_init) {
+    ST(tn) = t0;
+    ST(q) = 1;
+    ST(L) = 2;
+    ST(qwait) = ST(L);
+    ST(etamax) = bdfc::ETA_MAX_FS;
+    ST(nst) = ST(nfe) = ST(nfeDQ) = ST(nje) = ST(nsetups) = ST(nni) = ST(nnf) =
+        ST(ncfn) = ST(netf) = 0;
+    ST(nstlp) = ST(nscon) = ST(nstlj) = 0;
+    ST(next_h) = 0.0;
+    ST(hin) = h_init;
+    ST(acnrmcur) = ST(nls_jcur) = ST(jcur) = false;
+    ST(saved_tq5) = 0.0;
+#pragma unroll
+    for (int i = 0; i <= LMAX; i++) STA(tau, i) = 0.0;
+    ST(eta) = 1.0;
+    (void)w;
+    (void)s;
+  }
+  static KX_HD void add_stats(const W& w, long s) {
+    STA(tot, 0) += ST(nst); STA(tot, 1) += ST(nfe); STA(tot, 2) += ST(nje);
+    STA(tot, 3) += ST(nfeDQ); STA(tot, 4) += ST(nsetups); STA(tot, 5) += ST(nni);
+    STA(tot, 6) += ST(nnf); STA(tot, 7) += ST(netf);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// the kernel
+
+template <class Net, bool ET, bool MIXED>
+__global__ void __launch_bounds__(256, KX_PERSIST_WAVES_PER_EU)
+persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
+                                               typename std::conditional<
+                                                   MIXED, float,
+                                                   double>::type> w0,
+                     int* queue) {
+  using M = Persist<Net, ET, MIXED>;
+  using Prob = Problem<Net, ET>;
+  constexpr int N = M::N;
+  constexpr int NS = Net::NSPEC;
+  using namespace bdfc;
+  const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+
+  const SolveOpts& o = a.o;
+  const bool dq_jac = o.jac == 2 || (o.jac == 0 && ET);
+  // the analytic Jacobian here is the mass-action block at fixed Tgas
+  const bool supported = dq_jac || !(ET || Net::RATES_DENSITY);
+  const bool inv = dq_jac && Net::HAS_E;  // the charge invariant
+  const bool atolmin0 =
+      (ET ? (o.atol_T < o.atol_n ? o.atol_T : o.atol_n) : o.atol_n) == 0.0;
+  const long mxstep = o.max_steps == 0 ? 500 : o.max_steps;
+
+
+  int pc = P_FETCH;
+  Prob P;
+  P.hit_T0 = false;
+  // the rhs request of this lane: source vector, destination vector,
+  // precision (0: FP64, 1: FP32), and its return code
+  int rq_src = 0, rq_dst = 0, rq_prec = 0, rr = 0;
+  bool want = false;
+
+  for (;;) {
+    // ---- advance: run the machine to its next rhs request --------------
+    want = false;
+    while (!want && pc != P_IDLE) {
+      // the workspace's pointers and stride made opaque at every step of
+      // the machine: the addresses of the unrolled vector elements are then
+      // formed where they are used instead of being held across the loop
+      auto w = w0;
+      asm volatile("" : "+s"(w.v), "+s"(w.J), "+s"(w.A), "+s"(w.st),
+                   "+s"(w.piv), "+s"(w.S));
+      switch (pc) {
+        case P_FETCH: {
+          // one atomic per wave for the lanes that need a cell
+          const unsigned long long need = __ballot(1);
+          const int rank = __popcll(need & ((1ull << lane) - 1ull));
+          const int leader = __ffsll((long long)need) - 1;
+          int base = 0;
+          if (lane == leader) base = atomicAdd(queue, __popcll(need));
+          base = __shfl(base, leader);
+          const int c = base + rank;
+          if (c >= a.ncell) {
+            pc = P_IDLE;
+            break;
+          }
+          ST(cell) = c;
+          for (int i = 0; i < 9; ++i) STA(tot, i) = 0;
+          ST(restarts) = 0;
+          if (!(a.dt[c] > 0.0) || !supported) {
+            a.flags[c] = supported ? 0 : FLAG_UNSUPPORTED;
+            if (a.stats)
+              for (int i = 0; i < 9; ++i) a.stats[(long)c * 9 + i] = 0;
+            pc = P_FETCH;
+            break;
+          }
+          P.p = Net::params_from_row(a.par + (long)c * a.par_stride);
+          P.Tfix = a.T[c];
+          P.hit_T0 = false;
+          for (int i = 0; i < NS; ++i)
+            w.at(0, i, s) = a.n[c * a.n_cs + i * a.n_ss];
+          if (ET) w.at(0, N - 1, s) = a.T[c];
+          ST(tout) = a.dt[c];
+          M::reinit(w, s, 0.0, 0.0);
+          pc = P_SOLVE_BEGIN;
+          break;
+        }
+        case P_SOLVE_BEGIN: {
+          ST(tret) = ST(tn);
+          const double tdist = ::fabs(ST(tout) - ST(tn));
+          const double tround = DBL_EPSILON * mx(::fabs(ST(tn)), ::fabs(ST(tout)));
+          if (tdist == 0.0 || tdist < 2.0 * tround) {
+            ST(flag) = TOO_CLOSE;
+            pc = P_SOLVE_END;
+            break;
+          }
+          if (M::ewt_set(w, s, o, atolmin0, 0, V_EWT) != 0) {
+            ST(flag) = ILL_INPUT;
+            pc = P_SOLVE_END;
+            break;
+          }
+          rq_src = 0; rq_dst = 1; rq_prec = 0; want = true;
+          pc = P_FIRST_RHS;
+          break;
+        }
+        case P_FIRST_RHS: {
+          ST(nfe)++;
+          if (rr != 0) {
+            ST(flag) = rr < 0 ? RHSFUNC_FAIL : FIRST_RHSFUNC_ERR;
+            pc = P_SOLVE_END;
+            break;
+          }
+          ST(h) = ST(hin);
+          if (ST(h) != 0.0 && (ST(tout) - ST(tn)) * ST(h) < 0.0) {
+            ST(flag) = ILL_INPUT;
+            pc = P_SOLVE_END;
+            break;
+          }
+          pc = ST(h) == 0.0 ? P_HIN_BEGIN : P_AFTER_HIN;
+          break;
+        }
+        case P_HIN_BEGIN: {  // cvHin (bdf_cell.hpp:717-764)
+          const double tdist = ::fabs(ST(tout) - ST(tn));
+          const double tround = DBL_EPSILON * mx(::fabs(ST(tn)), ::fabs(ST(tout)));
+          ST(hin_hlb) = HLB_FACTOR * tround;
+          // cvUpperBoundH0 with temp1 = V_TEMPV, temp2 = V_ACOR
+          for (int i = 0; i < N; i++) w.at(V_ACOR, i, s) = ::fabs(w.at(0, i, s));
+          M::ewt_set(w, s, o, atolmin0, 0, V_TEMPV);
+          for (int i = 0; i < N; i++)
+            w.at(V_TEMPV, i, s) = 1.0 / w.at(V_TEMPV, i, s);
+          M::lsum(w, s, HUB_FACTOR, V_ACOR, 1.0, V_TEMPV, V_TEMPV);
+          for (int i = 0; i < N; i++) w.at(V_ACOR, i, s) = ::fabs(w.at(1, i, s));
+          double inv = 0.0;
+          for (int i = 0; i < N; i++) {
+            const double v = w.at(V_ACOR, i, s) / w.at(V_TEMPV, i, s);
+            w.at(V_TEMPV, i, s) = v;
+            if (::fabs(v) > inv) inv = ::fabs(v);
+          }
+          double hub = HUB_FACTOR * tdist;
+          if (hub * inv > 1.0) hub = 1.0 / inv;
+          ST(hin_hub) = hub;
+          ST(hin_hg) = rt(ST(hin_hlb) * hub);
+          if (hub < ST(hin_hlb)) {
+            ST(h) = ST(hin_hg);
+            pc = P_AFTER_HIN;
+            break;
+          }
+          ST(hin_hs) = ST(hin_hg);
+          ST(hin_hnew) = ST(hin_hg);
+          ST(count1) = 1;
+          ST(count2) = 1;
+          pc = P_HIN_REQ;
+          break;
+        }
+        case P_HIN_REQ: {
+          M::lsum(w, s, ST(hin_hg), 1, 1.0, 0, V_Y);
+          rq_src = V_Y; rq_dst = V_TEMPV; rq_prec = 0; want = true;
+          pc = P_HIN_RET;
+          break;
+        }
+        case P_HIN_RET: {
+          ST(nfe)++;
+          if (rr < 0) {
+            ST(flag) = RHSFUNC_FAIL;
+            pc = P_SOLVE_END;
+            break;
+          }
+          if (rr > 0) {
+            ST(hin_hg) *= 0.2;
+            ST(count2)++;
+            if (ST(count2) <= MAX_ITERS) {
+              pc = P_HIN_REQ;
+              break;
+            }
+            if (ST(count1) <= 2) {
+              ST(flag) = REPTD_RHSFUNC_ERR;
+              pc = P_SOLVE_END;
+              break;
+            }
+            ST(hin_hnew) = ST(hin_hs);
+            pc = P_HIN_END;
+            break;
+          }
+          const double hg = ST(hin_hg);
+          M::lsum(w, s, 1.0 / hg, V_TEMPV, -1.0 / hg, 1, V_TEMPV);
+          const double ydd = M::wrms(w, s, V_TEMPV);
+          const double hub = ST(hin_hub);
+          ST(hin_hs) = hg;
+          ST(hin_hnew) = (ydd * hub * hub > 2.0) ? rt(2.0 / ydd) : rt(hg * hub);
+          if (ST(count1) == MAX_ITERS) {
+            pc = P_HIN_END;
+            break;
+          }
+          const double hrat = ST(hin_hnew) / hg;
+          if (hrat > 0.5 && hrat < 2.0) {
+            pc = P_HIN_END;
+            break;
+          }
+          if (ST(count1) > 1 && hrat > 2.0) {
+            ST(hin_hnew) = hg;
+            pc = P_HIN_END;
+            break;
+          }
+          ST(hin_hg) = ST(hin_hnew);
+          ST(count1)++;
+          ST(count2) = 1;
+          pc = P_HIN_REQ;
+          break;
+        }
+        case P_HIN_END: {
+          double h0 = H_BIAS * ST(hin_hnew);
+          if (h0 < ST(hin_hlb)) h0 = ST(hin_hlb);
+          if (h0 > ST(hin_hub)) h0 = ST(hin_hub);
+          ST(h) = h0;
+          pc = P_AFTER_HIN;
+          break;
+        }
+        case P_AFTER_HIN: {
+          ST(hscale) = ST(h);
+          ST(hprime) = ST(h);
+          M::scal(w, s, ST(h), 1, 1);
+          ST(nstloc) = 0;
+          pc = P_LOOP_TOP;
+          break;
+        }
+        case P_LOOP_TOP: {
+          ST(next_h) = ST(h);
+          if (ST(nst) > 0 && M::ewt_set(w, s, o, atolmin0, 0, V_EWT) != 0) {
+            ST(flag) = ILL_INPUT;
+            pc = P_FAIL_END;
+            break;
+          }
+          if (mxstep > 0 && ST(nstloc) >= mxstep) {
+            ST(flag) = TOO_MUCH_WORK;
+            pc = P_FAIL_END;
+            break;
+          }
+          if (DBL_EPSILON * M::wrms(w, s, 0) > 1.0) {
+            ST(flag) = TOO_MUCH_ACC;
+            pc = P_FAIL_END;
+            break;
+          }
+          pc = P_STEP_BEGIN;
+          break;
+        }
+        case P_STEP_BEGIN: {
+          ST(ncf) = 0;
+          ST(nef) = 0;
+          ST(dsm) = 0.0;
+          if (ST(nst) > 0 && ST(hprime) != ST(h)) M::adjust(w, s);
+          ST(saved_t) = ST(tn);
+          ST(nflag) = FIRST_CALL;
+          pc = P_ATTEMPT;
+          break;
+        }
+        case P_ATTEMPT: {
+          M::predict(w, s);
+          M::set_coeffs(w, s);
+          pc = P_NLS_BEGIN;
+          break;
+        }
+        case P_NLS_BEGIN: {
+          const int nf = ST(nflag);
+          ST(convfail) = (nf == FIRST_CALL || nf == PREV_ERR_FAIL) ? NO_FAILURES
+                                                                  : FAIL_OTHER;
+          ST(callSetup) = nf == PREV_CONV_FAIL || nf == PREV_ERR_FAIL ||
+                         ST(nst) == 0 || ST(nst) >= ST(nstlp) + MSBP ||
+                         ::fabs(ST(gamrat) - 1.0) > DGMAX_LSETUP;
+          M::zero(w, s, V_ACOR);
+          ST(tol) = STA(tq, 4);
+          ST(jbad) = 0;
+          ST(niters) = 0;
+          ST(nconvfails) = 0;
+          pc = P_NLS_OUTER;
+          break;
+        }
+        case P_NLS_OUTER: {
+          ST(curiter) = 0;
+          ST(first_resid) = 1;
+          pc = P_RESID_REQ;
+          break;
+        }
+        case P_RESID_REQ: {
+          M::lsum(w, s, 1.0, 0, 1.0, V_ACOR, V_Y);
+          rq_src = V_Y; rq_dst = V_FTEMP; rq_prec = 0; want = true;
+          pc = P_RESID_RET;
+          break;
+        }
+        case P_RESID_RET: {
+          ST(nfe)++;
+          const int r = rr < 0 ? RHSFUNC_FAIL : (rr > 0 ? RHSFUNC_RECVR : BDF_OK);
+          if (r == BDF_OK) {
+            M::lsum(w, s, ST(rl1), 1, 1.0, V_ACOR, V_DELTA);
+            M::lsum(w, s, -ST(gamma), V_FTEMP, 1.0, V_DELTA, V_DELTA);
+          }
+          if (ST(first_resid)) {
+            if (r != BDF_OK) {
+              ST(nls_r) = r;
+              pc = P_NLS_EXIT;
+            } else {
+              pc = ST(callSetup) ? P_LSETUP_BEGIN : P_INNER_ITER;
+            }
+          } else {
+            if (r != BDF_OK) {
+              ST(nls_r) = r;
+              pc = P_INNER_EXIT;
+            } else {
+              pc = P_INNER_ITER;
+            }
+          }
+          break;
+        }
+        case P_LSETUP_BEGIN: {  // cvNlsLSetup + cvLsSetup
+          if (ST(jbad)) ST(convfail) = FAIL_BAD_J;
+          const double dg = ::fabs((ST(gamma) / ST(gammap)) - 1.0);
+          const bool jbad_ls = ST(nst) == 0 || ST(nst) >= ST(nstlj) + MSBJ ||
+                               (ST(convfail) == FAIL_BAD_J && dg < DGMAX_JBAD) ||
+                               ST(convfail) == FAIL_OTHER;
+          ST(jr) = 0;
+          if (!jbad_ls) {
+            ST(jcur) = false;
+            pc = P_LSETUP_FORM;
+            break;
+          }
+          ST(jcur) = true;
+          if (!dq_jac) {
+            // the analytic Jacobian at y (fixed Tgas)
+            double yr[N], Jr[N * N];
+            for (int i = 0; i < N; i++) yr[i] = w.at(V_Y, i, s);
+            for (int k = 0; k < N * N; k++) Jr[k] = 0.0;
+            ST(jr) = P.jac(yr, Jr);
+            for (int k = 0; k < N * N; k++) w.Jat(k, s) = typename M::MT(Jr[k]);
+            pc = P_LSETUP_FORM;
+            break;
+          }
+          pc = P_DQ_BEGIN;
+          break;
+        }
+        case P_DQ_BEGIN: {  // cvLsDenseDQJac (bdf_cell.hpp:1290-1311)
+          const double fn = M::wrms(w, s, V_FTEMP);
+          ST(dq32) = MIXED ? 1 : 0;
+          const double eps = ST(dq32) ? (double)FLT_EPSILON : DBL_EPSILON;
+          ST(minInc) = fn != 0.0 ? (MIN_INC_MULT * ::fabs(ST(h)) * eps *
+                                   (double)N * fn)
+                                : 1.0;
+          ST(jcol) = 0;
+          if (ST(dq32)) {
+            // the FP32 rhs at y, the base of the FP32 differences
+            rq_src = V_Y; rq_dst = V_TEMPV; rq_prec = 1; want = true;
+            pc = P_DQ_BASE_RET;
+          } else {
+            pc = P_DQ_REQ;
+          }
+          break;
+        }
+        case P_DQ_BASE_RET: {
+          if (rr != 0) {  // the FP32 rhs failed: this Jacobian in FP64
+            ST(dq32) = 0;
+            const double fn = M::wrms(w, s, V_FTEMP);
+            ST(minInc) = fn != 0.0 ? (MIN_INC_MULT * ::fabs(ST(h)) * DBL_EPSILON
+                                     * (double)N * fn)
+                                  : 1.0;
+          }
+          pc = P_DQ_REQ;
+          break;
+        }
+        case P_DQ_REQ: {
+          const int j = ST(jcol);
+          const double srur = ST(dq32) ? rt((double)FLT_EPSILON) : rt(DBL_EPSILON);
+          double ys = 0.0, ew = 0.0;
+          for (int i = 0; i < N; i++)
+            if (i == j) {
+              ys = w.at(V_Y, i, s);
+              ew = w.at(V_EWT, i, s);
+            }
+          ST(ysaved) = ys;
+          ST(inc) = mx(srur * ::fabs(ys), ST(minInc) / ew);
+          for (int i = 0; i < N; i++)
+            if (i == j) w.at(V_Y, i, s) += ST(inc);
+          rq_src = V_Y; rq_dst = V_VT1; rq_prec = ST(dq32); want = true;
+          pc = P_DQ_RET;
+          break;
+        }
+        case P_DQ_RET: {
+          ST(nfeDQ)++;
+          const int j = ST(jcol);
+          if (rr != 0) {
+            if (ST(dq32)) {  // redo the whole Jacobian in FP64
+              for (int i = 0; i < N; i++)
+                if (i == j) w.at(V_Y, i, s) = ST(ysaved);
+              ST(dq32) = 0;
+              const double fn = M::wrms(w, s, V_FTEMP);
+              ST(minInc) = fn != 0.0 ? (MIN_INC_MULT * ::fabs(ST(h)) *
+                                       DBL_EPSILON * (double)N * fn)
+                                    : 1.0;
+              ST(jcol) = 0;
+              pc = P_DQ_REQ;
+              break;
+            }
+            ST(jr) = rr;
+            pc = P_LSETUP_FORM;
+            break;
+          }
+          for (int i = 0; i < N; i++)
+            if (i == j) w.at(V_Y, i, s) = ST(ysaved);
+          const double ii = 1.0 / ST(inc);
+          if (ST(dq32)) {
+            const float iif = (float)ii;
+            for (int i = 0; i < N; i++)
+              w.Jat(i + j * N, s) = typename M::MT(
+                  ((float)w.at(V_VT1, i, s) - (float)w.at(V_TEMPV, i, s)) * iif);
+          } else {
+            // lin_sum(ii, vt1, -ii, ftemp): the a == -b form
+            for (int i = 0; i < N; i++)
+              w.Jat(i + j * N, s) = typename M::MT(
+                  ii * (w.at(V_VT1, i, s) - w.at(V_FTEMP, i, s)));
+          }
+          ST(jcol)++;
+          if (ST(jcol) < N) {
+            pc = P_DQ_REQ;
+          } else {
+            ST(jr) = 0;
+            pc = P_LSETUP_FORM;
+          }
+          break;
+        }
+        case P_LSETUP_FORM: {
+          int ret = ST(jr) < 0 ? -1 : (ST(jr) > 0 ? 1 : 0);
+          if (ST(jcur)) {
+            ST(nje)++;
+            ST(nstlj) = ST(nst);
+          }
+          if (ret == 0) ret = M::form_and_factor(w, s, ST(gamma)) > 0 ? 1 : 0;
+          ST(nsetups)++;
+          ST(nls_jcur) = ST(jcur);
+          ST(gamrat) = 1.0;
+          ST(gammap) = ST(gamma);
+          ST(crate) = 1.0;
+          ST(delnrm) = 0.0;
+          ST(nstlp) = ST(nst);
+          if (ret != 0) {
+            ST(nls_r) = ret < 0 ? LSETUP_FAIL : NLS_CONV_RECVR;
+            pc = P_NLS_EXIT;
+          } else {
+            pc = P_INNER_ITER;
+          }
+          break;
+        }
+        case P_INNER_ITER: {
+          ST(niters)++;
+          ST(cg) = inv ? M::inv_dot(w, s) : 0.0;
+          M::scal(w, s, -1.0, V_DELTA, V_DELTA);
+          M::lsolve(w, s, ST(gamrat));
+          if (inv) M::inv_set(w, s, -ST(cg));
+          M::lsum(w, s, 1.0, V_ACOR, 1.0, V_DELTA, V_ACOR);
+          const int r = M::conv_test(w, s, ST(curiter));
+          ST(curiter)++;
+          if (r == BDF_OK) {
+            ST(nls_jcur) = false;
+            ST(nls_r) = BDF_OK;
+            pc = P_NLS_EXIT;
+          } else if (r != NLS_CONTINUE) {
+            ST(nls_r) = r;
+            pc = P_INNER_EXIT;
+          } else if (ST(curiter) >= NLS_MAXCOR) {
+            ST(nls_r) = NLS_CONV_RECVR;
+            pc = P_INNER_EXIT;
+          } else {
+            ST(first_resid) = 0;
+            pc = P_RESID_REQ;
+          }
+          break;
+        }
+        case P_INNER_EXIT: {
+          if (ST(nls_r) > 0 && !ST(nls_jcur)) {
+            ST(nconvfails)++;
+            ST(callSetup) = 1;
+            ST(jbad) = 1;
+            M::zero(w, s, V_ACOR);
+            pc = P_NLS_OUTER;
+          } else {
+            pc = P_NLS_EXIT;
+          }
+          break;
+        }
+        case P_NLS_EXIT: {
+          const int r = ST(nls_r);
+          if (r != BDF_OK) ST(nconvfails)++;
+          ST(nni) += ST(niters);
+          ST(nnf) += ST(nconvfails);
+          if (r != BDF_OK) {
+            ST(nflag) = r;
+          } else {
+            M::lsum(w, s, 1.0, 0, 1.0, V_ACOR, V_Y);
+            if (!ST(acnrmcur)) ST(acnrm) = M::wrms(w, s, V_ACOR);
+            ST(jcur) = false;
+            ST(nflag) = BDF_OK;
+          }
+          pc = P_HANDLE;
+          break;
+        }
+        case P_HANDLE: {  // cvHandleNFlag, cvDoErrorTest
+          const int f = ST(nflag);
+          if (f != BDF_OK) {
+            ST(ncfn)++;
+            M::restore(w, s);
+            if (f < 0) {
+              ST(flag) = (f == LSETUP_FAIL || f == LSOLVE_FAIL ||
+                         f == RHSFUNC_FAIL) ? f : NLS_FAIL;
+              pc = P_FAIL_END;
+              break;
+            }
+            ST(ncf)++;
+            ST(etamax) = 1.0;
+            if (::fabs(ST(h)) <= 0.0 || ST(ncf) == o.max_conv_fails) {
+              if (f == NLS_CONV_RECVR) {
+                ST(flag) = CONV_FAILURE;
+                pc = P_FAIL_END;
+                break;
+              }
+              if (f == RHSFUNC_RECVR) {
+                ST(flag) = REPTD_RHSFUNC_ERR;
+                pc = P_FAIL_END;
+                break;
+              }
+            }
+            ST(eta) = mx(ETA_CF, 0.0);
+            ST(nflag) = PREV_CONV_FAIL;
+            M::rescale(w, s);
+            pc = P_ATTEMPT;
+            break;
+          }
+          const double dsm = ST(acnrm) * STA(tq, 2);
+          ST(dsm) = dsm;
+          if (dsm <= 1.0) {
+            pc = P_STEP_DONE;
+            break;
+          }
+          ST(nef)++;
+          ST(netf)++;
+          ST(nflag) = PREV_ERR_FAIL;
+          M::restore(w, s);
+          if (::fabs(ST(h)) <= 0.0 || ST(nef) == o.max_err_fails) {
+            ST(flag) = ERR_FAILURE;
+            pc = P_FAIL_END;
+            break;
+          }
+          ST(etamax) = 1.0;
+          if (ST(nef) <= MXNEF1) {
+            ST(eta) = 1.0 / (::pow(BIAS2 * dsm, 1.0 / ST(L)) + ADDON);
+            ST(eta) = mx(ETA_MIN_EF, mx(ST(eta), 0.0));
+            if (ST(nef) >= SMALL_NEF) ST(eta) = mn(ST(eta), ETA_MAX_EF);
+            M::rescale(w, s);
+            pc = P_ATTEMPT;
+            break;
+          }
+          if (ST(q) > 1) {
+            ST(eta) = mx(ETA_MIN_EF, 0.0);
+            M::order_change(w, s, -1);
+            ST(L) = ST(q);
+            ST(q)--;
+            ST(qwait) = ST(L);
+            M::rescale(w, s);
+            pc = P_ATTEMPT;
+            break;
+          }
+          ST(eta) = mx(ETA_MIN_EF, 0.0);
+          ST(h) *= ST(eta);
+          ST(next_h) = ST(h);
+          ST(hscale) = ST(h);
+          ST(qwait) = LONG_WAIT;
+          ST(nscon) = 0;
+          rq_src = 0; rq_dst = V_TEMPV; rq_prec = 0; want = true;
+          pc = P_ERRTEST_RHS;
+          break;
+        }
+        case P_ERRTEST_RHS: {
+          ST(nfe)++;
+          if (rr != 0) {
+            ST(flag) = rr < 0 ? RHSFUNC_FAIL : UNREC_RHSFUNC_ERR;
+            pc = P_FAIL_END;
+            break;
+          }
+          M::scal(w, s, ST(h), V_TEMPV, 1);
+          pc = P_ATTEMPT;
+          break;
+        }
+        case P_STEP_DONE: {
+          M::complete(w, s);
+          M::next_step(w, s);
+          ST(etamax) = ST(nst) <= SMALL_NST ? ETA_MAX_ES : ETA_MAX_GS;
+          M::scal(w, s, STA(tq, 2), V_ACOR, V_ACOR);
+          ST(nstloc)++;
+          if ((ST(tn) - ST(tout)) * ST(h) >= 0.0) {
+            ST(tret) = ST(tout);
+            M::dky(w, s, ST(tout));
+            ST(next_h) = ST(hprime);
+            ST(flag) = BDF_OK;
+            pc = P_SOLVE_END;
+          } else {
+            pc = P_LOOP_TOP;
+          }
+          break;
+        }
+        case P_FAIL_END: {
+          ST(tret) = ST(tn);
+          for (int i = 0; i < N; i++) w.at(V_Y, i, s) = w.at(0, i, s);
+          pc = P_SOLVE_END;
+          break;
+        }
+        case P_SOLVE_END: {  // bdf::integrate's restarts
+          M::add_stats(w, s);
+          const int f = ST(flag);
+          if (f >= 0) {
+            ST(flag) = 0;
+            pc = P_CELL_DONE;
+            break;
+          }
+          const bool again = f == ERR_FAILURE || f == CONV_FAILURE ||
+                             f == REPTD_RHSFUNC_ERR;
+          if (f == TOO_CLOSE || f == ILL_INPUT || f == RHSFUNC_FAIL ||
+              f == FIRST_RHSFUNC_ERR || !again ||
+              ST(restarts) >= o.max_restarts) {
+            pc = P_CELL_DONE;
+            break;
+          }
+          ST(restarts)++;
+          const double hc = ST(next_h);
+          // the reached state (zn[0], = y on a failure) is the new start
+          if (ST(tret) != ST(tn))
+            for (int i = 0; i < N; i++) w.at(0, i, s) = w.at(V_Y, i, s);
+          M::reinit(w, s, ST(tret), hc > 0.0 ? 0.1 * hc : 0.0);
+          pc = P_SOLVE_BEGIN;
+          break;
+        }
+        case P_CELL_DONE: {
+          const int c = ST(cell);
+          int flag = ST(flag);
+          double y[N];
+          for (int i = 0; i < N; i++) y[i] = w.at(V_Y, i, s);
+          if (flag == 0 && ET &&
+              !(y[N - 1] > 0.0 && y[N - 1] - y[N - 1] == 0.0))
+            flag = FLAG_NONPOSITIVE_T;
+          if (P.hit_T0 && (flag == REPTD_RHSFUNC_ERR || flag == RHSFUNC_FAIL ||
+                           flag == UNREC_RHSFUNC_ERR ||
+                           flag == FIRST_RHSFUNC_ERR))
+            flag = FLAG_NONPOSITIVE_T;
+          a.flags[c] = flag;
+          if (a.stats) {
+            long* sc = a.stats + (long)c * 9;
+            for (int i = 0; i < 8; ++i) sc[i] = STA(tot, i);
+            sc[8] = ST(restarts);
+          }
+          if (flag == 0) cell_finish<Net, ET>(a, c, y);
+          pc = P_FETCH;
+          break;
+        }
+        default:
+          pc = P_IDLE;
+          break;
+      }
+    }
+    // ---- the rhs of every lane that asked, at one program point ---------
+    const unsigned long long busy = __ballot(pc != P_IDLE);
+    if (busy == 0ull) break;
+    if (want) {
+      auto w = w0;
+      asm volatile("" : "+s"(w.v), "+s"(w.S));
+      double yr[N];
+      for (int i = 0; i < N; i++) yr[i] = w.at(rq_src, i, s);
+      double fr[N];
+      if (MIXED && rq_prec == 1) {
+        float y32[N], f32[N];
+        for (int i = 0; i < N; i++) y32[i] = (float)yr[i];
+        rr = P.template rhs_r<float>(y32, f32);
+        for (int i = 0; i < N; i++) fr[i] = (double)f32[i];
+      } else {
+        rr = P.template rhs_r<double>(yr, fr);
+      }
+      for (int i = 0; i < N; i++) w.at(rq_dst, i, s) = fr[i];
+    }
+  }
+}
+
+// bytes of the persistent kernel's workspace per lane
+template <int N, bool MIXED>
+constexpr long persist_lane_bytes() {
+  return (long)sizeof(double) * (V_COUNT * N + LANE_SLOTS) +
+         2L * N * N * (MIXED ? sizeof(float) : sizeof(double)) +
+         (long)sizeof(int) * N;
+}
+
+// the persistent kernel over resident arrays: workgroups of 256 lanes,
+// waves_per_cu waves on each of the device's CUs (0: automatic), the
+// workspace allocated for the resident lanes only
+template <class Net, bool ET, bool MIXED>
+int launch_persistent_t(CellsArgs a, int wpc, hipStream_t s) {
+  constexpr int N = Problem<Net, ET>::N;
+  using MT = typename std::conditional<MIXED, float, double>::type;
+  int dev = 0, ncu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return -1;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                            dev) != hipSuccess)
+    return -1;
+  if (wpc <= 0) wpc = 8;
+  long lanes = (long)ncu * wpc * 64;
+  // no more lanes than cells (rounded up to whole workgroups)
+  const long need = ((long)a.ncell + 255) / 256 * 256;
+  if (lanes > need) lanes = need;
+  lanes = (lanes + 255) / 256 * 256;
+  static thread_local void* buf = nullptr;
+  static thread_local size_t bytes = 0;
+  const size_t want = persist_lane_bytes<N, MIXED>() * lanes + 256;
+  if (want > bytes) {
+    if (buf) (void)hipFree(buf);
+    buf = nullptr;
+    bytes = 0;
+    if (hipMalloc(&buf, want) != hipSuccess) return FLAG_DEVICE_MEMORY;
+    bytes = want;
+  }
+  char* p = static_cast<char*>(buf);
+  int* queue = reinterpret_cast<int*>(p);
+  p += 256;
+  PersistWork<N, MT> w;
+  w.S = lanes;
+  w.v = reinterpret_cast<double*>(p);
+  p += sizeof(double) * V_COUNT * N * lanes;
+  w.J = reinterpret_cast<MT*>(p);
+  p += sizeof(MT) * N * N * lanes;
+  w.A = reinterpret_cast<MT*>(p);
+  p += sizeof(MT) * N * N * lanes;
+  w.st = reinterpret_cast<double*>(p);
+  p += sizeof(double) * LANE_SLOTS * lanes;
+  w.piv = reinterpret_cast<int*>(p);
+  hipLaunchKernelGGL(fill_flags_kernel, dim3(1), dim3(64), 0, s, 1, queue, 0);
+  hipLaunchKernelGGL((persist_cells_kernel<Net, ET, MIXED>),
+                     dim3(lanes / 256), dim3(256), 0, s, a, w, queue);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
 
 template <class Net, bool ET>
 inline int launch_persistent(CellsArgs a, int mixed, int chunk, int wpc,
                              hipStream_t s) {
-  (void)a; (void)mixed; (void)chunk; (void)wpc; (void)s;
-  return FLAG_UNSUPPORTED;
+  (void)chunk;
+  return mixed ? launch_persistent_t<Net, ET, true>(a, wpc, s)
+               : launch_persistent_t<Net, ET, false>(a, wpc, s);
 }
 
 }  // namespace kx

@@ -16,6 +16,7 @@
 //                       (codegen.py:3745-3779)
 #pragma once
 #include <cmath>
+#include <type_traits>
 
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
@@ -134,7 +135,13 @@ KX_HD inline R cooling_H2(R T, R nH, R nHj, R nH2, R ne, R nHe) {
                 + R(4.0406627) * l3_ - R(4.7274036) * l4 - R(8.8077017) * l5
                 + R(8.9167183) * l6 + R(6.4380698) * l7
                 - R(6.3701156) * l8) * ne;
-  cool += f * w24;
+  if constexpr (std::is_same<R, float>::value) {
+    // FP32: the capped fit (up to 1e30 below 100 K) times n_e overflows
+    // where the window is 0; weight first (0 stays 0)
+    cool += w24 == R(0) ? R(0) : f * w24;
+  } else {
+    cool += f * w24;
+  }
   // H2-He
   if (T > R(1e1) && T <= R(1e4))
     f = kx::pow(R(10), R(-23.689237) + R(2.1892372) * l3 - R(0.81520438) * l2
@@ -184,10 +191,19 @@ KX_HD inline R cooling_continuum(R T, R rho_in, R ntot) {
     kgas = kx::pow(R(10), R(1.000042) * kx::log10(kx::fmin(rho_in, R(0.5)))
                               + R(2.14989));
   const R rho = kx::fmax(rho_in, R(1e-40));
-  const R mu = rho / kx::fmax(ntot, R(1e-40)) / R(phys::p_mass);
-  const R lj = kx::sqrt(R(phys::pi) * R(phys::kB) * T / rho / R(phys::p_mass)
-                        / R(phys::gravity) / mu);
-  const R tau = lj * kgas * rho + R(1e-40);
+  R tau;
+  if constexpr (std::is_same<R, float>::value) {
+    // FP32: lj^2 = pi kB T ntot / (G rho^2) in factors that stay in range
+    // (the FP64 form's intermediate pi kB T / rho / m_p / G reaches 1e41)
+    const R lj = kx::sqrt(R(phys::pi * phys::kB / phys::gravity) * T)
+                 * kx::sqrt(ntot) / rho;
+    tau = lj * kgas * rho + R(1e-30);
+  } else {
+    const R mu = rho / kx::fmax(ntot, R(1e-40)) / R(phys::p_mass);
+    const R lj = kx::sqrt(R(phys::pi) * R(phys::kB) * T / rho
+                          / R(phys::p_mass) / R(phys::gravity) / mu);
+    tau = lj * kgas * rho + R(1e-40);
+  }
   const R beta = kx::fmin(R(1), R(1) / (tau * tau));
   return R(4) * R(phys::stefboltz) * T * T * T * T * kgas * rho * beta;
 }
@@ -201,6 +217,20 @@ KX_HD inline R h2_heat_fraction(R T, R nH, R nH2, R dd) {
                 / (R(1.6) * kx::exp(-(a * a)) * nH / dd
                    + R(1.4) * kx::exp(R(-1.2e4) / (T + R(1.2e3))) * nH2 / dd);
   return R(1) / (R(1) + ncr / dd);
+}
+
+// vibrational heat capacity C_vib / k = x^2 e^x / (e^x - 1)^2 of a
+// diatomic, x = theta_vib / T; FP32 in the equal form x^2 e^-x /
+// (1 - e^-x)^2 (e^x overflows FP32 below T ~ theta / 88)
+template <class R>
+KX_HD inline R vib_heat_capacity(R x) {
+  if constexpr (std::is_same<R, float>::value) {
+    const R e = kx::exp(-x);
+    return x * x * e / ((R(1) - e) * (R(1) - e));
+  } else {
+    const R e = kx::exp(x);
+    return x * x * e / ((e - R(1)) * (e - R(1)));
+  }
 }
 
 // rotational heat capacity C_rot / k of a diatomic with rotational

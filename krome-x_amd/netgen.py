@@ -808,10 +808,9 @@ def _emit_thermal(w, net, th, pnames):
         for s in dis:
             vp = "R(0)"
             if s.name in GAMMA_VIB_K:
-                w("    const R x_%s = R(%r) * iTv, e_%s = kx::exp(x_%s);"
-                  % (s.cid, GAMMA_VIB_K[s.name], s.cid, s.cid))
-                vp = ("R(2) * x_%s * x_%s * e_%s / ((e_%s - R(1)) * "
-                      "(e_%s - R(1)))" % ((s.cid,) * 5))
+                w("    const R x_%s = R(%r) * iTv;"
+                  % (s.cid, GAMMA_VIB_K[s.name]))
+                vp = "R(2) * kx::vib_heat_capacity(x_%s)" % s.cid
             rp = "R(2)"
             if s.name == "H2":
                 rp = ("R(2) * kx::gamma_rot_ortho_para(Tth, R(p.opratio) / "

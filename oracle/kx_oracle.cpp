@@ -19,6 +19,9 @@
 #include <cstdint>
 #include <cstring>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "net_primordial.h"
 #include "net_primordial3.h"
@@ -234,6 +237,21 @@ void funcs(int ncell, const double* n, const double* T, const double* par,
   }
 }
 
+template <class Net>
+void jacs(int ncell, const double* n, const double* T, const double* par,
+          double* J) {
+  constexpr int NS = Net::NSPEC;
+  for (int c = 0; c < ncell; ++c) {
+    Problem<Net> P;
+    P.p = Net::params_from_row(par + static_cast<size_t>(c) * Net::NPAR);
+    P.Tfix = T[c];
+    double* Jc = J + static_cast<size_t>(c) * NS * NS;
+    for (int k = 0; k < NS * NS; ++k) Jc[k] = 0.0;
+    P.jac(0.0, n + static_cast<size_t>(c) * NS, nullptr, Jc, nullptr,
+          nullptr, nullptr);
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -243,6 +261,22 @@ extern "C" {
 #else
 #define KXO_(name) kxo_##name
 #endif
+
+// the OpenMP threads of the batch loop (the CPU baseline's core count)
+void KXO_(set_threads)(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+int KXO_(get_threads)(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
 
 // a network of krome-x_amd/csrc/generated by name: 0 primordial,
 // 1 primordial3, -1 unknown
@@ -277,6 +311,21 @@ int KXO_(funcs)(int net, int ncell, const double* n, const double* T,
       return 0;
     case 1:
       funcs<kx::net_primordial3>(ncell, n, T, par, evolve_T, k, f);
+      return 0;
+  }
+  return -1;
+}
+
+// the fixed-Tgas analytic species Jacobian J[ncell][NSPEC^2] (column-
+// major per cell) at states n, T
+int KXO_(jac)(int net, int ncell, const double* n, const double* T,
+              const double* par, double* J) {
+  switch (net) {
+    case 0:
+      jacs<kx::net_primordial>(ncell, n, T, par, J);
+      return 0;
+    case 1:
+      jacs<kx::net_primordial3>(ncell, n, T, par, J);
       return 0;
   }
   return -1;

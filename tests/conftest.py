@@ -1,0 +1,106 @@
+import ctypes
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+REPO = Path(__file__).resolve().parent.parent
+PKG = REPO / "krome-x_amd"
+GOLDEN = REPO / "tests" / "golden"
+sys.path.insert(0, str(PKG))
+
+# the options row of the oracle's C entry points (oracle/kx_oracle.cpp)
+OPT_ORDER = ("evolve_T", "rtol", "atol_n", "atol_T", "linsol", "jac",
+             "max_steps", "max_err_fails", "max_conv_fails", "max_restarts",
+             "clip_negative", "conserve")
+NET_ID = {"primordial": 0, "primordial3": 1}
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers",
+                            "gpu: needs an AMD GPU (the HIP kernels)")
+
+
+def oracle_opts(**kw):
+    d = dict(evolve_T=0, rtol=1e-8, atol_n=1e-20, atol_T=1e-6, linsol=1,
+             jac=0, max_steps=100000, max_err_fails=7, max_conv_fails=10,
+             max_restarts=5, clip_negative=1, conserve=0)
+    d.update(kw)
+    return np.array([float(d[k]) for k in OPT_ORDER])
+
+
+class Oracle:
+    """ctypes front of oracle/_build/libkx_oracle.so (or the reference-
+    integrator build oracle/_ref/libkx_ref.so)"""
+
+    def __init__(self, path, prefix):
+        self.lib = ctypes.CDLL(str(path))
+        self.p = prefix
+
+    def solve(self, net, n, T, dt, par, opts):
+        D = ctypes.POINTER(ctypes.c_double)
+        n = np.array(n, dtype=np.float64, order="C", copy=True)
+        T = np.array(T, dtype=np.float64, order="C", copy=True)
+        dt = np.ascontiguousarray(dt, dtype=np.float64)
+        par = np.ascontiguousarray(par, dtype=np.float64)
+        nc = n.shape[0]
+        flags = np.zeros(nc, dtype=np.intc)
+        st = np.zeros((nc, 9), dtype=np.int64)
+        f = getattr(self.lib, self.p + "solve_cells")
+        r = f(NET_ID[net], nc, n.ctypes.data_as(D), T.ctypes.data_as(D),
+              dt.ctypes.data_as(D), par.ctypes.data_as(D),
+              par.shape[1] if par.ndim == 2 else 0,
+              np.ascontiguousarray(opts).ctypes.data_as(D),
+              flags.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+              st.ctypes.data_as(ctypes.POINTER(ctypes.c_long)))
+        return r, n, T, flags, st
+
+    def funcs(self, net, n, T, par, evolve_T):
+        D = ctypes.POINTER(ctypes.c_double)
+        nc, ns = n.shape
+        meta = __import__("cells").network_meta(net)
+        k = np.zeros((nc, meta["nrea"]))
+        f = np.zeros((nc, ns + 1))
+        getattr(self.lib, self.p + "funcs")(
+            NET_ID[net], nc, np.ascontiguousarray(n).ctypes.data_as(D),
+            np.ascontiguousarray(T).ctypes.data_as(D),
+            np.ascontiguousarray(par).ctypes.data_as(D), int(evolve_T),
+            k.ctypes.data_as(D), f.ctypes.data_as(D))
+        return k, f
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    so = REPO / "oracle" / "_build" / "libkx_oracle.so"
+    if not so.exists():
+        subprocess.run(["make", "-s", "-C", str(REPO / "oracle"), "all"],
+                       check=True)
+    return Oracle(so, "kxo_")
+
+
+@pytest.fixture(scope="session")
+def oracle_ref():
+    so = REPO / "oracle" / "_ref" / "libkx_ref.so"
+    if not so.exists():
+        src = Path("/root/reference/runtime/core/cvode/bdf_cell.hpp")
+        if not src.exists():
+            pytest.skip("the reference integrator build is not available")
+        subprocess.run(["make", "-s", "-C", str(REPO / "oracle"), "ref"],
+                       check=True)
+    return Oracle(so, "kxr_")
+
+
+def golden(name):
+    return np.load(GOLDEN / name)
+
+
+def rel_err(a, b, ntot_frac=1e-12):
+    """max relative difference over the species above ntot_frac of each
+    cell's total (smaller ones are below the absolute tolerance)"""
+    b = np.asarray(b)
+    ntot = np.abs(b).sum(axis=-1, keepdims=True)
+    big = np.abs(b) > ntot_frac * ntot
+    rel = np.abs(np.asarray(a) - b) / np.maximum(np.abs(b), 1e-300)
+    return float(rel[big].max()) if big.any() else 0.0
