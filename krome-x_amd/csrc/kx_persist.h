@@ -36,7 +36,16 @@
 // the waves per SIMD the persistent kernel is compiled for (its register
 // budget: 2 -> 256 VGPRs + AGPRs per lane)
 #ifndef KX_PERSIST_WAVES_PER_EU
-#define KX_PERSIST_WAVES_PER_EU 2
+#define KX_PERSIST_WAVES_PER_EU 1
+#endif
+// the persistent kernel's lane slots (the workspace stride): one wave per
+// SIMD on the 256 CUs of an MI355X
+#ifndef KX_PERSIST_LANES
+#define KX_PERSIST_LANES 65536
+#endif
+// the lanes' scalars in registers (1) or in HBM slots (0)
+#ifndef KX_STATE_IN_REGS
+#define KX_STATE_IN_REGS 1
 #endif
 
 namespace kx {
@@ -51,10 +60,23 @@ enum Pc : int {
   P_FAIL_END, P_SOLVE_END, P_CELL_DONE, P_IDLE
 };
 // vectors of a lane's workspace (index into the interleaved arrays)
+// (the first ones, the Nordsieck array, the error weights and the
+// accumulated correction, in LDS when they fit; the others in HBM)
 enum Vec : int {
-  V_ZN0 = 0, V_EWT = 6, V_Y, V_ACOR, V_TEMPV, V_FTEMP, V_DELTA, V_VT1,
+  V_ZN0 = 0, V_EWT = 6, V_ACOR, V_Y, V_TEMPV, V_FTEMP, V_DELTA, V_VT1,
   V_COUNT
 };
+// the LDS of a workgroup: 160 KB on gfx950, KX_PERSIST_BLOCK lanes
+#ifndef KX_PERSIST_BLOCK
+#define KX_PERSIST_BLOCK 256
+#endif
+constexpr long LDS_BYTES = 163840;
+// the vectors of a lane that live in LDS
+template <int N>
+constexpr int lds_vectors() {
+  const long v = LDS_BYTES / (8L * KX_PERSIST_BLOCK * N);
+  return v > 8 ? 8 : (int)v;
+}
 
 template <int N, class MT>
 struct PersistWork {
@@ -63,9 +85,17 @@ struct PersistWork {
   MT* A;         // N * N * S
   int* piv;      // N * S
   double* st;    // LANE_SLOTS * S: the lanes' scalars (LaneState)
-  long S;
+  // the interleave stride: the kernel's lane slots, a compile-time
+  // constant, so that the elements of a lane sit at constant offsets from
+  // one base (the compiler then sees that distinct elements never alias)
+  static constexpr long S = KX_PERSIST_LANES;
+  struct LaneState* sp;  // this lane's scalars when kept in registers
+  double* lds;    // lds_vectors<N>() * N * KX_PERSIST_BLOCK: this block's
+  int tid;        // the lane's slot in it
   KX_HD double& at(int vec, int i, long s) const {
-    return v[((long)vec * N + i) * S + s];
+    constexpr int NL = lds_vectors<N>();
+    if (vec < NL) return lds[((long)vec * N + i) * KX_PERSIST_BLOCK + tid];
+    return v[((long)(vec - NL) * N + i) * S + s];
   }
   KX_HD MT& Jat(int k, long s) const { return J[(long)k * S + s]; }
   KX_HD MT& Aat(int k, long s) const { return A[(long)k * S + s]; }
@@ -101,9 +131,14 @@ constexpr int LANE_SLOTS = sizeof(LaneState) / 8;
 static_assert(sizeof(LaneState) % 8 == 0, "8-byte slots");
 #define KX_SLOT(f) ((long)(offsetof(LaneState, f) / 8))
 // a member of this lane (st), an element of an array member (sta)
+#if KX_STATE_IN_REGS
+#define ST(f) (w.sp->f)
+#define STA(f, j) (w.sp->f[j])
+#else
 #define ST(f) (*reinterpret_cast<decltype(LaneState::f)*>( \
     &w.st[KX_SLOT(f) * w.S + s]))
 #define STA(f, j) (w.st[(KX_SLOT(f) + (j)) * w.S + s])
+#endif
 
 template <class Net, bool ET, bool MIXED>
 struct Persist {
@@ -459,6 +494,7 @@ struct Persist {
     }
     return 0;
   }
+
   // delta := A^-1 delta, times 2 / (1 + gamrat) when gamma changed since
   // the setup (bdf_cell.hpp:435-457, 1134-1153).  Mixed: the right-hand
   // side scaled by its largest magnitude into FP32, solved, scaled back.
@@ -585,7 +621,7 @@ struct Persist {
 // the kernel
 
 template <class Net, bool ET, bool MIXED>
-__global__ void __launch_bounds__(256, KX_PERSIST_WAVES_PER_EU)
+__global__ void __launch_bounds__(KX_PERSIST_BLOCK, KX_PERSIST_WAVES_PER_EU)
 persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
                                                typename std::conditional<
                                                    MIXED, float,
@@ -607,7 +643,8 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
   const bool atolmin0 =
       (ET ? (o.atol_T < o.atol_n ? o.atol_T : o.atol_n) : o.atol_n) == 0.0;
   const long mxstep = o.max_steps == 0 ? 500 : o.max_steps;
-
+  LaneState lane_state;
+  __shared__ double lds_vec[lds_vectors<N>() * N * KX_PERSIST_BLOCK];
 
   int pc = P_FETCH;
   Prob P;
@@ -625,62 +662,27 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
       // the machine: the addresses of the unrolled vector elements are then
       // formed where they are used instead of being held across the loop
       auto w = w0;
-      asm volatile("" : "+s"(w.v), "+s"(w.J), "+s"(w.A), "+s"(w.st),
-                   "+s"(w.piv), "+s"(w.S));
-      switch (pc) {
-        case P_FETCH: {
-          // one atomic per wave for the lanes that need a cell
-          const unsigned long long need = __ballot(1);
-          const int rank = __popcll(need & ((1ull << lane) - 1ull));
-          const int leader = __ffsll((long long)need) - 1;
-          int base = 0;
-          if (lane == leader) base = atomicAdd(queue, __popcll(need));
-          base = __shfl(base, leader);
-          const int c = base + rank;
-          if (c >= a.ncell) {
-            pc = P_IDLE;
-            break;
-          }
-          ST(cell) = c;
-          for (int i = 0; i < 9; ++i) STA(tot, i) = 0;
-          ST(restarts) = 0;
-          if (!(a.dt[c] > 0.0) || !supported) {
-            a.flags[c] = supported ? 0 : FLAG_UNSUPPORTED;
-            if (a.stats)
-              for (int i = 0; i < 9; ++i) a.stats[(long)c * 9 + i] = 0;
-            pc = P_FETCH;
-            break;
-          }
-          P.p = Net::params_from_row(a.par + (long)c * a.par_stride);
-          P.Tfix = a.T[c];
-          P.hit_T0 = false;
-          for (int i = 0; i < NS; ++i)
-            w.at(0, i, s) = a.n[c * a.n_cs + i * a.n_ss];
-          if (ET) w.at(0, N - 1, s) = a.T[c];
-          ST(tout) = a.dt[c];
-          M::reinit(w, s, 0.0, 0.0);
-          pc = P_SOLVE_BEGIN;
-          break;
-        }
-        case P_SOLVE_BEGIN: {
-          ST(tret) = ST(tn);
-          const double tdist = ::fabs(ST(tout) - ST(tn));
-          const double tround = DBL_EPSILON * mx(::fabs(ST(tn)), ::fabs(ST(tout)));
-          if (tdist == 0.0 || tdist < 2.0 * tround) {
-            ST(flag) = TOO_CLOSE;
-            pc = P_SOLVE_END;
-            break;
-          }
-          if (M::ewt_set(w, s, o, atolmin0, 0, V_EWT) != 0) {
-            ST(flag) = ILL_INPUT;
-            pc = P_SOLVE_END;
-            break;
-          }
-          rq_src = 0; rq_dst = 1; rq_prec = 0; want = true;
-          pc = P_FIRST_RHS;
-          break;
-        }
-        case P_FIRST_RHS: {
+      {
+        unsigned long long pv = (unsigned long long)w0.v,
+                           pj = (unsigned long long)w0.J,
+                           pa = (unsigned long long)w0.A,
+                           ps = (unsigned long long)w0.st,
+                           pp = (unsigned long long)w0.piv;
+        asm volatile("" : "+s"(pv), "+s"(pj), "+s"(pa), "+s"(ps), "+s"(pp));
+        w.v = (double*)pv;
+        w.J = (decltype(w.J))pj;
+        w.A = (decltype(w.A))pa;
+        w.st = (double*)ps;
+        w.piv = (int*)pp;
+      }
+      w.sp = &lane_state;
+      w.lds = lds_vec;
+      w.tid = threadIdx.x;
+      // the machine's stages in the order a step flows through them: a
+      // lane runs every stage its transitions reach in one pass (each
+      // stage once per wave and pass, masked to the lanes there); the
+      // rare backward transitions take another pass
+      if (pc == P_FIRST_RHS) do {
           ST(nfe)++;
           if (rr != 0) {
             ST(flag) = rr < 0 ? RHSFUNC_FAIL : FIRST_RHSFUNC_ERR;
@@ -695,47 +697,8 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
           }
           pc = ST(h) == 0.0 ? P_HIN_BEGIN : P_AFTER_HIN;
           break;
-        }
-        case P_HIN_BEGIN: {  // cvHin (bdf_cell.hpp:717-764)
-          const double tdist = ::fabs(ST(tout) - ST(tn));
-          const double tround = DBL_EPSILON * mx(::fabs(ST(tn)), ::fabs(ST(tout)));
-          ST(hin_hlb) = HLB_FACTOR * tround;
-          // cvUpperBoundH0 with temp1 = V_TEMPV, temp2 = V_ACOR
-          for (int i = 0; i < N; i++) w.at(V_ACOR, i, s) = ::fabs(w.at(0, i, s));
-          M::ewt_set(w, s, o, atolmin0, 0, V_TEMPV);
-          for (int i = 0; i < N; i++)
-            w.at(V_TEMPV, i, s) = 1.0 / w.at(V_TEMPV, i, s);
-          M::lsum(w, s, HUB_FACTOR, V_ACOR, 1.0, V_TEMPV, V_TEMPV);
-          for (int i = 0; i < N; i++) w.at(V_ACOR, i, s) = ::fabs(w.at(1, i, s));
-          double inv = 0.0;
-          for (int i = 0; i < N; i++) {
-            const double v = w.at(V_ACOR, i, s) / w.at(V_TEMPV, i, s);
-            w.at(V_TEMPV, i, s) = v;
-            if (::fabs(v) > inv) inv = ::fabs(v);
-          }
-          double hub = HUB_FACTOR * tdist;
-          if (hub * inv > 1.0) hub = 1.0 / inv;
-          ST(hin_hub) = hub;
-          ST(hin_hg) = rt(ST(hin_hlb) * hub);
-          if (hub < ST(hin_hlb)) {
-            ST(h) = ST(hin_hg);
-            pc = P_AFTER_HIN;
-            break;
-          }
-          ST(hin_hs) = ST(hin_hg);
-          ST(hin_hnew) = ST(hin_hg);
-          ST(count1) = 1;
-          ST(count2) = 1;
-          pc = P_HIN_REQ;
-          break;
-        }
-        case P_HIN_REQ: {
-          M::lsum(w, s, ST(hin_hg), 1, 1.0, 0, V_Y);
-          rq_src = V_Y; rq_dst = V_TEMPV; rq_prec = 0; want = true;
-          pc = P_HIN_RET;
-          break;
-        }
-        case P_HIN_RET: {
+      } while (0);
+      if (pc == P_HIN_RET) do {
           ST(nfe)++;
           if (rr < 0) {
             ST(flag) = RHSFUNC_FAIL;
@@ -783,154 +746,19 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
           ST(count2) = 1;
           pc = P_HIN_REQ;
           break;
-        }
-        case P_HIN_END: {
-          double h0 = H_BIAS * ST(hin_hnew);
-          if (h0 < ST(hin_hlb)) h0 = ST(hin_hlb);
-          if (h0 > ST(hin_hub)) h0 = ST(hin_hub);
-          ST(h) = h0;
-          pc = P_AFTER_HIN;
-          break;
-        }
-        case P_AFTER_HIN: {
-          ST(hscale) = ST(h);
-          ST(hprime) = ST(h);
-          M::scal(w, s, ST(h), 1, 1);
-          ST(nstloc) = 0;
-          pc = P_LOOP_TOP;
-          break;
-        }
-        case P_LOOP_TOP: {
-          ST(next_h) = ST(h);
-          if (ST(nst) > 0 && M::ewt_set(w, s, o, atolmin0, 0, V_EWT) != 0) {
-            ST(flag) = ILL_INPUT;
+      } while (0);
+      if (pc == P_ERRTEST_RHS) do {
+          ST(nfe)++;
+          if (rr != 0) {
+            ST(flag) = rr < 0 ? RHSFUNC_FAIL : UNREC_RHSFUNC_ERR;
             pc = P_FAIL_END;
             break;
           }
-          if (mxstep > 0 && ST(nstloc) >= mxstep) {
-            ST(flag) = TOO_MUCH_WORK;
-            pc = P_FAIL_END;
-            break;
-          }
-          if (DBL_EPSILON * M::wrms(w, s, 0) > 1.0) {
-            ST(flag) = TOO_MUCH_ACC;
-            pc = P_FAIL_END;
-            break;
-          }
-          pc = P_STEP_BEGIN;
-          break;
-        }
-        case P_STEP_BEGIN: {
-          ST(ncf) = 0;
-          ST(nef) = 0;
-          ST(dsm) = 0.0;
-          if (ST(nst) > 0 && ST(hprime) != ST(h)) M::adjust(w, s);
-          ST(saved_t) = ST(tn);
-          ST(nflag) = FIRST_CALL;
+          M::scal(w, s, ST(h), V_TEMPV, 1);
           pc = P_ATTEMPT;
           break;
-        }
-        case P_ATTEMPT: {
-          M::predict(w, s);
-          M::set_coeffs(w, s);
-          pc = P_NLS_BEGIN;
-          break;
-        }
-        case P_NLS_BEGIN: {
-          const int nf = ST(nflag);
-          ST(convfail) = (nf == FIRST_CALL || nf == PREV_ERR_FAIL) ? NO_FAILURES
-                                                                  : FAIL_OTHER;
-          ST(callSetup) = nf == PREV_CONV_FAIL || nf == PREV_ERR_FAIL ||
-                         ST(nst) == 0 || ST(nst) >= ST(nstlp) + MSBP ||
-                         ::fabs(ST(gamrat) - 1.0) > DGMAX_LSETUP;
-          M::zero(w, s, V_ACOR);
-          ST(tol) = STA(tq, 4);
-          ST(jbad) = 0;
-          ST(niters) = 0;
-          ST(nconvfails) = 0;
-          pc = P_NLS_OUTER;
-          break;
-        }
-        case P_NLS_OUTER: {
-          ST(curiter) = 0;
-          ST(first_resid) = 1;
-          pc = P_RESID_REQ;
-          break;
-        }
-        case P_RESID_REQ: {
-          M::lsum(w, s, 1.0, 0, 1.0, V_ACOR, V_Y);
-          rq_src = V_Y; rq_dst = V_FTEMP; rq_prec = 0; want = true;
-          pc = P_RESID_RET;
-          break;
-        }
-        case P_RESID_RET: {
-          ST(nfe)++;
-          const int r = rr < 0 ? RHSFUNC_FAIL : (rr > 0 ? RHSFUNC_RECVR : BDF_OK);
-          if (r == BDF_OK) {
-            M::lsum(w, s, ST(rl1), 1, 1.0, V_ACOR, V_DELTA);
-            M::lsum(w, s, -ST(gamma), V_FTEMP, 1.0, V_DELTA, V_DELTA);
-          }
-          if (ST(first_resid)) {
-            if (r != BDF_OK) {
-              ST(nls_r) = r;
-              pc = P_NLS_EXIT;
-            } else {
-              pc = ST(callSetup) ? P_LSETUP_BEGIN : P_INNER_ITER;
-            }
-          } else {
-            if (r != BDF_OK) {
-              ST(nls_r) = r;
-              pc = P_INNER_EXIT;
-            } else {
-              pc = P_INNER_ITER;
-            }
-          }
-          break;
-        }
-        case P_LSETUP_BEGIN: {  // cvNlsLSetup + cvLsSetup
-          if (ST(jbad)) ST(convfail) = FAIL_BAD_J;
-          const double dg = ::fabs((ST(gamma) / ST(gammap)) - 1.0);
-          const bool jbad_ls = ST(nst) == 0 || ST(nst) >= ST(nstlj) + MSBJ ||
-                               (ST(convfail) == FAIL_BAD_J && dg < DGMAX_JBAD) ||
-                               ST(convfail) == FAIL_OTHER;
-          ST(jr) = 0;
-          if (!jbad_ls) {
-            ST(jcur) = false;
-            pc = P_LSETUP_FORM;
-            break;
-          }
-          ST(jcur) = true;
-          if (!dq_jac) {
-            // the analytic Jacobian at y (fixed Tgas)
-            double yr[N], Jr[N * N];
-            for (int i = 0; i < N; i++) yr[i] = w.at(V_Y, i, s);
-            for (int k = 0; k < N * N; k++) Jr[k] = 0.0;
-            ST(jr) = P.jac(yr, Jr);
-            for (int k = 0; k < N * N; k++) w.Jat(k, s) = typename M::MT(Jr[k]);
-            pc = P_LSETUP_FORM;
-            break;
-          }
-          pc = P_DQ_BEGIN;
-          break;
-        }
-        case P_DQ_BEGIN: {  // cvLsDenseDQJac (bdf_cell.hpp:1290-1311)
-          const double fn = M::wrms(w, s, V_FTEMP);
-          ST(dq32) = MIXED ? 1 : 0;
-          const double eps = ST(dq32) ? (double)FLT_EPSILON : DBL_EPSILON;
-          ST(minInc) = fn != 0.0 ? (MIN_INC_MULT * ::fabs(ST(h)) * eps *
-                                   (double)N * fn)
-                                : 1.0;
-          ST(jcol) = 0;
-          if (ST(dq32)) {
-            // the FP32 rhs at y, the base of the FP32 differences
-            rq_src = V_Y; rq_dst = V_TEMPV; rq_prec = 1; want = true;
-            pc = P_DQ_BASE_RET;
-          } else {
-            pc = P_DQ_REQ;
-          }
-          break;
-        }
-        case P_DQ_BASE_RET: {
+      } while (0);
+      if (pc == P_DQ_BASE_RET) do {
           if (rr != 0) {  // the FP32 rhs failed: this Jacobian in FP64
             ST(dq32) = 0;
             const double fn = M::wrms(w, s, V_FTEMP);
@@ -940,25 +768,8 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
           }
           pc = P_DQ_REQ;
           break;
-        }
-        case P_DQ_REQ: {
-          const int j = ST(jcol);
-          const double srur = ST(dq32) ? rt((double)FLT_EPSILON) : rt(DBL_EPSILON);
-          double ys = 0.0, ew = 0.0;
-          for (int i = 0; i < N; i++)
-            if (i == j) {
-              ys = w.at(V_Y, i, s);
-              ew = w.at(V_EWT, i, s);
-            }
-          ST(ysaved) = ys;
-          ST(inc) = mx(srur * ::fabs(ys), ST(minInc) / ew);
-          for (int i = 0; i < N; i++)
-            if (i == j) w.at(V_Y, i, s) += ST(inc);
-          rq_src = V_Y; rq_dst = V_VT1; rq_prec = ST(dq32); want = true;
-          pc = P_DQ_RET;
-          break;
-        }
-        case P_DQ_RET: {
+      } while (0);
+      if (pc == P_DQ_RET) do {
           ST(nfeDQ)++;
           const int j = ST(jcol);
           if (rr != 0) {
@@ -1000,8 +811,147 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
             pc = P_LSETUP_FORM;
           }
           break;
-        }
-        case P_LSETUP_FORM: {
+      } while (0);
+      if (pc == P_RESID_RET) do {
+          ST(nfe)++;
+          const int r = rr < 0 ? RHSFUNC_FAIL : (rr > 0 ? RHSFUNC_RECVR : BDF_OK);
+          if (r == BDF_OK) {
+            M::lsum(w, s, ST(rl1), 1, 1.0, V_ACOR, V_DELTA);
+            M::lsum(w, s, -ST(gamma), V_FTEMP, 1.0, V_DELTA, V_DELTA);
+          }
+          if (ST(first_resid)) {
+            if (r != BDF_OK) {
+              ST(nls_r) = r;
+              pc = P_NLS_EXIT;
+            } else {
+              pc = ST(callSetup) ? P_LSETUP_BEGIN : P_INNER_ITER;
+            }
+          } else {
+            if (r != BDF_OK) {
+              ST(nls_r) = r;
+              pc = P_INNER_EXIT;
+            } else {
+              pc = P_INNER_ITER;
+            }
+          }
+          break;
+      } while (0);
+      if (pc == P_HIN_BEGIN) do {  // cvHin (bdf_cell.hpp:717-764)
+          const double tdist = ::fabs(ST(tout) - ST(tn));
+          const double tround = DBL_EPSILON * mx(::fabs(ST(tn)), ::fabs(ST(tout)));
+          ST(hin_hlb) = HLB_FACTOR * tround;
+          // cvUpperBoundH0 with temp1 = V_TEMPV, temp2 = V_ACOR
+          for (int i = 0; i < N; i++) w.at(V_ACOR, i, s) = ::fabs(w.at(0, i, s));
+          M::ewt_set(w, s, o, atolmin0, 0, V_TEMPV);
+          for (int i = 0; i < N; i++)
+            w.at(V_TEMPV, i, s) = 1.0 / w.at(V_TEMPV, i, s);
+          M::lsum(w, s, HUB_FACTOR, V_ACOR, 1.0, V_TEMPV, V_TEMPV);
+          for (int i = 0; i < N; i++) w.at(V_ACOR, i, s) = ::fabs(w.at(1, i, s));
+          double inv = 0.0;
+          for (int i = 0; i < N; i++) {
+            const double v = w.at(V_ACOR, i, s) / w.at(V_TEMPV, i, s);
+            w.at(V_TEMPV, i, s) = v;
+            if (::fabs(v) > inv) inv = ::fabs(v);
+          }
+          double hub = HUB_FACTOR * tdist;
+          if (hub * inv > 1.0) hub = 1.0 / inv;
+          ST(hin_hub) = hub;
+          ST(hin_hg) = rt(ST(hin_hlb) * hub);
+          if (hub < ST(hin_hlb)) {
+            ST(h) = ST(hin_hg);
+            pc = P_AFTER_HIN;
+            break;
+          }
+          ST(hin_hs) = ST(hin_hg);
+          ST(hin_hnew) = ST(hin_hg);
+          ST(count1) = 1;
+          ST(count2) = 1;
+          pc = P_HIN_REQ;
+          break;
+      } while (0);
+      if (pc == P_HIN_END) do {
+          double h0 = H_BIAS * ST(hin_hnew);
+          if (h0 < ST(hin_hlb)) h0 = ST(hin_hlb);
+          if (h0 > ST(hin_hub)) h0 = ST(hin_hub);
+          ST(h) = h0;
+          pc = P_AFTER_HIN;
+          break;
+      } while (0);
+      if (pc == P_HIN_REQ) do {
+          M::lsum(w, s, ST(hin_hg), 1, 1.0, 0, V_Y);
+          rq_src = V_Y; rq_dst = V_TEMPV; rq_prec = 0; want = true;
+          pc = P_HIN_RET;
+          break;
+      } while (0);
+      if (pc == P_AFTER_HIN) do {
+          ST(hscale) = ST(h);
+          ST(hprime) = ST(h);
+          M::scal(w, s, ST(h), 1, 1);
+          ST(nstloc) = 0;
+          pc = P_LOOP_TOP;
+          break;
+      } while (0);
+      if (pc == P_LSETUP_BEGIN) do {  // cvNlsLSetup + cvLsSetup
+          if (ST(jbad)) ST(convfail) = FAIL_BAD_J;
+          const double dg = ::fabs((ST(gamma) / ST(gammap)) - 1.0);
+          const bool jbad_ls = ST(nst) == 0 || ST(nst) >= ST(nstlj) + MSBJ ||
+                               (ST(convfail) == FAIL_BAD_J && dg < DGMAX_JBAD) ||
+                               ST(convfail) == FAIL_OTHER;
+          ST(jr) = 0;
+          if (!jbad_ls) {
+            ST(jcur) = false;
+            pc = P_LSETUP_FORM;
+            break;
+          }
+          ST(jcur) = true;
+          if (!dq_jac) {
+            // the analytic Jacobian at y (fixed Tgas)
+            double yr[N], Jr[N * N];
+            for (int i = 0; i < N; i++) yr[i] = w.at(V_Y, i, s);
+            for (int k = 0; k < N * N; k++) Jr[k] = 0.0;
+            ST(jr) = P.jac(yr, Jr);
+            for (int k = 0; k < N * N; k++) w.Jat(k, s) = typename M::MT(Jr[k]);
+            pc = P_LSETUP_FORM;
+            break;
+          }
+          pc = P_DQ_BEGIN;
+          break;
+      } while (0);
+      if (pc == P_DQ_BEGIN) do {  // cvLsDenseDQJac (bdf_cell.hpp:1290-1311)
+          const double fn = M::wrms(w, s, V_FTEMP);
+          ST(dq32) = MIXED ? 1 : 0;
+          const double eps = ST(dq32) ? (double)FLT_EPSILON : DBL_EPSILON;
+          ST(minInc) = fn != 0.0 ? (MIN_INC_MULT * ::fabs(ST(h)) * eps *
+                                   (double)N * fn)
+                                : 1.0;
+          ST(jcol) = 0;
+          if (ST(dq32)) {
+            // the FP32 rhs at y, the base of the FP32 differences
+            rq_src = V_Y; rq_dst = V_TEMPV; rq_prec = 1; want = true;
+            pc = P_DQ_BASE_RET;
+          } else {
+            pc = P_DQ_REQ;
+          }
+          break;
+      } while (0);
+      if (pc == P_DQ_REQ) do {
+          const int j = ST(jcol);
+          const double srur = ST(dq32) ? rt((double)FLT_EPSILON) : rt(DBL_EPSILON);
+          double ys = 0.0, ew = 0.0;
+          for (int i = 0; i < N; i++)
+            if (i == j) {
+              ys = w.at(V_Y, i, s);
+              ew = w.at(V_EWT, i, s);
+            }
+          ST(ysaved) = ys;
+          ST(inc) = mx(srur * ::fabs(ys), ST(minInc) / ew);
+          for (int i = 0; i < N; i++)
+            if (i == j) w.at(V_Y, i, s) += ST(inc);
+          rq_src = V_Y; rq_dst = V_VT1; rq_prec = ST(dq32); want = true;
+          pc = P_DQ_RET;
+          break;
+      } while (0);
+      if (pc == P_LSETUP_FORM) do {
           int ret = ST(jr) < 0 ? -1 : (ST(jr) > 0 ? 1 : 0);
           if (ST(jcur)) {
             ST(nje)++;
@@ -1022,8 +972,8 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
             pc = P_INNER_ITER;
           }
           break;
-        }
-        case P_INNER_ITER: {
+      } while (0);
+      if (pc == P_INNER_ITER) do {
           ST(niters)++;
           ST(cg) = inv ? M::inv_dot(w, s) : 0.0;
           M::scal(w, s, -1.0, V_DELTA, V_DELTA);
@@ -1047,8 +997,8 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
             pc = P_RESID_REQ;
           }
           break;
-        }
-        case P_INNER_EXIT: {
+      } while (0);
+      if (pc == P_INNER_EXIT) do {
           if (ST(nls_r) > 0 && !ST(nls_jcur)) {
             ST(nconvfails)++;
             ST(callSetup) = 1;
@@ -1059,8 +1009,8 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
             pc = P_NLS_EXIT;
           }
           break;
-        }
-        case P_NLS_EXIT: {
+      } while (0);
+      if (pc == P_NLS_EXIT) do {
           const int r = ST(nls_r);
           if (r != BDF_OK) ST(nconvfails)++;
           ST(nni) += ST(niters);
@@ -1075,8 +1025,8 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
           }
           pc = P_HANDLE;
           break;
-        }
-        case P_HANDLE: {  // cvHandleNFlag, cvDoErrorTest
+      } while (0);
+      if (pc == P_HANDLE) do {  // cvHandleNFlag, cvDoErrorTest
           const int f = ST(nflag);
           if (f != BDF_OK) {
             ST(ncfn)++;
@@ -1150,19 +1100,8 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
           rq_src = 0; rq_dst = V_TEMPV; rq_prec = 0; want = true;
           pc = P_ERRTEST_RHS;
           break;
-        }
-        case P_ERRTEST_RHS: {
-          ST(nfe)++;
-          if (rr != 0) {
-            ST(flag) = rr < 0 ? RHSFUNC_FAIL : UNREC_RHSFUNC_ERR;
-            pc = P_FAIL_END;
-            break;
-          }
-          M::scal(w, s, ST(h), V_TEMPV, 1);
-          pc = P_ATTEMPT;
-          break;
-        }
-        case P_STEP_DONE: {
+      } while (0);
+      if (pc == P_STEP_DONE) do {
           M::complete(w, s);
           M::next_step(w, s);
           ST(etamax) = ST(nst) <= SMALL_NST ? ETA_MAX_ES : ETA_MAX_GS;
@@ -1178,14 +1117,77 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
             pc = P_LOOP_TOP;
           }
           break;
-        }
-        case P_FAIL_END: {
+      } while (0);
+      if (pc == P_LOOP_TOP) do {
+          ST(next_h) = ST(h);
+          if (ST(nst) > 0 && M::ewt_set(w, s, o, atolmin0, 0, V_EWT) != 0) {
+            ST(flag) = ILL_INPUT;
+            pc = P_FAIL_END;
+            break;
+          }
+          if (mxstep > 0 && ST(nstloc) >= mxstep) {
+            ST(flag) = TOO_MUCH_WORK;
+            pc = P_FAIL_END;
+            break;
+          }
+          if (DBL_EPSILON * M::wrms(w, s, 0) > 1.0) {
+            ST(flag) = TOO_MUCH_ACC;
+            pc = P_FAIL_END;
+            break;
+          }
+          pc = P_STEP_BEGIN;
+          break;
+      } while (0);
+      if (pc == P_STEP_BEGIN) do {
+          ST(ncf) = 0;
+          ST(nef) = 0;
+          ST(dsm) = 0.0;
+          if (ST(nst) > 0 && ST(hprime) != ST(h)) M::adjust(w, s);
+          ST(saved_t) = ST(tn);
+          ST(nflag) = FIRST_CALL;
+          pc = P_ATTEMPT;
+          break;
+      } while (0);
+      if (pc == P_ATTEMPT) do {
+          M::predict(w, s);
+          M::set_coeffs(w, s);
+          pc = P_NLS_BEGIN;
+          break;
+      } while (0);
+      if (pc == P_NLS_BEGIN) do {
+          const int nf = ST(nflag);
+          ST(convfail) = (nf == FIRST_CALL || nf == PREV_ERR_FAIL) ? NO_FAILURES
+                                                                  : FAIL_OTHER;
+          ST(callSetup) = nf == PREV_CONV_FAIL || nf == PREV_ERR_FAIL ||
+                         ST(nst) == 0 || ST(nst) >= ST(nstlp) + MSBP ||
+                         ::fabs(ST(gamrat) - 1.0) > DGMAX_LSETUP;
+          M::zero(w, s, V_ACOR);
+          ST(tol) = STA(tq, 4);
+          ST(jbad) = 0;
+          ST(niters) = 0;
+          ST(nconvfails) = 0;
+          pc = P_NLS_OUTER;
+          break;
+      } while (0);
+      if (pc == P_NLS_OUTER) do {
+          ST(curiter) = 0;
+          ST(first_resid) = 1;
+          pc = P_RESID_REQ;
+          break;
+      } while (0);
+      if (pc == P_RESID_REQ) do {
+          M::lsum(w, s, 1.0, 0, 1.0, V_ACOR, V_Y);
+          rq_src = V_Y; rq_dst = V_FTEMP; rq_prec = 0; want = true;
+          pc = P_RESID_RET;
+          break;
+      } while (0);
+      if (pc == P_FAIL_END) do {
           ST(tret) = ST(tn);
           for (int i = 0; i < N; i++) w.at(V_Y, i, s) = w.at(0, i, s);
           pc = P_SOLVE_END;
           break;
-        }
-        case P_SOLVE_END: {  // bdf::integrate's restarts
+      } while (0);
+      if (pc == P_SOLVE_END) do {  // bdf::integrate's restarts
           M::add_stats(w, s);
           const int f = ST(flag);
           if (f >= 0) {
@@ -1209,8 +1211,8 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
           M::reinit(w, s, ST(tret), hc > 0.0 ? 0.1 * hc : 0.0);
           pc = P_SOLVE_BEGIN;
           break;
-        }
-        case P_CELL_DONE: {
+      } while (0);
+      if (pc == P_CELL_DONE) do {
           const int c = ST(cell);
           int flag = ST(flag);
           double y[N];
@@ -1231,18 +1233,72 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
           if (flag == 0) cell_finish<Net, ET>(a, c, y);
           pc = P_FETCH;
           break;
-        }
-        default:
-          pc = P_IDLE;
+      } while (0);
+      if (pc == P_FETCH) do {
+          // one atomic per wave for the lanes that need a cell
+          const unsigned long long need = __ballot(1);
+          const int rank = __popcll(need & ((1ull << lane) - 1ull));
+          const int leader = __ffsll((long long)need) - 1;
+          int base = 0;
+          if (lane == leader) base = atomicAdd(queue, __popcll(need));
+          base = __shfl(base, leader);
+          const int c = base + rank;
+          if (c >= a.ncell) {
+            pc = P_IDLE;
+            break;
+          }
+          ST(cell) = c;
+          for (int i = 0; i < 9; ++i) STA(tot, i) = 0;
+          ST(restarts) = 0;
+          if (!(a.dt[c] > 0.0) || !supported) {
+            a.flags[c] = supported ? 0 : FLAG_UNSUPPORTED;
+            if (a.stats)
+              for (int i = 0; i < 9; ++i) a.stats[(long)c * 9 + i] = 0;
+            pc = P_FETCH;
+            break;
+          }
+          P.p = Net::params_from_row(a.par + (long)c * a.par_stride);
+          P.Tfix = a.T[c];
+          P.hit_T0 = false;
+          for (int i = 0; i < NS; ++i)
+            w.at(0, i, s) = a.n[c * a.n_cs + i * a.n_ss];
+          if (ET) w.at(0, N - 1, s) = a.T[c];
+          ST(tout) = a.dt[c];
+          M::reinit(w, s, 0.0, 0.0);
+          pc = P_SOLVE_BEGIN;
           break;
-      }
+      } while (0);
+      if (pc == P_SOLVE_BEGIN) do {
+          ST(tret) = ST(tn);
+          const double tdist = ::fabs(ST(tout) - ST(tn));
+          const double tround = DBL_EPSILON * mx(::fabs(ST(tn)), ::fabs(ST(tout)));
+          if (tdist == 0.0 || tdist < 2.0 * tround) {
+            ST(flag) = TOO_CLOSE;
+            pc = P_SOLVE_END;
+            break;
+          }
+          if (M::ewt_set(w, s, o, atolmin0, 0, V_EWT) != 0) {
+            ST(flag) = ILL_INPUT;
+            pc = P_SOLVE_END;
+            break;
+          }
+          rq_src = 0; rq_dst = 1; rq_prec = 0; want = true;
+          pc = P_FIRST_RHS;
+          break;
+      } while (0);
     }
     // ---- the rhs of every lane that asked, at one program point ---------
     const unsigned long long busy = __ballot(pc != P_IDLE);
     if (busy == 0ull) break;
     if (want) {
       auto w = w0;
-      asm volatile("" : "+s"(w.v), "+s"(w.S));
+      {
+        unsigned long long pv = (unsigned long long)w0.v;
+        asm volatile("" : "+s"(pv));
+        w.v = (double*)pv;
+      }
+      w.lds = lds_vec;
+      w.tid = threadIdx.x;
       double yr[N];
       for (int i = 0; i < N; i++) yr[i] = w.at(rq_src, i, s);
       double fr[N];
@@ -1262,7 +1318,8 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
 // bytes of the persistent kernel's workspace per lane
 template <int N, bool MIXED>
 constexpr long persist_lane_bytes() {
-  return (long)sizeof(double) * (V_COUNT * N + LANE_SLOTS) +
+  return (long)sizeof(double) * ((V_COUNT - lds_vectors<N>()) * N +
+                                 LANE_SLOTS) +
          2L * N * N * (MIXED ? sizeof(float) : sizeof(double)) +
          (long)sizeof(int) * N;
 }
@@ -1279,15 +1336,18 @@ int launch_persistent_t(CellsArgs a, int wpc, hipStream_t s) {
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
                             dev) != hipSuccess)
     return -1;
-  if (wpc <= 0) wpc = 8;
+  if (wpc <= 0) wpc = 4 * KX_PERSIST_WAVES_PER_EU;
   long lanes = (long)ncu * wpc * 64;
-  // no more lanes than cells (rounded up to whole workgroups)
-  const long need = ((long)a.ncell + 255) / 256 * 256;
+  // no more lanes than cells (rounded up to whole workgroups) or slots
+  constexpr long B = KX_PERSIST_BLOCK;
+  const long need = ((long)a.ncell + B - 1) / B * B;
   if (lanes > need) lanes = need;
-  lanes = (lanes + 255) / 256 * 256;
+  lanes = (lanes + B - 1) / B * B;
+  if (lanes > KX_PERSIST_LANES) lanes = KX_PERSIST_LANES;
+  const long slots = KX_PERSIST_LANES;
   static thread_local void* buf = nullptr;
   static thread_local size_t bytes = 0;
-  const size_t want = persist_lane_bytes<N, MIXED>() * lanes + 256;
+  const size_t want = persist_lane_bytes<N, MIXED>() * slots + 256;
   if (want > bytes) {
     if (buf) (void)hipFree(buf);
     buf = nullptr;
@@ -1299,19 +1359,18 @@ int launch_persistent_t(CellsArgs a, int wpc, hipStream_t s) {
   int* queue = reinterpret_cast<int*>(p);
   p += 256;
   PersistWork<N, MT> w;
-  w.S = lanes;
   w.v = reinterpret_cast<double*>(p);
-  p += sizeof(double) * V_COUNT * N * lanes;
+  p += sizeof(double) * (V_COUNT - lds_vectors<N>()) * N * slots;
   w.J = reinterpret_cast<MT*>(p);
-  p += sizeof(MT) * N * N * lanes;
+  p += sizeof(MT) * N * N * slots;
   w.A = reinterpret_cast<MT*>(p);
-  p += sizeof(MT) * N * N * lanes;
+  p += sizeof(MT) * N * N * slots;
   w.st = reinterpret_cast<double*>(p);
-  p += sizeof(double) * LANE_SLOTS * lanes;
+  p += sizeof(double) * LANE_SLOTS * slots;
   w.piv = reinterpret_cast<int*>(p);
   hipLaunchKernelGGL(fill_flags_kernel, dim3(1), dim3(64), 0, s, 1, queue, 0);
   hipLaunchKernelGGL((persist_cells_kernel<Net, ET, MIXED>),
-                     dim3(lanes / 256), dim3(256), 0, s, a, w, queue);
+                     dim3(lanes / B), dim3(B), 0, s, a, w, queue);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

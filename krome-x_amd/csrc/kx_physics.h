@@ -20,7 +20,10 @@
 
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
-#define KX_HD __host__ __device__
+// every helper of the per-cell solve is inlined into its kernel: the
+// persistent kernel evaluates the rhs at one program point and keeps the
+// lanes' state in registers, which calls through the stack would defeat
+#define KX_HD __host__ __device__ __attribute__((always_inline))
 #else
 #define KX_HD
 #endif
