@@ -69,6 +69,33 @@ struct net_primordial {
       default: return 0.0;
     }
   }
+  // conserved combinations c_k . n (the nuclei of D, H, He, the charge): every
+  // reaction keeps them, so c_k^T J = 0 over the species
+  static constexpr int NCONS = 4;
+  KX_HD static constexpr int cons(int k, int i) {
+    switch (k * NSPEC + i) {
+      case 0: return 1;
+      case 1: return 1;
+      case 7: return 1;
+      case 14: return 1;
+      case 15: return 1;
+      case 16: return 1;
+      case 17: return 2;
+      case 18: return 2;
+      case 19: return 1;
+      case 32: return 1;
+      case 33: return 1;
+      case 34: return 1;
+      case 37: return 1;
+      case 39: return 1;
+      case 40: return -1;
+      case 42: return 1;
+      case 45: return 1;
+      case 46: return 2;
+      case 47: return -1;
+      default: return 0;
+    }
+  }
   template <class R> KX_HD static R ntot(const R* n) {
     R s = R(0);
     s += n[0];

@@ -65,6 +65,28 @@ struct net_primordial3 {
       default: return 0.0;
     }
   }
+  // conserved combinations c_k . n (the nuclei of H, He, the charge): every
+  // reaction keeps them, so c_k^T J = 0 over the species
+  static constexpr int NCONS = 3;
+  KX_HD static constexpr int cons(int k, int i) {
+    switch (k * NSPEC + i) {
+      case 0: return 1;
+      case 1: return 1;
+      case 2: return 1;
+      case 3: return 2;
+      case 4: return 2;
+      case 14: return 1;
+      case 15: return 1;
+      case 16: return 1;
+      case 19: return 1;
+      case 20: return -1;
+      case 22: return 1;
+      case 24: return 1;
+      case 25: return 2;
+      case 26: return -1;
+      default: return 0;
+    }
+  }
   template <class R> KX_HD static R ntot(const R* n) {
     R s = R(0);
     s += n[0];

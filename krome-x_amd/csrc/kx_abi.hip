@@ -128,9 +128,10 @@ struct Buf {
   }
 };
 struct DevCache {
-  Buf work, iwork, n, T, dt, par, flags, stats, queue;
+  Buf work, iwork, n, T, dt, par, flags, stats, queue, count;
   void release() {
-    for (Buf* b : {&work, &iwork, &n, &T, &dt, &par, &flags, &stats, &queue})
+    for (Buf* b : {&work, &iwork, &n, &T, &dt, &par, &flags, &stats, &queue,
+                   &count})
       b->release();
   }
 };
@@ -161,10 +162,31 @@ int launch_plain(kx::CellsArgs a, DevCache& dc, hipStream_t s) {
   return hip_ok(hipGetLastError(), "plain kernel launch") ? 0 : -1;
 }
 
+// HIP events around the last solve's kernel launches, on their stream
+hipEvent_t g_ev[2] = {nullptr, nullptr};
+bool g_ev_recorded = false;
+
+int solve_resident_(kx::CellsArgs a, const Opts& o, DevCache& dc,
+                    hipStream_t s);
+
 // the cells of one device call, in chunks that fit the memory budget:
 // arrays already resident; flags always written
 int solve_resident(kx::CellsArgs a, const Opts& o, DevCache& dc,
                    hipStream_t s) {
+  if (!g_ev[0] && (hipEventCreate(&g_ev[0]) != hipSuccess ||
+                   hipEventCreate(&g_ev[1]) != hipSuccess))
+    g_ev[0] = g_ev[1] = nullptr;
+  if (g_ev[0]) (void)hipEventRecord(g_ev[0], s);
+  const int r = solve_resident_(a, o, dc, s);
+  if (g_ev[0]) {
+    (void)hipEventRecord(g_ev[1], s);
+    g_ev_recorded = true;
+  }
+  return r;
+}
+
+int solve_resident_(kx::CellsArgs a, const Opts& o, DevCache& dc,
+                    hipStream_t s) {
   if (a.ncell <= 0) return 0;
   if (a.o.evolve_T && !Net::THERMAL) {
     return kx::FLAG_UNSUPPORTED;
@@ -388,6 +410,20 @@ int krome_device_finalize(void) {
   return 0;
 }
 
+// the duration of the last device solve's kernels (HIP events recorded
+// around their launches on their stream), waiting for them to finish
+int krome_device_last_kernel_ms(double* ms) {
+  std::lock_guard<std::mutex> lk(g_cache_mutex);
+  if (!ms || !g_ev_recorded) return -1;
+  float f = 0.0f;
+  if (!hip_ok(hipEventSynchronize(g_ev[1]), "hipEventSynchronize") ||
+      !hip_ok(hipEventElapsedTime(&f, g_ev[0], g_ev[1]),
+              "hipEventElapsedTime"))
+    return -1;
+  *ms = (double)f;
+  return 0;
+}
+
 }  // extern "C"
 
 namespace {
@@ -578,7 +614,9 @@ int krome_solve_cells_device(int ncell, double* n, int layout, long ld,
     if (!dc.flags.ensure(sizeof(int) * ncell)) return kx::FLAG_DEVICE_MEMORY;
     fl = static_cast<int*>(dc.flags.p);
   }
-  const int np = kx::params_check_device<Net>(ncell, P, pst, fl);
+  if (!dc.count.ensure(sizeof(int))) return kx::FLAG_DEVICE_MEMORY;
+  const int np = kx::params_check_device<Net>(ncell, P, pst, fl,
+                                              static_cast<int*>(dc.count.p));
   if (np < 0) return -1;
   if (np > 0) return kx::FLAG_PARAM_UNSET;
   const int rj = reject(h);
@@ -651,7 +689,9 @@ int krome_solve_cells_device_stats(int ncell, double* n, int layout, long ld,
     return -1;
   if (ncell == 0) return 0;
   std::lock_guard<std::mutex> lk(g_cache_mutex);
-  const int np = kx::params_check_device<Net>(ncell, par, par_stride, flags);
+  if (!g_cache.count.ensure(sizeof(int))) return -1;
+  const int np = kx::params_check_device<Net>(
+      ncell, par, par_stride, flags, static_cast<int*>(g_cache.count.p));
   if (np < 0) return -1;
   if (np > 0) return kx::FLAG_PARAM_UNSET;
   const int rj = reject(h);

@@ -16,13 +16,13 @@
 // a wave's accesses are coalesced); the method's scalars live in
 // registers.
 //
-// Precision: FP64 throughout, or (mixed) the Jacobian (difference
-// quotients of an FP32 rhs with FP32-sized increments), the Newton matrix
-// I - gamma J, its LU factorization and the triangular solves in FP32,
-// while the state, the Nordsieck history, the Newton residual (an FP64
-// rhs), the WRMS norms and the local error test stay FP64.  A Jacobian
-// whose FP32 evaluation fails (a non-finite FP32 rhs) is re-evaluated in
-// FP64.
+// Precision: FP64 throughout, or (mixed) the Jacobian (FP64 difference
+// quotients, stored in FP32), the Newton matrix, its LU factorization and
+// the triangular solves in FP32, while the state, the Nordsieck history,
+// the Newton residual (an FP64 rhs), the WRMS norms and the local error
+// test stay FP64; the Newton iteration refines the FP32 solves.  The
+// FP32 matrix is an equivalent form of I - gamma J that FP32 can hold
+// (conservation rows, error-weight scaling: form_and_factor).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -55,7 +55,7 @@ enum Pc : int {
   P_FETCH = 0, P_SOLVE_BEGIN, P_FIRST_RHS, P_HIN_BEGIN, P_HIN_REQ, P_HIN_RET,
   P_HIN_END, P_AFTER_HIN, P_LOOP_TOP, P_STEP_BEGIN, P_ATTEMPT, P_NLS_BEGIN,
   P_NLS_OUTER, P_RESID_REQ, P_RESID_RET, P_LSETUP_BEGIN, P_DQ_BEGIN,
-  P_DQ_BASE_RET, P_DQ_REQ, P_DQ_RET, P_LSETUP_FORM, P_INNER_ITER,
+  P_DQ_REQ, P_DQ_RET, P_LSETUP_FORM, P_INNER_ITER,
   P_INNER_EXIT, P_NLS_EXIT, P_HANDLE, P_ERRTEST_RHS, P_STEP_DONE,
   P_FAIL_END, P_SOLVE_END, P_CELL_DONE, P_IDLE
 };
@@ -116,7 +116,7 @@ struct LaneState {
   double saved_t, dsm, tol, cg, ysaved, inc, minInc, tret;
   double hin_hg, hin_hs, hin_hnew, hin_hub, hin_hlb;
   long ncf, nef, nflag, niters, nconvfails, curiter, first_resid, callSetup,
-      jbad, jcol, jr, count1, count2, nls_r, dq32;
+      jbad, jcol, jr, count1, count2, nls_r;
   long nstloc;
   // counters (this CVode call; the cell's totals)
   long nst, nfe, nfeDQ, nje, nsetups, nni, nnf, ncfn, netf;
@@ -138,6 +138,27 @@ static_assert(sizeof(LaneState) % 8 == 0, "8-byte slots");
 #define ST(f) (*reinterpret_cast<decltype(LaneState::f)*>( \
     &w.st[KX_SLOT(f) * w.S + s]))
 #define STA(f, j) (w.st[(KX_SLOT(f) + (j)) * w.S + s])
+#endif
+
+// a per-step record of one chosen cell, in debugging builds only
+#ifdef KX_TRACE
+__device__ double kx_trace_buf[8 * 32768];
+__device__ int kx_trace_n;
+__device__ long kx_trace_cell = -1;
+#define KX_TRACE_REC2(tag, a, b)                                          \
+  if (ST(cell) == kx_trace_cell) {                                        \
+    const int k_ = atomicAdd(&kx_trace_n, 1);                             \
+    if (k_ < 32768) {                                                     \
+      double* r_ = kx_trace_buf + 8 * k_;                                 \
+      r_[0] = (tag); r_[1] = ST(tn); r_[2] = ST(h); r_[3] = (double)ST(q); \
+      r_[4] = (a); r_[5] = (double)ST(nef); r_[6] = (double)ST(nst);      \
+      r_[7] = (b);                                                        \
+    }                                                                     \
+  }
+#define KX_TRACE_REC(tag, a) KX_TRACE_REC2(tag, a, ST(acnrm))
+#else
+#define KX_TRACE_REC2(tag, a, b)
+#define KX_TRACE_REC(tag, a)
 #endif
 
 template <class Net, bool ET, bool MIXED>
@@ -457,13 +478,95 @@ struct Persist {
 
   // ---- the Newton matrix -------------------------------------------------
   // A = I - gamma J and its LU with partial pivoting (bdf_cell.hpp:
-  // 1099-1108, 402-433), in the matrix precision
+  // 1099-1108, 402-433), in the matrix precision.
+  //
+  // Mixed precision factors, in FP32, a matrix with the same solution
+  // that FP32 can hold.  Where gamma J is stiff, the 1 of I - gamma J
+  // sits below FP32's resolution of the stiff entries, and along the
+  // combinations every reaction conserves (c_k^T J = 0: nuclei, charge)
+  // the FP32 matrix is singular, so its solutions are garbage there.  For
+  // each conserved combination one row, that of the species carrying the
+  // most of it, is replaced by c_k^T (its right-hand side by c_k^T b): the
+  // exact solution satisfies c_k^T x = c_k^T b since c_k^T A = c_k^T.
+  // The other rows i are scaled as D A D^-1 (the similarity with the
+  // error weights, D = diag(2^e_i), 2^e_i the power of two at or below
+  // component i's weight), so the FP32 rounding is relative to what the
+  // weighted convergence test sees; a replaced row is scaled by a power
+  // of two to a largest entry in [1, 2).  Powers of two keep the scaling
+  // exact.  Per row k, piv[k] packs the pivot row (bits 0-5), e_k + 1024
+  // (bits 6-16), the law replacing row k plus one, 0 for none (bits
+  // 17-19), and the replaced row's scale exponent + 1024 (bits 20-30).
+  static constexpr int NC = MIXED ? Net::NCONS : 0;
+  static_assert(!MIXED || (N <= 64 && Net::NCONS <= 7), "piv packing");
+  static KX_HD int pack(int pv, int e, int law, int rs) {
+    return pv | ((e + 1024) << 6) | (law << 17) | ((rs + 1024) << 20);
+  }
+  static KX_HD int unpack_pv(int p) { return MIXED ? (p & 63) : p; }
+  static KX_HD int unpack_e(int p) { return ((p >> 6) & 2047) - 1024; }
+  static KX_HD int unpack_law(int p) { return (p >> 17) & 7; }
+  static KX_HD int unpack_rs(int p) { return ((p >> 20) & 2047) - 1024; }
+  static KX_HD int clamp_exp(int k) {
+    return k < -1000 ? -1000 : (k > 1000 ? 1000 : k);
+  }
+
   static KX_HD int form_and_factor(const W& w, long s, double gamma) {
+    int e[N], law[N], rs[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) e[i] = law[i] = rs[i] = 0;
+    if (MIXED) {
+#pragma unroll
+      for (int i = 0; i < N; i++) e[i] = clamp_exp(::ilogb(w.at(V_EWT, i, s)));
+#pragma unroll
+      for (int k = 0; k < NC; k++) {
+        // the row replaced: the species carrying the most of law k
+        int r = -1;
+        double best = -1.0;
+#pragma unroll
+        for (int i = 0; i < NS; i++)
+          if (Net::cons(k, i) != 0 && law[i] == 0) {
+            const double v = ::fabs(Net::cons(k, i) * w.at(V_Y, i, s));
+            if (v > best) {
+              best = v;
+              r = i;
+            }
+          }
+        // its scale: the largest entry c_kj 2^-e_j into [1, 2)
+        int top = -2000;
+#pragma unroll
+        for (int j = 0; j < NS; j++)
+          if (Net::cons(k, j) != 0) {
+            const int t = ::ilogb((double)Net::cons(k, j)) - e[j];
+            top = t > top ? t : top;
+          }
+#pragma unroll
+        for (int i = 0; i < NS; i++)
+          if (i == r) {
+            law[i] = k + 1;
+            rs[i] = clamp_exp(-top);
+          }
+      }
+    }
     const MT c = MT(-gamma);
     for (int j = 0; j < N; j++) {
-      for (int i = 0; i < N; i++)
-        w.Aat(i + j * N, s) = w.Jat(i + j * N, s) * c;
-      w.Aat(j + j * N, s) += MT(1);
+      if (MIXED) {
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+          double a = -gamma * (double)w.Jat(i + j * N, s);
+          if (i == j) a += 1.0;
+          a = ::ldexp(a, e[i] - e[j]);
+#pragma unroll
+          for (int k = 0; k < NC; k++)
+            if (law[i] == k + 1)
+              a = j < NS ? ::ldexp((double)Net::cons(k, j), rs[i] - e[j])
+                         : 0.0;
+          a = a > 1e30 ? 1e30 : (a < -1e30 ? -1e30 : a);
+          w.Aat(i + j * N, s) = MT(a);
+        }
+      } else {
+        for (int i = 0; i < N; i++)
+          w.Aat(i + j * N, s) = w.Jat(i + j * N, s) * c;
+        w.Aat(j + j * N, s) += MT(1);
+      }
     }
     for (int k = 0; k < N; k++) {
       int l_ = k;
@@ -475,7 +578,7 @@ struct Persist {
           l_ = i;
         }
       }
-      w.piv[(long)k * w.S + s] = l_;
+      w.piv[(long)k * w.S + s] = MIXED ? pack(l_, e[k], law[k], rs[k]) : l_;
       if (w.Aat(l_ + k * N, s) == MT(0)) return k + 1;
       if (l_ != k)
         for (int j = 0; j < N; j++) {
@@ -497,18 +600,50 @@ struct Persist {
 
   // delta := A^-1 delta, times 2 / (1 + gamrat) when gamma changed since
   // the setup (bdf_cell.hpp:435-457, 1134-1153).  Mixed: the right-hand
-  // side scaled by its largest magnitude into FP32, solved, scaled back.
+  // side of the factored matrix (rows weighted by 2^e_i, the replaced
+  // rows c_k^T b scaled), scaled by its largest magnitude into FP32,
+  // solved, and the solution unweighted by 2^-e_i.
   static KX_HD void lsolve(const W& w, long s, double gamrat) {
     MT b[N];
+    int pv[N], e[N], law[N], rs[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      const int p = w.piv[(long)k * w.S + s];
+      pv[k] = unpack_pv(p);
+      e[k] = MIXED ? unpack_e(p) : 0;
+      law[k] = MIXED ? unpack_law(p) : 0;
+      rs[k] = MIXED ? unpack_rs(p) : 0;
+    }
     double sc = 1.0;
     if (MIXED) {
-      double m = 0.0;
-      for (int i = 0; i < N; i++) m = ::fmax(m, ::fabs(w.at(V_DELTA, i, s)));
+      double bs[N], cb[NC > 0 ? NC : 1], m = 0.0;
+#pragma unroll
+      for (int k = 0; k < NC; k++) {
+        double t = 0.0;
+#pragma unroll
+        for (int j = 0; j < NS; j++)
+          if (Net::cons(k, j) != 0) t += Net::cons(k, j) * w.at(V_DELTA, j, s);
+        cb[k] = t;
+      }
+#pragma unroll
+      for (int i = 0; i < N; i++) {
+        double v = ::ldexp(w.at(V_DELTA, i, s), e[i]);
+#pragma unroll
+        for (int k = 0; k < NC; k++)
+          if (law[i] == k + 1) v = ::ldexp(cb[k], rs[i]);
+        bs[i] = v;
+        m = ::fmax(m, ::fabs(v));
+      }
       sc = m > 0.0 ? m : 1.0;
+#pragma unroll
+      for (int i = 0; i < N; i++) b[i] = MT(bs[i] / sc);
+    } else {
+#pragma unroll
+      for (int i = 0; i < N; i++) b[i] = MT(w.at(V_DELTA, i, s));
     }
-    for (int i = 0; i < N; i++) b[i] = MT(w.at(V_DELTA, i, s) / sc);
+#pragma unroll
     for (int k = 0; k < N; k++) {
-      const int pk = w.piv[(long)k * w.S + s];
+      const int pk = pv[k];
       if (pk != k) {
         MT u = MT(0);
         for (int i = 0; i < N; i++) u = i == pk ? b[i] : u;
@@ -524,11 +659,12 @@ struct Persist {
       for (int i = 0; i < k; i++) b[i] -= w.Aat(i + k * N, s) * b[k];
     }
     b[0] /= w.Aat(0, s);
-    if (gamrat != 1.0) {
-      const double c = 2.0 / (1.0 + gamrat);
-      for (int i = 0; i < N; i++) w.at(V_DELTA, i, s) = c * (sc * double(b[i]));
-    } else {
-      for (int i = 0; i < N; i++) w.at(V_DELTA, i, s) = sc * double(b[i]);
+    const double c = gamrat != 1.0 ? 2.0 / (1.0 + gamrat) : 1.0;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      double x = double(b[i]);
+      if (MIXED) x = ::ldexp(sc * x, -e[i]);
+      w.at(V_DELTA, i, s) = gamrat != 1.0 ? c * x : x;
     }
   }
 
@@ -651,7 +787,7 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
   P.hit_T0 = false;
   // the rhs request of this lane: source vector, destination vector,
   // precision (0: FP64, 1: FP32), and its return code
-  int rq_src = 0, rq_dst = 0, rq_prec = 0, rr = 0;
+  int rq_src = 0, rq_dst = 0, rr = 0;
   bool want = false;
 
   for (;;) {
@@ -758,51 +894,25 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
           pc = P_ATTEMPT;
           break;
       } while (0);
-      if (pc == P_DQ_BASE_RET) do {
-          if (rr != 0) {  // the FP32 rhs failed: this Jacobian in FP64
-            ST(dq32) = 0;
-            const double fn = M::wrms(w, s, V_FTEMP);
-            ST(minInc) = fn != 0.0 ? (MIN_INC_MULT * ::fabs(ST(h)) * DBL_EPSILON
-                                     * (double)N * fn)
-                                  : 1.0;
-          }
-          pc = P_DQ_REQ;
-          break;
-      } while (0);
       if (pc == P_DQ_RET) do {
           ST(nfeDQ)++;
           const int j = ST(jcol);
           if (rr != 0) {
-            if (ST(dq32)) {  // redo the whole Jacobian in FP64
-              for (int i = 0; i < N; i++)
-                if (i == j) w.at(V_Y, i, s) = ST(ysaved);
-              ST(dq32) = 0;
-              const double fn = M::wrms(w, s, V_FTEMP);
-              ST(minInc) = fn != 0.0 ? (MIN_INC_MULT * ::fabs(ST(h)) *
-                                       DBL_EPSILON * (double)N * fn)
-                                    : 1.0;
-              ST(jcol) = 0;
-              pc = P_DQ_REQ;
-              break;
-            }
             ST(jr) = rr;
             pc = P_LSETUP_FORM;
             break;
           }
           for (int i = 0; i < N; i++)
             if (i == j) w.at(V_Y, i, s) = ST(ysaved);
+          // lin_sum(ii, vt1, -ii, ftemp): the a == -b form.  The
+          // differences are FP64 in both precisions; mixed precision stores
+          // them in FP32 (FP32 differences, with their sqrt(FLT_EPSILON)
+          // steps, blur the slow chemistry into the fast and cost the
+          // accuracy the FP64 residual cannot buy back)
           const double ii = 1.0 / ST(inc);
-          if (ST(dq32)) {
-            const float iif = (float)ii;
-            for (int i = 0; i < N; i++)
-              w.Jat(i + j * N, s) = typename M::MT(
-                  ((float)w.at(V_VT1, i, s) - (float)w.at(V_TEMPV, i, s)) * iif);
-          } else {
-            // lin_sum(ii, vt1, -ii, ftemp): the a == -b form
-            for (int i = 0; i < N; i++)
-              w.Jat(i + j * N, s) = typename M::MT(
-                  ii * (w.at(V_VT1, i, s) - w.at(V_FTEMP, i, s)));
-          }
+          for (int i = 0; i < N; i++)
+            w.Jat(i + j * N, s) = typename M::MT(
+                ii * (w.at(V_VT1, i, s) - w.at(V_FTEMP, i, s)));
           ST(jcol)++;
           if (ST(jcol) < N) {
             pc = P_DQ_REQ;
@@ -879,7 +989,7 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
       } while (0);
       if (pc == P_HIN_REQ) do {
           M::lsum(w, s, ST(hin_hg), 1, 1.0, 0, V_Y);
-          rq_src = V_Y; rq_dst = V_TEMPV; rq_prec = 0; want = true;
+          rq_src = V_Y; rq_dst = V_TEMPV; want = true;
           pc = P_HIN_RET;
           break;
       } while (0);
@@ -919,24 +1029,16 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
       } while (0);
       if (pc == P_DQ_BEGIN) do {  // cvLsDenseDQJac (bdf_cell.hpp:1290-1311)
           const double fn = M::wrms(w, s, V_FTEMP);
-          ST(dq32) = MIXED ? 1 : 0;
-          const double eps = ST(dq32) ? (double)FLT_EPSILON : DBL_EPSILON;
-          ST(minInc) = fn != 0.0 ? (MIN_INC_MULT * ::fabs(ST(h)) * eps *
-                                   (double)N * fn)
+          ST(minInc) = fn != 0.0 ? (MIN_INC_MULT * ::fabs(ST(h)) *
+                                   DBL_EPSILON * (double)N * fn)
                                 : 1.0;
           ST(jcol) = 0;
-          if (ST(dq32)) {
-            // the FP32 rhs at y, the base of the FP32 differences
-            rq_src = V_Y; rq_dst = V_TEMPV; rq_prec = 1; want = true;
-            pc = P_DQ_BASE_RET;
-          } else {
-            pc = P_DQ_REQ;
-          }
+          pc = P_DQ_REQ;
           break;
       } while (0);
       if (pc == P_DQ_REQ) do {
           const int j = ST(jcol);
-          const double srur = ST(dq32) ? rt((double)FLT_EPSILON) : rt(DBL_EPSILON);
+          const double srur = rt(DBL_EPSILON);
           double ys = 0.0, ew = 0.0;
           for (int i = 0; i < N; i++)
             if (i == j) {
@@ -947,7 +1049,7 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
           ST(inc) = mx(srur * ::fabs(ys), ST(minInc) / ew);
           for (int i = 0; i < N; i++)
             if (i == j) w.at(V_Y, i, s) += ST(inc);
-          rq_src = V_Y; rq_dst = V_VT1; rq_prec = ST(dq32); want = true;
+          rq_src = V_Y; rq_dst = V_VT1; want = true;
           pc = P_DQ_RET;
           break;
       } while (0);
@@ -981,6 +1083,7 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
           if (inv) M::inv_set(w, s, -ST(cg));
           M::lsum(w, s, 1.0, V_ACOR, 1.0, V_DELTA, V_ACOR);
           const int r = M::conv_test(w, s, ST(curiter));
+          KX_TRACE_REC2(5.0, ST(delnrm), ST(crate));
           ST(curiter)++;
           if (r == BDF_OK) {
             ST(nls_jcur) = false;
@@ -1029,6 +1132,7 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
       if (pc == P_HANDLE) do {  // cvHandleNFlag, cvDoErrorTest
           const int f = ST(nflag);
           if (f != BDF_OK) {
+            KX_TRACE_REC(1.0, (double)f);
             ST(ncfn)++;
             M::restore(w, s);
             if (f < 0) {
@@ -1059,6 +1163,7 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
           }
           const double dsm = ST(acnrm) * STA(tq, 2);
           ST(dsm) = dsm;
+          KX_TRACE_REC(2.0, dsm);
           if (dsm <= 1.0) {
             pc = P_STEP_DONE;
             break;
@@ -1097,13 +1202,15 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
           ST(hscale) = ST(h);
           ST(qwait) = LONG_WAIT;
           ST(nscon) = 0;
-          rq_src = 0; rq_dst = V_TEMPV; rq_prec = 0; want = true;
+          rq_src = 0; rq_dst = V_TEMPV; want = true;
           pc = P_ERRTEST_RHS;
           break;
       } while (0);
       if (pc == P_STEP_DONE) do {
           M::complete(w, s);
           M::next_step(w, s);
+          KX_TRACE_REC(3.0, ST(eta));
+          KX_TRACE_REC2(6.0, w.at(0, N - 1, s), w.at(0, 3, s));
           ST(etamax) = ST(nst) <= SMALL_NST ? ETA_MAX_ES : ETA_MAX_GS;
           M::scal(w, s, STA(tq, 2), V_ACOR, V_ACOR);
           ST(nstloc)++;
@@ -1177,7 +1284,7 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
       } while (0);
       if (pc == P_RESID_REQ) do {
           M::lsum(w, s, 1.0, 0, 1.0, V_ACOR, V_Y);
-          rq_src = V_Y; rq_dst = V_FTEMP; rq_prec = 0; want = true;
+          rq_src = V_Y; rq_dst = V_FTEMP; want = true;
           pc = P_RESID_RET;
           break;
       } while (0);
@@ -1190,6 +1297,7 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
       if (pc == P_SOLVE_END) do {  // bdf::integrate's restarts
           M::add_stats(w, s);
           const int f = ST(flag);
+          KX_TRACE_REC(4.0, (double)f);
           if (f >= 0) {
             ST(flag) = 0;
             pc = P_CELL_DONE;
@@ -1282,7 +1390,7 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
             pc = P_SOLVE_END;
             break;
           }
-          rq_src = 0; rq_dst = 1; rq_prec = 0; want = true;
+          rq_src = 0; rq_dst = 1; want = true;
           pc = P_FIRST_RHS;
           break;
       } while (0);
@@ -1302,14 +1410,7 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
       double yr[N];
       for (int i = 0; i < N; i++) yr[i] = w.at(rq_src, i, s);
       double fr[N];
-      if (MIXED && rq_prec == 1) {
-        float y32[N], f32[N];
-        for (int i = 0; i < N; i++) y32[i] = (float)yr[i];
-        rr = P.template rhs_r<float>(y32, f32);
-        for (int i = 0; i < N; i++) fr[i] = (double)f32[i];
-      } else {
-        rr = P.template rhs_r<double>(yr, fr);
-      }
+      rr = P.template rhs_r<double>(yr, fr);
       for (int i = 0; i < N; i++) w.at(rq_dst, i, s) = fr[i];
     }
   }

@@ -1,7 +1,7 @@
 // Runtime physics of the per-cell right-hand side, for host (the CPU
 // oracle, g++) and gfx950 device code (hipcc), each a template on the real
-// type R (double: the FP64 path; float: the FP32 Jacobian of the mixed
-// path).  The formulas are the published fits KROME-X evaluates; their
+// type R (double on every path that runs; the generated rhs shares the
+// template).  The formulas are the published fits KROME-X evaluates; their
 // coefficients are physical data and are the same numbers:
 //   cooling_H2          Glover & Abel 2008 + HM79/Glover 2015 LTE limit
 //                       (runtime/core/cooling.hpp:74-177)
@@ -30,24 +30,19 @@
 
 namespace kx {
 
-// the math of both precisions under one name
+// the math of the rhs under one name
 KX_HD inline double exp(double x) { return ::exp(x); }
-KX_HD inline float exp(float x) { return ::expf(x); }
 KX_HD inline double log(double x) { return ::log(x); }
-KX_HD inline float log(float x) { return ::logf(x); }
 KX_HD inline double log10(double x) { return ::log10(x); }
-KX_HD inline float log10(float x) { return ::log10f(x); }
 KX_HD inline double sqrt(double x) { return ::sqrt(x); }
-KX_HD inline float sqrt(float x) { return ::sqrtf(x); }
 KX_HD inline double pow(double x, double y) { return ::pow(x, y); }
-KX_HD inline float pow(float x, float y) { return ::powf(x, y); }
 KX_HD inline double tanh(double x) { return ::tanh(x); }
-KX_HD inline float tanh(float x) { return ::tanhf(x); }
 KX_HD inline double fabs(double x) { return ::fabs(x); }
-KX_HD inline float fabs(float x) { return ::fabsf(x); }
 KX_HD inline double fmin(double x, double y) { return ::fmin(x, y); }
-KX_HD inline float fmin(float x, float y) { return ::fminf(x, y); }
 KX_HD inline double fmax(double x, double y) { return ::fmax(x, y); }
+// (FP32: the mixed-precision Newton matrix)
+KX_HD inline float fabs(float x) { return ::fabsf(x); }
+KX_HD inline float fmin(float x, float y) { return ::fminf(x, y); }
 KX_HD inline float fmax(float x, float y) { return ::fmaxf(x, y); }
 
 // x^N for an integer constant N by binary powering
@@ -138,13 +133,7 @@ KX_HD inline R cooling_H2(R T, R nH, R nHj, R nH2, R ne, R nHe) {
                 + R(4.0406627) * l3_ - R(4.7274036) * l4 - R(8.8077017) * l5
                 + R(8.9167183) * l6 + R(6.4380698) * l7
                 - R(6.3701156) * l8) * ne;
-  if constexpr (std::is_same<R, float>::value) {
-    // FP32: the capped fit (up to 1e30 below 100 K) times n_e overflows
-    // where the window is 0; weight first (0 stays 0)
-    cool += w24 == R(0) ? R(0) : f * w24;
-  } else {
-    cool += f * w24;
-  }
+  cool += f * w24;
   // H2-He
   if (T > R(1e1) && T <= R(1e4))
     f = kx::pow(R(10), R(-23.689237) + R(2.1892372) * l3 - R(0.81520438) * l2
@@ -194,19 +183,10 @@ KX_HD inline R cooling_continuum(R T, R rho_in, R ntot) {
     kgas = kx::pow(R(10), R(1.000042) * kx::log10(kx::fmin(rho_in, R(0.5)))
                               + R(2.14989));
   const R rho = kx::fmax(rho_in, R(1e-40));
-  R tau;
-  if constexpr (std::is_same<R, float>::value) {
-    // FP32: lj^2 = pi kB T ntot / (G rho^2) in factors that stay in range
-    // (the FP64 form's intermediate pi kB T / rho / m_p / G reaches 1e41)
-    const R lj = kx::sqrt(R(phys::pi * phys::kB / phys::gravity) * T)
-                 * kx::sqrt(ntot) / rho;
-    tau = lj * kgas * rho + R(1e-30);
-  } else {
-    const R mu = rho / kx::fmax(ntot, R(1e-40)) / R(phys::p_mass);
-    const R lj = kx::sqrt(R(phys::pi) * R(phys::kB) * T / rho
-                          / R(phys::p_mass) / R(phys::gravity) / mu);
-    tau = lj * kgas * rho + R(1e-40);
-  }
+  const R mu = rho / kx::fmax(ntot, R(1e-40)) / R(phys::p_mass);
+  const R lj = kx::sqrt(R(phys::pi) * R(phys::kB) * T / rho
+                        / R(phys::p_mass) / R(phys::gravity) / mu);
+  const R tau = lj * kgas * rho + R(1e-40);
   const R beta = kx::fmin(R(1), R(1) / (tau * tau));
   return R(4) * R(phys::stefboltz) * T * T * T * T * kgas * rho * beta;
 }
@@ -223,17 +203,11 @@ KX_HD inline R h2_heat_fraction(R T, R nH, R nH2, R dd) {
 }
 
 // vibrational heat capacity C_vib / k = x^2 e^x / (e^x - 1)^2 of a
-// diatomic, x = theta_vib / T; FP32 in the equal form x^2 e^-x /
-// (1 - e^-x)^2 (e^x overflows FP32 below T ~ theta / 88)
+// diatomic, x = theta_vib / T
 template <class R>
 KX_HD inline R vib_heat_capacity(R x) {
-  if constexpr (std::is_same<R, float>::value) {
-    const R e = kx::exp(-x);
-    return x * x * e / ((R(1) - e) * (R(1) - e));
-  } else {
-    const R e = kx::exp(x);
-    return x * x * e / ((e - R(1)) * (e - R(1)));
-  }
+  const R e = kx::exp(x);
+  return x * x * e / ((e - R(1)) * (e - R(1)));
 }
 
 // rotational heat capacity C_rot / k of a diatomic with rotational

@@ -27,9 +27,7 @@ __global__ inline void fill_flags_kernel(int ncell, int* flags, int v) {
 // flags receive -105 or 0
 template <class Net>
 inline int params_check_device(int ncell, const double* par, int pst,
-                               int* flags) {
-  int* cnt = nullptr;
-  if (hipMalloc(&cnt, sizeof(int)) != hipSuccess) return -1;
+                               int* flags, int* cnt) {
   int h = 0;
   hipError_t e = hipMemcpy(cnt, &h, sizeof(int), hipMemcpyHostToDevice);
   if (e == hipSuccess) {
@@ -37,7 +35,6 @@ inline int params_check_device(int ncell, const double* par, int pst,
                        dim3(256), 0, nullptr, ncell, par, pst, flags, cnt);
     e = hipMemcpy(&h, cnt, sizeof(int), hipMemcpyDeviceToHost);
   }
-  (void)hipFree(cnt);
   return e == hipSuccess ? h : -1;
 }
 

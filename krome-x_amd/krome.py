@@ -114,7 +114,8 @@ class Network:
                  [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_long,
                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                   ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
-                  ctypes.c_void_p])):
+                  ctypes.c_void_p]),
+                ("krome_device_last_kernel_ms", ctypes.c_int, [_D])):
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args

@@ -265,6 +265,25 @@ def _check_balance(net):
                             % (r.line, r.eq))
 
 
+def conserved_laws(net):
+    """[(name, [coefficient per species])]: the nuclei of every element
+    the species hold, then the charge when a species is charged, each kept
+    only if every reaction conserves it"""
+    elems = sorted({e for sp in net.species for e in sp.comp})
+    laws = [(e, [sp.comp.get(e, 0) for sp in net.species]) for e in elems]
+    if any(sp.charge for sp in net.species):
+        laws.append(("charge", [sp.charge for sp in net.species]))
+    idx = {sp.name: i for i, sp in enumerate(net.species)}
+    kept = []
+    for nm, c in laws:
+        ok = all(sum(c[idx[s]] for s, _ in r.reactants)
+                 == sum(c[idx[s]] for s in r.products)
+                 for r in net.reactions)
+        if ok:
+            kept.append((nm, c))
+    return kept
+
+
 def _group(net):
     """tile the ranged lines of identical stoichiometry into piecewise
     groups (network-format.md 3.1); an unranged line is its own group"""
@@ -609,6 +628,23 @@ def generate(knet_path, thermal=None, name=None):
     for i, m in enumerate(masses):
         w("      case %d: return %r;" % (i, m))
     w("      default: return 0.0;")
+    w("    }")
+    w("  }")
+    # conserved combinations: the nuclei of each element and the charge,
+    # kept where every reaction keeps them
+    laws = conserved_laws(net)
+    w("  // conserved combinations c_k . n (the nuclei of %s%s): every"
+      % (", ".join(nm for nm, _ in laws if nm != "charge"),
+         ", the charge" if any(nm == "charge" for nm, _ in laws) else ""))
+    w("  // reaction keeps them, so c_k^T J = 0 over the species")
+    w("  static constexpr int NCONS = %d;" % len(laws))
+    w("  KX_HD static constexpr int cons(int k, int i) {")
+    w("    switch (k * NSPEC + i) {")
+    for k, (_, c) in enumerate(laws):
+        for i, v in enumerate(c):
+            if v:
+                w("      case %d: return %d;" % (k * NS + i, v))
+    w("      default: return 0;")
     w("    }")
     w("  }")
     w("  template <class R> KX_HD static R ntot(const R* n) {")

@@ -11,6 +11,12 @@
 #include <cfloat>
 #include <cmath>
 
+// a hook for per-step tracing in debugging builds; empty otherwise
+#ifndef KXO_TRACE
+#define KXO_TRACE(tag, a)
+#define KXO_TRACE2(tag, a, b)
+#endif
+
 namespace kxo {
 
 constexpr int QMAX = 5;            // bdf_cell.hpp:103
@@ -438,15 +444,19 @@ class Bdf {
       set_coeffs();
       nflag = nls(nflag);
       const int kf = handle_nflag(&nflag, saved_t, &ncf);
+      if (kf != DO_ERROR_TEST) KXO_TRACE(1, (double)kf);
       if (kf == PREDICT_AGAIN) continue;
       if (kf != DO_ERROR_TEST) return kf;
       const int ef = error_test(&nflag, saved_t, &nef, &dsm);
+      KXO_TRACE(2, dsm);
       if (ef == TRY_AGAIN) continue;
       if (ef != OK) return ef;
       break;
     }
     complete();
     next_step(dsm);
+    KXO_TRACE(3, eta);
+    KXO_TRACE2(6, zn[0][N - 1], zn[0][3]);
     etamax = st.nst <= SMALL_NST ? ETA_MAX_ES : ETA_MAX_GS;
     c_::scal(N, tq[2], acor, acor);
     return OK;
@@ -601,6 +611,7 @@ class Bdf {
         if (inv) inv_set(delta, -cg);
         c_::lsum(N, 1.0, acor, 1.0, delta, acor);
         r = conv_test(it, tol);
+        KXO_TRACE2(5, delnrm, crate);
         it++;
         if (r == OK) {
           nls_jcur = false;

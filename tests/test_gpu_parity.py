@@ -5,9 +5,9 @@ Bar: FP64 kernels within 1e-6 relative of the reference on primordial3
 (rtol 1e-8) for every species above 1e-12 of the cell's total and for
 Tgas, 5e-5 on primordial (integrated at rtol 1e-6, where the last bits of
 the math library move the adaptive trajectory by up to ~10 rtol); flags
-equal.  The mixed-precision kernel within 1e-4 (the benchmark's
-agreement bar; its Newton iteration refines an FP32 Jacobian and LU, so
-it takes other steps than the FP64 solver).
+equal.  The mixed-precision kernel within 1e-5 (ten times inside the
+benchmark's 1e-4 agreement bar; its Newton iteration refines FP32 solves
+of an FP32-stored Jacobian, so it takes slightly different steps).
 """
 import sys
 
@@ -48,7 +48,7 @@ def _opts(net, d, kernel="plain", mixed=0, **kw):
 
 
 def _tol(netname, mixed):
-    return 1e-4 if mixed else TOL[netname]
+    return max(1e-5, TOL[netname]) if mixed else TOL[netname]
 
 
 @pytest.mark.parametrize("kernel,mixed", KERNELS)
@@ -78,11 +78,10 @@ def test_golden_cells(nets, netname, kernel, mixed, oracle):
 
 @pytest.mark.parametrize("kernel,mixed", KERNELS)
 def test_counters_match_oracle(nets, kernel, mixed, oracle):
-    """the FP64 kernels take the oracle's steps (up to the math library's
-    last bits): total steps, rhs evaluations and Jacobians within 2 %"""
+    """the kernels take the oracle's steps (up to the math library's last
+    bits; the mixed-precision Newton up to its FP32 solves): total steps,
+    rhs evaluations and Jacobians within 2 %"""
     import torch
-    if mixed:
-        pytest.skip("the mixed-precision Newton takes its own steps")
     d = golden("primordial3_solve.npz")
     net = nets["primordial3"]
     say("counters", kernel, mixed)
