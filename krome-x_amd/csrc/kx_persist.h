@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <utility>
 
 #include "kx_cell.h"
 #include "kx_plain.h"
@@ -42,10 +43,6 @@
 // SIMD on the 256 CUs of an MI355X
 #ifndef KX_PERSIST_LANES
 #define KX_PERSIST_LANES 65536
-#endif
-// the lanes' scalars in registers (1) or in HBM slots (0)
-#ifndef KX_STATE_IN_REGS
-#define KX_STATE_IN_REGS 1
 #endif
 
 namespace kx {
@@ -84,7 +81,7 @@ struct PersistWork {
   MT* J;         // N * N * S
   MT* A;         // N * N * S
   int* piv;      // N * S
-  double* st;    // LANE_SLOTS * S: the lanes' scalars (LaneState)
+  long* tot;     // TOT_SLOTS * S: the cells' counter totals
   // the interleave stride: the kernel's lane slots, a compile-time
   // constant, so that the elements of a lane sit at constant offsets from
   // one base (the compiler then sees that distinct elements never alias)
@@ -102,43 +99,38 @@ struct PersistWork {
 };
 
 // the scalars of a lane's machine (CVodeMem, the loop variables of the
-// suspended functions, and the cell bookkeeping)
+// suspended functions, and the cell bookkeeping), held in registers: the
+// integer members are 32-bit (a call's counters stay below max_steps
+// times a few, cell indices below 2^31) to leave registers to the stages
 struct LaneState {
   // CVodeMem
   double tn, h, hin, hprime, hscale, next_h, eta, etamax, etaq, etaqm1,
       etaqp1, saved_tq5;
   double tau[7], tq[6], l[6];
   double rl1, gamma, gammap, gamrat, crate, delp, delnrm, acnrm;
-  long q, qprime, qwait, L, convfail;
-  long nscon, nstlp, nstlj;
-  long acnrmcur, jcur, nls_jcur;
+  int q, qprime, qwait, L, convfail;
+  int nscon, nstlp, nstlj;
+  int acnrmcur, jcur, nls_jcur;
   // loop variables of cvStep / cvNls / cvHin / the DQ Jacobian
   double saved_t, dsm, tol, cg, ysaved, inc, minInc, tret;
   double hin_hg, hin_hs, hin_hnew, hin_hub, hin_hlb;
-  long ncf, nef, nflag, niters, nconvfails, curiter, first_resid, callSetup,
+  int ncf, nef, nflag, niters, nconvfails, curiter, first_resid, callSetup,
       jbad, jcol, jr, count1, count2, nls_r;
-  long nstloc;
-  // counters (this CVode call; the cell's totals)
-  long nst, nfe, nfeDQ, nje, nsetups, nni, nnf, ncfn, netf;
-  long tot[9];
-  long restarts, flag;
+  int nstloc;
+  // counters of this CVode call
+  int nst, nfe, nfeDQ, nje, nsetups, nni, nnf, ncfn, netf;
+  int restarts, flag;
   // the cell
-  long cell;
+  int cell;
   double tout;
 };
-// every member is one 8-byte slot: member k of lane s at slot k S + s
-constexpr int LANE_SLOTS = sizeof(LaneState) / 8;
-static_assert(sizeof(LaneState) % 8 == 0, "8-byte slots");
-#define KX_SLOT(f) ((long)(offsetof(LaneState, f) / 8))
 // a member of this lane (st), an element of an array member (sta)
-#if KX_STATE_IN_REGS
 #define ST(f) (w.sp->f)
 #define STA(f, j) (w.sp->f[j])
-#else
-#define ST(f) (*reinterpret_cast<decltype(LaneState::f)*>( \
-    &w.st[KX_SLOT(f) * w.S + s]))
-#define STA(f, j) (w.st[(KX_SLOT(f) + (j)) * w.S + s])
-#endif
+// the cell's counter totals over its restarts, in HBM (touched at a
+// call's end only): total k of lane s at k S + s
+constexpr int TOT_SLOTS = 8;
+#define TOT(k) (w.tot[(long)(k) * w.S + s])
 
 // a per-step record of one chosen cell, in debugging builds only
 #ifdef KX_TRACE
@@ -509,93 +501,165 @@ struct Persist {
     return k < -1000 ? -1000 : (k > 1000 ? 1000 : k);
   }
 
-  static KX_HD int form_and_factor(const W& w, long s, double gamma) {
-    int e[N], law[N], rs[N];
+  // the scaling exponents and the replaced rows of the mixed-precision
+  // matrix (see above)
+  static KX_HD void mixed_scaling(const W& w, long s, int* e, int* law,
+                                  int* rs) {
 #pragma unroll
-    for (int i = 0; i < N; i++) e[i] = law[i] = rs[i] = 0;
-    if (MIXED) {
-#pragma unroll
-      for (int i = 0; i < N; i++) e[i] = clamp_exp(::ilogb(w.at(V_EWT, i, s)));
-#pragma unroll
-      for (int k = 0; k < NC; k++) {
-        // the row replaced: the species carrying the most of law k
-        int r = -1;
-        double best = -1.0;
-#pragma unroll
-        for (int i = 0; i < NS; i++)
-          if (Net::cons(k, i) != 0 && law[i] == 0) {
-            const double v = ::fabs(Net::cons(k, i) * w.at(V_Y, i, s));
-            if (v > best) {
-              best = v;
-              r = i;
-            }
-          }
-        // its scale: the largest entry c_kj 2^-e_j into [1, 2)
-        int top = -2000;
-#pragma unroll
-        for (int j = 0; j < NS; j++)
-          if (Net::cons(k, j) != 0) {
-            const int t = ::ilogb((double)Net::cons(k, j)) - e[j];
-            top = t > top ? t : top;
-          }
-#pragma unroll
-        for (int i = 0; i < NS; i++)
-          if (i == r) {
-            law[i] = k + 1;
-            rs[i] = clamp_exp(-top);
-          }
-      }
+    for (int i = 0; i < N; i++) {
+      e[i] = clamp_exp(::ilogb(w.at(V_EWT, i, s)));
+      law[i] = rs[i] = 0;
     }
-    const MT c = MT(-gamma);
-    for (int j = 0; j < N; j++) {
-      if (MIXED) {
 #pragma unroll
-        for (int i = 0; i < N; i++) {
-          double a = -gamma * (double)w.Jat(i + j * N, s);
-          if (i == j) a += 1.0;
-          a = ::ldexp(a, e[i] - e[j]);
+    for (int k = 0; k < NC; k++) {
+      // the row replaced: the species carrying the most of law k
+      int r = -1;
+      double best = -1.0;
 #pragma unroll
-          for (int k = 0; k < NC; k++)
-            if (law[i] == k + 1)
-              a = j < NS ? ::ldexp((double)Net::cons(k, j), rs[i] - e[j])
-                         : 0.0;
-          a = a > 1e30 ? 1e30 : (a < -1e30 ? -1e30 : a);
-          w.Aat(i + j * N, s) = MT(a);
+      for (int i = 0; i < NS; i++)
+        if (Net::cons(k, i) != 0 && law[i] == 0) {
+          const double v = ::fabs(Net::cons(k, i) * w.at(V_Y, i, s));
+          if (v > best) {
+            best = v;
+            r = i;
+          }
         }
-      } else {
+      // its scale: the largest entry c_kj 2^-e_j into [1, 2)
+      int top = -2000;
+#pragma unroll
+      for (int j = 0; j < NS; j++)
+        if (Net::cons(k, j) != 0) {
+          const int t = ::ilogb((double)Net::cons(k, j)) - e[j];
+          top = t > top ? t : top;
+        }
+#pragma unroll
+      for (int i = 0; i < NS; i++)
+        if (i == r) {
+          law[i] = k + 1;
+          rs[i] = clamp_exp(-top);
+        }
+    }
+  }
+
+  static KX_HD int form_and_factor(const W& w, long s, double gamma) {
+    if constexpr (MIXED) {
+      return form_and_factor_regs(w, s, gamma);
+    } else {
+      // FP64: I - gamma J and its LU in place in the workspace, the
+      // reference's operations in its order
+      const MT c = MT(-gamma);
+      for (int j = 0; j < N; j++) {
         for (int i = 0; i < N; i++)
           w.Aat(i + j * N, s) = w.Jat(i + j * N, s) * c;
         w.Aat(j + j * N, s) += MT(1);
       }
-    }
-    for (int k = 0; k < N; k++) {
-      int l_ = k;
-      MT best = kx::fabs(w.Aat(k + k * N, s));
-      for (int i = k + 1; i < N; i++) {
-        const MT v = kx::fabs(w.Aat(i + k * N, s));
-        if (v > best) {
-          best = v;
-          l_ = i;
+      for (int k = 0; k < N; k++) {
+        int l_ = k;
+        MT best = kx::fabs(w.Aat(k + k * N, s));
+        for (int i = k + 1; i < N; i++) {
+          const MT v = kx::fabs(w.Aat(i + k * N, s));
+          if (v > best) {
+            best = v;
+            l_ = i;
+          }
+        }
+        w.piv[(long)k * w.S + s] = l_;
+        if (w.Aat(l_ + k * N, s) == MT(0)) return k + 1;
+        if (l_ != k)
+          for (int j = 0; j < N; j++) {
+            const MT t = w.Aat(l_ + j * N, s);
+            w.Aat(l_ + j * N, s) = w.Aat(k + j * N, s);
+            w.Aat(k + j * N, s) = t;
+          }
+        const MT r = MT(1) / w.Aat(k + k * N, s);
+        for (int i = k + 1; i < N; i++) w.Aat(i + k * N, s) *= r;
+        for (int j = k + 1; j < N; j++) {
+          const MT akj = w.Aat(k + j * N, s);
+          if (akj != MT(0))
+            for (int i = k + 1; i < N; i++)
+              w.Aat(i + j * N, s) -= akj * w.Aat(i + k * N, s);
         }
       }
-      w.piv[(long)k * w.S + s] = MIXED ? pack(l_, e[k], law[k], rs[k]) : l_;
-      if (w.Aat(l_ + k * N, s) == MT(0)) return k + 1;
-      if (l_ != k)
-        for (int j = 0; j < N; j++) {
-          const MT t = w.Aat(l_ + j * N, s);
-          w.Aat(l_ + j * N, s) = w.Aat(k + j * N, s);
-          w.Aat(k + j * N, s) = t;
-        }
-      const MT r = MT(1) / w.Aat(k + k * N, s);
-      for (int i = k + 1; i < N; i++) w.Aat(i + k * N, s) *= r;
-      for (int j = k + 1; j < N; j++) {
-        const MT akj = w.Aat(k + j * N, s);
-        if (akj != MT(0))
-          for (int i = k + 1; i < N; i++)
-            w.Aat(i + j * N, s) -= akj * w.Aat(i + k * N, s);
+      return 0;
+    }
+  }
+
+  // one column step k of the register LU (k a compile-time constant, so
+  // every index into a is one: the array stays in registers)
+  template <int K>
+  static KX_HD void lu_step(float (&a)[N * N], int& fail, const W& w, long s,
+                            const int* e, const int* law, const int* rs) {
+    int l = K;
+    float best = ::fabsf(a[K + K * N]);
+#pragma unroll
+    for (int i = K + 1; i < N; i++) {
+      const float v = ::fabsf(a[i + K * N]);
+      if (v > best) {
+        best = v;
+        l = i;
       }
     }
-    return 0;
+    w.piv[(long)K * w.S + s] = pack(l, e[K], law[K], rs[K]);
+    if (best == 0.0f && fail == 0) fail = K + 1;
+    if (l != K) {
+#pragma unroll
+      for (int j = 0; j < N; j++) {
+        const float tk = a[K + j * N];
+        float tl = tk;
+#pragma unroll
+        for (int i = K + 1; i < N; i++) tl = i == l ? a[i + j * N] : tl;
+        a[K + j * N] = tl;
+#pragma unroll
+        for (int i = K + 1; i < N; i++)
+          a[i + j * N] = i == l ? tk : a[i + j * N];
+      }
+    }
+    const float r = best != 0.0f ? 1.0f / a[K + K * N] : 0.0f;
+#pragma unroll
+    for (int i = K + 1; i < N; i++) a[i + K * N] *= r;
+#pragma unroll
+    for (int j = K + 1; j < N; j++) {
+      const float akj = a[K + j * N];
+#pragma unroll
+      for (int i = K + 1; i < N; i++)
+        a[i + j * N] = ::fmaf(-akj, a[i + K * N], a[i + j * N]);
+    }
+  }
+  template <int... K>
+  static KX_HD void lu_steps(float (&a)[N * N], int& fail, const W& w, long s,
+                             const int* e, const int* law, const int* rs,
+                             std::integer_sequence<int, K...>) {
+    (lu_step<K>(a, fail, w, s, e, law, rs), ...);
+  }
+
+  // mixed precision: the scaled FP32 matrix formed, factorized with
+  // partial pivoting and stored back from registers (indices fixed at
+  // compile time; the pivot row's swap by selects), so the factorization
+  // costs one load and one store of the matrix instead of a dependent
+  // chain of workspace round trips
+  static KX_HD int form_and_factor_regs(const W& w, long s, double gamma) {
+    int e[N], law[N], rs[N];
+    mixed_scaling(w, s, e, law, rs);
+    float a[N * N];
+#pragma unroll
+    for (int j = 0; j < N; j++)
+#pragma unroll
+      for (int i = 0; i < N; i++) {
+        double v = -gamma * (double)w.Jat(i + j * N, s);
+        if (i == j) v += 1.0;
+        v = ::ldexp(v, e[i] - e[j]);
+#pragma unroll
+        for (int k = 0; k < NC; k++)
+          if (law[i] == k + 1)
+            v = j < NS ? ::ldexp((double)Net::cons(k, j), rs[i] - e[j]) : 0.0;
+        v = v > 1e30 ? 1e30 : (v < -1e30 ? -1e30 : v);
+        a[i + j * N] = (float)v;
+      }
+    int fail = 0;
+    lu_steps(a, fail, w, s, e, law, rs, std::make_integer_sequence<int, N>{});
+#pragma unroll
+    for (int k = 0; k < N * N; k++) w.Aat(k, s) = a[k];
+    return fail;
   }
 
   // delta := A^-1 delta, times 2 / (1 + gamrat) when gamma changed since
@@ -747,9 +811,9 @@ struct Persist {
     (void)s;
   }
   static KX_HD void add_stats(const W& w, long s) {
-    STA(tot, 0) += ST(nst); STA(tot, 1) += ST(nfe); STA(tot, 2) += ST(nje);
-    STA(tot, 3) += ST(nfeDQ); STA(tot, 4) += ST(nsetups); STA(tot, 5) += ST(nni);
-    STA(tot, 6) += ST(nnf); STA(tot, 7) += ST(netf);
+    TOT(0) += ST(nst); TOT(1) += ST(nfe); TOT(2) += ST(nje);
+    TOT(3) += ST(nfeDQ); TOT(4) += ST(nsetups); TOT(5) += ST(nni);
+    TOT(6) += ST(nnf); TOT(7) += ST(netf);
   }
 };
 
@@ -802,13 +866,13 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
         unsigned long long pv = (unsigned long long)w0.v,
                            pj = (unsigned long long)w0.J,
                            pa = (unsigned long long)w0.A,
-                           ps = (unsigned long long)w0.st,
+                           ps = (unsigned long long)w0.tot,
                            pp = (unsigned long long)w0.piv;
         asm volatile("" : "+s"(pv), "+s"(pj), "+s"(pa), "+s"(ps), "+s"(pp));
         w.v = (double*)pv;
         w.J = (decltype(w.J))pj;
         w.A = (decltype(w.A))pa;
-        w.st = (double*)ps;
+        w.tot = (long*)ps;
         w.piv = (int*)pp;
       }
       w.sp = &lane_state;
@@ -1335,7 +1399,7 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
           a.flags[c] = flag;
           if (a.stats) {
             long* sc = a.stats + (long)c * 9;
-            for (int i = 0; i < 8; ++i) sc[i] = STA(tot, i);
+            for (int i = 0; i < 8; ++i) sc[i] = TOT(i);
             sc[8] = ST(restarts);
           }
           if (flag == 0) cell_finish<Net, ET>(a, c, y);
@@ -1356,7 +1420,7 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
             break;
           }
           ST(cell) = c;
-          for (int i = 0; i < 9; ++i) STA(tot, i) = 0;
+          for (int i = 0; i < TOT_SLOTS; ++i) TOT(i) = 0;
           ST(restarts) = 0;
           if (!(a.dt[c] > 0.0) || !supported) {
             a.flags[c] = supported ? 0 : FLAG_UNSUPPORTED;
@@ -1420,7 +1484,7 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
 template <int N, bool MIXED>
 constexpr long persist_lane_bytes() {
   return (long)sizeof(double) * ((V_COUNT - lds_vectors<N>()) * N +
-                                 LANE_SLOTS) +
+                                 TOT_SLOTS) +
          2L * N * N * (MIXED ? sizeof(float) : sizeof(double)) +
          (long)sizeof(int) * N;
 }
@@ -1466,8 +1530,8 @@ int launch_persistent_t(CellsArgs a, int wpc, hipStream_t s) {
   p += sizeof(MT) * N * N * slots;
   w.A = reinterpret_cast<MT*>(p);
   p += sizeof(MT) * N * N * slots;
-  w.st = reinterpret_cast<double*>(p);
-  p += sizeof(double) * LANE_SLOTS * slots;
+  w.tot = reinterpret_cast<long*>(p);
+  p += sizeof(long) * TOT_SLOTS * slots;
   w.piv = reinterpret_cast<int*>(p);
   hipLaunchKernelGGL(fill_flags_kernel, dim3(1), dim3(64), 0, s, 1, queue, 0);
   hipLaunchKernelGGL((persist_cells_kernel<Net, ET, MIXED>),
