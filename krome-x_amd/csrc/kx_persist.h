@@ -94,8 +94,21 @@ struct PersistWork {
     if (vec < NL) return lds[((long)vec * N + i) * KX_PERSIST_BLOCK + tid];
     return v[((long)(vec - NL) * N + i) * S + s];
   }
-  KX_HD MT& Jat(int k, long s) const { return J[(long)k * S + s]; }
-  KX_HD MT& Aat(int k, long s) const { return A[(long)k * S + s]; }
+  // mixed precision keeps J and A as pairs of FP32 elements when N is even
+  // (flattened column-major index k in pair k / 2: 8-byte accesses, the
+  // matrices moved with half the memory instructions)
+  static constexpr bool PAIRS = std::is_same<MT, float>::value && N % 2 == 0;
+  KX_HD static long mat_index(int k, long s) {
+    return PAIRS ? ((long)(k >> 1) * S + s) * 2 + (k & 1) : (long)k * S + s;
+  }
+  KX_HD MT& Jat(int k, long s) const { return J[mat_index(k, s)]; }
+  KX_HD MT& Aat(int k, long s) const { return A[mat_index(k, s)]; }
+  KX_HD float2& Jpair(int p, long s) const {
+    return reinterpret_cast<float2*>(J)[(long)p * S + s];
+  }
+  KX_HD float2& Apair(int p, long s) const {
+    return reinterpret_cast<float2*>(A)[(long)p * S + s];
+  }
 };
 
 // the scalars of a lane's machine (CVodeMem, the loop variables of the
@@ -614,7 +627,7 @@ struct Persist {
           a[i + j * N] = i == l ? tk : a[i + j * N];
       }
     }
-    const float r = best != 0.0f ? 1.0f / a[K + K * N] : 0.0f;
+    const float r = best != 0.0f ? __builtin_amdgcn_rcpf(a[K + K * N]) : 0.0f;
 #pragma unroll
     for (int i = K + 1; i < N; i++) a[i + K * N] *= r;
 #pragma unroll
@@ -632,6 +645,33 @@ struct Persist {
     (lu_step<K>(a, fail, w, s, e, law, rs), ...);
   }
 
+  // an FP32 matrix of the workspace into registers and back (pairs when
+  // the layout has them)
+  static KX_HD void load_mat(const W& w, long s, float* base, float* a) {
+    if constexpr (W::PAIRS) {
+#pragma unroll
+      for (int p = 0; p < N * N / 2; p++) {
+        const float2 v = reinterpret_cast<const float2*>(base)[(long)p * W::S + s];
+        a[2 * p] = v.x;
+        a[2 * p + 1] = v.y;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < N * N; k++) a[k] = base[(long)k * W::S + s];
+    }
+  }
+  static KX_HD void store_mat(const W& w, long s, float* base, const float* a) {
+    if constexpr (W::PAIRS) {
+#pragma unroll
+      for (int p = 0; p < N * N / 2; p++)
+        reinterpret_cast<float2*>(base)[(long)p * W::S + s] =
+            make_float2(a[2 * p], a[2 * p + 1]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < N * N; k++) base[(long)k * W::S + s] = a[k];
+    }
+  }
+
   // mixed precision: the scaled FP32 matrix formed, factorized with
   // partial pivoting and stored back from registers (indices fixed at
   // compile time; the pivot row's swap by selects), so the factorization
@@ -641,24 +681,25 @@ struct Persist {
     int e[N], law[N], rs[N];
     mixed_scaling(w, s, e, law, rs);
     float a[N * N];
+    load_mat(w, s, w.J, a);
+    // in FP32: the off-diagonal -gamma J_ij 2^(e_i - e_j) (the scaling
+    // exact); the diagonal 1 - gamma J_ii from FP64 (its scaling is 1)
+    const float gf = (float)(-gamma);
 #pragma unroll
     for (int j = 0; j < N; j++)
 #pragma unroll
       for (int i = 0; i < N; i++) {
-        double v = -gamma * (double)w.Jat(i + j * N, s);
-        if (i == j) v += 1.0;
-        v = ::ldexp(v, e[i] - e[j]);
+        float v = i == j ? (float)(1.0 - gamma * (double)a[i + j * N])
+                         : ::ldexpf(gf * a[i + j * N], e[i] - e[j]);
 #pragma unroll
         for (int k = 0; k < NC; k++)
           if (law[i] == k + 1)
-            v = j < NS ? ::ldexp((double)Net::cons(k, j), rs[i] - e[j]) : 0.0;
-        v = v > 1e30 ? 1e30 : (v < -1e30 ? -1e30 : v);
-        a[i + j * N] = (float)v;
+            v = j < NS ? ::ldexpf((float)Net::cons(k, j), rs[i] - e[j]) : 0.0f;
+        a[i + j * N] = ::fminf(::fmaxf(v, -1e30f), 1e30f);
       }
     int fail = 0;
     lu_steps(a, fail, w, s, e, law, rs, std::make_integer_sequence<int, N>{});
-#pragma unroll
-    for (int k = 0; k < N * N; k++) w.Aat(k, s) = a[k];
+    store_mat(w, s, w.A, a);
     return fail;
   }
 
@@ -716,13 +757,30 @@ struct Persist {
         b[k] = u;
       }
     }
-    for (int k = 0; k < N - 1; k++)
-      for (int i = k + 1; i < N; i++) b[i] -= w.Aat(i + k * N, s) * b[k];
-    for (int k = N - 1; k > 0; k--) {
-      b[k] /= w.Aat(k + k * N, s);
-      for (int i = 0; i < k; i++) b[i] -= w.Aat(i + k * N, s) * b[k];
+    if constexpr (MIXED) {
+      // the factors into registers (pairs), then the two triangular solves
+      float am[N * N];
+      load_mat(w, s, w.A, am);
+#pragma unroll
+      for (int k = 0; k < N - 1; k++)
+#pragma unroll
+        for (int i = k + 1; i < N; i++) b[i] -= am[i + k * N] * b[k];
+#pragma unroll
+      for (int k = N - 1; k > 0; k--) {
+        b[k] *= __builtin_amdgcn_rcpf(am[k + k * N]);
+#pragma unroll
+        for (int i = 0; i < k; i++) b[i] -= am[i + k * N] * b[k];
+      }
+      b[0] *= __builtin_amdgcn_rcpf(am[0]);
+    } else {
+      for (int k = 0; k < N - 1; k++)
+        for (int i = k + 1; i < N; i++) b[i] -= w.Aat(i + k * N, s) * b[k];
+      for (int k = N - 1; k > 0; k--) {
+        b[k] /= w.Aat(k + k * N, s);
+        for (int i = 0; i < k; i++) b[i] -= w.Aat(i + k * N, s) * b[k];
+      }
+      b[0] /= w.Aat(0, s);
     }
-    b[0] /= w.Aat(0, s);
     const double c = gamrat != 1.0 ? 2.0 / (1.0 + gamrat) : 1.0;
 #pragma unroll
     for (int i = 0; i < N; i++) {
