@@ -60,7 +60,7 @@ enum Pc : int {
 // (the first ones, the Nordsieck array, the error weights and the
 // accumulated correction, in LDS when they fit; the others in HBM)
 enum Vec : int {
-  V_ZN0 = 0, V_EWT = 6, V_ACOR, V_Y, V_TEMPV, V_FTEMP, V_DELTA, V_VT1,
+  V_ZN0 = 0, V_EWT = 6, V_ACOR, V_Y, V_TEMPV, V_FTEMP, V_VT1,
   V_COUNT
 };
 // the LDS of a workgroup: 160 KB on gfx950, KX_PERSIST_BLOCK lanes
@@ -251,30 +251,66 @@ struct Persist {
   }
 
   // ---- the BDF bookkeeping (bdf_cell.hpp:825-954, 1386-1488) ------------
+  // The Nordsieck operations below run element by element with the
+  // element's history in registers (each element of the array is read and
+  // written once), the same operations in the same order per element as
+  // the vector forms of bdf_cell.hpp.
   static KX_HD void rescale(const W& w, long s) {
-    double c = ST(eta);
+    double c[QMAX + 1];
+    c[1] = ST(eta);
 #pragma unroll
-    for (int j = 1; j <= QMAX; j++)
-      if (j <= ST(q)) {
-        scal(w, s, c, j, j);
-        c = ST(eta) * c;
-      }
+    for (int j = 2; j <= QMAX; j++) c[j] = ST(eta) * c[j - 1];
+    const int q = ST(q);
+#pragma unroll
+    for (int i = 0; i < N; i++)
+#pragma unroll
+      for (int j = 1; j <= QMAX; j++)
+        if (j <= q) w.at(j, i, s) = c[j] * w.at(j, i, s);
     ST(h) = ST(hscale) * ST(eta);
     ST(next_h) = ST(h);
     ST(hscale) = ST(h);
     ST(nscon) = 0;
   }
+  // the Newton residual delta = rl1 zn[1] + acor - gamma ftemp: the two
+  // N_VLinearSum calls of cvNlsResid (bdf_cell.hpp:1029-1030), their
+  // special cases included, fused element by element
+  static KX_HD void residual(const W& w, long s, double rl1, double gamma,
+                             double* dl) {
+    const double ng = -gamma;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      const double x = w.at(1, i, s), y = w.at(V_ACOR, i, s);
+      const double f = w.at(V_FTEMP, i, s);
+      const double t = rl1 == 1.0 ? x + y : (rl1 == -1.0 ? y - x : rl1 * x + y);
+      dl[i] = ng == 1.0 ? t + f : (ng == -1.0 ? t - f : t + ng * f);
+    }
+  }
+  // zn[j-1] += zn[j] (predict) or -= (restore) for k = 1..q, j = q..k
+  template <bool ADD>
+  static KX_HD void pascal(const W& w, long s) {
+    const int q = ST(q);
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      double z[QMAX + 1];
+#pragma unroll
+      for (int j = 0; j <= QMAX; j++) z[j] = j <= q ? w.at(j, i, s) : 0.0;
+#pragma unroll
+      for (int k = 1; k <= QMAX; k++)
+#pragma unroll
+        for (int j = QMAX; j >= k; j--)
+          if (k <= q && j <= q) z[j - 1] = ADD ? z[j - 1] + z[j] : z[j - 1] - z[j];
+#pragma unroll
+      for (int j = 0; j < QMAX; j++)
+        if (j < q) w.at(j, i, s) = z[j];
+    }
+  }
   static KX_HD void predict(const W& w, long s) {
     ST(tn) += ST(h);
-    for (int k = 1; k <= QMAX; k++)
-      for (int j = QMAX; j >= k; j--)
-        if (k <= ST(q) && j <= ST(q)) axpy(w, s, 1.0, j, j - 1);
+    pascal<true>(w, s);
   }
   static KX_HD void restore(const W& w, long s) {
     ST(tn) = ST(saved_t);
-    for (int k = 1; k <= QMAX; k++)
-      for (int j = QMAX; j >= k; j--)
-        if (k <= ST(q) && j <= ST(q)) axpy(w, s, -1.0, j, j - 1);
+    pascal<false>(w, s);
   }
   static KX_HD void order_up(const W& w, long s) {
 #pragma unroll
@@ -408,9 +444,16 @@ struct Persist {
       if (i <= ST(q)) STA(tau, i) = STA(tau, i - 1);
     if (ST(q) == 1 && ST(nst) > 1) STA(tau, 2) = STA(tau, 1);
     STA(tau, 1) = ST(h);
+    {
+      const int q = ST(q);
 #pragma unroll
-    for (int j = 0; j <= QMAX; j++)
-      if (j <= ST(q)) axpy(w, s, STA(l, j), V_ACOR, j);
+      for (int i = 0; i < N; i++) {
+        const double a = w.at(V_ACOR, i, s);
+#pragma unroll
+        for (int j = 0; j <= QMAX; j++)
+          if (j <= q) w.at(j, i, s) += STA(l, j) * a;
+      }
+    }
     ST(qwait)--;
     if (ST(qwait) == 1 && ST(q) != QMAX) {
       for (int i = 0; i < N; i++) w.at(QMAX, i, s) = w.at(V_ACOR, i, s);
@@ -708,7 +751,7 @@ struct Persist {
   // side of the factored matrix (rows weighted by 2^e_i, the replaced
   // rows c_k^T b scaled), scaled by its largest magnitude into FP32,
   // solved, and the solution unweighted by 2^-e_i.
-  static KX_HD void lsolve(const W& w, long s, double gamrat) {
+  static KX_HD void lsolve(const W& w, long s, double gamrat, double* dl) {
     MT b[N];
     int pv[N], e[N], law[N], rs[N];
 #pragma unroll
@@ -727,12 +770,12 @@ struct Persist {
         double t = 0.0;
 #pragma unroll
         for (int j = 0; j < NS; j++)
-          if (Net::cons(k, j) != 0) t += Net::cons(k, j) * w.at(V_DELTA, j, s);
+          if (Net::cons(k, j) != 0) t += Net::cons(k, j) * dl[j];
         cb[k] = t;
       }
 #pragma unroll
       for (int i = 0; i < N; i++) {
-        double v = ::ldexp(w.at(V_DELTA, i, s), e[i]);
+        double v = ::ldexp(dl[i], e[i]);
 #pragma unroll
         for (int k = 0; k < NC; k++)
           if (law[i] == k + 1) v = ::ldexp(cb[k], rs[i]);
@@ -744,7 +787,7 @@ struct Persist {
       for (int i = 0; i < N; i++) b[i] = MT(bs[i] / sc);
     } else {
 #pragma unroll
-      for (int i = 0; i < N; i++) b[i] = MT(w.at(V_DELTA, i, s));
+      for (int i = 0; i < N; i++) b[i] = MT(dl[i]);
     }
 #pragma unroll
     for (int k = 0; k < N; k++) {
@@ -786,24 +829,24 @@ struct Persist {
     for (int i = 0; i < N; i++) {
       double x = double(b[i]);
       if (MIXED) x = ::ldexp(sc * x, -e[i]);
-      w.at(V_DELTA, i, s) = gamrat != 1.0 ? c * x : x;
+      dl[i] = gamrat != 1.0 ? c * x : x;
     }
   }
 
   // the charge invariant of the corrections (bdf_cell.hpp:1267-1287)
   // (the charges c over the species; the electrons' row r)
-  static KX_HD double inv_dot(const W& w, long s) {
+  static KX_HD double inv_dot(const double* dl) {
     double sum = 0.0;
     for (int i = 0; i < NS; i++)
-      if (Net::charge(i) != 0) sum += Net::charge(i) * w.at(V_DELTA, i, s);
+      if (Net::charge(i) != 0) sum += Net::charge(i) * dl[i];
     return sum;
   }
-  static KX_HD void inv_set(const W& w, long s, double target) {
+  static KX_HD void inv_set(const W& w, long s, double target, double* dl) {
     constexpr int r = Net::I_E;
     double sum = 0.0, sa = 0.0, yr = 0.0, cr = 0.0;
     for (int i = 0; i < NS; i++) {
       if (i != r && Net::charge(i) != 0) {
-        sum += Net::charge(i) * w.at(V_DELTA, i, s);
+        sum += Net::charge(i) * dl[i];
         sa += ::fabs(Net::charge(i) * (w.at(0, i, s) + w.at(V_ACOR, i, s)));
       }
       if (i == r) {
@@ -812,12 +855,20 @@ struct Persist {
       }
     }
     const double xr = (target - sum) / cr;
-    if (::fabs(cr * yr) >= 1e-6 * sa && yr + xr >= 0.0) w.at(V_DELTA, r, s) = xr;
+    if (::fabs(cr * yr) >= 1e-6 * sa && yr + xr >= 0.0) dl[r] = xr;
   }
   // cvNlsConvTest (bdf_cell.hpp:1035-1048)
-  static KX_HD int conv_test(const W& w, long s, int m) {
+  static KX_HD int conv_test(const W& w, long s, int m, const double* dl) {
     using namespace bdfc;
-    ST(delnrm) = wrms(w, s, V_DELTA);
+    {
+      double sum = 0.0;
+#pragma unroll
+      for (int i = 0; i < N; i++) {
+        const double p = dl[i] * w.at(V_EWT, i, s);
+        sum += p * p;
+      }
+      ST(delnrm) = rt(sum / N);
+    }
     if (m > 0) ST(crate) = mx(CRDOWN * ST(crate), ST(delnrm) / ST(delp));
     const double dcon = ST(delnrm) * mn(1.0, ST(crate)) / ST(tol);
     if (dcon <= 1.0) {
@@ -1047,10 +1098,8 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
       if (pc == P_RESID_RET) do {
           ST(nfe)++;
           const int r = rr < 0 ? RHSFUNC_FAIL : (rr > 0 ? RHSFUNC_RECVR : BDF_OK);
-          if (r == BDF_OK) {
-            M::lsum(w, s, ST(rl1), 1, 1.0, V_ACOR, V_DELTA);
-            M::lsum(w, s, -ST(gamma), V_FTEMP, 1.0, V_DELTA, V_DELTA);
-          }
+          // the residual itself is formed where it is used (P_INNER_ITER),
+          // from ftemp, zn[1] and acor, which stay as they are until then
           if (ST(first_resid)) {
             if (r != BDF_OK) {
               ST(nls_r) = r;
@@ -1199,12 +1248,18 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
       } while (0);
       if (pc == P_INNER_ITER) do {
           ST(niters)++;
-          ST(cg) = inv ? M::inv_dot(w, s) : 0.0;
-          M::scal(w, s, -1.0, V_DELTA, V_DELTA);
-          M::lsolve(w, s, ST(gamrat));
-          if (inv) M::inv_set(w, s, -ST(cg));
-          M::lsum(w, s, 1.0, V_ACOR, 1.0, V_DELTA, V_ACOR);
-          const int r = M::conv_test(w, s, ST(curiter));
+          // the Newton correction in registers: residual, solve, charge
+          // invariant, accumulation and convergence test in one stage
+          double dl[N];
+          M::residual(w, s, ST(rl1), ST(gamma), dl);
+          ST(cg) = inv ? M::inv_dot(dl) : 0.0;
+#pragma unroll
+          for (int i = 0; i < N; i++) dl[i] = -1.0 * dl[i];
+          M::lsolve(w, s, ST(gamrat), dl);
+          if (inv) M::inv_set(w, s, -ST(cg), dl);
+#pragma unroll
+          for (int i = 0; i < N; i++) w.at(V_ACOR, i, s) += dl[i];
+          const int r = M::conv_test(w, s, ST(curiter), dl);
           KX_TRACE_REC2(5.0, ST(delnrm), ST(crate));
           ST(curiter)++;
           if (r == BDF_OK) {
