@@ -960,7 +960,9 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
   P.hit_T0 = false;
   // the rhs request of this lane: source vector, destination vector,
   // precision (0: FP64, 1: FP32), and its return code
-  int rq_src = 0, rq_dst = 0, rr = 0;
+  int rq_dst = 0, rr = 0;
+  // the state the rhs is requested at, handed to the rhs in registers
+  double yq[N];
   bool want = false;
 
   for (;;) {
@@ -1075,8 +1077,6 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
             pc = P_LSETUP_FORM;
             break;
           }
-          for (int i = 0; i < N; i++)
-            if (i == j) w.at(V_Y, i, s) = ST(ysaved);
           // lin_sum(ii, vt1, -ii, ftemp): the a == -b form.  The
           // differences are FP64 in both precisions; mixed precision stores
           // them in FP32 (FP32 differences, with their sqrt(FLT_EPSILON)
@@ -1159,8 +1159,16 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
           break;
       } while (0);
       if (pc == P_HIN_REQ) do {
-          M::lsum(w, s, ST(hin_hg), 1, 1.0, 0, V_Y);
-          rq_src = V_Y; rq_dst = V_TEMPV; want = true;
+          // y = hg zn[1] + zn[0] (N_VLinearSum with b = 1)
+          {
+            const double a = ST(hin_hg);
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+              const double x = w.at(1, i, s), y = w.at(0, i, s);
+              yq[i] = a == 1.0 ? x + y : (a == -1.0 ? y - x : a * x + y);
+            }
+          }
+          rq_dst = V_TEMPV; want = true;
           pc = P_HIN_RET;
           break;
       } while (0);
@@ -1208,19 +1216,24 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
           break;
       } while (0);
       if (pc == P_DQ_REQ) do {
+          // the perturbed state in registers only (y itself unchanged)
           const int j = ST(jcol);
           const double srur = rt(DBL_EPSILON);
           double ys = 0.0, ew = 0.0;
-          for (int i = 0; i < N; i++)
+#pragma unroll
+          for (int i = 0; i < N; i++) {
+            yq[i] = w.at(V_Y, i, s);
             if (i == j) {
-              ys = w.at(V_Y, i, s);
+              ys = yq[i];
               ew = w.at(V_EWT, i, s);
             }
+          }
           ST(ysaved) = ys;
           ST(inc) = mx(srur * ::fabs(ys), ST(minInc) / ew);
+#pragma unroll
           for (int i = 0; i < N; i++)
-            if (i == j) w.at(V_Y, i, s) += ST(inc);
-          rq_src = V_Y; rq_dst = V_VT1; want = true;
+            if (i == j) yq[i] += ST(inc);
+          rq_dst = V_VT1; want = true;
           pc = P_DQ_RET;
           break;
       } while (0);
@@ -1379,7 +1392,9 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
           ST(hscale) = ST(h);
           ST(qwait) = LONG_WAIT;
           ST(nscon) = 0;
-          rq_src = 0; rq_dst = V_TEMPV; want = true;
+#pragma unroll
+          for (int i = 0; i < N; i++) yq[i] = w.at(0, i, s);
+          rq_dst = V_TEMPV; want = true;
           pc = P_ERRTEST_RHS;
           break;
       } while (0);
@@ -1460,8 +1475,13 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
           break;
       } while (0);
       if (pc == P_RESID_REQ) do {
-          M::lsum(w, s, 1.0, 0, 1.0, V_ACOR, V_Y);
-          rq_src = V_Y; rq_dst = V_FTEMP; want = true;
+          // y = zn[0] + acor, kept for a Jacobian and handed to the rhs
+#pragma unroll
+          for (int i = 0; i < N; i++) {
+            yq[i] = w.at(0, i, s) + w.at(V_ACOR, i, s);
+            w.at(V_Y, i, s) = yq[i];
+          }
+          rq_dst = V_FTEMP; want = true;
           pc = P_RESID_RET;
           break;
       } while (0);
@@ -1567,7 +1587,9 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
             pc = P_SOLVE_END;
             break;
           }
-          rq_src = 0; rq_dst = 1; want = true;
+#pragma unroll
+          for (int i = 0; i < N; i++) yq[i] = w.at(0, i, s);
+          rq_dst = 1; want = true;
           pc = P_FIRST_RHS;
           break;
       } while (0);
@@ -1584,10 +1606,8 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
       }
       w.lds = lds_vec;
       w.tid = threadIdx.x;
-      double yr[N];
-      for (int i = 0; i < N; i++) yr[i] = w.at(rq_src, i, s);
       double fr[N];
-      rr = P.template rhs_r<double>(yr, fr);
+      rr = P.template rhs_r<double>(yq, fr);
       for (int i = 0; i < N; i++) w.at(rq_dst, i, s) = fr[i];
     }
   }
