@@ -935,7 +935,7 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
                                                typename std::conditional<
                                                    MIXED, float,
                                                    double>::type> w0,
-                     int* queue) {
+                     int* queue, const int* order) {
   using M = Persist<Net, ET, MIXED>;
   using Prob = Problem<Net, ET>;
   constexpr int N = M::N;
@@ -1547,11 +1547,11 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
           int base = 0;
           if (lane == leader) base = atomicAdd(queue, __popcll(need));
           base = __shfl(base, leader);
-          const int c = base + rank;
-          if (c >= a.ncell) {
+          if (base + rank >= a.ncell) {
             pc = P_IDLE;
             break;
           }
+          const int c = order[base + rank];
           ST(cell) = c;
           for (int i = 0; i < TOT_SLOTS; ++i) TOT(i) = 0;
           ST(restarts) = 0;
@@ -1613,6 +1613,47 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
   }
 }
 
+// ---- the queue's order: the densest cells first ---------------------------
+// A cell's work grows with its density (its reaction rates, and so its
+// stiffness, scale with it; for primordial3 collapse cells the correlation
+// of log(steps) with log(nH) is 0.86).  Handing the expensive cells out
+// first leaves the cheap ones for the end of the launch, where a late
+// expensive cell would keep its wave, and the launch, running alone.  A
+// counting sort over half-octave buckets of the total number density.
+constexpr int ORDER_BUCKETS = 256;
+
+template <class Net>
+__global__ void order_keys_kernel(CellsArgs a, int* key, int* hist) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.ncell) return;
+  double nt = 0.0;
+  for (int i = 0; i < Net::NSPEC; i++)
+    nt += ::fabs(a.n[(long)c * a.n_cs + i * a.n_ss]);
+  int b = (int)::floor(2.0 * ::log2(::fmax(nt, 1e-300))) + 128;
+  b = b < 0 ? 0 : (b > ORDER_BUCKETS - 1 ? ORDER_BUCKETS - 1 : b);
+  b = ORDER_BUCKETS - 1 - b;  // densest first
+  key[c] = b;
+  atomicAdd(&hist[b], 1);
+}
+
+// exclusive prefix sums of the bucket counts (one thread: 256 buckets)
+__global__ inline void order_scan_kernel(int* hist) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  int sum = 0;
+  for (int b = 0; b < ORDER_BUCKETS; b++) {
+    const int v = hist[b];
+    hist[b] = sum;
+    sum += v;
+  }
+}
+
+__global__ inline void order_scatter_kernel(int ncell, const int* key,
+                                            int* off, int* order) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  order[atomicAdd(&off[key[c]], 1)] = c;
+}
+
 // bytes of the persistent kernel's workspace per lane
 template <int N, bool MIXED>
 constexpr long persist_lane_bytes() {
@@ -1666,9 +1707,33 @@ int launch_persistent_t(CellsArgs a, int wpc, hipStream_t s) {
   w.tot = reinterpret_cast<long*>(p);
   p += sizeof(long) * TOT_SLOTS * slots;
   w.piv = reinterpret_cast<int*>(p);
+  // the queue's order (keys, bucket counts and offsets, the permutation)
+  static thread_local int* ord = nullptr;
+  static thread_local long ord_n = 0;
+  if (a.ncell > ord_n) {
+    if (ord) (void)hipFree(ord);
+    ord = nullptr;
+    ord_n = 0;
+    if (hipMalloc(&ord, sizeof(int) * (2L * a.ncell + ORDER_BUCKETS)) !=
+        hipSuccess)
+      return FLAG_DEVICE_MEMORY;
+    ord_n = a.ncell;
+  }
+  int* hist = ord;
+  int* key = ord + ORDER_BUCKETS;
+  int* order = key + a.ncell;
+  const int cb = (a.ncell + 255) / 256;
   hipLaunchKernelGGL(fill_flags_kernel, dim3(1), dim3(64), 0, s, 1, queue, 0);
+  hipLaunchKernelGGL(fill_flags_kernel, dim3(1), dim3(ORDER_BUCKETS), 0, s,
+                     ORDER_BUCKETS, hist, 0);
+  hipLaunchKernelGGL(order_keys_kernel<Net>, dim3(cb), dim3(256), 0, s, a,
+                     key, hist);
+  hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(64), 0, s, hist);
+  hipLaunchKernelGGL(order_scatter_kernel, dim3(cb), dim3(256), 0, s,
+                     a.ncell, key, hist, order);
   hipLaunchKernelGGL((persist_cells_kernel<Net, ET, MIXED>),
-                     dim3(lanes / B), dim3(B), 0, s, a, w, queue);
+                     dim3(lanes / B), dim3(B), 0, s, a, w, queue,
+                     (const int*)order);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
