@@ -1619,8 +1619,12 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
 // of log(steps) with log(nH) is 0.86).  Handing the expensive cells out
 // first leaves the cheap ones for the end of the launch, where a late
 // expensive cell would keep its wave, and the launch, running alone.  A
-// counting sort over half-octave buckets of the total number density.
-constexpr int ORDER_BUCKETS = 256;
+// counting sort over half-octave buckets of the total number density, and
+// within those over quarter-octave buckets of the temperature, so that the
+// lanes of a wave start on cells in the same branches of the piecewise
+// rate and cooling fits.
+constexpr int ORDER_NB = 256, ORDER_TB = 64;
+constexpr int ORDER_BUCKETS = ORDER_NB * ORDER_TB;
 
 template <class Net>
 __global__ void order_keys_kernel(CellsArgs a, int* key, int* hist) {
@@ -1630,13 +1634,16 @@ __global__ void order_keys_kernel(CellsArgs a, int* key, int* hist) {
   for (int i = 0; i < Net::NSPEC; i++)
     nt += ::fabs(a.n[(long)c * a.n_cs + i * a.n_ss]);
   int b = (int)::floor(2.0 * ::log2(::fmax(nt, 1e-300))) + 128;
-  b = b < 0 ? 0 : (b > ORDER_BUCKETS - 1 ? ORDER_BUCKETS - 1 : b);
-  b = ORDER_BUCKETS - 1 - b;  // densest first
+  b = b < 0 ? 0 : (b > ORDER_NB - 1 ? ORDER_NB - 1 : b);
+  b = ORDER_NB - 1 - b;  // densest first
+  int t = (int)::floor(4.0 * ::log2(::fmax(a.T[c], 1.0)));
+  t = t < 0 ? 0 : (t > ORDER_TB - 1 ? ORDER_TB - 1 : t);
+  b = b * ORDER_TB + t;
   key[c] = b;
   atomicAdd(&hist[b], 1);
 }
 
-// exclusive prefix sums of the bucket counts (one thread: 256 buckets)
+// exclusive prefix sums of the bucket counts (one thread)
 __global__ inline void order_scan_kernel(int* hist) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   int sum = 0;
@@ -1724,8 +1731,8 @@ int launch_persistent_t(CellsArgs a, int wpc, hipStream_t s) {
   int* order = key + a.ncell;
   const int cb = (a.ncell + 255) / 256;
   hipLaunchKernelGGL(fill_flags_kernel, dim3(1), dim3(64), 0, s, 1, queue, 0);
-  hipLaunchKernelGGL(fill_flags_kernel, dim3(1), dim3(ORDER_BUCKETS), 0, s,
-                     ORDER_BUCKETS, hist, 0);
+  hipLaunchKernelGGL(fill_flags_kernel, dim3(ORDER_BUCKETS / 256), dim3(256),
+                     0, s, ORDER_BUCKETS, hist, 0);
   hipLaunchKernelGGL(order_keys_kernel<Net>, dim3(cb), dim3(256), 0, s, a,
                      key, hist);
   hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(64), 0, s, hist);
