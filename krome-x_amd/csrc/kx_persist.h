@@ -481,7 +481,7 @@ struct Persist {
       ST(eta) = 1.0;
       return;
     }
-    ST(etaq) = 1.0 / (::pow(BIAS2 * ST(dsm), 1.0 / ST(L)) + ADDON);
+    ST(etaq) = 1.0 / (kx::pow(BIAS2 * ST(dsm), 1.0 / ST(L)) + ADDON);
     if (ST(qwait) != 0) {
       ST(eta) = ST(etaq);
       ST(qprime) = ST(q);
@@ -492,7 +492,7 @@ struct Persist {
     ST(etaqm1) = 0.0;
     if (ST(q) > 1) {
       const double ddn = wrms(w, s, ST(q)) * STA(tq, 1);
-      ST(etaqm1) = 1.0 / (::pow(BIAS1 * ddn, 1.0 / ST(q)) + ADDON);
+      ST(etaqm1) = 1.0 / (kx::pow(BIAS1 * ddn, 1.0 / ST(q)) + ADDON);
     }
     ST(etaqp1) = 0.0;
     if (ST(q) != QMAX && ST(saved_tq5) != 0.0) {
@@ -504,7 +504,7 @@ struct Persist {
       const double cquot = (STA(tq, 5) / ST(saved_tq5)) * p;
       lsum(w, s, -cquot, QMAX, 1.0, V_ACOR, V_TEMPV);
       const double dup = wrms(w, s, V_TEMPV) * STA(tq, 3);
-      ST(etaqp1) = 1.0 / (::pow(BIAS3 * dup, 1.0 / (ST(L) + 1)) + ADDON);
+      ST(etaqp1) = 1.0 / (kx::pow(BIAS3 * dup, 1.0 / (ST(L) + 1)) + ADDON);
     }
     const double etam = mx(ST(etaqm1), mx(ST(etaq), ST(etaqp1)));
     if (etam > ETA_MIN_FX && etam < ETA_MAX_FX) {
@@ -1369,7 +1369,7 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
           }
           ST(etamax) = 1.0;
           if (ST(nef) <= MXNEF1) {
-            ST(eta) = 1.0 / (::pow(BIAS2 * dsm, 1.0 / ST(L)) + ADDON);
+            ST(eta) = 1.0 / (kx::pow(BIAS2 * dsm, 1.0 / ST(L)) + ADDON);
             ST(eta) = mx(ETA_MIN_EF, mx(ST(eta), 0.0));
             if (ST(nef) >= SMALL_NEF) ST(eta) = mn(ST(eta), ETA_MAX_EF);
             M::rescale(w, s);
