@@ -275,12 +275,12 @@ struct Persist {
   // N_VLinearSum calls of cvNlsResid (bdf_cell.hpp:1029-1030), their
   // special cases included, fused element by element
   static KX_HD void residual(const W& w, long s, double rl1, double gamma,
-                             double* dl) {
+                             const double* fq, double* dl) {
     const double ng = -gamma;
 #pragma unroll
     for (int i = 0; i < N; i++) {
       const double x = w.at(1, i, s), y = w.at(V_ACOR, i, s);
-      const double f = w.at(V_FTEMP, i, s);
+      const double f = fq[i];
       const double t = rl1 == 1.0 ? x + y : (rl1 == -1.0 ? y - x : rl1 * x + y);
       dl[i] = ng == 1.0 ? t + f : (ng == -1.0 ? t - f : t + ng * f);
     }
@@ -961,8 +961,10 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
   // the rhs request of this lane: source vector, destination vector,
   // precision (0: FP64, 1: FP32), and its return code
   int rq_dst = 0, rr = 0;
-  // the state the rhs is requested at, handed to the rhs in registers
-  double yq[N];
+  // the state the rhs is requested at, handed to the rhs in registers;
+  // the last Newton-residual rhs f(y), kept in registers across ticks
+  // until the next one (the residual and the difference quotients read it)
+  double yq[N], fq[N];
   bool want = false;
 
   for (;;) {
@@ -1085,7 +1087,7 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
           const double ii = 1.0 / ST(inc);
           for (int i = 0; i < N; i++)
             w.Jat(i + j * N, s) = typename M::MT(
-                ii * (w.at(V_VT1, i, s) - w.at(V_FTEMP, i, s)));
+                ii * (w.at(V_VT1, i, s) - fq[i]));
           ST(jcol)++;
           if (ST(jcol) < N) {
             pc = P_DQ_REQ;
@@ -1207,7 +1209,16 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
           break;
       } while (0);
       if (pc == P_DQ_BEGIN) do {  // cvLsDenseDQJac (bdf_cell.hpp:1290-1311)
-          const double fn = M::wrms(w, s, V_FTEMP);
+          double fn = 0.0;
+          {
+            double sum = 0.0;
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+              const double p = fq[i] * w.at(V_EWT, i, s);
+              sum += p * p;
+            }
+            fn = rt(sum / N);
+          }
           ST(minInc) = fn != 0.0 ? (MIN_INC_MULT * ::fabs(ST(h)) *
                                    DBL_EPSILON * (double)N * fn)
                                 : 1.0;
@@ -1264,7 +1275,7 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
           // the Newton correction in registers: residual, solve, charge
           // invariant, accumulation and convergence test in one stage
           double dl[N];
-          M::residual(w, s, ST(rl1), ST(gamma), dl);
+          M::residual(w, s, ST(rl1), ST(gamma), fq, dl);
           ST(cg) = inv ? M::inv_dot(dl) : 0.0;
 #pragma unroll
           for (int i = 0; i < N; i++) dl[i] = -1.0 * dl[i];
@@ -1608,7 +1619,12 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
       w.tid = threadIdx.x;
       double fr[N];
       rr = P.template rhs_r<double>(yq, fr);
-      for (int i = 0; i < N; i++) w.at(rq_dst, i, s) = fr[i];
+      if (rq_dst == V_FTEMP) {
+#pragma unroll
+        for (int i = 0; i < N; i++) fq[i] = fr[i];
+      } else {
+        for (int i = 0; i < N; i++) w.at(rq_dst, i, s) = fr[i];
+      }
     }
   }
 }
