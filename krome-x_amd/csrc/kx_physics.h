@@ -30,21 +30,40 @@
 
 namespace kx {
 
-// the math of the rhs under one name
+// the math of the rhs under one name.  On the GPU every logarithm is
+// log2 (scaled by a constant): the rhs takes log, log10 and powers of the
+// same few arguments (Tgas, Te), and one log2 per argument then serves
+// them all; a logarithm of a constant folds at compile time.  Each form
+// differs from the C library's by a rounding or two (relative 1e-15, or
+// |y log2 x| 2^-52 for a power: under 1e-14 for the fits' arguments).
+// The host build (the oracle) keeps the C library's functions.
+#if defined(__HIP_DEVICE_COMPILE__)
+KX_HD inline double log2_(double x) {
+  return __builtin_constant_p(x) ? __builtin_log2(x) : ::log2(x);
+}
+#endif
 KX_HD inline double exp(double x) { return ::exp(x); }
-KX_HD inline double log(double x) { return ::log(x); }
-KX_HD inline double log10(double x) { return ::log10(x); }
+KX_HD inline double log(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return log2_(x) * 0.69314718055994530942;
+#else
+  return ::log(x);
+#endif
+}
+KX_HD inline double log10(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return log2_(x) * 0.30102999566398119521;
+#else
+  return ::log10(x);
+#endif
+}
 KX_HD inline double sqrt(double x) { return ::sqrt(x); }
-// x^y.  On the GPU as 2^(y log2 x): the rhs raises the same few bases
-// (Tgas, Te, 10) to many exponents, so log2 of a base is shared across its
-// powers, and each power is one exp2 instead of a full pow (relative
-// difference from pow ~ |y log2 x| 2^-52, under 1e-14 for the fits'
-// arguments).  The rhs's bases are positive (temperatures, densities,
-// 1 + ...); x = 0 gives pow's 0 or inf for y != 0, and y = 0 gives 1.
-// The host build (the oracle) keeps the C library's pow.
+// x^y, on the GPU as 2^(y log2 x).  The rhs's bases are positive
+// (temperatures, densities, 1 + ...); x = 0 gives pow's 0 or inf for
+// y != 0, and y = 0 gives 1.
 KX_HD inline double pow(double x, double y) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  return y == 0.0 ? 1.0 : ::exp2(y * ::log2(x));
+  return y == 0.0 ? 1.0 : ::exp2(y * log2_(x));
 #else
   return ::pow(x, y);
 #endif
