@@ -60,9 +60,12 @@ enum Pc : int {
 // (the first ones, the Nordsieck array, the error weights and the
 // accumulated correction, in LDS when they fit; the others in HBM)
 enum Vec : int {
-  V_ZN0 = 0, V_EWT = 6, V_ACOR, V_Y, V_TEMPV, V_FTEMP, V_VT1,
+  V_ZN0 = 0, V_EWT = 6, V_ACOR, V_Y, V_TEMPV, V_VT1,
   V_COUNT
 };
+// the rhs destination that is not a workspace vector: the Newton
+// residual's f(y), kept in registers (persist_cells_kernel, fq)
+constexpr int RQ_FTEMP = -1;
 // the LDS of a workgroup: 160 KB on gfx950, KX_PERSIST_BLOCK lanes
 #ifndef KX_PERSIST_BLOCK
 #define KX_PERSIST_BLOCK 256
@@ -125,7 +128,7 @@ struct LaneState {
   int nscon, nstlp, nstlj;
   int acnrmcur, jcur, nls_jcur;
   // loop variables of cvStep / cvNls / cvHin / the DQ Jacobian
-  double saved_t, dsm, tol, cg, ysaved, inc, minInc, tret;
+  double saved_t, dsm, tol, cg, inc, minInc, tret;
   double hin_hg, hin_hs, hin_hnew, hin_hub, hin_hlb;
   int ncf, nef, nflag, niters, nconvfails, curiter, first_resid, callSetup,
       jbad, jcol, jr, count1, count2, nls_r;
@@ -1239,7 +1242,6 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
               ew = w.at(V_EWT, i, s);
             }
           }
-          ST(ysaved) = ys;
           ST(inc) = mx(srur * ::fabs(ys), ST(minInc) / ew);
 #pragma unroll
           for (int i = 0; i < N; i++)
@@ -1492,7 +1494,7 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
             yq[i] = w.at(0, i, s) + w.at(V_ACOR, i, s);
             w.at(V_Y, i, s) = yq[i];
           }
-          rq_dst = V_FTEMP; want = true;
+          rq_dst = RQ_FTEMP; want = true;
           pc = P_RESID_RET;
           break;
       } while (0);
@@ -1619,7 +1621,7 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
       w.tid = threadIdx.x;
       double fr[N];
       rr = P.template rhs_r<double>(yq, fr);
-      if (rq_dst == V_FTEMP) {
+      if (rq_dst == RQ_FTEMP) {
 #pragma unroll
         for (int i = 0; i < N; i++) fq[i] = fr[i];
       } else {
