@@ -600,47 +600,91 @@ struct Persist {
     }
   }
 
+  // networks of up to REG_LU_N equations factorize in registers; larger
+  // ones in place in the workspace (the register arrays would spill)
+  static constexpr int REG_LU_N = 10;
   static KX_HD int form_and_factor(const W& w, long s, double gamma) {
-    if constexpr (MIXED) {
+    if constexpr (N > REG_LU_N) {
+      return form_and_factor_mem(w, s, gamma);
+    } else if constexpr (MIXED) {
       return form_and_factor_regs(w, s, gamma);
     } else {
-      // FP64: I - gamma J and its LU in place in the workspace, the
-      // reference's operations in its order
+      // FP64: I - gamma J and its LU with the reference's operations in
+      // its order, in registers (one load and one store of the matrix)
+      double a[N * N];
+#pragma unroll
+      for (int k = 0; k < N * N; k++) a[k] = w.Jat(k, s);
+      const double c = -gamma;
+#pragma unroll
+      for (int j = 0; j < N; j++) {
+#pragma unroll
+        for (int i = 0; i < N; i++) a[i + j * N] = a[i + j * N] * c;
+        a[j + j * N] += 1.0;
+      }
+      int fail = 0;
+      lu64_steps(a, fail, w, s, std::make_integer_sequence<int, N>{});
+#pragma unroll
+      for (int k = 0; k < N * N; k++) w.Aat(k, s) = a[k];
+      return fail;
+    }
+  }
+
+  // the Newton matrix formed and factorized in place in the workspace
+  // (either precision; mixed: the scaled matrix of form_and_factor_regs)
+  static KX_HD int form_and_factor_mem(const W& w, long s, double gamma) {
+    int e[N], law[N], rs[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) e[i] = law[i] = rs[i] = 0;
+    if constexpr (MIXED) {
+      mixed_scaling(w, s, e, law, rs);
+      const float gf = (float)(-gamma);
+      for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++) {
+          const float jv = w.Jat(i + j * N, s);
+          float v = i == j ? (float)(1.0 - gamma * (double)jv)
+                           : ::ldexpf(gf * jv, e[i] - e[j]);
+          for (int k = 0; k < NC; k++)
+            if (law[i] == k + 1)
+              v = j < NS ? ::ldexpf((float)Net::cons(k, j), rs[i] - e[j])
+                         : 0.0f;
+          w.Aat(i + j * N, s) = ::fminf(::fmaxf(v, -1e30f), 1e30f);
+        }
+    } else {
       const MT c = MT(-gamma);
       for (int j = 0; j < N; j++) {
         for (int i = 0; i < N; i++)
           w.Aat(i + j * N, s) = w.Jat(i + j * N, s) * c;
         w.Aat(j + j * N, s) += MT(1);
       }
-      for (int k = 0; k < N; k++) {
-        int l_ = k;
-        MT best = kx::fabs(w.Aat(k + k * N, s));
-        for (int i = k + 1; i < N; i++) {
-          const MT v = kx::fabs(w.Aat(i + k * N, s));
-          if (v > best) {
-            best = v;
-            l_ = i;
-          }
-        }
-        w.piv[(long)k * w.S + s] = l_;
-        if (w.Aat(l_ + k * N, s) == MT(0)) return k + 1;
-        if (l_ != k)
-          for (int j = 0; j < N; j++) {
-            const MT t = w.Aat(l_ + j * N, s);
-            w.Aat(l_ + j * N, s) = w.Aat(k + j * N, s);
-            w.Aat(k + j * N, s) = t;
-          }
-        const MT r = MT(1) / w.Aat(k + k * N, s);
-        for (int i = k + 1; i < N; i++) w.Aat(i + k * N, s) *= r;
-        for (int j = k + 1; j < N; j++) {
-          const MT akj = w.Aat(k + j * N, s);
-          if (akj != MT(0))
-            for (int i = k + 1; i < N; i++)
-              w.Aat(i + j * N, s) -= akj * w.Aat(i + k * N, s);
+    }
+    for (int k = 0; k < N; k++) {
+      int l_ = k;
+      MT best = kx::fabs(w.Aat(k + k * N, s));
+      for (int i = k + 1; i < N; i++) {
+        const MT v = kx::fabs(w.Aat(i + k * N, s));
+        if (v > best) {
+          best = v;
+          l_ = i;
         }
       }
-      return 0;
+      w.piv[(long)k * w.S + s] = MIXED ? pack(l_, e[k], law[k], rs[k]) : l_;
+      if (w.Aat(l_ + k * N, s) == MT(0)) return k + 1;
+      if (l_ != k)
+        for (int j = 0; j < N; j++) {
+          const MT t = w.Aat(l_ + j * N, s);
+          w.Aat(l_ + j * N, s) = w.Aat(k + j * N, s);
+          w.Aat(k + j * N, s) = t;
+        }
+      const MT r = MT(1) / w.Aat(k + k * N, s);
+      for (int i = k + 1; i < N; i++) w.Aat(i + k * N, s) *= r;
+      for (int j = k + 1; j < N; j++) {
+        const MT akj = w.Aat(k + j * N, s);
+        if (akj != MT(0))
+          for (int i = k + 1; i < N; i++)
+            w.Aat(i + j * N, s) -= akj * w.Aat(i + k * N, s);
+      }
     }
+    return 0;
   }
 
   // one column step k of the register LU (k a compile-time constant, so
@@ -683,6 +727,58 @@ struct Persist {
       for (int i = K + 1; i < N; i++)
         a[i + j * N] = ::fmaf(-akj, a[i + K * N], a[i + j * N]);
     }
+  }
+  // one column step k of the FP64 register LU (bdf_cell.hpp:402-433: the
+  // first largest pivot, the row swap, the column scaled by 1 / pivot,
+  // updates skipped for a zero a_kj); a zero pivot ends the factorization
+  template <int K>
+  static KX_HD void lu64_step(double (&a)[N * N], int& fail, const W& w,
+                              long s) {
+    if (fail) return;
+    int l = K;
+    double best = ::fabs(a[K + K * N]);
+#pragma unroll
+    for (int i = K + 1; i < N; i++) {
+      const double v = ::fabs(a[i + K * N]);
+      if (v > best) {
+        best = v;
+        l = i;
+      }
+    }
+    w.piv[(long)K * w.S + s] = l;
+    if (best == 0.0) {
+      fail = K + 1;
+      return;
+    }
+    if (l != K) {
+#pragma unroll
+      for (int j = 0; j < N; j++) {
+        const double tk = a[K + j * N];
+        double tl = tk;
+#pragma unroll
+        for (int i = K + 1; i < N; i++) tl = i == l ? a[i + j * N] : tl;
+        a[K + j * N] = tl;
+#pragma unroll
+        for (int i = K + 1; i < N; i++)
+          a[i + j * N] = i == l ? tk : a[i + j * N];
+      }
+    }
+    const double r = 1.0 / a[K + K * N];
+#pragma unroll
+    for (int i = K + 1; i < N; i++) a[i + K * N] *= r;
+#pragma unroll
+    for (int j = K + 1; j < N; j++) {
+      const double akj = a[K + j * N];
+#pragma unroll
+      for (int i = K + 1; i < N; i++)
+        a[i + j * N] =
+            akj != 0.0 ? a[i + j * N] - akj * a[i + K * N] : a[i + j * N];
+    }
+  }
+  template <int... K>
+  static KX_HD void lu64_steps(double (&a)[N * N], int& fail, const W& w,
+                               long s, std::integer_sequence<int, K...>) {
+    (lu64_step<K>(a, fail, w, s), ...);
   }
   template <int... K>
   static KX_HD void lu_steps(float (&a)[N * N], int& fail, const W& w, long s,
@@ -803,7 +899,7 @@ struct Persist {
         b[k] = u;
       }
     }
-    if constexpr (MIXED) {
+    if constexpr (MIXED && N <= REG_LU_N) {
       // the factors into registers (pairs), then the two triangular solves
       float am[N * N];
       load_mat(w, s, w.A, am);
@@ -1762,12 +1858,27 @@ int launch_persistent_t(CellsArgs a, int wpc, hipStream_t s) {
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// the persistent kernel factorizes its Newton matrix in registers: for
+// networks of up to KX_PERSIST_MAX_N equations; larger ones are refused
+// (-104) and take the plain kernel
+#ifndef KX_PERSIST_MAX_N
+#define KX_PERSIST_MAX_N 16
+#endif
+
 template <class Net, bool ET>
 inline int launch_persistent(CellsArgs a, int mixed, int chunk, int wpc,
                              hipStream_t s) {
   (void)chunk;
+  if constexpr (Problem<Net, ET>::N > KX_PERSIST_MAX_N) {
+    (void)a;
+    (void)mixed;
+    (void)wpc;
+    (void)s;
+    return FLAG_UNSUPPORTED;
+  } else {
   return mixed ? launch_persistent_t<Net, ET, true>(a, wpc, s)
                : launch_persistent_t<Net, ET, false>(a, wpc, s);
+  }
 }
 
 }  // namespace kx
