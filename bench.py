@@ -66,8 +66,9 @@ def flops_per_cell(stats, opc, N=10, q_avg=3.0):
 
 def cpu_baseline(w, nsample):
     """the oracle (CPU restatement of the reference solver, OpenMP over
-    cells) on the first nsample cells of the workload"""
-    lib = ctypes.CDLL(str(REPO / "oracle" / "_build" / "libkx_oracle.so"))
+    cells), built with the reference's production flags (oracle/Makefile
+    target fast), on the first nsample cells of the workload"""
+    lib = ctypes.CDLL(str(REPO / "oracle" / "_build" / "libkx_oracle_fast.so"))
     D = ctypes.POINTER(ctypes.c_double)
     cores = min(16, os.cpu_count() or 1)
     lib.kxo_set_threads(cores)
@@ -91,8 +92,9 @@ def cpu_baseline(w, nsample):
     return {"value": nsample / el, "unit": "cells/s", "cores": int(cores),
             "kind": "port",
             "sample": "the first %d of the same cells, CPU restatement of "
-                      "the reference per-cell solve (OpenMP dynamic), %.1f s "
-                      "wall, return %d" % (nsample, el, r)}, n, T
+                      "the reference per-cell solve (OpenMP dynamic, the "
+                      "reference's production flags -O3 -ffp-contract=fast), "
+                      "%.1f s wall, return %d" % (nsample, el, r)}, n, T
 
 
 def main():
