@@ -11,8 +11,10 @@
  *    the device for any number of cells, path 1 (per-cell host loop)
  *    returns KROME_FLAG_UNSUPPORTED; a cell that fails on the device keeps
  *    its input and its flag (no host re-solve);
- *  - conserve mode 2, linsol 2 (LAPACK) and 3 (sparse) return
- *    KROME_FLAG_UNSUPPORTED; networks without photo or dust bins only;
+ *  - linsol 2 (LAPACK) and 3 (sparse) return KROME_FLAG_UNSUPPORTED;
+ *    networks without photo or dust bins only;
+ *  - host-array calls (solves, element totals, conservation) copy their
+ *    rows to the device, compute there and copy back;
  *  - extra option keys select this build's kernels: device_kernel (0 the
  *    plain FP64 kernel, 1 the persistent batched kernel), mixed_precision
  *    (persistent kernel: 1 = FP32 Jacobian, LU and triangular solves with
@@ -123,8 +125,9 @@ int krome_solve_cells_x(int ncell, double* x, const double* rho,
  * the species-major (n[i ld + c], ld >= ncell, 0: ncell) or cell-major
  * layout, Tgas, dt, par and flags in device memory; the device is
  * synchronized before and after.  summary (host, may be NULL) counts the
- * cells solved and failed.  The mass-fraction variant returns
- * KROME_FLAG_UNSUPPORTED in this build. */
+ * cells solved and failed.  The mass-fraction variant takes x in either
+ * layout and rho[ncell] on the device (krome.cpp:1295-1304); the cells
+ * that succeed with dt > 0 are converted back, failed cells keep x. */
 typedef struct {
   long device;
   long host;
@@ -165,6 +168,55 @@ int krome_device_last_kernel_ms(double* ms);
 /* the name of a flag, e.g. "FLAG_TOO_MUCH_WORK" (the reference's
  * krome::flag_name, runtime/core/solve_cell.hpp:106-137) */
 const char* krome_flag_name(int flag);
+
+/* ---- element totals and consistent advection --------------------------- */
+
+/* species and element metadata (krome.h:642-650, krome.cpp:1182-1196):
+ * mass [g] (NaN out of range), charge (0 out of range), the number of
+ * elements, an element's name (NULL out of range; elements sorted by
+ * name), atoms of element e in species i (-1 out of range) */
+double krome_species_mass(int i);
+int krome_species_charge(int i);
+int krome_nelem(void);
+const char* krome_element_name(int e);
+int krome_species_natoms(int i, int e);
+
+/* element totals tot[ncell][nelem] [atoms cm^-3] of host rows n[ncell]
+ * [nspec]; _x: per gram of gas [atoms g^-1] from mass fractions
+ * (krome.h:473-477, krome.cpp:929-943).  0, or -1 on bad arguments.
+ * Computed on the device. */
+int krome_get_element_totals(int ncell, const double* n, double* tot);
+int krome_get_element_totals_x(int ncell, const double* x, double* tot);
+
+/* consistent advection (krome.h:478-495, krome.cpp:945-977): each cell of
+ * n[ncell][nspec] (or mass fractions x, ref per gram) moved to its
+ * reference element totals ref (ref_stride 0: one row for every cell,
+ * nelem: a row per cell) and to zero net charge: negatives clipped, the
+ * elements by the minimal relative correction, then the electrons from
+ * neutrality.  par, par_stride as for krome_solve_cells (only the
+ * network's freeze directives read them; these networks have none).
+ * flags[ncell] (may be NULL): 0, KROME_FLAG_CONSERVE_UNREACHED or the
+ * warning KROME_FLAG_CONSERVE_LARGE.  Returns the lowest failing cell's
+ * flag (negative), 0, or -1 on bad arguments.  Host rows are copied to
+ * the device, corrected there and copied back. */
+int krome_conserve_cells(int ncell, double* n, const double* ref,
+                         int ref_stride, const double* par, int par_stride,
+                         int* flags);
+int krome_conserve_cells_x(int ncell, double* x, const double* ref,
+                           int ref_stride, const double* par,
+                           int par_stride, int* flags);
+/* the same on device arrays (krome.h:351-360, krome.cpp:1306-1343): n or
+ * x in either layout (ld as for krome_solve_cells_device, ignored for the
+ * cell-major layout), ref, par and flags in device memory; the device is
+ * synchronized before the call and when it returns */
+int krome_conserve_cells_device(int ncell, double* n, int layout, long ld,
+                                const double* ref, int ref_stride,
+                                const double* par, int par_stride,
+                                int* flags);
+int krome_conserve_cells_x_device(int ncell, double* x, int layout, long ld,
+                                  const double* ref, int ref_stride,
+                                  const double* par, int par_stride,
+                                  int* flags);
 
 #ifdef __cplusplus
 }

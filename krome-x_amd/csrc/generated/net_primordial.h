@@ -69,6 +69,24 @@ struct net_primordial {
       default: return 0.0;
     }
   }
+  static constexpr int NELEM = 3;
+  KX_HD static constexpr int natoms(int i, int e) {
+    switch (i * 3 + e) {
+      case 0: return 1;
+      case 3: return 1;
+      case 7: return 1;
+      case 10: return 1;
+      case 13: return 1;
+      case 16: return 2;
+      case 19: return 2;
+      case 21: return 1;
+      case 22: return 1;
+      case 26: return 1;
+      case 29: return 1;
+      case 32: return 1;
+      default: return 0;
+    }
+  }
   // conserved combinations c_k . n (the nuclei of D, H, He, the charge): every
   // reaction keeps them, so c_k^T J = 0 over the species
   static constexpr int NCONS = 4;

@@ -65,6 +65,20 @@ struct net_primordial3 {
       default: return 0.0;
     }
   }
+  static constexpr int NELEM = 2;
+  KX_HD static constexpr int natoms(int i, int e) {
+    switch (i * 2 + e) {
+      case 0: return 1;
+      case 2: return 1;
+      case 4: return 1;
+      case 6: return 2;
+      case 8: return 2;
+      case 11: return 1;
+      case 13: return 1;
+      case 15: return 1;
+      default: return 0;
+    }
+  }
   // conserved combinations c_k . n (the nuclei of H, He, the charge): every
   // reaction keeps them, so c_k^T J = 0 over the species
   static constexpr int NCONS = 3;

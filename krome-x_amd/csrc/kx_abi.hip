@@ -15,6 +15,8 @@
 #include KX_NET_HEADER
 #include "kx_plain.h"
 #include "kx_persist.h"
+#include "kx_conserve.h"
+#include "kx_massfrac.h"
 #include "../../include/krome.h"
 
 using Net = kx::KX_NET;
@@ -128,10 +130,11 @@ struct Buf {
   }
 };
 struct DevCache {
-  Buf work, iwork, n, T, dt, par, flags, stats, queue, count;
+  Buf work, iwork, n, T, dt, par, flags, stats, queue, count, cons;
+  Buf xn, xin, rho;  // the mass-fraction entry points' densities
   void release() {
     for (Buf* b : {&work, &iwork, &n, &T, &dt, &par, &flags, &stats, &queue,
-                   &count})
+                   &count, &cons, &xn, &xin, &rho})
       b->release();
   }
 };
@@ -169,6 +172,28 @@ bool g_ev_recorded = false;
 int solve_resident_(kx::CellsArgs a, const Opts& o, DevCache& dc,
                     hipStream_t s);
 
+// the solve with the post-solve conservation of mode 2 (solve_cell.hpp:
+// 1484-1488): each cell's input totals taken before the kernels, the
+// integrated cells projected onto them after (modes 1 and 3 run inside
+// the solve kernels)
+int solve_conserved(kx::CellsArgs a, const Opts& o, DevCache& dc,
+                    hipStream_t s) {
+  if (a.o.conserve != 2 || a.ncell <= 0) return solve_resident_(a, o, dc, s);
+  if (!dc.cons.ensure(sizeof(double) * (size_t)a.ncell * (Net::NELEM + 1)))
+    return kx::FLAG_DEVICE_MEMORY;
+  double* tg = static_cast<double*>(dc.cons.p);
+  const int blocks = (a.ncell + 255) / 256;
+  hipLaunchKernelGGL(kx::conserve_targets_kernel<Net>, dim3(blocks),
+                     dim3(256), 0, s, a, tg);
+  kx::CellsArgs b = a;
+  b.o.conserve = 0;
+  const int r = solve_resident_(b, o, dc, s);
+  if (r != 0) return r;
+  hipLaunchKernelGGL(kx::conserve_post_kernel<Net>, dim3(blocks), dim3(256),
+                     0, s, a, (const double*)tg);
+  return hip_ok(hipGetLastError(), "conservation kernels") ? 0 : -1;
+}
+
 // the cells of one device call, in chunks that fit the memory budget:
 // arrays already resident; flags always written
 int solve_resident(kx::CellsArgs a, const Opts& o, DevCache& dc,
@@ -177,7 +202,7 @@ int solve_resident(kx::CellsArgs a, const Opts& o, DevCache& dc,
                    hipEventCreate(&g_ev[1]) != hipSuccess))
     g_ev[0] = g_ev[1] = nullptr;
   if (g_ev[0]) (void)hipEventRecord(g_ev[0], s);
-  const int r = solve_resident_(a, o, dc, s);
+  const int r = solve_conserved(a, o, dc, s);
   if (g_ev[0]) {
     (void)hipEventRecord(g_ev[1], s);
     g_ev_recorded = true;
@@ -446,7 +471,6 @@ kx::SolveOpts solve_opts(const krome_opts* h) {
 
 // what the whole call rejects before integrating (every cell flagged)
 int reject(const krome_opts* h) {
-  if (h->conserve == 2) return kx::FLAG_UNSUPPORTED;
   if (h->linsol == 2 || h->linsol == 3) return kx::FLAG_UNSUPPORTED;
   if (h->path == 1) return kx::FLAG_UNSUPPORTED;  // no host path here
   if (h->evolve_T && !Net::THERMAL) return kx::FLAG_UNSUPPORTED;
@@ -460,16 +484,53 @@ int first_flag(const int* fl, int ncell) {
   return 0;
 }
 
-}  // namespace
+// rows v in the layout -> (cell stride, species stride)
+void strides(int layout, long ld, int ncell, long& cs, long& ss) {
+  if (layout == KROME_LAYOUT_SPECIES_MAJOR) {
+    cs = 1;
+    ss = ld ? ld : ncell;
+  } else {
+    cs = NS;
+    ss = 1;
+  }
+}
 
-extern "C" {
+// the mass-fraction form around a resident solve (solve_cells.hpp:
+// 176-200, solve_kokkos.hpp:1450-1458, 1519-1528): a.n holds the caller's
+// x (in its layout); the densities live in dc.xn (species-major) during
+// the solve; the cells with flag 0 are converted back (with skip_dt, only
+// those with dt > 0, as the device path does); failed cells keep x
+int solve_x_resident(kx::CellsArgs a, const double* rho, bool skip_dt,
+                     int renorm, const Opts& o, DevCache& dc, hipStream_t s) {
+  const int nc = a.ncell;
+  if (!dc.xn.ensure(sizeof(double) * (size_t)nc * NS) ||
+      !dc.xin.ensure(sizeof(double) * (size_t)nc))
+    return kx::FLAG_DEVICE_MEMORY;
+  double* xn = static_cast<double*>(dc.xn.p);
+  double* xin = static_cast<double*>(dc.xin.p);
+  const dim3 g((nc + 255) / 256), b(256);
+  hipLaunchKernelGGL(kx::x_to_n_kernel<Net>, g, b, 0, s, nc,
+                     (const double*)a.n, a.n_cs, a.n_ss, rho, xn, xin);
+  kx::CellsArgs d = a;
+  d.n = xn;
+  d.n_cs = 1;
+  d.n_ss = nc;
+  const int r = solve_resident(d, o, dc, s);
+  if (r != 0) return r;
+  hipLaunchKernelGGL(kx::n_to_x_kernel<Net>, g, b, 0, s, nc,
+                     (const double*)xn, a.n, a.n_cs, a.n_ss, rho,
+                     (const double*)xin, (const int*)a.flags,
+                     skip_dt ? a.dt : nullptr, renorm);
+  return hip_ok(hipGetLastError(), "mass-fraction kernels") ? 0 : -1;
+}
 
-int krome_solve_cells(int ncell, double* n, double* Tgas, const double* dt,
-                      const double* par, int par_stride,
-                      const krome_cells* extra, int* flags,
-                      const krome_opts* h) {
-  (void)extra;  // no photon bins or dust bins in this build's networks
-  if (!h || ncell < 0 || !n || !Tgas || !dt) return -1;
+// krome_solve_cells / _x on host arrays (krome.cpp:396-470): rows copied
+// to the device, solved there, copied back; rho (may be NULL) makes v the
+// mass fractions
+int solve_host(int ncell, double* v, const double* rho, double* Tgas,
+               const double* dt, const double* par, int par_stride,
+               int* flags, const krome_opts* h) {
+  if (!h || ncell < 0 || !v || !Tgas || !dt) return -1;
   if (par && par_stride != 0 && par_stride != Net::NPAR) return -1;
   if (ncell == 0) return 0;
   std::vector<int> fl(ncell, 0);
@@ -498,15 +559,18 @@ int krome_solve_cells(int ncell, double* n, double* Tgas, const double* dt,
   const size_t pb = sizeof(double) * (size_t)(pst ? ncell : 1) * Net::NPAR;
   if (!dc.n.ensure(nb) || !dc.T.ensure(sizeof(double) * ncell) ||
       !dc.dt.ensure(sizeof(double) * ncell) || !dc.par.ensure(pb) ||
-      !dc.flags.ensure(sizeof(int) * ncell))
+      !dc.flags.ensure(sizeof(int) * ncell) ||
+      (rho && !dc.rho.ensure(sizeof(double) * ncell)))
     return kx::FLAG_DEVICE_MEMORY;
   hipStream_t s = nullptr;
-  bool ok = hip_ok(hipMemcpy(dc.n.p, n, nb, hipMemcpyHostToDevice), "H2D") &&
+  bool ok = hip_ok(hipMemcpy(dc.n.p, v, nb, hipMemcpyHostToDevice), "H2D") &&
             hip_ok(hipMemcpy(dc.T.p, Tgas, sizeof(double) * ncell,
                              hipMemcpyHostToDevice), "H2D") &&
             hip_ok(hipMemcpy(dc.dt.p, dt, sizeof(double) * ncell,
                              hipMemcpyHostToDevice), "H2D") &&
-            hip_ok(hipMemcpy(dc.par.p, P, pb, hipMemcpyHostToDevice), "H2D");
+            hip_ok(hipMemcpy(dc.par.p, P, pb, hipMemcpyHostToDevice), "H2D") &&
+            (!rho || hip_ok(hipMemcpy(dc.rho.p, rho, sizeof(double) * ncell,
+                                      hipMemcpyHostToDevice), "H2D"));
   if (!ok) return -1;
   kx::CellsArgs a;
   a.ncell = ncell;
@@ -520,10 +584,12 @@ int krome_solve_cells(int ncell, double* n, double* Tgas, const double* dt,
   a.flags = static_cast<int*>(dc.flags.p);
   a.stats = nullptr;
   a.o = solve_opts(h);
-  int r = solve_resident(a, *h, dc, s);
+  int r = rho ? solve_x_resident(a, static_cast<const double*>(dc.rho.p),
+                                 false, h->renormalize_x ? 1 : 0, *h, dc, s)
+              : solve_resident(a, *h, dc, s);
   if (r == 0) {
     ok = hip_ok(hipDeviceSynchronize(), "kernel") &&
-         hip_ok(hipMemcpy(n, dc.n.p, nb, hipMemcpyDeviceToHost), "D2H") &&
+         hip_ok(hipMemcpy(v, dc.n.p, nb, hipMemcpyDeviceToHost), "D2H") &&
          hip_ok(hipMemcpy(Tgas, dc.T.p, sizeof(double) * ncell,
                           hipMemcpyDeviceToHost), "D2H") &&
          hip_ok(hipMemcpy(fl.data(), dc.flags.p, sizeof(int) * ncell,
@@ -540,59 +606,19 @@ int krome_solve_cells(int ncell, double* n, double* Tgas, const double* dt,
   return first_flag(fl.data(), ncell);
 }
 
-int krome_solve_cells_x(int ncell, double* x, const double* rho,
-                        double* Tgas, const double* dt, const double* par,
-                        int par_stride, const krome_cells* extra, int* flags,
-                        const krome_opts* h) {
-  if (!h || ncell < 0 || !x || !rho || !Tgas || !dt) return -1;
-  if (par && par_stride != 0 && par_stride != Net::NPAR) return -1;
-  if (ncell == 0) return 0;
-  std::vector<double> n((size_t)ncell * NS), xin(ncell);
-  for (int c = 0; c < ncell; ++c) {
-    double s = 0.0;
-    for (int i = 0; i < NS; ++i) {
-      const double m = Net::mass_g(i);
-      const double xi = x[(size_t)c * NS + i];
-      if (m > 0.0) s += xi;
-      n[(size_t)c * NS + i] = m > 0.0 ? rho[c] * xi / m : 0.0;
-    }
-    xin[c] = s;
-  }
-  std::vector<int> fl(ncell, 0);
-  int* f = flags ? flags : fl.data();
-  const int r = krome_solve_cells(ncell, n.data(), Tgas, dt, par, par_stride,
-                                  extra, f, h);
-  for (int c = 0; c < ncell; ++c) {
-    if (f[c] != 0) continue;
-    double* xc = x + (size_t)c * NS;
-    double s = 0.0;
-    for (int i = 0; i < NS; ++i) {
-      xc[i] = Net::mass_g(i) * n[(size_t)c * NS + i] / rho[c];
-      if (Net::mass_g(i) > 0.0) s += xc[i];
-    }
-    if (h->renormalize_x && s > 0.0)
-      for (int i = 0; i < NS; ++i) xc[i] *= xin[c] / s;
-  }
-  return r;
-}
-
-// device-resident arrays (krome.h: krome_solve_cells_device): n in the
-// species-major (n[i ld + c]) or cell-major layout, everything on the
-// device; the call synchronizes before and after
-int krome_solve_cells_device(int ncell, double* n, int layout, long ld,
-                             double* Tgas, const double* dt,
-                             const double* par, int par_stride,
-                             const krome_cells* extra, int* flags,
-                             krome_device_summary* summary,
-                             const krome_opts* h) {
-  (void)extra;
-  if (!h || ncell < 0 || !n || !Tgas || !dt) return -1;
+// krome_solve_cells(_x)_device (krome.cpp:1237-1304): arrays on the
+// device, v in either layout (rho: mass fractions)
+int solve_device(int ncell, double* v, const double* rho, int layout,
+                 long ld, double* Tgas, const double* dt, const double* par,
+                 int par_stride, int* flags, krome_device_summary* summary,
+                 const krome_opts* h) {
+  if (summary) *summary = krome_device_summary{0, 0, 0};
+  if (!h || ncell < 0 || !v || !Tgas || !dt) return -1;
   if (layout != KROME_LAYOUT_SPECIES_MAJOR &&
       layout != KROME_LAYOUT_CELL_MAJOR)
     return -1;
+  if (ld != 0 && ld < ncell) return -1;
   if (par && par_stride != 0 && par_stride != Net::NPAR) return -1;
-  if (layout == KROME_LAYOUT_SPECIES_MAJOR && ld != 0 && ld < ncell)
-    return -1;
   if (ncell == 0) return 0;
   if (!hip_ok(hipDeviceSynchronize(), "fence")) return -1;
   std::lock_guard<std::mutex> lk(g_cache_mutex);
@@ -603,7 +629,8 @@ int krome_solve_cells_device(int ncell, double* n, int layout, long ld,
   if (!P) {
     std::vector<double> d(Net::NPAR);
     krome_params_default(d.data());
-    if (!dc.par.ensure(sizeof(double) * Net::NPAR)) return kx::FLAG_DEVICE_MEMORY;
+    if (!dc.par.ensure(sizeof(double) * Net::NPAR))
+      return kx::FLAG_DEVICE_MEMORY;
     if (!hip_ok(hipMemcpy(dc.par.p, d.data(), sizeof(double) * Net::NPAR,
                           hipMemcpyHostToDevice), "H2D"))
       return -1;
@@ -626,14 +653,8 @@ int krome_solve_cells_device(int ncell, double* n, int layout, long ld,
   }
   kx::CellsArgs a;
   a.ncell = ncell;
-  a.n = n;
-  if (layout == KROME_LAYOUT_SPECIES_MAJOR) {
-    a.n_cs = 1;
-    a.n_ss = ld ? ld : ncell;
-  } else {
-    a.n_cs = NS;
-    a.n_ss = 1;
-  }
+  a.n = v;
+  strides(layout, ld, ncell, a.n_cs, a.n_ss);
   a.T = Tgas;
   a.dt = dt;
   a.par = P;
@@ -641,35 +662,71 @@ int krome_solve_cells_device(int ncell, double* n, int layout, long ld,
   a.flags = fl;
   a.stats = nullptr;
   a.o = solve_opts(h);
-  int r = solve_resident(a, *h, dc, nullptr);
+  const int r = rho ? solve_x_resident(a, rho, true,
+                                       h->renormalize_x ? 1 : 0, *h, dc,
+                                       nullptr)
+                    : solve_resident(a, *h, dc, nullptr);
   if (r != 0) return r;
   if (!hip_ok(hipDeviceSynchronize(), "kernel")) return -1;
   std::vector<int> hf(ncell);
   if (!hip_ok(hipMemcpy(hf.data(), fl, sizeof(int) * ncell,
                         hipMemcpyDeviceToHost), "D2H"))
     return -1;
-  if (summary) {
-    summary->device = 0;
-    summary->host = 0;
-    summary->failed = 0;
+  if (summary)
     for (int c = 0; c < ncell; ++c) {
       if (hf[c] == 0) summary->device++;
       else summary->failed++;
     }
-  }
   return first_flag(hf.data(), ncell);
 }
 
+}  // namespace
+
+extern "C" {
+
+int krome_solve_cells(int ncell, double* n, double* Tgas, const double* dt,
+                      const double* par, int par_stride,
+                      const krome_cells* extra, int* flags,
+                      const krome_opts* h) {
+  (void)extra;  // no photon bins or dust bins in this build's networks
+  return solve_host(ncell, n, nullptr, Tgas, dt, par, par_stride, flags, h);
+}
+
+int krome_solve_cells_x(int ncell, double* x, const double* rho,
+                        double* Tgas, const double* dt, const double* par,
+                        int par_stride, const krome_cells* extra, int* flags,
+                        const krome_opts* h) {
+  (void)extra;
+  if (!rho) return -1;
+  return solve_host(ncell, x, rho, Tgas, dt, par, par_stride, flags, h);
+}
+
+// device-resident arrays (krome.h: krome_solve_cells_device): n in the
+// species-major (n[i ld + c]) or cell-major layout, everything on the
+// device; the call synchronizes before and after
+int krome_solve_cells_device(int ncell, double* n, int layout, long ld,
+                             double* Tgas, const double* dt,
+                             const double* par, int par_stride,
+                             const krome_cells* extra, int* flags,
+                             krome_device_summary* summary,
+                             const krome_opts* h) {
+  (void)extra;
+  return solve_device(ncell, n, nullptr, layout, ld, Tgas, dt, par,
+                      par_stride, flags, summary, h);
+}
+
+// the same with mass fractions x and the gas density rho[ncell] on the
+// device (krome.cpp:1295-1304)
 int krome_solve_cells_x_device(int ncell, double* x, const double* rho,
                                int layout, long ld, double* Tgas,
                                const double* dt, const double* par,
                                int par_stride, const krome_cells* extra,
                                int* flags, krome_device_summary* summary,
                                const krome_opts* h) {
-  (void)ncell; (void)x; (void)rho; (void)layout; (void)ld; (void)Tgas;
-  (void)dt; (void)par; (void)par_stride; (void)extra; (void)flags;
-  (void)summary; (void)h;
-  return kx::FLAG_UNSUPPORTED;
+  (void)extra;
+  if (!rho) return -1;
+  return solve_device(ncell, x, rho, layout, ld, Tgas, dt, par, par_stride,
+                      flags, summary, h);
 }
 
 // per-cell statistics of the last kind of call, for the benchmarks and the
@@ -702,13 +759,7 @@ int krome_solve_cells_device_stats(int ncell, double* n, int layout, long ld,
   kx::CellsArgs a;
   a.ncell = ncell;
   a.n = n;
-  if (layout == KROME_LAYOUT_SPECIES_MAJOR) {
-    a.n_cs = 1;
-    a.n_ss = ld ? ld : ncell;
-  } else {
-    a.n_cs = NS;
-    a.n_ss = 1;
-  }
+  strides(layout, ld, ncell, a.n_cs, a.n_ss);
   a.T = Tgas;
   a.dt = dt;
   a.par = par;
@@ -717,6 +768,179 @@ int krome_solve_cells_device_stats(int ncell, double* n, int layout, long ld,
   a.stats = stats;
   a.o = solve_opts(h);
   return solve_resident(a, *h, g_cache, nullptr);
+}
+
+// ---- element totals and consistent advection ----------------------------
+
+const char* krome_element_name(int e) {
+  return (e >= 0 && e < Net::NELEM) ? kElements[e] : nullptr;
+}
+int krome_nelem(void) { return Net::NELEM; }
+int krome_species_natoms(int i, int e) {
+  if (i < 0 || i >= NS || e < 0 || e >= Net::NELEM) return -1;
+  return Net::natoms(i, e);
+}
+double krome_species_mass(int i) {
+  return (i >= 0 && i < NS) ? Net::mass_g(i) : std::nan("");
+}
+int krome_species_charge(int i) {
+  return (i >= 0 && i < NS) ? Net::charge(i) : 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// the element totals of host rows, on the device
+int totals_host(int ncell, const double* v, double* tot, bool x) {
+  if (ncell < 0 || !v || !tot) return -1;
+  if (ncell == 0) return 0;
+  std::lock_guard<std::mutex> lk(g_cache_mutex);
+  DevCache& dc = g_cache;
+  const size_t vb = sizeof(double) * (size_t)ncell * NS;
+  const size_t tb = sizeof(double) * (size_t)ncell * Net::NELEM;
+  if (!dc.n.ensure(vb) || !dc.cons.ensure(tb)) return kx::FLAG_DEVICE_MEMORY;
+  if (!hip_ok(hipMemcpy(dc.n.p, v, vb, hipMemcpyHostToDevice), "H2D"))
+    return -1;
+  const dim3 g((ncell + 255) / 256), b(256);
+  const double* dv = static_cast<const double*>(dc.n.p);
+  double* dt = static_cast<double*>(dc.cons.p);
+  if (x)
+    hipLaunchKernelGGL((kx::totals_kernel<Net, true>), g, b, 0, nullptr,
+                       ncell, dv, (long)NS, 1L, dt);
+  else
+    hipLaunchKernelGGL((kx::totals_kernel<Net, false>), g, b, 0, nullptr,
+                       ncell, dv, (long)NS, 1L, dt);
+  if (!hip_ok(hipGetLastError(), "totals kernel") ||
+      !hip_ok(hipMemcpy(tot, dt, tb, hipMemcpyDeviceToHost), "D2H"))
+    return -1;
+  return 0;
+}
+
+// consistent advection of resident rows; the lowest failing cell's flag
+int conserve_resident(int ncell, double* v, long cs, long ss,
+                      const double* ref, long rs, int* flags, bool x,
+                      DevCache& dc) {
+  if (!dc.count.ensure(sizeof(int))) return kx::FLAG_DEVICE_MEMORY;
+  int* first = static_cast<int*>(dc.count.p);
+  if (!hip_ok(hipMemcpy(first, &ncell, sizeof(int), hipMemcpyHostToDevice),
+              "H2D"))
+    return -1;
+  const dim3 g((ncell + 255) / 256), b(256);
+  if (x)
+    hipLaunchKernelGGL((kx::conserve_kernel<Net, true>), g, b, 0, nullptr,
+                       ncell, v, cs, ss, ref, rs, flags, first);
+  else
+    hipLaunchKernelGGL((kx::conserve_kernel<Net, false>), g, b, 0, nullptr,
+                       ncell, v, cs, ss, ref, rs, flags, first);
+  if (!hip_ok(hipGetLastError(), "conserve kernel")) return -1;
+  int lo = ncell;
+  if (!hip_ok(hipMemcpy(&lo, first, sizeof(int), hipMemcpyDeviceToHost),
+              "D2H"))
+    return -1;
+  if (lo >= ncell) return 0;
+  int f = 0;
+  if (!hip_ok(hipMemcpy(&f, flags + lo, sizeof(int), hipMemcpyDeviceToHost),
+              "D2H"))
+    return -1;
+  return f;
+}
+
+// krome_conserve_cells(_x) on host rows (krome.cpp:943-977): the same
+// arguments and errors; the rows corrected on the device
+int conserve_host(int ncell, double* v, const double* ref, int ref_stride,
+                  const double* par, int par_stride, int* flags, bool x) {
+  if (ncell < 0 || !v || !ref || ref_stride < 0 || par_stride < 0) return -1;
+  if (ref_stride != 0 && ref_stride != Net::NELEM) return -1;
+  (void)par;  // the parameters select freeze directives: none here
+  if (ncell == 0) return 0;
+  std::lock_guard<std::mutex> lk(g_cache_mutex);
+  DevCache& dc = g_cache;
+  const size_t vb = sizeof(double) * (size_t)ncell * NS;
+  const size_t rb =
+      sizeof(double) * (size_t)(ref_stride ? ncell : 1) * Net::NELEM;
+  if (!dc.n.ensure(vb) || !dc.cons.ensure(rb) ||
+      !dc.flags.ensure(sizeof(int) * ncell))
+    return kx::FLAG_DEVICE_MEMORY;
+  if (!hip_ok(hipMemcpy(dc.n.p, v, vb, hipMemcpyHostToDevice), "H2D") ||
+      !hip_ok(hipMemcpy(dc.cons.p, ref, rb, hipMemcpyHostToDevice), "H2D"))
+    return -1;
+  int* df = static_cast<int*>(dc.flags.p);
+  const int r = conserve_resident(ncell, static_cast<double*>(dc.n.p),
+                                  (long)NS, 1L,
+                                  static_cast<const double*>(dc.cons.p),
+                                  (long)ref_stride, df, x, dc);
+  if (r == -1 || r == kx::FLAG_DEVICE_MEMORY) return r;
+  if (!hip_ok(hipMemcpy(v, dc.n.p, vb, hipMemcpyDeviceToHost), "D2H"))
+    return -1;
+  if (flags && !hip_ok(hipMemcpy(flags, df, sizeof(int) * ncell,
+                                 hipMemcpyDeviceToHost), "D2H"))
+    return -1;
+  return r;
+}
+
+// krome_conserve_cells(_x)_device (krome.cpp:1306-1343,
+// solve_kokkos.hpp:1787-1822): arrays on the device
+int conserve_device(int ncell, double* v, int layout, long ld,
+                    const double* ref, int ref_stride, const double* par,
+                    int par_stride, int* flags, bool x) {
+  if (ncell < 0 || !v || !ref ||
+      (ref_stride != 0 && ref_stride != Net::NELEM) ||
+      (par && par_stride != 0 && par_stride != Net::NPAR) ||
+      (layout != KROME_LAYOUT_SPECIES_MAJOR &&
+       layout != KROME_LAYOUT_CELL_MAJOR) ||
+      (ld != 0 && ld < ncell))
+    return -1;
+  if (ncell == 0) return 0;
+  if (!hip_ok(hipDeviceSynchronize(), "fence")) return -1;
+  std::lock_guard<std::mutex> lk(g_cache_mutex);
+  DevCache& dc = g_cache;
+  int* fl = flags;
+  if (!fl) {
+    if (!dc.flags.ensure(sizeof(int) * ncell)) return kx::FLAG_DEVICE_MEMORY;
+    fl = static_cast<int*>(dc.flags.p);
+  }
+  long cs, ss;
+  strides(layout, ld, ncell, cs, ss);
+  return conserve_resident(ncell, v, cs, ss, ref, (long)ref_stride, fl, x,
+                           dc);
+}
+
+}  // namespace
+
+extern "C" {
+
+int krome_get_element_totals(int ncell, const double* n, double* tot) {
+  return totals_host(ncell, n, tot, false);
+}
+int krome_get_element_totals_x(int ncell, const double* x, double* tot) {
+  return totals_host(ncell, x, tot, true);
+}
+int krome_conserve_cells(int ncell, double* n, const double* ref,
+                         int ref_stride, const double* par, int par_stride,
+                         int* flags) {
+  return conserve_host(ncell, n, ref, ref_stride, par, par_stride, flags,
+                       false);
+}
+int krome_conserve_cells_x(int ncell, double* x, const double* ref,
+                           int ref_stride, const double* par,
+                           int par_stride, int* flags) {
+  return conserve_host(ncell, x, ref, ref_stride, par, par_stride, flags,
+                       true);
+}
+int krome_conserve_cells_device(int ncell, double* n, int layout, long ld,
+                                const double* ref, int ref_stride,
+                                const double* par, int par_stride,
+                                int* flags) {
+  return conserve_device(ncell, n, layout, ld, ref, ref_stride, par,
+                         par_stride, flags, false);
+}
+int krome_conserve_cells_x_device(int ncell, double* x, int layout, long ld,
+                                  const double* ref, int ref_stride,
+                                  const double* par, int par_stride,
+                                  int* flags) {
+  return conserve_device(ncell, x, layout, ld, ref, ref_stride, par,
+                         par_stride, flags, true);
 }
 
 }  // extern "C"

@@ -110,12 +110,40 @@ class Network:
                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                   ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                   ctypes.c_void_p, ctypes.c_void_p]),
+                ("krome_solve_cells_x_device", ctypes.c_int,
+                 [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                  ctypes.c_int, ctypes.c_long, ctypes.c_void_p,
+                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                  ctypes.c_void_p]),
                 ("krome_solve_cells_device_stats", ctypes.c_int,
                  [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_long,
                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                   ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                   ctypes.c_void_p]),
-                ("krome_device_last_kernel_ms", ctypes.c_int, [_D])):
+                ("krome_device_last_kernel_ms", ctypes.c_int, [_D]),
+                ("krome_species_mass", ctypes.c_double, [ctypes.c_int]),
+                ("krome_species_charge", ctypes.c_int, [ctypes.c_int]),
+                ("krome_nelem", ctypes.c_int, []),
+                ("krome_element_name", ctypes.c_char_p, [ctypes.c_int]),
+                ("krome_species_natoms", ctypes.c_int,
+                 [ctypes.c_int, ctypes.c_int]),
+                ("krome_get_element_totals", ctypes.c_int,
+                 [ctypes.c_int, _D, _D]),
+                ("krome_get_element_totals_x", ctypes.c_int,
+                 [ctypes.c_int, _D, _D]),
+                ("krome_conserve_cells", ctypes.c_int,
+                 [ctypes.c_int, _D, _D, ctypes.c_int, _D, ctypes.c_int, _I]),
+                ("krome_conserve_cells_x", ctypes.c_int,
+                 [ctypes.c_int, _D, _D, ctypes.c_int, _D, ctypes.c_int, _I]),
+                ("krome_conserve_cells_device", ctypes.c_int,
+                 [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_long,
+                  ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                  ctypes.c_int, ctypes.c_void_p]),
+                ("krome_conserve_cells_x_device", ctypes.c_int,
+                 [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_long,
+                  ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                  ctypes.c_int, ctypes.c_void_p])):
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
@@ -132,6 +160,19 @@ class Network:
                          if L.krome_param_required_by(i)}
         self._pidx = {p: i for i, p in enumerate(self.params)}
         self._opts = None
+        # species and element metadata (the reference binding's
+        # krome.py:334-345)
+        self.masses = np.array([L.krome_species_mass(i)
+                                for i in range(self.nspec)])  # [g]
+        self.charges = np.array([L.krome_species_charge(i)
+                                 for i in range(self.nspec)])
+        self.elements = tuple(L.krome_element_name(e).decode()
+                              for e in range(L.krome_nelem()))
+        self.composition = np.array(
+            [[L.krome_species_natoms(i, e)
+              for e in range(len(self.elements))]
+             for i in range(self.nspec)], dtype=int).reshape(
+                 self.nspec, len(self.elements))
 
     def idx(self, name):
         return self.species.index(name)
@@ -224,6 +265,136 @@ class Network:
             import torch
             torch.cuda.synchronize()
         return r
+
+    def solve_x_device(self, x, rho, Tgas, dt, par, opts=None, flags=None,
+                       layout=None):
+        """device-resident solve with mass fractions on torch tensors
+        (krome_solve_cells_x_device): x [nspec, ncell] (species-major) or
+        [ncell, nspec] (layout LAYOUT_CELL_MAJOR), rho, Tgas, dt [ncell],
+        par [ncell, npar], float64 on the GPU; updated in place, failed
+        cells keep x.  flags: optional int32 [ncell] device tensor.
+        Returns the C function's flag"""
+        if layout is None:
+            layout = LAYOUT_SPECIES_MAJOR
+        ncell = int(Tgas.shape[0])
+        for t in (x, rho, Tgas, dt, par):
+            if not t.is_cuda or not t.is_contiguous():
+                raise KromeError("solve_x_device needs contiguous GPU "
+                                 "tensors")
+        shape = ((self.nspec, ncell) if layout == LAYOUT_SPECIES_MAJOR
+                 else (ncell, self.nspec))
+        if tuple(x.shape) != shape:
+            raise KromeError("x must be %s" % (shape,))
+        if tuple(par.shape) != (ncell, self.npar):
+            raise KromeError("par must be [ncell, npar]")
+        opts = self._options(opts)
+        return self._lib.krome_solve_cells_x_device(
+            ncell, x.data_ptr(), rho.data_ptr(), layout, 0, Tgas.data_ptr(),
+            dt.data_ptr(), par.data_ptr(), self.npar,
+            None, flags.data_ptr() if flags is not None else None, None,
+            opts._h)
+
+    # -- element totals and consistent advection (the reference binding's
+    #    krome.py:850-921) ------------------------------------------------
+
+    def element_totals(self, n):
+        """element totals [atoms cm^-3], [..., nelem] (labels: elements)"""
+        return self._totals(n, "krome_get_element_totals")
+
+    def element_totals_x(self, x):
+        """element totals per gram of gas [atoms g^-1] from the mass
+        fractions, [..., nelem] (legacy krome_conserveLinGetRef_x)"""
+        return self._totals(x, "krome_get_element_totals_x")
+
+    def _totals(self, v, fname):
+        single, v, ncell = self._cells(v)
+        out = np.zeros((ncell, max(len(self.elements), 1)))
+        r = getattr(self._lib, fname)(ncell, _dptr(v), _dptr(out))
+        if r != 0:
+            raise KromeError("%s returned %d" % (fname, r))
+        out = out[:, :len(self.elements)]
+        return out[0] if single else out
+
+    def conserve(self, n, ref, par=None, flags=None, check=True):
+        """consistent advection (legacy krome_conserveLin_x): the cells n
+        ([nspec] or [ncell, nspec], cm^-3) moved back to the reference
+        element totals ref ([nelem]: one row for all, or [ncell, nelem];
+        element_totals of a reference cell) and to zero net charge
+        (negatives clipped, the elements by the minimal relative
+        correction, then the electrons from neutrality).  par: the
+        parameters as for solve (only freeze directives read them).  flags:
+        an int array [ncell] receiving each cell's flag (0,
+        FLAG_CONSERVE_UNREACHED -112, the warning FLAG_CONSERVE_LARGE 2).
+        Returns the corrected densities (the input is not modified); raises
+        KromeError when a cell cannot reach its reference, unless
+        check=False"""
+        return self._conserve(n, ref, par, flags, check,
+                              "krome_conserve_cells")
+
+    def conserve_x(self, x, ref, par=None, flags=None, check=True):
+        """conserve() for mass fractions, ref per gram (element_totals_x)"""
+        return self._conserve(x, ref, par, flags, check,
+                              "krome_conserve_cells_x")
+
+    def _conserve(self, v, ref, par, flags, check, fname):
+        single, v, ncell = self._cells(v)
+        nel = len(self.elements)
+        R = np.ascontiguousarray(ref, dtype=float)
+        if R.shape == (nel,):
+            rs = 0
+        elif R.shape == (ncell, nel):
+            rs = nel
+        else:
+            raise KromeError("ref must have shape (%d,) or (%d, %d), got %s"
+                             % (nel, ncell, nel, R.shape))
+        # the parameters only select freeze directives: no REQUIRED check
+        P, ps = None, 0
+        if isinstance(par, dict):
+            P, ps = self.param_rows(ncell, **par), self.npar
+        elif par is not None:
+            P = np.ascontiguousarray(par, dtype=float)
+            if P.shape == (ncell, self.npar):
+                ps = self.npar
+            elif P.shape != (self.npar,):
+                raise KromeError("par must have shape (%d,) or (%d, %d), got "
+                                 "%s" % (self.npar, ncell, self.npar,
+                                         P.shape))
+        fl = np.zeros(ncell, dtype=np.intc)
+        r = getattr(self._lib, fname)(ncell, _dptr(v), _dptr(R), rs,
+                                      _dptr(P), ps, fl.ctypes.data_as(_I))
+        if r == -1:
+            raise KromeError("%s: bad arguments" % fname)
+        if flags is not None:
+            flags[...] = fl.reshape(np.shape(flags))
+        if check and r != 0:
+            raise KromeError("%s: %s (cells %s); pass check=False to get "
+                             "the result anyway" % (
+                                 fname, self.flag_message(r),
+                                 np.flatnonzero(fl < 0).tolist()))
+        return v[0] if single else v
+
+    def conserve_device(self, n, ref, flags=None, x=False, layout=None):
+        """consistent advection on torch tensors on the GPU, in place: n
+        [nspec, ncell] (species-major) or [ncell, nspec] (layout
+        LAYOUT_CELL_MAJOR), float64; ref [nelem] or [ncell, nelem]; flags an
+        int32 [ncell] device tensor (optional).  x: mass fractions.
+        Returns the C function's flag"""
+        if layout is None:
+            layout = LAYOUT_SPECIES_MAJOR
+        ncell = int(n.shape[1] if layout == LAYOUT_SPECIES_MAJOR
+                    else n.shape[0])
+        nel = len(self.elements)
+        for t in (n, ref) + ((flags,) if flags is not None else ()):
+            if not t.is_cuda or not t.is_contiguous():
+                raise KromeError("conserve_device needs contiguous GPU "
+                                 "tensors")
+        rs = 0 if tuple(ref.shape) == (nel,) else nel
+        if rs and tuple(ref.shape) != (ncell, nel):
+            raise KromeError("ref must be [nelem] or [ncell, nelem]")
+        f = (self._lib.krome_conserve_cells_x_device if x
+             else self._lib.krome_conserve_cells_device)
+        return f(ncell, n.data_ptr(), layout, 0, ref.data_ptr(), rs, None, 0,
+                 flags.data_ptr() if flags is not None else None)
 
     # -- inputs (the krome_solve_cells conventions) -----------------------
 

@@ -265,11 +265,17 @@ def _check_balance(net):
                             % (r.line, r.eq))
 
 
+def elements(net):
+    """the elements of the network's species, sorted (the order of the
+    element totals, krome_element_name)"""
+    return sorted({e for sp in net.species for e in sp.comp})
+
+
 def conserved_laws(net):
     """[(name, [coefficient per species])]: the nuclei of every element
     the species hold, then the charge when a species is charged, each kept
     only if every reaction conserves it"""
-    elems = sorted({e for sp in net.species for e in sp.comp})
+    elems = elements(net)
     laws = [(e, [sp.comp.get(e, 0) for sp in net.species]) for e in elems]
     if any(sp.charge for sp in net.species):
         laws.append(("charge", [sp.charge for sp in net.species]))
@@ -630,6 +636,21 @@ def generate(knet_path, thermal=None, name=None):
     w("      default: return 0.0;")
     w("    }")
     w("  }")
+    # element composition (query.hpp:152-170, codegen.py:3544-3566): the
+    # elements the species hold, sorted; natoms(i, e) atoms of element e in
+    # species i (every species here is an element composition: none opaque)
+    elems = elements(net)
+    w("  static constexpr int NELEM = %d;" % len(elems))
+    w("  KX_HD static constexpr int natoms(int i, int e) {")
+    w("    switch (i * %d + e) {" % max(len(elems), 1))
+    for i, sp in enumerate(net.species):
+        for e, el in enumerate(elems):
+            if sp.comp.get(el, 0):
+                w("      case %d: return %d;" % (i * len(elems) + e,
+                                               sp.comp[el]))
+    w("      default: return 0;")
+    w("    }")
+    w("  }")
     # conserved combinations: the nuclei of each element and the charge,
     # kept where every reaction keeps them
     laws = conserved_laws(net)
@@ -779,6 +800,9 @@ def generate(knet_path, thermal=None, name=None):
     w("}  // namespace kx")
     meta = {"name": ns, "species": [s.name for s in net.species],
             "charge": charges, "mass_g": masses, "params": pnames,
+            "elements": elems,
+            "natoms": [[sp.comp.get(e, 0) for e in elems]
+                       for sp in net.species],
             "param_defaults": [net.params.get(p, PARAM_DEFAULTS.get(p))
                                for p in pnames],
             "nrea": NR, "thermal": thermal_on,
@@ -909,6 +933,8 @@ def meta_header(meta, th):
          "// the species and parameter names of network %s" % meta["name"],
          "const char* const kSpecies[] = {%s};" % cstr(meta["species"]),
          "const char* const kParams[] = {%s};" % cstr(meta["params"]),
+         "const char* const kElements[] = {%s};" % cstr(
+             meta["elements"] or [""]),
          "const double kParamDefaults[] = {%s};" % ", ".join(
              "NAN" if (isinstance(d, float) and math.isnan(d)) else repr(float(d))
              for d in meta["param_defaults"]),

@@ -1,0 +1,219 @@
+// ORACLE (test infrastructure only; see kx_oracle.cpp): a CPU restatement
+// of KROME-X's element totals and consistent advection, the checker of the
+// device kernels in krome-x_amd/csrc/kx_conserve.h.
+//   totals, totals_x      runtime/core/query.hpp:152-170
+//   row                   runtime/core/solve_cell.hpp:1331-1335
+//   residual              solve_cell.hpp:1336-1353
+//   project               solve_cell.hpp:1355-1432
+//   fixed species         solve_cell.hpp:1434-1445 (opaque species and
+//                         freeze directives: the networks of
+//                         krome-x_amd/csrc/generated have neither)
+//   post-solve mode 2     solve_cell.hpp:1469-1489
+//   to_n, to_x            solve_cell.hpp:1491-1562
+//   cells                 runtime/core/solve_cells.hpp:216-242 (the lowest
+//                         failing cell's flag)
+// Plain loops over the tables of the generated network (element counts
+// natoms(i, e), charge(i), mass_g(i)), compiled without contraction like
+// the reference's bitwise test build.  Pinned by tests/test_oracle.py
+// against the reference's own outputs (tests/golden/*_conserve.npz) and
+// against the projection's defining properties.
+#pragma once
+#include <cmath>
+#include <vector>
+
+namespace kxo_cons {
+
+constexpr int FLAG_CONSERVE_UNREACHED = -112;
+constexpr int FLAG_CONSERVE_LARGE = 2;
+
+template <class Net>
+double row(int k, int i) {
+  return k < Net::NELEM ? Net::natoms(i, k) : Net::charge(i);
+}
+
+template <class Net>
+void totals(const double* n, double* tot) {
+  for (int e = 0; e < Net::NELEM; ++e) tot[e] = 0.0;
+  for (int i = 0; i < Net::NSPEC; ++i)
+    for (int e = 0; e < Net::NELEM; ++e)
+      tot[e] += Net::natoms(i, e) * n[i];
+}
+
+template <class Net>
+void totals_x(const double* x, double* tot) {
+  for (int e = 0; e < Net::NELEM; ++e) tot[e] = 0.0;
+  for (int i = 0; i < Net::NSPEC; ++i) {
+    const double m = Net::mass_g(i);
+    if (!(m > 0.0)) continue;
+    const double ni = x[i] / m;
+    for (int e = 0; e < Net::NELEM; ++e) tot[e] += Net::natoms(i, e) * ni;
+  }
+}
+
+// max over the nr rows of |target - A n| / (|target| + sum |A| n)
+template <class Net>
+double residual(const double* n, const double* target, int nr) {
+  double worst = 0.0;
+  for (int k = 0; k < nr; ++k) {
+    double t = 0.0, sc = std::fabs(target[k]);
+    for (int i = 0; i < Net::NSPEC; ++i) {
+      const double a = row<Net>(k, i) * n[i];
+      t += a;
+      sc += std::fabs(a);
+    }
+    if (sc > 0.0) worst = std::fmax(worst, std::fabs(target[k] - t) / sc);
+  }
+  return worst;
+}
+
+// n moved to the totals of the nr rows by n_i (1 + sum_k A_ki lambda_k),
+// (A W A^T) lambda = target - A n, W = diag(n) over the free species;
+// elimination with partial pivoting, pivots below 1e-14 of the largest
+// diagonal skipped (lambda 0); repeated after clipping negatives, at most
+// 20 times, until the residual is below 1e-13
+template <class Net>
+double project(double* n, const double* target, const bool* fixed, int nr) {
+  constexpr int NS = Net::NSPEC;
+  for (int it = 0; it < 20; ++it) {
+    std::vector<double> r(nr), M(nr * nr), lam(nr, 0.0);
+    std::vector<bool> used(nr, false);
+    for (int k = 0; k < nr; ++k) {
+      double t1 = 0.0;
+      for (int i = 0; i < NS; ++i) t1 += row<Net>(k, i) * n[i];
+      r[k] = target[k] - t1;
+      for (int l = 0; l < nr; ++l) {
+        double s = 0.0;
+        for (int i = 0; i < NS; ++i)
+          if (!fixed[i]) s += row<Net>(k, i) * row<Net>(l, i) * n[i];
+        M[k * nr + l] = s;
+      }
+    }
+    double scale = 0.0;
+    for (int k = 0; k < nr; ++k) scale = std::fmax(scale, std::fabs(M[k * nr + k]));
+    for (int c = 0; c < nr; ++c) {
+      int p = -1;
+      double best = 1e-14 * scale;
+      for (int q = c; q < nr; ++q)
+        if (std::fabs(M[q * nr + c]) > best) {
+          best = std::fabs(M[q * nr + c]);
+          p = q;
+        }
+      if (p < 0) continue;
+      if (p != c) {
+        for (int l = 0; l < nr; ++l) std::swap(M[p * nr + l], M[c * nr + l]);
+        std::swap(r[p], r[c]);
+      }
+      used[c] = true;
+      for (int q = c + 1; q < nr; ++q) {
+        const double f = M[q * nr + c] / M[c * nr + c];
+        for (int l = c; l < nr; ++l) M[q * nr + l] -= f * M[c * nr + l];
+        r[q] -= f * r[c];
+      }
+    }
+    for (int c = nr - 1; c >= 0; --c) {
+      if (!used[c]) continue;
+      double s = r[c];
+      for (int l = c + 1; l < nr; ++l) s -= M[c * nr + l] * lam[l];
+      lam[c] = s / M[c * nr + c];
+    }
+    bool clipped = false;
+    for (int i = 0; i < NS; ++i) {
+      if (fixed[i]) continue;
+      double d = 0.0;
+      for (int k = 0; k < nr; ++k) d += row<Net>(k, i) * lam[k];
+      n[i] += n[i] * d;
+      if (n[i] < 0.0) {
+        n[i] = 0.0;
+        clipped = true;
+      }
+    }
+    if (!clipped || residual<Net>(n, target, nr) < 1e-13) break;
+  }
+  return residual<Net>(n, target, nr);
+}
+
+// the post-solve conservation of mode 2: n onto the element totals and
+// the charge of its input n0
+template <class Net>
+void post_solve(const double* n0, double* n) {
+  constexpr int NR = Net::NELEM + 1;
+  double target[NR];
+  for (int k = 0; k < NR; ++k) {
+    target[k] = 0.0;
+    for (int i = 0; i < Net::NSPEC; ++i) target[k] += row<Net>(k, i) * n0[i];
+  }
+  bool fixed[Net::NSPEC] = {};
+  project<Net>(n, target, fixed, NR);
+}
+
+// consistent advection of one cell: 0, FLAG_CONSERVE_UNREACHED or
+// FLAG_CONSERVE_LARGE
+template <class Net>
+int to_n(double* n, const double* ref, const bool* fixed) {
+  constexpr int NS = Net::NSPEC, NE = Net::NELEM;
+  double n0[NS];
+  for (int i = 0; i < NS; ++i) {
+    if (!fixed[i] && n[i] < 0.0) n[i] = 0.0;
+    n0[i] = n[i];
+  }
+  double target[NE + 1];
+  for (int k = 0; k < NE; ++k) target[k] = ref[k];
+  target[NE] = 0.0;
+  project<Net>(n, target, fixed, NE);
+  bool charge_done = false;
+  if (Net::HAS_E && !fixed[Net::I_E]) {
+    double qi = 0.0;
+    for (int i = 0; i < NS; ++i)
+      if (i != Net::I_E) qi += Net::charge(i) * n[i];
+    if (qi >= 0.0) {
+      n[Net::I_E] = qi;
+      charge_done = true;
+    } else {
+      n[Net::I_E] = 0.0;
+    }
+  }
+  if (!charge_done) project<Net>(n, target, fixed, NE + 1);
+  if (residual<Net>(n, target, NE + 1) > 1e-12) return FLAG_CONSERVE_UNREACHED;
+  for (int i = 0; i < NS; ++i)
+    if (n0[i] > 0.0 && std::fabs(n[i] - n0[i]) > 0.5 * n0[i])
+      return FLAG_CONSERVE_LARGE;
+  return 0;
+}
+
+// the same on mass fractions (ref per gram): n = x / m, species without a
+// mass held
+template <class Net>
+int to_x(double* x, const double* ref) {
+  constexpr int NS = Net::NSPEC;
+  bool fixed[NS];
+  double n[NS];
+  for (int i = 0; i < NS; ++i) {
+    const double m = Net::mass_g(i);
+    fixed[i] = !(m > 0.0);
+    n[i] = m > 0.0 ? x[i] / m : 0.0;
+  }
+  const int f = to_n<Net>(n, ref, fixed);
+  for (int i = 0; i < NS; ++i)
+    if (!fixed[i]) x[i] = Net::mass_g(i) * n[i];
+  return f;
+}
+
+// rows v[ncell NSPEC] moved to the rows ref[c ref_rows NELEM] (ref_rows 0:
+// one for all); flags[ncell]; the lowest failing cell's flag, else 0
+template <class Net>
+int cells(int ncell, double* v, const double* ref, int ref_rows, bool x,
+          int* flags) {
+  constexpr int NS = Net::NSPEC;
+  bool fixed[NS] = {};
+  int first = 0;
+  for (int c = 0; c < ncell; ++c) {
+    double* vc = v + static_cast<size_t>(c) * NS;
+    const double* rc = ref + static_cast<size_t>(c) * ref_rows * Net::NELEM;
+    const int f = x ? to_x<Net>(vc, rc) : to_n<Net>(vc, rc, fixed);
+    if (flags) flags[c] = f;
+    if (first == 0 && f < 0) first = f;
+  }
+  return first;
+}
+
+}  // namespace kxo_cons

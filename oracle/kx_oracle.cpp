@@ -12,6 +12,8 @@
 //                         block; networks with density-dependent rates
 //                         are refused at fixed T)
 //   solve_cells_percell   runtime/core/solve_cells.hpp:47-118, 130-161
+//   conservation          oracle/kx_conserve.hpp (element totals, the
+//                         post-solve mode 2, consistent advection)
 // over the networks of krome-x_amd/csrc/generated, with the integrator of
 // oracle/kx_bdf.hpp, or with -DKX_REF_BDF the reference's own
 // runtime/core/cvode/bdf_cell.hpp (oracle/Makefile target ref).
@@ -25,6 +27,7 @@
 
 #include "net_primordial.h"
 #include "net_primordial3.h"
+#include "kx_conserve.hpp"
 
 #ifdef KX_REF_BDF
 #include "bdf_cell.hpp"
@@ -180,6 +183,8 @@ int solve_one(double* n, double& T, const double* row, double dt,
     for (int i = 0; i < NS; ++i)
       if (i != Net::I_E) qi += Net::charge(i) * n[i];
     n[Net::I_E] = std::fmax(qi, 0.0);
+  } else if (mode == 2) {  // solve_cell.hpp:1484-1488
+    kxo_cons::post_solve<Net>(n0, n);
   }
   if (evolve_T) T = y[NS];
   return 0;
@@ -190,7 +195,6 @@ int solve_cells(int ncell, double* n, double* T, const double* dt,
                 const double* par, int par_stride, const double* o,
                 int* flags, long* stats) {
   constexpr int NS = Net::NSPEC;
-  if (static_cast<int>(o[O_CONSERVE]) == 2) return FLAG_UNSUPPORTED;
   std::vector<double> dflt(Net::NPAR, 0.0);
   bool missing = false;
   for (int c = 0; c < ncell; ++c) {
@@ -327,6 +331,42 @@ int KXO_(jac)(int net, int ncell, const double* n, const double* T,
     case 1:
       jacs<kx::net_primordial3>(ncell, n, T, par, J);
       return 0;
+  }
+  return -1;
+}
+
+// element totals tot[ncell NELEM] of the rows v[ncell NSPEC]: densities
+// (x 0) or mass fractions (x 1, per gram)
+int KXO_(totals)(int net, int x, int ncell, const double* v, double* tot) {
+  auto run = [&](auto tag) {
+    using Net = decltype(tag);
+    for (int c = 0; c < ncell; ++c) {
+      const double* vc = v + static_cast<size_t>(c) * Net::NSPEC;
+      double* tc = tot + static_cast<size_t>(c) * Net::NELEM;
+      if (x) kxo_cons::totals_x<Net>(vc, tc);
+      else kxo_cons::totals<Net>(vc, tc);
+    }
+    return 0;
+  };
+  switch (net) {
+    case 0: return run(kx::net_primordial{});
+    case 1: return run(kx::net_primordial3{});
+  }
+  return -1;
+}
+
+// consistent advection of the rows v[ncell NSPEC] to ref (ref_rows 1: a
+// row per cell, 0: one for all); flags[ncell]; returns the lowest failing
+// cell's flag, else 0 (krome_conserve_cells / _x)
+int KXO_(conserve)(int net, int x, int ncell, double* v, const double* ref,
+                   int ref_rows, int* flags) {
+  switch (net) {
+    case 0:
+      return kxo_cons::cells<kx::net_primordial>(ncell, v, ref, ref_rows,
+                                                 x != 0, flags);
+    case 1:
+      return kxo_cons::cells<kx::net_primordial3>(ncell, v, ref, ref_rows,
+                                                  x != 0, flags);
   }
   return -1;
 }

@@ -70,6 +70,30 @@ class Oracle:
             k.ctypes.data_as(D), f.ctypes.data_as(D))
         return k, f
 
+    def totals(self, net, v, x=False):
+        D = ctypes.POINTER(ctypes.c_double)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        nel = len(__import__("cells").network_meta(net)["elements"])
+        tot = np.zeros((v.shape[0], nel))
+        r = getattr(self.lib, self.p + "totals")(
+            NET_ID[net], int(x), v.shape[0], v.ctypes.data_as(D),
+            tot.ctypes.data_as(D))
+        assert r == 0
+        return tot
+
+    def conserve(self, net, v, ref, x=False):
+        """(return value, corrected rows, flags); ref [nelem] (one row for
+        all) or [ncell, nelem]"""
+        D = ctypes.POINTER(ctypes.c_double)
+        v = np.array(v, dtype=np.float64, order="C", copy=True)
+        ref = np.ascontiguousarray(ref, dtype=np.float64)
+        fl = np.zeros(v.shape[0], dtype=np.intc)
+        r = getattr(self.lib, self.p + "conserve")(
+            NET_ID[net], int(x), v.shape[0], v.ctypes.data_as(D),
+            ref.ctypes.data_as(D), 1 if ref.ndim == 2 else 0,
+            fl.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        return r, v, fl
+
 
 @pytest.fixture(scope="session")
 def oracle():

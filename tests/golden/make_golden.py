@@ -14,7 +14,15 @@ For each workload it:
      krome::solve_cells (PATH_PERCELL) and evaluates the rates, the full
      rhs, the analytic Jacobian and the thermal terms at sample states;
   4. writes <name>_solve.npz and <name>_funcs.npz (inputs and outputs
-     only: numbers, no reference code).
+     only: numbers, no reference code);
+  5. for the conservation API (runtime/core/query.hpp:152-170,
+     solve_cell.hpp:1331-1562, solve_cells.hpp:216-260): the element
+     tables, krome::get_element_totals(_x) and krome::conserve_cells(_x) of
+     seeded perturbed cells, and solve_cells with SolverOpts::conserve 2
+     on the solve fixture's cells -> <name>_conserve.npz.
+
+  python3 tests/golden/make_golden.py [solve] [funcs] [conserve]
+  (default: all three)
 
 The generated project and the binaries stay in _work/ (not committed).
 """
@@ -72,21 +80,25 @@ def compile_driver(proj, name, mode):
     return exe
 
 
-def write_in(path, w):
+def write_in(path, w, opts=None, extra=None):
     n, T, dt, par = w["n"], w["T"], w["dt"], w["par"]
-    opts = np.array([float(w["opts"][k]) for k in OPT_KEYS])
+    if opts is None:
+        opts = np.array([float(w["opts"][k]) for k in OPT_KEYS])
+    opts = np.asarray(opts, dtype=np.float64)
     with open(path, "wb") as f:
         np.array([n.shape[0], n.shape[1], par.shape[1], opts.size],
                  dtype=np.int64).tofile(f)
         opts.tofile(f)
         for a in (n, T, dt, par):
             np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+        if extra is not None:
+            np.ascontiguousarray(extra, dtype=np.float64).tofile(f)
 
 
-def run_solve(exe, w, tag):
+def run_solve(exe, w, tag, opts=None):
     fin = BUILD / ("in_%s.bin" % tag)
     fout = BUILD / ("out_%s.bin" % tag)
-    write_in(fin, w)
+    write_in(fin, w, opts)
     env = dict(os.environ, OMP_NUM_THREADS=str(os.cpu_count() or 1))
     subprocess.run([str(exe), "solve", str(fin), str(fout)], check=True,
                    env=env)
@@ -132,34 +144,162 @@ def run_funcs(exe, w, tag, nrea, neq):
             raw[:, nrea + neq + neq * neq:])
 
 
-def main():
+# cells of the conservation fixtures; perturbation group = cell % 8
+NCONSERVE = 512
+
+
+def run_mode(exe, mode, w, tag, opts, extra=None):
+    fin = BUILD / ("in_%s.bin" % tag)
+    fout = BUILD / ("out_%s.bin" % tag)
+    write_in(fin, w, opts, extra)
+    subprocess.run([str(exe), mode, str(fin), str(fout)], check=True)
+    return np.fromfile(fout, dtype=np.float64), fout
+
+
+def ref_tables(exe, name, w):
+    raw, fout = run_mode(exe, "meta", w, name + "_meta", [0.0])
+    nelem = int(raw[0])
+    ns = w["n"].shape[1]
+    o = 1
+    elements = raw[o:o + ns * nelem].reshape(ns, nelem).astype(np.int64)
+    o += ns * nelem
+    mass = raw[o:o + ns]
+    charge = raw[o + ns:o + 2 * ns].astype(np.int64)
+    names = Path(str(fout) + ".names").read_text().split()
+    return {"element_names": np.array(names), "species_elements": elements,
+            "species_mass_g": mass, "species_charge": charge}
+
+
+def perturb(name, v, rng):
+    """seeded perturbations of the rows v by group (cell % 8): 0-2 each
+    species x (1 +- 30 %); 3 the same and two species negative; 4 the
+    hydrogen carriers x 3 (a large change); 5 no helium carriers (the
+    helium total cannot be reached); 6 H- ten times the cations (anions
+    dominate the charge); 7 the 30 % noise only (its reference row is the
+    next cell's).  The last cell is all zero"""
+    sp = cells.network_meta(name)["species"]
+    col = {s: i for i, s in enumerate(sp)}
+    nc, ns = v.shape
+    p = v * (1.0 + 0.3 * (2.0 * rng.random(v.shape) - 1.0))
+    g = np.arange(nc) % 8
+    hyd = [col[s] for s in sp if s[0] == "H" and not s.startswith("He")]
+    hel = [col[s] for s in sp if s.startswith("He")]
+    cat = [i for i, s in enumerate(sp) if s.endswith("+")]
+    for c in range(nc):
+        if g[c] == 3:
+            for i in rng.choice(ns, 2, replace=False):
+                p[c, i] = -0.1 * abs(p[c, i]) - 1e-30
+        elif g[c] == 4:
+            p[c, hyd] *= 3.0
+        elif g[c] == 5:
+            p[c, hel] = 0.0
+        elif g[c] == 6:
+            p[c, col["H-"]] = 10.0 * p[c, cat].sum()
+    p[-1] = 0.0
+    return p
+
+
+def make_conserve(name, exe, meta):
+    w = cells.WORKLOADS[name](NCONSERVE)
+    if "D" in meta["species"]:
+        # deuterium at a cosmic abundance in half of the cells (the
+        # workload's 1e-20 nH is below the projection's pivot threshold,
+        # 1e-14 of the largest diagonal: the D row stays unreached there)
+        col = {s: i for i, s in enumerate(meta["species"])}
+        n = w["n"].copy()
+        rows = (np.arange(NCONSERVE) // 8) % 2 == 0
+        nD = 2.6e-5 * n[rows, col["H"]]
+        n[rows, col["D"]] = nD
+        n[rows, col["D+"]] = 1e-4 * nD
+        n[rows, col["HD"]] = 1e-3 * nD
+        w = dict(w, n=n)
+    tabs = ref_tables(exe, name, w)
+    nelem = len(tabs["element_names"])
+    nc = NCONSERVE
+    # the totals of the workload cells: the reference rows of the cells
+    raw, _ = run_mode(exe, "totals", w, name + "_totals", [0.0])
+    ref_n = raw[:nc * nelem].reshape(nc, nelem)
+    mass = np.array(meta["mass_g"])
+    rho = w["n"] @ mass
+    x0 = w["n"] * mass[None, :] / rho[:, None]
+    wx = dict(w, n=x0)
+    raw, _ = run_mode(exe, "totals", wx, name + "_totals_x", [0.0])
+    ref_x = raw[nc * nelem:].reshape(nc, nelem)
+    ref_n = ref_n.copy()
+    ref_x = ref_x.copy()
+    g = np.arange(nc) % 8
+    shift = np.where(g == 7)[0]
+    ref_n[shift] = ref_n[(shift + 1) % nc]
+    ref_x[shift] = ref_x[(shift + 1) % nc]
+    rng = np.random.default_rng(20260 + len(meta["species"]))
+    pn = perturb(name, w["n"], rng)
+    px = perturb(name, x0, rng)
+    # the totals of the perturbed rows (negatives included), as n and as x
+    raw, _ = run_mode(exe, "totals", dict(w, n=pn), name + "_totals_p",
+                      [0.0])
+    tot_p, totx_p = (raw[:nc * nelem].reshape(nc, nelem),
+                     raw[nc * nelem:].reshape(nc, nelem))
+    out = {}
+    for mode, v0, ref in (("n", pn, ref_n), ("x", px, ref_x)):
+        raw, _ = run_mode(exe, "conserve", dict(w, n=v0),
+                          "%s_conserve_%s" % (name, mode),
+                          [1.0 if mode == "x" else 0.0], extra=ref)
+        ns = v0.shape[1]
+        out["in_" + mode] = v0
+        out["ref_" + mode] = ref
+        out["out_" + mode] = raw[:nc * ns].reshape(nc, ns)
+        out["flag_" + mode] = raw[nc * ns:].astype(np.int32)
+        print("%s conserve %s: flags %s" % (
+            name, mode, dict(zip(*np.unique(out["flag_" + mode],
+                                            return_counts=True)))))
+    # solve_cells with SolverOpts::conserve 2 on the solve fixture's cells
+    ws = cells.WORKLOADS[name](NSOLVE[name])
+    opts = [float(ws["opts"][k]) for k in OPT_KEYS] + [2.0]
+    n2, T2, f2, _, _ = run_solve(exe, ws, name + "_solve_c2", opts)
+    np.savez_compressed(
+        HERE / ("%s_conserve.npz" % name), par=w["par"], n0=w["n"],
+        x0=x0, rho=rho, tot_p=tot_p, totx_p=totx_p,
+        solve_n0=ws["n"], solve_T0=ws["T"], solve_dt=ws["dt"],
+        solve_par=ws["par"], solve_opts=np.array(opts), solve_n=n2,
+        solve_T=T2, solve_flag=f2, **tabs, **out)
+    print("%s conserve-2 solve: flags %s" % (name, np.unique(f2)))
+
+
+def main(argv=None):
+    parts = set(argv if argv else ("solve", "funcs", "conserve"))
     BUILD.mkdir(parents=True, exist_ok=True)
     for name in ("primordial", "primordial3"):
         meta = cells.network_meta(name)
         proj = generate(name)
         exes = {m: compile_driver(proj, name, m)
                 for m in ("bitwise", "production")}
+        if "conserve" in parts:
+            make_conserve(name, exes["bitwise"], meta)
+        if not parts & {"solve", "funcs"}:
+            continue
         w = cells.WORKLOADS[name](NSOLVE[name])
-        res = {}
-        for m, exe in exes.items():
-            n, T, flag, stats, secs = run_solve(exe, w, "%s_%s" % (name, m))
-            res.update({"n_" + m: n, "T_" + m: T, "flag_" + m: flag,
-                        "stats_" + m: stats, "seconds_" + m: secs})
-            print("%s %s: %d cells in %.2f s, flags %s" % (
-                name, m, n.shape[0], secs, np.unique(flag)))
-        np.savez_compressed(HERE / ("%s_solve.npz" % name), n0=w["n"],
-                            T0=w["T"], dt=w["dt"], par=w["par"],
-                            opts=np.array([float(w["opts"][k])
-                                           for k in OPT_KEYS]), **res)
-        fw = func_samples(name)
-        neq = len(meta["species"]) + (1 if w["opts"]["evolve_T"] else 0)
-        k, f, J, th = run_funcs(exes["bitwise"], fw, name + "_funcs",
-                                meta["nrea"], neq)
-        np.savez_compressed(HERE / ("%s_funcs.npz" % name), n=fw["n"],
-                            T=fw["T"], par=fw["par"], k=k, f=f, J=J,
-                            thermal=th)
-        print("%s funcs: %d samples" % (name, k.shape[0]))
-
+        if "solve" in parts:
+            res = {}
+            for m, exe in exes.items():
+                n, T, flag, stats, secs = run_solve(exe, w,
+                                                    "%s_%s" % (name, m))
+                res.update({"n_" + m: n, "T_" + m: T, "flag_" + m: flag,
+                            "stats_" + m: stats, "seconds_" + m: secs})
+                print("%s %s: %d cells in %.2f s, flags %s" % (
+                    name, m, n.shape[0], secs, np.unique(flag)))
+            np.savez_compressed(HERE / ("%s_solve.npz" % name), n0=w["n"],
+                                T0=w["T"], dt=w["dt"], par=w["par"],
+                                opts=np.array([float(w["opts"][k])
+                                               for k in OPT_KEYS]), **res)
+        if "funcs" in parts:
+            fw = func_samples(name)
+            neq = len(meta["species"]) + (1 if w["opts"]["evolve_T"] else 0)
+            k, f, J, th = run_funcs(exes["bitwise"], fw, name + "_funcs",
+                                    meta["nrea"], neq)
+            np.savez_compressed(HERE / ("%s_funcs.npz" % name), n=fw["n"],
+                                T=fw["T"], par=fw["par"], k=k, f=f, J=J,
+                                thermal=th)
+            print("%s funcs: %d samples" % (name, k.shape[0]))
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -17,6 +17,21 @@
 //                               (column-major, the solver's analytic
 //                               Jacobian), and the thermal terms
 //                               (8 values, 0 when not configured)
+//   ref_driver totals IN OUT    IN as above (n0 the cells).  OUT:
+//                               krome::get_element_totals of each row
+//                               [ncell NELEM], then get_element_totals_x
+//                               of the same rows read as mass fractions
+//   ref_driver conserve IN OUT  IN as above, opts[0] 0 (n0 densities) or
+//                               1 (mass fractions), followed by
+//                               ref[ncell NELEM].  OUT: krome::
+//                               conserve_cells(_x) (ref_stride 1, the
+//                               cells' params): rows[ncell nspec],
+//                               flag[ncell]
+//   ref_driver meta IN OUT      OUT: NELEM, species_elements[NSPEC NELEM],
+//                               species_mass_g[NSPEC], species_charge
+//                               [NSPEC] (f64); OUT.names: the element
+//                               names, one a line
+//   (solve: opts[7], when given, is SolverOpts::conserve)
 #include "krome.hpp"
 
 #include <chrono>
@@ -24,6 +39,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 namespace {
@@ -66,6 +82,7 @@ krome::SolverOpts solver_opts(const In& in) {
   so.jac = jc == 1 ? krome::JAC_ANALYTIC
                    : (jc == 2 ? krome::JAC_FD : krome::JAC_AUTO);
   so.max_steps = static_cast<long>(in.opts[6]);
+  if (in.nopt > 7) so.conserve = static_cast<int>(in.opts[7]);
   so.error_report = false;
   return so;
 }
@@ -169,11 +186,75 @@ int do_funcs(const In& in, const char* out) {
   return 0;
 }
 
+int do_meta(const char* out) {
+  FILE* f = std::fopen(out, "wb");
+  if (!f) return 3;
+  std::vector<double> v;
+  v.push_back(krome::NELEM);
+  for (int i = 0; i < krome::NSPEC * krome::NELEM; ++i)
+    v.push_back(species_elements[i]);
+  for (int i = 0; i < krome::NSPEC; ++i) v.push_back(species_mass_g[i]);
+  for (int i = 0; i < krome::NSPEC; ++i) v.push_back(species_charge[i]);
+  std::fwrite(v.data(), sizeof(double), v.size(), f);
+  std::fclose(f);
+  const std::string nm = std::string(out) + ".names";
+  f = std::fopen(nm.c_str(), "w");
+  if (!f) return 3;
+  for (int e = 0; e < krome::NELEM; ++e)
+    std::fprintf(f, "%s\n", krome::element_names[e]);
+  std::fclose(f);
+  return 0;
+}
+
+int do_totals(const In& in, const char* out) {
+  if (in.nspec != krome::NSPEC) return 2;
+  FILE* f = std::fopen(out, "wb");
+  if (!f) return 3;
+  std::vector<double> tot(krome::NELEM > 0 ? krome::NELEM : 1);
+  for (int64_t c = 0; c < in.ncell; ++c) {
+    krome::get_element_totals(in.n0.data() + c * in.nspec, tot.data());
+    std::fwrite(tot.data(), sizeof(double), krome::NELEM, f);
+  }
+  for (int64_t c = 0; c < in.ncell; ++c) {
+    krome::get_element_totals_x(in.n0.data() + c * in.nspec, tot.data());
+    std::fwrite(tot.data(), sizeof(double), krome::NELEM, f);
+  }
+  std::fclose(f);
+  return 0;
+}
+
+int do_conserve(const In& in, const std::vector<double>& ref,
+                const char* out) {
+  if (in.nspec != krome::NSPEC || in.npar != krome::NPAR) return 2;
+  std::vector<krome::Params> P(in.ncell);
+  for (int64_t c = 0; c < in.ncell; ++c)
+    krome::params_from_row(in.par.data() + c * in.npar, P[c]);
+  std::vector<double> v = in.n0;
+  std::vector<int> flags(in.ncell, 0);
+  const bool x = in.opts[0] != 0.0;
+  if (x)
+    krome::conserve_cells_x(static_cast<int>(in.ncell), v.data(), ref.data(),
+                            1, P.data(), 1, flags.data());
+  else
+    krome::conserve_cells(static_cast<int>(in.ncell), v.data(), ref.data(),
+                          1, P.data(), 1, flags.data());
+  FILE* f = std::fopen(out, "wb");
+  if (!f) return 3;
+  std::fwrite(v.data(), sizeof(double), v.size(), f);
+  for (int64_t c = 0; c < in.ncell; ++c) {
+    const double fl = flags[c];
+    std::fwrite(&fl, sizeof(double), 1, f);
+  }
+  std::fclose(f);
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
   if (argc != 4) {
-    std::fprintf(stderr, "usage: %s solve|funcs IN OUT\n", argv[0]);
+    std::fprintf(stderr, "usage: %s solve|funcs|totals|conserve|meta IN OUT\n",
+                 argv[0]);
     return 1;
   }
   In in;
@@ -183,5 +264,20 @@ int main(int argc, char** argv) {
   }
   if (!std::strcmp(argv[1], "solve")) return do_solve(in, argv[3]);
   if (!std::strcmp(argv[1], "funcs")) return do_funcs(in, argv[3]);
+  if (!std::strcmp(argv[1], "totals")) return do_totals(in, argv[3]);
+  if (!std::strcmp(argv[1], "meta")) return do_meta(argv[3]);
+  if (!std::strcmp(argv[1], "conserve")) {
+    // ref[ncell NELEM] follows the standard input in the same file
+    FILE* f = std::fopen(argv[2], "rb");
+    if (!f) return 1;
+    const long skip = 8 * (4 + in.nopt + in.ncell * in.nspec + 2 * in.ncell +
+                           in.ncell * in.npar);
+    std::fseek(f, skip, SEEK_SET);
+    std::vector<double> ref(static_cast<size_t>(in.ncell) * krome::NELEM);
+    const size_t m = std::fread(ref.data(), sizeof(double), ref.size(), f);
+    std::fclose(f);
+    if (m != ref.size()) return 1;
+    return do_conserve(in, ref, argv[3]);
+  }
   return 1;
 }

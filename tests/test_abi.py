@@ -126,3 +126,55 @@ def test_argument_checks_without_device_work(net):
                             fl.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                             o._h)
     assert r == -104 and (fl == -104).all()
+
+
+def test_element_metadata_vs_reference(net):
+    """element names, atom counts, masses and charges equal the tables of
+    the reference's generated project (tests/golden/*_conserve.npz)"""
+    from conftest import golden
+    g = golden("%s_conserve.npz" % net.name)
+    assert net.elements == tuple(g["element_names"])
+    assert np.array_equal(net.composition, g["species_elements"])
+    assert np.array_equal(net.masses, g["species_mass_g"])
+    assert np.array_equal(net.charges, g["species_charge"])
+    L = net._lib
+    assert L.krome_element_name(-1) is None
+    assert L.krome_element_name(len(net.elements)) is None
+    assert L.krome_species_natoms(net.nspec, 0) == -1
+    assert L.krome_species_natoms(0, len(net.elements)) == -1
+    assert math.isnan(L.krome_species_mass(net.nspec))
+    assert L.krome_species_charge(-1) == 0
+
+
+def test_conserve_argument_checks_without_device_work(net):
+    """krome.cpp:929-977, 1306-1343: -1 on bad arguments, 0 for no cells"""
+    L = net._lib
+    D = ctypes.POINTER(ctypes.c_double)
+    nel = len(net.elements)
+    n = np.ones((2, net.nspec))
+    ref = np.ones((2, nel))
+    tot = np.zeros((2, nel))
+    nd, rd = n.ctypes.data_as(D), ref.ctypes.data_as(D)
+    assert L.krome_get_element_totals(-1, nd, tot.ctypes.data_as(D)) == -1
+    assert L.krome_get_element_totals(2, None, tot.ctypes.data_as(D)) == -1
+    assert L.krome_get_element_totals_x(2, nd, None) == -1
+    assert L.krome_get_element_totals(0, nd, tot.ctypes.data_as(D)) == 0
+    for f in (L.krome_conserve_cells, L.krome_conserve_cells_x):
+        assert f(-1, nd, rd, nel, None, 0, None) == -1
+        assert f(2, None, rd, nel, None, 0, None) == -1
+        assert f(2, nd, None, nel, None, 0, None) == -1
+        assert f(2, nd, rd, nel + 1, None, 0, None) == -1
+        assert f(2, nd, rd, -1, None, 0, None) == -1
+        assert f(2, nd, rd, nel, None, -1, None) == -1
+        assert f(0, nd, rd, nel, None, 0, None) == 0
+    p = ctypes.c_void_p(16)  # never dereferenced: rejected or no cells
+    for f in (L.krome_conserve_cells_device, L.krome_conserve_cells_x_device):
+        assert f(-1, p, 0, 0, p, nel, None, 0, None) == -1
+        assert f(2, None, 0, 0, p, nel, None, 0, None) == -1
+        assert f(2, p, 0, 0, None, nel, None, 0, None) == -1
+        assert f(2, p, 0, 0, p, nel + 1, None, 0, None) == -1
+        assert f(2, p, 0, 0, p, nel, p, net.npar + 1, None) == -1
+        assert f(2, p, 2, 0, p, nel, None, 0, None) == -1
+        assert f(2, p, 0, 1, p, nel, None, 0, None) == -1
+        assert f(2, p, 1, 1, p, nel, None, 0, None) == -1
+        assert f(0, p, 0, 0, p, nel, None, 0, None) == 0

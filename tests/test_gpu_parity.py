@@ -206,3 +206,48 @@ def test_large_batch_properties(nets, oracle, kernel, mixed):
                                      oracle_opts(evolve_T=1))
     assert rel_err(nn[idx], no) < _tol("primordial3", mixed)
     assert np.max(np.abs(TT[idx] - To) / To) < _tol("primordial3", mixed)
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_mass_fractions(nets, layout):
+    """krome_solve_cells_x (host rows) and krome_solve_cells_x_device
+    (either layout): x = m n / rho of the reference's density solve, with
+    renormalize_x rescaled to the input sum; a dt <= 0 cell and a failed
+    cell keep their x on the device"""
+    import torch
+    net = nets["primordial3"]
+    d = golden("primordial3_solve.npz")
+    m = net.masses
+    nc = d["n0"].shape[0]
+    rho = (d["n0"] @ m) * 1.25  # the caller's rho: not only the species'
+    x0 = d["n0"] * m / rho[:, None]
+    say("mass fractions", layout)
+    o = _opts(net, d, "persistent", 0, renormalize_x=True)
+    res = net.solve_x(x0, rho, d["T0"], d["dt"], par=d["par"], opts=o,
+                      check=False)
+    assert res.ierr == 0 and (res.flags == 0).all()
+    xr = d["n_bitwise"] * m / rho[:, None]
+    xr *= (x0.sum(axis=1) / xr.sum(axis=1))[:, None]
+    assert rel_err(res.n, xr) < TOL["primordial3"]
+    assert np.max(np.abs(res.Tgas - d["T_bitwise"]) / d["T_bitwise"]) < \
+        TOL["primordial3"]
+    dt = d["dt"].copy()
+    dt[1] = 0.0
+    T0 = d["T0"].copy()
+    T0[2] = -5.0
+    xt = torch.tensor(x0.T.copy() if layout == 0 else x0, device="cuda")
+    fl = torch.zeros(nc, dtype=torch.int32, device="cuda")
+    r = net.solve_x_device(xt, torch.tensor(rho, device="cuda"),
+                           torch.tensor(T0, device="cuda"),
+                           torch.tensor(dt, device="cuda"),
+                           torch.tensor(d["par"], device="cuda"), opts=o,
+                           flags=fl, layout=layout)
+    torch.cuda.synchronize()
+    assert r == -110
+    f = fl.cpu().numpy()
+    assert f[2] == -110 and (np.delete(f, 2) == 0).all()
+    got = (xt.T if layout == 0 else xt).cpu().numpy()
+    assert np.array_equal(got[[1, 2]], x0[[1, 2]])
+    ok = np.ones(nc, dtype=bool)
+    ok[[1, 2]] = False
+    assert rel_err(got[ok], xr[ok]) < TOL["primordial3"]
