@@ -6,6 +6,7 @@ gamma EXACT, H2 opacity RIPAMONTI; rtol 1e-8, atol_n 1e-20, atol_T 1e-6).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--ncell N]
                     [--kernel persistent|plain] [--fp64]
+                    [--workload solve|conserve]
 
 A step is one solve of the whole batch from its initial state: the inputs
 are resident in HBM (species-major n[nspec, ncell]); each step first
@@ -17,6 +18,14 @@ test); beside it, in the same run: the plain FP64 one-thread-per-cell
 drop-in kernel on the same cells (ratio), the agreement of the two, and
 the CPU restatement of the reference solver on a bounded sample of the
 same cells on the host's cores (cpu_baseline).
+
+--workload conserve measures the consistent-advection kernel instead
+(krome_conserve_cells_device, HBM-bound): the same 2^20 cells, each
+species perturbed by up to 30 % (one cell in 16 with a negative H-),
+moved back to their own element totals; species-major rows, one reference
+row per cell.  A step restores the rows from a resident copy and runs the
+call; the roofline is the kernel's algorithmic bytes over its event-timed
+duration.
 """
 from __future__ import annotations
 
@@ -97,6 +106,97 @@ def cpu_baseline(w, nsample):
                       "%.1f s wall, return %d" % (nsample, el, r)}, n, T
 
 
+def conserve_bench(a):
+    """cells/s of the consistent-advection kernel (one JSON line)"""
+    import torch
+    import cells
+    import krome
+
+    dev = torch.device("cuda:0")
+    net = krome.load("primordial3")
+    w = cells.primordial3_cells(a.ncell)
+    ref = net.element_totals(w["n"])
+    rng = np.random.default_rng(7)
+    v0 = w["n"] * (1.0 + 0.3 * (2.0 * rng.random(w["n"].shape) - 1.0))
+    neg = np.arange(a.ncell) % 16 == 3
+    v0[neg, 2] = -1e-3 * v0[neg, 2]
+    vt0 = torch.tensor(np.ascontiguousarray(v0.T), device=dev)
+    vt = vt0.clone()
+    rt = torch.tensor(ref, device=dev)
+    fl = torch.zeros(a.ncell, dtype=torch.int32, device=dev)
+    kms = ctypes.c_double()
+    kp = ctypes.cast(ctypes.byref(kms), ctypes.POINTER(ctypes.c_double))
+    for _ in range(max(a.warmup, 1)):
+        vt.copy_(vt0)
+        net.conserve_device(vt, rt, flags=fl)
+    torch.cuda.synchronize()
+    kern = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        vt.copy_(vt0)
+        r = net.conserve_device(vt, rt, flags=fl)
+        if net._lib.krome_device_last_kernel_ms(kp) != 0:
+            raise SystemExit("no kernel time")
+        kern.append(kms.value)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = float(np.mean(kern))
+    ns, ne = net.nspec, len(net.elements)
+    # read the row and the reference row, write the row and the flag
+    bpc = 8 * (2 * ns + ne) + 4
+    achieved = bpc * a.ncell / (kern_ms * 1e-3)
+    flg = fl.cpu().numpy()
+    traffic = None
+    tf = REPO / "profiles" / "traffic.json"
+    if tf.exists():
+        traffic = json.loads(tf.read_text()).get("conserve")
+    cpu = None
+    if not a.no_cpu:
+        lib = ctypes.CDLL(str(REPO / "oracle" / "_build" /
+                              "libkx_oracle_fast.so"))
+        lib.kxo_set_threads(min(16, os.cpu_count() or 1))
+        cores = lib.kxo_get_threads()
+        m = min(a.cpu_sample, a.ncell)
+        vc = np.ascontiguousarray(v0[:m])
+        rc = np.ascontiguousarray(ref[:m])
+        fc = np.zeros(m, dtype=np.intc)
+        D = ctypes.POINTER(ctypes.c_double)
+        t1 = time.perf_counter()
+        lib.kxo_conserve(1, 0, m, vc.ctypes.data_as(D), rc.ctypes.data_as(D),
+                         1, fc.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        el = time.perf_counter() - t1
+        cpu = {"value": m / el, "unit": "cells/s", "cores": int(cores),
+               "kind": "port",
+               "sample": "the first %d of the same cells, CPU restatement "
+                         "of the reference's conserve_cells (OpenMP static, "
+                         "the reference's production flags), %.2f s wall"
+                         % (m, el)}
+    out = {
+        "metric": "cells/sec, consistent advection (krome_conserve_cells_"
+                  "device)",
+        "value": a.ncell * a.steps / wall, "unit": "cells/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "primordial3 collapse cells (seed 1), each "
+                               "species x (1 +- 30 %), back to their own "
+                               "element totals", "ncell": a.ncell,
+                   "layout": "species-major", "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9,
+                     "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM, "traffic": traffic,
+                     "bytes_per_cell": bpc, "kernel_ms": kern_ms},
+        "cpu_baseline": cpu,
+        "flags": {str(k): int(v) for k, v in zip(*np.unique(
+            flg, return_counts=True))},
+        "return": int(r),
+        "ratio_vs_cpu": (a.ncell / (kern_ms * 1e-3)) / cpu["value"]
+        if cpu else None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,12 +210,16 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 18)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--workload", default="solve",
+                    choices=("solve", "conserve"))
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: the cells are independent and "
               "this build has no multi-GPU path (--gpus %d refused)" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if a.workload == "conserve":
+        return conserve_bench(a)
     import torch
     import cells
     import krome

@@ -160,9 +160,9 @@ int krome_solve_cells_device_stats(int ncell, double* n, int layout, long ld,
                                    const krome_opts* h);
 
 /* extension (no counterpart in the reference): the duration in ms of the
- * last device solve's kernels, from HIP events recorded around their
- * launches on their stream; waits for them to finish.  0, or -1 when no
- * device solve ran yet. */
+ * last device solve's (or conservation's) kernels, from HIP events
+ * recorded around their launches on their stream; waits for them to
+ * finish.  0, or -1 when no such call ran yet. */
 int krome_device_last_kernel_ms(double* ms);
 
 /* the name of a flag, e.g. "FLAG_TOO_MUCH_WORK" (the reference's

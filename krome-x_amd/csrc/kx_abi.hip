@@ -165,9 +165,22 @@ int launch_plain(kx::CellsArgs a, DevCache& dc, hipStream_t s) {
   return hip_ok(hipGetLastError(), "plain kernel launch") ? 0 : -1;
 }
 
-// HIP events around the last solve's kernel launches, on their stream
+// HIP events around the last device call's kernel launches (a solve, or
+// a conservation), on their stream
 hipEvent_t g_ev[2] = {nullptr, nullptr};
 bool g_ev_recorded = false;
+
+void ev_begin(hipStream_t s) {
+  if (!g_ev[0] && (hipEventCreate(&g_ev[0]) != hipSuccess ||
+                   hipEventCreate(&g_ev[1]) != hipSuccess))
+    g_ev[0] = g_ev[1] = nullptr;
+  if (g_ev[0]) (void)hipEventRecord(g_ev[0], s);
+}
+void ev_end(hipStream_t s) {
+  if (!g_ev[0]) return;
+  (void)hipEventRecord(g_ev[1], s);
+  g_ev_recorded = true;
+}
 
 int solve_resident_(kx::CellsArgs a, const Opts& o, DevCache& dc,
                     hipStream_t s);
@@ -198,15 +211,9 @@ int solve_conserved(kx::CellsArgs a, const Opts& o, DevCache& dc,
 // arrays already resident; flags always written
 int solve_resident(kx::CellsArgs a, const Opts& o, DevCache& dc,
                    hipStream_t s) {
-  if (!g_ev[0] && (hipEventCreate(&g_ev[0]) != hipSuccess ||
-                   hipEventCreate(&g_ev[1]) != hipSuccess))
-    g_ev[0] = g_ev[1] = nullptr;
-  if (g_ev[0]) (void)hipEventRecord(g_ev[0], s);
+  ev_begin(s);
   const int r = solve_conserved(a, o, dc, s);
-  if (g_ev[0]) {
-    (void)hipEventRecord(g_ev[1], s);
-    g_ev_recorded = true;
-  }
+  ev_end(s);
   return r;
 }
 
@@ -435,8 +442,9 @@ int krome_device_finalize(void) {
   return 0;
 }
 
-// the duration of the last device solve's kernels (HIP events recorded
-// around their launches on their stream), waiting for them to finish
+// the duration of the last device solve's or conservation's kernels (HIP
+// events recorded around their launches on their stream), waiting for
+// them to finish
 int krome_device_last_kernel_ms(double* ms) {
   std::lock_guard<std::mutex> lk(g_cache_mutex);
   if (!ms || !g_ev_recorded) return -1;
@@ -827,12 +835,14 @@ int conserve_resident(int ncell, double* v, long cs, long ss,
               "H2D"))
     return -1;
   const dim3 g((ncell + 255) / 256), b(256);
+  ev_begin(nullptr);
   if (x)
     hipLaunchKernelGGL((kx::conserve_kernel<Net, true>), g, b, 0, nullptr,
                        ncell, v, cs, ss, ref, rs, flags, first);
   else
     hipLaunchKernelGGL((kx::conserve_kernel<Net, false>), g, b, 0, nullptr,
                        ncell, v, cs, ss, ref, rs, flags, first);
+  ev_end(nullptr);
   if (!hip_ok(hipGetLastError(), "conserve kernel")) return -1;
   int lo = ncell;
   if (!hip_ok(hipMemcpy(&lo, first, sizeof(int), hipMemcpyDeviceToHost),

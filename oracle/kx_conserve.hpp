@@ -10,11 +10,11 @@
 //                         krome-x_amd/csrc/generated have neither)
 //   post-solve mode 2     solve_cell.hpp:1469-1489
 //   to_n, to_x            solve_cell.hpp:1491-1562
-//   cells                 runtime/core/solve_cells.hpp:216-242 (the lowest
-//                         failing cell's flag)
+//   cells                 runtime/core/solve_cells.hpp:216-242 (OpenMP
+//                         over cells; the lowest failing cell's flag)
 // Plain loops over the tables of the generated network (element counts
 // natoms(i, e), charge(i), mass_g(i)), compiled without contraction like
-// the reference's bitwise test build.  Pinned by tests/test_oracle.py
+// the reference's bitwise test build.  Pinned by tests/test_conserve.py
 // against the reference's own outputs (tests/golden/*_conserve.npz) and
 // against the projection's defining properties.
 #pragma once
@@ -204,14 +204,18 @@ template <class Net>
 int cells(int ncell, double* v, const double* ref, int ref_rows, bool x,
           int* flags) {
   constexpr int NS = Net::NSPEC;
-  bool fixed[NS] = {};
-  int first = 0;
+  std::vector<int> fl(ncell > 0 ? ncell : 0);
+#pragma omp parallel for schedule(static)
   for (int c = 0; c < ncell; ++c) {
+    bool fixed[NS] = {};
     double* vc = v + static_cast<size_t>(c) * NS;
     const double* rc = ref + static_cast<size_t>(c) * ref_rows * Net::NELEM;
-    const int f = x ? to_x<Net>(vc, rc) : to_n<Net>(vc, rc, fixed);
-    if (flags) flags[c] = f;
-    if (first == 0 && f < 0) first = f;
+    fl[c] = x ? to_x<Net>(vc, rc) : to_n<Net>(vc, rc, fixed);
+  }
+  int first = 0;
+  for (int c = 0; c < ncell; ++c) {
+    if (flags) flags[c] = fl[c];
+    if (first == 0 && fl[c] < 0) first = fl[c];
   }
   return first;
 }
