@@ -87,6 +87,15 @@ struct net_primordial {
       default: return 0;
     }
   }
+  // static-pivot sparse LU of the species block of I - gamma J:
+  // 82 nonzeros, 86 in L + U with the fill, 359 flops per
+  // factorization (dense: 1152); rows in pivot order
+  static constexpr int SP_NNZ = 86;
+  static constexpr int SP_FLOPS = 359;
+  static constexpr int sp_perm[12] = {8, 9, 10, 6, 1, 0, 2, 3, 4, 5, 7, 11};
+  static constexpr int sp_rp[13] = {0, 3, 7, 10, 16, 23, 31, 40, 49, 57, 66, 74, 86};
+  static constexpr int sp_ci[86] = {0, 1, 11, 0, 1, 2, 11, 1, 2, 11, 3, 6, 7, 8, 9, 11, 4, 5, 6, 7, 9, 10, 11, 4, 5, 6, 7, 8, 9, 10, 11, 3, 4, 5, 6, 7, 8, 9, 10, 11, 3, 4, 5, 6, 7, 8, 9, 10, 11, 3, 5, 6, 7, 8, 9, 10, 11, 3, 4, 5, 6, 7, 8, 9, 10, 11, 4, 5, 6, 7, 8, 9, 10, 11, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11};
+  static constexpr int sp_di[12] = {0, 4, 8, 10, 16, 24, 34, 44, 53, 63, 72, 85};
   // conserved combinations c_k . n (the nuclei of D, H, He, the charge): every
   // reaction keeps them, so c_k^T J = 0 over the species
   static constexpr int NCONS = 4;

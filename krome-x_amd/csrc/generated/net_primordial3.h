@@ -79,6 +79,15 @@ struct net_primordial3 {
       default: return 0;
     }
   }
+  // static-pivot sparse LU of the species block of I - gamma J:
+  // 55 nonzeros, 55 in L + U with the fill, 174 flops per
+  // factorization (dense: 486); rows in pivot order
+  static constexpr int SP_NNZ = 55;
+  static constexpr int SP_FLOPS = 174;
+  static constexpr int sp_perm[9] = {5, 6, 7, 0, 1, 2, 3, 4, 8};
+  static constexpr int sp_rp[10] = {0, 3, 7, 10, 19, 25, 31, 40, 46, 55};
+  static constexpr int sp_ci[55] = {0, 1, 8, 0, 1, 2, 8, 1, 2, 8, 0, 1, 2, 3, 4, 5, 6, 7, 8, 3, 4, 5, 6, 7, 8, 3, 4, 5, 6, 7, 8, 0, 1, 2, 3, 4, 5, 6, 7, 8, 3, 4, 5, 6, 7, 8, 0, 1, 2, 3, 4, 5, 6, 7, 8};
+  static constexpr int sp_di[9] = {0, 4, 8, 13, 20, 27, 37, 44, 54};
   // conserved combinations c_k . n (the nuclei of H, He, the charge): every
   // reaction keeps them, so c_k^T J = 0 over the species
   static constexpr int NCONS = 3;

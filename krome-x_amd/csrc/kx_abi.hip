@@ -146,8 +146,10 @@ constexpr int neq() { return kx::Problem<Net, ET>::N; }
 
 // bytes of the plain kernel's workspace per cell
 size_t plain_cell_bytes(bool et) {
-  const int N = et ? neq<true>() : neq<false>();
-  return sizeof(double) * ((kx::LMAX + 9) * N + 2 * N * N) + sizeof(int) * N;
+  return et ? sizeof(double) * kx::cell_doubles<kx::Problem<Net, true>>() +
+                  sizeof(int) * neq<true>()
+            : sizeof(double) * kx::cell_doubles<kx::Problem<Net, false>>() +
+                  sizeof(int) * neq<false>();
 }
 
 // the plain kernel over a chunk of resident arrays
@@ -155,7 +157,9 @@ template <bool ET>
 int launch_plain(kx::CellsArgs a, DevCache& dc, hipStream_t s) {
   constexpr int N = neq<ET>();
   const long S = a.ncell;
-  if (!dc.work.ensure(sizeof(double) * kx::cell_doubles<N>() * S)) return -1;
+  if (!dc.work.ensure(sizeof(double) *
+                     kx::cell_doubles<kx::Problem<Net, ET>>() * S))
+    return -1;
   if (!dc.iwork.ensure(sizeof(int) * N * S)) return -1;
   const int tpb = 128;
   const int blocks = (a.ncell + tpb - 1) / tpb;
@@ -474,12 +478,15 @@ kx::SolveOpts solve_opts(const krome_opts* h) {
   o.jac = h->jac;
   o.clip_negative = h->clip_negative ? 1 : 0;
   o.conserve = h->conserve;
+  o.linsol = h->linsol;
   return o;
 }
 
 // what the whole call rejects before integrating (every cell flagged)
 int reject(const krome_opts* h) {
-  if (h->linsol == 2 || h->linsol == 3) return kx::FLAG_UNSUPPORTED;
+  if (h->linsol == 2) return kx::FLAG_UNSUPPORTED;  // no LAPACK here
+  // the persistent kernels factorize the dense Newton matrix only
+  if (h->linsol == 3 && h->kernel == 1) return kx::FLAG_UNSUPPORTED;
   if (h->path == 1) return kx::FLAG_UNSUPPORTED;  // no host path here
   if (h->evolve_T && !Net::THERMAL) return kx::FLAG_UNSUPPORTED;
   return 0;

@@ -71,6 +71,7 @@ struct CellOpts {
   long max_steps;
   int max_err_fails, max_conv_fails, max_restarts;
   bool dq_jac;
+  bool sparse;  // the static-pivot sparse LU of the species block
   int inv_row, inv_n;
   int inv_c[N];
 };
@@ -89,9 +90,14 @@ struct CellStats {
 };
 
 // doubles of a cell's workspace: Nordsieck array (6 N), 9 vectors, the
-// saved Jacobian and the Newton matrix (2 N^2); pivots are N ints
-template <int N>
-constexpr long cell_doubles() { return (LMAX + 9) * N + 2 * N * N; }
+// saved Jacobian (N^2) and the Newton matrix's storage (Prob::
+// NEWTON_DOUBLES: N^2, or the sparse factors when larger); pivots are N
+// ints
+template <class Prob>
+constexpr long cell_doubles() {
+  return (LMAX + 9) * Prob::N + (long)Prob::N * Prob::N +
+         Prob::NEWTON_DOUBLES;
+}
 
 // the integrator of one cell on strided storage; Prob provides
 //   int rhs(const double* y, double* f)   (registers, N values)
@@ -116,7 +122,7 @@ class Cell {
     vt1 = w; w += N * S;
     vt2 = w; w += N * S;
     vt3 = w; w += N * S;
-    A = w; w += N * N * S;
+    A = w; w += Prob::NEWTON_DOUBLES * S;
     J = w;
     piv = iwork;
     mxstep = o.max_steps == 0 ? 500 : o.max_steps;
@@ -619,7 +625,7 @@ class Cell {
       if (jr < 0) r = -1;
       else if (jr > 0) r = 1;
     }
-    if (r == 0) {
+    if (r == 0 && !o.sparse) {
       const double c = -gamma;
       for (int j = 0; j < N; j++) {
         for (int i = 0; i < N; i++) at(A, i + j * N) = at(J, i + j * N) * c;
@@ -631,7 +637,160 @@ class Cell {
       nstlj = st.nst;
     }
     if (r != 0) return r;
+    if (o.sparse) return sp_factor(-gamma);
     return lu() > 0 ? 1 : 0;
+  }
+
+  // ---- the static-pivot sparse LU (bdf_cell.hpp:1155-1264) -------------
+  // The species block in the pivot order and with the fill of the
+  // network's generated tables (Net::sp_perm, sp_rp, sp_ci, sp_di: uniform
+  // across the wave, scalar loads); the NB = N - NS border equations (Tgas)
+  // through their Schur complement.  A holds val[SP_NNZ], the row
+  // accumulator w[NS], Bt = L^-1 B (column b at b NS), Ct = C U^-1 (row r
+  // at r NS) and Z = D - Ct Bt (NB x NB, column-major), all strided by S.
+  using Net = typename Prob::NetT;
+  static constexpr int NS = Net::NSPEC;
+  static constexpr int NB = N - NS;
+  KX_HD double& spv(long k) const { return A[k * S]; }
+  static constexpr long SP_W = Net::SP_NNZ;
+  static constexpr long SP_BT = SP_W + NS;
+  static constexpr long SP_CT = SP_BT + (long)NS * NB;
+  static constexpr long SP_Z = SP_CT + (long)NB * NS;
+
+  // c J + I gathered and factorized: 0, or 1 (recoverable) at a zero or
+  // non-finite pivot
+  KX_HD int sp_factor(double c) {
+    for (int k = 0; k < NS; k++) {
+      const long row = Net::sp_perm[k];
+      for (int p = Net::sp_rp[k]; p < Net::sp_rp[k + 1]; p++)
+        spv(p) = at(J, row + (long)Net::sp_perm[Net::sp_ci[p]] * N) * c;
+      spv(Net::sp_di[k]) += 1.0;
+    }
+    for (int i = 0; i < NS; i++) {  // row i of L and U
+      const int p0 = Net::sp_rp[i], p1 = Net::sp_rp[i + 1];
+      for (int p = p0; p < p1; p++) spv(SP_W + Net::sp_ci[p]) = spv(p);
+      for (int p = p0; p < Net::sp_di[i]; p++) {
+        const int k = Net::sp_ci[p];
+        const double lik = spv(SP_W + k) / spv(Net::sp_di[k]);
+        spv(SP_W + k) = lik;
+        for (int q = Net::sp_di[k] + 1; q < Net::sp_rp[k + 1]; q++)
+          spv(SP_W + Net::sp_ci[q]) -= lik * spv(q);
+      }
+      const double pv = spv(SP_W + i);
+      if (pv == 0.0 || !(::fabs(pv) <= DBL_MAX)) return 1;
+      for (int p = p0; p < p1; p++) spv(p) = spv(SP_W + Net::sp_ci[p]);
+    }
+    if constexpr (NB > 0) {
+      for (int b = 0; b < NB; b++) {  // Bt = L^-1 B
+        const long x = SP_BT + (long)b * NS;
+        for (int k = 0; k < NS; k++)
+          spv(x + k) = at(J, Net::sp_perm[k] + (long)(NS + b) * N) * c;
+        sp_lower(x);
+      }
+      for (int r = 0; r < NB; r++) {  // Ct = C U^-1
+        const long x = SP_CT + (long)r * NS;
+        for (int k = 0; k < NS; k++)
+          spv(x + k) = at(J, NS + r + (long)Net::sp_perm[k] * N) * c;
+        for (int k = 0; k < NS; k++) {
+          const double xk = spv(x + k) / spv(Net::sp_di[k]);
+          spv(x + k) = xk;
+          if (xk != 0.0)
+            for (int q = Net::sp_di[k] + 1; q < Net::sp_rp[k + 1]; q++)
+              spv(x + Net::sp_ci[q]) -= xk * spv(q);
+        }
+      }
+      for (int b = 0; b < NB; b++)  // Z = D - Ct Bt
+        for (int r = 0; r < NB; r++) {
+          double z = at(J, NS + r + (long)(NS + b) * N) * c;
+          if (r == b) z += 1.0;
+          for (int k = 0; k < NS; k++)
+            z -= spv(SP_CT + (long)r * NS + k) * spv(SP_BT + (long)b * NS + k);
+          spv(SP_Z + r + (long)b * NB) = z;
+        }
+      return sp_border_lu() > 0 ? 1 : 0;
+    }
+    return 0;
+  }
+  // x = L^-1 x (unit lower triangle), x at A + x0 (strided)
+  KX_HD void sp_lower(long x0) {
+    for (int i = 0; i < NS; i++) {
+      double s = spv(x0 + i);
+      for (int p = Net::sp_rp[i]; p < Net::sp_di[i]; p++)
+        s -= spv(p) * spv(x0 + Net::sp_ci[p]);
+      spv(x0 + i) = s;
+    }
+  }
+  // the border's Schur complement: dense LU with partial pivoting
+  // (bdf_cell.hpp:402-433), pivots in piv
+  KX_HD int sp_border_lu() {
+    for (int k = 0; k < NB; k++) {
+      int l_ = k;
+      for (int i = k + 1; i < NB; i++)
+        if (::fabs(spv(SP_Z + i + k * NB)) > ::fabs(spv(SP_Z + l_ + k * NB)))
+          l_ = i;
+      piv[k * S] = l_;
+      if (spv(SP_Z + l_ + k * NB) == 0.0) return k + 1;
+      if (l_ != k)
+        for (int j = 0; j < NB; j++) {
+          const double t = spv(SP_Z + l_ + j * NB);
+          spv(SP_Z + l_ + j * NB) = spv(SP_Z + k + j * NB);
+          spv(SP_Z + k + j * NB) = t;
+        }
+      const double rr = 1.0 / spv(SP_Z + k + k * NB);
+      for (int i = k + 1; i < NB; i++) spv(SP_Z + i + k * NB) *= rr;
+      for (int j = k + 1; j < NB; j++) {
+        const double akj = spv(SP_Z + k + j * NB);
+        if (akj != 0.0)
+          for (int i = k + 1; i < NB; i++)
+            spv(SP_Z + i + j * NB) -= akj * spv(SP_Z + i + k * NB);
+      }
+    }
+    return 0;
+  }
+  // b = (c J + I)^-1 b with the factors of sp_factor (b in registers)
+  KX_HD void sp_solve(double* b) {
+    for (int k = 0; k < NS; k++) spv(SP_W + k) = b[Net::sp_perm[k]];
+    sp_lower(SP_W);
+    if constexpr (NB > 0) {
+      double bb[NB];
+      for (int r = 0; r < NB; r++) {
+        double s = b[NS + r];
+        for (int k = 0; k < NS; k++)
+          s -= spv(SP_CT + (long)r * NS + k) * spv(SP_W + k);
+        bb[r] = s;
+      }
+      for (int k = 0; k < NB; k++) {  // getrs on Z (bdf_cell.hpp:435-457)
+        const int pk = piv[k * S];
+        if (pk != k) {
+          double t = bb[k], u = 0.0;
+          for (int i = 0; i < NB; i++) u = i == pk ? bb[i] : u;
+          for (int i = 0; i < NB; i++) bb[i] = i == pk ? t : bb[i];
+          bb[k] = u;
+        }
+      }
+      for (int k = 0; k < NB - 1; k++)
+        for (int i = k + 1; i < NB; i++)
+          bb[i] -= spv(SP_Z + i + k * NB) * bb[k];
+      for (int k = NB - 1; k > 0; k--) {
+        bb[k] /= spv(SP_Z + k + k * NB);
+        for (int i = 0; i < k; i++) bb[i] -= spv(SP_Z + i + k * NB) * bb[k];
+      }
+      bb[0] /= spv(SP_Z);
+      for (int q = 0; q < NB; q++) {
+        b[NS + q] = bb[q];
+        const double v = bb[q];
+        if (v != 0.0)
+          for (int k = 0; k < NS; k++)
+            spv(SP_W + k) -= spv(SP_BT + (long)q * NS + k) * v;
+      }
+    }
+    for (int i = NS - 1; i >= 0; i--) {
+      double s = spv(SP_W + i);
+      for (int p = Net::sp_di[i] + 1; p < Net::sp_rp[i + 1]; p++)
+        s -= spv(p) * spv(SP_W + Net::sp_ci[p]);
+      spv(SP_W + i) = s / spv(Net::sp_di[i]);
+    }
+    for (int k = 0; k < NS; k++) b[Net::sp_perm[k]] = spv(SP_W + k);
   }
   KX_HD int an_jac() {
     double yr[N], Jr[N * N];
@@ -693,6 +852,15 @@ class Cell {
   KX_HD void lsolve(double* bv) {
     double b[N];
     for (int i = 0; i < N; i++) b[i] = at(bv, i);
+    if (o.sparse) {
+      sp_solve(b);
+      if (gamrat != 1.0) {
+        const double c = 2.0 / (1.0 + gamrat);
+        for (int i = 0; i < N; i++) b[i] = c * b[i];
+      }
+      for (int i = 0; i < N; i++) at(bv, i) = b[i];
+      return;
+    }
     for (int k = 0; k < N; k++) {
       const int pk = piv[k * S];
       if (pk != k) {

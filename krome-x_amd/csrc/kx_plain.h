@@ -41,6 +41,12 @@ KX_HD inline bool cell_opts(const SolveOpts& o,
   co.max_conv_fails = o.max_conv_fails;
   co.max_restarts = o.max_restarts;
   co.dq_jac = o.jac == 2 || (o.jac == 0 && ET);
+  // the linear solver (solve_cell.hpp:1601-1620): 3 sparse; 0 (auto)
+  // sparse where its factorization is at most a tenth of the dense one and
+  // the border at most 4 equations; else dense
+  co.sparse = o.linsol == 3 ||
+              (o.linsol == 0 && N - NS <= 4 &&
+               2.0 * NS * NS * NS / 3.0 >= 10.0 * Net::SP_FLOPS);
   co.inv_row = -1;
   co.inv_n = 0;
   for (int i = 0; i < N; ++i) co.inv_c[i] = i < NS ? Net::charge(i) : 0;

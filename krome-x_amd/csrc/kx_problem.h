@@ -17,13 +17,23 @@ struct SolveOpts {
   int evolve_T;      // Tgas evolved (a thermal network)
   int jac;           // 0 auto (analytic at fixed T, else DQ), 1 analytic, 2 DQ
   int clip_negative;
-  int conserve;      // 0 off, 1 electrons restore the charge, 3 neutralize
+  int conserve;      // 0 off, 1 electrons restore the charge, 2 project
+                     // onto the input totals, 3 neutralize
+  int linsol;        // 0 auto, 1 dense, 3 the static-pivot sparse LU
 };
 
 template <class Net, bool ET>
 struct Problem {
+  using NetT = Net;
   static constexpr int NS = Net::NSPEC;
   static constexpr int N = NS + (ET ? 1 : 0);
+  // doubles of the Newton matrix's storage: N x N dense, or the sparse
+  // factors, their row accumulator, the border's two blocks and its Schur
+  // complement (m = N - NS border equations)
+  static constexpr long SPARSE_DOUBLES =
+      Net::SP_NNZ + NS + 2L * NS * (N - NS) + (long)(N - NS) * (N - NS);
+  static constexpr long NEWTON_DOUBLES =
+      SPARSE_DOUBLES > (long)N * N ? SPARSE_DOUBLES : (long)N * N;
   typename Net::Params p;
   double Tfix;
   bool hit_T0;

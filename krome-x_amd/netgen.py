@@ -290,6 +290,80 @@ def conserved_laws(net):
     return kept
 
 
+def jac_pattern(net, em):
+    """the species block of I - gamma J as a set of (row, column): the
+    diagonal; each reaction's flux rows (its reactants and products) at
+    the columns of its kinetic reactants (mass action); for a rate that
+    reads densities, those columns too, and its flux rows dense when it
+    reads an aggregate (the reference's pattern, codegen.py:8264-8299)"""
+    ns = net.nspec
+    pat = {(i, i) for i in range(ns)}
+    for r in net.reactions:
+        rows = {net.idx(x) for x, _ in r.reactants} | {net.idx(x)
+                                                       for x in r.products}
+        cols = {net.idx(x) for x, kin in r.reactants if kin}
+        direct, agg = em.density_deps(Expr(r.rate, net))
+        cols |= direct
+        if agg:
+            cols = set(range(ns))
+        pat |= {(i, j) for i in rows for j in cols}
+    return pat
+
+
+def min_degree_order(ns, pat):
+    """minimum-degree elimination order of the symmetrized pattern: the
+    vertex of least degree first (ties: the lowest index); eliminating it
+    joins its neighbours into a clique"""
+    adj = [set() for _ in range(ns)]
+    for i, j in pat:
+        if i != j:
+            adj[i].add(j)
+            adj[j].add(i)
+    left = set(range(ns))
+    order = []
+    while left:
+        v = min(left, key=lambda u: (len(adj[u]), u))
+        left.remove(v)
+        order.append(v)
+        nb = adj[v]
+        for u in nb:
+            adj[u] = (adj[u] | nb) - {u, v}
+        adj[v] = set()
+    return order
+
+
+def sparse_lu_tables(net, em):
+    """the static-pivot sparse LU of the species block (the reference's
+    LINSOL_SPARSE, bdf_cell.hpp:1155-1264, tables as codegen.py:8302-8341):
+    perm[k] the species of pivot k; row k of L + U (fill included) at
+    ci[rp[k] .. rp[k+1]) in pivot order, columns ascending; di[k] the
+    position of its diagonal; flops of one factorization"""
+    ns = net.nspec
+    pat = jac_pattern(net, em)
+    perm = min_degree_order(ns, pat)
+    pos = {sp: k for k, sp in enumerate(perm)}
+    rows = [set() for _ in range(ns)]
+    for i, j in pat:
+        rows[pos[i]].add(pos[j])
+    for k in range(ns):  # symbolic elimination, no pivoting
+        upper = {c for c in rows[k] if c > k}
+        for i in range(k + 1, ns):
+            if k in rows[i]:
+                rows[i] |= upper
+    rp, ci, di = [0], [], []
+    for k in range(ns):
+        cols = sorted(rows[k])
+        di.append(len(ci) + cols.index(k))
+        ci += cols
+        rp.append(len(ci))
+    flops = 0
+    for k in range(ns):
+        below = sum(1 for i in range(k + 1, ns) if k in rows[i])
+        flops += below * (1 + 2 * (rp[k + 1] - di[k] - 1))
+    return {"perm": perm, "rp": rp, "ci": ci, "di": di, "nnz_a": len(pat),
+            "flops": flops}
+
+
 def _group(net):
     """tile the ranged lines of identical stoichiometry into piecewise
     groups (network-format.md 3.1); an unranged line is its own group"""
@@ -394,6 +468,21 @@ class Emitter:
             if any(EXTERNS.get(c) for c in e.calls):
                 return True
         return False
+
+    def density_deps(self, expr):
+        """(species read directly, whether an aggregate is read): the
+        densities an expression (with its vars) depends on; an aggregate
+        (ntot, Hnuclei, or an extern rate function of Hnuclei) couples it
+        to every species"""
+        es = [expr] + [self.exprs[v] for v in self.deps(expr)]
+        direct, agg = set(), False
+        for e in es:
+            if e.names & {"ntot", "Hnuclei"}:
+                agg = True
+            if any(EXTERNS.get(c) for c in e.calls):
+                agg = True
+            direct |= {int(nm[3:]) for nm in e.names if nm.startswith("__n")}
+        return direct, agg
 
     def c(self, node):
         """C++ of an AST node (type R)"""
@@ -651,6 +740,17 @@ def generate(knet_path, thermal=None, name=None):
     w("      default: return 0;")
     w("    }")
     w("  }")
+    # the static-pivot sparse LU of the species block (linsol 3)
+    sp = sparse_lu_tables(net, em)
+    w("  // static-pivot sparse LU of the species block of I - gamma J:")
+    w("  // %d nonzeros, %d in L + U with the fill, %d flops per"
+      % (sp["nnz_a"], len(sp["ci"]), sp["flops"]))
+    w("  // factorization (dense: %d); rows in pivot order" % (2 * NS ** 3 // 3))
+    w("  static constexpr int SP_NNZ = %d;" % len(sp["ci"]))
+    w("  static constexpr int SP_FLOPS = %d;" % sp["flops"])
+    for nm in ("perm", "rp", "ci", "di"):
+        w("  static constexpr int sp_%s[%d] = {%s};"
+          % (nm, len(sp[nm]), ", ".join(str(v) for v in sp[nm])))
     # conserved combinations: the nuclei of each element and the charge,
     # kept where every reaction keeps them
     laws = conserved_laws(net)
@@ -801,6 +901,8 @@ def generate(knet_path, thermal=None, name=None):
     meta = {"name": ns, "species": [s.name for s in net.species],
             "charge": charges, "mass_g": masses, "params": pnames,
             "elements": elems,
+            "sparse": {k: sp[k] for k in ("perm", "rp", "ci", "di",
+                                          "flops")},
             "natoms": [[sp.comp.get(e, 0) for e in elems]
                        for sp in net.species],
             "param_defaults": [net.params.get(p, PARAM_DEFAULTS.get(p))

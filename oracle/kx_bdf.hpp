@@ -30,7 +30,15 @@ enum : int {
   NLS_FAIL = -16, ILL_INPUT = -22, TOO_CLOSE = -27
 };
 
-// options of one integration (bdf_cell.hpp:133-170; dense LU only)
+// the species block's static-pivot sparse LU tables in pivot order
+// (bdf_cell.hpp:125-131): perm[k] the equation of pivot k; row k of L + U
+// at ci[rp[k] .. rp[k+1]), columns ascending; di[k] its diagonal
+struct Sparse {
+  int ns = 0, nnz = 0;
+  const int *perm = nullptr, *rp = nullptr, *ci = nullptr, *di = nullptr;
+};
+
+// options of one integration (bdf_cell.hpp:133-170)
 struct Opts {
   double rtol = 1e-4;
   const double* atol = nullptr;
@@ -42,6 +50,9 @@ struct Opts {
   // the linear invariant kept by the Newton corrections (charge)
   int inv_row = -1, inv_n = 0;
   const int* inv_c = nullptr;
+  // the static-pivot sparse LU instead of the dense one
+  bool sparse = false;
+  Sparse sp;
 };
 
 // CVODE's counters (bdf_cell.hpp:177-191)
@@ -56,8 +67,19 @@ struct Stats {
 };
 
 // workspace of an N-equation cell: Nordsieck array, 9 vectors, saved J,
-// Newton matrix (bdf_cell.hpp:210-220)
-inline long work_doubles(int N) { return (LMAX + 9) * (long)N + 2L * N * N; }
+// Newton matrix: N x N, or the sparse factors (nnz), their row
+// accumulator (ns), the border's L^-1 B and C U^-1 (ns x m each) and its
+// Schur complement (m x m), m = N - ns (bdf_cell.hpp:206-220)
+inline long newton_doubles(int N, bool sparse = false, int ns = 0,
+                           int nnz = 0) {
+  const long m = N - ns;
+  return sparse ? nnz + ns + 2L * ns * m + m * m : (long)N * N;
+}
+inline long work_doubles(int N, bool sparse = false, int ns = 0,
+                         int nnz = 0) {
+  return (LMAX + 9) * (long)N + (long)N * N +
+         newton_doubles(N, sparse, ns, nnz);
+}
 
 namespace c_ {
 // cvode.c constants (bdf_cell.hpp:241-272)
@@ -227,7 +249,7 @@ class Bdf {
       w += N;
     }
     A = w;
-    w += (long)N * N;
+    w += newton_doubles(N, o.sparse, o.sp.ns, o.sp.nnz);
     J = w;
     piv = iwork;
     mxstep = o.max_steps == 0 ? 500 : o.max_steps;
@@ -709,7 +731,7 @@ class Bdf {
       if (jr < 0) r = -1;
       else if (jr > 0) r = 1;
     }
-    if (r == 0) {
+    if (r == 0 && !o.sparse) {
       const double c = -gamma;
       for (int j = 0; j < N; j++) {
         for (int i = 0; i < N; i++) {
@@ -724,14 +746,122 @@ class Bdf {
       nstlj = st.nst;
     }
     if (r != 0) return r;
+    if (o.sparse) return sp_factor(-gamma);
     return c_::lu(A, N, piv) > 0 ? 1 : 0;
   }
 
   // cvLsSolve (bdf_cell.hpp:1134-1153)
   int lsolve(double* b) {
-    c_::lus(A, N, piv, b);
+    if (o.sparse) sp_solve(b);
+    else c_::lus(A, N, piv, b);
     if (gamrat != 1.0) c_::scal(N, 2.0 / (1.0 + gamrat), b, b);
     return OK;
+  }
+
+  // ---- the static-pivot sparse LU (bdf_cell.hpp:1155-1264) -------------
+  // A holds the factors val[nnz], a row accumulator w[ns], Bt = L^-1 B
+  // (column b at b ns), Ct = C U^-1 (row r at r ns) and the border's Schur
+  // complement Z = D - Ct Bt (m x m, column-major, pivots in piv)
+  double* sp_w() const { return A + o.sp.nnz; }
+  double* sp_Bt() const { return sp_w() + o.sp.ns; }
+  double* sp_Ct() const { return sp_Bt() + (long)o.sp.ns * (N - o.sp.ns); }
+  double* sp_Z() const {
+    return sp_Ct() + (long)(N - o.sp.ns) * o.sp.ns;
+  }
+
+  // c J + I gathered in pivot order and factorized: 0, or 1 (recoverable)
+  // at a zero or non-finite pivot
+  int sp_factor(double c) {
+    const Sparse& t = o.sp;
+    const int ns = t.ns, m = N - ns;
+    double* val = A;
+    double* w = sp_w();
+    for (int k = 0; k < ns; k++) {
+      const long row = t.perm[k];
+      for (int p = t.rp[k]; p < t.rp[k + 1]; p++)
+        val[p] = J[row + (long)t.perm[t.ci[p]] * N] * c;
+      val[t.di[k]] += 1.0;
+    }
+    for (int i = 0; i < ns; i++) {  // row i of L and U
+      for (int p = t.rp[i]; p < t.rp[i + 1]; p++) w[t.ci[p]] = val[p];
+      for (int p = t.rp[i]; p < t.di[i]; p++) {
+        const int k = t.ci[p];
+        const double lik = w[k] / val[t.di[k]];
+        w[k] = lik;
+        for (int q = t.di[k] + 1; q < t.rp[k + 1]; q++)
+          w[t.ci[q]] -= lik * val[q];
+      }
+      const double pv = w[i];
+      if (pv == 0.0 || !(std::fabs(pv) <= DBL_MAX)) return 1;
+      for (int p = t.rp[i]; p < t.rp[i + 1]; p++) val[p] = w[t.ci[p]];
+    }
+    if (m == 0) return 0;
+    double* Bt = sp_Bt();
+    double* Ct = sp_Ct();
+    double* Z = sp_Z();
+    for (int b = 0; b < m; b++) {
+      double* x = Bt + (long)b * ns;
+      for (int k = 0; k < ns; k++) x[k] = J[t.perm[k] + (long)(ns + b) * N] * c;
+      sp_lower(x);
+    }
+    for (int r = 0; r < m; r++) {
+      double* x = Ct + (long)r * ns;
+      for (int k = 0; k < ns; k++) x[k] = J[ns + r + (long)t.perm[k] * N] * c;
+      for (int k = 0; k < ns; k++) {
+        const double xk = x[k] / val[t.di[k]];
+        x[k] = xk;
+        if (xk != 0.0)
+          for (int q = t.di[k] + 1; q < t.rp[k + 1]; q++)
+            x[t.ci[q]] -= xk * val[q];
+      }
+    }
+    for (int b = 0; b < m; b++)
+      for (int r = 0; r < m; r++) {
+        double z = J[ns + r + (long)(ns + b) * N] * c;
+        if (r == b) z += 1.0;
+        for (int k = 0; k < ns; k++)
+          z -= Ct[(long)r * ns + k] * Bt[(long)b * ns + k];
+        Z[r + (long)b * m] = z;
+      }
+    return c_::lu(Z, m, piv) > 0 ? 1 : 0;
+  }
+  // x = L^-1 x (unit lower triangle, pivot order)
+  void sp_lower(double* x) const {
+    const Sparse& t = o.sp;
+    for (int i = 0; i < t.ns; i++) {
+      double s = x[i];
+      for (int p = t.rp[i]; p < t.di[i]; p++) s -= A[p] * x[t.ci[p]];
+      x[i] = s;
+    }
+  }
+  // b = (c J + I)^-1 b with the factors of sp_factor
+  void sp_solve(double* b) {
+    const Sparse& t = o.sp;
+    const int ns = t.ns, m = N - ns;
+    double* x = sp_w();
+    for (int k = 0; k < ns; k++) x[k] = b[t.perm[k]];
+    sp_lower(x);
+    if (m > 0) {
+      const double* Bt = sp_Bt();
+      const double* Ct = sp_Ct();
+      for (int r = 0; r < m; r++) {
+        double s = b[ns + r];
+        for (int k = 0; k < ns; k++) s -= Ct[(long)r * ns + k] * x[k];
+        b[ns + r] = s;
+      }
+      c_::lus(sp_Z(), m, piv, b + ns);
+      for (int q = 0; q < m; q++) {
+        const double v = b[ns + q];
+        if (v != 0.0)
+          for (int k = 0; k < ns; k++) x[k] -= Bt[(long)q * ns + k] * v;
+      }
+    }
+    for (int i = ns - 1; i >= 0; i--) {
+      double s = x[i];
+      for (int p = t.di[i] + 1; p < t.rp[i + 1]; p++) s -= A[p] * x[t.ci[p]];
+      x[i] = s / A[t.di[i]];
+    }
+    for (int k = 0; k < ns; k++) b[t.perm[k]] = x[k];
   }
 
   // the charge invariant of the corrections (bdf_cell.hpp:1267-1287)

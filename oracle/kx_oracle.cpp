@@ -141,7 +141,25 @@ int solve_one(double* n, double& T, const double* row, double dt,
     bo.inv_n = NS;
     bo.inv_c = charge;
   }
-  std::vector<double> work(bdf_::work_doubles(N));
+  // the linear solver (solve_cell.hpp:1601-1620): 3 the static-pivot
+  // sparse LU, 0 (auto) the same where its factorization costs at most a
+  // tenth of the dense one with at most 4 border equations, else dense
+  const int ls = static_cast<int>(o[O_LINSOL]);
+  const bool sparse =
+      ls == 3 || (ls == 0 && N - NS <= 4 &&
+                  2.0 * NS * NS * NS / 3.0 >= 10.0 * Net::SP_FLOPS);
+  if (ls == 2) return FLAG_UNSUPPORTED;
+  if (sparse) {
+    bo.sparse = true;
+    bo.sp.ns = NS;
+    bo.sp.nnz = Net::SP_NNZ;
+    bo.sp.perm = Net::sp_perm;
+    bo.sp.rp = Net::sp_rp;
+    bo.sp.ci = Net::sp_ci;
+    bo.sp.di = Net::sp_di;
+  }
+  std::vector<double> work(
+      bdf_::work_doubles(N, sparse, NS, Net::SP_NNZ));
   int iwork[NS + 1];
   bdf_::Stats bs;
   int nr = 0;

@@ -19,10 +19,13 @@ For each workload it:
      solve_cell.hpp:1331-1562, solve_cells.hpp:216-260): the element
      tables, krome::get_element_totals(_x) and krome::conserve_cells(_x) of
      seeded perturbed cells, and solve_cells with SolverOpts::conserve 2
-     on the solve fixture's cells -> <name>_conserve.npz.
+     on the solve fixture's cells -> <name>_conserve.npz;
+  6. the static-pivot sparse LU (LINSOL_SPARSE, bdf_cell.hpp:1155-1264):
+     the generated tables, and solve_cells with linsol 3 on the solve
+     fixture's cells -> <name>_sparse.npz.
 
-  python3 tests/golden/make_golden.py [solve] [funcs] [conserve]
-  (default: all three)
+  python3 tests/golden/make_golden.py [solve] [funcs] [conserve] [sparse]
+  (default: all four)
 
 The generated project and the binaries stay in _work/ (not committed).
 """
@@ -265,8 +268,29 @@ def make_conserve(name, exe, meta):
     print("%s conserve-2 solve: flags %s" % (name, np.unique(f2)))
 
 
+def make_sparse(name, exe):
+    w = cells.WORKLOADS[name](NSOLVE[name])
+    raw, _ = run_mode(exe, "sparse", w, name + "_sparse_tab", [0.0])
+    ns = w["n"].shape[1]
+    nnz = int(raw[0])
+    o = 2
+    tab = {}
+    for k, m in (("perm", ns), ("rp", ns + 1), ("ci", nnz), ("di", ns)):
+        tab["sp_" + k] = raw[o:o + m].astype(np.int64)
+        o += m
+    opts = [float(w["opts"][k]) for k in OPT_KEYS]
+    opts[OPT_KEYS.index("linsol")] = 3.0
+    n, T, f, st, secs = run_solve(exe, w, name + "_solve_sparse", opts)
+    np.savez_compressed(HERE / ("%s_sparse.npz" % name), n0=w["n"],
+                        T0=w["T"], dt=w["dt"], par=w["par"],
+                        opts=np.array(opts), n=n, T=T, flag=f, stats=st,
+                        sparse_flops=raw[1], **tab)
+    print("%s sparse: nnz %d, flops %d, solve flags %s, steps %d" % (
+        name, nnz, raw[1], np.unique(f), st[:, 0].sum()))
+
+
 def main(argv=None):
-    parts = set(argv if argv else ("solve", "funcs", "conserve"))
+    parts = set(argv if argv else ("solve", "funcs", "conserve", "sparse"))
     BUILD.mkdir(parents=True, exist_ok=True)
     for name in ("primordial", "primordial3"):
         meta = cells.network_meta(name)
@@ -275,6 +299,8 @@ def main(argv=None):
                 for m in ("bitwise", "production")}
         if "conserve" in parts:
             make_conserve(name, exes["bitwise"], meta)
+        if "sparse" in parts:
+            make_sparse(name, exes["bitwise"])
         if not parts & {"solve", "funcs"}:
             continue
         w = cells.WORKLOADS[name](NSOLVE[name])

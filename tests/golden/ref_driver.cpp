@@ -31,7 +31,11 @@
 //                               species_mass_g[NSPEC], species_charge
 //                               [NSPEC] (f64); OUT.names: the element
 //                               names, one a line
-//   (solve: opts[7], when given, is SolverOpts::conserve)
+//   ref_driver sparse IN OUT    OUT: SPARSE_NNZ, SPARSE_FLOPS, sp_perm
+//                               [NSPEC], sp_rp[NSPEC + 1], sp_ci[NNZ],
+//                               sp_di[NSPEC] (f64)
+//   (solve: opts[4] 1 dense, 3 sparse, else auto; opts[7], when given,
+//   is SolverOpts::conserve)
 #include "krome.hpp"
 
 #include <chrono>
@@ -77,7 +81,9 @@ krome::SolverOpts solver_opts(const In& in) {
   so.atol_n = in.opts[2];
   so.atol_T = in.opts[3];
   const int ls = static_cast<int>(in.opts[4]);
-  so.linsol = ls == 1 ? krome::LINSOL_DENSE : krome::LINSOL_AUTO;
+  so.linsol = ls == 1   ? krome::LINSOL_DENSE
+              : ls == 3 ? krome::LINSOL_SPARSE
+                        : krome::LINSOL_AUTO;
   const int jc = static_cast<int>(in.opts[5]);
   so.jac = jc == 1 ? krome::JAC_ANALYTIC
                    : (jc == 2 ? krome::JAC_FD : krome::JAC_AUTO);
@@ -206,6 +212,20 @@ int do_meta(const char* out) {
   return 0;
 }
 
+int do_sparse(const char* out) {
+  FILE* f = std::fopen(out, "wb");
+  if (!f) return 3;
+  std::vector<double> v = {double(krome::SPARSE_NNZ),
+                           double(krome::SPARSE_FLOPS)};
+  for (int i = 0; i < krome::NSPEC; ++i) v.push_back(sp_perm[i]);
+  for (int i = 0; i <= krome::NSPEC; ++i) v.push_back(sp_rp[i]);
+  for (int i = 0; i < krome::SPARSE_NNZ; ++i) v.push_back(sp_ci[i]);
+  for (int i = 0; i < krome::NSPEC; ++i) v.push_back(sp_di[i]);
+  std::fwrite(v.data(), sizeof(double), v.size(), f);
+  std::fclose(f);
+  return 0;
+}
+
 int do_totals(const In& in, const char* out) {
   if (in.nspec != krome::NSPEC) return 2;
   FILE* f = std::fopen(out, "wb");
@@ -253,7 +273,7 @@ int do_conserve(const In& in, const std::vector<double>& ref,
 
 int main(int argc, char** argv) {
   if (argc != 4) {
-    std::fprintf(stderr, "usage: %s solve|funcs|totals|conserve|meta IN OUT\n",
+    std::fprintf(stderr, "usage: %s solve|funcs|totals|conserve|meta|sparse IN OUT\n",
                  argv[0]);
     return 1;
   }
@@ -266,6 +286,7 @@ int main(int argc, char** argv) {
   if (!std::strcmp(argv[1], "funcs")) return do_funcs(in, argv[3]);
   if (!std::strcmp(argv[1], "totals")) return do_totals(in, argv[3]);
   if (!std::strcmp(argv[1], "meta")) return do_meta(argv[3]);
+  if (!std::strcmp(argv[1], "sparse")) return do_sparse(argv[3]);
   if (!std::strcmp(argv[1], "conserve")) {
     // ref[ncell NELEM] follows the standard input in the same file
     FILE* f = std::fopen(argv[2], "rb");
