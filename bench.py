@@ -6,7 +6,7 @@ gamma EXACT, H2 opacity RIPAMONTI; rtol 1e-8, atol_n 1e-20, atol_T 1e-6).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--ncell N]
                     [--kernel persistent|plain] [--fp64]
-                    [--workload solve|conserve]
+                    [--workload solve|conserve] [--linsol dense|sparse]
 
 A step is one solve of the whole batch from its initial state: the inputs
 are resident in HBM (species-major n[nspec, ncell]); each step first
@@ -52,16 +52,27 @@ PEAK_HBM = 8.0e12         # bytes/s
 BYTES_PER_CELL = 8 * (9 + 1 + 1 + 3) + 8 * (9 + 1) + 4
 
 
-def flops_per_cell(stats, opc, N=10, q_avg=3.0):
+def flops_per_cell(stats, opc, N=10, q_avg=3.0, sparse=None):
     """algorithmic FLOPs of each cell from its counters: rhs evaluations
     (Newton residuals + DQ Jacobian columns) x the generated rhs's
     operation count, the DQ column updates, the Newton matrix setups (A =
     I - gamma J and the LU), the triangular solves with the Newton vector
     updates, and the per-step Nordsieck / error-test vector work (at the
-    mean order q_avg)"""
+    mean order q_avg).  sparse (the network's sparse-LU tables): the
+    factorization's own flop count plus the border's blocks, Schur
+    complement and its dense LU, and the sparse triangular solves"""
     f_rhs = opc["full_rhs"]
     f_getrf = N * (N - 1) / 2 + (N - 1) * N * (2 * N - 1) / 3 + N
     f_getrs = 2 * N * N - N
+    if sparse is not None:
+        ns = len(sparse["perm"])
+        m = N - ns
+        nnz = len(sparse["ci"])
+        fm = m * (m - 1) / 2 + (m - 1) * m * (2 * m - 1) / 3 + m
+        f_getrf = (sparse["flops"] + 2 * m * nnz + 2 * m * m * ns + fm
+                   - (N * N + N))  # the dense A = I - gamma J is not formed
+        f_getrf += nnz + ns
+        f_getrs = 2 * nnz - ns + 4 * m * ns + 2 * m * m - m
     step = (q_avg * (q_avg + 1) / 2 + 2 * (q_avg + 1) + 12) * N
     st = stats.astype(np.float64)
     nst, nfe, nje, nfeDQ, nset, nni = (st[:, 0], st[:, 1], st[:, 2],
@@ -212,6 +223,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--workload", default="solve",
                     choices=("solve", "conserve"))
+    ap.add_argument("--linsol", default="dense", choices=("dense", "sparse"),
+                    help="the Newton matrix's LU (sparse: the plain kernel's "
+                         "static-pivot sparse LU, linsol 3)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: the cells are independent and "
@@ -242,7 +256,8 @@ def main():
                 error_report=False)
     mixed = a.kernel == "persistent" and not a.fp64
     head = net.options(**base, device_kernel=a.kernel,
-                       mixed_precision=1 if mixed else 0)
+                       mixed_precision=1 if mixed else 0,
+                       linsol=3 if a.linsol == "sparse" else 1)
     plain = net.options(**base, device_kernel="plain")
 
     def run(opts, st=None):
@@ -313,7 +328,9 @@ def main():
 
     # roofline of the solve kernel: algorithmic FLOPs per launch from the
     # cells' counters over the kernel's event-timed duration
-    fr, fdq, flu, fvec = flops_per_cell(st_head, opc)
+    fr, fdq, flu, fvec = flops_per_cell(
+        st_head, opc, sparse=(cells.network_meta("primordial3")["sparse"]
+                              if a.linsol == "sparse" else None))
     tot = float((fr + fdq + flu + fvec).sum())
     fp32 = float((fdq + flu).sum()) if mixed else 0.0
     achieved = tot / (kern_ms * 1e-3)
@@ -346,6 +363,7 @@ def main():
         "config": {"workload": "primordial3 collapse cells, Tgas evolved, "
                                "seed 1, dt = 0.01 tff, rtol 1e-8",
                    "ncell": a.ncell, "kernel": a.kernel,
+                   "linsol": a.linsol,
                    "mixed_precision": mixed, "parallelism": "1 GPU"},
         "roofline": roof, "cpu_baseline": cpu,
         "flags_nonzero": int((fl_head != 0).sum()),
