@@ -6,7 +6,8 @@ gamma EXACT, H2 opacity RIPAMONTI; rtol 1e-8, atol_n 1e-20, atol_T 1e-6).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--ncell N]
                     [--kernel persistent|plain] [--fp64]
-                    [--workload solve|conserve] [--linsol dense|sparse]
+                    [--workload solve|conserve|copy]
+                    [--linsol dense|sparse]
 
 A step is one solve of the whole batch from its initial state: the inputs
 are resident in HBM (species-major n[nspec, ncell]); each step first
@@ -26,6 +27,11 @@ moved back to their own element totals; species-major rows, one reference
 row per cell.  A step restores the rows from a resident copy and runs the
 call; the roofline is the kernel's algorithmic bytes over its event-timed
 duration.
+
+--workload copy is the calibration of the HBM traffic counters: one
+device-to-device copy of 2 GiB (far above the last-level cache), whose
+FETCH_SIZE and WRITE_SIZE under rocprofv3 --pmc give the factors the
+kernels' counters are scaled by (profiles/traffic.json).
 """
 from __future__ import annotations
 
@@ -208,6 +214,21 @@ def conserve_bench(a):
     print(json.dumps(out))
 
 
+def copy_calibration():
+    """one 2 GiB device-to-device copy (the traffic counters' reference)"""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    print(json.dumps({"workload": "copy", "bytes_each_way": n * 8,
+                      "seconds": el, "GB_per_s": 2 * n * 8 / el / 1e9}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -222,7 +243,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--workload", default="solve",
-                    choices=("solve", "conserve"))
+                    choices=("solve", "conserve", "copy"))
     ap.add_argument("--linsol", default="dense", choices=("dense", "sparse"),
                     help="the Newton matrix's LU (sparse: the plain kernel's "
                          "static-pivot sparse LU, linsol 3)")
@@ -234,6 +255,8 @@ def main():
         sys.exit(2)
     if a.workload == "conserve":
         return conserve_bench(a)
+    if a.workload == "copy":
+        return copy_calibration()
     import torch
     import cells
     import krome

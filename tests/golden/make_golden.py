@@ -5,7 +5,7 @@ Run in the container that has the reference (read-only) under
 
 For each workload it:
   1. runs the reference's code generator (python3 -m kromex, with the
-     workload's thermal options) into _work/golden_build/<name>;
+     workload's thermal options) into tests/golden/_build/<name>;
   2. compiles ref_driver.cpp against that project with g++, twice: the
      bitwise flags of the reference's test builds (-O2 -ffp-contract=off,
      codegen.py:9122-9123) and the production flags (-O3
@@ -27,7 +27,8 @@ For each workload it:
   python3 tests/golden/make_golden.py [solve] [funcs] [conserve] [sparse]
   (default: all four)
 
-The generated project and the binaries stay in _work/ (not committed).
+The generated project and the binaries stay in tests/golden/_build/ (git-
+ignored, not committed).
 """
 from __future__ import annotations
 
@@ -44,7 +45,7 @@ sys.path.insert(0, str(REPO / "krome-x_amd"))
 import cells  # noqa: E402
 
 REF = Path("/root/reference")
-BUILD = REPO / "_work" / "golden_build"
+BUILD = HERE / "_build"
 
 TOML = {
     "primordial": '[project]\nnetwork = "{net}/primordial.knet"\n'
