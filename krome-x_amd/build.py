@@ -17,6 +17,11 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 GEN = CSRC / "generated"
 NETWORKS = ("primordial", "primordial3")
+# test builds of a network with other compile-time forms: name -> (network,
+# extra flags); primordial_rhsloop evaluates the rhs and the species
+# Jacobian by the loop form over index tables (the form of networks above
+# netgen.RHS_LOOP_MIN reactions) instead of the unrolled code
+VARIANTS = {"primordial_rhsloop": ("primordial", ["-DKX_FORCE_RHS_LOOP"])}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          # no packed FP32 instructions (v_pk_*_f32) in any kernel
@@ -39,14 +44,19 @@ def regen(networks_dir=None):
     return True
 
 
-def build(net, verbose=False):
-    out = lib_path(net)
+def _command(net, variant, tmp):
+    extra = VARIANTS[variant][1] if variant else []
+    return [HIPCC, *FLAGS, *extra, "-I", str(CSRC), "-I", str(GEN),
+            "-DKX_NET=net_%s" % net,
+            '-DKX_NET_HEADER="net_%s.h"' % net,
+            '-DKX_META_HEADER="net_%s_meta.h"' % net,
+            str(CSRC / "kx_abi.hip"), "-o", str(tmp)]
+
+
+def build(net, verbose=False, variant=None):
+    out = lib_path(variant or net)
     tmp = out.with_suffix(".so.tmp")
-    cmd = [HIPCC, *FLAGS, "-I", str(CSRC), "-I", str(GEN),
-           "-DKX_NET=net_%s" % net,
-           '-DKX_NET_HEADER="net_%s.h"' % net,
-           '-DKX_META_HEADER="net_%s_meta.h"' % net,
-           str(CSRC / "kx_abi.hip"), "-o", str(tmp)]
+    cmd = _command(net, variant, tmp)
     if verbose:
         print(" ".join(cmd))
     subprocess.run(cmd, check=True)
@@ -55,10 +65,25 @@ def build(net, verbose=False):
 
 
 def build_all(verbose=False, regen_headers=False):
+    """every network library and test variant, compiled concurrently"""
     if regen_headers:
         regen()
-    return [build(n, verbose) for n in NETWORKS]
-
+    jobs = [(n, None) for n in NETWORKS] + [(net, v) for v, (net, _)
+                                            in VARIANTS.items()]
+    procs = []
+    for net, variant in jobs:
+        out = lib_path(variant or net)
+        tmp = out.with_suffix(".so.tmp")
+        cmd = _command(net, variant, tmp)
+        if verbose:
+            print(" ".join(cmd))
+        procs.append((subprocess.Popen(cmd), tmp, out))
+    failed = [p.args for p, _, _ in procs if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, failed[0])
+    for _, tmp, out in procs:
+        os.replace(tmp, out)
+    return [out for _, _, out in procs]
 
 if __name__ == "__main__":
     build_all(verbose=True, regen_headers="--regen" in sys.argv)

@@ -322,9 +322,52 @@ struct net_primordial {
     rates_n(Trate, n, p, k);
   }
   static constexpr int NREA_T = 38;
+  // the loop form's index tables (netgen.loop_tables)
+  static constexpr bool RHS_LOOP = false;
+  static constexpr int LT_KF_OFF[39] = {0, 2, 4, 6, 8, 10, 12, 14, 16, 18, 20, 22, 24, 26, 28, 30, 32, 34, 36, 38, 40, 42, 44, 46, 48, 50, 53, 56, 59, 62, 64, 66, 68, 70, 72, 74, 76, 78, 80};
+  static constexpr int LT_KF_SP[80] = {2, 11, 3, 11, 3, 11, 8, 11, 9, 11, 9, 11, 9, 11, 10, 11, 2, 11, 4, 2, 4, 2, 2, 3, 2, 3, 6, 2, 5, 3, 5, 11, 5, 2, 4, 11, 4, 2, 4, 2, 4, 3, 4, 3, 6, 11, 6, 11, 6, 4, 2, 2, 2, 2, 2, 2, 5, 2, 2, 5, 2, 2, 3, 0, 2, 1, 5, 1, 7, 3, 5, 0, 5, 0, 7, 2, 0, 4, 1, 11};
+  static constexpr int LT_SC_OFF[39] = {0, 5, 8, 11, 16, 19, 22, 27, 30, 33, 37, 41, 44, 47, 51, 55, 60, 65, 70, 75, 80, 84, 88, 92, 96, 100, 105, 110, 115, 120, 124, 128, 132, 136, 140, 144, 148, 152, 155};
+  static constexpr int LT_SC_ROW[155] = {2, 11, 3, 11, 11, 3, 11, 2, 3, 11, 2, 8, 11, 9, 11, 11, 9, 11, 8, 9, 11, 8, 9, 11, 10, 11, 11, 10, 11, 9, 2, 11, 4, 4, 2, 5, 11, 4, 2, 5, 11, 2, 3, 6, 2, 3, 6, 6, 2, 5, 3, 5, 3, 6, 2, 5, 11, 2, 2, 11, 5, 2, 2, 2, 2, 4, 11, 2, 11, 11, 4, 2, 2, 2, 11, 4, 2, 2, 2, 11, 4, 3, 2, 2, 4, 3, 6, 11, 6, 11, 2, 2, 6, 11, 2, 2, 6, 4, 2, 5, 2, 2, 2, 5, 2, 2, 2, 2, 5, 2, 5, 2, 2, 5, 5, 5, 2, 2, 5, 5, 3, 0, 2, 1, 2, 1, 3, 0, 5, 1, 7, 3, 7, 3, 5, 1, 5, 0, 7, 2, 5, 0, 7, 2, 7, 2, 5, 0, 0, 4, 7, 11, 1, 11, 0};
+  static constexpr int LT_SC_SGN[155] = {-1, -1, 1, 1, 1, -1, -1, 1, -1, -1, 1, -1, -1, 1, 1, 1, -1, -1, 1, -1, -1, 1, -1, -1, 1, 1, 1, -1, -1, 1, -1, -1, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1, -1, -1, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, 1, -1, -1, 1, 1, 1, -1, -1, 1, 1, 1, -1, -1, 1, 1, 1, -1, -1, 1, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, -1, 1, 1, -1, -1, -1, 1, 1, -1, -1, -1, 1, 1, -1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1};
+  static constexpr int LT_JT_OFF[39] = {0, 2, 4, 6, 8, 10, 12, 14, 16, 18, 20, 22, 24, 26, 28, 30, 32, 34, 36, 38, 40, 42, 44, 46, 48, 50, 51, 52, 54, 56, 58, 60, 62, 64, 66, 68, 70, 72, 74};
+  static constexpr int LT_JT_COL[74] = {2, 11, 3, 11, 3, 11, 8, 11, 9, 11, 9, 11, 9, 11, 10, 11, 2, 11, 2, 4, 2, 4, 2, 3, 2, 3, 2, 6, 3, 5, 5, 11, 2, 5, 4, 11, 2, 4, 2, 4, 3, 4, 3, 4, 6, 11, 6, 11, 4, 6, 2, 2, 2, 5, 2, 5, 0, 3, 1, 2, 1, 5, 3, 7, 0, 5, 0, 5, 2, 7, 0, 4, 1, 11};
+  static constexpr int LT_JT_MULT[74] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 3, 3, 2, 1, 2, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+  static constexpr int LT_JT_FOFF[75] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 31, 32, 33, 34, 35, 36, 37, 38, 39, 40, 41, 42, 43, 44, 45, 46, 47, 48, 49, 50, 52, 54, 56, 58, 60, 62, 63, 64, 65, 66, 67, 68, 69, 70, 71, 72, 73, 74, 75, 76, 77, 78, 79, 80};
+  static constexpr int LT_JF_SP[80] = {11, 2, 11, 3, 11, 3, 11, 8, 11, 9, 11, 9, 11, 9, 11, 10, 11, 2, 4, 2, 4, 2, 3, 2, 3, 2, 6, 2, 5, 3, 11, 5, 5, 2, 11, 4, 4, 2, 4, 2, 4, 3, 4, 3, 11, 6, 11, 6, 6, 4, 2, 2, 2, 2, 5, 2, 2, 2, 5, 2, 2, 2, 3, 0, 2, 1, 5, 1, 7, 3, 5, 0, 5, 0, 7, 2, 4, 0, 11, 1};
   // dn/dt = sum over reactions of S k prod(n), in reaction order
   template <class R>
-  KX_HD static void rhs(const R* n, const R* k, R* dn) {
+  KX_HD static void rhs_loop(const R* n, const R* k, R* dn) {
+    for (int i = 0; i < NSPEC; ++i) dn[i] = R(0);
+    for (int r = 0; r < NREA; ++r) {
+      R fl = k[r];
+      for (int p = LT_KF_OFF[r]; p < LT_KF_OFF[r + 1]; ++p)
+        fl = fl * n[LT_KF_SP[p]];
+      for (int q = LT_SC_OFF[r]; q < LT_SC_OFF[r + 1]; ++q) {
+        if (LT_SC_SGN[q] > 0) dn[LT_SC_ROW[q]] += fl;
+        else dn[LT_SC_ROW[q]] -= fl;
+      }
+    }
+  }
+  // d(dn_i)/d(n_j) at fixed rates into J (column-major, element
+  // (i, j) at J[i + j ld]; added to, the caller zeroes it)
+  template <class R>
+  KX_HD static void jac_species_loop(const R* n, const R* k, R* J,
+                                     int ld) {
+    for (int r = 0; r < NREA; ++r)
+      for (int t = LT_JT_OFF[r]; t < LT_JT_OFF[r + 1]; ++t) {
+        R d = k[r];
+        for (int p = LT_JT_FOFF[t]; p < LT_JT_FOFF[t + 1]; ++p)
+          d = d * n[LT_JF_SP[p]];
+        if (LT_JT_MULT[t] > 1) d = d * R(LT_JT_MULT[t]);
+        R* col = J + (long)LT_JT_COL[t] * ld;
+        for (int q = LT_SC_OFF[r]; q < LT_SC_OFF[r + 1]; ++q) {
+          if (LT_SC_SGN[q] > 0) col[LT_SC_ROW[q]] += d;
+          else col[LT_SC_ROW[q]] -= d;
+        }
+      }
+  }
+  template <class R>
+  KX_HD static void rhs_unrolled(const R* n, const R* k, R* dn) {
     for (int i = 0; i < NSPEC; ++i) dn[i] = R(0);
     R fl;
     fl = k[0]*n[2]*n[11];  // H + e- -> H+ + e- + e-
@@ -522,10 +565,9 @@ struct net_primordial {
     dn[0] += fl;
     (void)fl;
   }
-  // d(dn_i)/d(n_j) at fixed rates into J (column-major, element
-  // (i, j) at J[i + j ld]; added to, the caller zeroes it)
   template <class R>
-  KX_HD static void jac_species(const R* n, const R* k, R* J, int ld) {
+  KX_HD static void jac_species_unrolled(const R* n, const R* k, R* J,
+                                         int ld) {
     R d;
     d = k[0]*n[11];  // H + e- -> H+ + e- + e-, d/d n[2]
     J[2 + 2 * ld] -= d;
@@ -902,6 +944,21 @@ struct net_primordial {
     J[11 + 11 * ld] -= d;
     J[0 + 11 * ld] += d;
     (void)d; (void)n; (void)k; (void)J; (void)ld;
+  }
+#if defined(KX_FORCE_RHS_LOOP)
+  static constexpr bool RHS_LOOP_USED = true;
+#else
+  static constexpr bool RHS_LOOP_USED = RHS_LOOP;
+#endif
+  template <class R>
+  KX_HD static void rhs(const R* n, const R* k, R* dn) {
+    if constexpr (RHS_LOOP_USED) rhs_loop(n, k, dn);
+    else rhs_unrolled(n, k, dn);
+  }
+  template <class R>
+  KX_HD static void jac_species(const R* n, const R* k, R* J, int ld) {
+    if constexpr (RHS_LOOP_USED) jac_species_loop(n, k, J, ld);
+    else jac_species_unrolled(n, k, J, ld);
   }
 };
 }  // namespace kx

@@ -277,9 +277,52 @@ struct net_primordial3 {
     rates_n(Trate, n, p, k);
   }
   static constexpr int NREA_T = 25;
+  // the loop form's index tables (netgen.loop_tables)
+  static constexpr bool RHS_LOOP = false;
+  static constexpr int LT_KF_OFF[28] = {0, 2, 4, 6, 8, 10, 12, 14, 16, 18, 20, 22, 24, 26, 28, 30, 32, 34, 36, 38, 40, 42, 44, 46, 48, 51, 54, 56};
+  static constexpr int LT_KF_SP[56] = {0, 8, 1, 8, 1, 8, 5, 8, 6, 8, 6, 8, 6, 8, 7, 8, 0, 8, 2, 0, 0, 1, 0, 1, 4, 0, 3, 1, 3, 8, 3, 8, 3, 0, 2, 8, 2, 0, 2, 0, 2, 1, 2, 1, 4, 8, 4, 2, 0, 0, 0, 3, 0, 0, 3, 3};
+  static constexpr int LT_SC_OFF[28] = {0, 5, 8, 11, 16, 19, 22, 27, 30, 33, 37, 40, 43, 47, 51, 55, 60, 65, 70, 75, 80, 84, 88, 92, 96, 101, 106, 111};
+  static constexpr int LT_SC_ROW[111] = {0, 8, 1, 8, 8, 1, 8, 0, 1, 8, 0, 5, 8, 6, 8, 8, 6, 8, 5, 6, 8, 5, 6, 8, 7, 8, 8, 7, 8, 6, 0, 8, 2, 2, 0, 3, 8, 0, 1, 4, 0, 1, 4, 4, 0, 3, 1, 3, 1, 4, 0, 3, 8, 0, 2, 3, 8, 0, 0, 8, 3, 0, 0, 0, 0, 2, 8, 0, 8, 8, 2, 0, 0, 0, 8, 2, 0, 0, 0, 8, 2, 1, 0, 0, 2, 1, 4, 8, 4, 8, 0, 0, 4, 2, 0, 3, 0, 0, 0, 3, 0, 3, 0, 0, 3, 3, 3, 3, 0, 0, 3};
+  static constexpr int LT_SC_SGN[111] = {-1, -1, 1, 1, 1, -1, -1, 1, -1, -1, 1, -1, -1, 1, 1, 1, -1, -1, 1, -1, -1, 1, -1, -1, 1, 1, 1, -1, -1, 1, -1, -1, 1, -1, -1, 1, 1, -1, -1, 1, -1, -1, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, 1, -1, -1, 1, 1, 1, -1, -1, 1, 1, 1, -1, -1, 1, 1, 1, -1, -1, 1, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, -1, 1, 1, -1, -1, -1, 1, 1, -1, -1, 1, 1, 1};
+  static constexpr int LT_JT_OFF[28] = {0, 2, 4, 6, 8, 10, 12, 14, 16, 18, 20, 22, 24, 26, 28, 30, 32, 34, 36, 38, 40, 42, 44, 46, 48, 49, 51, 52};
+  static constexpr int LT_JT_COL[52] = {0, 8, 1, 8, 1, 8, 5, 8, 6, 8, 6, 8, 6, 8, 7, 8, 0, 8, 0, 2, 0, 1, 0, 1, 0, 4, 1, 3, 3, 8, 3, 8, 0, 3, 2, 8, 0, 2, 0, 2, 1, 2, 1, 2, 4, 8, 2, 4, 0, 0, 3, 3};
+  static constexpr int LT_JT_MULT[52] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 3, 2, 1, 2};
+  static constexpr int LT_JT_FOFF[53] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 31, 32, 33, 34, 35, 36, 37, 38, 39, 40, 41, 42, 43, 44, 45, 46, 47, 48, 50, 52, 54, 55};
+  static constexpr int LT_JF_SP[55] = {8, 0, 8, 1, 8, 1, 8, 5, 8, 6, 8, 6, 8, 6, 8, 7, 8, 0, 2, 0, 1, 0, 1, 0, 4, 0, 3, 1, 8, 3, 8, 3, 3, 0, 8, 2, 2, 0, 2, 0, 2, 1, 2, 1, 8, 4, 4, 2, 0, 0, 3, 0, 0, 0, 3};
   // dn/dt = sum over reactions of S k prod(n), in reaction order
   template <class R>
-  KX_HD static void rhs(const R* n, const R* k, R* dn) {
+  KX_HD static void rhs_loop(const R* n, const R* k, R* dn) {
+    for (int i = 0; i < NSPEC; ++i) dn[i] = R(0);
+    for (int r = 0; r < NREA; ++r) {
+      R fl = k[r];
+      for (int p = LT_KF_OFF[r]; p < LT_KF_OFF[r + 1]; ++p)
+        fl = fl * n[LT_KF_SP[p]];
+      for (int q = LT_SC_OFF[r]; q < LT_SC_OFF[r + 1]; ++q) {
+        if (LT_SC_SGN[q] > 0) dn[LT_SC_ROW[q]] += fl;
+        else dn[LT_SC_ROW[q]] -= fl;
+      }
+    }
+  }
+  // d(dn_i)/d(n_j) at fixed rates into J (column-major, element
+  // (i, j) at J[i + j ld]; added to, the caller zeroes it)
+  template <class R>
+  KX_HD static void jac_species_loop(const R* n, const R* k, R* J,
+                                     int ld) {
+    for (int r = 0; r < NREA; ++r)
+      for (int t = LT_JT_OFF[r]; t < LT_JT_OFF[r + 1]; ++t) {
+        R d = k[r];
+        for (int p = LT_JT_FOFF[t]; p < LT_JT_FOFF[t + 1]; ++p)
+          d = d * n[LT_JF_SP[p]];
+        if (LT_JT_MULT[t] > 1) d = d * R(LT_JT_MULT[t]);
+        R* col = J + (long)LT_JT_COL[t] * ld;
+        for (int q = LT_SC_OFF[r]; q < LT_SC_OFF[r + 1]; ++q) {
+          if (LT_SC_SGN[q] > 0) col[LT_SC_ROW[q]] += d;
+          else col[LT_SC_ROW[q]] -= d;
+        }
+      }
+  }
+  template <class R>
+  KX_HD static void rhs_unrolled(const R* n, const R* k, R* dn) {
     for (int i = 0; i < NSPEC; ++i) dn[i] = R(0);
     R fl;
     fl = k[0]*n[0]*n[8];  // H + e- -> H+ + e- + e-
@@ -422,10 +465,9 @@ struct net_primordial3 {
     dn[3] += fl;
     (void)fl;
   }
-  // d(dn_i)/d(n_j) at fixed rates into J (column-major, element
-  // (i, j) at J[i + j ld]; added to, the caller zeroes it)
   template <class R>
-  KX_HD static void jac_species(const R* n, const R* k, R* J, int ld) {
+  KX_HD static void jac_species_unrolled(const R* n, const R* k, R* J,
+                                         int ld) {
     R d;
     d = k[0]*n[8];  // H + e- -> H+ + e- + e-, d/d n[0]
     J[0 + 0 * ld] -= d;
@@ -692,6 +734,21 @@ struct net_primordial3 {
     J[0 + 3 * ld] += d;
     J[3 + 3 * ld] += d;
     (void)d; (void)n; (void)k; (void)J; (void)ld;
+  }
+#if defined(KX_FORCE_RHS_LOOP)
+  static constexpr bool RHS_LOOP_USED = true;
+#else
+  static constexpr bool RHS_LOOP_USED = RHS_LOOP;
+#endif
+  template <class R>
+  KX_HD static void rhs(const R* n, const R* k, R* dn) {
+    if constexpr (RHS_LOOP_USED) rhs_loop(n, k, dn);
+    else rhs_unrolled(n, k, dn);
+  }
+  template <class R>
+  KX_HD static void jac_species(const R* n, const R* k, R* J, int ld) {
+    if constexpr (RHS_LOOP_USED) jac_species_loop(n, k, J, ld);
+    else jac_species_unrolled(n, k, J, ld);
   }
   // thermal terms at the thermal temperature Tth (capped to
   // [2.73, 1e9] K by the caller) with the rates k of the state

@@ -364,6 +364,50 @@ def sparse_lu_tables(net, em):
             "flops": flops}
 
 
+# networks with more reactions than this evaluate the rhs and the species
+# Jacobian by loops over index tables (the reference's RHS_LOOP_MIN,
+# codegen.py:8151-8160)
+RHS_LOOP_MIN = 500
+
+
+def loop_tables(net):
+    """index tables of the loop-form rhs and species Jacobian, in the order
+    of the unrolled code (so both do the same operations in the same
+    order; the reference's tables, codegen.py:8162-8223):
+      kf_off/kf_sp   per reaction, its kinetic reactants (the flux factors)
+      sc_off/sc_row/sc_sgn   per reaction, the rows the flux is added to
+                     (reactants -1, products +1, duplicates kept)
+      jt_off         per reaction, its Jacobian terms (one per distinct
+                     kinetic reactant j, ascending): jt_col = j, jt_mult =
+                     its multiplicity, factors jf_sp[jt_foff[t] ..] = the
+                     other kinetic reactants in order"""
+    T = {k: [] for k in ("kf_off", "kf_sp", "sc_off", "sc_row", "sc_sgn",
+                         "jt_off", "jt_col", "jt_mult", "jt_foff", "jf_sp")}
+    for r in net.reactions:
+        kin = [net.idx(x) for x, k in r.reactants if k]
+        T["kf_off"].append(len(T["kf_sp"]))
+        T["kf_sp"] += kin
+        T["sc_off"].append(len(T["sc_row"]))
+        for x, _ in r.reactants:
+            T["sc_row"].append(net.idx(x))
+            T["sc_sgn"].append(-1)
+        for x in r.products:
+            T["sc_row"].append(net.idx(x))
+            T["sc_sgn"].append(1)
+        T["jt_off"].append(len(T["jt_col"]))
+        for j in sorted(set(kin)):
+            others = list(kin)
+            others.remove(j)
+            T["jt_col"].append(j)
+            T["jt_mult"].append(kin.count(j))
+            T["jt_foff"].append(len(T["jf_sp"]))
+            T["jf_sp"] += others
+    for off, tab in (("kf_off", "kf_sp"), ("sc_off", "sc_row"),
+                     ("jt_off", "jt_col"), ("jt_foff", "jf_sp")):
+        T[off].append(len(T[tab]))
+    return T
+
+
 def _group(net):
     """tile the ranged lines of identical stoichiometry into piecewise
     groups (network-format.md 3.1); an unranged line is its own group"""
@@ -655,6 +699,7 @@ def generate(knet_path, thermal=None, name=None):
     """C++ header text of a network (+ its thermal configuration) and a
     metadata dict (species, masses, params) for the host side"""
     net = parse_knet(knet_path)
+    loop_only = net.nrea > RHS_LOOP_MIN
     th = thermal or Thermal()
     pnames = param_list(net, th)
     em = Emitter(net, pnames)
@@ -853,44 +898,109 @@ def generate(knet_path, thermal=None, name=None):
     w("  static constexpr int NREA_T = %d;"
       % sum(len(g) for g in tgroups))
 
-    # rhs
+    # rhs and species Jacobian: unrolled (up to RHS_LOOP_MIN reactions),
+    # and as loops over index tables (always; the only form above it)
+    lt = loop_tables(net)
+    w("  // the loop form's index tables (netgen.loop_tables)")
+    w("  static constexpr bool RHS_LOOP = %s;"
+      % ("true" if loop_only else "false"))
+    for nm in ("kf_off", "kf_sp", "sc_off", "sc_row", "sc_sgn", "jt_off",
+               "jt_col", "jt_mult", "jt_foff", "jf_sp"):
+        vals = lt[nm] or [0]
+        w("  static constexpr int LT_%s[%d] = {%s};"
+          % (nm.upper(), len(vals), ", ".join(str(v) for v in vals)))
     w("  // dn/dt = sum over reactions of S k prod(n), in reaction order")
     w("  template <class R>")
-    w("  KX_HD static void rhs(const R* n, const R* k, R* dn) {")
+    w("  KX_HD static void rhs_loop(const R* n, const R* k, R* dn) {")
     w("    for (int i = 0; i < NSPEC; ++i) dn[i] = R(0);")
-    w("    R fl;")
-    for r in net.reactions:
-        prod = "*".join(["k[%d]" % r.index] + ["n[%d]" % net.idx(s)
-                                                for s, kin in r.reactants
-                                                if kin])
-        w("    fl = %s;  // %s" % (prod, r.eq))
-        for s, _ in r.reactants:
-            w("    dn[%d] -= fl;" % net.idx(s))
-        for s in r.products:
-            w("    dn[%d] += fl;" % net.idx(s))
-    w("    (void)fl;")
+    w("    for (int r = 0; r < NREA; ++r) {")
+    w("      R fl = k[r];")
+    w("      for (int p = LT_KF_OFF[r]; p < LT_KF_OFF[r + 1]; ++p)")
+    w("        fl = fl * n[LT_KF_SP[p]];")
+    w("      for (int q = LT_SC_OFF[r]; q < LT_SC_OFF[r + 1]; ++q) {")
+    w("        if (LT_SC_SGN[q] > 0) dn[LT_SC_ROW[q]] += fl;")
+    w("        else dn[LT_SC_ROW[q]] -= fl;")
+    w("      }")
+    w("    }")
     w("  }")
-    # analytic species Jacobian (fixed rates)
     w("  // d(dn_i)/d(n_j) at fixed rates into J (column-major, element")
     w("  // (i, j) at J[i + j ld]; added to, the caller zeroes it)")
     w("  template <class R>")
-    w("  KX_HD static void jac_species(const R* n, const R* k, R* J, int ld) {")
-    w("    R d;")
-    for r in net.reactions:
-        kin = [net.idx(s) for s, k_ in r.reactants if k_]
-        for j in sorted(set(kin)):
-            mult = kin.count(j)
-            others = list(kin)
-            others.remove(j)
-            terms = ["k[%d]" % r.index] + ["n[%d]" % o for o in others]
-            if mult > 1:
-                terms.append("R(%r)" % float(mult))
-            w("    d = %s;  // %s, d/d n[%d]" % ("*".join(terms), r.eq, j))
+    w("  KX_HD static void jac_species_loop(const R* n, const R* k, R* J,")
+    w("                                     int ld) {")
+    w("    for (int r = 0; r < NREA; ++r)")
+    w("      for (int t = LT_JT_OFF[r]; t < LT_JT_OFF[r + 1]; ++t) {")
+    w("        R d = k[r];")
+    w("        for (int p = LT_JT_FOFF[t]; p < LT_JT_FOFF[t + 1]; ++p)")
+    w("          d = d * n[LT_JF_SP[p]];")
+    w("        if (LT_JT_MULT[t] > 1) d = d * R(LT_JT_MULT[t]);")
+    w("        R* col = J + (long)LT_JT_COL[t] * ld;")
+    w("        for (int q = LT_SC_OFF[r]; q < LT_SC_OFF[r + 1]; ++q) {")
+    w("          if (LT_SC_SGN[q] > 0) col[LT_SC_ROW[q]] += d;")
+    w("          else col[LT_SC_ROW[q]] -= d;")
+    w("        }")
+    w("      }")
+    w("  }")
+    if not loop_only:
+        w("  template <class R>")
+        w("  KX_HD static void rhs_unrolled(const R* n, const R* k, R* dn) {")
+        w("    for (int i = 0; i < NSPEC; ++i) dn[i] = R(0);")
+        w("    R fl;")
+        for r in net.reactions:
+            prod = "*".join(["k[%d]" % r.index] + ["n[%d]" % net.idx(s)
+                                                    for s, kin in r.reactants
+                                                    if kin])
+            w("    fl = %s;  // %s" % (prod, r.eq))
             for s, _ in r.reactants:
-                w("    J[%d + %d * ld] -= d;" % (net.idx(s), j))
+                w("    dn[%d] -= fl;" % net.idx(s))
             for s in r.products:
-                w("    J[%d + %d * ld] += d;" % (net.idx(s), j))
-    w("    (void)d; (void)n; (void)k; (void)J; (void)ld;")
+                w("    dn[%d] += fl;" % net.idx(s))
+        w("    (void)fl;")
+        w("  }")
+        w("  template <class R>")
+        w("  KX_HD static void jac_species_unrolled(const R* n, const R* k, "
+          "R* J,")
+        w("                                         int ld) {")
+        w("    R d;")
+        for r in net.reactions:
+            kin = [net.idx(s) for s, k_ in r.reactants if k_]
+            for j in sorted(set(kin)):
+                mult = kin.count(j)
+                others = list(kin)
+                others.remove(j)
+                terms = ["k[%d]" % r.index] + ["n[%d]" % o for o in others]
+                if mult > 1:
+                    terms.append("R(%r)" % float(mult))
+                w("    d = %s;  // %s, d/d n[%d]" % ("*".join(terms), r.eq,
+                                                   j))
+                for s, _ in r.reactants:
+                    w("    J[%d + %d * ld] -= d;" % (net.idx(s), j))
+                for s in r.products:
+                    w("    J[%d + %d * ld] += d;" % (net.idx(s), j))
+        w("    (void)d; (void)n; (void)k; (void)J; (void)ld;")
+        w("  }")
+    # the form the solvers call: the loop form above RHS_LOOP_MIN
+    # reactions, or everywhere when compiled with KX_FORCE_RHS_LOOP
+    w("#if defined(KX_FORCE_RHS_LOOP)")
+    w("  static constexpr bool RHS_LOOP_USED = true;")
+    w("#else")
+    w("  static constexpr bool RHS_LOOP_USED = RHS_LOOP;")
+    w("#endif")
+    w("  template <class R>")
+    w("  KX_HD static void rhs(const R* n, const R* k, R* dn) {")
+    if loop_only:
+        w("    rhs_loop(n, k, dn);")
+    else:
+        w("    if constexpr (RHS_LOOP_USED) rhs_loop(n, k, dn);")
+        w("    else rhs_unrolled(n, k, dn);")
+    w("  }")
+    w("  template <class R>")
+    w("  KX_HD static void jac_species(const R* n, const R* k, R* J, int ld) {")
+    if loop_only:
+        w("    jac_species_loop(n, k, J, ld);")
+    else:
+        w("    if constexpr (RHS_LOOP_USED) jac_species_loop(n, k, J, ld);")
+        w("    else jac_species_unrolled(n, k, J, ld);")
     w("  }")
 
     # thermal

@@ -274,6 +274,32 @@ void jacs(int ncell, const double* n, const double* T, const double* par,
   }
 }
 
+// the species rhs and Jacobian at fixed rates in both generated forms,
+// unrolled and the loop form over index tables: f[2][ncell NSPEC],
+// J[2][ncell NSPEC^2] (column-major per cell)
+template <class Net>
+void forms(int ncell, const double* n, const double* T, const double* par,
+           double* f, double* J) {
+  constexpr int NS = Net::NSPEC;
+  const size_t nf = static_cast<size_t>(ncell) * NS;
+  const size_t nj = nf * NS;
+  for (int c = 0; c < ncell; ++c) {
+    Problem<Net> P;
+    P.p = Net::params_from_row(par + static_cast<size_t>(c) * Net::NPAR);
+    P.Tfix = T[c];
+    const double* y = n + static_cast<size_t>(c) * NS;
+    double k[Net::NREA + 1];
+    P.rates(P.rate_T(y), y, k);
+    Net::rhs_unrolled(y, k, f + static_cast<size_t>(c) * NS);
+    Net::rhs_loop(y, k, f + nf + static_cast<size_t>(c) * NS);
+    double* Ju = J + static_cast<size_t>(c) * NS * NS;
+    double* Jl = J + nj + static_cast<size_t>(c) * NS * NS;
+    for (int q = 0; q < NS * NS; ++q) Ju[q] = Jl[q] = 0.0;
+    Net::jac_species_unrolled(y, k, Ju, NS);
+    Net::jac_species_loop(y, k, Jl, NS);
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -348,6 +374,20 @@ int KXO_(jac)(int net, int ncell, const double* n, const double* T,
       return 0;
     case 1:
       jacs<kx::net_primordial3>(ncell, n, T, par, J);
+      return 0;
+  }
+  return -1;
+}
+
+// the species rhs and Jacobian in both generated forms (see forms())
+int KXO_(rhs_forms)(int net, int ncell, const double* n, const double* T,
+                    const double* par, double* f, double* J) {
+  switch (net) {
+    case 0:
+      forms<kx::net_primordial>(ncell, n, T, par, f, J);
+      return 0;
+    case 1:
+      forms<kx::net_primordial3>(ncell, n, T, par, f, J);
       return 0;
   }
   return -1;

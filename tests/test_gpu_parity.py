@@ -37,11 +37,9 @@ def nets():
 
 
 def _opts(net, d, kernel="plain", mixed=0, **kw):
-    o = d["opts"] if isinstance(d, dict) is False else None
-    if o is not None:
-        base = dict(evolve_T=bool(o[0]), rtol=o[1], atol_n=o[2], atol_T=o[3])
-    else:
-        base = {}
+    """options of a golden fixture's solve (its opts row), plus kw"""
+    o = d["opts"]
+    base = dict(evolve_T=bool(o[0]), rtol=o[1], atol_n=o[2], atol_T=o[3])
     base.update(kw)
     return net.options(device_kernel=kernel, mixed_precision=mixed,
                        error_report=False, **base)
@@ -54,8 +52,6 @@ def _tol(netname, mixed):
 @pytest.mark.parametrize("kernel,mixed", KERNELS)
 @pytest.mark.parametrize("netname", ["primordial3", "primordial"])
 def test_golden_cells(nets, netname, kernel, mixed, oracle):
-    if netname == "primordial" and kernel == "persistent" and mixed:
-        pass
     d = golden("%s_solve.npz" % netname)
     net = nets[netname]
     say("golden", netname, kernel, mixed, d["n0"].shape)
