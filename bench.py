@@ -6,7 +6,7 @@ gamma EXACT, H2 opacity RIPAMONTI; rtol 1e-8, atol_n 1e-20, atol_T 1e-6).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--ncell N]
                     [--kernel persistent|plain] [--fp64]
-                    [--workload solve|conserve|copy]
+                    [--workload solve|conserve|multi|copy]
                     [--linsol dense|sparse]
 
 A step is one solve of the whole batch from its initial state: the inputs
@@ -27,6 +27,13 @@ moved back to their own element totals; species-major rows, one reference
 row per cell.  A step restores the rows from a resident copy and runs the
 call; the roofline is the kernel's algorithmic bytes over its event-timed
 duration.
+
+--workload multi measures several networks in one persistent launch
+(libkrome_multi.so): ncell/4 of the primordial3 cells above and ncell/2
+primordial cells of the uneven batch (seed 2, density 1..1e12 cm^-3, dt
+1e8..1e14 s, Tgas fixed, rtol 1e-6) solved by one launch, against the
+same two batches solved by two launches of the networks' own libraries.
+FP64 Newton unless --mixed.
 
 --workload copy is the calibration of the HBM traffic counters: one
 device-to-device copy of 2 GiB (far above the last-level cache), whose
@@ -214,6 +221,98 @@ def conserve_bench(a):
     print(json.dumps(out))
 
 
+def multi_bench(a):
+    """one launch of two networks against two launches (one JSON line)"""
+    import torch
+    import cells
+    import krome
+
+    dev = torch.device("cuda:0")
+    # FP64 unless --mixed: on the cold, dense, long-dt primordial cells of
+    # this batch the FP32 Newton matrix loses the trace ions
+    mixed = 1 if a.mixed else 0
+    m = krome.load_multi()
+    nets = {n: krome.load(n) for n in ("primordial", "primordial3")}
+    n3, n1 = a.ncell // 4, a.ncell // 2
+    ws = [("primordial3", cells.primordial3_cells(n3), 1, 1e-8),
+          ("primordial", cells.c3_primordial_cells(n1), 0, 1e-6)]
+    batches, pristine = [], []
+    for name, w, et, rtol in ws:
+        b = dict(network=name, evolve_T=et, rtol=rtol, atol_n=1e-20,
+                 atol_T=1e-6,
+                 n=torch.tensor(np.ascontiguousarray(w["n"].T), device=dev),
+                 Tgas=torch.tensor(w["T"], device=dev),
+                 dt=torch.tensor(w["dt"], device=dev),
+                 par=torch.tensor(np.ascontiguousarray(w["par"]), device=dev),
+                 flags=torch.zeros(w["n"].shape[0], dtype=torch.int32,
+                                   device=dev))
+        batches.append(b)
+        pristine.append((b["n"].clone(), b["Tgas"].clone()))
+
+    def restore():
+        for b, (n0, T0) in zip(batches, pristine):
+            b["n"].copy_(n0)
+            b["Tgas"].copy_(T0)
+
+    kms = ctypes.c_double()
+    kp = ctypes.cast(ctypes.byref(kms), ctypes.POINTER(ctypes.c_double))
+
+    def separate():
+        total = 0.0
+        for b in batches:
+            net = nets[b["network"]]
+            o = net.options(evolve_T=bool(b["evolve_T"]), rtol=b["rtol"],
+                            atol_n=b["atol_n"], atol_T=b["atol_T"],
+                            device_kernel="persistent",
+                            mixed_precision=mixed, error_report=False)
+            r = net.solve_device(b["n"], b["Tgas"], b["dt"], b["par"],
+                                 opts=o, flags=b["flags"])
+            if r != 0 or net._lib.krome_device_last_kernel_ms(kp) != 0:
+                raise SystemExit("separate solve returned %d" % r)
+            total += kms.value
+        return total
+
+    for _ in range(max(a.warmup, 1)):
+        restore()
+        m.solve_device(batches, mixed)
+    pair_ms, sep_ms = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        restore()
+        r, ms = m.solve_device(batches, mixed)
+        if r != 0:
+            raise SystemExit("pair launch returned %d" % r)
+        pair_ms.append(ms)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    pair_out = [b["n"].cpu().numpy() for b in batches]
+    for _ in range(a.steps):
+        restore()
+        sep_ms.append(separate())
+    same = all(np.array_equal(p, b["n"].cpu().numpy())
+               for p, b in zip(pair_out, batches))
+    ncell = n1 + n3
+    pm, sm = float(np.mean(pair_ms)), float(np.mean(sep_ms))
+    out = {
+        "metric": "cells/sec, two networks in one persistent launch",
+        "value": ncell * a.steps / wall, "unit": "cells/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64+f32" if mixed else "f64", "data": "synthetic",
+        "config": {"workload": "2^18 primordial3 collapse cells (seed 1) + "
+                               "2^19 primordial cells of the uneven batch "
+                               "(seed 2, density 1..1e12, dt 1e8..1e14 s)",
+                   "ncell": ncell, "mixed_precision": bool(mixed),
+                   "parallelism": "1 GPU"},
+        "pair_kernel_ms": pm, "separate_kernel_ms": sm,
+        "speedup_vs_separate_launches": sm / pm,
+        "results_bitwise_equal_to_separate": bool(same),
+    }
+    print(json.dumps(out))
+
+
 def copy_calibration():
     """one 2 GiB device-to-device copy (the traffic counters' reference)"""
     import torch
@@ -239,11 +338,14 @@ def main():
                     choices=("persistent", "plain"))
     ap.add_argument("--fp64", action="store_true",
                     help="the persistent kernel with the FP64 Newton")
+    ap.add_argument("--mixed", action="store_true",
+                    help="--workload multi: the mixed-precision Newton "
+                         "(default FP64)")
     ap.add_argument("--cpu-sample", type=int, default=1 << 18)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--workload", default="solve",
-                    choices=("solve", "conserve", "copy"))
+                    choices=("solve", "conserve", "multi", "copy"))
     ap.add_argument("--linsol", default="dense", choices=("dense", "sparse"),
                     help="the Newton matrix's LU (sparse: the plain kernel's "
                          "static-pivot sparse LU, linsol 3)")
@@ -257,6 +359,8 @@ def main():
         return conserve_bench(a)
     if a.workload == "copy":
         return copy_calibration()
+    if a.workload == "multi":
+        return multi_bench(a)
     import torch
     import cells
     import krome

@@ -33,6 +33,10 @@ def lib_path(net):
     return HERE / ("libkrome_%s.so" % net)
 
 
+# the library of several networks in one persistent launch (kx_multi.hip)
+MULTI = "multi"
+
+
 def regen(networks_dir=None):
     nd = networks_dir or os.environ.get("KROME_NETWORKS",
                                         "/root/reference/networks")
@@ -45,6 +49,9 @@ def regen(networks_dir=None):
 
 
 def _command(net, variant, tmp):
+    if net == MULTI:
+        return [HIPCC, *FLAGS, "-I", str(CSRC), "-I", str(GEN),
+                str(CSRC / "kx_multi.hip"), "-o", str(tmp)]
     extra = VARIANTS[variant][1] if variant else []
     return [HIPCC, *FLAGS, *extra, "-I", str(CSRC), "-I", str(GEN),
             "-DKX_NET=net_%s" % net,
@@ -68,8 +75,9 @@ def build_all(verbose=False, regen_headers=False):
     """every network library and test variant, compiled concurrently"""
     if regen_headers:
         regen()
-    jobs = [(n, None) for n in NETWORKS] + [(net, v) for v, (net, _)
-                                            in VARIANTS.items()]
+    jobs = ([(n, None) for n in NETWORKS] + [(net, v) for v, (net, _)
+                                             in VARIANTS.items()]
+            + [(MULTI, None)])
     procs = []
     for net, variant in jobs:
         out = lib_path(variant or net)

@@ -18,6 +18,11 @@ Workloads (the species order of the generated networks):
                 dt = 0.01 tff, tff = sqrt(3 pi / (32 G rho)) per cell
                 (the compression time scale parameter), Tcmb 2.725,
                 opratio 3
+  c3_primordial 2^19 cells, seed 2: the primordial cells of the uneven
+                batch, density log-uniform 1..1e12 cm^-3 and dt
+                log-uniform 1e8..1e14 s (step counts spread over more than
+                100x); Tgas 10..1e4 K, x 1e-6..1e-2, y 1e-8..1e-3 as
+                primordial; fixed Tgas
 """
 from __future__ import annotations
 
@@ -111,4 +116,23 @@ def primordial3_cells(ncell=1 << 20, seed=1, first=0):
                      "max_steps": 100000}}
 
 
-WORKLOADS = {"primordial": primordial_cells, "primordial3": primordial3_cells}
+def c3_primordial_cells(ncell=1 << 19, seed=2, first=0):
+    """the primordial part of the uneven-batch workload"""
+    meta = network_meta("primordial")
+    c = np.arange(first, first + ncell)
+    nH = log_uniform(seed, c, 0, 1.0, 1e12)
+    T = log_uniform(seed, c, 1, 10.0, 1e4)
+    x = log_uniform(seed, c, 2, 1e-6, 1e-2)
+    y = log_uniform(seed, c, 3, 1e-8, 1e-3)
+    n = _species_rows(meta, nH, x, y)
+    dt = log_uniform(seed, c, 4, 1e8, 1e14)
+    par = np.tile(np.array(meta["param_defaults"], dtype=np.float64),
+                  (ncell, 1))
+    return {"n": n, "T": T, "dt": dt, "par": par,
+            "opts": {"evolve_T": 0, "rtol": 1e-6, "atol_n": 1e-20,
+                     "atol_T": 1e-6, "linsol": 1, "jac": 0,
+                     "max_steps": 100000}}
+
+
+WORKLOADS = {"primordial": primordial_cells, "primordial3": primordial3_cells,
+             "c3_primordial": c3_primordial_cells}

@@ -1028,13 +1028,27 @@ struct Persist {
 // ---------------------------------------------------------------------------
 // the kernel
 
+// the workspace view of one network's kernel
 template <class Net, bool ET, bool MIXED>
-__global__ void __launch_bounds__(KX_PERSIST_BLOCK, KX_PERSIST_WAVES_PER_EU)
-persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
-                                               typename std::conditional<
-                                                   MIXED, float,
-                                                   double>::type> w0,
-                     int* queue, const int* order) {
+using PersistWorkOf =
+    PersistWork<Problem<Net, ET>::N,
+                typename std::conditional<MIXED, float, double>::type>;
+
+// doubles of a block's LDS that one network's lanes use
+template <class Net, bool ET>
+constexpr long lds_doubles() {
+  constexpr int N = Problem<Net, ET>::N;
+  return (long)lds_vectors<N>() * N * KX_PERSIST_BLOCK;
+}
+
+// one network's cells on this wave: its lanes take cells from the
+// network's queue until the queue is exhausted and every lane of the wave
+// is idle.  lds_vec: the block's LDS, each lane using only its own column
+// (tid) of it, so waves of one block may run different networks at once.
+template <class Net, bool ET, bool MIXED>
+__device__ __attribute__((always_inline)) inline void persist_run(
+    CellsArgs a, PersistWorkOf<Net, ET, MIXED> w0, int* queue,
+    const int* order, double* lds_vec) {
   using M = Persist<Net, ET, MIXED>;
   using Prob = Problem<Net, ET>;
   constexpr int N = M::N;
@@ -1052,7 +1066,6 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
       (ET ? (o.atol_T < o.atol_n ? o.atol_T : o.atol_n) : o.atol_n) == 0.0;
   const long mxstep = o.max_steps == 0 ? 500 : o.max_steps;
   LaneState lane_state;
-  __shared__ double lds_vec[lds_vectors<N>() * N * KX_PERSIST_BLOCK];
 
   int pc = P_FETCH;
   Prob P;
@@ -1727,6 +1740,30 @@ persist_cells_kernel(CellsArgs a, PersistWork<Problem<Net, ET>::N,
   }
 }
 
+template <class Net, bool ET, bool MIXED>
+__global__ void __launch_bounds__(KX_PERSIST_BLOCK, KX_PERSIST_WAVES_PER_EU)
+persist_cells_kernel(CellsArgs a, PersistWorkOf<Net, ET, MIXED> w0,
+                     int* queue, const int* order) {
+  __shared__ double lds_vec[lds_doubles<Net, ET>()];
+  persist_run<Net, ET, MIXED>(a, w0, queue, order, lds_vec);
+}
+
+// two networks in one launch: every wave runs the first network's cells
+// until that queue is exhausted, then the second's, so that the waves that
+// finish the first early fill the launch with the second instead of
+// leaving its tail to a launch of its own
+template <class NetA, bool ETA, class NetB, bool ETB, bool MIXED>
+__global__ void __launch_bounds__(KX_PERSIST_BLOCK, KX_PERSIST_WAVES_PER_EU)
+persist_pair_kernel(CellsArgs a, PersistWorkOf<NetA, ETA, MIXED> wa,
+                    int* qa, const int* oa, CellsArgs b,
+                    PersistWorkOf<NetB, ETB, MIXED> wb, int* qb,
+                    const int* ob) {
+  constexpr long LA = lds_doubles<NetA, ETA>(), LB = lds_doubles<NetB, ETB>();
+  __shared__ double lds_vec[LA > LB ? LA : LB];
+  persist_run<NetA, ETA, MIXED>(a, wa, qa, oa, lds_vec);
+  persist_run<NetB, ETB, MIXED>(b, wb, qb, ob, lds_vec);
+}
+
 // ---- the queue's order: the densest cells first ---------------------------
 // A cell's work grows with its density (its reaction rates, and so its
 // stiffness, scale with it; for primordial3 collapse cells the correlation
@@ -1784,41 +1821,56 @@ constexpr long persist_lane_bytes() {
          (long)sizeof(int) * N;
 }
 
-// the persistent kernel over resident arrays: workgroups of 256 lanes,
-// waves_per_cu waves on each of the device's CUs (0: automatic), the
-// workspace allocated for the resident lanes only
-template <class Net, bool ET, bool MIXED>
-int launch_persistent_t(CellsArgs a, int wpc, hipStream_t s) {
-  constexpr int N = Problem<Net, ET>::N;
-  using MT = typename std::conditional<MIXED, float, double>::type;
+// a launch's device buffers, grown as needed and kept: the workspace of
+// the lane slots (the queue counter first), and the queue's order (keys,
+// bucket counts and offsets, the permutation)
+struct PersistBuffers {
+  void* buf = nullptr;
+  size_t bytes = 0;
+  int* ord = nullptr;
+  long ord_n = 0;
+};
+
+// resident lanes of a launch over ncell cells: waves_per_cu waves on each
+// CU (0: automatic), no more than the cells rounded up to whole workgroups
+// or the slots
+inline int persist_lanes(long ncell, int wpc, long& lanes) {
   int dev = 0, ncu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return -1;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
                             dev) != hipSuccess)
     return -1;
   if (wpc <= 0) wpc = 4 * KX_PERSIST_WAVES_PER_EU;
-  long lanes = (long)ncu * wpc * 64;
-  // no more lanes than cells (rounded up to whole workgroups) or slots
+  lanes = (long)ncu * wpc * 64;
   constexpr long B = KX_PERSIST_BLOCK;
-  const long need = ((long)a.ncell + B - 1) / B * B;
+  const long need = (ncell + B - 1) / B * B;
   if (lanes > need) lanes = need;
   lanes = (lanes + B - 1) / B * B;
   if (lanes > KX_PERSIST_LANES) lanes = KX_PERSIST_LANES;
+  return 0;
+}
+
+// one network's buffers and queue order, on stream s: the workspace view
+// w, the queue counter (zeroed) and the permutation the lanes take cells
+// in
+template <class Net, bool ET, bool MIXED>
+int persist_prepare(CellsArgs a, PersistBuffers& pb, hipStream_t s,
+                    PersistWorkOf<Net, ET, MIXED>& w, int*& queue,
+                    int*& order) {
+  constexpr int N = Problem<Net, ET>::N;
+  using MT = typename std::conditional<MIXED, float, double>::type;
   const long slots = KX_PERSIST_LANES;
-  static thread_local void* buf = nullptr;
-  static thread_local size_t bytes = 0;
   const size_t want = persist_lane_bytes<N, MIXED>() * slots + 256;
-  if (want > bytes) {
-    if (buf) (void)hipFree(buf);
-    buf = nullptr;
-    bytes = 0;
-    if (hipMalloc(&buf, want) != hipSuccess) return FLAG_DEVICE_MEMORY;
-    bytes = want;
+  if (want > pb.bytes) {
+    if (pb.buf) (void)hipFree(pb.buf);
+    pb.buf = nullptr;
+    pb.bytes = 0;
+    if (hipMalloc(&pb.buf, want) != hipSuccess) return FLAG_DEVICE_MEMORY;
+    pb.bytes = want;
   }
-  char* p = static_cast<char*>(buf);
-  int* queue = reinterpret_cast<int*>(p);
+  char* p = static_cast<char*>(pb.buf);
+  queue = reinterpret_cast<int*>(p);
   p += 256;
-  PersistWork<N, MT> w;
   w.v = reinterpret_cast<double*>(p);
   p += sizeof(double) * (V_COUNT - lds_vectors<N>()) * N * slots;
   w.J = reinterpret_cast<MT*>(p);
@@ -1828,21 +1880,18 @@ int launch_persistent_t(CellsArgs a, int wpc, hipStream_t s) {
   w.tot = reinterpret_cast<long*>(p);
   p += sizeof(long) * TOT_SLOTS * slots;
   w.piv = reinterpret_cast<int*>(p);
-  // the queue's order (keys, bucket counts and offsets, the permutation)
-  static thread_local int* ord = nullptr;
-  static thread_local long ord_n = 0;
-  if (a.ncell > ord_n) {
-    if (ord) (void)hipFree(ord);
-    ord = nullptr;
-    ord_n = 0;
-    if (hipMalloc(&ord, sizeof(int) * (2L * a.ncell + ORDER_BUCKETS)) !=
+  if (a.ncell > pb.ord_n) {
+    if (pb.ord) (void)hipFree(pb.ord);
+    pb.ord = nullptr;
+    pb.ord_n = 0;
+    if (hipMalloc(&pb.ord, sizeof(int) * (2L * a.ncell + ORDER_BUCKETS)) !=
         hipSuccess)
       return FLAG_DEVICE_MEMORY;
-    ord_n = a.ncell;
+    pb.ord_n = a.ncell;
   }
-  int* hist = ord;
-  int* key = ord + ORDER_BUCKETS;
-  int* order = key + a.ncell;
+  int* hist = pb.ord;
+  int* key = pb.ord + ORDER_BUCKETS;
+  order = key + a.ncell;
   const int cb = (a.ncell + 255) / 256;
   hipLaunchKernelGGL(fill_flags_kernel, dim3(1), dim3(64), 0, s, 1, queue, 0);
   hipLaunchKernelGGL(fill_flags_kernel, dim3(ORDER_BUCKETS / 256), dim3(256),
@@ -1852,9 +1901,49 @@ int launch_persistent_t(CellsArgs a, int wpc, hipStream_t s) {
   hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(64), 0, s, hist);
   hipLaunchKernelGGL(order_scatter_kernel, dim3(cb), dim3(256), 0, s,
                      a.ncell, key, hist, order);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// the persistent kernel over resident arrays: workgroups of 256 lanes,
+// waves_per_cu waves on each of the device's CUs (0: automatic), the
+// workspace allocated for the resident lanes only
+template <class Net, bool ET, bool MIXED>
+int launch_persistent_t(CellsArgs a, int wpc, hipStream_t s) {
+  long lanes = 0;
+  if (persist_lanes(a.ncell, wpc, lanes) != 0) return -1;
+  static thread_local PersistBuffers pb;
+  PersistWorkOf<Net, ET, MIXED> w;
+  int* queue = nullptr;
+  int* order = nullptr;
+  const int r = persist_prepare<Net, ET, MIXED>(a, pb, s, w, queue, order);
+  if (r != 0) return r;
+  constexpr long B = KX_PERSIST_BLOCK;
   hipLaunchKernelGGL((persist_cells_kernel<Net, ET, MIXED>),
                      dim3(lanes / B), dim3(B), 0, s, a, w, queue,
                      (const int*)order);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// two networks' cells in one persistent launch (persist_pair_kernel): the
+// lanes are those of the larger batch; each network keeps its own
+// workspace and queue
+template <class NetA, bool ETA, class NetB, bool ETB, bool MIXED>
+int launch_persistent_pair_t(CellsArgs a, CellsArgs b, int wpc,
+                             hipStream_t s) {
+  long lanes = 0;
+  if (persist_lanes(a.ncell > b.ncell ? a.ncell : b.ncell, wpc, lanes) != 0)
+    return -1;
+  static thread_local PersistBuffers pa, pb;
+  PersistWorkOf<NetA, ETA, MIXED> wa;
+  PersistWorkOf<NetB, ETB, MIXED> wb;
+  int *qa = nullptr, *oa = nullptr, *qb = nullptr, *ob = nullptr;
+  int r = persist_prepare<NetA, ETA, MIXED>(a, pa, s, wa, qa, oa);
+  if (r == 0) r = persist_prepare<NetB, ETB, MIXED>(b, pb, s, wb, qb, ob);
+  if (r != 0) return r;
+  constexpr long B = KX_PERSIST_BLOCK;
+  hipLaunchKernelGGL((persist_pair_kernel<NetA, ETA, NetB, ETB, MIXED>),
+                     dim3(lanes / B), dim3(B), 0, s, a, wa, qa,
+                     (const int*)oa, b, wb, qb, (const int*)ob);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

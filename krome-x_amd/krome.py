@@ -25,7 +25,8 @@ from pathlib import Path
 
 import numpy as np
 
-__all__ = ["load", "Network", "Options", "Result", "KromeError"]
+__all__ = ["load", "load_multi", "Network", "Multi", "Options", "Result",
+           "KromeError"]
 
 HERE = Path(__file__).resolve().parent
 _D = ctypes.POINTER(ctypes.c_double)
@@ -70,6 +71,72 @@ def load(network="primordial3", path=None):
     except ImportError:
         pass
     return Network(ctypes.CDLL(str(p)))
+
+
+class _MultiBatch(ctypes.Structure):
+    """krome_multi_batch (include/krome_multi.h)"""
+    _fields_ = [("network", ctypes.c_int), ("ncell", ctypes.c_int),
+                ("n", ctypes.c_void_p), ("Tgas", ctypes.c_void_p),
+                ("dt", ctypes.c_void_p), ("par", ctypes.c_void_p),
+                ("flags", ctypes.c_void_p), ("evolve_T", ctypes.c_int),
+                ("rtol", ctypes.c_double), ("atol_n", ctypes.c_double),
+                ("atol_T", ctypes.c_double), ("max_steps", ctypes.c_long)]
+
+
+def load_multi(path=None):
+    """the library of several networks in one persistent launch
+    (libkrome_multi.so, include/krome_multi.h)"""
+    p = Path(path) if path else lib_path("multi")
+    if not p.exists():
+        raise KromeError("%s is not built (python3 krome-x_amd/build.py)" % p)
+    try:
+        import torch  # noqa: F401  (the same HIP runtime as torch)
+    except ImportError:
+        pass
+    return Multi(ctypes.CDLL(str(p)))
+
+
+class Multi:
+    """several networks' cells in one persistent launch (this build's
+    extension, no counterpart in the reference): networks[i] names
+    network i of the library"""
+
+    def __init__(self, lib):
+        self._lib = lib
+        lib.krome_multi_nnet.restype = ctypes.c_int
+        lib.krome_multi_network_name.restype = ctypes.c_char_p
+        lib.krome_multi_network_name.argtypes = [ctypes.c_int]
+        lib.krome_multi_solve_device.restype = ctypes.c_int
+        lib.krome_multi_solve_device.argtypes = [
+            ctypes.POINTER(_MultiBatch), ctypes.c_int, ctypes.c_int, _D]
+        self.networks = [lib.krome_multi_network_name(i).decode()
+                         for i in range(lib.krome_multi_nnet())]
+
+    def solve_device(self, batches, mixed_precision=1):
+        """batches: dicts of network (name), n [nspec, ncell], Tgas, dt
+        [ncell], par [ncell, npar], flags (int32 [ncell]), all contiguous
+        float64 torch tensors on the GPU (updated in place), and evolve_T,
+        rtol, atol_n, atol_T, max_steps.  Returns (the C function's flag,
+        the launch's kernel ms)"""
+        arr = (_MultiBatch * len(batches))()
+        for k, b in enumerate(batches):
+            for key in ("n", "Tgas", "dt", "par", "flags"):
+                t = b[key]
+                if not t.is_cuda or not t.is_contiguous():
+                    raise KromeError("solve_device needs contiguous GPU "
+                                     "tensors (%s)" % key)
+            arr[k] = _MultiBatch(
+                self.networks.index(b["network"]), int(b["Tgas"].shape[0]),
+                b["n"].data_ptr(), b["Tgas"].data_ptr(), b["dt"].data_ptr(),
+                b["par"].data_ptr(), b["flags"].data_ptr(),
+                int(b.get("evolve_T", 0)), float(b.get("rtol", 1e-8)),
+                float(b.get("atol_n", 1e-20)), float(b.get("atol_T", 1e-6)),
+                int(b.get("max_steps", 100000)))
+        ms = ctypes.c_double()
+        r = self._lib.krome_multi_solve_device(
+            arr, len(batches), int(mixed_precision),
+            ctypes.cast(ctypes.byref(ms), _D))
+        return r, ms.value
 
 
 class Network:
