@@ -33,7 +33,7 @@ duration.
 primordial cells of the uneven batch (seed 2, density 1..1e12 cm^-3, dt
 1e8..1e14 s, Tgas fixed, rtol 1e-6) solved by one launch, against the
 same two batches solved by two launches of the networks' own libraries.
-FP64 Newton unless --mixed.
+Mixed precision unless --fp64, as for the solve workload.
 
 --workload copy is the calibration of the HBM traffic counters: one
 device-to-device copy of 2 GiB (far above the last-level cache), whose
@@ -228,9 +228,9 @@ def multi_bench(a):
     import krome
 
     dev = torch.device("cuda:0")
-    # FP64 unless --mixed: on the cold, dense, long-dt primordial cells of
-    # this batch the FP32 Newton matrix loses the trace ions
-    mixed = 1 if a.mixed else 0
+    # mixed precision unless --fp64 (it applies to primordial3's difference-
+    # quotient Jacobian; primordial's analytic one keeps the FP64 matrix)
+    mixed = 0 if a.fp64 else 1
     m = krome.load_multi()
     nets = {n: krome.load(n) for n in ("primordial", "primordial3")}
     n3, n1 = a.ncell // 4, a.ncell // 2
@@ -338,9 +338,6 @@ def main():
                     choices=("persistent", "plain"))
     ap.add_argument("--fp64", action="store_true",
                     help="the persistent kernel with the FP64 Newton")
-    ap.add_argument("--mixed", action="store_true",
-                    help="--workload multi: the mixed-precision Newton "
-                         "(default FP64)")
     ap.add_argument("--cpu-sample", type=int, default=1 << 18)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")

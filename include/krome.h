@@ -18,7 +18,9 @@
  *  - extra option keys select this build's kernels: device_kernel (0 the
  *    plain FP64 kernel, 1 the persistent batched kernel), mixed_precision
  *    (persistent kernel: 1 = FP32 Jacobian, LU and triangular solves with
- *    the FP64 Newton residual, state and error test), queue_chunk and
+ *    the FP64 Newton residual, state and error test, where the Jacobian is
+ *    by difference quotients: Tgas evolved or jac 2; the analytic-Jacobian
+ *    path keeps its FP64 matrix), queue_chunk and
  *    waves_per_cu (persistent kernel tuning; 0 = automatic).
  */
 #ifndef KROME_MI355X_H

@@ -139,10 +139,12 @@ int krome_multi_solve_device(const krome_multi_batch* b, int nbatch,
   (void)hipEventRecord(g_ev[0], s);
   int r;
   if (has_a && has_b) {
+    // primordial's Jacobian is analytic (Tgas fixed): its Newton matrix
+    // stays FP64, as in launch_persistent
     r = mixed_precision
-        ? kx::launch_persistent_pair_t<NetA, true, NetB, false, true>(
+        ? kx::launch_persistent_pair_t<NetA, true, true, NetB, false, false>(
               cells_args(*ba), cells_args(*bb), 0, s)
-        : kx::launch_persistent_pair_t<NetA, true, NetB, false, false>(
+        : kx::launch_persistent_pair_t<NetA, true, false, NetB, false, false>(
               cells_args(*ba), cells_args(*bb), 0, s);
   } else if (has_a) {
     r = kx::launch_persistent<NetA, true>(cells_args(*ba), mixed_precision,

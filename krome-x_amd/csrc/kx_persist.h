@@ -1752,16 +1752,16 @@ persist_cells_kernel(CellsArgs a, PersistWorkOf<Net, ET, MIXED> w0,
 // until that queue is exhausted, then the second's, so that the waves that
 // finish the first early fill the launch with the second instead of
 // leaving its tail to a launch of its own
-template <class NetA, bool ETA, class NetB, bool ETB, bool MIXED>
+template <class NetA, bool ETA, bool MA, class NetB, bool ETB, bool MB>
 __global__ void __launch_bounds__(KX_PERSIST_BLOCK, KX_PERSIST_WAVES_PER_EU)
-persist_pair_kernel(CellsArgs a, PersistWorkOf<NetA, ETA, MIXED> wa,
-                    int* qa, const int* oa, CellsArgs b,
-                    PersistWorkOf<NetB, ETB, MIXED> wb, int* qb,
+persist_pair_kernel(CellsArgs a, PersistWorkOf<NetA, ETA, MA> wa, int* qa,
+                    const int* oa, CellsArgs b,
+                    PersistWorkOf<NetB, ETB, MB> wb, int* qb,
                     const int* ob) {
   constexpr long LA = lds_doubles<NetA, ETA>(), LB = lds_doubles<NetB, ETB>();
   __shared__ double lds_vec[LA > LB ? LA : LB];
-  persist_run<NetA, ETA, MIXED>(a, wa, qa, oa, lds_vec);
-  persist_run<NetB, ETB, MIXED>(b, wb, qb, ob, lds_vec);
+  persist_run<NetA, ETA, MA>(a, wa, qa, oa, lds_vec);
+  persist_run<NetB, ETB, MB>(b, wb, qb, ob, lds_vec);
 }
 
 // ---- the queue's order: the densest cells first ---------------------------
@@ -1926,22 +1926,22 @@ int launch_persistent_t(CellsArgs a, int wpc, hipStream_t s) {
 
 // two networks' cells in one persistent launch (persist_pair_kernel): the
 // lanes are those of the larger batch; each network keeps its own
-// workspace and queue
-template <class NetA, bool ETA, class NetB, bool ETB, bool MIXED>
+// workspace and queue, and its own Newton-matrix precision
+template <class NetA, bool ETA, bool MA, class NetB, bool ETB, bool MB>
 int launch_persistent_pair_t(CellsArgs a, CellsArgs b, int wpc,
                              hipStream_t s) {
   long lanes = 0;
   if (persist_lanes(a.ncell > b.ncell ? a.ncell : b.ncell, wpc, lanes) != 0)
     return -1;
   static thread_local PersistBuffers pa, pb;
-  PersistWorkOf<NetA, ETA, MIXED> wa;
-  PersistWorkOf<NetB, ETB, MIXED> wb;
+  PersistWorkOf<NetA, ETA, MA> wa;
+  PersistWorkOf<NetB, ETB, MB> wb;
   int *qa = nullptr, *oa = nullptr, *qb = nullptr, *ob = nullptr;
-  int r = persist_prepare<NetA, ETA, MIXED>(a, pa, s, wa, qa, oa);
-  if (r == 0) r = persist_prepare<NetB, ETB, MIXED>(b, pb, s, wb, qb, ob);
+  int r = persist_prepare<NetA, ETA, MA>(a, pa, s, wa, qa, oa);
+  if (r == 0) r = persist_prepare<NetB, ETB, MB>(b, pb, s, wb, qb, ob);
   if (r != 0) return r;
   constexpr long B = KX_PERSIST_BLOCK;
-  hipLaunchKernelGGL((persist_pair_kernel<NetA, ETA, NetB, ETB, MIXED>),
+  hipLaunchKernelGGL((persist_pair_kernel<NetA, ETA, MA, NetB, ETB, MB>),
                      dim3(lanes / B), dim3(B), 0, s, a, wa, qa,
                      (const int*)oa, b, wb, qb, (const int*)ob);
   return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -1965,8 +1965,13 @@ inline int launch_persistent(CellsArgs a, int mixed, int chunk, int wpc,
     (void)s;
     return FLAG_UNSUPPORTED;
   } else {
-  return mixed ? launch_persistent_t<Net, ET, true>(a, wpc, s)
-               : launch_persistent_t<Net, ET, false>(a, wpc, s);
+  // the FP32 Newton matrix serves the difference-quotient Jacobian (Tgas
+  // evolved, or jac 2); with the analytic Jacobian (fixed Tgas) the matrix
+  // stays FP64: on cold, dense cells its FP32 form loses the trace ions and
+  // takes more steps than the FP64 one saves
+  const bool dq = a.o.jac == 2 || (a.o.jac == 0 && ET);
+  return mixed && dq ? launch_persistent_t<Net, ET, true>(a, wpc, s)
+                     : launch_persistent_t<Net, ET, false>(a, wpc, s);
   }
 }
 
