@@ -171,6 +171,34 @@ int krome_device_last_kernel_ms(double* ms);
  * krome::flag_name, runtime/core/solve_cell.hpp:106-137) */
 const char* krome_flag_name(int flag);
 
+/* ---- queries: rates, fluxes and the thermal balance of states ------------
+ * (krome.h:378-412, krome.cpp:106-133, 621-644, 753-770; query.hpp:37-108)
+ * n[ncell][nspec] and Tgas[ncell] on the host; par, par_stride as for
+ * krome_solve_cells (NULL: the defaults); Phi is ignored (these networks
+ * have no photon bins); h may be NULL.  Evaluated as the solver sees them:
+ * the rates at min(max(Tgas, Tcmb), 1e9), the thermal terms at
+ * min(max(Tgas, 2.73), 1e9).  0, -1 on bad arguments, or -105 when a cell's
+ * REQUIRED parameter is unset (that cell's outputs untouched).  Computed
+ * on the device. */
+int krome_get_rates(int ncell, const double* n, const double* Tgas,
+                    const double* par, int par_stride, const double* Phi,
+                    const krome_opts* h, double* k);
+int krome_get_fluxes(int ncell, const double* n, const double* Tgas,
+                     const double* par, int par_stride, const double* Phi,
+                     const krome_opts* h, double* fluxes);
+/* the thermal balance [erg cm^-3 s^-1]: totals heat[ncell], cool[ncell],
+ * the active terms heats[ncell][nheat], cools[ncell][ncool] (named by
+ * krome_heating_name / krome_cooling_name, the reference's order) and
+ * dTdt[ncell] [K/s]; any output may be NULL */
+int krome_nheat(void);
+int krome_ncool(void);
+const char* krome_heating_name(int i);
+const char* krome_cooling_name(int i);
+int krome_get_thermal(int ncell, const double* n, const double* Tgas,
+                      const double* par, int par_stride, const double* Phi,
+                      const krome_opts* h, double* heat, double* cool,
+                      double* heats, double* cools, double* dTdt);
+
 /* ---- element totals and consistent advection --------------------------- */
 
 /* species and element metadata (krome.h:642-650, krome.cpp:1182-1196):

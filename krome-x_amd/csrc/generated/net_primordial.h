@@ -23,6 +23,8 @@ struct net_primordial {
   static constexpr bool HAS_E = true;
   static constexpr int I_E = 11;
   static constexpr bool THERMAL = false;
+  static constexpr int NHEAT = 0;
+  static constexpr int NCOOL = 0;
   static constexpr bool RATES_DENSITY = false;
   // parameter row (param_names order)
   struct Params {
@@ -334,6 +336,16 @@ struct net_primordial {
   static constexpr int LT_JT_MULT[74] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 3, 3, 2, 1, 2, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
   static constexpr int LT_JT_FOFF[75] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 31, 32, 33, 34, 35, 36, 37, 38, 39, 40, 41, 42, 43, 44, 45, 46, 47, 48, 49, 50, 52, 54, 56, 58, 60, 62, 63, 64, 65, 66, 67, 68, 69, 70, 71, 72, 73, 74, 75, 76, 77, 78, 79, 80};
   static constexpr int LT_JF_SP[80] = {11, 2, 11, 3, 11, 3, 11, 8, 11, 9, 11, 9, 11, 9, 11, 10, 11, 2, 4, 2, 4, 2, 3, 2, 3, 2, 6, 2, 5, 3, 11, 5, 5, 2, 11, 4, 4, 2, 4, 2, 4, 3, 4, 3, 11, 6, 11, 6, 6, 4, 2, 2, 2, 2, 5, 2, 2, 2, 5, 2, 2, 2, 3, 0, 2, 1, 5, 1, 7, 3, 5, 0, 5, 0, 7, 2, 4, 0, 11, 1};
+  // the reaction fluxes k prod(n) [cm^-3 s^-1] (krome_get_fluxes)
+  template <class R>
+  KX_HD static void fluxes(const R* n, const R* k, R* fl) {
+    for (int r = 0; r < NREA; ++r) {
+      R f = k[r];
+      for (int p = LT_KF_OFF[r]; p < LT_KF_OFF[r + 1]; ++p)
+        f = f * n[LT_KF_SP[p]];
+      fl[r] = f;
+    }
+  }
   // dn/dt = sum over reactions of S k prod(n), in reaction order
   template <class R>
   KX_HD static void rhs_loop(const R* n, const R* k, R* dn) {

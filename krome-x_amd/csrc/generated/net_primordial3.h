@@ -20,6 +20,8 @@ struct net_primordial3 {
   static constexpr bool HAS_E = true;
   static constexpr int I_E = 8;
   static constexpr bool THERMAL = true;
+  static constexpr int NHEAT = 2;
+  static constexpr int NCOOL = 3;
   static constexpr bool RATES_DENSITY = true;
   // parameter row (param_names order)
   struct Params {
@@ -289,6 +291,16 @@ struct net_primordial3 {
   static constexpr int LT_JT_MULT[52] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 3, 2, 1, 2};
   static constexpr int LT_JT_FOFF[53] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 31, 32, 33, 34, 35, 36, 37, 38, 39, 40, 41, 42, 43, 44, 45, 46, 47, 48, 50, 52, 54, 55};
   static constexpr int LT_JF_SP[55] = {8, 0, 8, 1, 8, 1, 8, 5, 8, 6, 8, 6, 8, 6, 8, 7, 8, 0, 2, 0, 1, 0, 1, 0, 4, 0, 3, 1, 8, 3, 8, 3, 3, 0, 8, 2, 2, 0, 2, 0, 2, 1, 2, 1, 8, 4, 4, 2, 0, 0, 3, 0, 0, 0, 3};
+  // the reaction fluxes k prod(n) [cm^-3 s^-1] (krome_get_fluxes)
+  template <class R>
+  KX_HD static void fluxes(const R* n, const R* k, R* fl) {
+    for (int r = 0; r < NREA; ++r) {
+      R f = k[r];
+      for (int p = LT_KF_OFF[r]; p < LT_KF_OFF[r + 1]; ++p)
+        f = f * n[LT_KF_SP[p]];
+      fl[r] = f;
+    }
+  }
   // dn/dt = sum over reactions of S k prod(n), in reaction order
   template <class R>
   KX_HD static void rhs_loop(const R* n, const R* k, R* dn) {
@@ -795,6 +807,18 @@ struct net_primordial3 {
     cool += kx::cooling_continuum(Tth, rho(n), ntot(n));
     return (gamma_index(Tth, n, p) - R(1)) * (heat - cool)
         / (R(1.380649e-16) * ntot(n));
+  }
+  // the active thermal terms one by one, in the order of the names
+  // (heat: CHEM, COMPRESS; cool: H2, CHEM, CONT), the same expressions as thermal()
+  template <class R>
+  KX_HD static void thermal_terms(R Tth, const R* n, const R* k,
+                                  const Params& p, R* heats, R* cools) {
+    heats[0] = heating_chem(Tth, n, k);
+    heats[1] = ntot(n) * R(1.380649e-16) * Tth / R(p.tff);
+    cools[0] = kx::cooling_H2(Tth, n[0], n[1], n[3], n[8], n[5]) * kx::h2_opacity_ripamonti(ntot(n));
+    cools[1] = cooling_chem(n, k);
+    cools[2] = kx::cooling_continuum(Tth, rho(n), ntot(n));
+    (void)Tth; (void)n; (void)k; (void)p; (void)heats; (void)cools;
   }
 };
 }  // namespace kx

@@ -17,6 +17,7 @@
 #include "kx_persist.h"
 #include "kx_conserve.h"
 #include "kx_massfrac.h"
+#include "kx_query.h"
 #include "../../include/krome.h"
 
 using Net = kx::KX_NET;
@@ -863,6 +864,80 @@ int conserve_resident(int ncell, double* v, long cs, long ss,
   return f;
 }
 
+// the query functions on host rows (krome.cpp:106-133, 621-644, 753-770):
+// rates, fluxes or the thermal balance of the states n[ncell][nspec] at
+// Tgas, evaluated on the device; a cell with an unset REQUIRED parameter
+// is skipped (its outputs untouched) and its flag returned
+int query_host(int ncell, const double* n, const double* Tgas,
+               const double* par, int par_stride, int what, double* out,
+               double* heat, double* cool, double* heats, double* cools,
+               double* dTdt) {
+  if (ncell < 0 || !n || !Tgas) return -1;
+  if (par && par_stride != 0 && par_stride != Net::NPAR) return -1;
+  if (ncell == 0) return 0;
+  std::vector<double> dflt(Net::NPAR);
+  krome_params_default(dflt.data());
+  const double* P = par ? par : dflt.data();
+  const int pst = par ? par_stride : 0;
+  const int per = what == kx::Q_THERMAL ? 3 : Net::NREA;
+  const int nh = Net::NHEAT > 0 ? Net::NHEAT : 1;
+  const int nc = Net::NCOOL > 0 ? Net::NCOOL : 1;
+  std::lock_guard<std::mutex> lk(g_cache_mutex);
+  DevCache& dc = g_cache;
+  const size_t nb = sizeof(double) * (size_t)ncell * NS;
+  const size_t pb = sizeof(double) * (size_t)(pst ? ncell : 1) * Net::NPAR;
+  const size_t ob = sizeof(double) * (size_t)ncell * (per + nh + nc);
+  if (!dc.n.ensure(nb) || !dc.T.ensure(sizeof(double) * ncell) ||
+      !dc.par.ensure(pb) || !dc.work.ensure(ob) ||
+      !dc.flags.ensure(sizeof(int) * ncell))
+    return kx::FLAG_DEVICE_MEMORY;
+  if (!hip_ok(hipMemcpy(dc.n.p, n, nb, hipMemcpyHostToDevice), "H2D") ||
+      !hip_ok(hipMemcpy(dc.T.p, Tgas, sizeof(double) * ncell,
+                        hipMemcpyHostToDevice), "H2D") ||
+      !hip_ok(hipMemcpy(dc.par.p, P, pb, hipMemcpyHostToDevice), "H2D"))
+    return -1;
+  double* o = static_cast<double*>(dc.work.p);
+  double* hs = o + (size_t)ncell * per;
+  double* cs = hs + (size_t)ncell * nh;
+  hipLaunchKernelGGL(kx::query_kernel<Net>, dim3((ncell + 127) / 128),
+                     dim3(128), 0, nullptr, ncell,
+                     static_cast<const double*>(dc.n.p),
+                     static_cast<const double*>(dc.T.p),
+                     static_cast<const double*>(dc.par.p), pst, what, o, hs,
+                     cs, static_cast<int*>(dc.flags.p));
+  if (!hip_ok(hipGetLastError(), "query kernel")) return -1;
+  std::vector<double> ho((size_t)ncell * (per + nh + nc));
+  std::vector<int> fl(ncell);
+  if (!hip_ok(hipMemcpy(ho.data(), o, ob, hipMemcpyDeviceToHost), "D2H") ||
+      !hip_ok(hipMemcpy(fl.data(), dc.flags.p, sizeof(int) * ncell,
+                        hipMemcpyDeviceToHost), "D2H"))
+    return -1;
+  const double* hh = ho.data() + (size_t)ncell * per;
+  const double* hc = hh + (size_t)ncell * nh;
+  int first = 0;
+  for (int c = 0; c < ncell; ++c) {
+    if (fl[c] != 0) {
+      if (first == 0) first = fl[c];
+      continue;
+    }
+    const double* oc = ho.data() + (size_t)c * per;
+    if (what != kx::Q_THERMAL) {
+      for (int r = 0; r < Net::NREA; ++r) out[(size_t)c * Net::NREA + r] = oc[r];
+      continue;
+    }
+    if (heat) heat[c] = oc[0];
+    if (cool) cool[c] = oc[1];
+    if (dTdt) dTdt[c] = oc[2];
+    if (heats)
+      for (int i = 0; i < Net::NHEAT; ++i)
+        heats[(size_t)c * Net::NHEAT + i] = hh[(size_t)c * Net::NHEAT + i];
+    if (cools)
+      for (int i = 0; i < Net::NCOOL; ++i)
+        cools[(size_t)c * Net::NCOOL + i] = hc[(size_t)c * Net::NCOOL + i];
+  }
+  return first;
+}
+
 // krome_conserve_cells(_x) on host rows (krome.cpp:943-977): the same
 // arguments and errors; the rows corrected on the device
 int conserve_host(int ncell, double* v, const double* ref, int ref_stride,
@@ -926,6 +1001,42 @@ int conserve_device(int ncell, double* v, int layout, long ld,
 }  // namespace
 
 extern "C" {
+
+int krome_get_rates(int ncell, const double* n, const double* Tgas,
+                    const double* par, int par_stride, const double* Phi,
+                    const krome_opts* h, double* k) {
+  (void)Phi;  // no photon bins in these networks
+  (void)h;
+  if (!k) return -1;
+  return query_host(ncell, n, Tgas, par, par_stride, kx::Q_RATES, k, nullptr,
+                    nullptr, nullptr, nullptr, nullptr);
+}
+int krome_get_fluxes(int ncell, const double* n, const double* Tgas,
+                     const double* par, int par_stride, const double* Phi,
+                     const krome_opts* h, double* fluxes) {
+  (void)Phi;
+  (void)h;
+  if (!fluxes) return -1;
+  return query_host(ncell, n, Tgas, par, par_stride, kx::Q_FLUXES, fluxes,
+                    nullptr, nullptr, nullptr, nullptr, nullptr);
+}
+int krome_nheat(void) { return Net::NHEAT; }
+int krome_ncool(void) { return Net::NCOOL; }
+const char* krome_heating_name(int i) {
+  return (i >= 0 && i < Net::NHEAT) ? kHeatNames[i] : nullptr;
+}
+const char* krome_cooling_name(int i) {
+  return (i >= 0 && i < Net::NCOOL) ? kCoolNames[i] : nullptr;
+}
+int krome_get_thermal(int ncell, const double* n, const double* Tgas,
+                      const double* par, int par_stride, const double* Phi,
+                      const krome_opts* h, double* heat, double* cool,
+                      double* heats, double* cools, double* dTdt) {
+  (void)Phi;
+  (void)h;
+  return query_host(ncell, n, Tgas, par, par_stride, kx::Q_THERMAL, nullptr,
+                    heat, cool, heats, cools, dTdt);
+}
 
 int krome_get_element_totals(int ncell, const double* n, double* tot) {
   return totals_host(ncell, n, tot, false);

@@ -189,6 +189,19 @@ class Network:
                   ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                   ctypes.c_void_p]),
                 ("krome_device_last_kernel_ms", ctypes.c_int, [_D]),
+                ("krome_get_rates", ctypes.c_int,
+                 [ctypes.c_int, _D, _D, _D, ctypes.c_int, _D,
+                  ctypes.c_void_p, _D]),
+                ("krome_get_fluxes", ctypes.c_int,
+                 [ctypes.c_int, _D, _D, _D, ctypes.c_int, _D,
+                  ctypes.c_void_p, _D]),
+                ("krome_nheat", ctypes.c_int, []),
+                ("krome_ncool", ctypes.c_int, []),
+                ("krome_heating_name", ctypes.c_char_p, [ctypes.c_int]),
+                ("krome_cooling_name", ctypes.c_char_p, [ctypes.c_int]),
+                ("krome_get_thermal", ctypes.c_int,
+                 [ctypes.c_int, _D, _D, _D, ctypes.c_int, _D,
+                  ctypes.c_void_p, _D, _D, _D, _D, _D]),
                 ("krome_species_mass", ctypes.c_double, [ctypes.c_int]),
                 ("krome_species_charge", ctypes.c_int, [ctypes.c_int]),
                 ("krome_nelem", ctypes.c_int, []),
@@ -235,6 +248,10 @@ class Network:
                                  for i in range(self.nspec)])
         self.elements = tuple(L.krome_element_name(e).decode()
                               for e in range(L.krome_nelem()))
+        self.heating_terms = tuple(L.krome_heating_name(i).decode()
+                                   for i in range(L.krome_nheat()))
+        self.cooling_terms = tuple(L.krome_cooling_name(i).decode()
+                                   for i in range(L.krome_ncool()))
         self.composition = np.array(
             [[L.krome_species_natoms(i, e)
               for e in range(len(self.elements))]
@@ -360,6 +377,50 @@ class Network:
             dt.data_ptr(), par.data_ptr(), self.npar,
             None, flags.data_ptr() if flags is not None else None, None,
             opts._h)
+
+    # -- queries (the reference binding's rates / fluxes / thermal) --------
+
+    def rates(self, n, Tgas, par=None):
+        """rate coefficients [..., nrea] at the states (krome_get_rates)"""
+        return self._query(n, Tgas, par, "krome_get_rates")
+
+    def fluxes(self, n, Tgas, par=None):
+        """reaction fluxes k prod(n) [cm^-3 s^-1], [..., nrea]"""
+        return self._query(n, Tgas, par, "krome_get_fluxes")
+
+    def _query(self, n, Tgas, par, fname):
+        single, n, ncell = self._cells(n)
+        T = self._per_cell(Tgas, ncell)
+        P, stride = self._params(par, ncell)
+        out = np.zeros((ncell, self.nrea))
+        r = getattr(self._lib, fname)(ncell, _dptr(n), _dptr(T), _dptr(P),
+                                      stride, None, None, _dptr(out))
+        if r != 0:
+            raise KromeError("%s returned %d: %s" % (fname, r,
+                                                     self.flag_message(r)))
+        return out[0] if single else out
+
+    def thermal(self, n, Tgas, par=None):
+        """the thermal balance: dict of heat, cool [erg cm^-3 s^-1], the
+        terms heats [..., nheat] and cools [..., ncool] (labels:
+        heating_terms, cooling_terms) and dTdt [K/s]"""
+        single, n, ncell = self._cells(n)
+        T = self._per_cell(Tgas, ncell)
+        P, stride = self._params(par, ncell)
+        heat, cool, dTdt = (np.zeros(ncell) for _ in range(3))
+        hs = np.zeros((ncell, max(len(self.heating_terms), 1)))
+        cs = np.zeros((ncell, max(len(self.cooling_terms), 1)))
+        r = self._lib.krome_get_thermal(ncell, _dptr(n), _dptr(T), _dptr(P),
+                                        stride, None, None, _dptr(heat),
+                                        _dptr(cool), _dptr(hs), _dptr(cs),
+                                        _dptr(dTdt))
+        if r != 0:
+            raise KromeError("krome_get_thermal returned %d: %s"
+                             % (r, self.flag_message(r)))
+        out = {"heat": heat, "cool": cool, "dTdt": dTdt,
+               "heats": hs[:, :len(self.heating_terms)],
+               "cools": cs[:, :len(self.cooling_terms)]}
+        return {k: v[0] for k, v in out.items()} if single else out
 
     # -- element totals and consistent advection (the reference binding's
     #    krome.py:850-921) ------------------------------------------------

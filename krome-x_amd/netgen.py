@@ -725,6 +725,9 @@ def generate(knet_path, thermal=None, name=None):
     thermal_on = bool(th.cooling or th.heating)
     w("  static constexpr bool THERMAL = %s;"
       % ("true" if thermal_on else "false"))
+    hnames, cnames = thermal_term_names(th)
+    w("  static constexpr int NHEAT = %d;" % len(hnames))
+    w("  static constexpr int NCOOL = %d;" % len(cnames))
     # n-dependent rates -> no analytic Jacobian
     grp = groups(net)
     dens_groups = [g for g in grp
@@ -909,6 +912,16 @@ def generate(knet_path, thermal=None, name=None):
         vals = lt[nm] or [0]
         w("  static constexpr int LT_%s[%d] = {%s};"
           % (nm.upper(), len(vals), ", ".join(str(v) for v in vals)))
+    w("  // the reaction fluxes k prod(n) [cm^-3 s^-1] (krome_get_fluxes)")
+    w("  template <class R>")
+    w("  KX_HD static void fluxes(const R* n, const R* k, R* fl) {")
+    w("    for (int r = 0; r < NREA; ++r) {")
+    w("      R f = k[r];")
+    w("      for (int p = LT_KF_OFF[r]; p < LT_KF_OFF[r + 1]; ++p)")
+    w("        f = f * n[LT_KF_SP[p]];")
+    w("      fl[r] = f;")
+    w("    }")
+    w("  }")
     w("  // dn/dt = sum over reactions of S k prod(n), in reaction order")
     w("  template <class R>")
     w("  KX_HD static void rhs_loop(const R* n, const R* k, R* dn) {")
@@ -1011,6 +1024,7 @@ def generate(knet_path, thermal=None, name=None):
     meta = {"name": ns, "species": [s.name for s in net.species],
             "charge": charges, "mass_g": masses, "params": pnames,
             "elements": elems,
+            "heating": hnames, "cooling": cnames,
             "sparse": {k: sp[k] for k in ("perm", "rp", "ci", "di",
                                           "flops")},
             "natoms": [[sp.comp.get(e, 0) for e in elems]
@@ -1127,6 +1141,52 @@ def _emit_thermal(w, net, th, pnames):
     w("    return (gamma_index(Tth, n, p) - R(1)) * (heat - cool)")
     w("        / (R(%r) * ntot(n));" % BOLTZMANN_ERG)
     w("  }")
+    # the same terms one by one, for the thermal query (krome_get_thermal)
+    hs, cs = thermal_term_names(th)
+    w("  // the active thermal terms one by one, in the order of the names")
+    w("  // (heat: %s; cool: %s), the same expressions as thermal()"
+      % (", ".join(hs), ", ".join(cs)))
+    w("  template <class R>")
+    w("  KX_HD static void thermal_terms(R Tth, const R* n, const R* k,")
+    w("                                  const Params& p, R* heats, R* cools) {")
+    for i, t in enumerate(hs):
+        if t == "CHEM":
+            w("    heats[%d] = heating_chem(Tth, n, k);" % i)
+        elif t == "COMPRESS":
+            w("    heats[%d] = ntot(n) * R(%r) * Tth / R(p.tff);"
+              % (i, BOLTZMANN_ERG))
+    for i, t in enumerate(cs):
+        if t == "H2":
+            args = ", ".join("n[%d]" % idx(x) if x in names else "R(0)"
+                             for x in ("H", "H+", "H2", "e-", "He"))
+            op = ("kx::h2_opacity_ripamonti(ntot(n))"
+                  if th.h2_opacity == "RIPAMONTI" else "R(1)")
+            w("    cools[%d] = kx::cooling_H2(Tth, %s) * %s;" % (i, args, op))
+        elif t == "CHEM":
+            w("    cools[%d] = cooling_chem(n, k);" % i)
+        elif t == "CONT":
+            w("    cools[%d] = kx::cooling_continuum(Tth, rho(n), ntot(n));"
+              % i)
+    w("    (void)Tth; (void)n; (void)k; (void)p; (void)heats; (void)cools;")
+    w("  }")
+
+
+# the reference's fixed lists of thermal terms (solve_cell.hpp:206-231):
+# a build reports its active ones in this order
+HEAT_TERM_ORDER = ("CUSTOM", "CR", "CHEM", "DH", "COMPRESS", "VISCOUS",
+                   "PHOTODUST", "PHOTODUSTNET", "PHOTOAV", "PHOTO", "ZCIE",
+                   "NUCLEAR")
+COOL_TERM_ORDER = ("CUSTOM", "ATOMIC", "H2", "HD", "MOL", "Z", "Z_CIE",
+                   "Z_EXTENDED", "FLOOR", "CHEM", "COMPTON", "CIE",
+                   "EXPANSION", "CONT", "FF", "DH", "DUST", "DUSTTAB",
+                   "DUSTBINS")
+
+
+def thermal_term_names(th):
+    """(heating names, cooling names) of a thermal configuration's active
+    terms, in the reference's order"""
+    return ([t for t in HEAT_TERM_ORDER if t in th.heating],
+            [t for t in COOL_TERM_ORDER if t in th.cooling])
 
 
 def meta_header(meta, th):
@@ -1147,6 +1207,10 @@ def meta_header(meta, th):
          "const char* const kParams[] = {%s};" % cstr(meta["params"]),
          "const char* const kElements[] = {%s};" % cstr(
              meta["elements"] or [""]),
+         "const char* const kHeatNames[] = {%s};" % cstr(
+             meta["heating"] or [""]),
+         "const char* const kCoolNames[] = {%s};" % cstr(
+             meta["cooling"] or [""]),
          "const double kParamDefaults[] = {%s};" % ", ".join(
              "NAN" if (isinstance(d, float) and math.isnan(d)) else repr(float(d))
              for d in meta["param_defaults"]),
