@@ -97,8 +97,44 @@ int krome_device_initialize(void);
 int krome_device_finalize(void);
 int krome_device_initialize_id(int device_id);
 
-/* the optional per-cell arrays of a multi-cell call (krome.h:275-279);
- * unused by this build's networks (no photon or dust bins) */
+/* photon bins (krome.h:203-228, krome.cpp:313-377): nbins bins with edges
+ * 0 < Elo[b] < Ehi[b] [eV] (-1 otherwise); the bin-averaged cross sections
+ * of the photo reactions are computed once, on the host, when the bins or
+ * the in-bin spectrum (alpha: dN/dE ~ E^-alpha, default 1) are set.  A call
+ * with bins set needs the cells' fields Phi[ncell][nbins] [photons cm^-2
+ * s^-1] (krome_cells.Phi, or the queries' Phi), else -103.  Doppler
+ * broadening and a dust opacity of the bins are not built: only their
+ * "none" (0, 0 and NULL) is accepted */
+int krome_opts_set_photo(krome_opts* h, int nbins, const double* Elo,
+                         const double* Ehi);
+int krome_opts_nbins(const krome_opts* h);
+int krome_opts_set_photo_alpha(krome_opts* h, double alpha);
+int krome_opts_set_photo_broadening(krome_opts* h, double Tgas,
+                                    double vturb);
+int krome_opts_set_photo_dust_opacity(krome_opts* h, const double* kappa);
+
+/* photo reactions (krome.h:598-617, krome.cpp:1121-1131): count, reaction
+ * and reactant index (-1 out of range), photons per event (1), the cross
+ * section sigma_r(E) [cm^2] (0 outside its window, NaN for a bad r), and
+ * the bin-averaged cross sections sigma[nphoto][nbins] of h's bins (-1
+ * without bins) */
+int krome_nphoto(void);
+int krome_photo_reaction(int r);
+int krome_photo_reactant(int r);
+double krome_photo_pdiss(int r);
+double krome_photo_xsec(int r, double E);
+int krome_opts_photo_sigma(const krome_opts* h, double* sigma);
+/* nbins linear / logarithmic bins in [lower, upper] eV, lower > 0
+ * (krome.h:547-551) */
+int krome_bins_lin(int nbins, double lower, double upper, double* Elo,
+                   double* Ehi);
+int krome_bins_log(int nbins, double lower, double upper, double* Elo,
+                   double* Ehi);
+
+/* the optional per-cell arrays of a multi-cell call (krome.h:275-279):
+ * Phi[ncell][nbins] the photon fields when bins are set (host arrays for
+ * krome_solve_cells(_x), device arrays for the _device calls); no dust
+ * bins here (Td, adust unused) */
 typedef struct {
   double* Phi;
   double* Td;
@@ -154,7 +190,8 @@ int krome_solve_cells_x_device(int ncell, double* x, const double* rho,
  * rhs_evals_jac, lin_setups, newton_iters, newton_fails, err_fails,
  * restarts) into the device array stats[ncell][9] (may be NULL), for
  * benchmarks and tests.  par is required; the call is asynchronous on the
- * null stream (the caller synchronizes). */
+ * null stream (the caller synchronizes).  It takes no photon field: -103
+ * when h has photon bins (the photo rates are 0 without bins). */
 int krome_solve_cells_device_stats(int ncell, double* n, int layout, long ld,
                                    double* Tgas, const double* dt,
                                    const double* par, int par_stride,
@@ -174,8 +211,8 @@ const char* krome_flag_name(int flag);
 /* ---- queries: rates, fluxes and the thermal balance of states ------------
  * (krome.h:378-412, krome.cpp:106-133, 621-644, 753-770; query.hpp:37-108)
  * n[ncell][nspec] and Tgas[ncell] on the host; par, par_stride as for
- * krome_solve_cells (NULL: the defaults); Phi is ignored (these networks
- * have no photon bins); h may be NULL.  Evaluated as the solver sees them:
+ * krome_solve_cells (NULL: the defaults); Phi[ncell][nbins] the fields of
+ * the cells when h has photon bins; h may be NULL (no bins).  Evaluated as the solver sees them:
  * the rates at min(max(Tgas, Tcmb), 1e9), the thermal terms at
  * min(max(Tgas, 2.73), 1e9).  0, -1 on bad arguments, or -105 when a cell's
  * REQUIRED parameter is unset (that cell's outputs untouched).  Computed

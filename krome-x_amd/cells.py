@@ -23,6 +23,16 @@ Workloads (the species order of the generated networks):
                 log-uniform 1e8..1e14 s (step counts spread over more than
                 100x); Tgas 10..1e4 K, x 1e-6..1e-2, y 1e-8..1e-3 as
                 primordial; fixed Tgas
+  galaxy_ism    2^18 cells, seed 4: nH in 0.1..1e5 cm^-3, Tgas 10..1e4 K,
+                x in 1e-4..1e-1, y in 1e-6..1e-1, He = 0.0775 nH with
+                He+ = 1e-3 He, the other species 1e-20 nH (e- balancing
+                the charge); crate 1e-17..1e-15 s^-1; dt = 3.15e11 s;
+                Tgas evolved.  Photon bins: 32 logarithmic bins over
+                0.7..200 eV (PHOTO_BINS); the field of a cell is
+                f Phi0(E), f log-uniform in 1e-2..1e2 (also the cell's G0),
+                Phi0 = 2e7 photons cm^-2 s^-1 per bin below 13.6 eV (an
+                interstellar FUV field) and 1e4 (E/13.6)^-1.5 above it (a
+                weak ionizing background)
 """
 from __future__ import annotations
 
@@ -134,5 +144,56 @@ def c3_primordial_cells(ncell=1 << 19, seed=2, first=0):
                      "max_steps": 100000}}
 
 
+def photo_bins(nbins=32, emin=0.7, emax=200.0):
+    """(Elo, Ehi) [eV] of nbins logarithmic bins over [emin, emax]"""
+    e = emin * (emax / emin) ** (np.arange(nbins + 1) / nbins)
+    return e[:-1].copy(), e[1:].copy()
+
+
+PHOTO_BINS = photo_bins()
+
+
+def galaxy_ism_cells(ncell=1 << 18, seed=4, first=0):
+    """the galaxy_ism photochemistry workload, with Tgas evolved: the
+    cells, their parameter rows and their photon fields Phi[ncell][32]"""
+    meta = network_meta("galaxy_ism")
+    sp = meta["species"]
+    col = {s: i for i, s in enumerate(sp)}
+    c = np.arange(first, first + ncell)
+    nH = log_uniform(seed, c, 0, 0.1, 1e5)
+    T = log_uniform(seed, c, 1, 10.0, 1e4)
+    x = log_uniform(seed, c, 2, 1e-4, 1e-1)
+    y = log_uniform(seed, c, 3, 1e-6, 1e-1)
+    f = log_uniform(seed, c, 4, 1e-2, 1e2)
+    crate = log_uniform(seed, c, 5, 1e-17, 1e-15)
+    n = np.empty((ncell, len(sp)))
+    n[:] = 1e-20 * nH[:, None]
+    he = 0.0775 * nH
+    n[:, col["H+"]] = x * nH
+    n[:, col["H2"]] = y * nH
+    n[:, col["He"]] = he * (1.0 - 1e-3)
+    n[:, col["He+"]] = he * 1e-3
+    n[:, col["H"]] = nH * np.maximum(1.0 - x - 2.0 * y, 1e-20)
+    q = np.array([meta["charge"][i] for i in range(len(sp))], dtype=float)
+    q[col["e-"]] = 0.0
+    n[:, col["e-"]] = n @ q
+    dt = np.full(ncell, 3.15e11)
+    pn = meta["params"]
+    par = np.tile(np.array(meta["param_defaults"], dtype=np.float64),
+                  (ncell, 1))
+    par[:, pn.index("crate")] = crate
+    par[:, pn.index("G0")] = f
+    elo, ehi = PHOTO_BINS
+    em = np.sqrt(elo * ehi)
+    phi0 = np.where(ehi <= 13.6, 2e7, 1e4 * (em / 13.6) ** -1.5)
+    Phi = f[:, None] * phi0[None, :]
+    return {"n": n, "T": T, "dt": dt, "par": par, "Phi": Phi,
+            "bins": (elo, ehi),
+            "opts": {"evolve_T": 1, "rtol": 1e-8, "atol_n": 1e-20,
+                     "atol_T": 1e-6, "linsol": 1, "jac": 0,
+                     "max_steps": 100000}}
+
+
 WORKLOADS = {"primordial": primordial_cells, "primordial3": primordial3_cells,
-             "c3_primordial": c3_primordial_cells}
+             "c3_primordial": c3_primordial_cells,
+             "galaxy_ism": galaxy_ism_cells}

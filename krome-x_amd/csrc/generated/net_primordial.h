@@ -26,7 +26,12 @@ struct net_primordial {
   static constexpr int NHEAT = 0;
   static constexpr int NCOOL = 0;
   static constexpr bool RATES_DENSITY = false;
-  // parameter row (param_names order)
+  // photo reactions (photo(xsec=...)): their rates are the bin sums
+  // sum_b sigma_rb Phi_b of the cell's field, which the kernels read
+  // after the parameters, in the row's last NPHOTO slots
+  static constexpr int NPHOTO = 0;
+  static constexpr int NROW = NPAR + NPHOTO;
+  // parameter row (param_names order), then the photo sums
   struct Params {
     double gamma_ad;
     double Tcmb;
@@ -36,6 +41,17 @@ struct net_primordial {
     p.gamma_ad = row[0];
     p.Tcmb = row[1];
     return p;
+  }
+  static constexpr int photo_reaction[1] = {0};
+  static constexpr int photo_reactant[1] = {0};
+  static constexpr double photo_Emin[1] = {0.0};
+  static constexpr double photo_Emax[1] = {0.0};
+  // cross section [cm^2] of photo reaction q at photon energy E_eV
+  static double photo_xsec(int q, double E_eV) {
+    using R = double;
+    switch (q) {
+      default: return R(0) * E_eV;
+    }
   }
   // index of the first REQUIRED parameter still NaN, or -1
   KX_HD static int params_missing(const double* row) {

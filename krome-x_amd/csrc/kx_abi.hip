@@ -18,6 +18,7 @@
 #include "kx_conserve.h"
 #include "kx_massfrac.h"
 #include "kx_query.h"
+#include "kx_photo.h"
 #include "../../include/krome.h"
 
 using Net = kx::KX_NET;
@@ -52,6 +53,11 @@ struct Opts {
   // persistent kernel: cells per queue grab, workgroups per CU
   int queue_chunk = 0;
   int waves_per_cu = 0;
+  // photon bins (krome_opts_set_photo): edges [eV], the in-bin spectrum
+  // exponent and the bin-averaged cross sections sigma[NPHOTO][nbins]
+  int nbins = 0;
+  double photo_alpha = 1.0;
+  std::vector<double> Elo, Ehi, sigma;
 };
 
 struct Flag {
@@ -133,9 +139,10 @@ struct Buf {
 struct DevCache {
   Buf work, iwork, n, T, dt, par, flags, stats, queue, count, cons;
   Buf xn, xin, rho;  // the mass-fraction entry points' densities
+  Buf prow, sig, phi;  // photo: kernel rows, cross sections, a host field
   void release() {
     for (Buf* b : {&work, &iwork, &n, &T, &dt, &par, &flags, &stats, &queue,
-                   &count, &cons, &xn, &xin, &rho})
+                   &count, &cons, &xn, &xin, &rho, &prow, &sig, &phi})
       b->release();
   }
 };
@@ -151,6 +158,39 @@ size_t plain_cell_bytes(bool et) {
                   sizeof(int) * neq<true>()
             : sizeof(double) * kx::cell_doubles<kx::Problem<Net, false>>() +
                   sizeof(int) * neq<false>();
+}
+
+// the parameter rows the kernels read, Net::NROW wide: for a network with
+// photo reactions the caller's rows P (device, stride pst, 0: one row)
+// followed by each cell's photo bin sums over its field Phi (device
+// [ncell][nbins]; null: no field, the sums 0), built on the device and
+// returned in P, pst; for the others the caller's rows unchanged
+int kernel_rows(int ncell, const double*& P, int& pst, const double* Phi,
+                const Opts& o, DevCache& dc, hipStream_t s) {
+  if constexpr (Net::NPHOTO == 0) {
+    (void)ncell; (void)P; (void)pst; (void)Phi; (void)o; (void)dc; (void)s;
+    return 0;
+  } else {
+    const int B = (Phi && o.nbins > 0) ? o.nbins : 0;
+    const int rows = (B > 0 || pst != 0) ? ncell : 1;
+    if (!dc.prow.ensure(sizeof(double) * (size_t)rows * Net::NROW))
+      return kx::FLAG_DEVICE_MEMORY;
+    if (B > 0) {
+      const size_t sb = sizeof(double) * (size_t)Net::NPHOTO * B;
+      if (!dc.sig.ensure(sb)) return kx::FLAG_DEVICE_MEMORY;
+      if (!hip_ok(hipMemcpy(dc.sig.p, o.sigma.data(), sb,
+                            hipMemcpyHostToDevice), "H2D"))
+        return -1;
+    }
+    hipLaunchKernelGGL(kx::photo_rows_kernel<Net>, dim3((rows + 255) / 256),
+                       dim3(256), 0, s, rows, P, pst, B ? Phi : nullptr, B,
+                       static_cast<const double*>(dc.sig.p),
+                       static_cast<double*>(dc.prow.p));
+    if (!hip_ok(hipGetLastError(), "photo rows kernel")) return -1;
+    P = static_cast<const double*>(dc.prow.p);
+    pst = rows == 1 ? 0 : Net::NROW;
+    return 0;
+  }
 }
 
 // the plain kernel over a chunk of resident arrays
@@ -325,6 +365,84 @@ struct krome_opts : Opts {};
 
 krome_opts* krome_opts_new(void) { return new krome_opts(); }
 void krome_opts_free(krome_opts* h) { delete h; }
+
+// photon bins (krome.h:203-228, krome.cpp:313-377): the bins' cross
+// sections are averaged here, once, on the host (kx_photo.h)
+int krome_opts_set_photo(krome_opts* h, int nbins, const double* Elo,
+                         const double* Ehi) {
+  if (!h || nbins <= 0 || !Elo || !Ehi) return -1;
+  for (int b = 0; b < nbins; ++b)
+    if (!(Ehi[b] > Elo[b]) || !(Elo[b] > 0.0)) return -1;
+  h->Elo.assign(Elo, Elo + nbins);
+  h->Ehi.assign(Ehi, Ehi + nbins);
+  h->nbins = nbins;
+  kx::photo_bin_xsec<Net>(nbins, Elo, Ehi, h->photo_alpha, h->sigma);
+  return 0;
+}
+int krome_opts_nbins(const krome_opts* h) { return h ? h->nbins : 0; }
+int krome_opts_set_photo_alpha(krome_opts* h, double alpha) {
+  if (!h || !std::isfinite(alpha)) return -1;
+  h->photo_alpha = alpha;
+  if (h->nbins > 0)
+    kx::photo_bin_xsec<Net>(h->nbins, h->Elo.data(), h->Ehi.data(), alpha,
+                            h->sigma);
+  return 0;
+}
+// Doppler broadening of tabulated cross sections and a dust opacity of the
+// bins are not built here: only "none" is accepted
+int krome_opts_set_photo_broadening(krome_opts* h, double Tgas,
+                                    double vturb) {
+  if (!h || Tgas < 0.0 || vturb < 0.0) return -1;
+  return (Tgas == 0.0 && vturb == 0.0) ? 0 : -1;
+}
+int krome_opts_set_photo_dust_opacity(krome_opts* h, const double* kappa) {
+  if (!h || h->nbins == 0) return -1;
+  return kappa ? -1 : 0;
+}
+int krome_opts_photo_sigma(const krome_opts* h, double* sigma) {
+  if (!h || h->nbins == 0 || !sigma) return -1;
+  for (int i = 0; i < Net::NPHOTO * h->nbins; ++i) sigma[i] = h->sigma[i];
+  return 0;
+}
+int krome_nphoto(void) { return Net::NPHOTO; }
+int krome_photo_reaction(int r) {
+  return (r >= 0 && r < Net::NPHOTO) ? Net::photo_reaction[r] : -1;
+}
+int krome_photo_reactant(int r) {
+  return (r >= 0 && r < Net::NPHOTO) ? Net::photo_reactant[r] : -1;
+}
+double krome_photo_pdiss(int r) {
+  return (r >= 0 && r < Net::NPHOTO) ? 1.0 : std::nan("");
+}
+double krome_photo_xsec(int r, double E) {
+  if (r < 0 || r >= Net::NPHOTO) return std::nan("");
+  return (E >= Net::photo_Emin[r] && E <= Net::photo_Emax[r])
+             ? Net::photo_xsec(r, E)
+             : 0.0;
+}
+int krome_bins_lin(int nbins, double lower, double upper, double* Elo,
+                   double* Ehi) {
+  if (nbins <= 0 || !(upper > lower) || !(lower > 0.0) || !Elo || !Ehi)
+    return -1;
+  const double dE = (upper - lower) / nbins;
+  for (int b = 0; b < nbins; ++b) {
+    Elo[b] = lower + b * dE;
+    Ehi[b] = lower + (b + 1) * dE;
+  }
+  return 0;
+}
+int krome_bins_log(int nbins, double lower, double upper, double* Elo,
+                   double* Ehi) {
+  if (nbins <= 0 || !(upper > lower) || !(lower > 0.0) || !Elo || !Ehi)
+    return -1;
+  const double l0 = std::log10(lower);
+  const double dl = (std::log10(upper) - l0) / nbins;
+  for (int b = 0; b < nbins; ++b) {
+    Elo[b] = std::pow(10.0, l0 + b * dl);
+    Ehi[b] = std::pow(10.0, l0 + (b + 1) * dl);
+  }
+  return 0;
+}
 
 int krome_opts_set(krome_opts* h, const char* key, double v) {
   if (!h || !key) return -1;
@@ -545,10 +663,12 @@ int solve_x_resident(kx::CellsArgs a, const double* rho, bool skip_dt,
 // mass fractions
 int solve_host(int ncell, double* v, const double* rho, double* Tgas,
                const double* dt, const double* par, int par_stride,
-               int* flags, const krome_opts* h) {
+               const double* Phi, int* flags, const krome_opts* h) {
   if (!h || ncell < 0 || !v || !Tgas || !dt) return -1;
   if (par && par_stride != 0 && par_stride != Net::NPAR) return -1;
   if (ncell == 0) return 0;
+  // photon bins configured and no field (krome.cpp:415-417)
+  if (h->nbins > 0 && !Phi) return kx::FLAG_NO_FIELD;
   std::vector<int> fl(ncell, 0);
   // parameters: rows, or the defaults; a REQUIRED one unset: -105 for all
   std::vector<double> dflt(Net::NPAR);
@@ -573,10 +693,13 @@ int solve_host(int ncell, double* v, const double* rho, double* Tgas,
   DevCache& dc = h->device_cache ? g_cache : local;
   const size_t nb = sizeof(double) * (size_t)ncell * NS;
   const size_t pb = sizeof(double) * (size_t)(pst ? ncell : 1) * Net::NPAR;
+  const bool field = Net::NPHOTO > 0 && h->nbins > 0;
+  const size_t fb = sizeof(double) * (size_t)ncell * h->nbins;
   if (!dc.n.ensure(nb) || !dc.T.ensure(sizeof(double) * ncell) ||
       !dc.dt.ensure(sizeof(double) * ncell) || !dc.par.ensure(pb) ||
       !dc.flags.ensure(sizeof(int) * ncell) ||
-      (rho && !dc.rho.ensure(sizeof(double) * ncell)))
+      (rho && !dc.rho.ensure(sizeof(double) * ncell)) ||
+      (field && !dc.phi.ensure(fb)))
     return kx::FLAG_DEVICE_MEMORY;
   hipStream_t s = nullptr;
   bool ok = hip_ok(hipMemcpy(dc.n.p, v, nb, hipMemcpyHostToDevice), "H2D") &&
@@ -586,8 +709,16 @@ int solve_host(int ncell, double* v, const double* rho, double* Tgas,
                              hipMemcpyHostToDevice), "H2D") &&
             hip_ok(hipMemcpy(dc.par.p, P, pb, hipMemcpyHostToDevice), "H2D") &&
             (!rho || hip_ok(hipMemcpy(dc.rho.p, rho, sizeof(double) * ncell,
-                                      hipMemcpyHostToDevice), "H2D"));
+                                      hipMemcpyHostToDevice), "H2D")) &&
+            (!field || hip_ok(hipMemcpy(dc.phi.p, Phi, fb,
+                                        hipMemcpyHostToDevice), "H2D"));
   if (!ok) return -1;
+  const double* Pk = static_cast<const double*>(dc.par.p);
+  int pk = pst;
+  const int rr = kernel_rows(ncell, Pk, pk,
+                             field ? static_cast<const double*>(dc.phi.p)
+                                   : nullptr, *h, dc, s);
+  if (rr != 0) return rr;
   kx::CellsArgs a;
   a.ncell = ncell;
   a.n = static_cast<double*>(dc.n.p);
@@ -595,8 +726,8 @@ int solve_host(int ncell, double* v, const double* rho, double* Tgas,
   a.n_ss = 1;
   a.T = static_cast<double*>(dc.T.p);
   a.dt = static_cast<const double*>(dc.dt.p);
-  a.par = static_cast<const double*>(dc.par.p);
-  a.par_stride = pst;
+  a.par = Pk;
+  a.par_stride = pk;
   a.flags = static_cast<int*>(dc.flags.p);
   a.stats = nullptr;
   a.o = solve_opts(h);
@@ -626,8 +757,8 @@ int solve_host(int ncell, double* v, const double* rho, double* Tgas,
 // device, v in either layout (rho: mass fractions)
 int solve_device(int ncell, double* v, const double* rho, int layout,
                  long ld, double* Tgas, const double* dt, const double* par,
-                 int par_stride, int* flags, krome_device_summary* summary,
-                 const krome_opts* h) {
+                 int par_stride, const double* Phi, int* flags,
+                 krome_device_summary* summary, const krome_opts* h) {
   if (summary) *summary = krome_device_summary{0, 0, 0};
   if (!h || ncell < 0 || !v || !Tgas || !dt) return -1;
   if (layout != KROME_LAYOUT_SPECIES_MAJOR &&
@@ -636,6 +767,7 @@ int solve_device(int ncell, double* v, const double* rho, int layout,
   if (ld != 0 && ld < ncell) return -1;
   if (par && par_stride != 0 && par_stride != Net::NPAR) return -1;
   if (ncell == 0) return 0;
+  if (h->nbins > 0 && !Phi) return kx::FLAG_NO_FIELD;
   if (!hip_ok(hipDeviceSynchronize(), "fence")) return -1;
   std::lock_guard<std::mutex> lk(g_cache_mutex);
   DevCache& dc = g_cache;
@@ -667,6 +799,9 @@ int solve_device(int ncell, double* v, const double* rho, int layout,
     kx::fill_flags(ncell, fl, rj);
     return rj;
   }
+  const int rr = kernel_rows(ncell, P, pst, h->nbins > 0 ? Phi : nullptr,
+                             *h, dc, nullptr);
+  if (rr != 0) return rr;
   kx::CellsArgs a;
   a.ncell = ncell;
   a.n = v;
@@ -704,17 +839,18 @@ int krome_solve_cells(int ncell, double* n, double* Tgas, const double* dt,
                       const double* par, int par_stride,
                       const krome_cells* extra, int* flags,
                       const krome_opts* h) {
-  (void)extra;  // no photon bins or dust bins in this build's networks
-  return solve_host(ncell, n, nullptr, Tgas, dt, par, par_stride, flags, h);
+  // the photon field of the cells (extra->Phi); no dust bins here
+  return solve_host(ncell, n, nullptr, Tgas, dt, par, par_stride,
+                    extra ? extra->Phi : nullptr, flags, h);
 }
 
 int krome_solve_cells_x(int ncell, double* x, const double* rho,
                         double* Tgas, const double* dt, const double* par,
                         int par_stride, const krome_cells* extra, int* flags,
                         const krome_opts* h) {
-  (void)extra;
   if (!rho) return -1;
-  return solve_host(ncell, x, rho, Tgas, dt, par, par_stride, flags, h);
+  return solve_host(ncell, x, rho, Tgas, dt, par, par_stride,
+                    extra ? extra->Phi : nullptr, flags, h);
 }
 
 // device-resident arrays (krome.h: krome_solve_cells_device): n in the
@@ -726,9 +862,9 @@ int krome_solve_cells_device(int ncell, double* n, int layout, long ld,
                              const krome_cells* extra, int* flags,
                              krome_device_summary* summary,
                              const krome_opts* h) {
-  (void)extra;
   return solve_device(ncell, n, nullptr, layout, ld, Tgas, dt, par,
-                      par_stride, flags, summary, h);
+                      par_stride, extra ? extra->Phi : nullptr, flags,
+                      summary, h);
 }
 
 // the same with mass fractions x and the gas density rho[ncell] on the
@@ -739,10 +875,9 @@ int krome_solve_cells_x_device(int ncell, double* x, const double* rho,
                                int par_stride, const krome_cells* extra,
                                int* flags, krome_device_summary* summary,
                                const krome_opts* h) {
-  (void)extra;
   if (!rho) return -1;
   return solve_device(ncell, x, rho, layout, ld, Tgas, dt, par, par_stride,
-                      flags, summary, h);
+                      extra ? extra->Phi : nullptr, flags, summary, h);
 }
 
 // per-cell statistics of the last kind of call, for the benchmarks and the
@@ -761,6 +896,7 @@ int krome_solve_cells_device_stats(int ncell, double* n, int layout, long ld,
   if (layout == KROME_LAYOUT_SPECIES_MAJOR && ld != 0 && ld < ncell)
     return -1;
   if (ncell == 0) return 0;
+  if (h->nbins > 0) return kx::FLAG_NO_FIELD;  // no field argument here
   std::lock_guard<std::mutex> lk(g_cache_mutex);
   if (!g_cache.count.ensure(sizeof(int))) return -1;
   const int np = kx::params_check_device<Net>(
@@ -772,6 +908,10 @@ int krome_solve_cells_device_stats(int ncell, double* n, int layout, long ld,
     kx::fill_flags(ncell, flags, rj);
     return rj;
   }
+  int pst = par_stride;
+  const int rr = kernel_rows(ncell, par, pst, nullptr, *h, g_cache, nullptr);
+  if (rr != 0) return rr;
+  par_stride = pst;
   kx::CellsArgs a;
   a.ncell = ncell;
   a.n = n;
@@ -869,11 +1009,13 @@ int conserve_resident(int ncell, double* v, long cs, long ss,
 // Tgas, evaluated on the device; a cell with an unset REQUIRED parameter
 // is skipped (its outputs untouched) and its flag returned
 int query_host(int ncell, const double* n, const double* Tgas,
-               const double* par, int par_stride, int what, double* out,
-               double* heat, double* cool, double* heats, double* cools,
-               double* dTdt) {
+               const double* par, int par_stride, const double* Phi,
+               const krome_opts* h, int what, double* out, double* heat,
+               double* cool, double* heats, double* cools, double* dTdt) {
   if (ncell < 0 || !n || !Tgas) return -1;
   if (par && par_stride != 0 && par_stride != Net::NPAR) return -1;
+  const int B = h ? h->nbins : 0;
+  if (B > 0 && !Phi) return kx::FLAG_NO_FIELD;  // krome.cpp:113-116
   if (ncell == 0) return 0;
   std::vector<double> dflt(Net::NPAR);
   krome_params_default(dflt.data());
@@ -887,15 +1029,28 @@ int query_host(int ncell, const double* n, const double* Tgas,
   const size_t nb = sizeof(double) * (size_t)ncell * NS;
   const size_t pb = sizeof(double) * (size_t)(pst ? ncell : 1) * Net::NPAR;
   const size_t ob = sizeof(double) * (size_t)ncell * (per + nh + nc);
+  const bool field = Net::NPHOTO > 0 && B > 0;
+  const size_t fb = sizeof(double) * (size_t)ncell * B;
   if (!dc.n.ensure(nb) || !dc.T.ensure(sizeof(double) * ncell) ||
       !dc.par.ensure(pb) || !dc.work.ensure(ob) ||
-      !dc.flags.ensure(sizeof(int) * ncell))
+      !dc.flags.ensure(sizeof(int) * ncell) || (field && !dc.phi.ensure(fb)))
     return kx::FLAG_DEVICE_MEMORY;
   if (!hip_ok(hipMemcpy(dc.n.p, n, nb, hipMemcpyHostToDevice), "H2D") ||
       !hip_ok(hipMemcpy(dc.T.p, Tgas, sizeof(double) * ncell,
                         hipMemcpyHostToDevice), "H2D") ||
-      !hip_ok(hipMemcpy(dc.par.p, P, pb, hipMemcpyHostToDevice), "H2D"))
+      !hip_ok(hipMemcpy(dc.par.p, P, pb, hipMemcpyHostToDevice), "H2D") ||
+      (field && !hip_ok(hipMemcpy(dc.phi.p, Phi, fb, hipMemcpyHostToDevice),
+                        "H2D")))
     return -1;
+  const double* Pk = static_cast<const double*>(dc.par.p);
+  int pk = pst;
+  const Opts none;
+  const int rr = kernel_rows(ncell, Pk, pk,
+                             field ? static_cast<const double*>(dc.phi.p)
+                                   : nullptr,
+                             h ? static_cast<const Opts&>(*h) : none, dc,
+                             nullptr);
+  if (rr != 0) return rr;
   double* o = static_cast<double*>(dc.work.p);
   double* hs = o + (size_t)ncell * per;
   double* cs = hs + (size_t)ncell * nh;
@@ -903,7 +1058,7 @@ int query_host(int ncell, const double* n, const double* Tgas,
                      dim3(128), 0, nullptr, ncell,
                      static_cast<const double*>(dc.n.p),
                      static_cast<const double*>(dc.T.p),
-                     static_cast<const double*>(dc.par.p), pst, what, o, hs,
+                     Pk, pk, what, o, hs,
                      cs, static_cast<int*>(dc.flags.p));
   if (!hip_ok(hipGetLastError(), "query kernel")) return -1;
   std::vector<double> ho((size_t)ncell * (per + nh + nc));
@@ -1005,20 +1160,16 @@ extern "C" {
 int krome_get_rates(int ncell, const double* n, const double* Tgas,
                     const double* par, int par_stride, const double* Phi,
                     const krome_opts* h, double* k) {
-  (void)Phi;  // no photon bins in these networks
-  (void)h;
   if (!k) return -1;
-  return query_host(ncell, n, Tgas, par, par_stride, kx::Q_RATES, k, nullptr,
-                    nullptr, nullptr, nullptr, nullptr);
+  return query_host(ncell, n, Tgas, par, par_stride, Phi, h, kx::Q_RATES, k,
+                    nullptr, nullptr, nullptr, nullptr, nullptr);
 }
 int krome_get_fluxes(int ncell, const double* n, const double* Tgas,
                      const double* par, int par_stride, const double* Phi,
                      const krome_opts* h, double* fluxes) {
-  (void)Phi;
-  (void)h;
   if (!fluxes) return -1;
-  return query_host(ncell, n, Tgas, par, par_stride, kx::Q_FLUXES, fluxes,
-                    nullptr, nullptr, nullptr, nullptr, nullptr);
+  return query_host(ncell, n, Tgas, par, par_stride, Phi, h, kx::Q_FLUXES,
+                    fluxes, nullptr, nullptr, nullptr, nullptr, nullptr);
 }
 int krome_nheat(void) { return Net::NHEAT; }
 int krome_ncool(void) { return Net::NCOOL; }
@@ -1032,10 +1183,8 @@ int krome_get_thermal(int ncell, const double* n, const double* Tgas,
                       const double* par, int par_stride, const double* Phi,
                       const krome_opts* h, double* heat, double* cool,
                       double* heats, double* cools, double* dTdt) {
-  (void)Phi;
-  (void)h;
-  return query_host(ncell, n, Tgas, par, par_stride, kx::Q_THERMAL, nullptr,
-                    heat, cool, heats, cools, dTdt);
+  return query_host(ncell, n, Tgas, par, par_stride, Phi, h, kx::Q_THERMAL,
+                    nullptr, heat, cool, heats, cools, dTdt);
 }
 
 int krome_get_element_totals(int ncell, const double* n, double* tot) {

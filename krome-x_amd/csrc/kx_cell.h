@@ -29,7 +29,8 @@ enum : int {
 
 // KROME flags (runtime/krome.hpp:95-145)
 enum : int {
-  FLAG_UNSUPPORTED = -104, FLAG_PARAM_UNSET = -105, FLAG_NONPOSITIVE_T = -110,
+  FLAG_NO_FIELD = -103, FLAG_UNSUPPORTED = -104, FLAG_PARAM_UNSET = -105,
+  FLAG_NONPOSITIVE_T = -110,
   FLAG_DEVICE_MEMORY = -111
 };
 

@@ -1,0 +1,99 @@
+// Photon bins for the photo reactions of a generated network (KROME-X's
+// photo module, runtime/core/photo.hpp: PhotoModule::init 135-253,
+// photo_rates 357-375).
+//
+// The rate of photo reaction q in a cell is the bin sum
+//   k_q = sum_b sigma_qb Phi_b
+// of the cell's photon field Phi_b [photons cm^-2 s^-1] (held fixed during
+// a solve) with the bin-averaged cross sections
+//   sigma_qb = Int_{bin b ^ [Emin_q, Emax_q]} sigma_q(E) E^-alpha dE
+//              / Int_{bin b} E^-alpha dE,
+// the photons inside a bin distributed as dN/dE ~ E^-alpha (alpha 1: the
+// same number per ln E).
+//
+//   photo_bin_xsec  the table sigma[NPHOTO][B] of a set of bins, on the
+//                   host, once per bin set: the overlap of each bin with
+//                   the reaction's window by a 64-cell midpoint rule, in
+//                   ln E (alpha != 0) or in E (alpha 0), each cell weighted
+//                   by its exact Int E^-alpha dE, so a constant cross
+//                   section averages exactly
+//   photo_rows_kernel  once per solve, on the device: each cell's kernel
+//                   parameter row, the caller's NPAR parameters followed by
+//                   its NPHOTO bin sums (one thread per cell; the field's
+//                   row and the table are read once)
+//
+// The generated rates read the sums from the row (Params::ph), so the
+// solve kernels see a photo rate like any rate of the parameters: a
+// constant of the cell.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+namespace kx {
+
+// Int_a^c E^-alpha dE
+inline double photo_weight(double alpha, double a, double c) {
+  if (alpha == 1.0) return std::log(c / a);
+  if (alpha == 0.0) return c - a;
+  const double q = 1.0 - alpha;
+  return (std::pow(c, q) - std::pow(a, q)) / q;
+}
+
+// sigma[q B + b] of the bins [Elo[b], Ehi[b]] (0 < Elo < Ehi, checked by
+// the caller)
+template <class Net>
+void photo_bin_xsec(int B, const double* Elo, const double* Ehi,
+                    double alpha, std::vector<double>& sigma) {
+  constexpr int QUAD = 64;
+  sigma.assign((size_t)(Net::NPHOTO > 0 ? Net::NPHOTO : 1) * B, 0.0);
+  for (int q = 0; q < Net::NPHOTO; ++q) {
+    for (int b = 0; b < B; ++b) {
+      const double a = std::fmax(Elo[b], Net::photo_Emin[q]);
+      const double c = std::fmin(Ehi[b], Net::photo_Emax[q]);
+      if (!(c > a)) continue;
+      const double dE = (c - a) / QUAD;
+      const double dl = std::log(c / a) / QUAD;
+      double s = 0.0;
+      for (int i = 0; i < QUAD; ++i) {
+        double E, w;
+        if (alpha == 0.0) {
+          E = a + (i + 0.5) * dE;
+          w = dE;
+        } else {
+          E = a * std::exp((i + 0.5) * dl);
+          const double lo = a * std::exp(i * dl);
+          const double hi = i + 1 == QUAD ? c : a * std::exp((i + 1) * dl);
+          w = photo_weight(alpha, lo, hi);
+        }
+        s += Net::photo_xsec(q, E) * w;
+      }
+      sigma[(size_t)q * B + b] = s / photo_weight(alpha, Elo[b], Ehi[b]);
+    }
+  }
+}
+
+// out[c NROW ..]: the parameter row par[c pst ..] (pst 0: one row for
+// every cell), then sum_b sigma[q B + b] Phi[c B + b] for each photo
+// reaction q (0 without a field: B 0 or Phi null)
+template <class Net>
+__global__ void __launch_bounds__(256)
+photo_rows_kernel(int rows, const double* __restrict__ par, int pst,
+                  const double* __restrict__ Phi, int B,
+                  const double* __restrict__ sigma, double* __restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= rows) return;
+  const double* p = par + (long)c * pst;
+  double* o = out + (long)c * Net::NROW;
+  for (int i = 0; i < Net::NPAR; ++i) o[i] = p[i];
+  const double* f = Phi ? Phi + (long)c * B : nullptr;
+  for (int q = 0; q < Net::NPHOTO; ++q) {
+    double s = 0.0;
+    if (f)
+      for (int b = 0; b < B; ++b) s += sigma[q * B + b] * f[b];
+    o[Net::NPAR + q] = s;
+  }
+}
+
+}  // namespace kx

@@ -14,6 +14,15 @@
 //   h2_opacity_ripamonti  Ripamonti & Abel 2004 (codegen.py:3657-3660)
 //   gamma_rot, gamma_rot_ortho_para  exact rotational heat capacities
 //                       (codegen.py:3745-3779)
+//   H2_solomonLW        Solomon-process H2 photodissociation from J(12.87
+//                       eV), k = 1.38e9 J (runtime/core/h2_shielding.hpp:
+//                       84-94)
+//   H/He_recombination_on_dust  Weingartner & Draine 2001 grain-assisted
+//                       recombination fits (dust_rates.hpp:34-66)
+// and, on the host only, the photo cross sections the photon bins are
+// built from: sigma_v96 (Verner et al. 1996 fit), xsec_interp (a table
+// interpolated linearly, zero outside it) and H2_sigmaLW (the Lyman-Werner
+// line and continuum fits, h2_shielding.hpp:96-125).
 #pragma once
 #include <cmath>
 #include <type_traits>
@@ -334,6 +343,91 @@ KX_HD inline R dissH2_Martin96(R Tin, R n_H) {
     k += kx::pow(R(10), lk);
   }
   return k;
+}
+
+// Solomon-process H2 photodissociation rate [s^-1] from the intensity at
+// 12.87 eV (flux_eV in eV cm^-2 s^-1 Hz^-1 sr^-1): 1.38e9 J [erg units]
+template <class R>
+KX_HD inline R H2_solomonLW(R flux_eV) {
+  return R(1.38e9) * flux_eV * R(1.602176634e-12);
+}
+
+// Weingartner & Draine 2001 grain-assisted recombination [cm^3 s^-1]:
+// k = s / (1 + a psi^b (1 + c T^d psi^(e + f ln T))), s = pre d2g / 0.01,
+// psi = G0 sqrt(T) / n_e.  No electrons: 0 (the psi -> inf limit);
+// psi <= 0 (no field): s (the psi -> 0 limit)
+template <class R>
+KX_HD inline R wd01_recombination(R pre, R a, R b, R c, R d, R e, R f, R T,
+                                  R ne, R G0, R d2g) {
+  if (ne < R(1e-20)) return R(0);
+  const R s = pre * (d2g / R(1e-2));
+  const R psi = G0 * kx::sqrt(T) / ne;
+  if (psi <= R(0)) return s;
+  return s / (R(1) + a * kx::pow(psi, b)
+                         * (R(1) + c * kx::pow(T, d)
+                                       * kx::pow(psi, e + f * kx::log(T))));
+}
+template <class R>
+KX_HD inline R H_recombination_on_dust(R T, R ne, R G0, R d2g) {
+  return wd01_recombination(R(1.225e-13), R(8.074e-6), R(1.378), R(5.087e2),
+                            R(0.01586), R(-0.4723), R(-1.102e-5), T, ne, G0,
+                            d2g);
+}
+template <class R>
+KX_HD inline R He_recombination_on_dust(R T, R ne, R G0, R d2g) {
+  return wd01_recombination(R(5.572e-14), R(3.185e-7), R(1.512), R(5.115e3),
+                            R(3.903e-7), R(-0.4956), R(-5.494e-7), T, ne, G0,
+                            d2g);
+}
+
+// ---- photo cross sections [cm^2] of photon energy E [eV], host only -----
+
+// Verner et al. 1996 analytic fit of a ground-state photoionization cross
+// section: x = E/E0 - y0, y = sqrt(x^2 + y1^2),
+// sigma = 1e-18 s0 ((x-1)^2 + yw^2) y^(P/2 - 5.5) (1 + sqrt(y/ya))^-P
+inline double sigma_v96(double E, double E0, double s0, double ya, double P,
+                        double yw, double y0, double y1) {
+  const double x = E / E0 - y0;
+  const double y = std::sqrt(x * x + y1 * y1);
+  return 1e-18 * s0 * ((x - 1.0) * (x - 1.0) + yw * yw)
+         * std::pow(y, 0.5 * P - 5.5) * std::pow(1.0 + std::sqrt(y / ya), -P);
+}
+
+// a tabulated cross section (tE ascending), linear between the points and
+// zero outside [tE[0], tE[n-1]]
+inline double xsec_interp(double E, const double* tE, const double* tS,
+                          int n) {
+  if (E < tE[0] || E > tE[n - 1]) return 0.0;
+  int lo = 0, hi = n - 1;
+  while (hi - lo > 1) {
+    const int m = (lo + hi) / 2;
+    if (tE[m] <= E) lo = m;
+    else hi = m;
+  }
+  const double f = (E - tE[lo]) / (tE[lo + 1] - tE[lo]);
+  return tS[lo] + f * (tS[lo + 1] - tS[lo]);
+}
+
+// the effective H2 Lyman-Werner cross section: the Lyman and Werner fits of
+// para (v=0) and ortho (v=1) H2, each 1e-18 10^(polynomial in E) on its
+// window, weighted by the ortho:para ratio 3
+inline double H2_sigmaLW(double E) {
+  double sL0 = 0.0, sW0 = 0.0, sL1 = 0.0, sW1 = 0.0;
+  if (E > 14.675 && E < 16.820)
+    sL0 = 1e-18 * std::pow(10.0, 15.1289 - 1.05139 * E);
+  else if (E > 16.820 && E < 17.6)
+    sL0 = 1e-18 * std::pow(10.0, -31.41 + 1.8042e-2 * E * E * E
+                                     - 4.2339e-5 * std::pow(E, 5.0));
+  if (E > 14.675 && E < 17.7)
+    sW0 = 1e-18 * std::pow(10.0, 13.5311 - 0.9182618 * E);
+  if (E > 14.159 && E < 15.302)
+    sL1 = 1e-18 * std::pow(10.0, 12.0218406 - 0.819429 * E);
+  else if (E > 15.302 && E < 17.2)
+    sL1 = 1e-18 * std::pow(10.0, 16.04644 - 1.082438 * E);
+  if (E > 14.159 && E < 17.2)
+    sW1 = 1e-18 * std::pow(10.0, 12.87367 - 0.85088597 * E);
+  const double para = 1.0 / (3.0 + 1.0);
+  return para * (sL0 + sW0) + (1.0 - para) * (sL1 + sW1);
 }
 
 }  // namespace kx

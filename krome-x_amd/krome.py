@@ -54,6 +54,19 @@ def _dptr(a):
     return None if a is None else a.ctypes.data_as(_D)
 
 
+class _Extra(ctypes.Structure):
+    """krome_cells: the optional per-cell arrays of a multi-cell call"""
+    _fields_ = [("Phi", ctypes.c_void_p), ("Td", ctypes.c_void_p),
+                ("adust", ctypes.c_void_p)]
+
+
+def _extra(F):
+    """a krome_cells pointer holding the fields F (None: no extra)"""
+    if F is None:
+        return None
+    return ctypes.byref(_Extra(F.ctypes.data, None, None))
+
+
 def lib_path(network):
     return HERE / ("libkrome_%s.so" % network)
 
@@ -223,7 +236,29 @@ class Network:
                 ("krome_conserve_cells_x_device", ctypes.c_int,
                  [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_long,
                   ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
-                  ctypes.c_int, ctypes.c_void_p])):
+                  ctypes.c_int, ctypes.c_void_p]),
+                ("krome_opts_set_photo", ctypes.c_int,
+                 [ctypes.c_void_p, ctypes.c_int, _D, _D]),
+                ("krome_opts_nbins", ctypes.c_int, [ctypes.c_void_p]),
+                ("krome_opts_set_photo_alpha", ctypes.c_int,
+                 [ctypes.c_void_p, ctypes.c_double]),
+                ("krome_opts_set_photo_broadening", ctypes.c_int,
+                 [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
+                ("krome_opts_set_photo_dust_opacity", ctypes.c_int,
+                 [ctypes.c_void_p, _D]),
+                ("krome_opts_photo_sigma", ctypes.c_int,
+                 [ctypes.c_void_p, _D]),
+                ("krome_nphoto", ctypes.c_int, []),
+                ("krome_photo_reaction", ctypes.c_int, [ctypes.c_int]),
+                ("krome_photo_reactant", ctypes.c_int, [ctypes.c_int]),
+                ("krome_photo_pdiss", ctypes.c_double, [ctypes.c_int]),
+                ("krome_photo_xsec", ctypes.c_double,
+                 [ctypes.c_int, ctypes.c_double]),
+                ("krome_bins_lin", ctypes.c_int,
+                 [ctypes.c_int, ctypes.c_double, ctypes.c_double, _D, _D]),
+                ("krome_bins_log", ctypes.c_int,
+                 [ctypes.c_int, ctypes.c_double, ctypes.c_double, _D,
+                  _D])):
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
@@ -285,24 +320,27 @@ class Network:
     # -- solves ----------------------------------------------------------
 
     def solve(self, n, Tgas, dt, par=None, Phi=None, opts=None, check=True):
-        """integrate every cell over its own dt [s] (krome_solve_cells)"""
-        if Phi is not None:
-            raise KromeError("this network has no photon bins")
+        """integrate every cell over its own dt [s] (krome_solve_cells);
+        Phi: the photon flux per bin [cm^-2 s^-1], needed once
+        opts.set_photo() was called: Phi[nbins] for every cell or
+        Phi[ncell, nbins]"""
         single, n, ncell = self._cells(n)
         T = self._per_cell(Tgas, ncell)
         dtv = self._per_cell(dt, ncell)
         opts = self._options(opts)
         P, stride = self._params(par, ncell)
+        F = self._phi(Phi, ncell, opts)
         flags = np.zeros(ncell, dtype=np.intc)
         ierr = self._lib.krome_solve_cells(
-            ncell, _dptr(n), _dptr(T), _dptr(dtv), _dptr(P), stride, None,
-            flags.ctypes.data_as(_I), opts._h)
+            ncell, _dptr(n), _dptr(T), _dptr(dtv), _dptr(P), stride,
+            _extra(F), flags.ctypes.data_as(_I), opts._h)
         res = Result(n[0] if single else n, T[0] if single else T,
                      flags[0] if single else flags, ierr)
         self._check(check, ierr, flags, "krome_solve_cells")
         return res
 
-    def solve_x(self, x, rho, Tgas, dt, par=None, opts=None, check=True):
+    def solve_x(self, x, rho, Tgas, dt, par=None, Phi=None, opts=None,
+                check=True):
         """solve() with mass fractions x and the gas density rho [g cm^-3]
         (krome_solve_cells_x)"""
         single, x, ncell = self._cells(x)
@@ -311,22 +349,24 @@ class Network:
         dtv = self._per_cell(dt, ncell)
         opts = self._options(opts)
         P, stride = self._params(par, ncell)
+        F = self._phi(Phi, ncell, opts)
         flags = np.zeros(ncell, dtype=np.intc)
         ierr = self._lib.krome_solve_cells_x(
             ncell, _dptr(x), _dptr(rh), _dptr(T), _dptr(dtv), _dptr(P),
-            stride, None, flags.ctypes.data_as(_I), opts._h)
+            stride, _extra(F), flags.ctypes.data_as(_I), opts._h)
         res = Result(x[0] if single else x, T[0] if single else T,
                      flags[0] if single else flags, ierr)
         self._check(check, ierr, flags, "krome_solve_cells_x")
         return res
 
     def solve_device(self, n, Tgas, dt, par, opts=None, flags=None,
-                     stats=None, sync=True):
+                     stats=None, sync=True, Phi=None):
         """device-resident solve on torch tensors: n[nspec, ncell]
         (species-major, float64), Tgas, dt [ncell], par[ncell, npar], all
         on the GPU; updated in place.  flags (int32 [ncell]) and stats
-        (int64 [ncell, 9]) optional device tensors.  Returns the C
-        function's flag."""
+        (int64 [ncell, 9]) optional device tensors; Phi[ncell, nbins] the
+        photon fields on the GPU (krome_solve_cells_device with
+        extra->Phi; no stats then).  Returns the C function's flag."""
         ncell = int(Tgas.shape[0])
         for t in (n, Tgas, dt, par):
             if not t.is_cuda or not t.is_contiguous():
@@ -336,6 +376,24 @@ class Network:
         if tuple(par.shape) != (ncell, self.npar):
             raise KromeError("par must be [ncell, npar]")
         opts = self._options(opts)
+        if Phi is not None:
+            if stats is not None:
+                raise KromeError("no per-cell counters with a field")
+            if (not Phi.is_cuda or not Phi.is_contiguous()
+                    or tuple(Phi.shape) != (ncell, opts.nbins)):
+                raise KromeError("Phi must be a contiguous GPU tensor "
+                                 "[ncell, nbins]")
+            ext = _Extra(Phi.data_ptr(), None, None)
+            r = self._lib.krome_solve_cells_device(
+                ncell, n.data_ptr(), LAYOUT_SPECIES_MAJOR, ncell,
+                Tgas.data_ptr(), dt.data_ptr(), par.data_ptr(), self.npar,
+                ctypes.byref(ext),
+                flags.data_ptr() if flags is not None else None, None,
+                opts._h)
+            if sync:
+                import torch
+                torch.cuda.synchronize()
+            return r
         if flags is None or stats is not None:
             import torch
             if flags is None:
@@ -380,21 +438,25 @@ class Network:
 
     # -- queries (the reference binding's rates / fluxes / thermal) --------
 
-    def rates(self, n, Tgas, par=None):
-        """rate coefficients [..., nrea] at the states (krome_get_rates)"""
-        return self._query(n, Tgas, par, "krome_get_rates")
+    def rates(self, n, Tgas, par=None, Phi=None, opts=None):
+        """rate coefficients [..., nrea] at the states (krome_get_rates),
+        the photo rates of the fields Phi over opts' photon bins"""
+        return self._query(n, Tgas, par, Phi, opts, "krome_get_rates")
 
-    def fluxes(self, n, Tgas, par=None):
+    def fluxes(self, n, Tgas, par=None, Phi=None, opts=None):
         """reaction fluxes k prod(n) [cm^-3 s^-1], [..., nrea]"""
-        return self._query(n, Tgas, par, "krome_get_fluxes")
+        return self._query(n, Tgas, par, Phi, opts, "krome_get_fluxes")
 
-    def _query(self, n, Tgas, par, fname):
+    def _query(self, n, Tgas, par, Phi, opts, fname):
         single, n, ncell = self._cells(n)
         T = self._per_cell(Tgas, ncell)
         P, stride = self._params(par, ncell)
+        h = opts._h if opts is not None else None
+        F = self._phi(Phi, ncell, opts) if opts is not None else None
         out = np.zeros((ncell, self.nrea))
         r = getattr(self._lib, fname)(ncell, _dptr(n), _dptr(T), _dptr(P),
-                                      stride, None, None, _dptr(out))
+                                      stride, _dptr(F) if F is not None
+                                      else None, h, _dptr(out))
         if r != 0:
             raise KromeError("%s returned %d: %s" % (fname, r,
                                                      self.flag_message(r)))
@@ -547,6 +609,56 @@ class Network:
             self._opts = Options(self)
         return self._opts
 
+    def _phi(self, Phi, ncell, opts):
+        """the fields as [ncell, nbins] rows (None when no bins are set)"""
+        nb = opts.nbins
+        if Phi is None:
+            return None
+        if nb == 0:
+            raise KromeError("Phi given but no photon bins (opts.set_photo)")
+        F = np.asarray(Phi, dtype=np.float64)
+        if F.ndim == 1:
+            F = np.tile(F, (ncell, 1))
+        F = np.ascontiguousarray(F)
+        if F.shape != (ncell, nb):
+            raise KromeError("Phi must be [nbins] or [ncell, nbins]")
+        return F
+
+    # -- photon bins (krome.h photo functions) -----------------------------
+
+    @property
+    def nphoto(self):
+        return self._lib.krome_nphoto()
+
+    def photo_reactions(self):
+        """[(reaction index, reactant index)] of the photo reactions"""
+        return [(self._lib.krome_photo_reaction(r),
+                 self._lib.krome_photo_reactant(r))
+                for r in range(self.nphoto)]
+
+    def photo_xsec(self, r, E):
+        """cross section [cm^2] of photo reaction r at energies E [eV] (0
+        outside its window)"""
+        E = np.atleast_1d(np.asarray(E, dtype=np.float64))
+        return np.array([self._lib.krome_photo_xsec(r, float(e))
+                         for e in E])
+
+    def bins_log(self, nbins, lower, upper):
+        """(Elo, Ehi) of nbins logarithmic bins in [lower, upper] eV"""
+        lo, hi = np.zeros(nbins), np.zeros(nbins)
+        if self._lib.krome_bins_log(nbins, lower, upper, _dptr(lo),
+                                    _dptr(hi)) != 0:
+            raise KromeError("invalid binning")
+        return lo, hi
+
+    def bins_lin(self, nbins, lower, upper):
+        """(Elo, Ehi) of nbins linear bins in [lower, upper] eV"""
+        lo, hi = np.zeros(nbins), np.zeros(nbins)
+        if self._lib.krome_bins_lin(nbins, lower, upper, _dptr(lo),
+                                    _dptr(hi)) != 0:
+            raise KromeError("invalid binning")
+        return lo, hi
+
     def _params(self, par, ncell):
         stride, P = 0, None
         if isinstance(par, dict):
@@ -610,6 +722,37 @@ class Options:
                                          ctypes.byref(v)) != 0:
             raise KromeError("unknown option %r" % key)
         return v.value
+
+    def set_photo(self, Elo, Ehi):
+        """photon bins [eV edges]; Phi[..., nbins] is then required by
+        solve() (the bins' cross sections are averaged once, here)"""
+        lo = np.ascontiguousarray(Elo, dtype=np.float64)
+        hi = np.ascontiguousarray(Ehi, dtype=np.float64)
+        if lo.shape != hi.shape or lo.ndim != 1:
+            raise KromeError("Elo and Ehi must be 1-D arrays of one shape")
+        if self._net._lib.krome_opts_set_photo(self._h, lo.size, _dptr(lo),
+                                               _dptr(hi)) != 0:
+            raise KromeError("invalid photon bins (need 0 < Elo < Ehi)")
+        return self
+
+    def set_photo_alpha(self, alpha):
+        """the in-bin photon spectrum dN/dE ~ E^-alpha (1: the same number
+        per ln E); rebuilds the bins already set"""
+        if self._net._lib.krome_opts_set_photo_alpha(self._h,
+                                                     float(alpha)) != 0:
+            raise KromeError("alpha must be finite")
+        return self
+
+    @property
+    def nbins(self):
+        return self._net._lib.krome_opts_nbins(self._h)
+
+    def photo_sigma(self):
+        """the bin-averaged cross sections [nphoto, nbins] [cm^2]"""
+        s = np.zeros((max(self._net.nphoto, 1), self.nbins))
+        if self._net._lib.krome_opts_photo_sigma(self._h, _dptr(s)) != 0:
+            raise KromeError("no photon bins")
+        return s[:self._net.nphoto]
 
     def close(self):
         if self._h is not None:

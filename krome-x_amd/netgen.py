@@ -60,14 +60,31 @@ CONSTANTS = {"boltzmann_eV": BOLTZMANN_EV, "boltzmann_erg": BOLTZMANN_ERG,
 # default values of the predefined parameters (kromex/knet.py:53-58)
 PARAM_DEFAULTS = {"gamma_ad": 5.0 / 3.0, "Tcmb": 2.725, "tff": math.nan,
                   "opratio": 3.0, "G0": 0.0, "d2g": 0.01, "Av": 0.0,
-                  "zredshift": 0.0}
+                  "zredshift": 0.0, "Tdust": 10.0}
 _ELEMENTS_BY_LEN = sorted(MASS_AMU, key=len, reverse=True)
 _SPECIES_RE = re.compile(r"^(?P<curly>\{)?(?P<core>[A-Za-z][A-Za-z0-9]*)"
                          r"(?P<labels>(?:_[A-Za-z0-9]+)*)(?P<charge>\++|-+)?"
                          r"(?P<curly2>\})?$")
-# the extern rate functions this build provides (kx_physics.h) and
-# whether they read the densities
-EXTERNS = {"dissH2_Martin96": True}
+# the extern functions this build provides (kx_physics.h): name ->
+# (the densities they read: None, "agg" for an aggregate (every species)
+# or one species; the parameters they read; their C++ call, with the
+# knet arguments as {0}, {1} ... and the e- index as {e}).  H2_sigmaLW is
+# a cross section, called only inside photo(xsec=...)
+EXTERNS = {
+    "dissH2_Martin96": ("agg", (), "kx::dissH2_Martin96(Tgas, Hnuclei(n))"),
+    "H2_solomonLW": (None, (), "kx::H2_solomonLW({0})"),
+    "H_recombination_on_dust": (
+        "e-", ("d2g", "G0"),
+        "kx::H_recombination_on_dust(Tgas, n[{e}], R(p.G0), R(p.d2g))"),
+    "He_recombination_on_dust": (
+        "e-", ("d2g", "G0"),
+        "kx::He_recombination_on_dust(Tgas, n[{e}], R(p.G0), R(p.d2g))"),
+    "H2_sigmaLW": (None, (), None),
+}
+# the predefined parameters in the order the parameter rows list them,
+# after the network's own (sorted): kromex/codegen.py:287-312
+UNIVERSAL_PARAMS = ("Tdust", "G0", "d2g", "Av", "gamma_ad", "Tcmb", "tff",
+                    "zredshift", "opratio")
 
 
 class KnetError(Exception):
@@ -447,6 +464,226 @@ def groups(net):
 
 
 # ---------------------------------------------------------------------------
+# photo reactions: photo(xsec=..., Emin=..., Emax=...)
+# (docs/network-format.md 5.4; the resolution of kromex/codegen.py:
+# 7530-7629 and its file readers, codegen.py:686-812)
+
+CLIGHT = 2.99792458e10
+PLANCK_EV = CONSTANTS["planck_eV"]
+# h c [eV Angstrom] and [eV nm]: the SWRI files tabulate the wavelength in
+# Angstrom, the Leiden files in nm (the same products as the reference)
+HC_EV_AA = CLIGHT * PLANCK_EV / 1e-8
+HC_EV_NM = CLIGHT * PLANCK_EV / 1e-7
+# the cross-section fits a database entry may name, host C++ (kx_photo_xsec.h)
+PHOTO_FITS = ("sigma_v96",)
+
+
+def is_photo(r):
+    return r.rate.replace(" ", "").startswith("photo(")
+
+
+def _dnum(t):
+    """a Fortran-style number (1.0d-3)"""
+    return float(t.strip().lower().replace("d", "e"))
+
+
+def photo_database(data_dir):
+    """the photoionization entries of data/database/photoionization.dat:
+    [{type, reacts, prods, limits, rate}] with species lists sorted and
+    the electron named e-"""
+    out, cur = [], None
+    path = Path(data_dir) / "database" / "photoionization.dat"
+    for line in path.read_text().splitlines():
+        t = line.strip()
+        if not t.startswith("@") or ":" not in t:
+            continue
+        key, val = (x.strip() for x in t[1:].split(":", 1))
+        if key == "type":
+            cur = {"type": val}
+            out.append(cur)
+        elif cur is None:
+            continue
+        elif key in ("reacts", "prods"):
+            cur[key] = sorted("e-" if x.strip().upper() == "E" else x.strip()
+                              for x in val.split(","))
+        elif key == "limits":
+            cur[key] = [_dnum(x) for x in val.split(",")]
+        elif key == "rate":
+            cur[key] = val
+    return out
+
+
+def leiden_xsec(data_dir, reactant, products):
+    """(E [eV] ascending, sigma [cm^2]) of a Leiden database file
+    <reactant>__<p1>_<p2>.dat: the photoionization column when an electron
+    is among the products, else photodissociation"""
+    prods = sorted(products)
+    folder = Path(data_dir) / "database" / "leiden_xsecs"
+    for f in sorted(folder.glob("%s__*.dat" % reactant)):
+        toks = f.stem.split("__", 1)[1].split("_")
+        if sorted("e-" if t.upper() == "E" else t for t in toks) == prods:
+            break
+    else:
+        raise KnetError("no Leiden cross section for %s -> %s"
+                        % (reactant, " + ".join(prods)))
+    col = 3 if "e-" in prods else 2
+    pts = []
+    for line in f.read_text().splitlines():
+        t = line.strip()
+        if t and not t.startswith("#"):
+            v = t.split()
+            pts.append((HC_EV_NM / float(v[0]), float(v[col])))
+    pts.sort()
+    return [e for e, _ in pts], [x for _, x in pts]
+
+
+def swri_xsec(data_dir, reactant, products):
+    """(E [eV] ascending, sigma [cm^2]) of the SWRI (phidrates) branch of
+    <reactant>.dat whose products match: excited-state labels folded onto
+    the ground state (columns summed); a run of values bracketed by the
+    1e-35 sentinel on both sides is a spectral line, kept as a box of its
+    mean height between the bracketing energies; other sentinels are
+    dropped; the table spans its first to last nonzero value"""
+    prods = sorted(products)
+    path = Path(data_dir) / "database" / "swri_xsecs" / ("%s.dat" % reactant)
+    if not path.exists():
+        raise KnetError("no SWRI file for %s" % reactant)
+    header, rows = [], []
+    for line in path.read_text().splitlines():
+        toks = line.split()
+        if not toks:
+            continue
+        if toks[0] == "Lambda":
+            header, rows = toks, []
+            continue
+        try:
+            rows.append([float(x) for x in toks])
+        except ValueError:
+            pass
+    for state in ("1D", "1S", "3P"):
+        header = [h.replace(state, "") for h in header]
+    merged = set()
+    for i in range(1, len(header)):
+        for j in range(i + 1, len(header)):
+            if header[j] == header[i] and j not in merged:
+                merged.add(j)
+                for row in rows:
+                    row[i] += row[j]
+    branches = [sorted("e-" if x.upper() == "E" else x
+                       for x in h.replace("+", "+/E/").split("/") if x)
+                for h in header]
+    if prods not in branches:
+        raise KnetError("no SWRI branch %s in %s" % ("+".join(prods),
+                                                     path.name))
+    col = branches.index(prods)
+    pts = [(HC_EV_AA / r[0], r[col]) for r in rows[::-1] if r[0] > 0.0]
+    sentinel = [v == 1e-35 for _, v in pts]
+    pairs, q = [], 0
+    while q < len(pts):
+        if sentinel[q]:
+            q += 1
+            continue
+        e = q
+        while e < len(pts) and not sentinel[e]:
+            e += 1
+        run = pts[q:e]
+        if q > 0 and e < len(pts):
+            lo, hi = pts[q - 1][0], pts[e][0]
+            h = sum(v for _, v in run) / len(run)
+            pairs += [(lo * (1.0 - 1e-9), 0.0), (lo, h), (hi, h),
+                      (hi * (1.0 + 1e-9), 0.0)]
+        else:
+            pairs += run
+        q = e
+    pairs.sort()
+    table = []
+    for e, v in pairs:
+        if not table or e != table[-1][0]:
+            table.append((e, v))
+    while table and table[0][1] == 0.0:
+        table.pop(0)
+    while table and table[-1][1] == 0.0:
+        table.pop()
+    return [e for e, _ in table], [v for _, v in table]
+
+
+def photo_reactions(net, data_dir):
+    """[{r, reactant, Emin, Emax, kind, code | table}] of the photo()
+    reactions in reaction order: kind "fit" (photo(xsec=auto): the
+    database's outer-shell fits, piecewise from each segment's threshold),
+    "table" (xsec="swri" / "leiden"), or "expr" (a function of E_eV,
+    extern cross sections included)"""
+    db = None
+    out = []
+    for r in net.reactions:
+        if not is_photo(r):
+            continue
+        call = ast.parse(r.rate.strip(), mode="eval").body
+        kw = {k.arg: k.value for k in call.keywords}
+        if "xsec" not in kw or call.args:
+            raise KnetError("line %d: photo() needs xsec=" % r.line)
+        kin = [s for s, k in r.reactants if k]
+        if len(kin) != 1:
+            raise KnetError("line %d: a photo reaction has one reactant"
+                            % r.line)
+        ent = {"r": r.index, "reactant": net.idx(kin[0]),
+               "Emin": None, "Emax": None}
+        for b in ("Emin", "Emax"):
+            if b in kw:
+                ent[b] = float(ast.literal_eval(kw[b]))
+        x = kw["xsec"]
+        if isinstance(x, ast.Name) and x.id == "auto":
+            if db is None:
+                db = photo_database(data_dir)
+            segs = sorted((e for e in db if e.get("type") == "photoion"
+                           and e.get("reacts") == sorted(kin)
+                           and e.get("prods") == sorted(r.products)),
+                          key=lambda e: e["limits"][0])
+            if not segs:
+                raise KnetError("line %d: no database cross section for %s"
+                                % (r.line, r.eq))
+            code = None
+            for e in segs:
+                m = re.match(r"\s*(\w+)\s*\((.*)\)\s*$", e["rate"])
+                if not m or m.group(1) not in PHOTO_FITS:
+                    raise KnetError("line %d: database fit %r is not "
+                                    "supported" % (r.line, e["rate"]))
+                args = [a.strip() for a in m.group(2).split(",")]
+                if args[0].lower() != "energy_ev":
+                    raise KnetError("line %d: unexpected fit arguments"
+                                    % r.line)
+                expr = "kx::%s(E_eV, %s)" % (m.group(1), ", ".join(
+                    repr(_dnum(a)) for a in args[1:]))
+                code = expr if code is None else "(E_eV >= %r ? %s : %s)" % (
+                    e["limits"][0], expr, code)
+            ent.update(kind="fit", code=code)
+            if ent["Emin"] is None:
+                ent["Emin"] = segs[0]["limits"][0]
+        elif isinstance(x, ast.Constant) and isinstance(x.value, str):
+            src = x.value.lower()
+            if src == "leiden":
+                tab = leiden_xsec(data_dir, kin[0], r.products)
+            elif src == "swri":
+                tab = swri_xsec(data_dir, kin[0], r.products)
+            else:
+                raise KnetError("line %d: cross-section source %r is not "
+                                "supported" % (r.line, x.value))
+            ent.update(kind="table", table=tab)
+            if ent["Emin"] is None:
+                ent["Emin"] = tab[0][0]
+            if ent["Emax"] is None:
+                ent["Emax"] = tab[0][-1]
+        else:
+            ent.update(kind="expr", ast=x)
+        if ent["Emin"] is None:
+            ent["Emin"] = 0.0
+        if ent["Emax"] is None:
+            ent["Emax"] = math.inf
+        out.append(ent)
+    return out
+
+
+# ---------------------------------------------------------------------------
 # expressions
 
 _N_CALL = re.compile(r"\bn\(\s*([^()\s]+)\s*\)")
@@ -509,7 +746,7 @@ class Emitter:
                 return True
             if any(nm.startswith("__n") for nm in e.names):
                 return True
-            if any(EXTERNS.get(c) for c in e.calls):
+            if any(EXTERNS[c][0] for c in e.calls if c in EXTERNS):
                 return True
         return False
 
@@ -523,8 +760,12 @@ class Emitter:
         for e in es:
             if e.names & {"ntot", "Hnuclei"}:
                 agg = True
-            if any(EXTERNS.get(c) for c in e.calls):
-                agg = True
+            for c in e.calls:
+                dep = EXTERNS[c][0] if c in EXTERNS else None
+                if dep == "agg":
+                    agg = True
+                elif dep is not None:
+                    direct.add(self.net.idx(dep))
             direct |= {int(nm[3:]) for nm in e.names if nm.startswith("__n")}
         return direct, agg
 
@@ -538,6 +779,8 @@ class Emitter:
                 return "v_" + nm
             if nm == "Tgas":
                 return "Tgas"
+            if nm == "E_eV":
+                return "E_eV"
             if nm == "Tgas_eV":
                 return "(Tgas * R(%r))" % BOLTZMANN_EV
             if nm == "ntot":
@@ -595,8 +838,11 @@ class Emitter:
                 return "((%s %s %s) ? %s : %s)" % (
                     self.c(cnd.left), op, self.c(cnd.comparators[0]),
                     self.c(args[1]), self.c(args[2]))
-            if f in EXTERNS:
-                return "kx::%s(Tgas, Hnuclei(n))" % f
+            if f in EXTERNS and EXTERNS[f][2] is not None:
+                has_e = any(s.name == "e-" for s in self.net.species)
+                return EXTERNS[f][2].format(
+                    *[self.c(a) for a in args],
+                    e=self.net.idx("e-") if has_e else -1)
             raise KnetError("unknown function %r" % f)
         raise KnetError("unsupported syntax %s" % ast.dump(node))
 
@@ -651,6 +897,9 @@ class Thermal:
     heating: tuple = ()
     gamma: str = "CONST"
     h2_opacity: str = ""
+    # Tgas may be evolved even with no cooling or heating term (dT/dt = 0,
+    # as the reference's thermal ODE gives with empty lists)
+    evolve: bool = False
 
     def __post_init__(self):
         for c in self.cooling:
@@ -666,21 +915,27 @@ class Thermal:
 
 
 def param_list(net, th):
-    """the parameter rows' order: gamma_ad (CONST gamma), Tcmb, the
-    configured processes' (tff for COMPRESS), opratio (EXACT gamma with
-    H2), then the network's own params"""
-    out = []
+    """the parameter rows' order (codegen.py:3408-3420): the network's own
+    params sorted, then the predefined ones in use in their fixed order:
+    gamma_ad (CONST gamma), Tcmb (always), tff (COMPRESS), opratio (EXACT
+    gamma with H2), those the externs read (G0, d2g for the dust
+    recombination fits) and those a rate or var names"""
+    used = {"Tcmb"}
     if th.gamma == "CONST":
-        out.append("gamma_ad")
-    out.append("Tcmb")
+        used.add("gamma_ad")
     if "COMPRESS" in th.heating:
-        out.append("tff")
+        used.add("tff")
     names = {s.name for s in net.species}
     if th.gamma == "EXACT" and "H2" in names:
-        out.append("opratio")
-    for p in net.params:
-        if p not in out:
-            out.append(p)
+        used.add("opratio")
+    for e in net.externs:
+        used |= set(EXTERNS[e][1])
+    words = set()
+    for t in list(net.vars.values()) + [r.rate for r in net.reactions]:
+        words |= set(re.findall(r"[A-Za-z_][A-Za-z0-9_]*", t))
+    used |= words & set(UNIVERSAL_PARAMS)
+    out = sorted(net.params)
+    out += [u for u in UNIVERSAL_PARAMS if u in used and u not in out]
     return out
 
 
@@ -695,10 +950,17 @@ def _num(x):
     return repr(float(x))
 
 
-def generate(knet_path, thermal=None, name=None):
+def generate(knet_path, thermal=None, name=None, data_dir=None):
     """C++ header text of a network (+ its thermal configuration) and a
-    metadata dict (species, masses, params) for the host side"""
+    metadata dict (species, masses, params) for the host side; data_dir
+    holds the cross-section database and files of photo() reactions
+    (default: the data/ folder beside the network's folder)"""
     net = parse_knet(knet_path)
+    if data_dir is None:
+        data_dir = Path(knet_path).resolve().parent.parent / "data"
+    photo = photo_reactions(net, data_dir)
+    photo_of = {e["r"]: q for q, e in enumerate(photo)}
+    NPH = len(photo)
     loop_only = net.nrea > RHS_LOOP_MIN
     th = thermal or Thermal()
     pnames = param_list(net, th)
@@ -722,7 +984,7 @@ def generate(knet_path, thermal=None, name=None):
     has_e = any(s.name == "e-" for s in net.species)
     w("  static constexpr bool HAS_E = %s;" % ("true" if has_e else "false"))
     w("  static constexpr int I_E = %d;" % (net.idx("e-") if has_e else -1))
-    thermal_on = bool(th.cooling or th.heating)
+    thermal_on = bool(th.cooling or th.heating or th.evolve)
     w("  static constexpr bool THERMAL = %s;"
       % ("true" if thermal_on else "false"))
     hnames, cnames = thermal_term_names(th)
@@ -731,20 +993,32 @@ def generate(knet_path, thermal=None, name=None):
     # n-dependent rates -> no analytic Jacobian
     grp = groups(net)
     dens_groups = [g for g in grp
-                   if any(em.density_dependent(Expr(r.rate, net)) for r in g)]
+                   if any(not is_photo(r)
+                          and em.density_dependent(Expr(r.rate, net))
+                          for r in g)]
     w("  static constexpr bool RATES_DENSITY = %s;"
       % ("true" if dens_groups else "false"))
-    w("  // parameter row (param_names order)")
+    w("  // photo reactions (photo(xsec=...)): their rates are the bin sums")
+    w("  // sum_b sigma_rb Phi_b of the cell's field, which the kernels read")
+    w("  // after the parameters, in the row's last NPHOTO slots")
+    w("  static constexpr int NPHOTO = %d;" % NPH)
+    w("  static constexpr int NROW = NPAR + NPHOTO;")
+    w("  // parameter row (param_names order), then the photo sums")
     w("  struct Params {")
     for p in pnames:
         w("    double %s;" % p)
+    if NPH:
+        w("    double ph[NPHOTO];")
     w("  };")
     w("  KX_HD static Params params_from_row(const double* row) {")
     w("    Params p;")
     for i, p in enumerate(pnames):
         w("    p.%s = row[%d];" % (p, i))
+    if NPH:
+        w("    for (int r = 0; r < NPHOTO; ++r) p.ph[r] = row[NPAR + r];")
     w("    return p;")
     w("  }")
+    _emit_photo(w, net, em, photo)
     req = [i for i, p in enumerate(pnames)
            if math.isnan(net.params.get(p, PARAM_DEFAULTS.get(p, 0.0)))]
     w("  // index of the first REQUIRED parameter still NaN, or -1")
@@ -841,6 +1115,10 @@ def generate(knet_path, thermal=None, name=None):
     # rates
     def emit_group(g, indent):
         out = []
+        if len(g) == 1 and g[0].index in photo_of:
+            r = g[0]
+            return ["%sk[%d] = R(p.ph[%d]);  // %s (photo)"
+                    % (indent, r.index, photo_of[r.index], r.eq)]
         ex = [Expr(r.rate, net) for r in g]
         if not g[0].ranged:
             r = g[0]
@@ -1032,9 +1310,63 @@ def generate(knet_path, thermal=None, name=None):
             "param_defaults": [net.params.get(p, PARAM_DEFAULTS.get(p))
                                for p in pnames],
             "nrea": NR, "thermal": thermal_on,
+            "photo": [{"reaction": e["r"], "reactant": e["reactant"],
+                       "Emin": e["Emin"], "Emax": e["Emax"]}
+                      for e in photo],
             "reactions": [r.eq for r in net.reactions],
             "rates_density": bool(dens_groups)}
     return "\n".join(L) + "\n", meta
+
+
+def _emit_photo(w, net, em, photo):
+    """the photo reactions' tables and cross sections: photo_reaction,
+    photo_reactant, the energy window [photo_Emin, photo_Emax] [eV], and
+    photo_xsec(q, E) [cm^2], a host function for the bins' set-up"""
+    n = max(len(photo), 1)
+
+    def arr(typ, nm, vals, fmt):
+        vals = list(vals) or [0]
+        w("  static constexpr %s %s[%d] = {%s};"
+          % (typ, nm, n, ", ".join(fmt(v) for v in vals)))
+    arr("int", "photo_reaction", (e["r"] for e in photo), str)
+    arr("int", "photo_reactant", (e["reactant"] for e in photo), str)
+    arr("double", "photo_Emin", (e["Emin"] for e in photo), _num)
+    arr("double", "photo_Emax", (e["Emax"] for e in photo), _num)
+    for q, e in enumerate(photo):
+        if e["kind"] != "table":
+            continue
+        E, S = e["table"]
+        w("  // tabulated cross section of %s: %d points, %.4g-%.4g eV"
+          % (net.reactions[e["r"]].eq, len(E), E[0], E[-1]))
+        for nm, vals in (("E", E), ("S", S)):
+            w("  static constexpr double PX%d_%s[%d] = {" % (q, nm, len(vals)))
+            for i in range(0, len(vals), 6):
+                w("    " + ", ".join(repr(float(v)) for v in vals[i:i + 6])
+                  + ",")
+            w("  };")
+    w("  // cross section [cm^2] of photo reaction q at photon energy E_eV")
+    w("  static double photo_xsec(int q, double E_eV) {")
+    w("    using R = double;")
+    w("    switch (q) {")
+    for q, e in enumerate(photo):
+        if e["kind"] == "fit":
+            code = e["code"]
+        elif e["kind"] == "table":
+            code = "kx::xsec_interp(E_eV, PX%d_E, PX%d_S, %d)" % (
+                q, q, len(e["table"][0]))
+        else:
+            x = e["ast"]
+            if (isinstance(x, ast.Call) and isinstance(x.func, ast.Name)
+                    and x.func.id in EXTERNS
+                    and EXTERNS[x.func.id][2] is None):
+                code = "kx::%s(%s)" % (x.func.id, em.c(x.args[0]))
+            else:
+                code = em.c(x)
+        w("      case %d: return %s;  // %s"
+          % (q, code, net.reactions[e["r"]].eq))
+    w("      default: return R(0) * E_eV;")
+    w("    }")
+    w("  }")
 
 
 def _emit_thermal(w, net, th, pnames):
@@ -1225,6 +1557,10 @@ CONFIGS = {
                     Thermal(cooling=("H2", "CONT", "CHEM"),
                             heating=("COMPRESS", "CHEM"), gamma="EXACT",
                             h2_opacity="RIPAMONTI")),
+    # photochemistry: 8 photo() reactions over photon bins, the Solomon
+    # rate and grain-assisted recombination; the default thermal options
+    # (no cooling or heating term), Tgas evolved
+    "galaxy_ism": ("galaxy_ism.knet", Thermal(evolve=True)),
 }
 
 

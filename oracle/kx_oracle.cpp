@@ -14,6 +14,8 @@
 //   solve_cells_percell   runtime/core/solve_cells.hpp:47-118, 130-161
 //   conservation          oracle/kx_conserve.hpp (element totals, the
 //                         post-solve mode 2, consistent advection)
+//   photon bins           oracle/kx_photo.hpp (PhotoModule::init, the
+//                         photo rates as bin sums in the parameter rows)
 // over the networks of krome-x_amd/csrc/generated, with the integrator of
 // oracle/kx_bdf.hpp, or with -DKX_REF_BDF the reference's own
 // runtime/core/cvode/bdf_cell.hpp (oracle/Makefile target ref).
@@ -27,7 +29,9 @@
 
 #include "net_primordial.h"
 #include "net_primordial3.h"
+#include "net_galaxy_ism.h"
 #include "kx_conserve.hpp"
+#include "kx_photo.hpp"
 
 #ifdef KX_REF_BDF
 #include "bdf_cell.hpp"
@@ -248,7 +252,7 @@ void funcs(int ncell, const double* n, const double* T, const double* par,
   constexpr int NS = Net::NSPEC;
   for (int c = 0; c < ncell; ++c) {
     Problem<Net> P;
-    P.p = Net::params_from_row(par + static_cast<size_t>(c) * Net::NPAR);
+    P.p = Net::params_from_row(par + static_cast<size_t>(c) * Net::NROW);
     P.evolve_T = evolve_T != 0;
     P.Tfix = T[c];
     double y[NS + 1];
@@ -265,7 +269,7 @@ void jacs(int ncell, const double* n, const double* T, const double* par,
   constexpr int NS = Net::NSPEC;
   for (int c = 0; c < ncell; ++c) {
     Problem<Net> P;
-    P.p = Net::params_from_row(par + static_cast<size_t>(c) * Net::NPAR);
+    P.p = Net::params_from_row(par + static_cast<size_t>(c) * Net::NROW);
     P.Tfix = T[c];
     double* Jc = J + static_cast<size_t>(c) * NS * NS;
     for (int k = 0; k < NS * NS; ++k) Jc[k] = 0.0;
@@ -285,7 +289,7 @@ void forms(int ncell, const double* n, const double* T, const double* par,
   const size_t nj = nf * NS;
   for (int c = 0; c < ncell; ++c) {
     Problem<Net> P;
-    P.p = Net::params_from_row(par + static_cast<size_t>(c) * Net::NPAR);
+    P.p = Net::params_from_row(par + static_cast<size_t>(c) * Net::NROW);
     P.Tfix = T[c];
     const double* y = n + static_cast<size_t>(c) * NS;
     double k[Net::NREA + 1];
@@ -331,7 +335,27 @@ int KXO_(get_threads)(void) {
 int KXO_(network)(const char* name) {
   if (!std::strcmp(name, "primordial")) return 0;
   if (!std::strcmp(name, "primordial3")) return 1;
+  if (!std::strcmp(name, "galaxy_ism")) return 2;
   return -1;
+}
+
+// the bin-averaged cross sections sigma[NPHOTO B] of the bins [Elo, Ehi]
+// with the in-bin spectrum E^-alpha (PhotoModule::init)
+int KXO_(photo_bins)(int net, int B, const double* Elo, const double* Ehi,
+                     double alpha, double* sigma) {
+  if (net != 2) return -1;
+  kxo_photo::bins<kx::net_galaxy_ism>(B, Elo, Ehi, alpha, sigma);
+  return 0;
+}
+
+// the rows the solve reads for a photo network: par[ncell][NPAR] (stride
+// pst, 0 for one row) followed by the bin sums of Phi[ncell][B]
+int KXO_(photo_rows)(int net, int ncell, const double* par, int pst,
+                     const double* Phi, int B, const double* sigma,
+                     double* out) {
+  if (net != 2) return -1;
+  kxo_photo::rows<kx::net_galaxy_ism>(ncell, par, pst, Phi, B, sigma, out);
+  return 0;
 }
 
 // krome_solve_cells semantics over per-cell parameter rows (par_stride 0:
@@ -346,6 +370,9 @@ int KXO_(solve_cells)(int net, int ncell, double* n, double* T,
     case 1:
       return solve_cells<kx::net_primordial3>(ncell, n, T, dt, par,
                                               par_stride, opts, flags, stats);
+    case 2:
+      return solve_cells<kx::net_galaxy_ism>(ncell, n, T, dt, par,
+                                             par_stride, opts, flags, stats);
   }
   return -1;
 }
@@ -359,6 +386,9 @@ int KXO_(funcs)(int net, int ncell, const double* n, const double* T,
       return 0;
     case 1:
       funcs<kx::net_primordial3>(ncell, n, T, par, evolve_T, k, f);
+      return 0;
+    case 2:
+      funcs<kx::net_galaxy_ism>(ncell, n, T, par, evolve_T, k, f);
       return 0;
   }
   return -1;
@@ -409,6 +439,7 @@ int KXO_(totals)(int net, int x, int ncell, const double* v, double* tot) {
   switch (net) {
     case 0: return run(kx::net_primordial{});
     case 1: return run(kx::net_primordial3{});
+    case 2: return run(kx::net_galaxy_ism{});
   }
   return -1;
 }
@@ -425,6 +456,9 @@ int KXO_(conserve)(int net, int x, int ncell, double* v, const double* ref,
     case 1:
       return kxo_cons::cells<kx::net_primordial3>(ncell, v, ref, ref_rows,
                                                   x != 0, flags);
+    case 2:
+      return kxo_cons::cells<kx::net_galaxy_ism>(ncell, v, ref, ref_rows,
+                                                 x != 0, flags);
   }
   return -1;
 }

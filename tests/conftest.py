@@ -15,7 +15,7 @@ sys.path.insert(0, str(PKG))
 OPT_ORDER = ("evolve_T", "rtol", "atol_n", "atol_T", "linsol", "jac",
              "max_steps", "max_err_fails", "max_conv_fails", "max_restarts",
              "clip_negative", "conserve")
-NET_ID = {"primordial": 0, "primordial3": 1}
+NET_ID = {"primordial": 0, "primordial3": 1, "galaxy_ism": 2}
 
 
 def pytest_configure(config):
@@ -69,6 +69,35 @@ class Oracle:
             np.ascontiguousarray(par).ctypes.data_as(D), int(evolve_T),
             k.ctypes.data_as(D), f.ctypes.data_as(D))
         return k, f
+
+    def photo_bins(self, net, elo, ehi, alpha=1.0):
+        """the bin-averaged cross sections [NPHOTO, B] of the bins"""
+        D = ctypes.POINTER(ctypes.c_double)
+        elo = np.ascontiguousarray(elo, dtype=np.float64)
+        ehi = np.ascontiguousarray(ehi, dtype=np.float64)
+        nph = len(__import__("cells").network_meta(net)["photo"])
+        sig = np.zeros((nph, elo.size))
+        r = getattr(self.lib, self.p + "photo_bins")(
+            NET_ID[net], elo.size, elo.ctypes.data_as(D),
+            ehi.ctypes.data_as(D), ctypes.c_double(alpha),
+            sig.ctypes.data_as(D))
+        assert r == 0
+        return sig
+
+    def photo_rows(self, net, par, Phi, sigma):
+        """the solve's rows of a photo network: par then the bin sums"""
+        D = ctypes.POINTER(ctypes.c_double)
+        par = np.ascontiguousarray(par, dtype=np.float64)
+        Phi = np.ascontiguousarray(Phi, dtype=np.float64)
+        sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+        nc = Phi.shape[0]
+        out = np.zeros((nc, par.shape[-1] + sigma.shape[0]))
+        r = getattr(self.lib, self.p + "photo_rows")(
+            NET_ID[net], nc, par.ctypes.data_as(D),
+            par.shape[1] if par.ndim == 2 else 0, Phi.ctypes.data_as(D),
+            Phi.shape[1], sigma.ctypes.data_as(D), out.ctypes.data_as(D))
+        assert r == 0
+        return out
 
     def totals(self, net, v, x=False):
         D = ctypes.POINTER(ctypes.c_double)

@@ -55,6 +55,8 @@ TOML = {
                    'cooling = ["H2", "CONT", "CHEM"]\n'
                    'heating = ["COMPRESS", "CHEM"]\ngamma = "EXACT"\n'
                    'h2_opacity = "RIPAMONTI"\n',
+    "galaxy_ism": '[project]\nnetwork = "{net}/galaxy_ism.knet"\n'
+                  'outdir = "{out}"\n',
 }
 # cells of the solve fixtures (the first ones of the seeded workload)
 NSOLVE = {"primordial": 1024, "primordial3": 256}
@@ -290,10 +292,109 @@ def make_sparse(name, exe):
         name, nnz, raw[1], np.unique(f), st[:, 0].sum()))
 
 
+# photon bins: cells of the photo solve fixture, the energies at which the
+# cross sections are sampled, the in-bin spectra of the bin tables
+NPHOTO_SOLVE = 256
+PHOTO_ALPHAS = (1.0, 0.0, 2.0)
+
+
+def photo_energies(meta):
+    """a log grid over 0.3..3e4 eV and each photo window's edges with
+    their neighbours"""
+    E = list(np.geomspace(0.3, 3e4, 3000))
+    for ph in meta["photo"]:
+        for e in (ph["Emin"], ph["Emax"]):
+            if np.isfinite(e):
+                E += [e * (1 - 1e-9), e, e * (1 + 1e-9)]
+    return np.array(sorted(E))
+
+
+def run_photo(exe, mode, w, tag, alpha, E):
+    """ref_driver pbins / prates / psolve over w's bins and fields"""
+    fin = BUILD / ("in_%s.bin" % tag)
+    fout = BUILD / ("out_%s.bin" % tag)
+    write_in(fin, w)
+    elo, ehi = w["bins"]
+    with open(fin, "ab") as f:
+        np.array([len(elo)], dtype=np.int64).tofile(f)
+        np.array([alpha], dtype=np.float64).tofile(f)
+        for a in (elo, ehi, w["Phi"]):
+            np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+        np.array([len(E)], dtype=np.int64).tofile(f)
+        np.ascontiguousarray(E, dtype=np.float64).tofile(f)
+    env = dict(os.environ, OMP_NUM_THREADS=str(os.cpu_count() or 1))
+    subprocess.run([str(exe), mode, str(fin), str(fout)], check=True,
+                   env=env)
+    return np.fromfile(fout, dtype=np.float64)
+
+
+def make_photo(name, exes, meta):
+    """galaxy_ism's photon bins: the reference's cross sections, bin tables
+    (three in-bin spectra), rates with the fields, and solves"""
+    exe = exes["bitwise"]
+    w = cells.WORKLOADS[name](NPHOTO_SOLVE)
+    E = photo_energies(meta)
+    B = len(w["bins"][0])
+    out = {"E": E, "Elo": w["bins"][0], "Ehi": w["bins"][1]}
+    for alpha in PHOTO_ALPHAS:
+        raw = run_photo(exe, "pbins", w, name + "_pbins", alpha, E)
+        nph = len(meta["photo"])
+        o = nph * B
+        out["sigma_a%g" % alpha] = raw[:o].reshape(nph, B)
+        out["xsec"] = raw[o:o + nph * len(E)].reshape(nph, len(E))
+        o += nph * len(E)
+        tail = raw[o:]
+        assert int(tail[0]) == nph
+        out["photo_reaction"] = tail[1:1 + nph].astype(np.int64)
+        out["photo_reactant"] = tail[1 + nph:1 + 2 * nph].astype(np.int64)
+        out["photo_Emin"] = tail[1 + 2 * nph:1 + 3 * nph]
+        out["photo_Emax"] = tail[1 + 3 * nph:1 + 4 * nph]
+    # rates with the fields: the workload's cells and the T grid
+    fw = func_samples(name)
+    fw["bins"] = w["bins"]
+    nf = fw["n"].shape[0]
+    fw["Phi"] = np.concatenate([w["Phi"][:64],
+                                np.resize(w["Phi"], (nf - 64, B))])
+    raw = run_photo(exe, "prates", fw, name + "_prates", 1.0, E)
+    out.update(fn=fw["n"], fT=fw["T"], fpar=fw["par"], fPhi=fw["Phi"],
+               k=raw.reshape(nf, meta["nrea"]))
+    opts = np.array([float(w["opts"][k]) for k in OPT_KEYS])
+    for m, ex in exes.items():
+        raw = run_photo(ex, "psolve", w, "%s_psolve_%s" % (name, m), 1.0, E)
+        nc, ns = w["n"].shape
+        o = 0
+        out["n_" + m] = raw[o:o + nc * ns].reshape(nc, ns)
+        o += nc * ns
+        out["T_" + m] = raw[o:o + nc]
+        o += nc
+        out["flag_" + m] = raw[o:o + nc].astype(np.int32)
+        o += nc
+        out["stats_" + m] = raw[o:o + nc * 9].reshape(nc, 9).astype(np.int64)
+        print("%s psolve %s: %d cells, flags %s, steps %d..%d" % (
+            name, m, nc, np.unique(out["flag_" + m]),
+            out["stats_" + m][:, 0].min(), out["stats_" + m][:, 0].max()))
+    np.savez_compressed(HERE / ("%s_photo.npz" % name), n0=w["n"], T0=w["T"],
+                        dt=w["dt"], par=w["par"], Phi=w["Phi"], opts=opts,
+                        **out)
+
+
 def main(argv=None):
-    parts = set(argv if argv else ("solve", "funcs", "conserve", "sparse"))
+    parts = set(argv if argv else ("solve", "funcs", "conserve", "sparse",
+                                   "photo"))
     BUILD.mkdir(parents=True, exist_ok=True)
+    if "photo" in parts:
+        name = "galaxy_ism"
+        proj = generate(name)
+        exes = {m: compile_driver(proj, name, m)
+                for m in ("bitwise", "production")}
+        meta = cells.network_meta(name)
+        make_photo(name, exes, meta)
+        w = cells.WORKLOADS[name](64)
+        np.savez_compressed(HERE / ("%s_tables.npz" % name),
+                            **ref_tables(exes["bitwise"], name, w))
     for name in ("primordial", "primordial3"):
+        if not parts & {"solve", "funcs", "conserve", "sparse"}:
+            break
         meta = cells.network_meta(name)
         proj = generate(name)
         exes = {m: compile_driver(proj, name, m)

@@ -36,6 +36,17 @@
 //                               sp_di[NSPEC] (f64)
 //   (solve: opts[4] 1 dense, 3 sparse, else auto; opts[7], when given,
 //   is SolverOpts::conserve)
+//   ref_driver pbins|prates|psolve IN OUT   photon bins (networks with
+//                               photo reactions): IN as above followed by
+//                               i64 B; f64 alpha, Elo[B], Ehi[B],
+//                               Phi[ncell B]; i64 NE; f64 E[NE].  pbins
+//                               OUT: the PhotoModule's bin-averaged
+//                               sigma[NPHOTO B], photo_xsec[r](E)
+//                               [NPHOTO NE], NPHOTO, photo_reaction,
+//                               photo_reactant, photo_Emin, photo_Emax;
+//                               prates OUT: krome::get_rates with the
+//                               cell's field, k[ncell NREA]; psolve OUT:
+//                               as solve, with the bins and Phi
 #include "krome.hpp"
 
 #include <chrono>
@@ -243,6 +254,116 @@ int do_totals(const In& in, const char* out) {
   return 0;
 }
 
+// the photon bins and fields after the standard input
+struct PhotoIn {
+  int64_t B = 0, NE = 0;
+  double alpha = 1.0;
+  std::vector<double> Elo, Ehi, Phi, E;
+};
+
+bool read_photo(const char* path, const In& in, PhotoIn& ph) {
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return false;
+  const long skip = sizeof(int64_t) * 4 +
+      sizeof(double) * (in.nopt + in.ncell * in.nspec + 2 * in.ncell +
+                        in.ncell * in.npar);
+  bool ok = std::fseek(f, skip, SEEK_SET) == 0 &&
+            std::fread(&ph.B, sizeof(int64_t), 1, f) == 1 &&
+            std::fread(&ph.alpha, sizeof(double), 1, f) == 1;
+  auto rd = [&](std::vector<double>& v, size_t m) {
+    v.resize(m);
+    return std::fread(v.data(), sizeof(double), m, f) == m;
+  };
+  ok = ok && rd(ph.Elo, ph.B) && rd(ph.Ehi, ph.B) &&
+       rd(ph.Phi, in.ncell * ph.B) &&
+       std::fread(&ph.NE, sizeof(int64_t), 1, f) == 1 && rd(ph.E, ph.NE);
+  std::fclose(f);
+  return ok;
+}
+
+krome::PhotoModule photo_module(const PhotoIn& ph) {
+  krome::Radiation rad;
+  rad.Elo = ph.Elo;
+  rad.Ehi = ph.Ehi;
+  rad.Phi.assign(ph.B, 0.0);
+  rad.alpha = ph.alpha;
+  krome::PhotoModule pm;
+  pm.init(rad);
+  return pm;
+}
+
+int do_photo(const char* mode, const In& in, const PhotoIn& ph,
+             const char* out) {
+  if (in.nspec != krome::NSPEC || in.npar != krome::NPAR) return 2;
+  const krome::PhotoModule pm = photo_module(ph);
+  FILE* f = std::fopen(out, "wb");
+  if (!f) return 3;
+  std::vector<krome::Params> P(in.ncell);
+  for (int64_t c = 0; c < in.ncell; ++c)
+    krome::params_from_row(in.par.data() + c * in.npar, P[c]);
+  if (!std::strcmp(mode, "pbins")) {
+    std::fwrite(pm.sigma.data(), sizeof(double), pm.sigma.size(), f);
+    for (int r = 0; r < krome::NPHOTO; ++r)
+      for (int64_t e = 0; e < ph.NE; ++e) {
+        const double v = krome::photo_xsec[r](ph.E[e]);
+        std::fwrite(&v, sizeof(double), 1, f);
+      }
+    std::vector<double> v = {double(krome::NPHOTO)};
+    for (int r = 0; r < krome::NPHOTO; ++r) v.push_back(photo_reaction[r]);
+    for (int r = 0; r < krome::NPHOTO; ++r) v.push_back(photo_reactant[r]);
+    for (int r = 0; r < krome::NPHOTO; ++r) v.push_back(photo_Emin[r]);
+    for (int r = 0; r < krome::NPHOTO; ++r) v.push_back(photo_Emax[r]);
+    std::fwrite(v.data(), sizeof(double), v.size(), f);
+  } else if (!std::strcmp(mode, "prates")) {
+    krome::SolverOpts so = solver_opts(in);
+    so.photo = &pm;
+    std::vector<double> phi(ph.B), k(krome::NREA);
+    for (int64_t c = 0; c < in.ncell; ++c) {
+      for (int64_t b = 0; b < ph.B; ++b) phi[b] = ph.Phi[c * ph.B + b];
+      so.Phi = phi.data();
+      if (krome::get_rates(in.n0.data() + c * in.nspec, in.T[c], P[c], so,
+                           k.data()) != 0)
+        return 4;
+      std::fwrite(k.data(), sizeof(double), krome::NREA, f);
+    }
+  } else {
+    krome::SolverOpts so = solver_opts(in);
+    so.photo = &pm;
+    krome::CellsOpts co;
+    co.path = krome::PATH_PERCELL;
+    std::vector<double> Phi = ph.Phi;
+    co.Phi = Phi.data();
+    std::vector<int> flags(in.ncell, 0);
+    std::vector<krome::SolveStats> st(in.ncell);
+    co.flags = flags.data();
+    co.stats = st.data();
+    std::vector<double> n = in.n0, T = in.T;
+    const auto t0 = std::chrono::steady_clock::now();
+    krome::solve_cells(static_cast<int>(in.ncell), n.data(), T.data(),
+                       in.dt.data(), P.data(), so, co);
+    const double secs = std::chrono::duration<double>(
+                            std::chrono::steady_clock::now() - t0).count();
+    std::fwrite(n.data(), sizeof(double), n.size(), f);
+    std::fwrite(T.data(), sizeof(double), T.size(), f);
+    for (int64_t c = 0; c < in.ncell; ++c) {
+      const double fl = flags[c];
+      std::fwrite(&fl, sizeof(double), 1, f);
+    }
+    for (int64_t c = 0; c < in.ncell; ++c) {
+      const krome::SolveStats& s = st[c];
+      const double v[9] = {double(s.steps), double(s.rhs_evals),
+                           double(s.jac_evals), double(s.rhs_evals_jac),
+                           double(s.lin_setups), double(s.newton_iters),
+                           double(s.newton_fails), double(s.err_fails),
+                           double(s.restarts)};
+      std::fwrite(v, sizeof(double), 9, f);
+    }
+    std::fwrite(&secs, sizeof(double), 1, f);
+  }
+  std::fclose(f);
+  return 0;
+}
+
 int do_conserve(const In& in, const std::vector<double>& ref,
                 const char* out) {
   if (in.nspec != krome::NSPEC || in.npar != krome::NPAR) return 2;
@@ -287,6 +408,12 @@ int main(int argc, char** argv) {
   if (!std::strcmp(argv[1], "totals")) return do_totals(in, argv[3]);
   if (!std::strcmp(argv[1], "meta")) return do_meta(argv[3]);
   if (!std::strcmp(argv[1], "sparse")) return do_sparse(argv[3]);
+  if (!std::strcmp(argv[1], "pbins") || !std::strcmp(argv[1], "prates") ||
+      !std::strcmp(argv[1], "psolve")) {
+    PhotoIn ph;
+    if (!read_photo(argv[2], in, ph)) return 1;
+    return do_photo(argv[1], in, ph, argv[3]);
+  }
   if (!std::strcmp(argv[1], "conserve")) {
     // ref[ncell NELEM] follows the standard input in the same file
     FILE* f = std::fopen(argv[2], "rb");
