@@ -6,7 +6,7 @@ gamma EXACT, H2 opacity RIPAMONTI; rtol 1e-8, atol_n 1e-20, atol_T 1e-6).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--ncell N]
                     [--kernel persistent|plain] [--fp64]
-                    [--workload solve|conserve|multi|copy]
+                    [--workload solve|conserve|multi|photo|copy]
                     [--linsol dense|sparse]
 
 A step is one solve of the whole batch from its initial state: the inputs
@@ -34,6 +34,17 @@ primordial cells of the uneven batch (seed 2, density 1..1e12 cm^-3, dt
 1e8..1e14 s, Tgas fixed, rtol 1e-6) solved by one launch, against the
 same two batches solved by two launches of the networks' own libraries.
 Mixed precision unless --fp64, as for the solve workload.
+
+--workload photo measures the photochemistry network galaxy_ism with
+photon bins: 2^18 cells (seed 4: nH 0.1..1e5 cm^-3, Tgas 10..1e4 K,
+dt 3.15e11 s, Tgas evolved, rtol 1e-8), 32 logarithmic bins over
+0.7..200 eV and a field per cell (krome-x_amd/cells.py galaxy_ism).  Each
+step restores the cells and solves them through krome_solve_cells_device
+with the fields on the device (the photo sums kernel, then the solve);
+the plain FP64 kernel, the persistent FP64 kernel and the persistent
+mixed-precision kernel are timed in turn, with the error of each per
+species against the plain FP64 result and, on a sample, against the
+oracle; the oracle on the host's cores is the CPU baseline.
 
 --workload copy is the calibration of the HBM traffic counters: one
 device-to-device copy of 2 GiB (far above the last-level cache), whose
@@ -313,6 +324,119 @@ def multi_bench(a):
     print(json.dumps(out))
 
 
+def photo_bench(a):
+    """galaxy_ism with photon bins: three kernels, one JSON line"""
+    import torch
+    import cells
+    import krome
+    sys.path.insert(0, str(REPO / "tests"))
+
+    dev = torch.device("cuda:0")
+    net = krome.load("galaxy_ism")
+    nc = a.ncell
+    w = cells.galaxy_ism_cells(nc)
+    elo, ehi = w["bins"]
+    n0 = torch.tensor(np.ascontiguousarray(w["n"].T), device=dev)
+    T0 = torch.tensor(w["T"], device=dev)
+    dt = torch.tensor(w["dt"], device=dev)
+    par = torch.tensor(np.ascontiguousarray(w["par"]), device=dev)
+    Phi = torch.tensor(np.ascontiguousarray(w["Phi"]), device=dev)
+    n, T = n0.clone(), T0.clone()
+    fl = torch.zeros(nc, dtype=torch.int32, device=dev)
+    kms = ctypes.c_double()
+    kp = ctypes.cast(ctypes.byref(kms), ctypes.POINTER(ctypes.c_double))
+    runs = {}
+    for key, kernel, mixed in (("plain_fp64", "plain", 0),
+                               ("persistent_fp64", "persistent", 0),
+                               ("persistent_mixed", "persistent", 1)):
+        o = net.options(evolve_T=1, rtol=1e-8, atol_n=1e-20, atol_T=1e-6,
+                        linsol=1, device_kernel=kernel,
+                        mixed_precision=mixed,
+                        error_report=False).set_photo(elo, ehi)
+        for _ in range(max(a.warmup, 1)):
+            n.copy_(n0)
+            T.copy_(T0)
+            net.solve_device(n, T, dt, par, opts=o, flags=fl, Phi=Phi)
+        ms = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            n.copy_(n0)
+            T.copy_(T0)
+            r = net.solve_device(n, T, dt, par, opts=o, flags=fl, Phi=Phi)
+            if r != 0 or net._lib.krome_device_last_kernel_ms(kp) != 0:
+                raise SystemExit("%s solve returned %d" % (key, r))
+            ms.append(kms.value)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        runs[key] = {"cells_per_s": nc * a.steps / wall,
+                     "ms_per_step": 1e3 * wall / a.steps,
+                     "kernel_ms": float(np.mean(ms)),
+                     "n": n.cpu().numpy().T.copy(),
+                     "flags": fl.cpu().numpy().copy()}
+        print("photo %s: %.1f ms/step" % (key, runs[key]["ms_per_step"]),
+              file=sys.stderr, flush=True)
+    ref = runs["plain_fp64"]["n"]
+    big = ref > 1e-12 * ref.sum(axis=1, keepdims=True)
+    species = net.species
+
+    def errs(x):
+        d = np.where(big, np.abs(x - ref) / np.maximum(np.abs(ref), 1e-300),
+                     0.0)
+        return {sp: float(d[:, i].max()) for i, sp in enumerate(species)}
+    out_runs = {}
+    for key, r in runs.items():
+        out_runs[key] = {k: r[k] for k in ("cells_per_s", "ms_per_step",
+                                           "kernel_ms")}
+        out_runs[key]["flags_equal_to_plain"] = bool(
+            np.array_equal(r["flags"], runs["plain_fp64"]["flags"]))
+        if key != "plain_fp64":
+            out_runs[key]["max_rel_err_vs_plain_fp64"] = errs(r["n"])
+    # the oracle on a sample: the CPU baseline and the FP64 check
+    from conftest import Oracle, oracle_opts
+    orc = Oracle(REPO / "oracle" / "_build" / "libkx_oracle_fast.so", "kxo_")
+    cores = min(16, os.cpu_count() or 1)
+    orc.lib.kxo_set_threads(cores)
+    m = min(a.cpu_sample, nc)
+    sig = orc.photo_bins("galaxy_ism", elo, ehi, 1.0)
+    rows = orc.photo_rows("galaxy_ism", w["par"][:m], w["Phi"][:m], sig)
+    t0 = time.perf_counter()
+    ro, no, _, flo, _ = orc.solve("galaxy_ism", w["n"][:m], w["T"][:m],
+                                  w["dt"][:m], rows,
+                                  oracle_opts(**w["opts"]))
+    el = time.perf_counter() - t0
+    bo = no > 1e-12 * no.sum(axis=1, keepdims=True)
+    for key, r in runs.items():
+        d = np.abs(r["n"][:m] - no) / np.maximum(np.abs(no), 1e-300)
+        out_runs[key]["max_rel_err_vs_oracle_sample"] = float(d[bo].max())
+        out_runs[key]["flags_equal_to_oracle_sample"] = bool(
+            np.array_equal(r["flags"][:m], flo))
+    head = runs["persistent_mixed"]
+    out = {
+        "metric": "cells/sec to t_end, galaxy_ism with photon bins "
+                  "(persistent mixed precision)",
+        "value": head["cells_per_s"], "unit": "cells/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": head["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32",
+        "data": "synthetic",
+        "config": {"workload": "%d galaxy_ism cells (seed 4), 32 photon "
+                               "bins 0.7..200 eV, a field per cell, Tgas "
+                               "evolved, rtol 1e-8" % nc,
+                   "ncell": nc, "parallelism": "1 GPU"},
+        "kernels": out_runs,
+        "mixed_vs_plain_fp64": runs["plain_fp64"]["ms_per_step"]
+        / head["ms_per_step"],
+        "cpu_baseline": {"value": m / el, "unit": "cells/s",
+                         "cores": int(orc.lib.kxo_get_threads()),
+                         "kind": "port",
+                         "sample": "the first %d cells, the oracle with the "
+                                   "reference's production flags, %.1f s "
+                                   "wall, return %d" % (m, el, ro)},
+    }
+    print(json.dumps(out))
+
+
 def copy_calibration():
     """one 2 GiB device-to-device copy (the traffic counters' reference)"""
     import torch
@@ -342,7 +466,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--workload", default="solve",
-                    choices=("solve", "conserve", "multi", "copy"))
+                    choices=("solve", "conserve", "multi", "photo", "copy"))
     ap.add_argument("--linsol", default="dense", choices=("dense", "sparse"),
                     help="the Newton matrix's LU (sparse: the plain kernel's "
                          "static-pivot sparse LU, linsol 3)")
@@ -356,6 +480,12 @@ def main():
         return conserve_bench(a)
     if a.workload == "copy":
         return copy_calibration()
+    if a.workload == "photo":
+        if a.ncell == 1 << 20:
+            a.ncell = 1 << 18
+        if a.cpu_sample == 1 << 18:
+            a.cpu_sample = a.ncell
+        return photo_bench(a)
     if a.workload == "multi":
         return multi_bench(a)
     import torch

@@ -182,10 +182,14 @@ int kernel_rows(int ncell, const double*& P, int& pst, const double* Phi,
                             hipMemcpyHostToDevice), "H2D"))
         return -1;
     }
-    hipLaunchKernelGGL(kx::photo_rows_kernel<Net>, dim3((rows + 255) / 256),
-                       dim3(256), 0, s, rows, P, pst, B ? Phi : nullptr, B,
+    int block = 256;
+    size_t lds = 0;
+    if (B > 0) kx::photo_rows_shape(B, block, lds);
+    hipLaunchKernelGGL(kx::photo_rows_kernel<Net>,
+                       dim3((rows + block - 1) / block), dim3(block), lds, s,
+                       rows, P, pst, B ? Phi : nullptr, B,
                        static_cast<const double*>(dc.sig.p),
-                       static_cast<double*>(dc.prow.p));
+                       static_cast<double*>(dc.prow.p), lds > 0 ? 1 : 0);
     if (!hip_ok(hipGetLastError(), "photo rows kernel")) return -1;
     P = static_cast<const double*>(dc.prow.p);
     pst = rows == 1 ? 0 : Net::NROW;

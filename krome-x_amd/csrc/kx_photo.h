@@ -19,8 +19,9 @@
 //                   section averages exactly
 //   photo_rows_kernel  once per solve, on the device: each cell's kernel
 //                   parameter row, the caller's NPAR parameters followed by
-//                   its NPHOTO bin sums (one thread per cell; the field's
-//                   row and the table are read once)
+//                   its NPHOTO bin sums (one thread per cell, the block's
+//                   fields staged through LDS; HBM-bound: 8 (B + NROW) bytes
+//                   per cell plus the caller's row)
 //
 // The generated rates read the sums from the row (Params::ph), so the
 // solve kernels see a photo rate like any rate of the parameters: a
@@ -76,24 +77,58 @@ void photo_bin_xsec(int B, const double* Elo, const double* Ehi,
 
 // out[c NROW ..]: the parameter row par[c pst ..] (pst 0: one row for
 // every cell), then sum_b sigma[q B + b] Phi[c B + b] for each photo
-// reaction q (0 without a field: B 0 or Phi null)
+// reaction q (0 without a field: B 0 or Phi null).  With a field, a block
+// of blockDim.x cells first copies its fields, one contiguous stretch of
+// HBM, into LDS with coalesced loads (rows padded to B + 1 doubles, so the
+// lanes' reads of one bin fall in different banks); each thread then sums
+// its own row from LDS, b ascending per reaction.  lds_rows 0: no LDS
+// stage, each thread reads its row from HBM (fields too wide for LDS).
 template <class Net>
 __global__ void __launch_bounds__(256)
 photo_rows_kernel(int rows, const double* __restrict__ par, int pst,
                   const double* __restrict__ Phi, int B,
-                  const double* __restrict__ sigma, double* __restrict__ out) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= rows) return;
-  const double* p = par + (long)c * pst;
-  double* o = out + (long)c * Net::NROW;
-  for (int i = 0; i < Net::NPAR; ++i) o[i] = p[i];
-  const double* f = Phi ? Phi + (long)c * B : nullptr;
-  for (int q = 0; q < Net::NPHOTO; ++q) {
-    double s = 0.0;
-    if (f)
-      for (int b = 0; b < B; ++b) s += sigma[q * B + b] * f[b];
-    o[Net::NPAR + q] = s;
+                  const double* __restrict__ sigma, double* __restrict__ out,
+                  int lds_rows) {
+  extern __shared__ double tile[];
+  const long c0 = (long)blockIdx.x * blockDim.x;
+  const int nrow = (int)((rows - c0) < blockDim.x ? (rows - c0) : blockDim.x);
+  const bool staged = Phi && lds_rows > 0;
+  if (staged) {
+    const double* src = Phi + c0 * B;
+    for (int i = threadIdx.x; i < nrow * B; i += blockDim.x) {
+      const int r = i / B;
+      tile[r * (B + 1) + (i - r * B)] = src[i];
+    }
   }
+  __syncthreads();
+  const long c = c0 + threadIdx.x;
+  if (c >= rows) return;
+  const double* p = par + c * pst;
+  double* o = out + c * Net::NROW;
+  for (int i = 0; i < Net::NPAR; ++i) o[i] = p[i];
+  double s[Net::NPHOTO > 0 ? Net::NPHOTO : 1];
+  for (int q = 0; q < Net::NPHOTO; ++q) s[q] = 0.0;
+  if (Phi) {
+    const double* f = staged ? tile + threadIdx.x * (B + 1) : Phi + c * B;
+    for (int b = 0; b < B; ++b) {
+      const double fb = f[b];
+      for (int q = 0; q < Net::NPHOTO; ++q) s[q] += sigma[q * B + b] * fb;
+    }
+  }
+  for (int q = 0; q < Net::NPHOTO; ++q) o[Net::NPAR + q] = s[q];
+}
+
+// the launch shape of photo_rows_kernel for B bins: cells per block and
+// dynamic LDS bytes (64 KiB at most; 0: unstaged)
+inline void photo_rows_shape(int B, int& block, size_t& lds) {
+  const size_t cap = 64 * 1024;
+  block = 256;
+  for (; block >= 64; block /= 2) {
+    lds = (size_t)block * (B + 1) * sizeof(double);
+    if (lds <= cap) return;
+  }
+  block = 256;
+  lds = 0;
 }
 
 }  // namespace kx

@@ -14,7 +14,7 @@ import pytest
 import cells
 from conftest import PKG, REPO
 
-NETS = ("primordial", "primordial3")
+NETS = ("primordial", "primordial3", "galaxy_ism")
 
 
 def declared():
@@ -130,9 +130,11 @@ def test_argument_checks_without_device_work(net):
 
 def test_element_metadata_vs_reference(net):
     """element names, atom counts, masses and charges equal the tables of
-    the reference's generated project (tests/golden/*_conserve.npz)"""
+    the reference's generated project (tests/golden/*_conserve.npz,
+    galaxy_ism_tables.npz)"""
     from conftest import golden
-    g = golden("%s_conserve.npz" % net.name)
+    g = golden("%s_%s.npz" % (net.name, "tables" if net.name == "galaxy_ism"
+                              else "conserve"))
     assert net.elements == tuple(g["element_names"])
     assert np.array_equal(net.composition, g["species_elements"])
     assert np.array_equal(net.masses, g["species_mass_g"])
