@@ -23,6 +23,11 @@ Workloads (the species order of the generated networks):
                 log-uniform 1e8..1e14 s (step counts spread over more than
                 100x); Tgas 10..1e4 K, x 1e-6..1e-2, y 1e-8..1e-3 as
                 primordial; fixed Tgas
+  c3_primordialZ  2^18 cells, seed 2, cells 2^19 onwards: the primordialZ
+                cells of the uneven batch, drawn as c3_primordial, with the
+                metals C = 2.69e-6 nH and O = 4.90e-6 nH (neutral); fixed
+                Tgas, the difference-quotient Jacobian (jac 2: its density-
+                gated rates have no analytic dk/dn here), rtol 1e-6
   galaxy_ism    2^18 cells, seed 4: nH in 0.1..1e5 cm^-3, Tgas 10..1e4 K,
                 x in 1e-4..1e-1, y in 1e-6..1e-1, He = 0.0775 nH with
                 He+ = 1e-3 He, the other species 1e-20 nH (e- balancing
@@ -144,6 +149,30 @@ def c3_primordial_cells(ncell=1 << 19, seed=2, first=0):
                      "max_steps": 100000}}
 
 
+def c3_primordialZ_cells(ncell=1 << 18, seed=2, first=1 << 19):
+    """the primordialZ part of the uneven-batch workload (cells after the
+    primordial part's): the primordial composition of c3_primordial and
+    the metals C = 2.69e-6 nH, O = 4.90e-6 nH (a hundredth of solar),
+    neutral; fixed Tgas"""
+    meta = network_meta("primordialZ")
+    c = np.arange(first, first + ncell)
+    nH = log_uniform(seed, c, 0, 1.0, 1e12)
+    T = log_uniform(seed, c, 1, 10.0, 1e4)
+    x = log_uniform(seed, c, 2, 1e-6, 1e-2)
+    y = log_uniform(seed, c, 3, 1e-8, 1e-3)
+    n = _species_rows(meta, nH, x, y)
+    col = {s: i for i, s in enumerate(meta["species"])}
+    n[:, col["C"]] = 2.69e-6 * nH
+    n[:, col["O"]] = 4.90e-6 * nH
+    dt = log_uniform(seed, c, 4, 1e8, 1e14)
+    par = np.tile(np.array(meta["param_defaults"], dtype=np.float64),
+                  (ncell, 1))
+    return {"n": n, "T": T, "dt": dt, "par": par,
+            "opts": {"evolve_T": 0, "rtol": 1e-6, "atol_n": 1e-20,
+                     "atol_T": 1e-6, "linsol": 1, "jac": 2,
+                     "max_steps": 100000}}
+
+
 def photo_bins(nbins=32, emin=0.7, emax=200.0):
     """(Elo, Ehi) [eV] of nbins logarithmic bins over [emin, emax]"""
     e = emin * (emax / emin) ** (np.arange(nbins + 1) / nbins)
@@ -196,4 +225,5 @@ def galaxy_ism_cells(ncell=1 << 18, seed=4, first=0):
 
 WORKLOADS = {"primordial": primordial_cells, "primordial3": primordial3_cells,
              "c3_primordial": c3_primordial_cells,
-             "galaxy_ism": galaxy_ism_cells}
+             "galaxy_ism": galaxy_ism_cells,
+             "c3_primordialZ": c3_primordialZ_cells}

@@ -608,8 +608,12 @@ kx::SolveOpts solve_opts(const krome_opts* h) {
 // what the whole call rejects before integrating (every cell flagged)
 int reject(const krome_opts* h) {
   if (h->linsol == 2) return kx::FLAG_UNSUPPORTED;  // no LAPACK here
-  // the persistent kernels factorize the dense Newton matrix only
-  if (h->linsol == 3 && h->kernel == 1) return kx::FLAG_UNSUPPORTED;
+  // the persistent kernels factorize the dense Newton matrix only, for
+  // networks of up to KX_PERSIST_MAX_N equations
+  if (h->kernel == 1 &&
+      (h->linsol == 3 ||
+       kx::Problem<Net, false>::N + (h->evolve_T ? 1 : 0) > KX_PERSIST_MAX_N))
+    return kx::FLAG_UNSUPPORTED;
   if (h->path == 1) return kx::FLAG_UNSUPPORTED;  // no host path here
   if (h->evolve_T && !Net::THERMAL) return kx::FLAG_UNSUPPORTED;
   return 0;

@@ -607,6 +607,70 @@ def swri_xsec(data_dir, reactant, products):
     return [e for e, _ in table], [v for _, v in table]
 
 
+# the rate database of `auto` rates (docs/network-format.md section 7):
+# data/database/*.dat read as hydroHe.dat, recombination_badnell.dat, then
+# the others in sorted order; the first entry whose sorted reactants and
+# products match wins.  Its expression is in Tgas and the legacy
+# shortcuts below, with Fortran-style numbers.
+AUTO_FIRST = ("hydroHe.dat", "recombination_badnell.dat")
+AUTO_SHORTCUTS = {"T32": "(Tgas/300.0)", "invT": "(1.0/Tgas)",
+                  "Te": "Tgas_eV", "invTe": "(1.0/Tgas_eV)",
+                  "lnTe": "log(Tgas_eV)", "T": "Tgas"}
+_AUTO_DB = {}
+
+
+def auto_database(data_dir):
+    """[(file, {type, reacts, prods, limits?, rate})] of the rate entries
+    (not photoionization), in the resolution order"""
+    key = str(data_dir)
+    if key in _AUTO_DB:
+        return _AUTO_DB[key]
+    folder = Path(data_dir) / "database"
+    files = [folder / f for f in AUTO_FIRST]
+    files += sorted(f for f in folder.glob("*.dat")
+                    if f.name not in AUTO_FIRST
+                    and f.name != "photoionization.dat")
+    out = []
+    for f in files:
+        cur = None
+        for line in f.read_text(errors="replace").splitlines():
+            t = line.strip()
+            if not t.startswith("@") or ":" not in t:
+                continue
+            k, v = (x.strip() for x in t[1:].split(":", 1))
+            if k == "type":
+                cur = {"type": v}
+                out.append((f.name, cur))
+            elif cur is None:
+                continue
+            elif k in ("reacts", "prods"):
+                cur[k] = sorted("e-" if x.strip().upper() == "E"
+                                else x.strip() for x in v.split(","))
+            elif k in ("rate", "limits"):
+                cur[k] = v
+    _AUTO_DB[key] = out
+    return out
+
+
+def resolve_auto(r, data_dir):
+    """the rate expression (in the .knet rate language) of an `auto`
+    reaction: the first database entry with its reactants and products"""
+    reacts = sorted(s for s, _ in r.reactants)
+    prods = sorted(r.products)
+    for fname, e in auto_database(data_dir):
+        if e.get("reacts") == reacts and e.get("prods") == prods \
+                and "rate" in e:
+            if "limits" in e:
+                raise KnetError("line %d: database entry for %s (%s) has "
+                                "temperature limits, not supported"
+                                % (r.line, r.eq, fname))
+            expr = re.sub(r"(\d\.?\d*)[dD]([+-]?\d+)", r"\1e\2", e["rate"])
+            return re.sub(r"[A-Za-z_][A-Za-z0-9_]*",
+                          lambda m: AUTO_SHORTCUTS.get(m.group(0),
+                                                       m.group(0)), expr)
+    raise KnetError("line %d: no database rate for %s" % (r.line, r.eq))
+
+
 def photo_reactions(net, data_dir):
     """[{r, reactant, Emin, Emax, kind, code | table}] of the photo()
     reactions in reaction order: kind "fit" (photo(xsec=auto): the
@@ -958,6 +1022,9 @@ def generate(knet_path, thermal=None, name=None, data_dir=None):
     net = parse_knet(knet_path)
     if data_dir is None:
         data_dir = Path(knet_path).resolve().parent.parent / "data"
+    for r in net.reactions:
+        if r.rate.strip() == "auto":
+            r.rate = resolve_auto(r, data_dir)
     photo = photo_reactions(net, data_dir)
     photo_of = {e["r"]: q for q, e in enumerate(photo)}
     NPH = len(photo)
@@ -1561,6 +1628,9 @@ CONFIGS = {
     # rate and grain-assisted recombination; the default thermal options
     # (no cooling or heating term), Tgas evolved
     "galaxy_ism": ("galaxy_ism.knet", Thermal(evolve=True)),
+    # 27 species with the metals C and O, density-gated rates
+    # (where(ntot > 1e14, ...)) and two database rates; Tgas fixed
+    "primordialZ": ("primordialZ.knet", Thermal()),
 }
 
 

@@ -30,6 +30,7 @@
 #include "net_primordial.h"
 #include "net_primordial3.h"
 #include "net_galaxy_ism.h"
+#include "net_primordialZ.h"
 #include "kx_conserve.hpp"
 #include "kx_photo.hpp"
 
@@ -331,11 +332,12 @@ int KXO_(get_threads)(void) {
 }
 
 // a network of krome-x_amd/csrc/generated by name: 0 primordial,
-// 1 primordial3, -1 unknown
+// 1 primordial3, 2 galaxy_ism, 3 primordialZ, -1 unknown
 int KXO_(network)(const char* name) {
   if (!std::strcmp(name, "primordial")) return 0;
   if (!std::strcmp(name, "primordial3")) return 1;
   if (!std::strcmp(name, "galaxy_ism")) return 2;
+  if (!std::strcmp(name, "primordialZ")) return 3;
   return -1;
 }
 
@@ -373,6 +375,9 @@ int KXO_(solve_cells)(int net, int ncell, double* n, double* T,
     case 2:
       return solve_cells<kx::net_galaxy_ism>(ncell, n, T, dt, par,
                                              par_stride, opts, flags, stats);
+    case 3:
+      return solve_cells<kx::net_primordialZ>(ncell, n, T, dt, par,
+                                              par_stride, opts, flags, stats);
   }
   return -1;
 }
@@ -389,6 +394,9 @@ int KXO_(funcs)(int net, int ncell, const double* n, const double* T,
       return 0;
     case 2:
       funcs<kx::net_galaxy_ism>(ncell, n, T, par, evolve_T, k, f);
+      return 0;
+    case 3:
+      funcs<kx::net_primordialZ>(ncell, n, T, par, evolve_T, k, f);
       return 0;
   }
   return -1;
@@ -440,6 +448,7 @@ int KXO_(totals)(int net, int x, int ncell, const double* v, double* tot) {
     case 0: return run(kx::net_primordial{});
     case 1: return run(kx::net_primordial3{});
     case 2: return run(kx::net_galaxy_ism{});
+    case 3: return run(kx::net_primordialZ{});
   }
   return -1;
 }
@@ -459,6 +468,9 @@ int KXO_(conserve)(int net, int x, int ncell, double* v, const double* ref,
     case 2:
       return kxo_cons::cells<kx::net_galaxy_ism>(ncell, v, ref, ref_rows,
                                                  x != 0, flags);
+    case 3:
+      return kxo_cons::cells<kx::net_primordialZ>(ncell, v, ref, ref_rows,
+                                                  x != 0, flags);
   }
   return -1;
 }

@@ -15,7 +15,8 @@ sys.path.insert(0, str(PKG))
 OPT_ORDER = ("evolve_T", "rtol", "atol_n", "atol_T", "linsol", "jac",
              "max_steps", "max_err_fails", "max_conv_fails", "max_restarts",
              "clip_negative", "conserve")
-NET_ID = {"primordial": 0, "primordial3": 1, "galaxy_ism": 2}
+NET_ID = {"primordial": 0, "primordial3": 1, "galaxy_ism": 2,
+          "primordialZ": 3}
 
 
 def pytest_configure(config):

@@ -57,9 +57,14 @@ TOML = {
                    'h2_opacity = "RIPAMONTI"\n',
     "galaxy_ism": '[project]\nnetwork = "{net}/galaxy_ism.knet"\n'
                   'outdir = "{out}"\n',
+    "primordialZ": '[project]\nnetwork = "{net}/primordialZ.knet"\n'
+                   'outdir = "{out}"\n',
 }
 # cells of the solve fixtures (the first ones of the seeded workload)
-NSOLVE = {"primordial": 1024, "primordial3": 256}
+NSOLVE = {"primordial": 1024, "primordial3": 256, "primordialZ": 256}
+# the workload of a network's fixtures (cells.WORKLOADS)
+WORKLOAD = {"primordial": "primordial", "primordial3": "primordial3",
+            "primordialZ": "c3_primordialZ"}
 OPT_KEYS = ("evolve_T", "rtol", "atol_n", "atol_T", "linsol", "jac",
             "max_steps")
 T_GRID = (12.0, 97.0, 314.0, 1.1e3, 5.5e3, 2.3e4, 1.7e5, 3e6, 8e7)
@@ -125,7 +130,7 @@ def run_solve(exe, w, tag, opts=None):
 def func_samples(name):
     """workload cells, and the same densities over the T grid (every
     piece, clamp and cut of the rates)"""
-    gen = cells.WORKLOADS[name]
+    gen = cells.WORKLOADS[WORKLOAD.get(name, name)]
     w = gen(64, first=0)
     base = gen(len(T_GRID) * 8, first=5000)
     T = np.repeat(np.array(T_GRID), 8)
@@ -272,7 +277,7 @@ def make_conserve(name, exe, meta):
 
 
 def make_sparse(name, exe):
-    w = cells.WORKLOADS[name](NSOLVE[name])
+    w = cells.WORKLOADS[WORKLOAD[name]](NSOLVE[name])
     raw, _ = run_mode(exe, "sparse", w, name + "_sparse_tab", [0.0])
     ns = w["n"].shape[1]
     nnz = int(raw[0])
@@ -379,10 +384,15 @@ def make_photo(name, exes, meta):
 
 
 def main(argv=None):
+    """argv: the parts to write (solve funcs conserve sparse photo;
+    default all) and, optionally, the networks (default all)"""
+    argv = list(argv or [])
+    only = {a for a in argv if a in TOML}
+    argv = [a for a in argv if a not in TOML]
     parts = set(argv if argv else ("solve", "funcs", "conserve", "sparse",
                                    "photo"))
     BUILD.mkdir(parents=True, exist_ok=True)
-    if "photo" in parts:
+    if "photo" in parts and (not only or "galaxy_ism" in only):
         name = "galaxy_ism"
         proj = generate(name)
         exes = {m: compile_driver(proj, name, m)
@@ -392,20 +402,26 @@ def main(argv=None):
         w = cells.WORKLOADS[name](64)
         np.savez_compressed(HERE / ("%s_tables.npz" % name),
                             **ref_tables(exes["bitwise"], name, w))
-    for name in ("primordial", "primordial3"):
+    for name in ("primordial", "primordial3", "primordialZ"):
         if not parts & {"solve", "funcs", "conserve", "sparse"}:
             break
+        if only and name not in only:
+            continue
         meta = cells.network_meta(name)
         proj = generate(name)
         exes = {m: compile_driver(proj, name, m)
                 for m in ("bitwise", "production")}
-        if "conserve" in parts:
+        if "conserve" in parts and name != "primordialZ":
             make_conserve(name, exes["bitwise"], meta)
+        if name == "primordialZ" and parts & {"solve", "conserve"}:
+            w = cells.WORKLOADS[WORKLOAD[name]](64)
+            np.savez_compressed(HERE / ("%s_tables.npz" % name),
+                                **ref_tables(exes["bitwise"], name, w))
         if "sparse" in parts:
             make_sparse(name, exes["bitwise"])
         if not parts & {"solve", "funcs"}:
             continue
-        w = cells.WORKLOADS[name](NSOLVE[name])
+        w = cells.WORKLOADS[WORKLOAD[name]](NSOLVE[name])
         if "solve" in parts:
             res = {}
             for m, exe in exes.items():

@@ -23,8 +23,8 @@ import pytest
 import cells
 from conftest import golden, oracle_opts, rel_err
 
-NETS = ("primordial", "primordial3")
-TOL = {"primordial": 5e-5, "primordial3": 1e-6}
+NETS = ("primordial", "primordial3", "primordialZ")
+TOL = {"primordial": 5e-5, "primordial3": 1e-6, "primordialZ": 3e-4}
 
 
 def say(*a):
@@ -101,8 +101,8 @@ def test_gpu_sparse_vs_reference(nets, oracle, net):
     o = d["opts"]
     say("sparse", net)
     opts = N.options(evolve_T=bool(o[0]), rtol=o[1], atol_n=o[2],
-                     atol_T=o[3], linsol=3, device_kernel="plain",
-                     error_report=False)
+                     atol_T=o[3], linsol=3, jac=int(o[5]),
+                     device_kernel="plain", error_report=False)
     res = N.solve(d["n0"], d["T0"], d["dt"], par=d["par"], opts=opts,
                   check=False)
     assert res.ierr == 0 and np.array_equal(res.flags, d["flag"])
@@ -113,7 +113,8 @@ def test_gpu_sparse_vs_reference(nets, oracle, net):
     assert rel_err(res.n, n) < TOL[net]
     # the persistent kernels factorize the dense matrix only: refused
     op = N.options(evolve_T=bool(o[0]), rtol=o[1], linsol=3,
-                   device_kernel="persistent", error_report=False)
+                   jac=int(o[5]), device_kernel="persistent",
+                   error_report=False)
     res2 = N.solve(d["n0"][:8], d["T0"][:8], d["dt"][:8], par=d["par"][:8],
                    opts=op, check=False)
     assert res2.ierr == -104 and (res2.flags == -104).all()
