@@ -16,7 +16,8 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 GEN = CSRC / "generated"
-NETWORKS = ("primordial", "primordial3", "galaxy_ism", "primordialZ")
+NETWORKS = ("primordial", "primordial3", "galaxy_ism", "primordialZ",
+            "galaxy_ism_heat")
 # test builds of a network with other compile-time forms: name -> (network,
 # extra flags); primordial_rhsloop evaluates the rhs and the species
 # Jacobian by the loop form over index tables (the form of networks above

@@ -38,6 +38,8 @@ Workloads (the species order of the generated networks):
                 Phi0 = 2e7 photons cm^-2 s^-1 per bin below 13.6 eV (an
                 interstellar FUV field) and 1e4 (E/13.6)^-1.5 above it (a
                 weak ionizing background)
+  galaxy_ism_heat  the same cells for galaxy_ism with its thermal balance
+                (heating CHEM + PHOTO, cooling H2 + CHEM)
 """
 from __future__ import annotations
 
@@ -182,10 +184,12 @@ def photo_bins(nbins=32, emin=0.7, emax=200.0):
 PHOTO_BINS = photo_bins()
 
 
-def galaxy_ism_cells(ncell=1 << 18, seed=4, first=0):
+def galaxy_ism_cells(ncell=1 << 18, seed=4, first=0, network="galaxy_ism"):
     """the galaxy_ism photochemistry workload, with Tgas evolved: the
-    cells, their parameter rows and their photon fields Phi[ncell][32]"""
-    meta = network_meta("galaxy_ism")
+    cells, their parameter rows and their photon fields Phi[ncell][32]
+    (network: galaxy_ism, or galaxy_ism_heat, the same network with its
+    thermal balance)"""
+    meta = network_meta(network)
     sp = meta["species"]
     col = {s: i for i, s in enumerate(sp)}
     c = np.arange(first, first + ncell)
@@ -226,4 +230,6 @@ def galaxy_ism_cells(ncell=1 << 18, seed=4, first=0):
 WORKLOADS = {"primordial": primordial_cells, "primordial3": primordial3_cells,
              "c3_primordial": c3_primordial_cells,
              "galaxy_ism": galaxy_ism_cells,
+             "galaxy_ism_heat": lambda ncell=1 << 18, seed=4, first=0:
+                 galaxy_ism_cells(ncell, seed, first, "galaxy_ism_heat"),
              "c3_primordialZ": c3_primordialZ_cells}

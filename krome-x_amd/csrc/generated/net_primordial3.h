@@ -27,6 +27,10 @@ struct net_primordial3 {
   // sum_b sigma_rb Phi_b of the cell's field, which the kernels read
   // after the parameters, in the row's last NPHOTO slots
   static constexpr int NPHOTO = 0;
+  // PHOTO heating: the energy sums sum_b sigmaE_qb Phi_b follow the
+  // rate sums in the row (sigmaE: sigma times the photoelectron's
+  // energy E - Emin_q, bin-averaged)
+  static constexpr bool PHOTO_HEAT = false;
   static constexpr int NROW = NPAR + NPHOTO;
   // parameter row (param_names order), then the photo sums
   struct Params {
@@ -812,7 +816,7 @@ struct net_primordial3 {
     return gsum > R(0) ? R(1) + ysum / gsum : R(5) / R(3);
   }
   // dTgas/dt: heating minus cooling (the reference's term order:
-  // heat CHEM, COMPRESS; cool H2, CHEM, CONT)
+  // heat CHEM, COMPRESS, PHOTO; cool H2, CHEM, CONT)
   template <class R>
   KX_HD static R thermal(R Tth, const R* n, const R* k, const Params& p) {
     R heat = R(0), cool = R(0);

@@ -45,6 +45,10 @@ struct net_primordialZ {
   // sum_b sigma_rb Phi_b of the cell's field, which the kernels read
   // after the parameters, in the row's last NPHOTO slots
   static constexpr int NPHOTO = 0;
+  // PHOTO heating: the energy sums sum_b sigmaE_qb Phi_b follow the
+  // rate sums in the row (sigmaE: sigma times the photoelectron's
+  // energy E - Emin_q, bin-averaged)
+  static constexpr bool PHOTO_HEAT = false;
   static constexpr int NROW = NPAR + NPHOTO;
   // parameter row (param_names order), then the photo sums
   struct Params {

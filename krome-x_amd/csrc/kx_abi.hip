@@ -54,7 +54,8 @@ struct Opts {
   int queue_chunk = 0;
   int waves_per_cu = 0;
   // photon bins (krome_opts_set_photo): edges [eV], the in-bin spectrum
-  // exponent and the bin-averaged cross sections sigma[NPHOTO][nbins]
+  // exponent and the bin-averaged cross sections sigma[NPHOTO][nbins],
+  // followed by their photoelectron-energy averages sigmaE[NPHOTO][nbins]
   int nbins = 0;
   double photo_alpha = 1.0;
   std::vector<double> Elo, Ehi, sigma;
@@ -176,7 +177,9 @@ int kernel_rows(int ncell, const double*& P, int& pst, const double* Phi,
     if (!dc.prow.ensure(sizeof(double) * (size_t)rows * Net::NROW))
       return kx::FLAG_DEVICE_MEMORY;
     if (B > 0) {
-      const size_t sb = sizeof(double) * (size_t)Net::NPHOTO * B;
+      // sigma, then sigmaE when the network has PHOTO heating
+      const size_t sb = sizeof(double) * (size_t)Net::NPHOTO * B *
+                        (Net::PHOTO_HEAT ? 2 : 1);
       if (!dc.sig.ensure(sb)) return kx::FLAG_DEVICE_MEMORY;
       if (!hip_ok(hipMemcpy(dc.sig.p, o.sigma.data(), sb,
                             hipMemcpyHostToDevice), "H2D"))

@@ -43,12 +43,15 @@ inline double photo_weight(double alpha, double a, double c) {
 }
 
 // sigma[q B + b] of the bins [Elo[b], Ehi[b]] (0 < Elo < Ehi, checked by
-// the caller)
+// the caller), followed by sigmaE[NPHOTO B] at sigma + NPHOTO B: the same
+// average of sigma(E) times the photoelectron's energy max(E - Emin_q, 0)
+// [cm^2 eV] (photo.hpp:227-246), the photoheating's table
 template <class Net>
 void photo_bin_xsec(int B, const double* Elo, const double* Ehi,
                     double alpha, std::vector<double>& sigma) {
   constexpr int QUAD = 64;
-  sigma.assign((size_t)(Net::NPHOTO > 0 ? Net::NPHOTO : 1) * B, 0.0);
+  const size_t NB = (size_t)(Net::NPHOTO > 0 ? Net::NPHOTO : 1) * B;
+  sigma.assign(2 * NB, 0.0);
   for (int q = 0; q < Net::NPHOTO; ++q) {
     for (int b = 0; b < B; ++b) {
       const double a = std::fmax(Elo[b], Net::photo_Emin[q]);
@@ -56,7 +59,7 @@ void photo_bin_xsec(int B, const double* Elo, const double* Ehi,
       if (!(c > a)) continue;
       const double dE = (c - a) / QUAD;
       const double dl = std::log(c / a) / QUAD;
-      double s = 0.0;
+      double s = 0.0, se = 0.0;
       for (int i = 0; i < QUAD; ++i) {
         double E, w;
         if (alpha == 0.0) {
@@ -68,16 +71,21 @@ void photo_bin_xsec(int B, const double* Elo, const double* Ehi,
           const double hi = i + 1 == QUAD ? c : a * std::exp((i + 1) * dl);
           w = photo_weight(alpha, lo, hi);
         }
-        s += Net::photo_xsec(q, E) * w;
+        const double x = Net::photo_xsec(q, E) * w;
+        s += x;
+        se += x * std::fmax(E - Net::photo_Emin[q], 0.0);
       }
-      sigma[(size_t)q * B + b] = s / photo_weight(alpha, Elo[b], Ehi[b]);
+      const double wb = photo_weight(alpha, Elo[b], Ehi[b]);
+      sigma[(size_t)q * B + b] = s / wb;
+      sigma[NB + (size_t)q * B + b] = se / wb;
     }
   }
 }
 
 // out[c NROW ..]: the parameter row par[c pst ..] (pst 0: one row for
 // every cell), then sum_b sigma[q B + b] Phi[c B + b] for each photo
-// reaction q (0 without a field: B 0 or Phi null).  With a field, a block
+// reaction q (0 without a field: B 0 or Phi null), and with PHOTO heating
+// the same sums over sigmaE (at sigma + NPHOTO B).  With a field, a block
 // of blockDim.x cells first copies its fields, one contiguous stretch of
 // HBM, into LDS with coalesced loads (rows padded to B + 1 doubles, so the
 // lanes' reads of one bin fall in different banks); each thread then sums
@@ -106,16 +114,17 @@ photo_rows_kernel(int rows, const double* __restrict__ par, int pst,
   const double* p = par + c * pst;
   double* o = out + c * Net::NROW;
   for (int i = 0; i < Net::NPAR; ++i) o[i] = p[i];
-  double s[Net::NPHOTO > 0 ? Net::NPHOTO : 1];
-  for (int q = 0; q < Net::NPHOTO; ++q) s[q] = 0.0;
+  constexpr int NQ = Net::NPHOTO * (Net::PHOTO_HEAT ? 2 : 1);
+  double s[NQ > 0 ? NQ : 1];
+  for (int q = 0; q < NQ; ++q) s[q] = 0.0;
   if (Phi) {
     const double* f = staged ? tile + threadIdx.x * (B + 1) : Phi + c * B;
     for (int b = 0; b < B; ++b) {
       const double fb = f[b];
-      for (int q = 0; q < Net::NPHOTO; ++q) s[q] += sigma[q * B + b] * fb;
+      for (int q = 0; q < NQ; ++q) s[q] += sigma[q * B + b] * fb;
     }
   }
-  for (int q = 0; q < Net::NPHOTO; ++q) o[Net::NPAR + q] = s[q];
+  for (int q = 0; q < NQ; ++q) o[Net::NPAR + q] = s[q];
 }
 
 // the launch shape of photo_rows_kernel for B bins: cells per block and

@@ -462,18 +462,22 @@ class Network:
                                                      self.flag_message(r)))
         return out[0] if single else out
 
-    def thermal(self, n, Tgas, par=None):
+    def thermal(self, n, Tgas, par=None, Phi=None, opts=None):
         """the thermal balance: dict of heat, cool [erg cm^-3 s^-1], the
         terms heats [..., nheat] and cools [..., ncool] (labels:
-        heating_terms, cooling_terms) and dTdt [K/s]"""
+        heating_terms, cooling_terms) and dTdt [K/s]; the photoheating of
+        the fields Phi over opts' photon bins"""
         single, n, ncell = self._cells(n)
         T = self._per_cell(Tgas, ncell)
         P, stride = self._params(par, ncell)
+        h = opts._h if opts is not None else None
+        F = self._phi(Phi, ncell, opts) if opts is not None else None
         heat, cool, dTdt = (np.zeros(ncell) for _ in range(3))
         hs = np.zeros((ncell, max(len(self.heating_terms), 1)))
         cs = np.zeros((ncell, max(len(self.cooling_terms), 1)))
         r = self._lib.krome_get_thermal(ncell, _dptr(n), _dptr(T), _dptr(P),
-                                        stride, None, None, _dptr(heat),
+                                        stride, _dptr(F) if F is not None
+                                        else None, h, _dptr(heat),
                                         _dptr(cool), _dptr(hs), _dptr(cs),
                                         _dptr(dTdt))
         if r != 0:

@@ -952,7 +952,7 @@ GAMMA_ROT_K = {s: v * HC_K for s, v in {
     "O2": 1.4264, "CO": 2.78}.items()}
 
 COOLING = ("H2", "CONT", "CHEM")
-HEATING = ("COMPRESS", "CHEM")
+HEATING = ("COMPRESS", "CHEM", "PHOTO")
 
 
 @dataclass
@@ -1068,14 +1068,25 @@ def generate(knet_path, thermal=None, name=None, data_dir=None):
     w("  // photo reactions (photo(xsec=...)): their rates are the bin sums")
     w("  // sum_b sigma_rb Phi_b of the cell's field, which the kernels read")
     w("  // after the parameters, in the row's last NPHOTO slots")
+    photo_heat = "PHOTO" in th.heating
+    if photo_heat and not NPH:
+        raise KnetError("PHOTO heating without photo reactions")
     w("  static constexpr int NPHOTO = %d;" % NPH)
-    w("  static constexpr int NROW = NPAR + NPHOTO;")
+    w("  // PHOTO heating: the energy sums sum_b sigmaE_qb Phi_b follow the")
+    w("  // rate sums in the row (sigmaE: sigma times the photoelectron's")
+    w("  // energy E - Emin_q, bin-averaged)")
+    w("  static constexpr bool PHOTO_HEAT = %s;"
+      % ("true" if photo_heat else "false"))
+    w("  static constexpr int NROW = NPAR + NPHOTO%s;"
+      % (" * 2" if photo_heat else ""))
     w("  // parameter row (param_names order), then the photo sums")
     w("  struct Params {")
     for p in pnames:
         w("    double %s;" % p)
     if NPH:
         w("    double ph[NPHOTO];")
+    if photo_heat:
+        w("    double phe[NPHOTO];")
     w("  };")
     w("  KX_HD static Params params_from_row(const double* row) {")
     w("    Params p;")
@@ -1083,6 +1094,9 @@ def generate(knet_path, thermal=None, name=None, data_dir=None):
         w("    p.%s = row[%d];" % (p, i))
     if NPH:
         w("    for (int r = 0; r < NPHOTO; ++r) p.ph[r] = row[NPAR + r];")
+    if photo_heat:
+        w("    for (int r = 0; r < NPHOTO; ++r)")
+        w("      p.phe[r] = row[NPAR + NPHOTO + r];")
     w("    return p;")
     w("  }")
     _emit_photo(w, net, em, photo)
@@ -1363,7 +1377,7 @@ def generate(knet_path, thermal=None, name=None, data_dir=None):
 
     # thermal
     if thermal_on:
-        _emit_thermal(w, net, th, pnames)
+        _emit_thermal(w, net, th, pnames, photo)
     w("};")
     w("}  // namespace kx")
     meta = {"name": ns, "species": [s.name for s in net.species],
@@ -1436,7 +1450,24 @@ def _emit_photo(w, net, em, photo):
     w("  }")
 
 
-def _emit_thermal(w, net, th, pnames):
+# the Lyman-Werner band [eV] (kromex/knet.py LW_BAND)
+LW_BAND = (11.2, 13.6)
+
+
+def is_solomon(r, photo_entry=None):
+    """H2 -> H + H by Lyman-Werner line pumping (kromex/codegen.py:
+    7210-7230): a rate through H2_solomonLW, or a photo rate whose window
+    ends inside the band.  Such reactions heat by Solomon pumping (CHEM)
+    and deposit no photoelectron"""
+    R = sorted(s for s, _ in r.reactants)
+    if R != ["H2"] or sorted(r.products) != ["H", "H"]:
+        return False
+    if photo_entry is not None:
+        return photo_entry["Emax"] <= LW_BAND[1] + 1e-9
+    return "H2_solomonLW" in re.findall(r"[A-Za-z_][A-Za-z0-9_]*", r.rate)
+
+
+def _emit_thermal(w, net, th, pnames, photo=()):
     names = {s.name for s in net.species}
     idx = net.idx
     w("  // thermal terms at the thermal temperature Tth (capped to")
@@ -1460,17 +1491,37 @@ def _emit_thermal(w, net, th, pnames):
         w("    return c * R(%r);" % EV_TO_ERG)
         w("  }")
     if "CHEM" in th.heating:
+        # the formation heating, then Solomon pumping: (18.7 ffac + 0.4) eV
+        # per dissociation (codegen.py:4952-5029)
+        pe = {e["r"]: e for e in photo}
+        sol = [r for r in net.reactions if is_solomon(r, pe.get(r.index))]
         w("  template <class R>")
         w("  KX_HD static R heating_chem(R Tth, const R* n, const R* k) {")
         w("    R h = R(0);")
-        if heat:
+        if heat or sol:
             w("    const R ffac = kx::h2_heat_fraction(Tth, n[%d], n[%d], "
               "Hnuclei(n));" % (idx("H"), idx("H2")))
             for r, q, ff in heat:
                 prod = "*".join("n[%d]" % idx(s) for s, _ in r.reactants)
                 w("    h += k[%d] * (R(%r) * ffac) * %s;  // %s"
                   % (r.index, q, prod, r.eq))
+            for r in sol:
+                w("    h += k[%d] * (R(18.7) * ffac + R(0.4)) * n[%d];"
+                  "  // Solomon: %s" % (r.index, idx("H2"), r.eq))
         w("    (void)Tth;")
+        w("    return h * R(%r);" % EV_TO_ERG)
+        w("  }")
+    if "PHOTO" in th.heating:
+        # photo.hpp:322-342: each absorption deposits the photoelectron's
+        # energy; the sum over the photo reactions, then eV -> erg
+        w("  template <class R>")
+        w("  KX_HD static R heating_photo(const R* n, const Params& p) {")
+        w("    R h = R(0);")
+        for q, e in enumerate(photo):
+            if is_solomon(net.reactions[e["r"]], e):
+                continue  # line pumping: no photoelectron
+            w("    h += R(p.phe[%d]) * n[%d];  // %s"
+              % (q, e["reactant"], net.reactions[e["r"]].eq))
         w("    return h * R(%r);" % EV_TO_ERG)
         w("  }")
     # gamma
@@ -1517,7 +1568,7 @@ def _emit_thermal(w, net, th, pnames):
     w("  }")
     # dT/dt
     w("  // dTgas/dt: heating minus cooling (the reference's term order:")
-    w("  // heat CHEM, COMPRESS; cool H2, CHEM, CONT)")
+    w("  // heat CHEM, COMPRESS, PHOTO; cool H2, CHEM, CONT)")
     w("  template <class R>")
     w("  KX_HD static R thermal(R Tth, const R* n, const R* k, const Params& p) {")
     w("    R heat = R(0), cool = R(0);")
@@ -1525,6 +1576,8 @@ def _emit_thermal(w, net, th, pnames):
         w("    heat += heating_chem(Tth, n, k);")
     if "COMPRESS" in th.heating:
         w("    heat += ntot(n) * R(%r) * Tth / R(p.tff);" % BOLTZMANN_ERG)
+    if "PHOTO" in th.heating:
+        w("    heat += heating_photo(n, p);")
     if "H2" in th.cooling:
         if "H2" not in names:
             raise KnetError("H2 cooling without H2")
@@ -1554,6 +1607,8 @@ def _emit_thermal(w, net, th, pnames):
         elif t == "COMPRESS":
             w("    heats[%d] = ntot(n) * R(%r) * Tth / R(p.tff);"
               % (i, BOLTZMANN_ERG))
+        elif t == "PHOTO":
+            w("    heats[%d] = heating_photo(n, p);" % i)
     for i, t in enumerate(cs):
         if t == "H2":
             args = ", ".join("n[%d]" % idx(x) if x in names else "R(0)"
@@ -1631,6 +1686,11 @@ CONFIGS = {
     # 27 species with the metals C and O, density-gated rates
     # (where(ntot > 1e14, ...)) and two database rates; Tgas fixed
     "primordialZ": ("primordialZ.knet", Thermal()),
+    # galaxy_ism with a thermal balance: photoheating of the fields and the
+    # chemical heating against H2 and chemical cooling, Tgas evolved
+    "galaxy_ism_heat": ("galaxy_ism.knet",
+                        Thermal(cooling=("H2", "CHEM"),
+                                heating=("CHEM", "PHOTO"))),
 }
 
 

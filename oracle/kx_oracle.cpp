@@ -31,6 +31,7 @@
 #include "net_primordial3.h"
 #include "net_galaxy_ism.h"
 #include "net_primordialZ.h"
+#include "net_galaxy_ism_heat.h"
 #include "kx_conserve.hpp"
 #include "kx_photo.hpp"
 
@@ -332,21 +333,28 @@ int KXO_(get_threads)(void) {
 }
 
 // a network of krome-x_amd/csrc/generated by name: 0 primordial,
-// 1 primordial3, 2 galaxy_ism, 3 primordialZ, -1 unknown
+// 1 primordial3, 2 galaxy_ism, 3 primordialZ, 4 galaxy_ism_heat, -1
+// unknown
 int KXO_(network)(const char* name) {
   if (!std::strcmp(name, "primordial")) return 0;
   if (!std::strcmp(name, "primordial3")) return 1;
   if (!std::strcmp(name, "galaxy_ism")) return 2;
   if (!std::strcmp(name, "primordialZ")) return 3;
+  if (!std::strcmp(name, "galaxy_ism_heat")) return 4;
   return -1;
 }
 
 // the bin-averaged cross sections sigma[NPHOTO B] of the bins [Elo, Ehi]
-// with the in-bin spectrum E^-alpha (PhotoModule::init)
+// with the in-bin spectrum E^-alpha, then sigmaE[NPHOTO B]
+// (PhotoModule::init)
 int KXO_(photo_bins)(int net, int B, const double* Elo, const double* Ehi,
                      double alpha, double* sigma) {
-  if (net != 2) return -1;
-  kxo_photo::bins<kx::net_galaxy_ism>(B, Elo, Ehi, alpha, sigma);
+  if (net == 2)
+    kxo_photo::bins<kx::net_galaxy_ism>(B, Elo, Ehi, alpha, sigma);
+  else if (net == 4)
+    kxo_photo::bins<kx::net_galaxy_ism_heat>(B, Elo, Ehi, alpha, sigma);
+  else
+    return -1;
   return 0;
 }
 
@@ -355,8 +363,13 @@ int KXO_(photo_bins)(int net, int B, const double* Elo, const double* Ehi,
 int KXO_(photo_rows)(int net, int ncell, const double* par, int pst,
                      const double* Phi, int B, const double* sigma,
                      double* out) {
-  if (net != 2) return -1;
-  kxo_photo::rows<kx::net_galaxy_ism>(ncell, par, pst, Phi, B, sigma, out);
+  if (net == 2)
+    kxo_photo::rows<kx::net_galaxy_ism>(ncell, par, pst, Phi, B, sigma, out);
+  else if (net == 4)
+    kxo_photo::rows<kx::net_galaxy_ism_heat>(ncell, par, pst, Phi, B, sigma,
+                                             out);
+  else
+    return -1;
   return 0;
 }
 
@@ -378,6 +391,10 @@ int KXO_(solve_cells)(int net, int ncell, double* n, double* T,
     case 3:
       return solve_cells<kx::net_primordialZ>(ncell, n, T, dt, par,
                                               par_stride, opts, flags, stats);
+    case 4:
+      return solve_cells<kx::net_galaxy_ism_heat>(ncell, n, T, dt, par,
+                                                  par_stride, opts, flags,
+                                                  stats);
   }
   return -1;
 }
@@ -397,6 +414,9 @@ int KXO_(funcs)(int net, int ncell, const double* n, const double* T,
       return 0;
     case 3:
       funcs<kx::net_primordialZ>(ncell, n, T, par, evolve_T, k, f);
+      return 0;
+    case 4:
+      funcs<kx::net_galaxy_ism_heat>(ncell, n, T, par, evolve_T, k, f);
       return 0;
   }
   return -1;

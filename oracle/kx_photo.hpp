@@ -10,6 +10,9 @@
 //                    / Int_bin E^-alpha dE            (photo.hpp:186-250)
 //   row[NPAR + q]  = sum_b sigma[q B + b] Phi_b, b ascending
 //                                                     (photo.hpp:357-375)
+//   sigmaE[q B + b] = the same average of sigma(E) max(E - Emin_q, 0),
+//   and with PHOTO heating row[NPAR + NPHOTO + q] = sum_b sigmaE Phi_b
+//                                                     (photo.hpp:227-246)
 #pragma once
 #include <cmath>
 #include <vector>
@@ -24,14 +27,18 @@ inline double wint(double alpha, double lo, double hi) {
          (1.0 - alpha);
 }
 
+// sigma[NPHOTO B] then sigmaE[NPHOTO B]
 template <class Net>
 void bins(int B, const double* Elo, const double* Ehi, double alpha,
           double* sigma) {
   const int nq = 64;
+  const size_t nb = static_cast<size_t>(Net::NPHOTO) * B;
   for (int q = 0; q < Net::NPHOTO; ++q)
     for (int b = 0; b < B; ++b) {
       double* out = sigma + static_cast<size_t>(q) * B + b;
+      double* outE = out + nb;
       *out = 0.0;
+      *outE = 0.0;
       const double lo = Elo[b] > Net::photo_Emin[q] ? Elo[b]
                                                     : Net::photo_Emin[q];
       const double hi = Ehi[b] < Net::photo_Emax[q] ? Ehi[b]
@@ -39,18 +46,24 @@ void bins(int B, const double* Elo, const double* Ehi, double alpha,
       if (hi <= lo) continue;
       const double step = (hi - lo) / nq;
       const double lstep = std::log(hi / lo) / nq;
-      double acc = 0.0;
+      double acc = 0.0, accE = 0.0;
       for (int i = 0; i < nq; ++i) {
+        double E, x;
         if (alpha == 0.0) {
-          acc += Net::photo_xsec(q, lo + (i + 0.5) * step) * step;
+          E = lo + (i + 0.5) * step;
+          x = Net::photo_xsec(q, E) * step;
         } else {
           const double e0 = lo * std::exp(i * lstep);
           const double e1 = (i == nq - 1) ? hi : lo * std::exp((i + 1) * lstep);
-          acc += Net::photo_xsec(q, lo * std::exp((i + 0.5) * lstep)) *
-                 wint(alpha, e0, e1);
+          E = lo * std::exp((i + 0.5) * lstep);
+          x = Net::photo_xsec(q, E) * wint(alpha, e0, e1);
         }
+        acc += x;
+        const double Ee = E - Net::photo_Emin[q];
+        accE += x * (Ee > 0.0 ? Ee : 0.0);
       }
       *out = acc / wint(alpha, Elo[b], Ehi[b]);
+      *outE = accE / wint(alpha, Elo[b], Ehi[b]);
     }
 }
 
@@ -62,7 +75,8 @@ void rows(int ncell, const double* par, int pst, const double* Phi, int B,
     double* o = out + static_cast<size_t>(c) * Net::NROW;
     for (int i = 0; i < Net::NPAR; ++i)
       o[i] = par[static_cast<size_t>(c) * pst + i];
-    for (int q = 0; q < Net::NPHOTO; ++q) {
+    const int nq = Net::NPHOTO * (Net::PHOTO_HEAT ? 2 : 1);
+    for (int q = 0; q < nq; ++q) {
       double s = 0.0;
       for (int b = 0; b < B; ++b)
         s += sigma[static_cast<size_t>(q) * B + b] *

@@ -16,7 +16,7 @@ OPT_ORDER = ("evolve_T", "rtol", "atol_n", "atol_T", "linsol", "jac",
              "max_steps", "max_err_fails", "max_conv_fails", "max_restarts",
              "clip_negative", "conserve")
 NET_ID = {"primordial": 0, "primordial3": 1, "galaxy_ism": 2,
-          "primordialZ": 3}
+          "primordialZ": 3, "galaxy_ism_heat": 4}
 
 
 def pytest_configure(config):
@@ -71,13 +71,14 @@ class Oracle:
             k.ctypes.data_as(D), f.ctypes.data_as(D))
         return k, f
 
-    def photo_bins(self, net, elo, ehi, alpha=1.0):
-        """the bin-averaged cross sections [NPHOTO, B] of the bins"""
+    def photo_tables(self, net, elo, ehi, alpha=1.0):
+        """the bin-averaged cross sections [NPHOTO, B] of the bins, then
+        their photoelectron-energy averages: [2 NPHOTO, B]"""
         D = ctypes.POINTER(ctypes.c_double)
         elo = np.ascontiguousarray(elo, dtype=np.float64)
         ehi = np.ascontiguousarray(ehi, dtype=np.float64)
         nph = len(__import__("cells").network_meta(net)["photo"])
-        sig = np.zeros((nph, elo.size))
+        sig = np.zeros((2 * nph, elo.size))
         r = getattr(self.lib, self.p + "photo_bins")(
             NET_ID[net], elo.size, elo.ctypes.data_as(D),
             ehi.ctypes.data_as(D), ctypes.c_double(alpha),
@@ -85,14 +86,23 @@ class Oracle:
         assert r == 0
         return sig
 
+    def photo_bins(self, net, elo, ehi, alpha=1.0):
+        """the bin-averaged cross sections [NPHOTO, B] of the bins"""
+        nph = len(__import__("cells").network_meta(net)["photo"])
+        return self.photo_tables(net, elo, ehi, alpha)[:nph]
+
     def photo_rows(self, net, par, Phi, sigma):
-        """the solve's rows of a photo network: par then the bin sums"""
+        """the solve's rows of a photo network: par then the bin sums
+        (sigma: the tables, both halves with PHOTO heating)"""
         D = ctypes.POINTER(ctypes.c_double)
         par = np.ascontiguousarray(par, dtype=np.float64)
         Phi = np.ascontiguousarray(Phi, dtype=np.float64)
         sigma = np.ascontiguousarray(sigma, dtype=np.float64)
         nc = Phi.shape[0]
-        out = np.zeros((nc, par.shape[-1] + sigma.shape[0]))
+        meta = __import__("cells").network_meta(net)
+        nq = len(meta["photo"]) * (2 if "PHOTO" in meta["heating"] else 1)
+        assert sigma.shape[0] >= nq
+        out = np.zeros((nc, par.shape[-1] + nq))
         r = getattr(self.lib, self.p + "photo_rows")(
             NET_ID[net], nc, par.ctypes.data_as(D),
             par.shape[1] if par.ndim == 2 else 0, Phi.ctypes.data_as(D),

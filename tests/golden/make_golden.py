@@ -59,6 +59,10 @@ TOML = {
                   'outdir = "{out}"\n',
     "primordialZ": '[project]\nnetwork = "{net}/primordialZ.knet"\n'
                    'outdir = "{out}"\n',
+    "galaxy_ism_heat": '[project]\nnetwork = "{net}/galaxy_ism.knet"\n'
+                       'outdir = "{out}"\n[thermal]\n'
+                       'cooling = ["H2", "CHEM"]\n'
+                       'heating = ["CHEM", "PHOTO"]\n',
 }
 # cells of the solve fixtures (the first ones of the seeded workload)
 NSOLVE = {"primordial": 1024, "primordial3": 256, "primordialZ": 256}
@@ -346,6 +350,8 @@ def make_photo(name, exes, meta):
         nph = len(meta["photo"])
         o = nph * B
         out["sigma_a%g" % alpha] = raw[:o].reshape(nph, B)
+        out["sigmaE_a%g" % alpha] = raw[o:2 * o].reshape(nph, B)
+        o *= 2
         out["xsec"] = raw[o:o + nph * len(E)].reshape(nph, len(E))
         o += nph * len(E)
         tail = raw[o:]
@@ -361,8 +367,12 @@ def make_photo(name, exes, meta):
     fw["Phi"] = np.concatenate([w["Phi"][:64],
                                 np.resize(w["Phi"], (nf - 64, B))])
     raw = run_photo(exe, "prates", fw, name + "_prates", 1.0, E)
+    nph = len(meta["photo"])
+    raw = raw.reshape(nf, meta["nrea"] + 34 + nph)
     out.update(fn=fw["n"], fT=fw["T"], fpar=fw["par"], fPhi=fw["Phi"],
-               k=raw.reshape(nf, meta["nrea"]))
+               k=raw[:, :meta["nrea"]],
+               thermal=raw[:, meta["nrea"]:meta["nrea"] + 34],
+               photo_heat=raw[:, meta["nrea"] + 34:])
     opts = np.array([float(w["opts"][k]) for k in OPT_KEYS])
     for m, ex in exes.items():
         raw = run_photo(ex, "psolve", w, "%s_psolve_%s" % (name, m), 1.0, E)
@@ -392,8 +402,9 @@ def main(argv=None):
     parts = set(argv if argv else ("solve", "funcs", "conserve", "sparse",
                                    "photo"))
     BUILD.mkdir(parents=True, exist_ok=True)
-    if "photo" in parts and (not only or "galaxy_ism" in only):
-        name = "galaxy_ism"
+    for name in ("galaxy_ism", "galaxy_ism_heat"):
+        if "photo" not in parts or (only and name not in only):
+            continue
         proj = generate(name)
         exes = {m: compile_driver(proj, name, m)
                 for m in ("bitwise", "production")}

@@ -41,11 +41,16 @@
 //                               i64 B; f64 alpha, Elo[B], Ehi[B],
 //                               Phi[ncell B]; i64 NE; f64 E[NE].  pbins
 //                               OUT: the PhotoModule's bin-averaged
-//                               sigma[NPHOTO B], photo_xsec[r](E)
+//                               sigma[NPHOTO B], sigmaE[NPHOTO B],
+//                               photo_xsec[r](E)
 //                               [NPHOTO NE], NPHOTO, photo_reaction,
 //                               photo_reactant, photo_Emin, photo_Emax;
-//                               prates OUT: krome::get_rates with the
-//                               cell's field, k[ncell NREA]; psolve OUT:
+//                               prates OUT per cell: krome::get_rates
+//                               with the cell's field, k[NREA], then
+//                               krome::get_thermal's heat, cool, dTdt,
+//                               heats[12], cools[19] (the active terms
+//                               first, zeros after) and get_photo_heating
+//                               [NPHOTO]; psolve OUT:
 //                               as solve, with the bins and Phi
 #include "krome.hpp"
 
@@ -303,6 +308,7 @@ int do_photo(const char* mode, const In& in, const PhotoIn& ph,
     krome::params_from_row(in.par.data() + c * in.npar, P[c]);
   if (!std::strcmp(mode, "pbins")) {
     std::fwrite(pm.sigma.data(), sizeof(double), pm.sigma.size(), f);
+    std::fwrite(pm.sigmaE.data(), sizeof(double), pm.sigmaE.size(), f);
     for (int r = 0; r < krome::NPHOTO; ++r)
       for (int64_t e = 0; e < ph.NE; ++e) {
         const double v = krome::photo_xsec[r](ph.E[e]);
@@ -321,10 +327,18 @@ int do_photo(const char* mode, const In& in, const PhotoIn& ph,
     for (int64_t c = 0; c < in.ncell; ++c) {
       for (int64_t b = 0; b < ph.B; ++b) phi[b] = ph.Phi[c * ph.B + b];
       so.Phi = phi.data();
-      if (krome::get_rates(in.n0.data() + c * in.nspec, in.T[c], P[c], so,
-                           k.data()) != 0)
-        return 4;
+      const double* nc = in.n0.data() + c * in.nspec;
+      if (krome::get_rates(nc, in.T[c], P[c], so, k.data()) != 0) return 4;
       std::fwrite(k.data(), sizeof(double), krome::NREA, f);
+      double th[3 + 12 + 19] = {0.0};
+      if (krome::get_thermal(nc, in.T[c], P[c], so, th, th + 1, th + 3,
+                             th + 15, th + 2) != 0)
+        return 5;
+      std::fwrite(th, sizeof(double), 3 + 12 + 19, f);
+      std::vector<double> ph(krome::NPHOTO > 0 ? krome::NPHOTO : 1, 0.0);
+      if (krome::get_photo_heating(nc, in.T[c], P[c], so, ph.data()) != 0)
+        return 6;
+      std::fwrite(ph.data(), sizeof(double), krome::NPHOTO, f);
     }
   } else {
     krome::SolverOpts so = solver_opts(in);

@@ -14,7 +14,8 @@ import pytest
 import cells
 from conftest import PKG, REPO
 
-NETS = ("primordial", "primordial3", "galaxy_ism", "primordialZ")
+NETS = ("primordial", "primordial3", "galaxy_ism", "primordialZ",
+        "galaxy_ism_heat")
 
 
 def declared():
@@ -131,10 +132,10 @@ def test_argument_checks_without_device_work(net):
 def test_element_metadata_vs_reference(net):
     """element names, atom counts, masses and charges equal the tables of
     the reference's generated project (tests/golden/*_conserve.npz,
-    galaxy_ism_tables.npz, primordialZ_tables.npz)"""
+    galaxy_ism(_heat)_tables.npz, primordialZ_tables.npz)"""
     from conftest import golden
     g = golden("%s_%s.npz" % (net.name, "tables" if net.name in (
-        "galaxy_ism", "primordialZ") else "conserve"))
+        "galaxy_ism", "primordialZ", "galaxy_ism_heat") else "conserve"))
     assert net.elements == tuple(g["element_names"])
     assert np.array_equal(net.composition, g["species_elements"])
     assert np.array_equal(net.masses, g["species_mass_g"])
