@@ -17,7 +17,9 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 GEN = CSRC / "generated"
 NETWORKS = ("primordial", "primordial3", "galaxy_ism", "primordialZ",
-            "galaxy_ism_heat")
+            "galaxy_ism_heat", "NO", "SM", "coolHCN", "lowmetal",
+            "primordial2", "primordial_wD", "primordial_photo",
+            "primordial_photoH2")
 # test builds of a network with other compile-time forms: name -> (network,
 # extra flags); primordial_rhsloop evaluates the rhs and the species
 # Jacobian by the loop form over index tables (the form of networks above

@@ -227,9 +227,55 @@ def galaxy_ism_cells(ncell=1 << 18, seed=4, first=0, network="galaxy_ism"):
                      "max_steps": 100000}}
 
 
+def generic_cells(name, ncell=256, seed=5, first=0):
+    """cells of any generated network for parity checks: nH log-uniform in
+    1..1e6 cm^-3, Tgas 10..1e4 K, each species but e- at 1e-8..1e-2 nH
+    (log-uniform, its own draw), e- balancing the charge (at least
+    1e-20 nH), dt log-uniform 1e8..1e12 s, the default parameters; Tgas
+    fixed, rtol 1e-6, the difference-quotient Jacobian where a rate reads
+    the densities (jac 2), else the analytic one; for a network with photo
+    reactions the photon bins and fields of galaxy_ism_cells"""
+    meta = network_meta(name)
+    sp = meta["species"]
+    c = np.arange(first, first + ncell)
+    nH = log_uniform(seed, c, 0, 1.0, 1e6)
+    T = log_uniform(seed, c, 1, 10.0, 1e4)
+    dt = log_uniform(seed, c, 2, 1e8, 1e12)
+    n = np.empty((ncell, len(sp)))
+    for i, s in enumerate(sp):
+        n[:, i] = nH * log_uniform(seed + 1000 * (i + 1), c, 0, 1e-8, 1e-2)
+    if "e-" in sp:
+        e = sp.index("e-")
+        q = np.array(meta["charge"], dtype=float)
+        q[e] = 0.0
+        n[:, e] = np.maximum(n @ q, 1e-20 * nH)
+    par = np.tile(np.array(meta["param_defaults"], dtype=np.float64),
+                  (ncell, 1))
+    w = {"n": n, "T": T, "dt": dt, "par": par,
+         "opts": {"evolve_T": 0, "rtol": 1e-6, "atol_n": 1e-20,
+                  "atol_T": 1e-6, "linsol": 1,
+                  "jac": 2 if meta["rates_density"] else 0,
+                  "max_steps": 100000}}
+    if meta["photo"]:
+        elo, ehi = PHOTO_BINS
+        f = log_uniform(seed, c, 3, 1e-2, 1e2)
+        em = np.sqrt(elo * ehi)
+        phi0 = np.where(ehi <= 13.6, 2e7, 1e4 * (em / 13.6) ** -1.5)
+        w["bins"] = (elo, ehi)
+        w["Phi"] = f[:, None] * phi0[None, :]
+    return w
+
+
+# the other generated networks, each with the generic cells
+EXTRA_NETWORKS = ("NO", "SM", "coolHCN", "lowmetal", "primordial2",
+                  "primordial_wD", "primordial_photo", "primordial_photoH2")
+
 WORKLOADS = {"primordial": primordial_cells, "primordial3": primordial3_cells,
              "c3_primordial": c3_primordial_cells,
              "galaxy_ism": galaxy_ism_cells,
              "galaxy_ism_heat": lambda ncell=1 << 18, seed=4, first=0:
                  galaxy_ism_cells(ncell, seed, first, "galaxy_ism_heat"),
              "c3_primordialZ": c3_primordialZ_cells}
+for _n in EXTRA_NETWORKS:
+    WORKLOADS[_n] = (lambda name: lambda ncell=256, seed=5, first=0:
+                     generic_cells(name, ncell, seed, first))(_n)

@@ -1692,6 +1692,12 @@ CONFIGS = {
                         Thermal(cooling=("H2", "CHEM"),
                                 heating=("CHEM", "PHOTO"))),
 }
+# the other shipped networks this generator covers, at the default thermal
+# options (Tgas fixed), each pinned against the reference's generated code
+# and solver on a generic seeded workload (cells.generic_cells)
+EXTRA = ("NO", "SM", "coolHCN", "lowmetal", "primordial2", "primordial_wD",
+         "primordial_photo", "primordial_photoH2")
+CONFIGS.update({n: ("%s.knet" % n, Thermal()) for n in EXTRA})
 
 
 def main(argv=None):

@@ -32,6 +32,14 @@
 #include "net_galaxy_ism.h"
 #include "net_primordialZ.h"
 #include "net_galaxy_ism_heat.h"
+#include "net_NO.h"
+#include "net_SM.h"
+#include "net_coolHCN.h"
+#include "net_lowmetal.h"
+#include "net_primordial2.h"
+#include "net_primordial_wD.h"
+#include "net_primordial_photo.h"
+#include "net_primordial_photoH2.h"
 #include "kx_conserve.hpp"
 #include "kx_photo.hpp"
 
@@ -306,6 +314,27 @@ void forms(int ncell, const double* n, const double* T, const double* par,
   }
 }
 
+// the networks of krome-x_amd/csrc/generated, by id (the same ids as
+// tests/conftest.py NET_ID)
+#define KXO_NETS(X)                                                        \
+  X(0, primordial) X(1, primordial3) X(2, galaxy_ism) X(3, primordialZ)    \
+  X(4, galaxy_ism_heat) X(5, NO) X(6, SM) X(7, coolHCN) X(8, lowmetal)     \
+  X(9, primordial2) X(10, primordial_wD) X(11, primordial_photo)           \
+  X(12, primordial_photoH2)
+
+// f(Net{}) for network id net, or -1
+template <class F>
+int with_net(int net, F&& f) {
+  switch (net) {
+#define KXO_CASE(id, nm) \
+  case id:               \
+    return f(kx::net_##nm{});
+    KXO_NETS(KXO_CASE)
+#undef KXO_CASE
+  }
+  return -1;
+}
+
 }  // namespace
 
 extern "C" {
@@ -332,30 +361,29 @@ int KXO_(get_threads)(void) {
 #endif
 }
 
-// a network of krome-x_amd/csrc/generated by name: 0 primordial,
-// 1 primordial3, 2 galaxy_ism, 3 primordialZ, 4 galaxy_ism_heat, -1
-// unknown
+// a network's id by name, or -1
 int KXO_(network)(const char* name) {
-  if (!std::strcmp(name, "primordial")) return 0;
-  if (!std::strcmp(name, "primordial3")) return 1;
-  if (!std::strcmp(name, "galaxy_ism")) return 2;
-  if (!std::strcmp(name, "primordialZ")) return 3;
-  if (!std::strcmp(name, "galaxy_ism_heat")) return 4;
+#define KXO_NAME(id, nm) \
+  if (!std::strcmp(name, #nm)) return id;
+  KXO_NETS(KXO_NAME)
+#undef KXO_NAME
   return -1;
 }
 
 // the bin-averaged cross sections sigma[NPHOTO B] of the bins [Elo, Ehi]
 // with the in-bin spectrum E^-alpha, then sigmaE[NPHOTO B]
-// (PhotoModule::init)
+// (PhotoModule::init); -1 for a network without photo reactions
 int KXO_(photo_bins)(int net, int B, const double* Elo, const double* Ehi,
                      double alpha, double* sigma) {
-  if (net == 2)
-    kxo_photo::bins<kx::net_galaxy_ism>(B, Elo, Ehi, alpha, sigma);
-  else if (net == 4)
-    kxo_photo::bins<kx::net_galaxy_ism_heat>(B, Elo, Ehi, alpha, sigma);
-  else
-    return -1;
-  return 0;
+  return with_net(net, [&](auto tag) {
+    using Net = decltype(tag);
+    if constexpr (Net::NPHOTO == 0) {
+      return -1;
+    } else {
+      kxo_photo::bins<Net>(B, Elo, Ehi, alpha, sigma);
+      return 0;
+    }
+  });
 }
 
 // the rows the solve reads for a photo network: par[ncell][NPAR] (stride
@@ -363,14 +391,15 @@ int KXO_(photo_bins)(int net, int B, const double* Elo, const double* Ehi,
 int KXO_(photo_rows)(int net, int ncell, const double* par, int pst,
                      const double* Phi, int B, const double* sigma,
                      double* out) {
-  if (net == 2)
-    kxo_photo::rows<kx::net_galaxy_ism>(ncell, par, pst, Phi, B, sigma, out);
-  else if (net == 4)
-    kxo_photo::rows<kx::net_galaxy_ism_heat>(ncell, par, pst, Phi, B, sigma,
-                                             out);
-  else
-    return -1;
-  return 0;
+  return with_net(net, [&](auto tag) {
+    using Net = decltype(tag);
+    if constexpr (Net::NPHOTO == 0) {
+      return -1;
+    } else {
+      kxo_photo::rows<Net>(ncell, par, pst, Phi, B, sigma, out);
+      return 0;
+    }
+  });
 }
 
 // krome_solve_cells semantics over per-cell parameter rows (par_stride 0:
@@ -378,83 +407,44 @@ int KXO_(photo_rows)(int net, int ncell, const double* par, int pst,
 int KXO_(solve_cells)(int net, int ncell, double* n, double* T,
                       const double* dt, const double* par, int par_stride,
                       const double* opts, int* flags, long* stats) {
-  switch (net) {
-    case 0:
-      return solve_cells<kx::net_primordial>(ncell, n, T, dt, par,
-                                             par_stride, opts, flags, stats);
-    case 1:
-      return solve_cells<kx::net_primordial3>(ncell, n, T, dt, par,
-                                              par_stride, opts, flags, stats);
-    case 2:
-      return solve_cells<kx::net_galaxy_ism>(ncell, n, T, dt, par,
-                                             par_stride, opts, flags, stats);
-    case 3:
-      return solve_cells<kx::net_primordialZ>(ncell, n, T, dt, par,
-                                              par_stride, opts, flags, stats);
-    case 4:
-      return solve_cells<kx::net_galaxy_ism_heat>(ncell, n, T, dt, par,
-                                                  par_stride, opts, flags,
-                                                  stats);
-  }
-  return -1;
+  return with_net(net, [&](auto tag) {
+    return solve_cells<decltype(tag)>(ncell, n, T, dt, par, par_stride,
+                                      opts, flags, stats);
+  });
 }
 
 // rates k[ncell NREA] and the full rhs f[ncell (NSPEC + 1)] at states
 int KXO_(funcs)(int net, int ncell, const double* n, const double* T,
                 const double* par, int evolve_T, double* k, double* f) {
-  switch (net) {
-    case 0:
-      funcs<kx::net_primordial>(ncell, n, T, par, evolve_T, k, f);
-      return 0;
-    case 1:
-      funcs<kx::net_primordial3>(ncell, n, T, par, evolve_T, k, f);
-      return 0;
-    case 2:
-      funcs<kx::net_galaxy_ism>(ncell, n, T, par, evolve_T, k, f);
-      return 0;
-    case 3:
-      funcs<kx::net_primordialZ>(ncell, n, T, par, evolve_T, k, f);
-      return 0;
-    case 4:
-      funcs<kx::net_galaxy_ism_heat>(ncell, n, T, par, evolve_T, k, f);
-      return 0;
-  }
-  return -1;
+  return with_net(net, [&](auto tag) {
+    funcs<decltype(tag)>(ncell, n, T, par, evolve_T, k, f);
+    return 0;
+  });
 }
 
 // the fixed-Tgas analytic species Jacobian J[ncell][NSPEC^2] (column-
 // major per cell) at states n, T
 int KXO_(jac)(int net, int ncell, const double* n, const double* T,
               const double* par, double* J) {
-  switch (net) {
-    case 0:
-      jacs<kx::net_primordial>(ncell, n, T, par, J);
-      return 0;
-    case 1:
-      jacs<kx::net_primordial3>(ncell, n, T, par, J);
-      return 0;
-  }
-  return -1;
+  return with_net(net, [&](auto tag) {
+    jacs<decltype(tag)>(ncell, n, T, par, J);
+    return 0;
+  });
 }
 
 // the species rhs and Jacobian in both generated forms (see forms())
 int KXO_(rhs_forms)(int net, int ncell, const double* n, const double* T,
                     const double* par, double* f, double* J) {
-  switch (net) {
-    case 0:
-      forms<kx::net_primordial>(ncell, n, T, par, f, J);
-      return 0;
-    case 1:
-      forms<kx::net_primordial3>(ncell, n, T, par, f, J);
-      return 0;
-  }
-  return -1;
+  return with_net(net, [&](auto tag) {
+    forms<decltype(tag)>(ncell, n, T, par, f, J);
+    return 0;
+  });
 }
 
 // element totals tot[ncell NELEM] of the rows v[ncell NSPEC]: densities
 // (x 0) or mass fractions (x 1, per gram)
 int KXO_(totals)(int net, int x, int ncell, const double* v, double* tot) {
-  auto run = [&](auto tag) {
+  return with_net(net, [&](auto tag) {
     using Net = decltype(tag);
     for (int c = 0; c < ncell; ++c) {
       const double* vc = v + static_cast<size_t>(c) * Net::NSPEC;
@@ -463,14 +453,7 @@ int KXO_(totals)(int net, int x, int ncell, const double* v, double* tot) {
       else kxo_cons::totals<Net>(vc, tc);
     }
     return 0;
-  };
-  switch (net) {
-    case 0: return run(kx::net_primordial{});
-    case 1: return run(kx::net_primordial3{});
-    case 2: return run(kx::net_galaxy_ism{});
-    case 3: return run(kx::net_primordialZ{});
-  }
-  return -1;
+  });
 }
 
 // consistent advection of the rows v[ncell NSPEC] to ref (ref_rows 1: a
@@ -478,21 +461,10 @@ int KXO_(totals)(int net, int x, int ncell, const double* v, double* tot) {
 // cell's flag, else 0 (krome_conserve_cells / _x)
 int KXO_(conserve)(int net, int x, int ncell, double* v, const double* ref,
                    int ref_rows, int* flags) {
-  switch (net) {
-    case 0:
-      return kxo_cons::cells<kx::net_primordial>(ncell, v, ref, ref_rows,
-                                                 x != 0, flags);
-    case 1:
-      return kxo_cons::cells<kx::net_primordial3>(ncell, v, ref, ref_rows,
-                                                  x != 0, flags);
-    case 2:
-      return kxo_cons::cells<kx::net_galaxy_ism>(ncell, v, ref, ref_rows,
-                                                 x != 0, flags);
-    case 3:
-      return kxo_cons::cells<kx::net_primordialZ>(ncell, v, ref, ref_rows,
-                                                  x != 0, flags);
-  }
-  return -1;
+  return with_net(net, [&](auto tag) {
+    return kxo_cons::cells<decltype(tag)>(ncell, v, ref, ref_rows, x != 0,
+                                          flags);
+  });
 }
 
 }  // extern "C"

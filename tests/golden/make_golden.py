@@ -393,12 +393,39 @@ def make_photo(name, exes, meta):
                         **out)
 
 
+def make_extra(name, exe):
+    """a generated network beyond the benchmark ones: the reference's
+    rates and rhs at the generic cells and the T grid, and its solves of
+    256 generic cells (with the photon fields for a photo network)"""
+    meta = cells.network_meta(name)
+    ns = len(meta["species"])
+    w = cells.WORKLOADS[name](256)
+    fw = func_samples(name)
+    if meta["photo"]:
+        exes = {"bitwise": exe}
+        make_photo(name, exes, meta)
+        return
+    k, f, J, th = run_funcs(exe, fw, name + "_funcs", meta["nrea"], ns)
+    n, T, flag, stats, secs = run_solve(exe, w, name + "_bitwise")
+    np.savez_compressed(HERE / ("%s_extra.npz" % name), fn=fw["n"],
+                        fT=fw["T"], fpar=fw["par"], k=k, f=f, n0=w["n"],
+                        T0=w["T"], dt=w["dt"], par=w["par"],
+                        opts=np.array([float(w["opts"][o])
+                                       for o in OPT_KEYS]),
+                        n_bitwise=n, T_bitwise=T, flag_bitwise=flag,
+                        stats_bitwise=stats)
+    print("%s: funcs %d, solve flags %s, steps %d..%d" % (
+        name, k.shape[0], np.unique(flag), stats[:, 0].min(),
+        stats[:, 0].max()))
+
+
 def main(argv=None):
     """argv: the parts to write (solve funcs conserve sparse photo;
     default all) and, optionally, the networks (default all)"""
     argv = list(argv or [])
-    only = {a for a in argv if a in TOML}
-    argv = [a for a in argv if a not in TOML]
+    names = set(TOML) | set(cells.EXTRA_NETWORKS)
+    only = {a for a in argv if a in names}
+    argv = [a for a in argv if a not in names]
     parts = set(argv if argv else ("solve", "funcs", "conserve", "sparse",
                                    "photo"))
     BUILD.mkdir(parents=True, exist_ok=True)
@@ -413,6 +440,13 @@ def main(argv=None):
         w = cells.WORKLOADS[name](64)
         np.savez_compressed(HERE / ("%s_tables.npz" % name),
                             **ref_tables(exes["bitwise"], name, w))
+    for name in cells.EXTRA_NETWORKS:
+        if "extra" not in parts or (only and name not in only):
+            continue
+        TOML.setdefault(name, '[project]\nnetwork = "{net}/%s.knet"\n'
+                        'outdir = "{out}"\n' % name)
+        proj = generate(name)
+        make_extra(name, compile_driver(proj, name, "bitwise"))
     for name in ("primordial", "primordial3", "primordialZ"):
         if not parts & {"solve", "funcs", "conserve", "sparse"}:
             break
