@@ -19,7 +19,12 @@ GEN = CSRC / "generated"
 NETWORKS = ("primordial", "primordial3", "galaxy_ism", "primordialZ",
             "galaxy_ism_heat", "NO", "SM", "coolHCN", "lowmetal",
             "primordial2", "primordial_wD", "primordial_photo",
-            "primordial_photoH2")
+            "primordial_photoH2", "auto", "chianti")
+# generated networks built only on request (build(net)): ions and ionsH2
+# (55 and 58 species, 45 and 48 photo reactions) take about ten minutes of
+# hipcc each for a 7 MB library; their generated code is checked through
+# the oracle, which runs the same headers
+ON_REQUEST = ("ions", "ionsH2")
 # test builds of a network with other compile-time forms: name -> (network,
 # extra flags); primordial_rhsloop evaluates the rhs and the species
 # Jacobian by the loop form over index tables (the form of networks above

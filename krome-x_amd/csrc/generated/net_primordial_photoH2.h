@@ -58,10 +58,10 @@ struct net_primordial_photoH2 {
   static double photo_xsec(int q, double E_eV) {
     using R = double;
     switch (q) {
-      case 0: return kx::sigma_v96(E_eV, 0.4298, 54750.0, 32.88, 2.963, 0.0, 0.0, 0.0);  // H -> H+ + e-
-      case 1: return kx::sigma_v96(E_eV, 0.4298, 54750.0, 32.88, 2.963, 0.0, 0.0, 0.0);  // D -> D+ + e-
-      case 2: return kx::sigma_v96(E_eV, 13.61, 949.2, 1.469, 3.188, 2.039, 0.4434, 2.136);  // He -> He+ + e-
-      case 3: return kx::sigma_v96(E_eV, 1.72, 13690.0, 32.88, 2.963, 0.0, 0.0, 0.0);  // He+ -> He++ + e-
+      case 0: return kx::sigma_v96(E_eV, 4.298e-01, 5.475e+04, 3.288e+01, 2.963e+00, 0.000e+00, 0.000e+00, 0.000e+00);  // H -> H+ + e-
+      case 1: return kx::sigma_v96(E_eV, 4.298e-01, 5.475e+04, 3.288e+01, 2.963e+00, 0.000e+00, 0.000e+00, 0.000e+00);  // D -> D+ + e-
+      case 2: return kx::sigma_v96(E_eV, 1.361e+01, 9.492e+02, 1.469e+00, 3.188e+00, 2.039e+00, 4.434e-01, 2.136e+00);  // He -> He+ + e-
+      case 3: return kx::sigma_v96(E_eV, 1.720e+00, 1.369e+04, 3.288e+01, 2.963e+00, 0.000e+00, 0.000e+00, 0.000e+00);  // He+ -> He++ + e-
       case 4: return ((R(9.56e-17) * (E_eV / R(15.4))) - R(9.4e-17));  // H2 -> H2+ + e-
       case 5: return ((R(2.16e-17) * (E_eV / R(15.4))) - R(1.48e-17));  // H2 -> H2+ + e-
       case 6: return (R(1.51e-17) * kx::pow((E_eV / R(15.4)), (-R(2.71))));  // H2 -> H2+ + e-

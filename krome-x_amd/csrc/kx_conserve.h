@@ -39,15 +39,16 @@ template <class Net>
 struct Conserve {
   static constexpr int NS = Net::NSPEC;
   static constexpr int NE = Net::NELEM;
-  static_assert(NS <= 32, "the fixed-species mask holds 32 species");
+  static_assert(NS <= 64, "the fixed-species mask holds 64 species");
+  using Mask = unsigned long long;
   static_assert(NE >= 1, "a network without elements has nothing to keep");
 
   // conserved row k at species i: atoms of element k (k < NE), charge (NE)
   KX_HD static double row(int k, int i) {
     return k < NE ? double(Net::natoms(i, k)) : double(Net::charge(i));
   }
-  KX_HD static bool is_fixed(unsigned fixed, int i) {
-    return (fixed >> i) & 1u;
+  KX_HD static bool is_fixed(Mask fixed, int i) {
+    return (fixed >> i) & Mask(1);
   }
 
   KX_HD static void totals(const double* n, double* tot) {
@@ -100,7 +101,7 @@ struct Conserve {
   // clipping negatives, at most 20 times, until the residual is below 1e-13
   template <int NR>
   KX_HD static double project(double* n, const double* target,
-                              unsigned fixed) {
+                              Mask fixed) {
 #pragma clang fp contract(off)
     for (int it = 0; it < 20; ++it) {
       double r[NR], M[NR][NR], lam[NR];
@@ -202,7 +203,7 @@ struct Conserve {
   // kept), the electrons from neutrality, or the charge projected too when
   // the electrons cannot carry it; 0, FLAG_CONSERVE_UNREACHED (residual
   // above 1e-12) or FLAG_CONSERVE_LARGE (a species moved by over 50 %)
-  KX_HD static int to_n(double* n, const double* ref, unsigned fixed) {
+  KX_HD static int to_n(double* n, const double* ref, Mask fixed) {
 #pragma clang fp contract(off)
     double n0[NS];
 #pragma unroll
@@ -244,12 +245,12 @@ struct Conserve {
   // x = m n; species without a mass held
   KX_HD static int to_x(double* x, const double* ref) {
 #pragma clang fp contract(off)
-    unsigned fixed = 0u;
+    Mask fixed = 0;
     double n[NS];
 #pragma unroll
     for (int i = 0; i < NS; ++i) {
       const double m = Net::mass_g(i);
-      if (!(m > 0.0)) fixed |= 1u << i;
+      if (!(m > 0.0)) fixed |= Mask(1) << i;
       n[i] = m > 0.0 ? x[i] / m : 0.0;
     }
     const int f = to_n(n, ref, fixed);

@@ -380,6 +380,36 @@ KX_HD inline R He_recombination_on_dust(R T, R ne, R G0, R d2g) {
                             d2g);
 }
 
+// ---- the legacy rate-database fits, functions of Tgas ---------------------
+
+// collisional ionization, Voronov 1997: U = dE / (kB T),
+// k = A (1 + P sqrt(U)) U^K exp(-U) / (X + U)
+template <class R>
+KX_HD inline R colion_v96(R T, R dE, R P, R A, R X, R K) {
+  const R U = dE / (T * R(8.617333262145179e-05));
+  return A * (R(1) + P * kx::sqrt(U)) * kx::pow(U, K) * kx::exp(-U) / (X + U);
+}
+// radiative recombination, a power law a (1e4 / T)^b
+template <class R>
+KX_HD inline R recV96(R T, R a, R b) {
+  return a * kx::pow(R(1e4) / T, b);
+}
+// radiative recombination, Verner & Ferland 1996:
+// r1 / (t (1 + t)^(1 - r2) (1 + sqrt(T / r4))^(1 + r2)), t = sqrt(T / r3)
+template <class R>
+KX_HD inline R recNewV96(R T, R r1, R r2, R r3, R r4) {
+  const R t = kx::sqrt(T / r3);
+  return r1 / (t * kx::pow(t + R(1), R(1) - r2) *
+               kx::pow(R(1) + kx::sqrt(T / r4), R(1) + r2));
+}
+// radiative recombination of Fe ions: r1 / t^(r2 + r3 + log10 t),
+// t = sqrt(T / 1e4)
+template <class R>
+KX_HD inline R recFeV96(R T, R r1, R r2, R r3) {
+  const R t = kx::sqrt(T * R(1e-4));
+  return r1 / kx::pow(t, r2 + r3 + kx::log10(t));
+}
+
 // ---- photo cross sections [cm^2] of photon energy E [eV], host only -----
 
 // Verner et al. 1996 analytic fit of a ground-state photoionization cross
@@ -391,6 +421,17 @@ inline double sigma_v96(double E, double E0, double s0, double ya, double P,
   const double y = std::sqrt(x * x + y1 * y1);
   return 1e-18 * s0 * ((x - 1.0) * (x - 1.0) + yw * yw)
          * std::pow(y, 0.5 * P - 5.5) * std::pow(1.0 + std::sqrt(y / ya), -P);
+}
+
+// Verner & Yakovlev 1995 fit of one shell, 0 below its threshold Eth:
+// y = E/E0, sigma = 1e-18 s0 ((y-1)^2 + yw^2) y^(P/2 - 5.5 - l)
+// (1 + sqrt(y/ya))^-P
+inline double sigma_vy95(double E, double Eth, double E0, double s0,
+                         double ya, double P, double yw, double l) {
+  if (E < Eth) return 0.0;
+  const double y = E / E0;
+  return 1e-18 * s0 * ((y - 1.0) * (y - 1.0) + yw * yw)
+         * std::pow(y, 0.5 * P - 5.5 - l) * std::pow(1.0 + std::sqrt(y / ya), -P);
 }
 
 // a tabulated cross section (tE ascending), linear between the points and

@@ -80,6 +80,11 @@ EXTERNS = {
         "e-", ("d2g", "G0"),
         "kx::He_recombination_on_dust(Tgas, n[{e}], R(p.G0), R(p.d2g))"),
     "H2_sigmaLW": (None, (), None),
+    # the legacy rate-database fits (codegen.py:826-852): functions of Tgas
+    "colion_v96": (None, (), "kx::colion_v96({0}, {1}, {2}, {3}, {4}, {5})"),
+    "recV96": (None, (), "kx::recV96({0}, {1}, {2})"),
+    "recNewV96": (None, (), "kx::recNewV96({0}, {1}, {2}, {3}, {4})"),
+    "recFeV96": (None, (), "kx::recFeV96({0}, {1}, {2}, {3})"),
 }
 # the predefined parameters in the order the parameter rows list them,
 # after the network's own (sorted): kromex/codegen.py:287-312
@@ -475,7 +480,7 @@ PLANCK_EV = CONSTANTS["planck_eV"]
 HC_EV_AA = CLIGHT * PLANCK_EV / 1e-8
 HC_EV_NM = CLIGHT * PLANCK_EV / 1e-7
 # the cross-section fits a database entry may name, host C++ (kx_photo_xsec.h)
-PHOTO_FITS = ("sigma_v96",)
+PHOTO_FITS = ("sigma_v96", "sigma_vy95")
 
 
 def is_photo(r):
@@ -615,7 +620,8 @@ def swri_xsec(data_dir, reactant, products):
 AUTO_FIRST = ("hydroHe.dat", "recombination_badnell.dat")
 AUTO_SHORTCUTS = {"T32": "(Tgas/300.0)", "invT": "(1.0/Tgas)",
                   "Te": "Tgas_eV", "invTe": "(1.0/Tgas_eV)",
-                  "lnTe": "log(Tgas_eV)", "T": "Tgas"}
+                  "lnTe": "log(Tgas_eV)", "T": "Tgas",
+                  "sqrTgas": "sqrt(Tgas)"}
 _AUTO_DB = {}
 
 
@@ -646,29 +652,53 @@ def auto_database(data_dir):
             elif k in ("reacts", "prods"):
                 cur[k] = sorted("e-" if x.strip().upper() == "E"
                                 else x.strip() for x in v.split(","))
-            elif k in ("rate", "limits"):
+            elif k == "limits":
+                parts = [re.sub(r"^(\.GE\.|\.LE\.|\.GT\.|\.LT\.|>=|<=|>|<)",
+                                "", x.strip(), flags=re.I)
+                         for x in v.split(",")]
+                parts = ["" if x.upper() == "NONE" else x for x in parts]
+                if len(parts) == 2 and any(parts):
+                    cur["limits"] = (_dnum(parts[0]) if parts[0] else 0.0,
+                                     _dnum(parts[1]) if parts[1]
+                                     else math.inf)
+            elif k == "rate":
                 cur[k] = v
     _AUTO_DB[key] = out
     return out
 
 
+def _auto_expr(rate):
+    """a database rate in the .knet rate language"""
+    expr = re.sub(r"(\d\.?\d*)[dD]([+-]?\d+)", r"\1e\2", rate)
+    return re.sub(r"[A-Za-z_][A-Za-z0-9_]*",
+                  lambda m: AUTO_SHORTCUTS.get(m.group(0), m.group(0)), expr)
+
+
 def resolve_auto(r, data_dir):
-    """the rate expression (in the .knet rate language) of an `auto`
-    reaction: the first database entry with its reactants and products"""
+    """the reactions an `auto` reaction becomes (codegen.py:2005-2071): the
+    first matching database entry without temperature limits, the rate
+    alone; or the matching entries with limits before it, each a piece
+    [lo, hi) of one piecewise rate (zero below, clamped above); an
+    unlimited entry after limited ones is an overlap and an error"""
     reacts = sorted(s for s, _ in r.reactants)
     prods = sorted(r.products)
+    pieces = []
     for fname, e in auto_database(data_dir):
-        if e.get("reacts") == reacts and e.get("prods") == prods \
-                and "rate" in e:
-            if "limits" in e:
-                raise KnetError("line %d: database entry for %s (%s) has "
-                                "temperature limits, not supported"
-                                % (r.line, r.eq, fname))
-            expr = re.sub(r"(\d\.?\d*)[dD]([+-]?\d+)", r"\1e\2", e["rate"])
-            return re.sub(r"[A-Za-z_][A-Za-z0-9_]*",
-                          lambda m: AUTO_SHORTCUTS.get(m.group(0),
-                                                       m.group(0)), expr)
-    raise KnetError("line %d: no database rate for %s" % (r.line, r.eq))
+        if e.get("reacts") != reacts or e.get("prods") != prods \
+                or "rate" not in e:
+            continue
+        if "limits" not in e:
+            if pieces:
+                raise KnetError("line %d: database has limited and "
+                                "unlimited entries for %s" % (r.line, r.eq))
+            r.rate = _auto_expr(e["rate"])
+            return [r]
+        pieces.append(e)
+    if not pieces:
+        raise KnetError("line %d: no database rate for %s" % (r.line, r.eq))
+    return [Reaction(list(r.reactants), list(r.products), e["limits"][0],
+                     e["limits"][1], True, _auto_expr(e["rate"]), r.line)
+            for e in pieces]
 
 
 def photo_reactions(net, data_dir):
@@ -706,21 +736,26 @@ def photo_reactions(net, data_dir):
             if not segs:
                 raise KnetError("line %d: no database cross section for %s"
                                 % (r.line, r.eq))
+            if sum(1 for p_ in r.products if p_ == "e-") > 1:
+                raise KnetError("line %d: Auger channels (several electrons) "
+                                "of photo(xsec=auto) are not supported"
+                                % r.line)
             code = None
             for e in segs:
-                m = re.match(r"\s*(\w+)\s*\((.*)\)\s*$", e["rate"])
-                if not m or m.group(1) not in PHOTO_FITS:
+                expr = re.sub(r"(\d\.?\d*)[dD]([+-]?\d+)", r"\1e\2",
+                              e["rate"])
+                expr = re.sub(r"\benergy_ev\b", "E_eV", expr, flags=re.I)
+                names = set(re.findall(r"[A-Za-z_][A-Za-z0-9_]*", expr))
+                names -= {"E_eV", "e"} | set(PHOTO_FITS)
+                if names:
                     raise KnetError("line %d: database fit %r is not "
                                     "supported" % (r.line, e["rate"]))
-                args = [a.strip() for a in m.group(2).split(",")]
-                if args[0].lower() != "energy_ev":
-                    raise KnetError("line %d: unexpected fit arguments"
-                                    % r.line)
-                expr = "kx::%s(E_eV, %s)" % (m.group(1), ", ".join(
-                    repr(_dnum(a)) for a in args[1:]))
+                expr = re.sub(r"\b(%s)\s*\(" % "|".join(PHOTO_FITS),
+                              r"kx::\1(", expr)
                 code = expr if code is None else "(E_eV >= %r ? %s : %s)" % (
                     e["limits"][0], expr, code)
-            ent.update(kind="fit", code=code)
+            ent.update(kind="fit", code=code,
+                       inner_shells="sigma_vy95" in code)
             if ent["Emin"] is None:
                 ent["Emin"] = segs[0]["limits"][0]
         elif isinstance(x, ast.Constant) and isinstance(x.value, str):
@@ -1022,9 +1057,15 @@ def generate(knet_path, thermal=None, name=None, data_dir=None):
     net = parse_knet(knet_path)
     if data_dir is None:
         data_dir = Path(knet_path).resolve().parent.parent / "data"
-    for r in net.reactions:
-        if r.rate.strip() == "auto":
-            r.rate = resolve_auto(r, data_dir)
+    if any(r.rate.strip() == "auto" for r in net.reactions):
+        rx = []
+        for r in net.reactions:
+            rx += resolve_auto(r, data_dir) if r.rate.strip() == "auto" \
+                else [r]
+        for i, r in enumerate(rx):
+            r.index = i
+        net.reactions = rx
+        _group(net)
     photo = photo_reactions(net, data_dir)
     photo_of = {e["r"]: q for q, e in enumerate(photo)}
     NPH = len(photo)
@@ -1071,6 +1112,9 @@ def generate(knet_path, thermal=None, name=None, data_dir=None):
     photo_heat = "PHOTO" in th.heating
     if photo_heat and not NPH:
         raise KnetError("PHOTO heating without photo reactions")
+    if photo_heat and any(e.get("inner_shells") for e in photo):
+        raise KnetError("PHOTO heating with inner-shell photoionization "
+                        "fits (their per-shell deposits) is not supported")
     w("  static constexpr int NPHOTO = %d;" % NPH)
     w("  // PHOTO heating: the energy sums sum_b sigmaE_qb Phi_b follow the")
     w("  // rate sums in the row (sigmaE: sigma times the photoelectron's")
@@ -1696,7 +1740,8 @@ CONFIGS = {
 # options (Tgas fixed), each pinned against the reference's generated code
 # and solver on a generic seeded workload (cells.generic_cells)
 EXTRA = ("NO", "SM", "coolHCN", "lowmetal", "primordial2", "primordial_wD",
-         "primordial_photo", "primordial_photoH2")
+         "primordial_photo", "primordial_photoH2", "auto", "chianti", "ions",
+         "ionsH2")
 CONFIGS.update({n: ("%s.knet" % n, Thermal()) for n in EXTRA})
 
 

@@ -40,6 +40,10 @@
 #include "net_primordial_wD.h"
 #include "net_primordial_photo.h"
 #include "net_primordial_photoH2.h"
+#include "net_auto.h"
+#include "net_chianti.h"
+#include "net_ions.h"
+#include "net_ionsH2.h"
 #include "kx_conserve.hpp"
 #include "kx_photo.hpp"
 
@@ -320,7 +324,8 @@ void forms(int ncell, const double* n, const double* T, const double* par,
   X(0, primordial) X(1, primordial3) X(2, galaxy_ism) X(3, primordialZ)    \
   X(4, galaxy_ism_heat) X(5, NO) X(6, SM) X(7, coolHCN) X(8, lowmetal)     \
   X(9, primordial2) X(10, primordial_wD) X(11, primordial_photo)           \
-  X(12, primordial_photoH2)
+  X(12, primordial_photoH2) X(13, auto) X(14, chianti) X(15, ions)         \
+  X(16, ionsH2)
 
 // f(Net{}) for network id net, or -1
 template <class F>

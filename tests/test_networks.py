@@ -105,18 +105,17 @@ def test_oracle_solve_vs_reference(oracle, net):
 @pytest.fixture(scope="module")
 def libs():
     import krome
-    out = {}
-    for n in NETS:
-        if not (PKG / ("libkrome_%s.so" % n)).exists():
-            pytest.skip("libkrome_%s.so not built" % n)
-        out[n] = krome.load(n)
-    return out
+    return {n: krome.load(n) for n in NETS
+            if (PKG / ("libkrome_%s.so" % n)).exists()}
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("kernel", ["plain", "persistent"])
 @pytest.mark.parametrize("net", NETS)
 def test_gpu_solve_vs_reference(libs, oracle, net, kernel):
+    if net not in libs:
+        # ions, ionsH2: built on request only (krome-x_amd/build.py)
+        pytest.skip("libkrome_%s.so is built on request only" % net)
     g = fixture(net)
     N = libs[net]
     o = g["opts"]
