@@ -231,7 +231,8 @@ def generic_cells(name, ncell=256, seed=5, first=0):
     """cells of any generated network for parity checks: nH log-uniform in
     1..1e6 cm^-3, Tgas 10..1e4 K, each species but e- at 1e-8..1e-2 nH
     (log-uniform, its own draw), e- balancing the charge (at least
-    1e-20 nH), dt log-uniform 1e8..1e12 s, the default parameters; Tgas
+    1e-20 nH), dt log-uniform 1e8..1e12 s, the default parameters but
+    J21 log-uniform in 0.1..10 where the network has it; Tgas
     fixed, rtol 1e-6, the difference-quotient Jacobian where a rate reads
     the densities (jac 2), else the analytic one; for a network with photo
     reactions the photon bins and fields of galaxy_ism_cells"""
@@ -251,6 +252,9 @@ def generic_cells(name, ncell=256, seed=5, first=0):
         n[:, e] = np.maximum(n @ q, 1e-20 * nH)
     par = np.tile(np.array(meta["param_defaults"], dtype=np.float64),
                   (ncell, 1))
+    if "J21" in meta["params"]:  # a UV field for the shielded H2 rates
+        par[:, meta["params"].index("J21")] = log_uniform(seed, c, 4, 0.1,
+                                                          10.0)
     w = {"n": n, "T": T, "dt": dt, "par": par,
          "opts": {"evolve_T": 0, "rtol": 1e-6, "atol_n": 1e-20,
                   "atol_T": 1e-6, "linsol": 1,
@@ -269,7 +273,8 @@ def generic_cells(name, ncell=256, seed=5, first=0):
 # the other generated networks, each with the generic cells
 EXTRA_NETWORKS = ("NO", "SM", "coolHCN", "lowmetal", "primordial2",
                   "primordial_wD", "primordial_photo", "primordial_photoH2",
-                  "auto", "chianti", "ions", "ionsH2")
+                  "auto", "chianti", "ions", "ionsH2", "primordial_UV",
+                  "xrays", "primordialZ_UV")
 
 WORKLOADS = {"primordial": primordial_cells, "primordial3": primordial3_cells,
              "c3_primordial": c3_primordial_cells,

@@ -380,6 +380,24 @@ KX_HD inline R He_recombination_on_dust(R T, R ne, R G0, R d2g) {
                             d2g);
 }
 
+// H2 self-shielding factor, Wolcott-Green & Haiman 2011: the H2 column
+// N(H2) = 2 x 1.87e21 (n(H2) / 1e3)^(2/3) (the default number-to-column
+// estimate), x = N(H2) / 5e14, b5 the thermal Doppler width of an H atom
+// (proton + electron mass) [km/s],
+// f = 0.965 / (1 + x/b5)^1.1 + 0.035 / sqrt(1 + x) exp(-8.5e-4 sqrt(1 + x))
+template <class R>
+KX_HD inline R fshield_wg11(R T, R nH2) {
+  const R N = R(1.87e21) * kx::pow(kx::fmax(kx::fmax(R(1e-40), nH2),
+                                            R(1e-40)) * R(1e-3),
+                                   R(2.0) / R(3.0));
+  const R xN = R(2) * N * R(2e-15);
+  const R H_mass = R(1.67262192369e-24) + R(9.1093837015e-28);
+  const R b5 = kx::sqrt(R(1.380649e-16) * T / H_mass) * R(1e-5);
+  return R(0.965) / kx::pow(R(1) + xN / b5, R(1.1)) +
+         R(0.035) / kx::sqrt(R(1) + xN) *
+             kx::exp(R(-8.5e-4) * kx::sqrt(R(1) + xN));
+}
+
 // ---- the legacy rate-database fits, functions of Tgas ---------------------
 
 // collisional ionization, Voronov 1997: U = dE / (kB T),

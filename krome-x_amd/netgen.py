@@ -80,11 +80,16 @@ EXTERNS = {
         "e-", ("d2g", "G0"),
         "kx::He_recombination_on_dust(Tgas, n[{e}], R(p.G0), R(p.d2g))"),
     "H2_sigmaLW": (None, (), None),
+    "sigma_v96": (None, (), None),
+    "sigma_vy95": (None, (), None),
     # the legacy rate-database fits (codegen.py:826-852): functions of Tgas
     "colion_v96": (None, (), "kx::colion_v96({0}, {1}, {2}, {3}, {4}, {5})"),
     "recV96": (None, (), "kx::recV96({0}, {1}, {2})"),
     "recNewV96": (None, (), "kx::recNewV96({0}, {1}, {2}, {3}, {4})"),
     "recFeV96": (None, (), "kx::recFeV96({0}, {1}, {2}, {3})"),
+    # H2 self-shielding at the default options (radiation.shielding WG11,
+    # num2col default): a function of Tgas and n(H2) (codegen.py:1903-1939)
+    "krome_fshield": ("H2", (), "kx::fshield_wg11(Tgas, n[{h2}])"),
 }
 # the predefined parameters in the order the parameter rows list them,
 # after the network's own (sorted): kromex/codegen.py:287-312
@@ -938,10 +943,11 @@ class Emitter:
                     self.c(cnd.left), op, self.c(cnd.comparators[0]),
                     self.c(args[1]), self.c(args[2]))
             if f in EXTERNS and EXTERNS[f][2] is not None:
-                has_e = any(s.name == "e-" for s in self.net.species)
+                names = {s.name for s in self.net.species}
                 return EXTERNS[f][2].format(
                     *[self.c(a) for a in args],
-                    e=self.net.idx("e-") if has_e else -1)
+                    e=self.net.idx("e-") if "e-" in names else -1,
+                    h2=self.net.idx("H2") if "H2" in names else -1)
             raise KnetError("unknown function %r" % f)
         raise KnetError("unsupported syntax %s" % ast.dump(node))
 
@@ -1484,7 +1490,8 @@ def _emit_photo(w, net, em, photo):
             if (isinstance(x, ast.Call) and isinstance(x.func, ast.Name)
                     and x.func.id in EXTERNS
                     and EXTERNS[x.func.id][2] is None):
-                code = "kx::%s(%s)" % (x.func.id, em.c(x.args[0]))
+                code = "kx::%s(%s)" % (x.func.id, ", ".join(
+                    em.c(a) for a in x.args))
             else:
                 code = em.c(x)
         w("      case %d: return %s;  // %s"
@@ -1741,7 +1748,7 @@ CONFIGS = {
 # and solver on a generic seeded workload (cells.generic_cells)
 EXTRA = ("NO", "SM", "coolHCN", "lowmetal", "primordial2", "primordial_wD",
          "primordial_photo", "primordial_photoH2", "auto", "chianti", "ions",
-         "ionsH2")
+         "ionsH2", "primordial_UV", "xrays", "primordialZ_UV")
 CONFIGS.update({n: ("%s.knet" % n, Thermal()) for n in EXTRA})
 
 
