@@ -319,11 +319,14 @@ class Network:
 
     # -- solves ----------------------------------------------------------
 
-    def solve(self, n, Tgas, dt, par=None, Phi=None, opts=None, check=True):
+    def solve(self, n, Tgas, dt, par=None, Phi=None, opts=None, check=True,
+              Td=None, adust=None):
         """integrate every cell over its own dt [s] (krome_solve_cells);
         Phi: the photon flux per bin [cm^-2 s^-1], needed once
         opts.set_photo() was called: Phi[nbins] for every cell or
-        Phi[ncell, nbins]"""
+        Phi[ncell, nbins].  Td and adust (the reference binding's dust-bin
+        arguments) must be None: this build has no dust bins"""
+        self._no_dust(Td, adust)
         single, n, ncell = self._cells(n)
         T = self._per_cell(Tgas, ncell)
         dtv = self._per_cell(dt, ncell)
@@ -335,14 +338,16 @@ class Network:
             ncell, _dptr(n), _dptr(T), _dptr(dtv), _dptr(P), stride,
             _extra(F), flags.ctypes.data_as(_I), opts._h)
         res = Result(n[0] if single else n, T[0] if single else T,
-                     flags[0] if single else flags, ierr)
+                     flags[0] if single else flags, ierr,
+                     None if F is None else (F[0] if single else F))
         self._check(check, ierr, flags, "krome_solve_cells")
         return res
 
     def solve_x(self, x, rho, Tgas, dt, par=None, Phi=None, opts=None,
-                check=True):
+                check=True, Td=None, adust=None):
         """solve() with mass fractions x and the gas density rho [g cm^-3]
         (krome_solve_cells_x)"""
+        self._no_dust(Td, adust)
         single, x, ncell = self._cells(x)
         rh = self._per_cell(rho, ncell)
         T = self._per_cell(Tgas, ncell)
@@ -355,7 +360,8 @@ class Network:
             ncell, _dptr(x), _dptr(rh), _dptr(T), _dptr(dtv), _dptr(P),
             stride, _extra(F), flags.ctypes.data_as(_I), opts._h)
         res = Result(x[0] if single else x, T[0] if single else T,
-                     flags[0] if single else flags, ierr)
+                     flags[0] if single else flags, ierr,
+                     None if F is None else (F[0] if single else F))
         self._check(check, ierr, flags, "krome_solve_cells_x")
         return res
 
@@ -613,6 +619,11 @@ class Network:
             self._opts = Options(self)
         return self._opts
 
+    @staticmethod
+    def _no_dust(Td, adust):
+        if Td is not None or adust is not None:
+            raise KromeError("Td / adust: this build has no dust bins")
+
     def _phi(self, Phi, ncell, opts):
         """the fields as [ncell, nbins] rows (None when no bins are set)"""
         nb = opts.nbins
@@ -771,10 +782,13 @@ class Options:
 
 
 class Result:
-    """solve() output: n, Tgas, per-cell flags (0 = success), ierr"""
+    """solve() output: n, Tgas, Phi (the fields given, None without photon
+    bins; held fixed during the solve), per-cell flags (0 = success), ierr;
+    Td and adust are None (no dust bins here)"""
 
-    def __init__(self, n, Tgas, flags, ierr):
+    def __init__(self, n, Tgas, flags, ierr, Phi=None):
         self.n, self.Tgas, self.flags, self.ierr = n, Tgas, flags, ierr
+        self.Phi, self.Td, self.adust = Phi, None, None
 
     @property
     def ok(self):

@@ -449,3 +449,19 @@ def test_gpu_photoheating_solve_vs_reference(kernel, mixed):
     res = N.solve(g["n0"], g["T0"], g["dt"], par=g["par"], Phi=g["Phi"],
                   opts=o, check=False)
     heat_solve_check(res.n, res.Tgas, res.flags, g, 1e-4 if mixed else 1e-5)
+
+
+def test_solve_signature_as_the_reference_binding():
+    """Network.solve takes the reference binding's arguments
+    (runtime/interfaces/python/krome.py:458-507): Td and adust, refused
+    here before any work (no dust bins); a field without bins is refused
+    too"""
+    import krome
+    N = load()
+    n = np.ones(N.nspec)
+    with pytest.raises(krome.KromeError):
+        N.solve(n, 100.0, 1.0, Td=np.ones(2))
+    with pytest.raises(krome.KromeError):
+        N.solve(n, 100.0, 1.0, adust=np.ones(2))
+    with pytest.raises(krome.KromeError):
+        N.solve(n, 100.0, 1.0, Phi=np.ones(32))
