@@ -6,7 +6,7 @@ gamma EXACT, H2 opacity RIPAMONTI; rtol 1e-8, atol_n 1e-20, atol_T 1e-6).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--ncell N]
                     [--kernel persistent|plain] [--fp64]
-                    [--workload solve|conserve|multi|photo|copy]
+                    [--workload solve|conserve|multi|photo|network|copy]
                     [--linsol dense|sparse]
 
 A step is one solve of the whole batch from its initial state: the inputs
@@ -45,6 +45,14 @@ the plain FP64 kernel, the persistent FP64 kernel and the persistent
 mixed-precision kernel are timed in turn, with the error of each per
 species against the plain FP64 result and, on a sample, against the
 oracle; the oracle on the host's cores is the CPU baseline.
+
+--workload network --network NAME --cells WORKLOAD measures one generated
+network on a workload of krome-x_amd/cells.py (default: c3_primordialZ,
+the primordialZ cells of the uneven batch): one device-resident solve of
+--ncell cells by the plain FP64 kernel with the dense LU and with the
+static-pivot sparse LU, and by the persistent FP64 kernel where the
+network has at most 16 equations; cells/s from the kernels' events, the
+steps per cell, the nonzero flags.
 
 --workload copy is the calibration of the HBM traffic counters: one
 device-to-device copy of 2 GiB (far above the last-level cache), whose
@@ -437,6 +445,59 @@ def photo_bench(a):
     print(json.dumps(out))
 
 
+def network_bench(a):
+    """one generated network's kernels on one workload (one JSON line)"""
+    import torch
+    import cells
+    import krome
+
+    net = krome.load(a.network)
+    w = cells.WORKLOADS[a.cells](a.ncell)
+    o = w["opts"]
+    dev = torch.device("cuda:0")
+    n0 = torch.tensor(np.ascontiguousarray(w["n"].T), device=dev)
+    T0 = torch.tensor(w["T"], device=dev)
+    dt = torch.tensor(w["dt"], device=dev)
+    par = torch.tensor(np.ascontiguousarray(w["par"]), device=dev)
+    kms = ctypes.c_double()
+    kp = ctypes.cast(ctypes.byref(kms), ctypes.POINTER(ctypes.c_double))
+    runs = [("plain_dense", "plain", 1), ("plain_sparse", "plain", 3)]
+    if net.nspec + int(o["evolve_T"]) <= 16:
+        runs.append(("persistent_dense", "persistent", 1))
+    out = {}
+    for key, kernel, ls in runs:
+        opts = net.options(evolve_T=bool(o["evolve_T"]), rtol=o["rtol"],
+                           atol_n=o["atol_n"], atol_T=o["atol_T"],
+                           jac=int(o["jac"]), linsol=ls,
+                           max_steps=int(o["max_steps"]),
+                           device_kernel=kernel, error_report=False)
+        n, T = n0.clone(), T0.clone()
+        fl = torch.zeros(a.ncell, dtype=torch.int32, device=dev)
+        st = torch.zeros((a.ncell, 9), dtype=torch.int64, device=dev)
+        r = net.solve_device(n, T, dt, par, opts=opts, flags=fl, stats=st)
+        if net._lib.krome_device_last_kernel_ms(kp) != 0:
+            raise SystemExit("%s: no kernel time" % key)
+        s_ = st.cpu().numpy()[:, 0]
+        out[key] = {"return": r, "kernel_ms": kms.value,
+                    "cells_per_s": a.ncell / (kms.value * 1e-3),
+                    "flags_nonzero": int((fl.cpu().numpy() != 0).sum()),
+                    "steps_per_cell": {"p50": float(np.percentile(s_, 50)),
+                                       "p99": float(np.percentile(s_, 99)),
+                                       "max": int(s_.max())}}
+        print("%s %s: %.1f ms" % (a.network, key, kms.value),
+              file=sys.stderr, flush=True)
+    best = max(out.values(), key=lambda v: v["cells_per_s"])
+    print(json.dumps({
+        "metric": "cells/sec to t_end, %s" % a.network,
+        "value": best["cells_per_s"], "unit": "cells/s", "n_gpus": 1,
+        "steps": 1, "warmup": 0, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "%d cells of %s" % (a.ncell, a.cells),
+                   "network": a.network, "parallelism": "1 GPU"},
+        "kernels": out}))
+
+
 def copy_calibration():
     """one 2 GiB device-to-device copy (the traffic counters' reference)"""
     import torch
@@ -466,7 +527,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--workload", default="solve",
-                    choices=("solve", "conserve", "multi", "photo", "copy"))
+                    choices=("solve", "conserve", "multi", "photo", "network",
+                             "copy"))
+    ap.add_argument("--network", default="primordialZ")
+    ap.add_argument("--cells", default="c3_primordialZ")
     ap.add_argument("--linsol", default="dense", choices=("dense", "sparse"),
                     help="the Newton matrix's LU (sparse: the plain kernel's "
                          "static-pivot sparse LU, linsol 3)")
@@ -480,6 +544,10 @@ def main():
         return conserve_bench(a)
     if a.workload == "copy":
         return copy_calibration()
+    if a.workload == "network":
+        if a.ncell == 1 << 20:
+            a.ncell = 1 << 16
+        return network_bench(a)
     if a.workload == "photo":
         if a.ncell == 1 << 20:
             a.ncell = 1 << 18
