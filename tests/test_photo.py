@@ -465,3 +465,50 @@ def test_solve_signature_as_the_reference_binding():
         N.solve(n, 100.0, 1.0, adust=np.ones(2))
     with pytest.raises(krome.KromeError):
         N.solve(n, 100.0, 1.0, Phi=np.ones(32))
+
+
+@pytest.mark.gpu
+def test_gpu_field_forms():
+    """the other forms of the field argument on the device: one Phi row for
+    every cell equals that row repeated; krome_solve_cells_x with fields
+    equals the density solve converted to mass fractions; the fluxes with
+    fields are k prod(n) of the rates with fields"""
+    g = golden("%s_photo.npz" % NET)
+    N = load()
+    m = 64
+    o = photo_opts(N, g, evolve_T=1, rtol=1e-8, atol_n=1e-20, atol_T=1e-6,
+                   linsol=1, device_kernel="plain")
+    row = g["Phi"][5]
+    say("field forms")
+    a = N.solve(g["n0"][:m], g["T0"][:m], g["dt"][:m], par=g["par"][:m],
+                Phi=row, opts=o, check=False)
+    b = N.solve(g["n0"][:m], g["T0"][:m], g["dt"][:m], par=g["par"][:m],
+                Phi=np.tile(row, (m, 1)), opts=o, check=False)
+    assert a.ierr == b.ierr == 0
+    assert np.array_equal(a.n, b.n) and np.array_equal(a.Tgas, b.Tgas)
+    assert a.Phi is not None and a.Td is None
+    # mass fractions: x = m_i n_i / rho, solved with the same fields
+    mass = np.array(N.masses)
+    n0 = g["n0"][:m]
+    rho = n0 @ mass
+    x0 = n0 * mass / rho[:, None]
+    c = N.solve(n0, g["T0"][:m], g["dt"][:m], par=g["par"][:m],
+                Phi=g["Phi"][:m], opts=o, check=False)
+    xs = N.solve_x(x0, rho, g["T0"][:m], g["dt"][:m], par=g["par"][:m],
+                   Phi=g["Phi"][:m], opts=o, check=False)
+    assert c.ierr == xs.ierr == 0
+    xn = c.n * mass / rho[:, None]
+    big = xn > 1e-12
+    assert rel(xs.n[big], xn[big]).max() < 1e-6
+    # fluxes with fields
+    k = N.rates(g["fn"], g["fT"], par=g["fpar"], Phi=g["fPhi"], opts=o)
+    fl = N.fluxes(g["fn"], g["fT"], par=g["fpar"], Phi=g["fPhi"], opts=o)
+    meta = cells.network_meta(NET)
+    sp = {s: i for i, s in enumerate(meta["species"])}
+    want = np.array(k, copy=True)
+    for r, eq in enumerate(meta["reactions"]):
+        for t in eq.split("->")[0].split(" + "):
+            want[:, r] *= g["fn"][:, sp[t.strip()]]
+    nz = want != 0.0
+    assert np.array_equal(fl == 0.0, want == 0.0)
+    assert rel(fl[nz], want[nz]).max() < 1e-12
