@@ -11,16 +11,20 @@
  *    the device for any number of cells, path 1 (per-cell host loop)
  *    returns KROME_FLAG_UNSUPPORTED; a cell that fails on the device keeps
  *    its input and its flag (no host re-solve);
- *  - linsol 2 (LAPACK) and 3 (sparse) return KROME_FLAG_UNSUPPORTED;
- *    networks without photo or dust bins only;
+ *  - linsol 2 (LAPACK) returns KROME_FLAG_UNSUPPORTED; linsol 3 (the
+ *    static-pivot sparse LU) runs in the plain kernel, and the persistent
+ *    kernel returns KROME_FLAG_UNSUPPORTED for it and for networks above
+ *    16 equations;
+ *  - photon bins and fields as the reference's (no broadening, dust
+ *    opacity, shielding or co-evolved field); no dust bins;
  *  - host-array calls (solves, element totals, conservation) copy their
  *    rows to the device, compute there and copy back;
  *  - extra option keys select this build's kernels: device_kernel (0 the
  *    plain FP64 kernel, 1 the persistent batched kernel), mixed_precision
  *    (persistent kernel: 1 = FP32 Jacobian, LU and triangular solves with
- *    the FP64 Newton residual, state and error test, where the Jacobian is
- *    by difference quotients: Tgas evolved or jac 2; the analytic-Jacobian
- *    path keeps its FP64 matrix), queue_chunk and
+ *    the FP64 Newton residual, state and error test, with Tgas evolved
+ *    and the difference-quotient Jacobian; with Tgas fixed the matrix
+ *    stays FP64), queue_chunk and
  *    waves_per_cu (persistent kernel tuning; 0 = automatic).
  */
 #ifndef KROME_MI355X_H

@@ -44,7 +44,7 @@ int krome_multi_npar(int i);
 
 /* solve nbatch (1 or 2) batches of distinct networks in one persistent
  * launch; mixed_precision as the option of that name (primordial's
- * analytic Jacobian keeps the FP64 matrix); kernel_ms (may be
+ * Tgas is fixed: its matrix stays FP64); kernel_ms (may be
  * NULL) receives the launch's duration from HIP events.  Synchronizes
  * before and after.  Returns 0, the lowest failing cell's flag of the
  * first batch (primordial3 first) with a failure, -105 when a REQUIRED

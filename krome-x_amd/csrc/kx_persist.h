@@ -1965,13 +1965,14 @@ inline int launch_persistent(CellsArgs a, int mixed, int chunk, int wpc,
     (void)s;
     return FLAG_UNSUPPORTED;
   } else {
-  // the FP32 Newton matrix serves the difference-quotient Jacobian (Tgas
-  // evolved, or jac 2); with the analytic Jacobian (fixed Tgas) the matrix
-  // stays FP64: on cold, dense cells its FP32 form loses the trace ions and
-  // takes more steps than the FP64 one saves
+  // the FP32 Newton matrix serves the difference-quotient Jacobian with
+  // Tgas evolved; with Tgas fixed the matrix stays FP64.  With the analytic
+  // Jacobian its FP32 form lost the trace ions of cold, dense cells; with
+  // difference quotients at fixed Tgas it failed cells of coolHCN,
+  // primordial2 and primordial_wD (rhs failures, convergence failures)
   const bool dq = a.o.jac == 2 || (a.o.jac == 0 && ET);
-  return mixed && dq ? launch_persistent_t<Net, ET, true>(a, wpc, s)
-                     : launch_persistent_t<Net, ET, false>(a, wpc, s);
+  return mixed && ET && dq ? launch_persistent_t<Net, ET, true>(a, wpc, s)
+                           : launch_persistent_t<Net, ET, false>(a, wpc, s);
   }
 }
 

@@ -114,6 +114,28 @@ def libs():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("net", ["coolHCN", "primordial2", "primordial_wD"])
+def test_gpu_mixed_keeps_fp64_at_fixed_tgas(libs, net):
+    """with Tgas fixed the persistent kernel keeps the FP64 Newton matrix
+    whatever mixed_precision says (its FP32 form failed cells of these
+    networks): the results are the FP64 kernel's, bitwise"""
+    g = fixture(net)
+    N = libs[net]
+    o = g["opts"]
+    out = []
+    for mixed in (0, 1):
+        opts = N.options(evolve_T=False, rtol=o[1], atol_n=o[2], atol_T=o[3],
+                         jac=int(o[5]), linsol=1, device_kernel="persistent",
+                         mixed_precision=mixed, error_report=False)
+        say("fixed-Tgas mixed", net, mixed)
+        out.append(N.solve(g["n0"], g["T0"], g["dt"], par=g["par"],
+                           opts=opts, check=False))
+    assert out[0].ierr == out[1].ierr == 0
+    assert np.array_equal(out[0].n, out[1].n)
+    assert np.array_equal(out[0].flags, out[1].flags)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("kernel", ["plain", "persistent"])
 @pytest.mark.parametrize("net", NETS)
 def test_gpu_solve_vs_reference(libs, oracle, net, kernel):
