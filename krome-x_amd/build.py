@@ -20,7 +20,7 @@ NETWORKS = ("primordial", "primordial3", "galaxy_ism", "primordialZ",
             "galaxy_ism_heat", "NO", "SM", "coolHCN", "lowmetal",
             "primordial2", "primordial_wD", "primordial_photo",
             "primordial_photoH2", "auto", "chianti", "primordial_UV",
-            "xrays", "primordialZ_UV")
+            "xrays", "primordialZ_UV", "photobin")
 # generated networks built only on request (build(net)): ions and ionsH2
 # (55 and 58 species, 45 and 48 photo reactions) take about ten minutes of
 # hipcc each for a 7 MB library; their generated code is checked through

@@ -274,7 +274,7 @@ def generic_cells(name, ncell=256, seed=5, first=0):
 EXTRA_NETWORKS = ("NO", "SM", "coolHCN", "lowmetal", "primordial2",
                   "primordial_wD", "primordial_photo", "primordial_photoH2",
                   "auto", "chianti", "ions", "ionsH2", "primordial_UV",
-                  "xrays", "primordialZ_UV")
+                  "xrays", "primordialZ_UV", "photobin")
 
 WORKLOADS = {"primordial": primordial_cells, "primordial3": primordial3_cells,
              "c3_primordial": c3_primordial_cells,

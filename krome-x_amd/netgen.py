@@ -1748,7 +1748,7 @@ CONFIGS = {
 # and solver on a generic seeded workload (cells.generic_cells)
 EXTRA = ("NO", "SM", "coolHCN", "lowmetal", "primordial2", "primordial_wD",
          "primordial_photo", "primordial_photoH2", "auto", "chianti", "ions",
-         "ionsH2", "primordial_UV", "xrays", "primordialZ_UV")
+         "ionsH2", "primordial_UV", "xrays", "primordialZ_UV", "photobin")
 CONFIGS.update({n: ("%s.knet" % n, Thermal()) for n in EXTRA})
 
 

@@ -47,6 +47,7 @@
 #include "net_primordial_UV.h"
 #include "net_xrays.h"
 #include "net_primordialZ_UV.h"
+#include "net_photobin.h"
 #include "kx_conserve.hpp"
 #include "kx_photo.hpp"
 
@@ -328,7 +329,8 @@ void forms(int ncell, const double* n, const double* T, const double* par,
   X(4, galaxy_ism_heat) X(5, NO) X(6, SM) X(7, coolHCN) X(8, lowmetal)     \
   X(9, primordial2) X(10, primordial_wD) X(11, primordial_photo)           \
   X(12, primordial_photoH2) X(13, auto) X(14, chianti) X(15, ions)         \
-  X(16, ionsH2) X(17, primordial_UV) X(18, xrays) X(19, primordialZ_UV)
+  X(16, ionsH2) X(17, primordial_UV) X(18, xrays) X(19, primordialZ_UV)  \
+  X(20, photobin)
 
 // f(Net{}) for network id net, or -1
 template <class F>

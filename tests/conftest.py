@@ -20,7 +20,8 @@ NET_ID = {"primordial": 0, "primordial3": 1, "galaxy_ism": 2,
           "coolHCN": 7, "lowmetal": 8, "primordial2": 9, "primordial_wD": 10,
           "primordial_photo": 11, "primordial_photoH2": 12, "auto": 13,
           "chianti": 14, "ions": 15, "ionsH2": 16, "primordial_UV": 17,
-          "xrays": 18, "primordialZ_UV": 19}
+          "xrays": 18, "primordialZ_UV": 19,
+          "photobin": 20}
 
 
 def pytest_configure(config):

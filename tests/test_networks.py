@@ -1,7 +1,7 @@
 """The generator on the other shipped networks it covers (NO, SM, coolHCN,
 lowmetal, primordial2, primordial_wD, primordial_photo,
-primordial_photoH2, auto, chianti, ions, ionsH2; krome-x_amd/netgen.py
-EXTRA), each at the reference's
+primordial_photoH2, auto, chianti, ions, ionsH2, primordial_UV, xrays,
+primordialZ_UV, photobin; krome-x_amd/netgen.py EXTRA), each at the reference's
 default thermal options (Tgas fixed), pinned against the reference's own
 generated project (tests/golden/<net>_extra.npz, or <net>_photo.npz with
 photon bins and fields; tests/golden/make_golden.py extra):
