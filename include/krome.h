@@ -24,8 +24,11 @@
  *    (persistent kernel: 1 = FP32 Jacobian, LU and triangular solves with
  *    the FP64 Newton residual, state and error test, with Tgas evolved
  *    and the difference-quotient Jacobian; with Tgas fixed the matrix
- *    stays FP64), queue_chunk and
- *    waves_per_cu (persistent kernel tuning; 0 = automatic).
+ *    stays FP64), queue_chunk (persistent kernel: each wave takes this
+ *    many cells of the queue per atomic and hands them to its idle lanes;
+ *    0, or below 64, takes exactly the cells its idle lanes need) and
+ *    waves_per_cu (resident waves per CU; 0 = automatic).  Neither
+ *    changes a cell's result, only which lane integrates it.
  */
 #ifndef KROME_MI355X_H
 #define KROME_MI355X_H

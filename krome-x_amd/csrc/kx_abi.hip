@@ -141,14 +141,36 @@ struct DevCache {
   Buf work, iwork, n, T, dt, par, flags, stats, queue, count, cons;
   Buf xn, xin, rho;  // the mass-fraction entry points' densities
   Buf prow, sig, phi;  // photo: kernel rows, cross sections, a host field
+  Buf pwork, pord;     // the persistent kernel's workspace and queue order
   void release() {
     for (Buf* b : {&work, &iwork, &n, &T, &dt, &par, &flags, &stats, &queue,
-                   &count, &cons, &xn, &xin, &rho, &prow, &sig, &phi})
+                   &count, &cons, &xn, &xin, &rho, &prow, &sig, &phi, &pwork,
+                   &pord})
       b->release();
   }
 };
 DevCache g_cache;
 std::mutex g_cache_mutex;
+
+// a call's private buffers (device_cache off), released on every exit
+struct LocalCache {
+  DevCache dc;
+  bool on;
+  ~LocalCache() {
+    if (on) dc.release();
+  }
+};
+
+// where a solve's kernels take their device memory: the library's buffers
+// (dc, grown as needed), or fixed regions (dc null: the scratch of the
+// entry point that allocates nothing)
+struct SolveMem {
+  DevCache* dc = nullptr;
+  char* work = nullptr;
+  size_t work_bytes = 0;
+  char* ord = nullptr;
+  size_t ord_bytes = 0;
+};
 
 template <bool ET>
 constexpr int neq() { return kx::Problem<Net, ET>::N; }
@@ -200,20 +222,27 @@ int kernel_rows(int ncell, const double*& P, int& pst, const double* Phi,
   }
 }
 
-// the plain kernel over a chunk of resident arrays
+// the plain kernel over a chunk of resident arrays: its workspace in the
+// library's buffers, or (dc null) at work, the doubles of the chunk's
+// cells followed by their pivot ints
 template <bool ET>
-int launch_plain(kx::CellsArgs a, DevCache& dc, hipStream_t s) {
+int launch_plain(kx::CellsArgs a, DevCache* dc, char* work, hipStream_t s) {
   constexpr int N = neq<ET>();
   const long S = a.ncell;
-  if (!dc.work.ensure(sizeof(double) *
-                     kx::cell_doubles<kx::Problem<Net, ET>>() * S))
-    return -1;
-  if (!dc.iwork.ensure(sizeof(int) * N * S)) return -1;
+  const size_t db =
+      sizeof(double) * kx::cell_doubles<kx::Problem<Net, ET>>() * S;
+  double* wd = reinterpret_cast<double*>(work);
+  int* wi = reinterpret_cast<int*>(work + db);
+  if (dc) {
+    if (!dc->work.ensure(db)) return -1;
+    if (!dc->iwork.ensure(sizeof(int) * N * S)) return -1;
+    wd = static_cast<double*>(dc->work.p);
+    wi = static_cast<int*>(dc->iwork.p);
+  }
   const int tpb = 128;
   const int blocks = (a.ncell + tpb - 1) / tpb;
   hipLaunchKernelGGL((kx::plain_cells_kernel<Net, ET>), dim3(blocks),
-                     dim3(tpb), 0, s, a, static_cast<double*>(dc.work.p),
-                     static_cast<int*>(dc.iwork.p), S);
+                     dim3(tpb), 0, s, a, wd, wi, S);
   return hip_ok(hipGetLastError(), "plain kernel launch") ? 0 : -1;
 }
 
@@ -234,8 +263,14 @@ void ev_end(hipStream_t s) {
   g_ev_recorded = true;
 }
 
+int solve_kernels(kx::CellsArgs a, const Opts& o, const SolveMem& m,
+                  hipStream_t s);
 int solve_resident_(kx::CellsArgs a, const Opts& o, DevCache& dc,
-                    hipStream_t s);
+                    hipStream_t s) {
+  SolveMem m;
+  m.dc = &dc;
+  return solve_kernels(a, o, m, s);
+}
 
 // the solve with the post-solve conservation of mode 2 (solve_cell.hpp:
 // 1484-1488): each cell's input totals taken before the kernels, the
@@ -269,30 +304,50 @@ int solve_resident(kx::CellsArgs a, const Opts& o, DevCache& dc,
   return r;
 }
 
-int solve_resident_(kx::CellsArgs a, const Opts& o, DevCache& dc,
-                    hipStream_t s) {
+// the solve kernels over resident arrays, their device memory from m
+int solve_kernels(kx::CellsArgs a, const Opts& o, const SolveMem& m,
+                  hipStream_t s) {
   if (a.ncell <= 0) return 0;
   if (a.o.evolve_T && !Net::THERMAL) {
     return kx::FLAG_UNSUPPORTED;
   }
+  DevCache* dc = m.dc;
   if (o.kernel == 1) {
+    // the workspace and the queue order: the library's buffers, or the
+    // fixed regions when they are large enough
+    auto mem = [&](size_t wb, size_t ob, kx::PersistMem& pm) -> int {
+      if (dc) {
+        if (!dc->pwork.ensure(wb) || !dc->pord.ensure(ob))
+          return kx::FLAG_DEVICE_MEMORY;
+        pm.work = dc->pwork.p;
+        pm.ord = dc->pord.p;
+        return 0;
+      }
+      if (wb > m.work_bytes || ob > m.ord_bytes) return kx::FLAG_UNSUPPORTED;
+      pm.work = m.work;
+      pm.ord = m.ord;
+      return 0;
+    };
     if constexpr (Net::THERMAL)
       if (a.o.evolve_T)
         return kx::launch_persistent<Net, true>(a, o.mixed, o.queue_chunk,
-                                                o.waves_per_cu, s);
+                                                o.waves_per_cu, s, mem);
     return kx::launch_persistent<Net, false>(a, o.mixed, o.queue_chunk,
-                                             o.waves_per_cu, s);
+                                             o.waves_per_cu, s, mem);
   }
-  // chunking of the plain kernel by the memory budget
+  // chunking of the plain kernel by the memory budget (or the fixed
+  // region)
   size_t budget;
-  if (o.device_memory < 0) {
+  if (!dc) {
+    budget = m.work_bytes;
+  } else if (o.device_memory < 0) {
     budget = (size_t)-1;
   } else if (o.device_memory > 0) {
     budget = (size_t)o.device_memory;
   } else {
     size_t fr = 0, tot = 0;
     if (!hip_ok(hipMemGetInfo(&fr, &tot), "hipMemGetInfo")) return -1;
-    budget = (size_t)(0.8 * (double)(fr + dc.work.bytes + dc.iwork.bytes));
+    budget = (size_t)(0.8 * (double)(fr + dc->work.bytes + dc->iwork.bytes));
   }
   const size_t per = plain_cell_bytes(a.o.evolve_T != 0);
   long chunk = (long)(budget / per);
@@ -309,10 +364,10 @@ int solve_resident_(kx::CellsArgs a, const Opts& o, DevCache& dc,
     b.stats = a.stats ? a.stats + c0 * 9 : nullptr;
     int r;
     if constexpr (Net::THERMAL)
-      r = a.o.evolve_T ? launch_plain<true>(b, dc, s)
-                       : launch_plain<false>(b, dc, s);
+      r = a.o.evolve_T ? launch_plain<true>(b, dc, m.work, s)
+                       : launch_plain<false>(b, dc, m.work, s);
     else
-      r = launch_plain<false>(b, dc, s);
+      r = launch_plain<false>(b, dc, m.work, s);
     if (r != 0) return r;
   }
   return 0;
@@ -700,8 +755,8 @@ int solve_host(int ncell, double* v, const double* rho, double* Tgas,
     return rj;
   }
   std::lock_guard<std::mutex> lk(g_cache_mutex);
-  DevCache local;
-  DevCache& dc = h->device_cache ? g_cache : local;
+  LocalCache local{DevCache(), !h->device_cache};
+  DevCache& dc = h->device_cache ? g_cache : local.dc;
   const size_t nb = sizeof(double) * (size_t)ncell * NS;
   const size_t pb = sizeof(double) * (size_t)(pst ? ncell : 1) * Net::NPAR;
   const bool field = Net::NPHOTO > 0 && h->nbins > 0;
@@ -754,7 +809,6 @@ int solve_host(int ncell, double* v, const double* rho, double* Tgas,
                           hipMemcpyDeviceToHost), "D2H");
     if (!ok) r = -1;
   }
-  if (!h->device_cache) local.release();
   if (r != 0) {
     if (r == kx::FLAG_DEVICE_MEMORY && flags)
       for (int c = 0; c < ncell; ++c) flags[c] = r;

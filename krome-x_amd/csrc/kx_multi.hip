@@ -41,6 +41,36 @@ bool hip_ok(hipError_t e, const char* what) {
   return false;
 }
 
+// a device buffer grown as needed and kept (guarded by g_mutex)
+struct Buf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  bool ensure(size_t b) {
+    if (b <= bytes) return true;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (!hip_ok(hipMalloc(&p, b), "hipMalloc")) return false;
+    bytes = b;
+    return true;
+  }
+};
+// each network's persistent workspace and queue order
+Buf g_work[NNET], g_ord[NNET];
+
+// the memory of network k's persistent launch
+kx::PersistMem persist_mem(int k, size_t wb, size_t ob, int& r) {
+  kx::PersistMem m;
+  r = 0;
+  if (!g_work[k].ensure(wb) || !g_ord[k].ensure(ob)) {
+    r = kx::FLAG_DEVICE_MEMORY;
+    return m;
+  }
+  m.work = g_work[k].p;
+  m.ord = g_ord[k].p;
+  return m;
+}
+
 int nspec_of(int net) { return net == 0 ? NetB::NSPEC : NetA::NSPEC; }
 int npar_of(int net) { return net == 0 ? NetB::NPAR : NetA::NPAR; }
 
@@ -138,20 +168,38 @@ int krome_multi_solve_device(const krome_multi_batch* b, int nbatch,
   hipStream_t s = nullptr;
   (void)hipEventRecord(g_ev[0], s);
   int r;
+  auto mem_of = [](int k) {
+    return [k](size_t wb, size_t ob, kx::PersistMem& m) -> int {
+      int r = 0;
+      m = persist_mem(k, wb, ob, r);
+      return r;
+    };
+  };
   if (has_a && has_b) {
     // primordial's Jacobian is analytic (Tgas fixed): its Newton matrix
     // stays FP64, as in launch_persistent
+    const kx::CellsArgs xa = cells_args(*ba), xb = cells_args(*bb);
+    const size_t wa = mixed_precision
+                          ? kx::persist_work_bytes<NetA, true, true>()
+                          : kx::persist_work_bytes<NetA, true, false>();
+    int ra = 0, rb = 0;
+    const kx::PersistMem ma =
+        persist_mem(1, wa, kx::persist_order_bytes(xa.ncell), ra);
+    const kx::PersistMem mb = persist_mem(
+        0, kx::persist_work_bytes<NetB, false, false>(),
+        kx::persist_order_bytes(xb.ncell), rb);
+    if (ra != 0 || rb != 0) return ra != 0 ? ra : rb;
     r = mixed_precision
         ? kx::launch_persistent_pair_t<NetA, true, true, NetB, false, false>(
-              cells_args(*ba), cells_args(*bb), 0, s)
+              xa, xb, 0, 0, s, ma, mb)
         : kx::launch_persistent_pair_t<NetA, true, false, NetB, false, false>(
-              cells_args(*ba), cells_args(*bb), 0, s);
+              xa, xb, 0, 0, s, ma, mb);
   } else if (has_a) {
     r = kx::launch_persistent<NetA, true>(cells_args(*ba), mixed_precision,
-                                          0, 0, s);
+                                          0, 0, s, mem_of(1));
   } else {
     r = kx::launch_persistent<NetB, false>(cells_args(*bb), mixed_precision,
-                                           0, 0, s);
+                                           0, 0, s, mem_of(0));
   }
   (void)hipEventRecord(g_ev[1], s);
   if (r != 0) return r;

@@ -1041,14 +1041,27 @@ constexpr long lds_doubles() {
   return (long)lds_vectors<N>() * N * KX_PERSIST_BLOCK;
 }
 
+// a network's work queue: a counter over the permutation order of its
+// cells; chunk > 0 makes a wave take chunk cells per atomic into a
+// reservoir (base, left) of its own (two ints of res per wave, in the
+// workspace's head), which it hands out to its idle lanes before it takes
+// the next chunk (chunk 0: exactly the cells its idle lanes need, one
+// atomic per fetch)
+struct PersistQueue {
+  int* counter;
+  const int* order;
+  int chunk;
+  int* res;
+};
+
 // one network's cells on this wave: its lanes take cells from the
 // network's queue until the queue is exhausted and every lane of the wave
 // is idle.  lds_vec: the block's LDS, each lane using only its own column
 // (tid) of it, so waves of one block may run different networks at once.
 template <class Net, bool ET, bool MIXED>
 __device__ __attribute__((always_inline)) inline void persist_run(
-    CellsArgs a, PersistWorkOf<Net, ET, MIXED> w0, int* queue,
-    const int* order, double* lds_vec) {
+    CellsArgs a, PersistWorkOf<Net, ET, MIXED> w0, PersistQueue qu,
+    double* lds_vec) {
   using M = Persist<Net, ET, MIXED>;
   using Prob = Problem<Net, ET>;
   constexpr int N = M::N;
@@ -1056,6 +1069,14 @@ __device__ __attribute__((always_inline)) inline void persist_run(
   using namespace bdfc;
   const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
+  // this wave's reservoir, empty at the start (only the wave's leader
+  // lanes touch its two ints, by atomics)
+  int* res = qu.res + 2 * (s / 64);
+  const int chunk = qu.chunk >= 64 ? qu.chunk : 0;
+  if (chunk > 0 && lane == 0) {
+    __hip_atomic_store(&res[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&res[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 
   const SolveOpts& o = a.o;
   const bool dq_jac = o.jac == 2 || (o.jac == 0 && ET);
@@ -1662,18 +1683,43 @@ __device__ __attribute__((always_inline)) inline void persist_run(
           break;
       } while (0);
       if (pc == P_FETCH) do {
-          // one atomic per wave for the lanes that need a cell
+          // one atomic per wave for the lanes that need a cell (or, with
+          // a chunk, the cells of the wave's reservoir first)
           const unsigned long long need = __ballot(1);
           const int rank = __popcll(need & ((1ull << lane) - 1ull));
+          const int cnt = __popcll(need);
           const int leader = __ffsll((long long)need) - 1;
-          int base = 0;
-          if (lane == leader) base = atomicAdd(queue, __popcll(need));
-          base = __shfl(base, leader);
-          if (base + rank >= a.ncell) {
+          int idx = 0;
+          if (chunk == 0) {
+            int base = 0;
+            if (lane == leader) base = atomicAdd(qu.counter, cnt);
+            idx = __shfl(base, leader) + rank;
+          } else {
+            // lanes rank < left take the reservoir's cells; the others
+            // the start of a new chunk (cnt - left <= 64 <= chunk)
+            int rb = 0, rl = 0, nb = 0;
+            if (lane == leader) {
+              rb = __hip_atomic_load(&res[0], __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+              rl = __hip_atomic_load(&res[1], __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+              if (rl < cnt) nb = atomicAdd(qu.counter, chunk);
+              __hip_atomic_store(&res[0], rl < cnt ? nb + (cnt - rl) : rb + cnt,
+                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              __hip_atomic_store(&res[1],
+                                 rl < cnt ? chunk - (cnt - rl) : rl - cnt,
+                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            rb = __shfl(rb, leader);
+            rl = __shfl(rl, leader);
+            nb = __shfl(nb, leader);
+            idx = rank < rl ? rb + rank : nb + (rank - rl);
+          }
+          if (idx >= a.ncell) {
             pc = P_IDLE;
             break;
           }
-          const int c = order[base + rank];
+          const int c = qu.order[idx];
           ST(cell) = c;
           for (int i = 0; i < TOT_SLOTS; ++i) TOT(i) = 0;
           ST(restarts) = 0;
@@ -1743,9 +1789,9 @@ __device__ __attribute__((always_inline)) inline void persist_run(
 template <class Net, bool ET, bool MIXED>
 __global__ void __launch_bounds__(KX_PERSIST_BLOCK, KX_PERSIST_WAVES_PER_EU)
 persist_cells_kernel(CellsArgs a, PersistWorkOf<Net, ET, MIXED> w0,
-                     int* queue, const int* order) {
+                     PersistQueue q) {
   __shared__ double lds_vec[lds_doubles<Net, ET>()];
-  persist_run<Net, ET, MIXED>(a, w0, queue, order, lds_vec);
+  persist_run<Net, ET, MIXED>(a, w0, q, lds_vec);
 }
 
 // two networks in one launch: every wave runs the first network's cells
@@ -1754,14 +1800,13 @@ persist_cells_kernel(CellsArgs a, PersistWorkOf<Net, ET, MIXED> w0,
 // leaving its tail to a launch of its own
 template <class NetA, bool ETA, bool MA, class NetB, bool ETB, bool MB>
 __global__ void __launch_bounds__(KX_PERSIST_BLOCK, KX_PERSIST_WAVES_PER_EU)
-persist_pair_kernel(CellsArgs a, PersistWorkOf<NetA, ETA, MA> wa, int* qa,
-                    const int* oa, CellsArgs b,
-                    PersistWorkOf<NetB, ETB, MB> wb, int* qb,
-                    const int* ob) {
+persist_pair_kernel(CellsArgs a, PersistWorkOf<NetA, ETA, MA> wa,
+                    PersistQueue qa, CellsArgs b,
+                    PersistWorkOf<NetB, ETB, MB> wb, PersistQueue qb) {
   constexpr long LA = lds_doubles<NetA, ETA>(), LB = lds_doubles<NetB, ETB>();
   __shared__ double lds_vec[LA > LB ? LA : LB];
-  persist_run<NetA, ETA, MA>(a, wa, qa, oa, lds_vec);
-  persist_run<NetB, ETB, MB>(b, wb, qb, ob, lds_vec);
+  persist_run<NetA, ETA, MA>(a, wa, qa, lds_vec);
+  persist_run<NetB, ETB, MB>(b, wb, qb, lds_vec);
 }
 
 // ---- the queue's order: the densest cells first ---------------------------
@@ -1820,15 +1865,28 @@ constexpr long persist_lane_bytes() {
          2L * N * N * (MIXED ? sizeof(float) : sizeof(double)) +
          (long)sizeof(int) * N;
 }
+// the workspace's head: the queue counter, then two ints per wave of the
+// lane slots (the queue's reservoirs)
+constexpr size_t PERSIST_HEAD = 256 + 2 * sizeof(int) * (KX_PERSIST_LANES / 64);
+// bytes of one network's persistent workspace: the head, then the lane
+// slots' vectors, matrices, totals and pivots
+template <class Net, bool ET, bool MIXED>
+constexpr size_t persist_work_bytes() {
+  return (size_t)persist_lane_bytes<Problem<Net, ET>::N, MIXED>() *
+             KX_PERSIST_LANES + PERSIST_HEAD;
+}
+// bytes of the queue's order of ncell cells: the bucket counts, the keys
+// and the permutation
+inline size_t persist_order_bytes(long ncell) {
+  return sizeof(int) * (2 * (size_t)ncell + ORDER_BUCKETS);
+}
 
-// a launch's device buffers, grown as needed and kept: the workspace of
-// the lane slots (the queue counter first), and the queue's order (keys,
-// bucket counts and offsets, the permutation)
-struct PersistBuffers {
-  void* buf = nullptr;
-  size_t bytes = 0;
-  int* ord = nullptr;
-  long ord_n = 0;
+// the device memory of one network's persistent launch, given by the
+// caller: the workspace (persist_work_bytes) and the order
+// (persist_order_bytes of the call's cells)
+struct PersistMem {
+  void* work = nullptr;
+  void* ord = nullptr;
 };
 
 // resident lanes of a launch over ncell cells: waves_per_cu waves on each
@@ -1850,27 +1908,19 @@ inline int persist_lanes(long ncell, int wpc, long& lanes) {
   return 0;
 }
 
-// one network's buffers and queue order, on stream s: the workspace view
-// w, the queue counter (zeroed) and the permutation the lanes take cells
-// in
+// one network's workspace view and queue in the memory m, on stream s:
+// the queue counter zeroed (by a kernel: no memset node when the caller
+// captures the stream) and the permutation the lanes take cells in
 template <class Net, bool ET, bool MIXED>
-int persist_prepare(CellsArgs a, PersistBuffers& pb, hipStream_t s,
-                    PersistWorkOf<Net, ET, MIXED>& w, int*& queue,
-                    int*& order) {
+int persist_prepare(CellsArgs a, PersistMem m, int chunk, hipStream_t s,
+                    PersistWorkOf<Net, ET, MIXED>& w, PersistQueue& q) {
   constexpr int N = Problem<Net, ET>::N;
   using MT = typename std::conditional<MIXED, float, double>::type;
   const long slots = KX_PERSIST_LANES;
-  const size_t want = persist_lane_bytes<N, MIXED>() * slots + 256;
-  if (want > pb.bytes) {
-    if (pb.buf) (void)hipFree(pb.buf);
-    pb.buf = nullptr;
-    pb.bytes = 0;
-    if (hipMalloc(&pb.buf, want) != hipSuccess) return FLAG_DEVICE_MEMORY;
-    pb.bytes = want;
-  }
-  char* p = static_cast<char*>(pb.buf);
-  queue = reinterpret_cast<int*>(p);
-  p += 256;
+  char* p = static_cast<char*>(m.work);
+  int* queue = reinterpret_cast<int*>(p);
+  int* res = reinterpret_cast<int*>(p + 256);
+  p += PERSIST_HEAD;
   w.v = reinterpret_cast<double*>(p);
   p += sizeof(double) * (V_COUNT - lds_vectors<N>()) * N * slots;
   w.J = reinterpret_cast<MT*>(p);
@@ -1880,18 +1930,9 @@ int persist_prepare(CellsArgs a, PersistBuffers& pb, hipStream_t s,
   w.tot = reinterpret_cast<long*>(p);
   p += sizeof(long) * TOT_SLOTS * slots;
   w.piv = reinterpret_cast<int*>(p);
-  if (a.ncell > pb.ord_n) {
-    if (pb.ord) (void)hipFree(pb.ord);
-    pb.ord = nullptr;
-    pb.ord_n = 0;
-    if (hipMalloc(&pb.ord, sizeof(int) * (2L * a.ncell + ORDER_BUCKETS)) !=
-        hipSuccess)
-      return FLAG_DEVICE_MEMORY;
-    pb.ord_n = a.ncell;
-  }
-  int* hist = pb.ord;
-  int* key = pb.ord + ORDER_BUCKETS;
-  order = key + a.ncell;
+  int* hist = static_cast<int*>(m.ord);
+  int* key = hist + ORDER_BUCKETS;
+  int* order = key + a.ncell;
   const int cb = (a.ncell + 255) / 256;
   hipLaunchKernelGGL(fill_flags_kernel, dim3(1), dim3(64), 0, s, 1, queue, 0);
   hipLaunchKernelGGL(fill_flags_kernel, dim3(ORDER_BUCKETS / 256), dim3(256),
@@ -1901,26 +1942,29 @@ int persist_prepare(CellsArgs a, PersistBuffers& pb, hipStream_t s,
   hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(64), 0, s, hist);
   hipLaunchKernelGGL(order_scatter_kernel, dim3(cb), dim3(256), 0, s,
                      a.ncell, key, hist, order);
+  q.counter = queue;
+  q.order = order;
+  q.chunk = chunk;
+  q.res = res;
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-// the persistent kernel over resident arrays: workgroups of 256 lanes,
-// waves_per_cu waves on each of the device's CUs (0: automatic), the
-// workspace allocated for the resident lanes only
+// the persistent kernel over resident arrays in the memory m: workgroups
+// of 256 lanes, waves_per_cu waves on each of the device's CUs (0:
+// automatic); chunk: cells per queue grab (0: as many as the wave's idle
+// lanes)
 template <class Net, bool ET, bool MIXED>
-int launch_persistent_t(CellsArgs a, int wpc, hipStream_t s) {
+int launch_persistent_t(CellsArgs a, int wpc, int chunk, hipStream_t s,
+                        PersistMem m) {
   long lanes = 0;
   if (persist_lanes(a.ncell, wpc, lanes) != 0) return -1;
-  static thread_local PersistBuffers pb;
   PersistWorkOf<Net, ET, MIXED> w;
-  int* queue = nullptr;
-  int* order = nullptr;
-  const int r = persist_prepare<Net, ET, MIXED>(a, pb, s, w, queue, order);
+  PersistQueue q;
+  const int r = persist_prepare<Net, ET, MIXED>(a, m, chunk, s, w, q);
   if (r != 0) return r;
   constexpr long B = KX_PERSIST_BLOCK;
   hipLaunchKernelGGL((persist_cells_kernel<Net, ET, MIXED>),
-                     dim3(lanes / B), dim3(B), 0, s, a, w, queue,
-                     (const int*)order);
+                     dim3(lanes / B), dim3(B), 0, s, a, w, q);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1928,22 +1972,20 @@ int launch_persistent_t(CellsArgs a, int wpc, hipStream_t s) {
 // lanes are those of the larger batch; each network keeps its own
 // workspace and queue, and its own Newton-matrix precision
 template <class NetA, bool ETA, bool MA, class NetB, bool ETB, bool MB>
-int launch_persistent_pair_t(CellsArgs a, CellsArgs b, int wpc,
-                             hipStream_t s) {
+int launch_persistent_pair_t(CellsArgs a, CellsArgs b, int wpc, int chunk,
+                             hipStream_t s, PersistMem ma, PersistMem mb) {
   long lanes = 0;
   if (persist_lanes(a.ncell > b.ncell ? a.ncell : b.ncell, wpc, lanes) != 0)
     return -1;
-  static thread_local PersistBuffers pa, pb;
   PersistWorkOf<NetA, ETA, MA> wa;
   PersistWorkOf<NetB, ETB, MB> wb;
-  int *qa = nullptr, *oa = nullptr, *qb = nullptr, *ob = nullptr;
-  int r = persist_prepare<NetA, ETA, MA>(a, pa, s, wa, qa, oa);
-  if (r == 0) r = persist_prepare<NetB, ETB, MB>(b, pb, s, wb, qb, ob);
+  PersistQueue qa, qb;
+  int r = persist_prepare<NetA, ETA, MA>(a, ma, chunk, s, wa, qa);
+  if (r == 0) r = persist_prepare<NetB, ETB, MB>(b, mb, chunk, s, wb, qb);
   if (r != 0) return r;
   constexpr long B = KX_PERSIST_BLOCK;
   hipLaunchKernelGGL((persist_pair_kernel<NetA, ETA, MA, NetB, ETB, MB>),
-                     dim3(lanes / B), dim3(B), 0, s, a, wa, qa,
-                     (const int*)oa, b, wb, qb, (const int*)ob);
+                     dim3(lanes / B), dim3(B), 0, s, a, wa, qa, b, wb, qb);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1954,25 +1996,47 @@ int launch_persistent_pair_t(CellsArgs a, CellsArgs b, int wpc,
 #define KX_PERSIST_MAX_N 16
 #endif
 
+// whether a launch factorizes the FP32 Newton matrix: with the
+// difference-quotient Jacobian and Tgas evolved; with Tgas fixed the
+// matrix stays FP64.  With the analytic Jacobian its FP32 form lost the
+// trace ions of cold, dense cells; with difference quotients at fixed Tgas
+// it failed cells of coolHCN, primordial2 and primordial_wD (rhs failures,
+// convergence failures)
+template <bool ET>
+inline bool persist_mixed(const CellsArgs& a, int mixed) {
+  const bool dq = a.o.jac == 2 || (a.o.jac == 0 && ET);
+  return mixed && ET && dq;
+}
+
+// bytes of the workspace a launch needs (0: the network is refused)
 template <class Net, bool ET>
-inline int launch_persistent(CellsArgs a, int mixed, int chunk, int wpc,
-                             hipStream_t s) {
-  (void)chunk;
+inline size_t persist_need(const CellsArgs& a, int mixed) {
   if constexpr (Problem<Net, ET>::N > KX_PERSIST_MAX_N) {
     (void)a;
     (void)mixed;
-    (void)wpc;
-    (void)s;
+    return 0;
+  } else {
+    return persist_mixed<ET>(a, mixed) ? persist_work_bytes<Net, ET, true>()
+                                       : persist_work_bytes<Net, ET, false>();
+  }
+}
+
+// the persistent launch with its memory from the caller: mem(work_bytes,
+// order_bytes, m) fills m, returning 0 or a flag
+template <class Net, bool ET, class Mem>
+inline int launch_persistent(CellsArgs a, int mixed, int chunk, int wpc,
+                             hipStream_t s, Mem&& mem) {
+  if constexpr (Problem<Net, ET>::N > KX_PERSIST_MAX_N) {
+    (void)a; (void)mixed; (void)chunk; (void)wpc; (void)s; (void)mem;
     return FLAG_UNSUPPORTED;
   } else {
-  // the FP32 Newton matrix serves the difference-quotient Jacobian with
-  // Tgas evolved; with Tgas fixed the matrix stays FP64.  With the analytic
-  // Jacobian its FP32 form lost the trace ions of cold, dense cells; with
-  // difference quotients at fixed Tgas it failed cells of coolHCN,
-  // primordial2 and primordial_wD (rhs failures, convergence failures)
-  const bool dq = a.o.jac == 2 || (a.o.jac == 0 && ET);
-  return mixed && ET && dq ? launch_persistent_t<Net, ET, true>(a, wpc, s)
-                           : launch_persistent_t<Net, ET, false>(a, wpc, s);
+    PersistMem m;
+    const int r = mem(persist_need<Net, ET>(a, mixed),
+                      persist_order_bytes(a.ncell), m);
+    if (r != 0) return r;
+    return persist_mixed<ET>(a, mixed)
+               ? launch_persistent_t<Net, ET, true>(a, wpc, chunk, s, m)
+               : launch_persistent_t<Net, ET, false>(a, wpc, chunk, s, m);
   }
 }
 

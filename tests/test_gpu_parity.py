@@ -258,3 +258,32 @@ def test_mass_fractions(nets, layout):
     ok = np.ones(nc, dtype=bool)
     ok[[1, 2]] = False
     assert rel_err(got[ok], xr[ok]) < TOL["primordial3"]
+
+
+@pytest.mark.parametrize("mixed", [0, 1])
+def test_queue_chunk(nets, mixed):
+    """queue_chunk only changes which lane takes which cell, never a
+    cell's arithmetic: 8192 cells give bitwise the same rows and flags
+    with the wave reservoir of 64 and 1024 cells per grab as with exact
+    grabs (0); a value below 64 is exact grabs too"""
+    import torch
+    net = nets["primordial3"]
+    w = cells.primordial3_cells(8192, first=777)
+    say("queue chunk", mixed)
+    outs = []
+    for chunk in (0, 64, 1024, 17):
+        n = torch.tensor(np.ascontiguousarray(w["n"].T), device="cuda")
+        T = torch.tensor(w["T"], device="cuda")
+        fl = torch.full((8192,), 7, dtype=torch.int32, device="cuda")
+        o = net.options(device_kernel="persistent", mixed_precision=mixed,
+                        evolve_T=True, rtol=1e-8, atol_n=1e-20, atol_T=1e-6,
+                        queue_chunk=chunk)
+        r = net.solve_device(n, T, torch.tensor(w["dt"], device="cuda"),
+                             torch.tensor(w["par"], device="cuda"), opts=o,
+                             flags=fl)
+        assert r == 0
+        outs.append((n.cpu().numpy(), T.cpu().numpy(), fl.cpu().numpy()))
+    for n, T, f in outs[1:]:
+        assert np.array_equal(f, outs[0][2]) and (f == 0).all()
+        assert np.array_equal(n, outs[0][0])
+        assert np.array_equal(T, outs[0][1])
