@@ -191,6 +191,29 @@ int krome_solve_cells_x_device(int ncell, double* x, const double* rho,
                                int* flags, krome_device_summary* summary,
                                const krome_opts* h);
 
+/* This build's extension, no counterpart in the reference: the
+ * device-resident solve of krome_solve_cells_device for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results, less the photon field; it calls no hipMalloc or hipFree: its
+ * buffers (the persistent workspace, the queue order, a parameter row and
+ * flags when none are given) live in one static device array per GPU,
+ * held from the library's load until the process ends (about 140 MB for
+ * primordial3: the FP64 persistent workspace of 65536 lanes, capped at
+ * 160 MiB, plus 12.5 MB).  At most 2^20 cells per call (-1 beyond);
+ * networks with photo reactions, photon bins and conserve 2 return
+ * KROME_FLAG_UNSUPPORTED with every cell flagged; the persistent kernel
+ * returns it too where its workspace exceeds the cap (above about 12
+ * equations), the plain kernel then runs in chunks that fit.  Calls on
+ * one GPU share its array: host threads are serialized by a lock, and
+ * calls that overlap on one GPU from several processes race. */
+int krome_solve_cells_device_noalloc(int ncell, double* n, int layout,
+                                     long ld, double* Tgas, const double* dt,
+                                     const double* par, int par_stride,
+                                     int* flags,
+                                     krome_device_summary* summary,
+                                     const krome_opts* h);
+
 /* This build's extension, no counterpart in the reference's C ABI: the
  * device-resident solve with the per-cell counters of krome::SolveStats
  * (runtime/krome.hpp:257-268; 9 per cell: steps, rhs_evals, jac_evals,

@@ -106,6 +106,9 @@ const Flag kFlags[] = {
 };
 
 kx::SolveOpts solve_opts(const krome_opts* h);
+// the device address of the current device's scratch (the solve that
+// allocates nothing), looked up on the device's first use
+unsigned char* scratch_of_current_device();
 
 // ---- device state ------------------------------------------------------
 
@@ -615,6 +618,9 @@ int krome_device_initialize_id(int device_id) {
     if (!hip_ok(hipSetDevice(device_id), "hipSetDevice"))
       return kx::FLAG_UNSUPPORTED;
   }
+  // the selected device's scratch address, for the solve that allocates
+  // nothing
+  if (!scratch_of_current_device()) return kx::FLAG_UNSUPPORTED;
   g_dev_init = true;
   return 0;
 }
@@ -989,6 +995,195 @@ int krome_solve_cells_device_stats(int ncell, double* n, int layout, long ld,
   a.stats = stats;
   a.o = solve_opts(h);
   return solve_resident(a, *h, g_cache, nullptr);
+}
+
+}  // extern "C"
+
+// ---- the solve that allocates nothing ------------------------------------
+// krome_solve_cells_device_noalloc keeps every buffer its kernels need in
+// one file-scope device array, scratch, which each GPU holds from the
+// library's load until the process ends.  Its regions (bytes):
+//   head   256: the parameter-check count, the lowest failing cell, the
+//          counts of integrated and failed cells (ints)
+//   par    the default parameter row (when the caller gives none)
+//   flags  4 x KX_NOALLOC_MAX_CELLS (when the caller gives none)
+//   order  the persistent queue's order for KX_NOALLOC_MAX_CELLS cells
+//   work   the persistent workspace of this network (FP64, its largest
+//          form) when that is at most KX_NOALLOC_WORK_CAP, else 64 MiB; the
+//          plain kernel runs in chunks of as many cells as fit there
+namespace {
+
+#ifndef KX_NOALLOC_MAX_CELLS
+#define KX_NOALLOC_MAX_CELLS (1L << 20)
+#endif
+#ifndef KX_NOALLOC_WORK_CAP
+#define KX_NOALLOC_WORK_CAP (160UL << 20)
+#endif
+
+constexpr size_t na_align(size_t b) { return (b + 255) / 256 * 256; }
+template <bool ET>
+constexpr size_t na_persist() {
+  if constexpr (kx::Problem<Net, ET>::N > KX_PERSIST_MAX_N) return 0;
+  else return kx::persist_work_bytes<Net, ET, false>();
+}
+constexpr size_t na_persist_max() {
+  const size_t a = na_persist<false>();
+  const size_t b = Net::THERMAL ? na_persist<true>() : 0;
+  return a > b ? a : b;
+}
+constexpr size_t NA_HEAD = 256;
+constexpr size_t NA_PAR = na_align(sizeof(double) * (Net::NPAR + 1));
+constexpr size_t NA_FLAGS = na_align(sizeof(int) * KX_NOALLOC_MAX_CELLS);
+constexpr size_t NA_ORDER =
+    na_align(sizeof(int) * (2 * KX_NOALLOC_MAX_CELLS + kx::ORDER_BUCKETS));
+constexpr size_t NA_WORK =
+    (na_persist_max() > 0 && na_persist_max() <= KX_NOALLOC_WORK_CAP)
+        ? na_align(na_persist_max())
+        : (64UL << 20);
+constexpr size_t NA_BYTES = NA_HEAD + NA_PAR + NA_FLAGS + NA_ORDER + NA_WORK;
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+// the device address of each GPU's scratch, looked up once per device with
+// that device current (the array's name is only a host-side stand-in)
+constexpr int NA_MAX_DEVICES = 64;
+unsigned char* g_scratch[NA_MAX_DEVICES] = {};
+std::mutex g_scratch_mutex;
+
+unsigned char* scratch_of_current_device() {
+  int dev = 0;
+  if (!hip_ok(hipGetDevice(&dev), "hipGetDevice") || dev < 0 ||
+      dev >= NA_MAX_DEVICES)
+    return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mutex);
+  if (!g_scratch[dev]) {
+    void* ptr = nullptr;
+    if (!hip_ok(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)),
+                "hipGetSymbolAddress(scratch)"))
+      return nullptr;
+    g_scratch[dev] = static_cast<unsigned char*>(ptr);
+  }
+  return g_scratch[dev];
+}
+
+// per cell: the lowest failing cell (atomicMin) and the counts of
+// integrated and failed cells, into head[1..3]
+__global__ void na_summary_kernel(int ncell, const int* flags, int* head) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  if (flags[c] != 0) {
+    atomicMin(&head[1], c);
+    atomicAdd(&head[3], 1);
+  } else {
+    atomicAdd(&head[2], 1);
+  }
+}
+// head[1..3] = (ncell, 0, 0) before the summary
+__global__ void na_head_kernel(int ncell, int* head) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    head[1] = ncell;
+    head[2] = 0;
+    head[3] = 0;
+  }
+}
+
+std::mutex g_noalloc_mutex;
+
+}  // namespace
+
+extern "C" {
+
+int krome_solve_cells_device_noalloc(int ncell, double* n, int layout,
+                                     long ld, double* Tgas, const double* dt,
+                                     const double* par, int par_stride,
+                                     int* flags,
+                                     krome_device_summary* summary,
+                                     const krome_opts* h) {
+  if (summary) *summary = krome_device_summary{0, 0, 0};
+  if (!h || ncell < 0 || !n || !Tgas || !dt) return -1;
+  if (layout != KROME_LAYOUT_SPECIES_MAJOR &&
+      layout != KROME_LAYOUT_CELL_MAJOR)
+    return -1;
+  if (ld != 0 && ld < ncell) return -1;
+  if (par && par_stride != 0 && par_stride != Net::NPAR) return -1;
+  if (ncell == 0) return 0;
+  if (ncell > KX_NOALLOC_MAX_CELLS) {
+    std::fprintf(stderr,
+                 "krome: krome_solve_cells_device_noalloc: %d cells exceed "
+                 "its maximum of %ld\n",
+                 ncell, (long)KX_NOALLOC_MAX_CELLS);
+    return -1;
+  }
+  if (!hip_ok(hipDeviceSynchronize(), "fence")) return -1;
+  unsigned char* base = scratch_of_current_device();
+  if (!base) return -1;
+  std::lock_guard<std::mutex> lk(g_noalloc_mutex);
+  int* head = reinterpret_cast<int*>(base);
+  double* dpar = reinterpret_cast<double*>(base + NA_HEAD);
+  int* dflags = reinterpret_cast<int*>(base + NA_HEAD + NA_PAR);
+  char* ord = reinterpret_cast<char*>(base + NA_HEAD + NA_PAR + NA_FLAGS);
+  char* work = ord + NA_ORDER;
+  int* fl = flags ? flags : dflags;
+  // no photon field and no post-solve projection here (their rows and
+  // totals would need buffers of their own)
+  const bool unsupported =
+      Net::NPHOTO > 0 || h->nbins > 0 || h->conserve == 2;
+  const double* P = par;
+  int pst = par ? par_stride : 0;
+  if (!P) {
+    double d[Net::NPAR > 0 ? Net::NPAR : 1];
+    krome_params_default(d);
+    if (!hip_ok(hipMemcpy(dpar, d, sizeof(double) * Net::NPAR,
+                          hipMemcpyHostToDevice), "H2D"))
+      return -1;
+    P = dpar;
+  }
+  const int np = kx::params_check_device<Net>(ncell, P, pst, fl, head);
+  if (np < 0) return -1;
+  if (np > 0) return kx::FLAG_PARAM_UNSET;
+  int rj = unsupported ? kx::FLAG_UNSUPPORTED : reject(h);
+  kx::CellsArgs a;
+  a.ncell = ncell;
+  a.n = n;
+  strides(layout, ld, ncell, a.n_cs, a.n_ss);
+  a.T = Tgas;
+  a.dt = dt;
+  a.par = P;
+  a.par_stride = pst;
+  a.flags = fl;
+  a.stats = nullptr;
+  a.o = solve_opts(h);
+  SolveMem m;
+  m.work = work;
+  m.work_bytes = NA_WORK;
+  m.ord = ord;
+  m.ord_bytes = NA_ORDER;
+  int r = rj;
+  if (rj == 0) r = solve_kernels(a, *h, m, nullptr);
+  if (r == kx::FLAG_UNSUPPORTED || (rj != 0 && r != 0)) {
+    kx::fill_flags(ncell, fl, r);
+    return r;
+  }
+  if (r != 0) return r;
+  const dim3 g((ncell + 255) / 256), b(256);
+  hipLaunchKernelGGL(na_head_kernel, dim3(1), dim3(64), 0, nullptr, ncell,
+                     head);
+  hipLaunchKernelGGL(na_summary_kernel, g, b, 0, nullptr, ncell,
+                     (const int*)fl, head);
+  int hh[4] = {0, 0, 0, 0};
+  if (!hip_ok(hipGetLastError(), "summary kernel") ||
+      !hip_ok(hipMemcpy(hh, head, sizeof(hh), hipMemcpyDeviceToHost), "D2H"))
+    return -1;
+  if (summary) {
+    summary->device = hh[2];
+    summary->failed = hh[3];
+  }
+  if (hh[1] >= ncell) return 0;
+  int f = 0;
+  if (!hip_ok(hipMemcpy(&f, fl + hh[1], sizeof(int), hipMemcpyDeviceToHost),
+              "D2H"))
+    return -1;
+  return f;
 }
 
 // ---- element totals and consistent advection ----------------------------
