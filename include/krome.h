@@ -214,6 +214,36 @@ int krome_solve_cells_device_noalloc(int ncell, double* n, int layout,
                                      krome_device_summary* summary,
                                      const krome_opts* h);
 
+/* This build's extension, no counterpart in the reference (which has no
+ * adjoint): the discrete adjoint of the per-cell BDF solve.  Integrates
+ * the cells like krome_solve_cells_device (n species-major [NSPEC][ncell],
+ * Tgas, dt, flags [ncell], par [ncell][NPAR] or one row, all in device
+ * memory, n and Tgas updated in place) with every rate k_r scaled by
+ * exp(lnk[c][r]) (lnk [ncell][NREA] on the device, NULL: the network's
+ * rates), in FP64 with its own integrator.  With a cotangent of the
+ * final state (gn [NSPEC][ncell] and/or gT [ncell], device, NULL: zero;
+ * both NULL: the forward only), it also writes the vector-Jacobian
+ * products g_lnk [ncell][NREA] (by lnk), g_n0 [NSPEC][ncell] (by the
+ * initial densities) and, with evolve_T, g_T0 [ncell] (by the initial
+ * Tgas): the gradient of the integration as the forward run stepped it
+ * (its step sizes, orders and accepted attempts held fixed).  The forward
+ * is integrated again for the gradient, with a checkpoint every
+ * checkpoint_steps accepted steps (option, default 32) in a pool of
+ * checkpoint_memory bytes (option, 0: room for about 1536 steps per cell
+ * of a launch of up to 65536 cells); a cell that runs out of it gets
+ * KROME_FLAG_DEVICE_MEMORY and, like any failed cell, keeps its inputs
+ * with the identity's gradient.  mem_bytes (host, may be NULL): the peak
+ * device bytes of the call.  conserve != 0, photo reactions, photon bins,
+ * linsol 2 and path 1 return KROME_FLAG_UNSUPPORTED with every cell
+ * flagged; device_kernel and mixed_precision are ignored.  Returns 0, the
+ * lowest failing cell's flag, -105, or -1 on bad arguments. */
+int krome_adjoint_cells_device(int ncell, double* n, double* Tgas,
+                               const double* dt, const double* par,
+                               int par_stride, const double* lnk, int* flags,
+                               const double* gn, const double* gT,
+                               double* g_lnk, double* g_n0, double* g_T0,
+                               double* mem_bytes, const krome_opts* h);
+
 /* This build's extension, no counterpart in the reference's C ABI: the
  * device-resident solve with the per-cell counters of krome::SolveStats
  * (runtime/krome.hpp:257-268; 9 per cell: steps, rhs_evals, jac_evals,

@@ -840,5 +840,20 @@ struct net_primordial3 {
     cools[2] = kx::cooling_continuum(Tth, rho(n), ntot(n));
     (void)Tth; (void)n; (void)k; (void)p; (void)heats; (void)cools;
   }
+  // d(heat - cool)/dk_r [erg cm^-3 s^-1 per unit rate] into c[NREA]:
+  // the CHEM terms, the only thermal terms that read the rates
+  template <class R>
+  KX_HD static void chem_dk(R Tth, const R* n, R* c) {
+    for (int r = 0; r < NREA; ++r) c[r] = R(0);
+    const R ffac = kx::h2_heat_fraction(Tth, n[0], n[3], Hnuclei(n));
+    c[9] += (R(3.53) * ffac) * n[2]*n[0] * R(1.602176634e-12);
+    c[12] += (R(1.83) * ffac) * n[4]*n[0] * R(1.602176634e-12);
+    c[24] += (R(4.48) * ffac) * n[0]*n[0]*n[0] * R(1.602176634e-12);
+    c[25] += (R(4.48) * ffac) * n[3]*n[0]*n[0] * R(1.602176634e-12);
+    c[15] -= R(4.48) * n[3]*n[8] * R(1.602176634e-12);
+    c[16] -= R(4.48) * n[3]*n[0] * R(1.602176634e-12);
+    c[26] -= R(4.48) * n[3]*n[3] * R(1.602176634e-12);
+    (void)Tth; (void)n;
+  }
 };
 }  // namespace kx

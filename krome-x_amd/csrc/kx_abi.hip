@@ -19,6 +19,7 @@
 #include "kx_massfrac.h"
 #include "kx_query.h"
 #include "kx_photo.h"
+#include "kx_adjoint.h"
 #include "../../include/krome.h"
 
 using Net = kx::KX_NET;
@@ -53,6 +54,10 @@ struct Opts {
   // persistent kernel: cells per queue grab, workgroups per CU
   int queue_chunk = 0;
   int waves_per_cu = 0;
+  // the adjoint: accepted steps between checkpoints, the checkpoint
+  // pool's bytes (0: room for about 1536 steps per cell of a launch)
+  int checkpoint_steps = 32;
+  long long checkpoint_memory = 0;
   // photon bins (krome_opts_set_photo): edges [eV], the in-bin spectrum
   // exponent and the bin-averaged cross sections sigma[NPHOTO][nbins],
   // followed by their photoelectron-energy averages sigmaE[NPHOTO][nbins]
@@ -145,10 +150,11 @@ struct DevCache {
   Buf xn, xin, rho;  // the mass-fraction entry points' densities
   Buf prow, sig, phi;  // photo: kernel rows, cross sections, a host field
   Buf pwork, pord;     // the persistent kernel's workspace and queue order
+  Buf awork, aiwork, apool, anext;  // the adjoint's workspace and pool
   void release() {
     for (Buf* b : {&work, &iwork, &n, &T, &dt, &par, &flags, &stats, &queue,
                    &count, &cons, &xn, &xin, &rho, &prow, &sig, &phi, &pwork,
-                   &pord})
+                   &pord, &awork, &aiwork, &apool, &anext})
       b->release();
   }
 };
@@ -565,6 +571,16 @@ int krome_opts_set(krome_opts* h, const char* key, double v) {
     h->queue_chunk = (int)v;
     return 0;
   }
+  if (is("checkpoint_steps")) {
+    if (v < 1 || v > 1024) return -1;
+    h->checkpoint_steps = (int)v;
+    return 0;
+  }
+  if (is("checkpoint_memory")) {
+    if (v < 0) return -1;
+    h->checkpoint_memory = (long long)v;
+    return 0;
+  }
   if (is("waves_per_cu")) {
     if (v < 0) return -1;
     h->waves_per_cu = (int)v;
@@ -600,6 +616,8 @@ int krome_opts_get(const krome_opts* h, const char* key, double* v) {
   else if (is("mixed_precision")) *v = h->mixed;
   else if (is("queue_chunk")) *v = h->queue_chunk;
   else if (is("waves_per_cu")) *v = h->waves_per_cu;
+  else if (is("checkpoint_steps")) *v = h->checkpoint_steps;
+  else if (is("checkpoint_memory")) *v = (double)h->checkpoint_memory;
   else return -1;
   return 0;
 }
@@ -1184,6 +1202,156 @@ int krome_solve_cells_device_noalloc(int ncell, double* n, int layout,
               "D2H"))
     return -1;
   return f;
+}
+
+}  // extern "C"
+
+// ---- the discrete adjoint --------------------------------------------------
+namespace {
+
+// cells of one adjoint launch (its workspace is sized for them)
+constexpr long ADJ_CHUNK = 65536;
+
+template <bool ET>
+int adjoint_launch(kx::AdjArgs g, const Opts& o, long& peak) {
+  using Prob = kx::AdjProblem<Net, ET>;
+  constexpr long CKD = kx::adj_ck_doubles<Net, ET>();
+  const bool bwd = g.gn || g.gT;
+  const int seg = o.checkpoint_steps;
+  const int fails = o.max_err_fails + o.max_conv_fails;
+  g.seg_steps = seg;
+  g.margin = bwd ? 4 * fails + 8 : 0;
+  g.cap = bwd ? 4 * seg + g.margin : 0;
+  g.ycap = bwd ? seg + 2 * (o.max_err_fails + 2) + 16 : 0;
+  const long nc = g.a.ncell;
+  const long S = nc < ADJ_CHUNK ? nc : ADJ_CHUNK;
+  const size_t wd = sizeof(double) *
+                    (size_t)kx::adj_cell_doubles<Net, ET>(g.cap, g.ycap) * S;
+  const size_t wi = sizeof(int) * (size_t)Prob::N * S;
+  long slots = 0;
+  if (bwd) {
+    // automatic: room for about 1536 steps per cell on average (the
+    // cells share the pool)
+    const long per = seg < 32 ? 1536 / seg : 48;
+    slots = o.checkpoint_memory > 0
+                ? (long)(o.checkpoint_memory / (long long)(CKD * 8))
+                : per * S;
+    if (slots < 1) return kx::FLAG_DEVICE_MEMORY;
+  }
+  DevCache& dc = g_cache;
+  if (!dc.awork.ensure(wd) || !dc.aiwork.ensure(wi) ||
+      (bwd && (!dc.apool.ensure(sizeof(double) * (size_t)slots * CKD) ||
+               !dc.anext.ensure(sizeof(unsigned long long)))))
+    return kx::FLAG_DEVICE_MEMORY;
+  g.work = static_cast<double*>(dc.awork.p);
+  g.iwork = static_cast<int*>(dc.aiwork.p);
+  g.S = S;
+  g.pool = static_cast<double*>(dc.apool.p);
+  g.pool_slots = slots;
+  g.pool_next = static_cast<unsigned long long*>(dc.anext.p);
+  peak = 0;
+  for (long c0 = 0; c0 < nc; c0 += S) {
+    kx::AdjArgs b = g;
+    const long m = (nc - c0) < S ? (nc - c0) : S;
+    b.a.ncell = (int)m;
+    b.a.n = g.a.n + c0;  // species-major: element i of cell c at i ld + c
+    b.a.T = g.a.T + c0;
+    b.a.dt = g.a.dt + c0;
+    b.a.par = g.a.par + c0 * g.a.par_stride;
+    b.a.flags = g.a.flags + c0;
+    b.lnk = g.lnk ? g.lnk + c0 * Net::NREA : nullptr;
+    b.gn = g.gn ? g.gn + c0 : nullptr;
+    b.gT = g.gT ? g.gT + c0 : nullptr;
+    b.g_lnk = g.g_lnk ? g.g_lnk + c0 * Net::NREA : nullptr;
+    b.g_n0 = g.g_n0 ? g.g_n0 + c0 : nullptr;
+    b.g_T0 = g.g_T0 ? g.g_T0 + c0 : nullptr;
+    unsigned long long zero = 0;
+    if (bwd && !hip_ok(hipMemcpy(b.pool_next, &zero, sizeof(zero),
+                                 hipMemcpyHostToDevice), "H2D"))
+      return -1;
+    hipLaunchKernelGGL((kx::adjoint_cells_kernel<Net, ET>),
+                       dim3((m + 63) / 64), dim3(64), 0, nullptr, b);
+    if (!hip_ok(hipGetLastError(), "adjoint kernel")) return -1;
+    unsigned long long used = 0;
+    if (bwd) {
+      if (!hip_ok(hipMemcpy(&used, b.pool_next, sizeof(used),
+                            hipMemcpyDeviceToHost), "D2H"))
+        return -1;
+      if ((long)used > slots) used = slots;
+    }
+    const long p = (long)(wd + wi + used * CKD * sizeof(double));
+    if (p > peak) peak = p;
+  }
+  return hip_ok(hipDeviceSynchronize(), "adjoint kernel") ? 0 : -1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int krome_adjoint_cells_device(int ncell, double* n, double* Tgas,
+                               const double* dt, const double* par,
+                               int par_stride, const double* lnk, int* flags,
+                               const double* gn, const double* gT,
+                               double* g_lnk, double* g_n0, double* g_T0,
+                               double* mem_bytes, const krome_opts* h) {
+  if (!h || ncell < 0 || !n || !Tgas || !dt || !par || !flags) return -1;
+  if (par_stride != 0 && par_stride != Net::NPAR) return -1;
+  const bool bwd = gn || gT;
+  if (bwd && (!g_lnk || !g_n0 || (h->evolve_T && !g_T0))) return -1;
+  if (ncell == 0) return 0;
+  if (!hip_ok(hipDeviceSynchronize(), "fence")) return -1;
+  std::lock_guard<std::mutex> lk(g_cache_mutex);
+  if (!g_cache.count.ensure(sizeof(int))) return kx::FLAG_DEVICE_MEMORY;
+  const int np = kx::params_check_device<Net>(
+      ncell, par, par_stride, flags, static_cast<int*>(g_cache.count.p));
+  if (np < 0) return -1;
+  if (np > 0) return kx::FLAG_PARAM_UNSET;
+  // the adjoint differentiates the plain solve: no post-solve conservation
+  // (its projection is another map), no photon fields, no host path
+  // (device_kernel and mixed_precision do not apply: its own FP64 kernel)
+  const int rj = (h->linsol == 2 || h->path == 1 || h->conserve != 0 ||
+                  Net::NPHOTO > 0 || h->nbins > 0 ||
+                  (h->evolve_T && !Net::THERMAL))
+                     ? kx::FLAG_UNSUPPORTED
+                     : 0;
+  if (rj != 0) {
+    kx::fill_flags(ncell, flags, rj);
+    return rj;
+  }
+  kx::AdjArgs g;
+  g.a.ncell = ncell;
+  g.a.n = n;
+  g.a.n_cs = 1;
+  g.a.n_ss = ncell;
+  g.a.T = Tgas;
+  g.a.dt = dt;
+  g.a.par = par;
+  g.a.par_stride = par_stride;
+  g.a.flags = flags;
+  g.a.stats = nullptr;
+  g.a.o = solve_opts(h);
+  g.ld = ncell;
+  g.lnk = lnk;
+  g.gn = gn;
+  g.gT = gT;
+  g.g_lnk = g_lnk;
+  g.g_n0 = g_n0;
+  g.g_T0 = g_T0;
+  long peak = 0;
+  int r;
+  if constexpr (Net::THERMAL)
+    r = h->evolve_T ? adjoint_launch<true>(g, *h, peak)
+                    : adjoint_launch<false>(g, *h, peak);
+  else
+    r = adjoint_launch<false>(g, *h, peak);
+  if (mem_bytes) *mem_bytes = (double)peak;
+  if (r != 0) return r;
+  std::vector<int> hf(ncell);
+  if (!hip_ok(hipMemcpy(hf.data(), flags, sizeof(int) * ncell,
+                        hipMemcpyDeviceToHost), "D2H"))
+    return -1;
+  return first_flag(hf.data(), ncell);
 }
 
 // ---- element totals and consistent advection ----------------------------

@@ -100,14 +100,27 @@ constexpr long cell_doubles() {
          Prob::NEWTON_DOUBLES;
 }
 
+// the recorder of the integrator's operations on its Nordsieck array:
+// none (the solve kernels).  An active Tape (the adjoint kernel's,
+// kx_adjoint.h) receives, in the order they happen, every linear map the
+// step applies to zn (rescale, the Pascal predictor and its undo, order
+// changes), each accepted step's corrector (order, gamma, l, the new
+// state) and each zn[1] = h f(zn[0]).
+struct NoTape {
+  static constexpr bool ACTIVE = false;
+};
+
 // the integrator of one cell on strided storage; Prob provides
 //   int rhs(const double* y, double* f)   (registers, N values)
 //   int jac(const double* y, double* J)   (registers, N x N col-major,
 //                                          zeroed by the caller)
-template <int N, class Prob>
+template <int N, class Prob, class Tape = NoTape>
 class Cell {
  public:
   CellStats st;
+  Tape tp;
+  // iter(): the integration goes on
+  static constexpr int ITER_CONTINUE = 1;
 
   KX_HD Cell(Prob& prob, double* work, int* iwork, long stride,
              const CellOpts<N>& opts)
@@ -153,6 +166,17 @@ class Cell {
 
   // CVode(tout, CV_NORMAL) (bdf_cell.hpp:566-638); yout registers
   KX_HD int solve(double tout, double* yout, double* tret) {
+    const int r = begin(tout, tret);
+    if (r != BDF_OK) return r;
+    long nloc = 0;
+    for (;;) {
+      const int k = iter(tout, yout, tret, nloc);
+      if (k != ITER_CONTINUE) return k;
+    }
+  }
+  // the part of CVode before its step loop: TOO_CLOSE, the weights, the
+  // first rhs, the initial step, zn[1] = h f(zn[0]); BDF_OK or a flag
+  KX_HD int begin(double tout, double* tret) {
     using namespace bdfc;
     *tret = tn;
     const double tdist = ::fabs(tout - tn);
@@ -171,9 +195,16 @@ class Cell {
     }
     hscale = h; h0u = h; hprime = h;
     scal(h, zn[1], zn[1]);
-    long nloc = 0;
+    if constexpr (Tape::ACTIVE) tape_zn1f();
+    return BDF_OK;
+  }
+  // one pass of CVode's step loop (nloc: the steps of this call):
+  // ITER_CONTINUE, BDF_OK at tout (yout = y(tout)), or a failure flag
+  // (yout = zn[0], the last accepted state)
+  KX_HD int iter(double tout, double* yout, double* tret, long& nloc) {
+    using namespace bdfc;
     int istate = BDF_OK;
-    for (;;) {
+    do {
       next_h = h;
       next_q = q;
       if (st.nst > 0 && ewt_set(zn[0], ewt) != 0) { istate = ILL_INPUT; break; }
@@ -191,15 +222,83 @@ class Cell {
         next_h = hprime;
         return BDF_OK;
       }
-    }
+      return ITER_CONTINUE;
+    } while (0);
     *tret = tn;
     for (int i = 0; i < N; i++) yout[i] = at(zn[0], i);
     return istate;
+  }
+  // the interpolation weights of CVodeGetDky(t, 0): y(t) = sum_j c[j]
+  // zn[j], j <= the returned order
+  KX_HD int dky_weights(double t, double* c) const {
+    const double s = (t - tn) / h;
+    for (int j = 0; j <= QMAX; j++) {
+      double v = 1.0;
+      for (int i = 0; i < j; i++) v *= s;
+      c[j] = j <= q ? v : 0.0;
+    }
+    return q;
+  }
+
+  // the state that persists from one step to the next, as doubles (the
+  // integer members exactly): the scalars, the Nordsieck array, the error
+  // weights, the saved Jacobian, the Newton matrix's storage and its
+  // pivots; element k at ck[k * cs]
+  static constexpr int CK_SCALARS = 64;
+  static constexpr long CK_DOUBLES = CK_SCALARS + (long)(LMAX + 1) * N +
+                                     (long)N * N + Prob::NEWTON_DOUBLES + N;
+  KX_HD void save(double* ck, long cs) const { ckpt<true>(ck, cs); }
+  KX_HD void load(const double* ck, long cs) {
+    ckpt<false>(const_cast<double*>(ck), cs);
   }
 
   KX_HD double current_step() const { return next_h; }
 
  private:
+  template <bool SAVE, class T>
+  KX_HD static void ck1(double* ck, long cs, long& k, T& v) {
+    if (SAVE) ck[k * cs] = (double)v;
+    else v = (T)ck[k * cs];
+    ++k;
+  }
+  template <bool SAVE>
+  KX_HD void ckpt(double* ck, long cs) const {
+    Cell& m = const_cast<Cell&>(*this);
+    long k = 0;
+    double* d[] = {&m.tn, &m.h, &m.hin, &m.hprime, &m.hscale, &m.h0u,
+                   &m.next_h, &m.hu, &m.eta, &m.etamax, &m.etaq, &m.etaqm1,
+                   &m.etaqp1, &m.tolsf, &m.saved_tq5, &m.rl1, &m.gamma,
+                   &m.gammap, &m.gamrat, &m.crate, &m.delp, &m.delnrm,
+                   &m.acnrm};
+    for (double* v : d) ck1<SAVE>(ck, cs, k, *v);
+    for (int i = 0; i <= LMAX; i++) ck1<SAVE>(ck, cs, k, m.tau[i]);
+    for (int i = 0; i < 6; i++) ck1<SAVE>(ck, cs, k, m.tq[i]);
+    for (int i = 0; i < LMAX; i++) ck1<SAVE>(ck, cs, k, m.l[i]);
+    int* iv[] = {&m.q, &m.qprime, &m.next_q, &m.qu, &m.L, &m.qwait,
+                 &m.convfail};
+    for (int* v : iv) ck1<SAVE>(ck, cs, k, *v);
+    long* lv[] = {&m.nscon, &m.nstlp, &m.nstlj, &m.st.nst, &m.st.nfe,
+                  &m.st.nfeDQ, &m.st.nje, &m.st.nsetups, &m.st.nni,
+                  &m.st.nnf, &m.st.ncfn, &m.st.netf};
+    for (long* v : lv) ck1<SAVE>(ck, cs, k, *v);
+    bool* bv[] = {&m.acnrmcur, &m.jcur, &m.nls_jcur};
+    for (bool* v : bv) ck1<SAVE>(ck, cs, k, *v);
+    k = CK_SCALARS;
+    for (int j = 0; j < LMAX; j++)
+      for (int i = 0; i < N; i++) ck1<SAVE>(ck, cs, k, m.at(m.zn[j], i));
+    for (int i = 0; i < N; i++) ck1<SAVE>(ck, cs, k, m.at(m.ewt, i));
+    for (long i = 0; i < (long)N * N; i++) ck1<SAVE>(ck, cs, k, m.J[i * S]);
+    for (long i = 0; i < Prob::NEWTON_DOUBLES; i++)
+      ck1<SAVE>(ck, cs, k, m.A[i * S]);
+    for (int i = 0; i < N; i++) ck1<SAVE>(ck, cs, k, m.piv[i * S]);
+  }
+  // the tape's records of zn[1] = h f(zn[0]) and of an accepted step
+  KX_HD void tape_zn1f() {
+    double yv[N];
+    for (int i = 0; i < N; i++) yv[i] = at(zn[0], i);
+    tp.zn1f(h, yv);
+  }
+
   Prob& P;
   const long S;
   const CellOpts<N>& o;
@@ -432,6 +531,7 @@ class Cell {
       xiold = xi;
     }
     const double A1 = (-a0 - a1) / prod;
+    if constexpr (Tape::ACTIVE) tp.order_up(q, L, A1, l);
     scal(A1, zn[QMAX], zn[L]);
     for (int j = 2; j <= q; j++) axpy(l[j], zn[L], zn[j]);
   }
@@ -444,9 +544,11 @@ class Cell {
       const double xi = hsum / hscale;
       for (int i = j + 2; i >= 2; i--) l[i] = l[i] * xi + l[i - 1];
     }
+    if constexpr (Tape::ACTIVE) tp.order_down(q, l);
     for (int j = 2; j < q; j++) axpy(-l[j], zn[q], zn[j]);
   }
   KX_HD void rescale() {
+    if constexpr (Tape::ACTIVE) tp.rescale(eta, q);
     double c = eta;
     for (int j = 1; j <= q; j++) {
       scal(c, zn[j], zn[j]);
@@ -458,11 +560,13 @@ class Cell {
     nscon = 0;
   }
   KX_HD void predict() {
+    if constexpr (Tape::ACTIVE) tp.pascal(1, q);
     tn += h;
     for (int k = 1; k <= q; k++)
       for (int j = q; j >= k; j--) axpy(1.0, zn[j], zn[j - 1]);
   }
   KX_HD void restore(double saved_t) {
+    if constexpr (Tape::ACTIVE) tp.pascal(-1, q);
     tn = saved_t;
     for (int k = 1; k <= q; k++)
       for (int j = q; j >= k; j--) axpy(-1.0, zn[j], zn[j - 1]);
@@ -968,6 +1072,7 @@ class Cell {
     if (r < 0) return RHSFUNC_FAIL;
     if (r > 0) return UNREC_RHSFUNC_ERR;
     scal(h, tempv, zn[1]);
+    if constexpr (Tape::ACTIVE) tape_zn1f();
     return TRY_AGAIN;
   }
   // cvCompleteStep (bdf_cell.hpp:1386-1400)
@@ -981,9 +1086,15 @@ class Cell {
     tau[1] = h;
     for (int j = 0; j <= q; j++) axpy(l[j], acor, zn[j]);
     qwait--;
-    if (qwait == 1 && q != QMAX) {
+    const bool save_q = qwait == 1 && q != QMAX;
+    if (save_q) {
       for (int i = 0; i < N; i++) at(zn[QMAX], i) = at(acor, i);
       saved_tq5 = tq[5];
+    }
+    if constexpr (Tape::ACTIVE) {
+      double yv[N];
+      for (int i = 0; i < N; i++) yv[i] = at(zn[0], i);
+      tp.step(q, gamma, rl1, l, yv, save_q);
     }
   }
   // cvPrepareNextStep and the order / step choice (bdf_cell.hpp:1403-1488)
@@ -1033,6 +1144,7 @@ class Cell {
       eta = etaqp1;
       qprime = q + 1;
       for (int i = 0; i < N; i++) at(zn[QMAX], i) = at(acor, i);
+      if constexpr (Tape::ACTIVE) tp.saveq();
     }
     set_eta();
   }

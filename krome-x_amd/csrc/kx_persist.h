@@ -1666,7 +1666,7 @@ __device__ __attribute__((always_inline)) inline void persist_run(
           double y[N];
           for (int i = 0; i < N; i++) y[i] = w.at(V_Y, i, s);
           if (flag == 0 && ET &&
-              !(y[N - 1] > 0.0 && y[N - 1] - y[N - 1] == 0.0))
+              !(y[N - 1] > 0.0 && kx::finite(y[N - 1])))
             flag = FLAG_NONPOSITIVE_T;
           if (P.hit_T0 && (flag == REPTD_RHSFUNC_ERR || flag == RHSFUNC_FAIL ||
                            flag == UNREC_RHSFUNC_ERR ||

@@ -81,6 +81,11 @@ KX_HD inline double tanh(double x) { return ::tanh(x); }
 KX_HD inline double fabs(double x) { return ::fabs(x); }
 KX_HD inline double fmin(double x, double y) { return ::fmin(x, y); }
 KX_HD inline double fmax(double x, double y) { return ::fmax(x, y); }
+// a finite value (not inf, not NaN).  By the builtin, not by x - x == 0:
+// with multiply-add contraction, x - x of a product a b compiles to
+// fma(a, b, -a b), the product's rounding error, which need not be 0
+KX_HD inline bool finite(double x) { return __builtin_isfinite(x); }
+KX_HD inline bool finite(float x) { return __builtin_isfinite(x); }
 // (FP32: the mixed-precision Newton matrix)
 KX_HD inline float fabs(float x) { return ::fabsf(x); }
 KX_HD inline float fmin(float x, float y) { return ::fminf(x, y); }
@@ -102,6 +107,15 @@ KX_HD inline R ipow(R x) {
     else return h * h;
   }
 }
+
+}  // namespace kx
+
+// the dual numbers of the adjoint's exact Jacobians: declared here, after
+// the scalar math and before every template written over it, so that the
+// kx:: calls of those templates find the dual overloads too
+#include "kx_dual.h"
+
+namespace kx {
 
 namespace phys {
 constexpr double kB = 1.380649e-16;           // erg / K

@@ -86,6 +86,7 @@ KX_HD inline void cell_finish(const CellsArgs& a, int c, const double* y) {
   if (ET) a.T[c] = y[NS];
 }
 
+#if defined(__HIPCC__)
 template <class Net, bool ET>
 __global__ void __launch_bounds__(128)
 plain_cells_kernel(CellsArgs a, double* work, int* iwork, long S) {
@@ -130,7 +131,7 @@ plain_cells_kernel(CellsArgs a, double* work, int* iwork, long S) {
     const double hc = cv.current_step();
     cv.reinit(t, y, hc > 0.0 ? 0.1 * hc : 0.0);
   }
-  if (flag == 0 && ET && !(y[N - 1] > 0.0 && y[N - 1] - y[N - 1] == 0.0))
+  if (flag == 0 && ET && !(y[N - 1] > 0.0 && kx::finite(y[N - 1])))
     flag = FLAG_NONPOSITIVE_T;
   if (P.hit_T0 && (flag == REPTD_RHSFUNC_ERR || flag == RHSFUNC_FAIL ||
                    flag == UNREC_RHSFUNC_ERR || flag == FIRST_RHSFUNC_ERR))
@@ -143,5 +144,6 @@ plain_cells_kernel(CellsArgs a, double* work, int* iwork, long S) {
   }
   if (flag == 0) cell_finish<Net, ET>(a, c, y);
 }
+#endif
 
 }  // namespace kx

@@ -74,7 +74,7 @@ struct Problem {
   }
   template <class R>
   KX_HD static bool kx_isfinite(R x) {
-    return x == x && x - x == R(0);
+    return kx::finite(x);
   }
 };
 

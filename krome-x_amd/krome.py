@@ -41,7 +41,8 @@ _KEYS = ("evolve_T", "jac", "linsol", "rtol", "atol_n", "atol_T", "atol_u",
          "clip_negative", "error_report", "renormalize_x", "conserve",
          "path", "device_cache", "device_memory", "min_device",
          "min_interleave", "device_kernel", "mixed_precision",
-         "queue_chunk", "waves_per_cu")
+         "queue_chunk", "waves_per_cu", "checkpoint_steps",
+         "checkpoint_memory")
 LAYOUT_SPECIES_MAJOR = 0
 LAYOUT_CELL_MAJOR = 1
 
@@ -202,6 +203,14 @@ class Network:
                   ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                   ctypes.c_void_p]),
                 ("krome_device_last_kernel_ms", ctypes.c_int, [_D]),
+                ("krome_solve_cells_device_noalloc", ctypes.c_int,
+                 [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_long,
+                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                  ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                  ctypes.c_void_p]),
+                ("krome_adjoint_cells_device", ctypes.c_int,
+                 [ctypes.c_int] + [ctypes.c_void_p] * 4 + [ctypes.c_int] +
+                 [ctypes.c_void_p] * 9),
                 ("krome_get_rates", ctypes.c_int,
                  [ctypes.c_int, _D, _D, _D, ctypes.c_int, _D,
                   ctypes.c_void_p, _D]),
@@ -441,6 +450,55 @@ class Network:
             dt.data_ptr(), par.data_ptr(), self.npar,
             None, flags.data_ptr() if flags is not None else None, None,
             opts._h)
+
+    def adjoint_device(self, n, Tgas, dt, par, opts=None, lnk=None,
+                       flags=None, gn=None, gT=None):
+        """the discrete adjoint's solve on torch tensors (this build's
+        extension, krome_adjoint_cells_device): n [nspec, ncell] species-
+        major, Tgas, dt [ncell], par [ncell, npar], float64 on the GPU, n
+        and Tgas updated in place; lnk [ncell, nrea] the log-rate offsets
+        (rates k_r exp(lnk_r); None: 0).  With the cotangents gn [nspec,
+        ncell] and/or gT [ncell] of the final state, also returns the
+        gradients (g_lnk [ncell, nrea], g_n0 [nspec, ncell], g_T0 [ncell]
+        or None at fixed Tgas).  Returns (flag, gradients or None, the
+        peak device bytes of the adjoint)"""
+        import torch
+        ncell = int(Tgas.shape[0])
+        ts = [n, Tgas, dt, par] + [t for t in (lnk, gn, gT) if t is not None]
+        for t in ts:
+            if (not t.is_cuda or not t.is_contiguous()
+                    or t.dtype != torch.float64):
+                raise KromeError("adjoint_device needs contiguous float64 "
+                                 "GPU tensors")
+        if tuple(n.shape) != (self.nspec, ncell):
+            raise KromeError("n must be [nspec, ncell]")
+        if tuple(par.shape) != (ncell, self.npar):
+            raise KromeError("par must be [ncell, npar]")
+        if lnk is not None and tuple(lnk.shape) != (ncell, self.nrea):
+            raise KromeError("lnk must be [ncell, nrea]")
+        opts = self._options(opts)
+        if flags is None:
+            flags = torch.zeros(ncell, dtype=torch.int32, device=n.device)
+        grads = None
+        ptrs = [None] * 5
+        if gn is not None or gT is not None:
+            et = bool(opts["evolve_T"])
+            g_lnk = torch.zeros((ncell, self.nrea), dtype=torch.float64,
+                                device=n.device)
+            g_n0 = torch.zeros_like(n)
+            g_T0 = torch.zeros_like(Tgas) if et else None
+            grads = (g_lnk, g_n0, g_T0)
+            ptrs = [None if gn is None else gn.data_ptr(),
+                    None if gT is None else gT.data_ptr(),
+                    g_lnk.data_ptr(), g_n0.data_ptr(),
+                    None if g_T0 is None else g_T0.data_ptr()]
+        mem = ctypes.c_double(0.0)
+        r = self._lib.krome_adjoint_cells_device(
+            ncell, n.data_ptr(), Tgas.data_ptr(), dt.data_ptr(),
+            par.data_ptr(), self.npar,
+            None if lnk is None else lnk.data_ptr(), flags.data_ptr(),
+            *ptrs, ctypes.addressof(mem), opts._h)
+        return r, grads, mem.value
 
     # -- queries (the reference binding's rates / fluxes / thermal) --------
 

@@ -1674,6 +1674,34 @@ def _emit_thermal(w, net, th, pnames, photo=()):
               % i)
     w("    (void)Tth; (void)n; (void)k; (void)p; (void)heats; (void)cools;")
     w("  }")
+    # the derivatives of heating minus cooling by each rate: the CHEM terms
+    # are the only thermal terms that read k, each linear in its rate (the
+    # adjoint's d(dT/dt)/d log k)
+    w("  // d(heat - cool)/dk_r [erg cm^-3 s^-1 per unit rate] into c[NREA]:")
+    w("  // the CHEM terms, the only thermal terms that read the rates")
+    w("  template <class R>")
+    w("  KX_HD static void chem_dk(R Tth, const R* n, R* c) {")
+    w("    for (int r = 0; r < NREA; ++r) c[r] = R(0);")
+    if "CHEM" in th.heating:
+        pe = {e["r"]: e for e in photo}
+        sol = [r for r in net.reactions if is_solomon(r, pe.get(r.index))]
+        if heat or sol:
+            w("    const R ffac = kx::h2_heat_fraction(Tth, n[%d], n[%d], "
+              "Hnuclei(n));" % (idx("H"), idx("H2")))
+            for r, q, ff in heat:
+                prod = "*".join("n[%d]" % idx(s) for s, _ in r.reactants)
+                w("    c[%d] += (R(%r) * ffac) * %s * R(%r);"
+                  % (r.index, q, prod, EV_TO_ERG))
+            for r in sol:
+                w("    c[%d] += (R(18.7) * ffac + R(0.4)) * n[%d] * R(%r);"
+                  % (r.index, idx("H2"), EV_TO_ERG))
+    if "CHEM" in th.cooling:
+        for r, q, _ in cool:
+            prod = "*".join("n[%d]" % idx(s) for s, _ in r.reactants)
+            w("    c[%d] -= R(%r) * %s * R(%r);" % (r.index, q, prod,
+                                                    EV_TO_ERG))
+    w("    (void)Tth; (void)n;")
+    w("  }")
 
 
 # the reference's fixed lists of thermal terms (solve_cell.hpp:206-231):
