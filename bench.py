@@ -6,7 +6,8 @@ gamma EXACT, H2 opacity RIPAMONTI; rtol 1e-8, atol_n 1e-20, atol_T 1e-6).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--ncell N]
                     [--kernel persistent|plain] [--fp64]
-                    [--workload solve|conserve|multi|photo|network|copy]
+                    [--workload solve|conserve|multi|photo|network|copy|
+                                adjoint]
                     [--linsol dense|sparse]
 
 A step is one solve of the whole batch from its initial state: the inputs
@@ -53,6 +54,13 @@ the primordialZ cells of the uneven batch): one device-resident solve of
 static-pivot sparse LU, and by the persistent FP64 kernel where the
 network has at most 16 equations; cells/s from the kernels' events, the
 steps per cell, the nonzero flags.
+
+--workload adjoint measures the discrete adjoint (C4): 65536 primordial3
+collapse cells (seed 3, rtol 1e-8), one forward solve and the two backward
+passes giving the gradients of log(n_H2/nH) and log(Tgas) at t_end by all
+log k_r, log n_i(0) and log Tgas(0); cells/s of the whole job, the peak
+device memory of a backward pass (workspace and checkpoints), and the
+plain FP64 kernel's time on the same cells beside the forward's.
 
 --workload copy is the calibration of the HBM traffic counters: one
 device-to-device copy of 2 GiB (far above the last-level cache), whose
@@ -498,6 +506,85 @@ def network_bench(a):
         "kernels": out}))
 
 
+def adjoint_bench(a):
+    """C4: the discrete adjoint on primordial3 collapse cells (seed 3, the
+    C2 physics, FP64): one forward solve and the two backward passes that
+    give the gradients of log(n_H2/nH) and log(Tgas) at t_end by all log
+    k_r, log n_i(0) and log Tgas(0) (each backward integrates again with a
+    checkpoint every 32 steps and runs the segments backwards)"""
+    import torch
+    import cells
+    import krome
+
+    net = krome.load("primordial3")
+    w = cells.primordial3_cells(a.ncell, seed=3)
+    dev = torch.device("cuda:0")
+    t = lambda x: torch.tensor(np.ascontiguousarray(x), dtype=torch.float64,
+                               device=dev)
+    n0, T0, dt, par = t(w["n"].T), t(w["T"]), t(w["dt"]), t(w["par"])
+    opts = net.options(evolve_T=True, rtol=1e-8, atol_n=1e-20, atol_T=1e-6,
+                       error_report=False)
+    wH = np.array(net.composition[:, net.elements.index("H")], dtype=float)
+    iH2 = net.idx("H2")
+    res = {}
+    for rep in range(a.warmup + 1):
+        n, T = n0.clone(), T0.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r, _, _ = net.adjoint_device(n, T, dt, par, opts)
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t0
+        nf, Tf = n.cpu().numpy().T, T.cpu().numpy()
+        nH = nf @ wH
+        g1 = -wH[None, :] / nH[:, None]
+        g1[:, iH2] += 1.0 / nf[:, iH2]
+        t_bwd, mem, flags = [], 0.0, 0
+        for gn, gT in ((g1, np.zeros(a.ncell)),
+                       (np.zeros_like(nf), 1.0 / Tf)):
+            n, T = n0.clone(), T0.clone()
+            fl = torch.zeros(a.ncell, dtype=torch.int32, device=dev)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            rb, grads, m = net.adjoint_device(n, T, dt, par, opts, flags=fl,
+                                              gn=t(gn.T), gT=t(gT))
+            torch.cuda.synchronize()
+            t_bwd.append(time.perf_counter() - t0)
+            mem = max(mem, m)
+            flags += int((fl.cpu().numpy() != 0).sum())
+        res = {"forward_s": t_fwd, "backward_s": t_bwd, "peak_bytes": mem,
+               "flags_nonzero": flags, "return": r}
+        print("adjoint rep %d: forward %.2f s, backward %s s, %.1f MB" % (
+            rep, t_fwd, ["%.2f" % x for x in t_bwd], mem / 1e6),
+            file=sys.stderr, flush=True)
+    # the plain FP64 kernel on the same cells (the forward's yardstick)
+    n, T = n0.clone(), T0.clone()
+    po = net.options(evolve_T=True, rtol=1e-8, atol_n=1e-20, atol_T=1e-6,
+                     device_kernel="plain", error_report=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    net.solve_device(n, T, dt, par, opts=po)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    total = res["forward_s"] + sum(res["backward_s"])
+    print(json.dumps({
+        "metric": "cells/sec, gradients of log(n_H2/nH) and log(Tgas) at "
+                  "t_end by all log k_r, log n_i(0), log Tgas(0) "
+                  "(forward + 2 adjoint passes)",
+        "value": a.ncell / total, "unit": "cells/s", "n_gpus": 1,
+        "steps": 1, "warmup": a.warmup, "ms_per_step": total * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "%d primordial3 collapse cells (seed 3), "
+                   "rtol 1e-8, checkpoint every 32 steps" % a.ncell,
+                   "parallelism": "1 GPU"},
+        "forward_s": res["forward_s"], "backward_s": res["backward_s"],
+        "forward_cells_per_s": a.ncell / res["forward_s"],
+        "plain_fp64_forward_s": t_plain,
+        "peak_device_bytes": res["peak_bytes"],
+        "peak_bytes_per_cell": res["peak_bytes"] / min(a.ncell, 65536),
+        "flags_nonzero": res["flags_nonzero"]}))
+
+
 def copy_calibration():
     """one 2 GiB device-to-device copy (the traffic counters' reference)"""
     import torch
@@ -528,7 +615,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--workload", default="solve",
                     choices=("solve", "conserve", "multi", "photo", "network",
-                             "copy"))
+                             "copy", "adjoint"))
     ap.add_argument("--network", default="primordialZ")
     ap.add_argument("--cells", default="c3_primordialZ")
     ap.add_argument("--linsol", default="dense", choices=("dense", "sparse"),
@@ -556,6 +643,10 @@ def main():
         return photo_bench(a)
     if a.workload == "multi":
         return multi_bench(a)
+    if a.workload == "adjoint":
+        if a.ncell == 1 << 20:
+            a.ncell = 1 << 16
+        return adjoint_bench(a)
     import torch
     import cells
     import krome

@@ -24,8 +24,14 @@ For each workload it:
      the generated tables, and solve_cells with linsol 3 on the solve
      fixture's cells -> <name>_sparse.npz.
 
+  7. for the adjoint (C4): primordial3 with one rate multiplier per reaction
+     line, checked to scale its own rate only, solved unperturbed and with
+     each log k_r, log n_i(0) and log Tgas(0) moved by +-h (h = 1e-5, 1e-6,
+     rtol 1e-10) -> primordial3_adjoint.npz (the outputs log(n_H2/nH) and
+     log(Tgas) of every solve).
+
   python3 tests/golden/make_golden.py [solve] [funcs] [conserve] [sparse]
-  (default: all four)
+  [photo] [extra] [adjoint] (default: all but extra)
 
 The generated project and the binaries stay in tests/golden/_build/ (git-
 ignored, not committed).
@@ -393,6 +399,114 @@ def make_photo(name, exes, meta):
                         **out)
 
 
+# ---- the adjoint's perturbed solves -----------------------------------------
+# C4: the gradients of log(n_H2/nH) and log(Tgas) at t_end by every log k_r,
+# log n_i(0) and log Tgas(0), as central differences of the reference's own
+# solves.  The reference has no per-reaction scale at run time (rate_scale
+# is network-wide, docs/network-format.md:712-720), so each reaction line of
+# primordial3.knet gets a parameter of its own, kmulNN (NN its index among
+# the reaction lines, the index of its rate), multiplying its rate
+ADJ_STEPS = (1e-5, 1e-6)
+ADJ_RTOL = 1e-10
+ADJ_CELLS = 256
+
+
+def adjoint_knet(src, dst):
+    """src with every reaction line's rate times kmulNN (default 1);
+    returns the number of reaction lines"""
+    out, j = [], 0
+    for ln in src.read_text().splitlines():
+        body = ln.split("#", 1)[0]
+        if "->" in body and "]" in body.split("->", 1)[1]:
+            head, _, expr = body.partition("]")
+            out.append("%s] (%s)*kmul%02d" % (head, expr.strip(), j))
+            j += 1
+        else:
+            out.append(ln)
+    at = next(k for k, x in enumerate(out) if x.startswith("krome_format"))
+    out[at + 1:at + 1] = ["param kmul%02d = 1.0" % k for k in range(j)]
+    dst.write_text("\n".join(out) + "\n")
+    return j
+
+
+def make_adjoint():
+    name = "primordial3_kmul"
+    BUILD.mkdir(parents=True, exist_ok=True)
+    knet = BUILD / "primordial3_kmul.knet"
+    R = adjoint_knet(REF / "networks" / "primordial3.knet", knet)
+    TOML[name] = TOML["primordial3"].replace(
+        '"{net}/primordial3.knet"', '"%s"' % knet)
+    proj = generate(name)
+    exe = compile_driver(proj, name, "bitwise")
+    pn = BUILD / "kmul_params.txt"
+    subprocess.run([str(exe), "params", "-", str(pn)], check=True)
+    names = pn.read_text().split()
+    meta = cells.network_meta("primordial3")
+    assert meta["nrea"] == R
+    w = cells.primordial3_cells(ADJ_CELLS, seed=3)
+    nc, ns = w["n"].shape
+
+    def rows(par, kmul):
+        """parameter rows of the kmul project: the base rows' fields by name,
+        the multipliers kmul [ncell, R]"""
+        out = np.zeros((par.shape[0], len(names)))
+        for k, nm in enumerate(names):
+            out[:, k] = (kmul[:, int(nm[4:])] if nm.startswith("kmul")
+                         else par[:, meta["params"].index(nm)])
+        return out
+    # each multiplier scales its own rate and no other (the index map)
+    fw = func_samples("primordial3")
+    nf = fw["n"].shape[0]
+    one = np.ones((nf, R))
+    k0 = run_funcs(exe, dict(fw, par=rows(fw["par"], one)), "kmul_f0", R,
+                   ns + 1)[0]
+    for r in range(R):
+        km = one.copy()
+        km[:, r] = 2.0
+        k1 = run_funcs(exe, dict(fw, par=rows(fw["par"], km)), "kmul_f", R,
+                       ns + 1)[0]
+        expect = k0.copy()
+        expect[:, r] *= 2.0
+        assert np.array_equal(k1, expect), r
+    # the solves: the cells unperturbed, then per step h, per parameter
+    # (the R log k_r, the ns log n_i(0), log Tgas(0)) and sign
+    P = R + ns + 1
+    runs = [(None, 0, 0.0)] + [(h, p, sg) for h in ADJ_STEPS
+                               for p in range(P) for sg in (1.0, -1.0)]
+    n = np.concatenate([w["n"]] * len(runs))
+    T = np.concatenate([w["T"]] * len(runs))
+    km = np.ones((len(runs) * nc, R))
+    for k, (h, p, sg) in enumerate(runs):
+        blk = slice(k * nc, (k + 1) * nc)
+        if h is None:
+            continue
+        if p < R:
+            km[blk, p] = np.exp(sg * h)
+        elif p < R + ns:
+            n[blk, p - R] *= np.exp(sg * h)
+        else:
+            T[blk] *= np.exp(sg * h)
+    big = {"n": n, "T": T, "dt": np.concatenate([w["dt"]] * len(runs)),
+           "par": rows(np.concatenate([w["par"]] * len(runs)), km)}
+    opts = np.array([1, ADJ_RTOL, 1e-20, 1e-6, 1, 0, 1e6])
+    nf_, Tf, flag, stats, secs = run_solve(exe, big, "kmul_solve", opts)
+    wH = np.array(meta["natoms"])[:, meta["elements"].index("H")].astype(
+        float)
+    iH2 = meta["species"].index("H2")
+    L1 = (np.log(nf_[:, iH2]) - np.log(nf_ @ wH)).reshape(len(runs), nc)
+    L2 = np.log(Tf).reshape(len(runs), nc)
+    fl = flag.reshape(len(runs), nc)
+    shape = (len(ADJ_STEPS), P, 2, nc)
+    np.savez_compressed(
+        HERE / "primordial3_adjoint.npz", n0=w["n"], T0=w["T"], dt=w["dt"],
+        par=w["par"], steps=np.array(ADJ_STEPS), rtol=ADJ_RTOL,
+        L1_0=L1[0], L2_0=L2[0], L1=L1[1:].reshape(shape),
+        L2=L2[1:].reshape(shape), flags=fl[1:].reshape(shape),
+        flags_0=fl[0], n_final=nf_[:nc], T_final=Tf[:nc])
+    print("primordial3 adjoint: %d solves of %d cells in %.1f s, flags %s"
+          % (len(runs), nc, secs, np.unique(fl)))
+
+
 def make_extra(name, exe):
     """a generated network beyond the benchmark ones: the reference's
     rates and rhs at the generic cells and the T grid, and its solves of
@@ -427,8 +541,10 @@ def main(argv=None):
     only = {a for a in argv if a in names}
     argv = [a for a in argv if a not in names]
     parts = set(argv if argv else ("solve", "funcs", "conserve", "sparse",
-                                   "photo"))
+                                   "photo", "adjoint"))
     BUILD.mkdir(parents=True, exist_ok=True)
+    if "adjoint" in parts and (not only or "primordial3" in only):
+        make_adjoint()
     for name in ("galaxy_ism", "galaxy_ism_heat"):
         if "photo" not in parts or (only and name not in only):
             continue

@@ -404,14 +404,27 @@ int do_conserve(const In& in, const std::vector<double>& ref,
   return 0;
 }
 
+// the parameter row's field names (krome::param_names), one a line
+int do_params(const char* out) {
+  FILE* f = std::fopen(out, "w");
+  if (!f) return 3;
+  for (int i = 0; i < krome::NPAR; ++i)
+    std::fprintf(f, "%s\n", krome::param_names[i]);
+  std::fclose(f);
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
   if (argc != 4) {
-    std::fprintf(stderr, "usage: %s solve|funcs|totals|conserve|meta|sparse IN OUT\n",
+    std::fprintf(stderr,
+                 "usage: %s solve|funcs|totals|conserve|meta|sparse|params "
+                 "IN OUT\n",
                  argv[0]);
     return 1;
   }
+  if (!std::strcmp(argv[1], "params")) return do_params(argv[3]);
   In in;
   if (!read_in(argv[2], in)) {
     std::fprintf(stderr, "cannot read %s\n", argv[2]);

@@ -494,3 +494,76 @@ def test_autograd_solve(p3):
     for a, b in ((lnk.grad, gl), (n0.grad, gn0), (T0.grad, gT0)):
         assert np.array_equal(a.cpu().numpy(), b)
     assert solve.last_memory > 0
+
+
+def _ref_fd(d, k):
+    """the reference's central differences [2 steps][ncell, P] of output k
+    (0: log(n_H2/nH), 1: log Tgas) from primordial3_adjoint.npz"""
+    L = d["L1"] if k == 0 else d["L2"]
+    steps = d["steps"]
+    return [((L[i, :, 0] - L[i, :, 1]) / (2 * steps[i])).T
+            for i in range(len(steps))]
+
+
+def _ref_check(adj, d, k, name, cells_=None):
+    fd5, fd6 = _ref_fd(d, k)
+    if cells_ is not None:
+        fd5, fd6 = fd5[cells_], fd6[cells_]
+    conv, err, spread = _settled(adj, fd5, fd6)
+    ok = err <= 1e-3 + spread
+    say(name, "vs the reference's perturbed solves: settled %.3f" %
+        conv.mean(), "median %.2e p99 %.2e" % (np.median(err[conv]),
+                                                np.quantile(err[conv], 0.99)),
+        "within 1e-3 + spread %.4f" % ok[conv].mean())
+    assert conv.mean() > 0.7
+    assert ok[conv].mean() > 0.99
+    # the two solvers differ in the last bits of their rate expressions
+    # (the forward agrees within 1e-6): a median far inside the bar
+    assert np.median(err[conv]) < 1e-4
+
+
+def test_adjoint_vs_reference_perturbed_host(host):
+    """the adjoint (its host build) against the reference's own perturbed
+    solves (tests/golden/primordial3_adjoint.npz, written by make_golden.py:
+    primordial3 with a multiplier per reaction, each log k_r, log n_i(0)
+    and log Tgas(0) moved by +-1e-5 and +-1e-6 at rtol 1e-10), on the first
+    32 of its 256 C4 cells, both outputs"""
+    d = golden("primordial3_adjoint.npz")
+    sel = np.arange(32)
+    w = {"n": d["n0"][sel], "T": d["T0"][sel], "dt": d["dt"][sel],
+         "par": d["par"][sel]}
+    nc, ns = w["n"].shape
+    iH2 = 3
+    wH = np.array([1, 1, 1, 2, 2, 0, 0, 0, 0.0])
+    fl, n, T, *_ = _host_adjoint(host, w, np.zeros((nc, ns)), np.zeros(nc),
+                                 rtol=float(d["rtol"]))
+    assert (fl == 0).all()
+    assert rel_err(n, d["n_final"][sel]) < 1e-6
+    nH = n @ wH
+    g1 = -wH[None, :] / nH[:, None]
+    g1[:, iH2] += 1.0 / n[:, iH2]
+    for k, (gn, gT) in enumerate(((g1, np.zeros(nc)),
+                                  (np.zeros((nc, ns)), 1.0 / T))):
+        fl, _, _, gl, gn0, gT0 = _host_adjoint(host, w, gn, gT,
+                                               rtol=float(d["rtol"]))
+        adj = np.hstack([gl, gn0 * w["n"], (gT0 * w["T"])[:, None]])
+        _ref_check(adj, d, k, "host output %d" % k, sel)
+
+
+@pytest.mark.gpu
+def test_adjoint_vs_reference_perturbed(p3):
+    """the GPU adjoint against the reference's perturbed solves, all 256
+    C4 cells, both outputs (as the host test)"""
+    d = golden("primordial3_adjoint.npz")
+    w = {"n": d["n0"], "T": d["T0"], "dt": d["dt"], "par": d["par"]}
+    o = _opts(p3, rtol=float(d["rtol"]))
+    say("adjoint vs reference perturbed solves")
+    r, n, T = _forward(p3, w, o)
+    assert r == 0
+    assert rel_err(n, d["n_final"]) < 1e-6
+    for k in (0, 1):
+        (_, gn, gT) = _outputs(p3, n, T)[k]
+        rr, fl, _, _, gl, gn0, gT0, _ = _grads(p3, w, o, gn, gT)
+        assert rr == 0 and (fl == 0).all()
+        adj = np.hstack([gl, gn0 * w["n"], (gT0 * w["T"])[:, None]])
+        _ref_check(adj, d, k, "output %d" % k)
