@@ -149,12 +149,23 @@ def cpu_baseline(w, nsample):
                             flags.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                             None)
     el = time.perf_counter() - t0
+    ratio = None
+    tf = REPO / "tests" / "golden" / "cpu_timing.json"
+    if tf.exists():
+        ratio = json.loads(tf.read_text()).get("primordial3", {}).get(
+            "restatement_over_reference")
     return {"value": nsample / el, "unit": "cells/s", "cores": int(cores),
             "kind": "port",
+            "restatement_over_reference_in_container": ratio,
             "sample": "the first %d of the same cells, CPU restatement of "
                       "the reference per-cell solve (OpenMP dynamic, the "
                       "reference's production flags -O3 -ffp-contract=fast), "
-                      "%.1f s wall, return %d" % (nsample, el, r)}, n, T
+                      "%.1f s wall, return %d; restatement_over_reference"
+                      "_in_container: the restatement's cells/s over the "
+                      "reference's own per-cell solve on the same cells "
+                      "and cores, measured where the reference builds "
+                      "(tests/golden/time_cpu.py)" % (nsample, el, r)}, \
+        n, T, flags
 
 
 def conserve_bench(a):
@@ -613,6 +624,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 18)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-fp64-persistent", action="store_true",
+                    help="skip the persistent FP64 kernel's leg")
     ap.add_argument("--workload", default="solve",
                     choices=("solve", "conserve", "multi", "photo", "network",
                              "copy", "adjoint"))
@@ -672,6 +685,8 @@ def main():
                        mixed_precision=1 if mixed else 0,
                        linsol=3 if a.linsol == "sparse" else 1)
     plain = net.options(**base, device_kernel="plain")
+    pers64 = net.options(**base, device_kernel="persistent",
+                         mixed_precision=0, linsol=1)
 
     def run(opts, st=None):
         n.copy_(n0)
@@ -714,30 +729,55 @@ def main():
     ms_per_step = 1e3 * wall / a.steps
     value = a.ncell * a.steps / wall
 
+    def agreement(n_ref, T_ref, fl_ref, nh, Th, flh):
+        """the headline's result nh [nspec, m] against a reference result
+        on the same cells: max relative error per species (over the
+        values above 1e-12 of the cell total) and of Tgas, over the cells
+        both finish; flags equal"""
+        ok = (fl_ref == 0) & (flh == 0)
+        ntot = np.abs(n_ref).sum(axis=0)
+        big = np.abs(n_ref) > 1e-12 * ntot[None, :]
+        rel = np.abs(nh - n_ref) / np.maximum(np.abs(n_ref), 1e-300)
+        rel = np.where(big & ok[None, :], rel, 0.0)
+        Trel = np.where(ok, np.abs(Th - T_ref) / T_ref, 0.0)
+        return {"max_rel_species": float(rel.max()),
+                "max_rel_by_species": {sp: float(rel[i].max())
+                                       for i, sp in enumerate(net.species)},
+                "max_rel_Tgas": float(Trel.max()),
+                "flags_equal": bool(np.array_equal(fl_ref, flh))}
+
+    def one_solve(opts):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        run(opts)  # per-cell flags are compared, not fatal
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - t1
+        kms1 = ctypes.c_double()
+        net._lib.krome_device_last_kernel_ms(
+            ctypes.cast(ctypes.byref(kms1), ctypes.POINTER(ctypes.c_double)))
+        return (tp, kms1.value, n.cpu().numpy(), T.cpu().numpy(),
+                flags.cpu().numpy())
+
     # the plain FP64 drop-in on the same cells (ratio, agreement)
     plain_res = None
     if not a.no_plain and a.kernel != "plain":
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        run(plain)
-        torch.cuda.synchronize()
-        tp = time.perf_counter() - t1
-        n_pl = n.cpu().numpy()
-        T_pl = T.cpu().numpy()
-        fl_pl = flags.cpu().numpy()
-        ok = (fl_pl == 0) & (fl_head == 0)
-        ntot = n_pl.sum(axis=0)
-        big = np.abs(n_pl) > 1e-12 * ntot[None, :]
-        rel = np.abs(n_head - n_pl) / np.maximum(np.abs(n_pl), 1e-300)
-        rel = np.where(big & ok[None, :], rel, 0.0)
-        Trel = np.where(ok, np.abs(T_head - T_pl) / T_pl, 0.0)
-        plain_res = {"value": a.ncell / tp, "seconds": tp,
-                     "max_rel_species_vs_fp64": float(rel.max()),
-                     "max_rel_species_vs_fp64_by_species": {
-                         s: float(rel[i].max())
-                         for i, s in enumerate(net.species)},
-                     "max_rel_Tgas_vs_fp64": float(Trel.max()),
-                     "flags_equal": bool(np.array_equal(fl_pl, fl_head))}
+        tp, kp, n_pl, T_pl, fl_pl = one_solve(plain)
+        ag = agreement(n_pl, T_pl, fl_pl, n_head, T_head, fl_head)
+        plain_res = {"value": a.ncell / tp, "seconds": tp, "kernel_ms": kp,
+                     "max_rel_species_vs_fp64": ag["max_rel_species"],
+                     "max_rel_species_vs_fp64_by_species":
+                         ag["max_rel_by_species"],
+                     "max_rel_Tgas_vs_fp64": ag["max_rel_Tgas"],
+                     "flags_equal": ag["flags_equal"]}
+    # the persistent kernel in FP64 on the same cells: what the mixed
+    # precision adds over persistence
+    p64_res = None
+    if mixed and not a.no_fp64_persistent:
+        tq, kq, n_q, T_q, fl_q = one_solve(pers64)
+        ag = agreement(n_q, T_q, fl_q, n_head, T_head, fl_head)
+        p64_res = {"value": a.ncell / tq, "seconds": tq, "kernel_ms": kq,
+                   "flags_nonzero": int((fl_q != 0).sum()),
+                   "mixed_vs_persistent_fp64": ag}
 
     # roofline of the solve kernel: algorithmic FLOPs per launch from the
     # cells' counters over the kernel's event-timed duration
@@ -745,17 +785,22 @@ def main():
         st_head, opc, sparse=(cells.network_meta("primordial3")["sparse"]
                               if a.linsol == "sparse" else None))
     tot = float((fr + fdq + flu + fvec).sum())
-    fp32 = float((fdq + flu).sum()) if mixed else 0.0
+    # the FP32 share: the Newton matrix's formation, LU and triangular
+    # solves (the difference quotients and every rhs run in FP64)
+    fp32 = float(flu.sum()) if mixed else 0.0
     achieved = tot / (kern_ms * 1e-3)
-    traffic = None
+    traffic, traffic_src = None, None
     tf = REPO / "profiles" / "traffic.json"
     if tf.exists():
-        traffic = json.loads(tf.read_text()).get(
-            "mixed" if mixed else ("persistent_fp64" if a.kernel ==
-                                   "persistent" else "plain"))
-    roof = {"bound": "mfma", "achieved": achieved / 1e12,
+        tj = json.loads(tf.read_text())
+        key = "mixed" if mixed else ("persistent_fp64" if a.kernel ==
+                                     "persistent" else "plain")
+        traffic = tj.get(key)
+        traffic_src = tj.get("_source_" + key)
+    roof = {"bound": "fp64_valu", "achieved": achieved / 1e12,
             "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64, "traffic": traffic,
+            "traffic_source": traffic_src,
             "compute": "FP64 vector ALU (MI355X FP64 vector peak = FP64 "
                        "matrix peak, 78.6 TFLOP/s); transcendentals count "
                        "one op each",
@@ -764,9 +809,14 @@ def main():
             "algorithmic_bytes_per_launch": BYTES_PER_CELL * a.ncell,
             "hbm_frac": BYTES_PER_CELL * a.ncell / (kern_ms * 1e-3)
             / PEAK_HBM}
-    cpu = None
+    cpu, oracle_ag = None, None
     if not a.no_cpu:
-        cpu, _, _ = cpu_baseline(w, min(a.cpu_sample, a.ncell))
+        m = min(a.cpu_sample, a.ncell)
+        cpu, n_or, T_or, fl_or = cpu_baseline(w, m)
+        # the headline's result against the oracle on the sample
+        oracle_ag = agreement(np.ascontiguousarray(n_or.T), T_or, fl_or,
+                              n_head[:, :m], T_head[:m], fl_head[:m])
+        oracle_ag["cells"] = m
     steps = st_head[:, 0]
     out = {
         "metric": METRIC, "value": value, "unit": "cells/s", "n_gpus": 1,
@@ -786,6 +836,10 @@ def main():
         "plain_fp64": plain_res,
         "ratio_vs_plain_fp64": (value / plain_res["value"]
                                 if plain_res else None),
+        "persistent_fp64": p64_res,
+        "ratio_mixed_vs_persistent_fp64": (value / p64_res["value"]
+                                           if p64_res else None),
+        "headline_vs_oracle": oracle_ag,
         "ratio_vs_cpu": value / cpu["value"] if cpu else None,
     }
     print(json.dumps(out))

@@ -97,6 +97,19 @@ void krome_opts_free(krome_opts* h);
 int krome_opts_set(krome_opts* h, const char* key, double value);
 int krome_opts_get(const krome_opts* h, const char* key, double* value);
 
+/* per-species tolerances (krome.h:175-185): atol[NSPEC] [cm^-3] /
+ * rtol[NSPEC] replacing atol_n / rtol for each species; a negative entry
+ * takes the scalar, NULL the scalar for every species.  Copied into the
+ * handle.  Set returns -1 for a NaN, or a zero atol.  Get writes the
+ * tolerance each species gets (its entry, or the scalar).  With some
+ * species' rtol different from the scalar the error weights are
+ * 1 / fma(rtol_i, |y_i|, atol_i) (the reference's per-slot form) in every
+ * kernel */
+int krome_opts_set_atol_species(krome_opts* h, const double* atol);
+int krome_opts_set_rtol_species(krome_opts* h, const double* rtol);
+int krome_opts_get_atol_species(const krome_opts* h, double* atol);
+int krome_opts_get_rtol_species(const krome_opts* h, double* rtol);
+
 /* the device backend (krome.h:193-201): compiled in (1); initialize
  * selects the HIP device (device_id < 0: the current one) */
 int krome_has_device(void);

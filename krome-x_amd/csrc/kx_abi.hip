@@ -64,6 +64,10 @@ struct Opts {
   int nbins = 0;
   double photo_alpha = 1.0;
   std::vector<double> Elo, Ehi, sigma;
+  // per-species tolerances (krome_opts_set_atol_species / rtol_species):
+  // NSPEC entries each, empty when unset; a negative entry takes the
+  // scalar
+  std::vector<double> atol_sp, rtol_sp;
 };
 
 struct Flag {
@@ -111,6 +115,7 @@ const Flag kFlags[] = {
 };
 
 kx::SolveOpts solve_opts(const krome_opts* h);
+double species_tol(const std::vector<double>& v, int i, double scalar);
 // the device address of the current device's scratch (the solve that
 // allocates nothing), looked up on the device's first use
 unsigned char* scratch_of_current_device();
@@ -151,10 +156,11 @@ struct DevCache {
   Buf prow, sig, phi;  // photo: kernel rows, cross sections, a host field
   Buf pwork, pord;     // the persistent kernel's workspace and queue order
   Buf awork, aiwork, apool, anext;  // the adjoint's workspace and pool
+  Buf tol;  // the per-equation tolerances of a call (SolveOpts::tol)
   void release() {
     for (Buf* b : {&work, &iwork, &n, &T, &dt, &par, &flags, &stats, &queue,
                    &count, &cons, &xn, &xin, &rho, &prow, &sig, &phi, &pwork,
-                   &pord, &awork, &aiwork, &apool, &anext})
+                   &pord, &awork, &aiwork, &apool, &anext, &tol})
       b->release();
   }
 };
@@ -622,6 +628,43 @@ int krome_opts_get(const krome_opts* h, const char* key, double* v) {
   return 0;
 }
 
+// per-species tolerances (krome.h:175-185, krome.cpp:496-538): v[NSPEC]
+// copied into the handle (NULL: the scalar for every species); a NaN, or a
+// zero atol, is refused with the handle unchanged; get writes what each
+// species gets (its entry, or the scalar where the entry is negative)
+int krome_opts_set_atol_species(krome_opts* h, const double* atol) {
+  if (!h) return -1;
+  if (!atol) {
+    h->atol_sp.clear();
+    return 0;
+  }
+  for (int i = 0; i < NS; ++i)
+    if (std::isnan(atol[i]) || atol[i] == 0.0) return -1;
+  h->atol_sp.assign(atol, atol + NS);
+  return 0;
+}
+int krome_opts_set_rtol_species(krome_opts* h, const double* rtol) {
+  if (!h) return -1;
+  if (!rtol) {
+    h->rtol_sp.clear();
+    return 0;
+  }
+  for (int i = 0; i < NS; ++i)
+    if (std::isnan(rtol[i])) return -1;
+  h->rtol_sp.assign(rtol, rtol + NS);
+  return 0;
+}
+int krome_opts_get_atol_species(const krome_opts* h, double* atol) {
+  if (!h || !atol) return -1;
+  for (int i = 0; i < NS; ++i) atol[i] = species_tol(h->atol_sp, i, h->atol_n);
+  return 0;
+}
+int krome_opts_get_rtol_species(const krome_opts* h, double* rtol) {
+  if (!h || !rtol) return -1;
+  for (int i = 0; i < NS; ++i) rtol[i] = species_tol(h->rtol_sp, i, h->rtol);
+  return 0;
+}
+
 // ---- device ------------------------------------------------------------
 
 int krome_has_device(void) { return 1; }
@@ -685,6 +728,56 @@ kx::SolveOpts solve_opts(const krome_opts* h) {
   o.conserve = h->conserve;
   o.linsol = h->linsol;
   return o;
+}
+
+// a species' tolerance: its entry, or the scalar where the handle has no
+// array or the entry is negative (solve_cell.hpp:866-869)
+double species_tol(const std::vector<double>& v, int i, double scalar) {
+  return !v.empty() && !(v[i] < 0.0) ? v[i] : scalar;
+}
+
+// the per-equation tolerances of a call, SolveOpts::tol's layout (N
+// absolute values, then N relative ones, N = NSPEC + evolve_T; Tgas takes
+// atol_T and the scalar rtol), into t; false when the handle has none.
+// rtol_v: some species' rtol differs from the scalar, so the integrator
+// uses per-slot weights (solve_cell.hpp:872-879, 2129-2139)
+bool tol_rows(const krome_opts* h, std::vector<double>& t, int& rtol_v) {
+  if (h->atol_sp.empty() && h->rtol_sp.empty()) return false;
+  const int N = NS + (h->evolve_T ? 1 : 0);
+  t.assign(2 * N, 0.0);
+  rtol_v = 0;
+  for (int i = 0; i < NS; ++i) {
+    t[i] = species_tol(h->atol_sp, i, h->atol_n);
+    t[N + i] = species_tol(h->rtol_sp, i, h->rtol);
+    rtol_v |= t[N + i] != h->rtol;
+  }
+  if (h->evolve_T) {
+    t[NS] = h->atol_T;
+    t[N + NS] = h->rtol;
+  }
+  return true;
+}
+
+// the call's per-equation tolerances on the device, in buf (grown) or at
+// fixed (room for 2 (NSPEC + 1) doubles), their address in o.tol
+int attach_tol(kx::SolveOpts& o, const krome_opts* h, Buf* buf,
+               double* fixed) {
+  std::vector<double> t;
+  int rv = 0;
+  o.tol = nullptr;
+  o.rtol_v = 0;
+  if (!tol_rows(h, t, rv)) return 0;
+  double* d = fixed;
+  if (buf) {
+    if (!buf->ensure(sizeof(double) * t.size())) return kx::FLAG_DEVICE_MEMORY;
+    d = static_cast<double*>(buf->p);
+  }
+  if (!hip_ok(hipMemcpy(d, t.data(), sizeof(double) * t.size(),
+                        hipMemcpyHostToDevice), "H2D"))
+    return -1;
+  o.tol = d;
+  o.rtol_v = rv;
+  return 0;
 }
 
 // what the whole call rejects before integrating (every cell flagged)
@@ -821,6 +914,7 @@ int solve_host(int ncell, double* v, const double* rho, double* Tgas,
   a.flags = static_cast<int*>(dc.flags.p);
   a.stats = nullptr;
   a.o = solve_opts(h);
+  if (const int rt = attach_tol(a.o, h, &dc.tol, nullptr)) return rt;
   int r = rho ? solve_x_resident(a, static_cast<const double*>(dc.rho.p),
                                  false, h->renormalize_x ? 1 : 0, *h, dc, s)
               : solve_resident(a, *h, dc, s);
@@ -902,6 +996,7 @@ int solve_device(int ncell, double* v, const double* rho, int layout,
   a.flags = fl;
   a.stats = nullptr;
   a.o = solve_opts(h);
+  if (const int rt = attach_tol(a.o, h, &dc.tol, nullptr)) return rt;
   const int r = rho ? solve_x_resident(a, rho, true,
                                        h->renormalize_x ? 1 : 0, *h, dc,
                                        nullptr)
@@ -1012,6 +1107,7 @@ int krome_solve_cells_device_stats(int ncell, double* n, int layout, long ld,
   a.flags = flags;
   a.stats = stats;
   a.o = solve_opts(h);
+  if (const int rt = attach_tol(a.o, h, &g_cache.tol, nullptr)) return rt;
   return solve_resident(a, *h, g_cache, nullptr);
 }
 
@@ -1023,7 +1119,8 @@ int krome_solve_cells_device_stats(int ncell, double* n, int layout, long ld,
 // library's load until the process ends.  Its regions (bytes):
 //   head   256: the parameter-check count, the lowest failing cell, the
 //          counts of integrated and failed cells (ints)
-//   par    the default parameter row (when the caller gives none)
+//   par    the default parameter row (when the caller gives none), then
+//          the per-equation tolerances (2 (NSPEC + 1) doubles)
 //   flags  4 x KX_NOALLOC_MAX_CELLS (when the caller gives none)
 //   order  the persistent queue's order for KX_NOALLOC_MAX_CELLS cells
 //   work   the persistent workspace of this network (FP64, its largest
@@ -1050,7 +1147,8 @@ constexpr size_t na_persist_max() {
   return a > b ? a : b;
 }
 constexpr size_t NA_HEAD = 256;
-constexpr size_t NA_PAR = na_align(sizeof(double) * (Net::NPAR + 1));
+constexpr size_t NA_PAR =
+    na_align(sizeof(double) * (Net::NPAR + 1 + 2 * (NS + 1)));
 constexpr size_t NA_FLAGS = na_align(sizeof(int) * KX_NOALLOC_MAX_CELLS);
 constexpr size_t NA_ORDER =
     na_align(sizeof(int) * (2 * KX_NOALLOC_MAX_CELLS + kx::ORDER_BUCKETS));
@@ -1171,6 +1269,9 @@ int krome_solve_cells_device_noalloc(int ncell, double* n, int layout,
   a.flags = fl;
   a.stats = nullptr;
   a.o = solve_opts(h);
+  if (rj == 0)
+    if (const int rt = attach_tol(a.o, h, nullptr, dpar + Net::NPAR + 1))
+      return rt;
   SolveMem m;
   m.work = work;
   m.work_bytes = NA_WORK;
@@ -1331,6 +1432,7 @@ int krome_adjoint_cells_device(int ncell, double* n, double* Tgas,
   g.a.flags = flags;
   g.a.stats = nullptr;
   g.a.o = solve_opts(h);
+  if (const int rt = attach_tol(g.a.o, h, &g_cache.tol, nullptr)) return rt;
   g.ld = ncell;
   g.lnk = lnk;
   g.gn = gn;

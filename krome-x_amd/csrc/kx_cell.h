@@ -69,6 +69,10 @@ template <int N>
 struct CellOpts {
   double rtol;
   double atol[N];
+  // per-equation relative tolerances (KROME's rtol_species): the weights
+  // are then 1 / fma(rtolv_i, |y_i|, atol_i) (bdf_cell.hpp:691-697)
+  bool rtol_v;
+  double rtolv[N];
   long max_steps;
   int max_err_fails, max_conv_fails, max_restarts;
   bool dq_jac;
@@ -391,8 +395,18 @@ class Cell {
     return bdfc::rt(s / N);
   }
 
-  // cvEwtSetSV (bdf_cell.hpp:689-713)
+  // cvEwtSetSV, or KROME's per-species form (bdf_cell.hpp:689-713)
   KX_HD int ewt_set(const double* yc, double* w) {
+    if (o.rtol_v) {
+      for (int i = 0; i < N; i++) {
+        const double d =
+            ::fma(o.rtolv[i], ::fabs(at(const_cast<double*>(yc), i)),
+                  o.atol[i]);
+        if (!(d > 0.0)) return -1;
+        at(w, i) = 1.0 / d;
+      }
+      return 0;
+    }
     double v[N];
     for (int i = 0; i < N; i++) v[i] = ::fabs(at(const_cast<double*>(yc), i));
     for (int i = 0; i < N; i++) v[i] = (o.rtol * v[i]) + o.atol[i];

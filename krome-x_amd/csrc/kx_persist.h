@@ -237,10 +237,21 @@ struct Persist {
   }
   // cvEwtSetSV into vector dst (bdf_cell.hpp:689-713)
   static KX_HD double atol(const SolveOpts& o, int i) {
+    if (o.tol != nullptr) return o.tol[i];
     return i < NS ? o.atol_n : o.atol_T;
   }
   static KX_HD int ewt_set(const W& w, long s, const SolveOpts& o,
                            bool atolmin0, int src, int dst) {
+    if (o.rtol_v) {  // per-species rtol (o.tol set)
+      int bad = 0;
+      for (int i = 0; i < N; i++) {
+        const double d = ::fma(o.tol[N + i], ::fabs(w.at(src, i, s)),
+                               o.tol[i]);
+        bad |= !(d > 0.0);
+        w.at(dst, i, s) = 1.0 / d;
+      }
+      return bad ? -1 : 0;
+    }
     double v[N];
     for (int i = 0; i < N; i++) v[i] = ::fabs(w.at(src, i, s));
     for (int i = 0; i < N; i++) v[i] = (o.rtol * v[i]) + atol(o, i);
@@ -1083,8 +1094,13 @@ __device__ __attribute__((always_inline)) inline void persist_run(
   // the analytic Jacobian here is the mass-action block at fixed Tgas
   const bool supported = dq_jac || !(ET || Net::RATES_DENSITY);
   const bool inv = dq_jac && Net::HAS_E;  // the charge invariant
-  const bool atolmin0 =
+  bool atolmin0 =
       (ET ? (o.atol_T < o.atol_n ? o.atol_T : o.atol_n) : o.atol_n) == 0.0;
+  if (o.tol != nullptr) {
+    double m = o.tol[0];
+    for (int i = 1; i < N; i++) m = o.tol[i] < m ? o.tol[i] : m;
+    atolmin0 = m == 0.0;
+  }
   const long mxstep = o.max_steps == 0 ? 500 : o.max_steps;
   LaneState lane_state;
 

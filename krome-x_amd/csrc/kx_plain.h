@@ -36,6 +36,12 @@ KX_HD inline bool cell_opts(const SolveOpts& o,
   co.rtol = o.rtol;
   for (int i = 0; i < NS; ++i) co.atol[i] = o.atol_n;
   if (ET) co.atol[N - 1] = o.atol_T;
+  co.rtol_v = o.tol != nullptr && o.rtol_v != 0;
+  if (o.tol != nullptr)
+    for (int i = 0; i < N; ++i) {
+      co.atol[i] = o.tol[i];
+      co.rtolv[i] = co.rtol_v ? o.tol[N + i] : o.rtol;
+    }
   co.max_steps = o.max_steps;
   co.max_err_fails = o.max_err_fails;
   co.max_conv_fails = o.max_conv_fails;

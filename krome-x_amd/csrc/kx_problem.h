@@ -20,6 +20,13 @@ struct SolveOpts {
   int conserve;      // 0 off, 1 electrons restore the charge, 2 project
                      // onto the input totals, 3 neutralize
   int linsol;        // 0 auto, 1 dense, 3 the static-pivot sparse LU
+  // per-equation tolerances (device memory, the call's N equations):
+  // tol[0..N) the absolute ones, tol[N..2N) the relative ones (read only
+  // with rtol_v); nullptr: atol_n for every species, atol_T for Tgas and
+  // the scalar rtol (KROME's atol_species / rtol_species resolved per
+  // species on the host, solve_cell.hpp:866-879, 2129-2139)
+  const double* tol = nullptr;
+  int rtol_v = 0;    // the weights 1 / fma(rtol_i, |y_i|, atol_i)
 };
 
 template <class Net, bool ET>

@@ -52,7 +52,47 @@ class KromeError(RuntimeError):
 
 
 def _dptr(a):
-    return None if a is None else a.ctypes.data_as(_D)
+    """double* of a float64 numpy array (None: NULL)"""
+    if a is None:
+        return None
+    return ctypes.cast(a.ctypes.data, _D)
+
+
+class _Batch:
+    """one batch call's arrays in the C interface's conventions.
+
+    The cell rows (n or x, [ncell, nspec] row-major, or one cell as
+    [nspec]) are copied into a fresh C-ordered float64 block: the C
+    function updates that block in place, so the caller's array never
+    changes.  Per-cell scalars (Tgas, dt, rho) are expanded to one value
+    per cell.  take() gives an output back in the shape of the input: a
+    single cell given 1-D comes back 1-D, its scalars as scalars."""
+
+    def __init__(self, net, rows, what="n"):
+        a = np.asarray(rows, dtype=np.float64)
+        if a.ndim not in (1, 2) or a.shape[-1] != net.nspec:
+            raise KromeError(
+                "%s: expected %d values per cell ([%d] or [ncell, %d]), got "
+                "shape %s" % (what, net.nspec, net.nspec, net.nspec,
+                              a.shape))
+        self.one = a.ndim == 1
+        self.rows = np.array(a.reshape(-1, net.nspec), order="C")
+        self.ncell = self.rows.shape[0]
+
+    def column(self, v, what):
+        """one float64 per cell from a scalar or an [ncell] array"""
+        out = np.empty(self.ncell)
+        try:
+            out[:] = np.asarray(v, dtype=np.float64)
+        except ValueError:
+            raise KromeError("%s: expected a scalar or %d values, got shape "
+                             "%s" % (what, self.ncell, np.shape(v))) from None
+        return out
+
+    def take(self, a):
+        if a is None or not self.one:
+            return a
+        return a[0]
 
 
 class _Extra(ctypes.Structure):
@@ -177,6 +217,14 @@ class Network:
                  [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_double]),
                 ("krome_opts_get", ctypes.c_int,
                  [ctypes.c_void_p, ctypes.c_char_p, _D]),
+                ("krome_opts_set_atol_species", ctypes.c_int,
+                 [ctypes.c_void_p, _D]),
+                ("krome_opts_set_rtol_species", ctypes.c_int,
+                 [ctypes.c_void_p, _D]),
+                ("krome_opts_get_atol_species", ctypes.c_int,
+                 [ctypes.c_void_p, _D]),
+                ("krome_opts_get_rtol_species", ctypes.c_int,
+                 [ctypes.c_void_p, _D]),
                 ("krome_has_device", ctypes.c_int, []),
                 ("krome_device_initialize", ctypes.c_int, []),
                 ("krome_device_finalize", ctypes.c_int, []),
@@ -335,44 +383,37 @@ class Network:
         opts.set_photo() was called: Phi[nbins] for every cell or
         Phi[ncell, nbins].  Td and adust (the reference binding's dust-bin
         arguments) must be None: this build has no dust bins"""
-        self._no_dust(Td, adust)
-        single, n, ncell = self._cells(n)
-        T = self._per_cell(Tgas, ncell)
-        dtv = self._per_cell(dt, ncell)
-        opts = self._options(opts)
-        P, stride = self._params(par, ncell)
-        F = self._phi(Phi, ncell, opts)
-        flags = np.zeros(ncell, dtype=np.intc)
-        ierr = self._lib.krome_solve_cells(
-            ncell, _dptr(n), _dptr(T), _dptr(dtv), _dptr(P), stride,
-            _extra(F), flags.ctypes.data_as(_I), opts._h)
-        res = Result(n[0] if single else n, T[0] if single else T,
-                     flags[0] if single else flags, ierr,
-                     None if F is None else (F[0] if single else F))
-        self._check(check, ierr, flags, "krome_solve_cells")
-        return res
+        return self._solve("krome_solve_cells", n, (), Tgas, dt, par, Phi,
+                           opts, check, Td, adust)
 
     def solve_x(self, x, rho, Tgas, dt, par=None, Phi=None, opts=None,
                 check=True, Td=None, adust=None):
         """solve() with mass fractions x and the gas density rho [g cm^-3]
         (krome_solve_cells_x)"""
-        self._no_dust(Td, adust)
-        single, x, ncell = self._cells(x)
-        rh = self._per_cell(rho, ncell)
-        T = self._per_cell(Tgas, ncell)
-        dtv = self._per_cell(dt, ncell)
-        opts = self._options(opts)
-        P, stride = self._params(par, ncell)
-        F = self._phi(Phi, ncell, opts)
-        flags = np.zeros(ncell, dtype=np.intc)
-        ierr = self._lib.krome_solve_cells_x(
-            ncell, _dptr(x), _dptr(rh), _dptr(T), _dptr(dtv), _dptr(P),
+        return self._solve("krome_solve_cells_x", x, (("rho", rho),), Tgas,
+                           dt, par, Phi, opts, check, Td, adust)
+
+    def _solve(self, fname, rows, lead, Tgas, dt, par, Phi, opts, check, Td,
+               adust):
+        """the host-array solves: fname(ncell, rows, *lead, Tgas, dt, par,
+        stride, extra, flags, opts) on a private copy of the inputs"""
+        if Td is not None or adust is not None:
+            raise KromeError("Td / adust need dust bins, which this build "
+                             "does not have")
+        b = _Batch(self, rows)
+        opts = self._opts_or_default(opts)
+        cols = [b.column(v, what) for what, v in
+                (*lead, ("Tgas", Tgas), ("dt", dt))]
+        P, stride = self._param_block(par, b.ncell)
+        F = self._phi(Phi, b.ncell, opts)
+        flags = np.zeros(b.ncell, dtype=np.intc)
+        ierr = getattr(self._lib, fname)(
+            b.ncell, _dptr(b.rows), *(_dptr(c) for c in cols), _dptr(P),
             stride, _extra(F), flags.ctypes.data_as(_I), opts._h)
-        res = Result(x[0] if single else x, T[0] if single else T,
-                     flags[0] if single else flags, ierr,
-                     None if F is None else (F[0] if single else F))
-        self._check(check, ierr, flags, "krome_solve_cells_x")
-        return res
+        if check and ierr != 0:
+            self._raise_failed(fname, ierr, flags)
+        return Result(b.take(b.rows), b.take(cols[-2]), b.take(F),
+                      b.take(flags), ierr)
 
     def solve_device(self, n, Tgas, dt, par, opts=None, flags=None,
                      stats=None, sync=True, Phi=None):
@@ -390,7 +431,7 @@ class Network:
             raise KromeError("n must be [nspec, ncell]")
         if tuple(par.shape) != (ncell, self.npar):
             raise KromeError("par must be [ncell, npar]")
-        opts = self._options(opts)
+        opts = self._opts_or_default(opts)
         if Phi is not None:
             if stats is not None:
                 raise KromeError("no per-cell counters with a field")
@@ -444,7 +485,7 @@ class Network:
             raise KromeError("x must be %s" % (shape,))
         if tuple(par.shape) != (ncell, self.npar):
             raise KromeError("par must be [ncell, npar]")
-        opts = self._options(opts)
+        opts = self._opts_or_default(opts)
         return self._lib.krome_solve_cells_x_device(
             ncell, x.data_ptr(), rho.data_ptr(), layout, 0, Tgas.data_ptr(),
             dt.data_ptr(), par.data_ptr(), self.npar,
@@ -476,7 +517,7 @@ class Network:
             raise KromeError("par must be [ncell, npar]")
         if lnk is not None and tuple(lnk.shape) != (ncell, self.nrea):
             raise KromeError("lnk must be [ncell, nrea]")
-        opts = self._options(opts)
+        opts = self._opts_or_default(opts)
         if flags is None:
             flags = torch.zeros(ncell, dtype=torch.int32, device=n.device)
         grads = None
@@ -511,46 +552,46 @@ class Network:
         """reaction fluxes k prod(n) [cm^-3 s^-1], [..., nrea]"""
         return self._query(n, Tgas, par, Phi, opts, "krome_get_fluxes")
 
+    def _query_inputs(self, n, Tgas, par, Phi, opts):
+        """(batch, Tgas column, parameter rows, stride, fields, opts
+        handle) of a query; no opts: no photon bins"""
+        b = _Batch(self, n)
+        P, stride = self._param_block(par, b.ncell)
+        F = None if opts is None else self._phi(Phi, b.ncell, opts)
+        return (b, b.column(Tgas, "Tgas"), P, stride, F,
+                None if opts is None else opts._h)
+
     def _query(self, n, Tgas, par, Phi, opts, fname):
-        single, n, ncell = self._cells(n)
-        T = self._per_cell(Tgas, ncell)
-        P, stride = self._params(par, ncell)
-        h = opts._h if opts is not None else None
-        F = self._phi(Phi, ncell, opts) if opts is not None else None
-        out = np.zeros((ncell, self.nrea))
-        r = getattr(self._lib, fname)(ncell, _dptr(n), _dptr(T), _dptr(P),
-                                      stride, _dptr(F) if F is not None
-                                      else None, h, _dptr(out))
+        b, T, P, stride, F, h = self._query_inputs(n, Tgas, par, Phi, opts)
+        out = np.zeros((b.ncell, self.nrea))
+        r = getattr(self._lib, fname)(b.ncell, _dptr(b.rows), _dptr(T),
+                                      _dptr(P), stride, _dptr(F), h,
+                                      _dptr(out))
         if r != 0:
-            raise KromeError("%s returned %d: %s" % (fname, r,
-                                                     self.flag_message(r)))
-        return out[0] if single else out
+            raise KromeError("%s: %s (flag %d)" % (fname,
+                                                   self.flag_message(r), r))
+        return b.take(out)
 
     def thermal(self, n, Tgas, par=None, Phi=None, opts=None):
         """the thermal balance: dict of heat, cool [erg cm^-3 s^-1], the
         terms heats [..., nheat] and cools [..., ncool] (labels:
         heating_terms, cooling_terms) and dTdt [K/s]; the photoheating of
         the fields Phi over opts' photon bins"""
-        single, n, ncell = self._cells(n)
-        T = self._per_cell(Tgas, ncell)
-        P, stride = self._params(par, ncell)
-        h = opts._h if opts is not None else None
-        F = self._phi(Phi, ncell, opts) if opts is not None else None
-        heat, cool, dTdt = (np.zeros(ncell) for _ in range(3))
-        hs = np.zeros((ncell, max(len(self.heating_terms), 1)))
-        cs = np.zeros((ncell, max(len(self.cooling_terms), 1)))
-        r = self._lib.krome_get_thermal(ncell, _dptr(n), _dptr(T), _dptr(P),
-                                        stride, _dptr(F) if F is not None
-                                        else None, h, _dptr(heat),
-                                        _dptr(cool), _dptr(hs), _dptr(cs),
-                                        _dptr(dTdt))
+        b, T, P, stride, F, h = self._query_inputs(n, Tgas, par, Phi, opts)
+        heat, cool, dTdt = (np.zeros(b.ncell) for _ in range(3))
+        hs = np.zeros((b.ncell, max(len(self.heating_terms), 1)))
+        cs = np.zeros((b.ncell, max(len(self.cooling_terms), 1)))
+        r = self._lib.krome_get_thermal(b.ncell, _dptr(b.rows), _dptr(T),
+                                        _dptr(P), stride, _dptr(F), h,
+                                        _dptr(heat), _dptr(cool), _dptr(hs),
+                                        _dptr(cs), _dptr(dTdt))
         if r != 0:
-            raise KromeError("krome_get_thermal returned %d: %s"
-                             % (r, self.flag_message(r)))
+            raise KromeError("krome_get_thermal: %s (flag %d)"
+                             % (self.flag_message(r), r))
         out = {"heat": heat, "cool": cool, "dTdt": dTdt,
                "heats": hs[:, :len(self.heating_terms)],
                "cools": cs[:, :len(self.cooling_terms)]}
-        return {k: v[0] for k, v in out.items()} if single else out
+        return {k: b.take(v) for k, v in out.items()}
 
     # -- element totals and consistent advection (the reference binding's
     #    krome.py:850-921) ------------------------------------------------
@@ -565,13 +606,13 @@ class Network:
         return self._totals(x, "krome_get_element_totals_x")
 
     def _totals(self, v, fname):
-        single, v, ncell = self._cells(v)
-        out = np.zeros((ncell, max(len(self.elements), 1)))
-        r = getattr(self._lib, fname)(ncell, _dptr(v), _dptr(out))
+        b = _Batch(self, v)
+        nel = len(self.elements)
+        out = np.zeros((b.ncell, max(nel, 1)))
+        r = getattr(self._lib, fname)(b.ncell, _dptr(b.rows), _dptr(out))
         if r != 0:
-            raise KromeError("%s returned %d" % (fname, r))
-        out = out[:, :len(self.elements)]
-        return out[0] if single else out
+            raise KromeError("%s: flag %d" % (fname, r))
+        return b.take(out[:, :nel])
 
     def conserve(self, n, ref, par=None, flags=None, check=True):
         """consistent advection (legacy krome_conserveLin_x): the cells n
@@ -595,41 +636,27 @@ class Network:
                               "krome_conserve_cells_x")
 
     def _conserve(self, v, ref, par, flags, check, fname):
-        single, v, ncell = self._cells(v)
+        b = _Batch(self, v)
         nel = len(self.elements)
-        R = np.ascontiguousarray(ref, dtype=float)
-        if R.shape == (nel,):
-            rs = 0
-        elif R.shape == (ncell, nel):
-            rs = nel
-        else:
-            raise KromeError("ref must have shape (%d,) or (%d, %d), got %s"
-                             % (nel, ncell, nel, R.shape))
-        # the parameters only select freeze directives: no REQUIRED check
-        P, ps = None, 0
-        if isinstance(par, dict):
-            P, ps = self.param_rows(ncell, **par), self.npar
-        elif par is not None:
-            P = np.ascontiguousarray(par, dtype=float)
-            if P.shape == (ncell, self.npar):
-                ps = self.npar
-            elif P.shape != (self.npar,):
-                raise KromeError("par must have shape (%d,) or (%d, %d), got "
-                                 "%s" % (self.npar, ncell, self.npar,
-                                         P.shape))
-        fl = np.zeros(ncell, dtype=np.intc)
-        r = getattr(self._lib, fname)(ncell, _dptr(v), _dptr(R), rs,
-                                      _dptr(P), ps, fl.ctypes.data_as(_I))
+        R = np.ascontiguousarray(ref, dtype=np.float64)
+        shapes = {(nel,): 0, (b.ncell, nel): nel}
+        if R.shape not in shapes:
+            raise KromeError("ref: expected [%d] or [%d, %d], got shape %s"
+                             % (nel, b.ncell, nel, R.shape))
+        # the parameters only select freeze directives: REQUIRED ones may
+        # stay unset here
+        P, ps = self._param_block(par, b.ncell, require=False)
+        fl = np.zeros(b.ncell, dtype=np.intc)
+        r = getattr(self._lib, fname)(b.ncell, _dptr(b.rows), _dptr(R),
+                                      shapes[R.shape], _dptr(P), ps,
+                                      fl.ctypes.data_as(_I))
         if r == -1:
-            raise KromeError("%s: bad arguments" % fname)
+            raise KromeError("%s rejected its arguments" % fname)
         if flags is not None:
             flags[...] = fl.reshape(np.shape(flags))
         if check and r != 0:
-            raise KromeError("%s: %s (cells %s); pass check=False to get "
-                             "the result anyway" % (
-                                 fname, self.flag_message(r),
-                                 np.flatnonzero(fl < 0).tolist()))
-        return v[0] if single else v
+            self._raise_failed(fname, r, np.where(fl < 0, fl, 0))
+        return b.take(b.rows)
 
     def conserve_device(self, n, ref, flags=None, x=False, layout=None):
         """consistent advection on torch tensors on the GPU, in place: n
@@ -656,31 +683,13 @@ class Network:
 
     # -- inputs (the krome_solve_cells conventions) -----------------------
 
-    def _cells(self, n):
-        n = np.array(n, dtype=float, order="C", copy=True)
-        if n.ndim == 0 or n.ndim > 2 or n.shape[-1] != self.nspec:
-            raise KromeError("n must have shape (%d,) or (ncell, %d), got %s"
-                             % (self.nspec, self.nspec, n.shape))
-        single = n.ndim == 1
-        n = n.reshape(-1, self.nspec)
-        return single, n, n.shape[0]
-
-    @staticmethod
-    def _per_cell(v, ncell):
-        return np.array(np.broadcast_to(np.asarray(v, dtype=float),
-                                        (ncell,)), order="C", copy=True)
-
-    def _options(self, opts):
-        if opts is not None:
-            return opts
-        if self._opts is None:
-            self._opts = Options(self)
-        return self._opts
-
-    @staticmethod
-    def _no_dust(Td, adust):
-        if Td is not None or adust is not None:
-            raise KromeError("Td / adust: this build has no dust bins")
+    def _opts_or_default(self, opts):
+        """opts, or the network's default handle (made on first use)"""
+        if opts is None:
+            if self._opts is None:
+                self._opts = Options(self)
+            opts = self._opts
+        return opts
 
     def _phi(self, Phi, ncell, opts):
         """the fields as [ncell, nbins] rows (None when no bins are set)"""
@@ -732,37 +741,45 @@ class Network:
             raise KromeError("invalid binning")
         return lo, hi
 
-    def _params(self, par, ncell):
-        stride, P = 0, None
-        if isinstance(par, dict):
-            P = self.param_rows(ncell, **par)
-            stride = self.npar
-        elif par is not None:
-            P = np.ascontiguousarray(par, dtype=float)
-            if P.shape == (self.npar,):
-                stride = 0
-            elif P.shape == (ncell, self.npar):
-                stride = self.npar
-            else:
-                raise KromeError("par must have shape (%d,) or (%d, %d), "
-                                 "got %s" % (self.npar, ncell, self.npar,
-                                             P.shape))
-        rows = (self.params_default()[None, :] if P is None
-                else P.reshape(-1, self.npar))
-        unset = [s for s in self.required
-                 if np.isnan(rows[:, self._pidx[s]]).any()]
-        if unset:
-            raise KromeError("required parameter(s) not set: %s" % "; ".join(
-                "%s (for %s)" % (s, self.required[s]) for s in unset))
+    def _param_block(self, par, ncell, require=True):
+        """(rows, stride) for a C call: par None (every parameter at its
+        default: rows None, stride 0), a dict {name: value or [ncell]
+        values} over the defaults, one row [npar] shared by every cell
+        (stride 0) or rows [ncell, npar] (stride npar).  With require, a
+        REQUIRED parameter left NaN in any row is an error here, before
+        the library sees it"""
+        if par is None:
+            P, stride = None, 0
+        elif isinstance(par, dict):
+            P, stride = self.param_rows(ncell, **par), self.npar
+        else:
+            P = np.ascontiguousarray(par, dtype=np.float64)
+            stride = {(self.npar,): 0,
+                      (ncell, self.npar): self.npar}.get(P.shape)
+            if stride is None:
+                raise KromeError("par: expected [%d] or [%d, %d], got shape "
+                                 "%s" % (self.npar, ncell, self.npar,
+                                         P.shape))
+        if require and self.required:
+            block = np.atleast_2d(self.params_default() if P is None else P)
+            unset = [k for k in self.required
+                     if np.isnan(block[:, self._pidx[k]]).any()]
+            if unset:
+                raise KromeError(
+                    "REQUIRED parameter%s still NaN: %s" % (
+                        "s" if len(unset) > 1 else "",
+                        ", ".join("%s (read by %s)" % (k, self.required[k])
+                                  for k in unset)))
         return P, stride
 
-    def _check(self, check, ierr, flags, what):
-        if check and ierr != 0:
-            bad = np.flatnonzero(np.atleast_1d(flags))
-            raise KromeError("%s returned %d: %s (failed cells: %s); pass "
-                             "check=False to get the result anyway" % (
-                                 what, ierr, self.flag_message(ierr),
-                                 bad.tolist() or "n/a"))
+    def _raise_failed(self, what, ierr, flags):
+        bad = np.flatnonzero(np.atleast_1d(flags))
+        shown = bad[:16].tolist()
+        more = " and %d more" % (bad.size - 16) if bad.size > 16 else ""
+        raise KromeError("%s: %s (flag %d) in %d cell(s): %s%s; check=False "
+                         "returns the result with its per-cell flags" % (
+                             what, self.flag_message(ierr), ierr, bad.size,
+                             shown, more))
 
 
 class Options:
@@ -775,26 +792,69 @@ class Options:
     queue_chunk, waves_per_cu."""
 
     def __init__(self, net, **values):
-        self._net = net
-        self._h = net._lib.krome_opts_new()
-        for k, v in values.items():
-            self[k] = v
+        # the handle keeps a reference to its network: the library stays
+        # loaded while the handle lives
+        self._net, self._lib = net, net._lib
+        self._h = self._lib.krome_opts_new()
+        self.update(**values)
 
-    def __setitem__(self, key, value):
+    def update(self, **values):
+        """set several keys; returns self"""
+        for key, value in values.items():
+            self._set(key, value)
+        return self
+
+    def _set(self, key, value):
         if key not in _KEYS:
-            raise KromeError("unknown option %r" % key)
-        if key in _ENUMS and isinstance(value, str):
-            value = _ENUMS[key].index(value)
-        if self._net._lib.krome_opts_set(self._h, key.encode(),
-                                         float(value)) != 0:
-            raise KromeError("invalid value %r for option %s" % (value, key))
+            raise KromeError("%r is not an option key (keys: %s)"
+                             % (key, ", ".join(_KEYS)))
+        names = _ENUMS.get(key)
+        if names is not None and isinstance(value, str):
+            if value not in names:
+                raise KromeError("option %s takes %s, not %r"
+                                 % (key, " / ".join(names), value))
+            value = names.index(value)
+        if self._lib.krome_opts_set(self._h, key.encode(), float(value)):
+            raise KromeError("option %s: the library refused %r"
+                             % (key, value))
+
+    __setitem__ = _set
 
     def __getitem__(self, key):
         v = ctypes.c_double()
-        if self._net._lib.krome_opts_get(self._h, key.encode(),
+        if self._lib.krome_opts_get(self._h, key.encode(),
                                          ctypes.byref(v)) != 0:
             raise KromeError("unknown option %r" % key)
         return v.value
+
+    def set_species_tolerances(self, atol=None, rtol=None):
+        """per-species tolerances (krome_opts_set_atol_species /
+        rtol_species): atol[nspec] [cm^-3] and rtol[nspec] in place of
+        atol_n and rtol, a negative entry keeping the scalar; None leaves
+        that kind as it is, "scalar" clears it"""
+        ns = self._net.nspec
+        for what, v in (("atol", atol), ("rtol", rtol)):
+            if v is None:
+                continue
+            setter = getattr(self._lib, "krome_opts_set_%s_species" % what)
+            if isinstance(v, str) and v == "scalar":
+                setter(self._h, None)
+                continue
+            a = np.ascontiguousarray(v, dtype=np.float64)
+            if a.shape != (ns,):
+                raise KromeError("%s: expected %d values, got shape %s"
+                                 % (what, ns, a.shape))
+            if setter(self._h, _dptr(a)) != 0:
+                raise KromeError("%s: NaN%s refused" % (
+                    what, " or zero" if what == "atol" else ""))
+        return self
+
+    def species_tolerances(self):
+        """(atol[nspec], rtol[nspec]): the tolerance each species gets"""
+        a, r = np.zeros(self._net.nspec), np.zeros(self._net.nspec)
+        self._lib.krome_opts_get_atol_species(self._h, _dptr(a))
+        self._lib.krome_opts_get_rtol_species(self._h, _dptr(r))
+        return a, r
 
     def set_photo(self, Elo, Ehi):
         """photon bins [eV edges]; Phi[..., nbins] is then required by
@@ -803,7 +863,7 @@ class Options:
         hi = np.ascontiguousarray(Ehi, dtype=np.float64)
         if lo.shape != hi.shape or lo.ndim != 1:
             raise KromeError("Elo and Ehi must be 1-D arrays of one shape")
-        if self._net._lib.krome_opts_set_photo(self._h, lo.size, _dptr(lo),
+        if self._lib.krome_opts_set_photo(self._h, lo.size, _dptr(lo),
                                                _dptr(hi)) != 0:
             raise KromeError("invalid photon bins (need 0 < Elo < Ehi)")
         return self
@@ -811,25 +871,25 @@ class Options:
     def set_photo_alpha(self, alpha):
         """the in-bin photon spectrum dN/dE ~ E^-alpha (1: the same number
         per ln E); rebuilds the bins already set"""
-        if self._net._lib.krome_opts_set_photo_alpha(self._h,
+        if self._lib.krome_opts_set_photo_alpha(self._h,
                                                      float(alpha)) != 0:
             raise KromeError("alpha must be finite")
         return self
 
     @property
     def nbins(self):
-        return self._net._lib.krome_opts_nbins(self._h)
+        return self._lib.krome_opts_nbins(self._h)
 
     def photo_sigma(self):
         """the bin-averaged cross sections [nphoto, nbins] [cm^2]"""
         s = np.zeros((max(self._net.nphoto, 1), self.nbins))
-        if self._net._lib.krome_opts_photo_sigma(self._h, _dptr(s)) != 0:
+        if self._lib.krome_opts_photo_sigma(self._h, _dptr(s)) != 0:
             raise KromeError("no photon bins")
         return s[:self._net.nphoto]
 
     def close(self):
         if self._h is not None:
-            self._net._lib.krome_opts_free(self._h)
+            self._lib.krome_opts_free(self._h)
             self._h = None
 
     def __del__(self):
@@ -840,18 +900,31 @@ class Options:
 
 
 class Result:
-    """solve() output: n, Tgas, Phi (the fields given, None without photon
-    bins; held fixed during the solve), per-cell flags (0 = success), ierr;
-    Td and adust are None (no dust bins here)"""
+    """what solve() returns, in the reference binding's field order: n,
+    Tgas, Phi (the fields passed, held fixed during the solve; None without
+    photon bins), flags (per cell, 0 = success), ierr (the flag of the
+    first failing cell, 0 when all succeed), Td and adust (None: this build
+    has no dust bins)"""
 
-    def __init__(self, n, Tgas, flags, ierr, Phi=None):
-        self.n, self.Tgas, self.flags, self.ierr = n, Tgas, flags, ierr
-        self.Phi, self.Td, self.adust = Phi, None, None
+    def __init__(self, n, Tgas, Phi, flags, ierr, Td=None, adust=None):
+        self.n = n
+        self.Tgas = Tgas
+        self.Phi = Phi
+        self.flags = flags
+        self.ierr = int(ierr)
+        self.Td = Td
+        self.adust = adust
 
     @property
     def ok(self):
-        return self.ierr == 0
+        """True when every cell succeeded"""
+        return not self.ierr
+
+    def failed(self):
+        """indices of the cells with a nonzero flag"""
+        return np.flatnonzero(np.atleast_1d(self.flags) != 0)
 
     def __repr__(self):
-        return "<krome.Result ierr=%d, %d cell(s)>" % (
-            self.ierr, np.atleast_1d(self.flags).size)
+        ncell = np.atleast_1d(self.flags).size
+        return "Result(%d cell%s, ierr %d, %d failed)" % (
+            ncell, "" if ncell == 1 else "s", self.ierr, self.failed().size)

@@ -107,6 +107,10 @@ class Species:
     comp: dict
     charge: int
     mass_amu: float
+    # a name that is not an element composition (GRAIN0, a generic
+    # collider): no atoms, zero mass before its charge's electrons
+    # (docs/network-format.md:60-66)
+    opaque: bool = False
 
     @property
     def cid(self):
@@ -124,25 +128,35 @@ def parse_species(tok):
     m = _SPECIES_RE.match(tok)
     if not m or bool(m.group("curly")) != bool(m.group("curly2")):
         raise KnetError("malformed species %r" % tok)
-    core, comp, i = m.group("core"), {}, 0
-    while i < len(core):
-        for el in _ELEMENTS_BY_LEN:
-            if core.startswith(el, i):
-                i += len(el)
-                digits = ""
-                while i < len(core) and core[i].isdigit():
-                    digits += core[i]
-                    i += 1
-                comp[el] = comp.get(el, 0) + int(digits or 1)
-                break
-        else:
-            raise KnetError("species %r is not an element composition" % tok)
+    core = m.group("core")
+    comp = _composition(core)
     ch = m.group("charge") or ""
     charge = len(ch) * (1 if ch.startswith("+") else -1) if ch else 0
+    opaque = comp is None
+    if opaque:
+        comp = {}
     mass = sum(MASS_AMU[e] * c for e, c in comp.items())
     name = core + (m.group("labels") or "") + ch
-    return (Species(name, comp, charge, mass - charge * ELECTRON_MASS_AMU),
-            not m.group("curly"))
+    return (Species(name, comp, charge, mass - charge * ELECTRON_MASS_AMU,
+                    opaque), not m.group("curly"))
+
+
+def _composition(core):
+    """{element: atoms} of a species core read greedily, the longest
+    element symbol first; None when some part is not an element symbol"""
+    comp, i = {}, 0
+    while i < len(core):
+        el = next((e for e in _ELEMENTS_BY_LEN if core.startswith(e, i)),
+                  None)
+        if el is None:
+            return None
+        i += len(el)
+        j = i
+        while j < len(core) and core[j].isdigit():
+            j += 1
+        comp[el] = comp.get(el, 0) + int(core[i:j] or 1)
+        i = j
+    return comp
 
 
 @dataclass
@@ -547,6 +561,37 @@ def leiden_xsec(data_dir, reactant, products):
     return [e for e, _ in pts], [x for _, x in pts]
 
 
+def _swri_last_table(text):
+    """(column labels, numeric rows) of the last table of an SWRI
+    (phidrates) cross-section file: a table starts at its "Lambda" label
+    line; lines that are not all numbers are skipped"""
+    tables = []
+    for line in text.splitlines():
+        toks = line.split()
+        if toks and toks[0] == "Lambda":
+            tables.append((toks, []))
+        elif toks and tables:
+            try:
+                tables[-1][1].append([float(x) for x in toks])
+            except ValueError:
+                pass
+    return tables[-1] if tables else ([], [])
+
+
+def _swri_products(label):
+    """the products an SWRI branch label names: '/' separates them, a
+    species ending in '+' brings its photoelectron, 'E' is an electron"""
+    out = []
+    for tok in re.findall(r"[^/+]*\+|[^/+]+", label):
+        if tok.upper() == "E":
+            out.append("e-")
+        else:
+            out.append(tok)
+            if tok.endswith("+"):
+                out.append("e-")
+    return out
+
+
 def swri_xsec(data_dir, reactant, products):
     """(E [eV] ascending, sigma [cm^2]) of the SWRI (phidrates) branch of
     <reactant>.dat whose products match: excited-state labels folded onto
@@ -554,39 +599,33 @@ def swri_xsec(data_dir, reactant, products):
     1e-35 sentinel on both sides is a spectral line, kept as a box of its
     mean height between the bracketing energies; other sentinels are
     dropped; the table spans its first to last nonzero value"""
-    prods = sorted(products)
+    want = sorted(products)
     path = Path(data_dir) / "database" / "swri_xsecs" / ("%s.dat" % reactant)
     if not path.exists():
         raise KnetError("no SWRI file for %s" % reactant)
-    header, rows = [], []
-    for line in path.read_text().splitlines():
-        toks = line.split()
-        if not toks:
-            continue
-        if toks[0] == "Lambda":
-            header, rows = toks, []
-            continue
-        try:
-            rows.append([float(x) for x in toks])
-        except ValueError:
-            pass
-    for state in ("1D", "1S", "3P"):
-        header = [h.replace(state, "") for h in header]
-    merged = set()
-    for i in range(1, len(header)):
-        for j in range(i + 1, len(header)):
-            if header[j] == header[i] and j not in merged:
-                merged.add(j)
-                for row in rows:
-                    row[i] += row[j]
-    branches = [sorted("e-" if x.upper() == "E" else x
-                       for x in h.replace("+", "+/E/").split("/") if x)
-                for h in header]
-    if prods not in branches:
-        raise KnetError("no SWRI branch %s in %s" % ("+".join(prods),
+    label, rows = _swri_last_table(path.read_text())
+    # one branch per distinct label once the excited states are folded
+    # onto the ground state; its column is the sum of its labels' columns,
+    # accumulated left to right
+    groups = {}
+    for j, h in enumerate(label[1:], 1):
+        groups.setdefault(re.sub("1D|1S|3P", "", h), []).append(j)
+    cols = {}
+    for h, js in groups.items():
+        prods = sorted(_swri_products(h))
+        if prods not in [c[0] for c in cols.values()]:
+            cols[h] = (prods, js)
+    match = [js for prods, js in cols.values() if prods == want]
+    if not match:
+        raise KnetError("no SWRI branch %s in %s" % ("+".join(want),
                                                      path.name))
-    col = branches.index(prods)
-    pts = [(HC_EV_AA / r[0], r[col]) for r in rows[::-1] if r[0] > 0.0]
+
+    def value(r, js=match[0]):
+        v = r[js[0]]
+        for j in js[1:]:
+            v += r[j]
+        return v
+    pts = [(HC_EV_AA / r[0], value(r)) for r in reversed(rows) if r[0] > 0.0]
     sentinel = [v == 1e-35 for _, v in pts]
     pairs, q = [], 0
     while q < len(pts):

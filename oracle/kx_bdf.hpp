@@ -42,6 +42,8 @@ struct Sparse {
 struct Opts {
   double rtol = 1e-4;
   const double* atol = nullptr;
+  // per-equation relative tolerances (KROME's rtol_species), or nullptr
+  const double* rtol_v = nullptr;
   long max_steps = 500;
   int max_err_fails = 7;
   int max_conv_fails = 10;
@@ -363,8 +365,17 @@ class Bdf {
   long nscon = 0, nstlp = 0, nstlj = 0;
   bool acnrmcur = false, jcur = false, nls_jcur = false;
 
-  // cvEwtSetSV (bdf_cell.hpp:689-713)
+  // cvEwtSetSV, or with rtol_v KROME's per-slot weights
+  // 1 / fma(rtol_i, |y_i|, atol_i) (bdf_cell.hpp:689-713)
   int ewt_set(const double* yc, double* w) {
+    if (o.rtol_v != nullptr) {
+      for (int i = 0; i < N; i++) {
+        const double d = std::fma(o.rtol_v[i], std::fabs(yc[i]), o.atol[i]);
+        if (!(d > 0.0)) return -1;
+        w[i] = 1.0 / d;
+      }
+      return 0;
+    }
     for (int i = 0; i < N; i++) w[i] = std::fabs(yc[i]);
     for (int i = 0; i < N; i++) w[i] = (o.rtol * w[i]) + o.atol[i];
     if (atolmin0) {

@@ -125,9 +125,18 @@ struct Problem {
   }
 };
 
+// per-equation tolerances of a solve (krome_opts_set_atol_species /
+// rtol_species resolved per species: solve_cell.hpp:866-879, 2129-2139):
+// atol[NSPEC] and rtol[NSPEC] effective values, each nullptr for the
+// scalar
+struct SpeciesTol {
+  const double* atol = nullptr;
+  const double* rtol = nullptr;
+};
+
 template <class Net>
 int solve_one(double* n, double& T, const double* row, double dt,
-              const double* o, long* stats) {
+              const double* o, long* stats, const SpeciesTol& st = {}) {
   constexpr int NS = Net::NSPEC;
   if (Net::params_missing(row) >= 0) return FLAG_PARAM_UNSET;
   const bool evolve_T = o[O_EVOLVE_T] != 0.0;
@@ -140,20 +149,25 @@ int solve_one(double* n, double& T, const double* row, double dt,
   P.evolve_T = evolve_T;
   P.Tfix = T;
   const int N = NS + (evolve_T ? 1 : 0);
-  double y[NS + 1], atol[NS + 1];
+  double y[NS + 1], atol[NS + 1], rtolv[NS + 1];
+  bool rtol_varies = false;
   for (int i = 0; i < NS; ++i) {
     y[i] = n[i];
-    atol[i] = o[O_ATOL_N];
+    atol[i] = st.atol ? st.atol[i] : o[O_ATOL_N];
+    rtolv[i] = st.rtol ? st.rtol[i] : o[O_RTOL];
+    rtol_varies |= rtolv[i] != o[O_RTOL];
   }
   if (evolve_T) {
     y[NS] = T;
     atol[NS] = o[O_ATOL_T];
+    rtolv[NS] = o[O_RTOL];
   }
   int charge[NS];
   for (int i = 0; i < NS; ++i) charge[i] = Net::charge(i);
   bdf_::Opts bo;
   bo.rtol = o[O_RTOL];
   bo.atol = atol;
+  bo.rtol_v = rtol_varies ? rtolv : nullptr;
   bo.max_steps = static_cast<long>(o[O_MAX_STEPS]);
   bo.max_err_fails = static_cast<int>(o[O_MAX_ERR_FAILS]);
   bo.max_conv_fails = static_cast<int>(o[O_MAX_CONV_FAILS]);
@@ -233,7 +247,7 @@ int solve_one(double* n, double& T, const double* row, double dt,
 template <class Net>
 int solve_cells(int ncell, double* n, double* T, const double* dt,
                 const double* par, int par_stride, const double* o,
-                int* flags, long* stats) {
+                int* flags, long* stats, const SpeciesTol& st = {}) {
   constexpr int NS = Net::NSPEC;
   std::vector<double> dflt(Net::NPAR, 0.0);
   bool missing = false;
@@ -251,7 +265,7 @@ int solve_cells(int ncell, double* n, double* T, const double* dt,
     if (dt[c] > 0.0) {
       fl[c] = solve_one<Net>(n + static_cast<size_t>(c) * NS, T[c],
                              par + static_cast<size_t>(c) * par_stride,
-                             dt[c], o, sc);
+                             dt[c], o, sc, st);
     } else if (sc) {
       for (int i = 0; i < 9; ++i) sc[i] = 0;
     }
@@ -420,6 +434,22 @@ int KXO_(solve_cells)(int net, int ncell, double* n, double* T,
   return with_net(net, [&](auto tag) {
     return solve_cells<decltype(tag)>(ncell, n, T, dt, par, par_stride,
                                       opts, flags, stats);
+  });
+}
+
+// kxo_solve_cells with per-species tolerances: atol[NSPEC], rtol[NSPEC]
+// the values each species gets (nullptr: the scalar for all)
+int KXO_(solve_cells_tol)(int net, int ncell, double* n, double* T,
+                          const double* dt, const double* par,
+                          int par_stride, const double* opts,
+                          const double* atol, const double* rtol, int* flags,
+                          long* stats) {
+  SpeciesTol st;
+  st.atol = atol;
+  st.rtol = rtol;
+  return with_net(net, [&](auto tag) {
+    return solve_cells<decltype(tag)>(ncell, n, T, dt, par, par_stride,
+                                      opts, flags, stats, st);
   });
 }
 

@@ -45,7 +45,10 @@ class Oracle:
         self.lib = ctypes.CDLL(str(path))
         self.p = prefix
 
-    def solve(self, net, n, T, dt, par, opts):
+    def solve(self, net, n, T, dt, par, opts, tol=None):
+        """(return value, n, T, flags, stats) of solve_cells; tol: the
+        per-species (atol[NSPEC], rtol[NSPEC]) each species gets (an entry
+        None: the scalar for every species)"""
         D = ctypes.POINTER(ctypes.c_double)
         n = np.array(n, dtype=np.float64, order="C", copy=True)
         T = np.array(T, dtype=np.float64, order="C", copy=True)
@@ -54,13 +57,20 @@ class Oracle:
         nc = n.shape[0]
         flags = np.zeros(nc, dtype=np.intc)
         st = np.zeros((nc, 9), dtype=np.int64)
-        f = getattr(self.lib, self.p + "solve_cells")
-        r = f(NET_ID[net], nc, n.ctypes.data_as(D), T.ctypes.data_as(D),
-              dt.ctypes.data_as(D), par.ctypes.data_as(D),
-              par.shape[1] if par.ndim == 2 else 0,
-              np.ascontiguousarray(opts).ctypes.data_as(D),
-              flags.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-              st.ctypes.data_as(ctypes.POINTER(ctypes.c_long)))
+        head = (NET_ID[net], nc, n.ctypes.data_as(D), T.ctypes.data_as(D),
+                dt.ctypes.data_as(D), par.ctypes.data_as(D),
+                par.shape[1] if par.ndim == 2 else 0,
+                np.ascontiguousarray(opts).ctypes.data_as(D))
+        tail = (flags.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                st.ctypes.data_as(ctypes.POINTER(ctypes.c_long)))
+        if tol is None:
+            r = getattr(self.lib, self.p + "solve_cells")(*head, *tail)
+        else:
+            keep = [None if t is None else
+                    np.ascontiguousarray(t, dtype=np.float64) for t in tol]
+            r = getattr(self.lib, self.p + "solve_cells_tol")(
+                *head, *(None if t is None else t.ctypes.data_as(D)
+                         for t in keep), *tail)
         return r, n, T, flags, st
 
     def funcs(self, net, n, T, par, evolve_T):

@@ -30,8 +30,13 @@ For each workload it:
      rtol 1e-10) -> primordial3_adjoint.npz (the outputs log(n_H2/nH) and
      log(Tgas) of every solve).
 
+  8. per-species tolerances (SolverOpts::atol_species / rtol_species,
+     solve_cell.hpp:866-879, 2129-2139): primordial and primordial3 solved
+     with per-species atol only, then with per-species atol and rtol (some
+     entries negative: the scalar) -> <name>_tol.npz.
+
   python3 tests/golden/make_golden.py [solve] [funcs] [conserve] [sparse]
-  [photo] [extra] [adjoint] (default: all but extra)
+  [photo] [extra] [adjoint] [tol] (default: all but extra)
 
 The generated project and the binaries stay in tests/golden/_build/ (git-
 ignored, not committed).
@@ -307,6 +312,52 @@ def make_sparse(name, exe):
         name, nnz, raw[1], np.unique(f), st[:, 0].sum()))
 
 
+# per-species tolerances (krome_opts_set_atol_species / rtol_species):
+# each network's tolerance rows, negative entries taking the scalar
+NTOL = 256
+
+
+def species_tols(meta, scalar_rtol):
+    """(atol[NSPEC], rtol[NSPEC]) of the per-species fixtures: tighter
+    absolute tolerances on the trace ions, a looser one on He, relative
+    tolerances that differ by species, and some entries negative (the
+    scalar)"""
+    sp = meta["species"]
+    atol = np.full(len(sp), -1.0)
+    rtol = np.full(len(sp), -1.0)
+    for i, name in enumerate(sp):
+        if name in ("H-", "H2+", "He++", "D+"):
+            atol[i] = 1e-26
+        elif name == "He":
+            atol[i] = 1e-14
+        elif name == "H2":
+            atol[i] = 1e-22
+            rtol[i] = scalar_rtol * 0.1
+        elif name == "e-":
+            rtol[i] = scalar_rtol * 0.5
+        elif name == "H+":
+            rtol[i] = scalar_rtol * 4.0
+    return atol, rtol
+
+
+def make_tol(name, exe, meta):
+    w = cells.WORKLOADS[WORKLOAD[name]](NTOL)
+    base = [float(w["opts"][k]) for k in OPT_KEYS] + [0.0]
+    atol, rtol = species_tols(meta, base[OPT_KEYS.index("rtol")])
+    out = {"n0": w["n"], "T0": w["T"], "dt": w["dt"], "par": w["par"],
+           "atol_species": atol, "rtol_species": rtol}
+    none = np.full(len(atol), -1.0)
+    for tag, a, r in (("atol", atol, none), ("both", atol, rtol)):
+        opts = np.array(base + list(a) + list(r))
+        n, T, f, st, secs = run_solve(exe, w, "%s_tol_%s" % (name, tag),
+                                      opts)
+        out.update({"opts_" + tag: opts, "n_" + tag: n, "T_" + tag: T,
+                    "flag_" + tag: f, "stats_" + tag: st})
+        print("%s tol %s: flags %s, steps %d" % (name, tag, np.unique(f),
+                                                 st[:, 0].sum()))
+    np.savez_compressed(HERE / ("%s_tol.npz" % name), **out)
+
+
 # photon bins: cells of the photo solve fixture, the energies at which the
 # cross sections are sampled, the in-bin spectra of the bin tables
 NPHOTO_SOLVE = 256
@@ -541,7 +592,7 @@ def main(argv=None):
     only = {a for a in argv if a in names}
     argv = [a for a in argv if a not in names]
     parts = set(argv if argv else ("solve", "funcs", "conserve", "sparse",
-                                   "photo", "adjoint"))
+                                   "photo", "adjoint", "tol"))
     BUILD.mkdir(parents=True, exist_ok=True)
     if "adjoint" in parts and (not only or "primordial3" in only):
         make_adjoint()
@@ -564,7 +615,7 @@ def main(argv=None):
         proj = generate(name)
         make_extra(name, compile_driver(proj, name, "bitwise"))
     for name in ("primordial", "primordial3", "primordialZ"):
-        if not parts & {"solve", "funcs", "conserve", "sparse"}:
+        if not parts & {"solve", "funcs", "conserve", "sparse", "tol"}:
             break
         if only and name not in only:
             continue
@@ -580,6 +631,8 @@ def main(argv=None):
                                 **ref_tables(exes["bitwise"], name, w))
         if "sparse" in parts:
             make_sparse(name, exes["bitwise"])
+        if "tol" in parts and name != "primordialZ":
+            make_tol(name, exes["bitwise"], meta)
         if not parts & {"solve", "funcs"}:
             continue
         w = cells.WORKLOADS[WORKLOAD[name]](NSOLVE[name])

@@ -35,7 +35,8 @@
 //                               [NSPEC], sp_rp[NSPEC + 1], sp_ci[NNZ],
 //                               sp_di[NSPEC] (f64)
 //   (solve: opts[4] 1 dense, 3 sparse, else auto; opts[7], when given,
-//   is SolverOpts::conserve)
+//   is SolverOpts::conserve; with 8 + 2 NSPEC options, opts[8..] are the
+//   per-species atol then rtol, SolverOpts::atol_species / rtol_species)
 //   ref_driver pbins|prates|psolve IN OUT   photon bins (networks with
 //                               photo reactions): IN as above followed by
 //                               i64 B; f64 alpha, Elo[B], Ehi[B],
@@ -105,6 +106,12 @@ krome::SolverOpts solver_opts(const In& in) {
                    : (jc == 2 ? krome::JAC_FD : krome::JAC_AUTO);
   so.max_steps = static_cast<long>(in.opts[6]);
   if (in.nopt > 7) so.conserve = static_cast<int>(in.opts[7]);
+  // opts[8 .. 8 + NSPEC) per-species atol, then NSPEC per-species rtol
+  // (SolverOpts::atol_species / rtol_species; negative: the scalar)
+  if (in.nopt >= 8 + 2 * krome::NSPEC) {
+    so.atol_species = in.opts.data() + 8;
+    so.rtol_species = in.opts.data() + 8 + krome::NSPEC;
+  }
   so.error_report = false;
   return so;
 }
