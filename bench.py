@@ -351,6 +351,101 @@ def multi_bench(a):
     print(json.dumps(out))
 
 
+def c3_bench(a):
+    """the uneven batch: 2^19 primordial + 2^18 primordialZ + 2^17 cloud
+    cells (scaled by --ncell / 2^20) in one persistent launch of
+    libkrome_multi_c3 (one JSON line)"""
+    import torch
+    import cells
+    import krome
+
+    dev = torch.device("cuda:0")
+    m = krome.load_multi("multi_c3")
+    m.set_launch(a.waves_per_cu, a.queue_chunk)
+    scale = a.ncell / float(1 << 20)
+    parts = (("primordial", cells.c3_primordial_cells, 1 << 19),
+             ("primordialZ", cells.c3_primordialZ_cells, 1 << 18),
+             ("cloud", cells.c3_cloud_cells, 1 << 17))
+    batches, pristine = [], []
+    for name, gen, full in parts:
+        nc = max(64, int(full * scale))
+        w = gen(nc)
+        o = w["opts"]
+        b = dict(network=name, evolve_T=int(o["evolve_T"]), rtol=o["rtol"],
+                 atol_n=o["atol_n"], atol_T=o["atol_T"],
+                 max_steps=int(o["max_steps"]), jac=int(o["jac"]),
+                 n=torch.tensor(np.ascontiguousarray(w["n"].T), device=dev),
+                 Tgas=torch.tensor(w["T"], device=dev),
+                 dt=torch.tensor(w["dt"], device=dev),
+                 par=torch.tensor(np.ascontiguousarray(w["par"]), device=dev),
+                 flags=torch.zeros(nc, dtype=torch.int32, device=dev),
+                 stats=torch.zeros((nc, 9), dtype=torch.int64, device=dev))
+        batches.append(b)
+        pristine.append((b["n"].clone(), b["Tgas"].clone()))
+
+    def restore():
+        for b, (n0, T0) in zip(batches, pristine):
+            b["n"].copy_(n0)
+            b["Tgas"].copy_(T0)
+
+    res = {}
+    for mixed in (0, 1):
+        for _ in range(a.warmup):
+            restore()
+            m.solve_device(batches, mixed)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        kms = []
+        for _ in range(a.steps):
+            restore()
+            r, ms = m.solve_device(batches, mixed)
+            print("c3 launch mixed=%d: %.1f ms, return %d" % (mixed, ms, r),
+                  file=sys.stderr, flush=True)
+            kms.append(ms)
+        torch.cuda.synchronize()
+        res[mixed] = (time.perf_counter() - t0, float(np.mean(kms)))
+    ncell = sum(b["Tgas"].shape[0] for b in batches)
+    per = {}
+    for b in batches:
+        st = b["stats"].cpu().numpy()[:, 0]
+        fl = b["flags"].cpu().numpy()
+        per[b["network"]] = {
+            "cells": int(st.size), "flags_nonzero": int((fl != 0).sum()),
+            "steps_p50": float(np.percentile(st, 50)),
+            "steps_p99": float(np.percentile(st, 99)),
+            "steps_max": int(st.max())}
+    allst = np.concatenate([b["stats"].cpu().numpy()[:, 0]
+                            for b in batches])
+    wall = res[1][0]
+    out = {
+        "metric": "cells/sec to t_end, three networks in one persistent "
+                  "launch (uneven batch)",
+        "value": ncell * a.steps / wall, "unit": "cells/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "uneven batch: primordial, primordialZ and "
+                               "cloud cells (seed 2, density 1..1e12, dt "
+                               "1e8..1e14 s, Tgas fixed) in one launch",
+                   "ncell": ncell, "parallelism": "1 GPU",
+                   "waves_per_cu": a.waves_per_cu,
+                   "queue_chunk": a.queue_chunk},
+        "fp64_cells_per_s": ncell * a.steps / res[0][0],
+        "fp64_kernel_ms": res[0][1],
+        "mixed_cells_per_s": ncell * a.steps / res[1][0],
+        "mixed_kernel_ms": res[1][1],
+        "mixed_note": "every network here runs at fixed Tgas, where the "
+                      "Newton matrix stays FP64: the two runs are the same "
+                      "arithmetic",
+        "steps_per_cell": {"p50": float(np.percentile(allst, 50)),
+                           "p99": float(np.percentile(allst, 99)),
+                           "max": int(allst.max())},
+        "networks": per,
+    }
+    print(json.dumps(out))
+
+
 def photo_bench(a):
     """galaxy_ism with photon bins: three kernels, one JSON line"""
     import torch
@@ -628,7 +723,9 @@ def main():
                     help="skip the persistent FP64 kernel's leg")
     ap.add_argument("--workload", default="solve",
                     choices=("solve", "conserve", "multi", "photo", "network",
-                             "copy", "adjoint"))
+                             "copy", "adjoint", "c3"))
+    ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--queue-chunk", type=int, default=0)
     ap.add_argument("--network", default="primordialZ")
     ap.add_argument("--cells", default="c3_primordialZ")
     ap.add_argument("--linsol", default="dense", choices=("dense", "sparse"),
@@ -656,6 +753,8 @@ def main():
         return photo_bench(a)
     if a.workload == "multi":
         return multi_bench(a)
+    if a.workload == "c3":
+        return c3_bench(a)
     if a.workload == "adjoint":
         if a.ncell == 1 << 20:
             a.ncell = 1 << 16

@@ -12,23 +12,27 @@
  *    returns KROME_FLAG_UNSUPPORTED; a cell that fails on the device keeps
  *    its input and its flag (no host re-solve);
  *  - linsol 2 (LAPACK) returns KROME_FLAG_UNSUPPORTED; linsol 3 (the
- *    static-pivot sparse LU) runs in the plain kernel, and the persistent
- *    kernel returns KROME_FLAG_UNSUPPORTED for it and for networks above
- *    16 equations;
+ *    static-pivot sparse LU) runs in the plain kernel and the lane queue;
+ *    the lockstep persistent kernel returns KROME_FLAG_UNSUPPORTED for it;
  *  - photon bins and fields as the reference's (no broadening, dust
  *    opacity, shielding or co-evolved field); no dust bins;
  *  - host-array calls (solves, element totals, conservation) copy their
  *    rows to the device, compute there and copy back;
  *  - extra option keys select this build's kernels: device_kernel (0 the
- *    plain FP64 kernel, 1 the persistent batched kernel), mixed_precision
- *    (persistent kernel: 1 = FP32 Jacobian, LU and triangular solves with
- *    the FP64 Newton residual, state and error test, with Tgas evolved
- *    and the difference-quotient Jacobian; with Tgas fixed the matrix
- *    stays FP64), queue_chunk (persistent kernel: each wave takes this
- *    many cells of the queue per atomic and hands them to its idle lanes;
- *    0, or below 64, takes exactly the cells its idle lanes need) and
- *    waves_per_cu (resident waves per CU; 0 = automatic).  Neither
- *    changes a cell's result, only which lane integrates it.
+ *    plain FP64 kernel; 1 the persistent batched kernel: the lockstep
+ *    kernel up to 16 equations, the lane queue above; 2 the lane queue
+ *    for any network: each lane solves whole cells and takes the next
+ *    from a device work queue), mixed_precision (lockstep kernel: 1 =
+ *    FP32 Jacobian, LU and triangular solves with the FP64 Newton
+ *    residual, state and error test, with Tgas evolved and the
+ *    difference-quotient Jacobian; with Tgas fixed the matrix stays FP64),
+ *    queue_chunk (lockstep kernel: each wave takes this many cells of the
+ *    queue per atomic and hands them to its idle lanes; 0, or below 64,
+ *    takes exactly the cells its idle lanes need), waves_per_cu (resident
+ *    waves per CU; 0 = automatic) and lane_block (the lane queue's lanes
+ *    per workgroup, 64 or 128).  None changes a cell's result, only which
+ *    lane integrates it (the lane queue computes the plain kernel's
+ *    arithmetic).
  */
 #ifndef KROME_MI355X_H
 #define KROME_MI355X_H
@@ -90,7 +94,8 @@ const char* krome_param_required_by(int i);
 
 /* solver options, an opaque handle set by key (krome.h:116-173): the keys
  * of krome.h plus this build's device_kernel, mixed_precision,
- * queue_chunk, waves_per_cu; -1 for an unknown key or a bad value */
+ * queue_chunk, waves_per_cu, lane_block; -1 for an unknown key or a bad
+ * value */
 typedef struct krome_opts krome_opts;
 krome_opts* krome_opts_new(void);
 void krome_opts_free(krome_opts* h);

@@ -17,7 +17,7 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 GEN = CSRC / "generated"
 NETWORKS = ("primordial", "primordial3", "galaxy_ism", "primordialZ",
-            "galaxy_ism_heat", "NO", "SM", "coolHCN", "lowmetal",
+            "galaxy_ism_heat", "cloud", "NO", "SM", "coolHCN", "lowmetal",
             "primordial2", "primordial_wD", "primordial_photo",
             "primordial_photoH2", "auto", "chianti", "primordial_UV",
             "xrays", "primordialZ_UV", "photobin")
@@ -42,8 +42,42 @@ def lib_path(net):
     return HERE / ("libkrome_%s.so" % net)
 
 
-# the library of several networks in one persistent launch (kx_multi.hip)
-MULTI = "multi"
+# libraries of several networks in one persistent launch (csrc/kx_multi.h
+# over a file generated here): name -> (the networks in the C interface's
+# order, each with Tgas evolved or fixed; the launch order, the costliest
+# cells first)
+MULTI = {
+    "multi": ((("primordial", False), ("primordial3", True)),
+              ("primordial3", "primordial")),
+    # the uneven batch: the lockstep machine for primordial, the lane queue
+    # for primordialZ (27 equations) and cloud (452)
+    "multi_c3": ((("primordial", False), ("primordialZ", False),
+                  ("cloud", False)),
+                 ("cloud", "primordialZ", "primordial")),
+}
+
+
+def multi_source(name):
+    """write csrc/generated/<name>.hip, the network list of a multi
+    library, and return its path"""
+    nets, order = MULTI[name]
+    index = {n: i for i, (n, _) in enumerate(nets)}
+    et = dict(nets)
+    lines = ["// generated by krome-x_amd/build.py: the networks of the multi",
+             "// library %s (C-interface index, name, Tgas evolved), in launch"
+             % name, "// order"]
+    lines += ['#include "net_%s.h"' % n for n in order]
+    lines += ['#define KX_MULTI_NAME "%s"' % name,
+              "#define KX_MULTI_COUNT %d" % len(nets),
+              "#define KX_MULTI_NETS(X) %s" % " ".join(
+                  "X(%d, %s, %s)" % (index[n], n, "true" if et[n] else "false")
+                  for n in order),
+              '#include "kx_multi.h"', ""]
+    path = GEN / ("%s.hip" % name)
+    text = "\n".join(lines)
+    if not path.exists() or path.read_text() != text:
+        path.write_text(text)
+    return path
 
 
 def regen(networks_dir=None):
@@ -58,9 +92,9 @@ def regen(networks_dir=None):
 
 
 def _command(net, variant, tmp):
-    if net == MULTI:
+    if net in MULTI:
         return [HIPCC, *FLAGS, "-I", str(CSRC), "-I", str(GEN),
-                str(CSRC / "kx_multi.hip"), "-o", str(tmp)]
+                str(multi_source(net)), "-o", str(tmp)]
     extra = VARIANTS[variant][1] if variant else []
     return [HIPCC, *FLAGS, *extra, "-I", str(CSRC), "-I", str(GEN),
             "-DKX_NET=net_%s" % net,
@@ -86,7 +120,7 @@ def build_all(verbose=False, regen_headers=False):
         regen()
     jobs = ([(n, None) for n in NETWORKS] + [(net, v) for v, (net, _)
                                              in VARIANTS.items()]
-            + [(MULTI, None)])
+            + [(m, None) for m in MULTI])
     procs = []
     for net, variant in jobs:
         out = lib_path(variant or net)

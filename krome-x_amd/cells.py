@@ -175,6 +175,58 @@ def c3_primordialZ_cells(ncell=1 << 18, seed=2, first=1 << 19):
                      "max_steps": 100000}}
 
 
+# the cloud cells' elements other than H and He, as fractions of nH (the
+# "low-metal" initial abundances of dark-cloud models; the ions as the
+# singly charged species), and the grains
+CLOUD_METALS = {"C+": 7.3e-5, "N": 2.14e-5, "O": 1.76e-4, "S+": 8e-8,
+                "Si+": 8e-9, "Fe+": 3e-9, "Na+": 2e-9, "Mg+": 7e-9,
+                "P+": 3e-9, "Cl+": 4e-9, "F": 6.68e-9}
+CLOUD_GRAINS = 1.3e-12
+
+
+def c3_cloud_cells(ncell=1 << 17, seed=2, first=(1 << 19) + (1 << 18)):
+    """the cloud part of the uneven-batch workload (cells after the
+    primordial and primordialZ parts'): density 1..1e12 cm^-3 and dt
+    1e8..1e14 s as the other parts; Tgas 10..300 K (the range of the
+    network's rate fits; fixed), H2/nH 0.1..0.5, H+ = x nH with x
+    1e-8..1e-4, He 0.0775 nH, the metals of CLOUD_METALS, GRAIN0 1.3e-12
+    nH, electrons for neutrality, the rest 1e-20 nH; parameters crate
+    1.3e-17 s^-1, gas_dust_ratio 0.01 and Av 0.1..10 per cell"""
+    meta = network_meta("cloud")
+    sp = meta["species"]
+    col = {s: i for i, s in enumerate(sp)}
+    c = np.arange(first, first + ncell)
+    nH = log_uniform(seed, c, 0, 1.0, 1e12)
+    T = log_uniform(seed, c, 1, 10.0, 300.0)
+    x = log_uniform(seed, c, 2, 1e-8, 1e-4)
+    y = log_uniform(seed, c, 3, 0.1, 0.5)
+    dt = log_uniform(seed, c, 4, 1e8, 1e14)
+    av = log_uniform(seed, c, 5, 0.1, 10.0)
+    n = np.empty((ncell, len(sp)))
+    n[:] = 1e-20 * nH[:, None]
+    n[:, col["H2"]] = y * nH
+    n[:, col["H+"]] = x * nH
+    n[:, col["He"]] = 0.0775 * nH
+    n[:, col["H"]] = nH * (1.0 - 2.0 * y - x)
+    for s, f in CLOUD_METALS.items():
+        n[:, col[s]] = f * nH
+    n[:, col["GRAIN0"]] = CLOUD_GRAINS * nH
+    q = np.array(meta["charge"], dtype=float)
+    ie = col["e-"]
+    n[:, ie] = 0.0
+    n[:, ie] = n @ q
+    par = np.tile(np.array(meta["param_defaults"], dtype=np.float64),
+                  (ncell, 1))
+    pc = {p: i for i, p in enumerate(meta["params"])}
+    par[:, pc["crate"]] = 1.3e-17
+    par[:, pc["gas_dust_ratio"]] = 0.01
+    par[:, pc["Av"]] = av
+    return {"n": n, "T": T, "dt": dt, "par": par,
+            "opts": {"evolve_T": 0, "rtol": 1e-6, "atol_n": 1e-20,
+                     "atol_T": 1e-6, "linsol": 0, "jac": 0,
+                     "max_steps": 100000}}
+
+
 def photo_bins(nbins=32, emin=0.7, emax=200.0):
     """(Elo, Ehi) [eV] of nbins logarithmic bins over [emin, emax]"""
     e = emin * (emax / emin) ** (np.arange(nbins + 1) / nbins)
@@ -281,7 +333,8 @@ WORKLOADS = {"primordial": primordial_cells, "primordial3": primordial3_cells,
              "galaxy_ism": galaxy_ism_cells,
              "galaxy_ism_heat": lambda ncell=1 << 18, seed=4, first=0:
                  galaxy_ism_cells(ncell, seed, first, "galaxy_ism_heat"),
-             "c3_primordialZ": c3_primordialZ_cells}
+             "c3_primordialZ": c3_primordialZ_cells,
+             "c3_cloud": c3_cloud_cells}
 for _n in EXTRA_NETWORKS:
     WORKLOADS[_n] = (lambda name: lambda ncell=256, seed=5, first=0:
                      generic_cells(name, ncell, seed, first))(_n)

@@ -51,9 +51,12 @@ struct Opts {
   // persistent kernel; its Jacobian / LU precision (0 FP64, 1 FP32)
   int kernel = 0;
   int mixed = 0;
-  // persistent kernel: cells per queue grab, workgroups per CU
+  // persistent kernel: cells per queue grab, waves per CU; the lane
+  // queue's lanes per workgroup (device_kernel 2, and 1 above 16
+  // equations)
   int queue_chunk = 0;
   int waves_per_cu = 0;
+  int lane_block = 128;
   // the adjoint: accepted steps between checkpoints, the checkpoint
   // pool's bytes (0: room for about 1536 steps per cell of a launch)
   int checkpoint_steps = 32;
@@ -192,9 +195,9 @@ constexpr int neq() { return kx::Problem<Net, ET>::N; }
 
 // bytes of the plain kernel's workspace per cell
 size_t plain_cell_bytes(bool et) {
-  return et ? sizeof(double) * kx::cell_doubles<kx::Problem<Net, true>>() +
+  return et ? sizeof(double) * kx::cell_doubles<kx::PlainProblem<Net, true>>() +
                   sizeof(int) * neq<true>()
-            : sizeof(double) * kx::cell_doubles<kx::Problem<Net, false>>() +
+            : sizeof(double) * kx::cell_doubles<kx::PlainProblem<Net, false>>() +
                   sizeof(int) * neq<false>();
 }
 
@@ -245,7 +248,7 @@ int launch_plain(kx::CellsArgs a, DevCache* dc, char* work, hipStream_t s) {
   constexpr int N = neq<ET>();
   const long S = a.ncell;
   const size_t db =
-      sizeof(double) * kx::cell_doubles<kx::Problem<Net, ET>>() * S;
+      sizeof(double) * kx::cell_doubles<kx::PlainProblem<Net, ET>>() * S;
   double* wd = reinterpret_cast<double*>(work);
   int* wi = reinterpret_cast<int*>(work + db);
   if (dc) {
@@ -327,7 +330,7 @@ int solve_kernels(kx::CellsArgs a, const Opts& o, const SolveMem& m,
     return kx::FLAG_UNSUPPORTED;
   }
   DevCache* dc = m.dc;
-  if (o.kernel == 1) {
+  if (o.kernel == 1 || o.kernel == 2) {
     // the workspace and the queue order: the library's buffers, or the
     // fixed regions when they are large enough
     auto mem = [&](size_t wb, size_t ob, kx::PersistMem& pm) -> int {
@@ -343,6 +346,14 @@ int solve_kernels(kx::CellsArgs a, const Opts& o, const SolveMem& m,
       pm.ord = m.ord;
       return 0;
     };
+    if (o.kernel == 2) {  // the lane queue, for any network
+      if constexpr (Net::THERMAL)
+        if (a.o.evolve_T)
+          return kx::launch_lane_queue<Net, true>(a, o.waves_per_cu, s, mem,
+                                                  o.lane_block);
+      return kx::launch_lane_queue<Net, false>(a, o.waves_per_cu, s, mem,
+                                               o.lane_block);
+    }
     if constexpr (Net::THERMAL)
       if (a.o.evolve_T)
         return kx::launch_persistent<Net, true>(a, o.mixed, o.queue_chunk,
@@ -563,8 +574,13 @@ int krome_opts_set(krome_opts* h, const char* key, double v) {
   if (is("min_device")) { h->min_device = (int)v; return 0; }
   if (is("min_interleave")) { h->min_interleave = (int)v; return 0; }
   if (is("device_kernel")) {
-    if (v < 0 || v > 1) return -1;
+    if (v < 0 || v > 2) return -1;
     h->kernel = (int)v;
+    return 0;
+  }
+  if (is("lane_block")) {
+    if (v != 64 && v != 128) return -1;
+    h->lane_block = (int)v;
     return 0;
   }
   if (is("mixed_precision")) {
@@ -622,6 +638,7 @@ int krome_opts_get(const krome_opts* h, const char* key, double* v) {
   else if (is("mixed_precision")) *v = h->mixed;
   else if (is("queue_chunk")) *v = h->queue_chunk;
   else if (is("waves_per_cu")) *v = h->waves_per_cu;
+  else if (is("lane_block")) *v = h->lane_block;
   else if (is("checkpoint_steps")) *v = h->checkpoint_steps;
   else if (is("checkpoint_memory")) *v = (double)h->checkpoint_memory;
   else return -1;
@@ -783,11 +800,11 @@ int attach_tol(kx::SolveOpts& o, const krome_opts* h, Buf* buf,
 // what the whole call rejects before integrating (every cell flagged)
 int reject(const krome_opts* h) {
   if (h->linsol == 2) return kx::FLAG_UNSUPPORTED;  // no LAPACK here
-  // the persistent kernels factorize the dense Newton matrix only, for
-  // networks of up to KX_PERSIST_MAX_N equations
-  if (h->kernel == 1 &&
-      (h->linsol == 3 ||
-       kx::Problem<Net, false>::N + (h->evolve_T ? 1 : 0) > KX_PERSIST_MAX_N))
+  // the lockstep persistent kernel (networks of up to KX_PERSIST_MAX_N
+  // equations) factorizes the dense Newton matrix only; larger networks
+  // take the lane queue, which runs the plain solve with either LU
+  if (h->kernel == 1 && h->linsol == 3 &&
+      kx::Problem<Net, false>::N + (h->evolve_T ? 1 : 0) <= KX_PERSIST_MAX_N)
     return kx::FLAG_UNSUPPORTED;
   if (h->path == 1) return kx::FLAG_UNSUPPORTED;  // no host path here
   if (h->evolve_T && !Net::THERMAL) return kx::FLAG_UNSUPPORTED;
@@ -1442,11 +1459,14 @@ int krome_adjoint_cells_device(int ncell, double* n, double* Tgas,
   g.g_T0 = g_T0;
   long peak = 0;
   int r;
-  if constexpr (Net::THERMAL)
+  if constexpr (Net::NSPEC > KX_BIG_N)  // its dense per-lane Jacobians
+    r = kx::FLAG_UNSUPPORTED;
+  else if constexpr (Net::THERMAL)
     r = h->evolve_T ? adjoint_launch<true>(g, *h, peak)
                     : adjoint_launch<false>(g, *h, peak);
   else
     r = adjoint_launch<false>(g, *h, peak);
+  if (r == kx::FLAG_UNSUPPORTED) kx::fill_flags(ncell, flags, r);
   if (mem_bytes) *mem_bytes = (double)peak;
   if (r != 0) return r;
   std::vector<int> hf(ncell);

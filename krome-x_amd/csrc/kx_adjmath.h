@@ -19,6 +19,9 @@ struct AdjProblem {
   static constexpr int NS = Net::NSPEC;
   static constexpr int N = NS + (ET ? 1 : 0);
   static constexpr long NEWTON_DOUBLES = Problem<Net, ET>::NEWTON_DOUBLES;
+  static constexpr bool STRIDED = false;
+  static constexpr long J_DOUBLES = (long)N * N;
+  static constexpr long K_DOUBLES = 0;
   typename Net::Params p;
   double Tfix;
   bool hit_T0;

@@ -19,6 +19,14 @@ namespace kx {
 constexpr int QMAX = 5;
 constexpr int LMAX = QMAX + 1;
 
+// one cell's vector in the interleaved workspace, as an indexable view:
+// element i at p[i s]
+struct SView {
+  double* p;
+  long s;
+  KX_HD double& operator[](long i) const { return p[i * s]; }
+};
+
 // CVODE's flags (bdf_cell.hpp:107-122)
 enum : int {
   BDF_OK = 0, TOO_MUCH_WORK = -1, TOO_MUCH_ACC = -2, ERR_FAILURE = -3,
@@ -65,7 +73,11 @@ KX_HD inline double rt(double x) { return x <= 0.0 ? 0.0 : ::sqrt(x); }
 // rtol, per-equation absolute tolerances, the step budget and failure
 // limits, the Jacobian (difference quotients or the problem's) and the
 // charge invariant of the Newton corrections (inv_row >= 0: on)
-template <int N>
+#ifndef KX_BIG_N
+// more equations than this: the strided form (BigProblem, kx_problem.h)
+#define KX_BIG_N 64
+#endif
+template <int N, bool BIG = (N > KX_BIG_N)>
 struct CellOpts {
   double rtol;
   double atol[N];
@@ -79,6 +91,26 @@ struct CellOpts {
   bool sparse;  // the static-pivot sparse LU of the species block
   int inv_row, inv_n;
   int inv_c[N];
+  KX_HD double atol_at(int i) const { return atol[i]; }
+  KX_HD double rtol_at(int i) const { return rtolv[i]; }
+  KX_HD int invc(int i) const { return inv_c[i]; }
+};
+// the options of a large network (Tgas fixed, the analytic Jacobian and
+// the sparse LU): per-equation tolerances read from the call's rows in
+// device memory, no charge invariant (it pairs with difference quotients)
+template <int N>
+struct CellOpts<N, true> {
+  double rtol, atol_n;
+  const double* tol;  // atol[N] then rtol[N] (device), or nullptr
+  bool rtol_v;
+  long max_steps;
+  int max_err_fails, max_conv_fails, max_restarts;
+  bool dq_jac;
+  bool sparse;
+  int inv_row, inv_n;
+  KX_HD double atol_at(int i) const { return tol ? tol[i] : atol_n; }
+  KX_HD double rtol_at(int i) const { return tol ? tol[N + i] : rtol; }
+  KX_HD int invc(int) const { return 0; }
 };
 
 // CVODE's counters (bdf_cell.hpp:177-191)
@@ -95,13 +127,14 @@ struct CellStats {
 };
 
 // doubles of a cell's workspace: Nordsieck array (6 N), 9 vectors, the
-// saved Jacobian (N^2) and the Newton matrix's storage (Prob::
-// NEWTON_DOUBLES: N^2, or the sparse factors when larger); pivots are N
-// ints
+// saved Jacobian (Prob::J_DOUBLES: N^2, or the sparse pattern's values of
+// a large network), the Newton matrix's storage (Prob::NEWTON_DOUBLES:
+// N^2, or the sparse factors when larger) and the rates of a large
+// network (Prob::K_DOUBLES); pivots are N ints
 template <class Prob>
 constexpr long cell_doubles() {
-  return (LMAX + 9) * Prob::N + (long)Prob::N * Prob::N +
-         Prob::NEWTON_DOUBLES;
+  return (LMAX + 9) * Prob::N + Prob::J_DOUBLES + Prob::NEWTON_DOUBLES +
+         Prob::K_DOUBLES;
 }
 
 // the recorder of the integrator's operations on its Nordsieck array:
@@ -141,15 +174,22 @@ class Cell {
     vt2 = w; w += N * S;
     vt3 = w; w += N * S;
     A = w; w += Prob::NEWTON_DOUBLES * S;
-    J = w;
+    J = w; w += Prob::J_DOUBLES * S;
+    K = w;
     piv = iwork;
     mxstep = o.max_steps == 0 ? 500 : o.max_steps;
-    double m = o.atol[0];
-    for (int i = 1; i < N; i++) m = o.atol[i] < m ? o.atol[i] : m;
+    double m = o.atol_at(0);
+    for (int i = 1; i < N; i++) m = o.atol_at(i) < m ? o.atol_at(i) : m;
     atolmin0 = m == 0.0;
   }
 
   KX_HD double& at(double* v, int i) const { return v[(long)i * S]; }
+  // the view of a workspace vector the strided problems read and write
+  KX_HD SView view(const double* v) const {
+    return SView{const_cast<double*>(v), S};
+  }
+  // the rates' rows of a large network (Prob::K_DOUBLES of them)
+  KX_HD double* rates_rows() const { return K; }
 
   KX_HD void reinit(double t0, const double* y0, double h_init) {
     tn = t0;
@@ -310,6 +350,7 @@ class Cell {
   bool atolmin0;
   double* zn[LMAX];
   double *ewt, *y, *acor, *tempv, *ftemp, *delta, *vt1, *vt2, *vt3, *A, *J;
+  double* K;
   int* piv;
   double tn = 0, h = 0, hin = 0, hprime = 0, hscale = 0, h0u = 0, next_h = 0,
          hu = 0, eta = 1, etamax = 0, etaq = 0, etaqm1 = 0, etaqp1 = 0,
@@ -323,11 +364,15 @@ class Cell {
 
   // ---- vector kernels (the serial N_Vector, bdf_cell.hpp:293-400) ----
   KX_HD int rhs_v(const double* yv, double* fv) {
-    double yr[N], fr[N];
-    for (int i = 0; i < N; i++) yr[i] = at(const_cast<double*>(yv), i);
-    const int r = P.rhs(yr, fr);
-    for (int i = 0; i < N; i++) at(fv, i) = fr[i];
-    return r;
+    if constexpr (Prob::STRIDED) {
+      return P.rhs_s(view(yv), view(fv));
+    } else {
+      double yr[N], fr[N];
+      for (int i = 0; i < N; i++) yr[i] = at(const_cast<double*>(yv), i);
+      const int r = P.rhs(yr, fr);
+      for (int i = 0; i < N; i++) at(fv, i) = fr[i];
+      return r;
+    }
   }
   KX_HD void axpy(double a, const double* x, double* yv) {
     if (a == 1.0) {
@@ -400,23 +445,37 @@ class Cell {
     if (o.rtol_v) {
       for (int i = 0; i < N; i++) {
         const double d =
-            ::fma(o.rtolv[i], ::fabs(at(const_cast<double*>(yc), i)),
-                  o.atol[i]);
+            ::fma(o.rtol_at(i), ::fabs(at(const_cast<double*>(yc), i)),
+                  o.atol_at(i));
         if (!(d > 0.0)) return -1;
         at(w, i) = 1.0 / d;
       }
       return 0;
     }
-    double v[N];
-    for (int i = 0; i < N; i++) v[i] = ::fabs(at(const_cast<double*>(yc), i));
-    for (int i = 0; i < N; i++) v[i] = (o.rtol * v[i]) + o.atol[i];
-    if (atolmin0) {
-      double m = v[0];
-      for (int i = 1; i < N; i++) m = v[i] < m ? v[i] : m;
-      if (m <= 0.0) return -1;
+    if constexpr (Prob::STRIDED) {  // the same steps in the workspace
+      for (int i = 0; i < N; i++)
+        at(w, i) = (o.rtol * ::fabs(at(const_cast<double*>(yc), i))) +
+                   o.atol_at(i);
+      if (atolmin0) {
+        double m = at(w, 0);
+        for (int i = 1; i < N; i++) m = at(w, i) < m ? at(w, i) : m;
+        if (m <= 0.0) return -1;
+      }
+      for (int i = 0; i < N; i++) at(w, i) = 1.0 / at(w, i);
+      return 0;
+    } else {
+      double v[N];
+      for (int i = 0; i < N; i++)
+        v[i] = ::fabs(at(const_cast<double*>(yc), i));
+      for (int i = 0; i < N; i++) v[i] = (o.rtol * v[i]) + o.atol[i];
+      if (atolmin0) {
+        double m = v[0];
+        for (int i = 1; i < N; i++) m = v[i] < m ? v[i] : m;
+        if (m <= 0.0) return -1;
+      }
+      for (int i = 0; i < N; i++) at(w, i) = 1.0 / v[i];
+      return 0;
     }
-    for (int i = 0; i < N; i++) at(w, i) = 1.0 / v[i];
-    return 0;
   }
 
   // cvHin, cvUpperBoundH0, cvYddNorm (bdf_cell.hpp:717-794)
@@ -744,7 +803,7 @@ class Cell {
       if (jr < 0) r = -1;
       else if (jr > 0) r = 1;
     }
-    if (r == 0 && !o.sparse) {
+    if (r == 0 && !o.sparse && !Prob::STRIDED) {
       const double c = -gamma;
       for (int j = 0; j < N; j++) {
         for (int i = 0; i < N; i++) at(A, i + j * N) = at(J, i + j * N) * c;
@@ -756,8 +815,9 @@ class Cell {
       nstlj = st.nst;
     }
     if (r != 0) return r;
-    if (o.sparse) return sp_factor(-gamma);
-    return lu() > 0 ? 1 : 0;
+    if (o.sparse || Prob::STRIDED) return sp_factor(-gamma);
+    if constexpr (!Prob::STRIDED) return lu() > 0 ? 1 : 0;
+    return 1;
   }
 
   // ---- the static-pivot sparse LU (bdf_cell.hpp:1155-1264) -------------
@@ -781,8 +841,11 @@ class Cell {
   KX_HD int sp_factor(double c) {
     for (int k = 0; k < NS; k++) {
       const long row = Net::sp_perm[k];
-      for (int p = Net::sp_rp[k]; p < Net::sp_rp[k + 1]; p++)
-        spv(p) = at(J, row + (long)Net::sp_perm[Net::sp_ci[p]] * N) * c;
+      for (int p = Net::sp_rp[k]; p < Net::sp_rp[k + 1]; p++) {
+        // a large network's J holds the pattern's values in this order
+        if constexpr (Prob::STRIDED) spv(p) = at(J, p) * c;
+        else spv(p) = at(J, row + (long)Net::sp_perm[Net::sp_ci[p]] * N) * c;
+      }
       spv(Net::sp_di[k]) += 1.0;
     }
     for (int i = 0; i < NS; i++) {  // row i of L and U
@@ -867,7 +930,8 @@ class Cell {
     return 0;
   }
   // b = (c J + I)^-1 b with the factors of sp_factor (b in registers)
-  KX_HD void sp_solve(double* b) {
+  template <class BV>
+  KX_HD void sp_solve(BV& b) {
     for (int k = 0; k < NS; k++) spv(SP_W + k) = b[Net::sp_perm[k]];
     sp_lower(SP_W);
     if constexpr (NB > 0) {
@@ -912,15 +976,20 @@ class Cell {
     for (int k = 0; k < NS; k++) b[Net::sp_perm[k]] = spv(SP_W + k);
   }
   KX_HD int an_jac() {
-    double yr[N], Jr[N * N];
-    for (int i = 0; i < N; i++) yr[i] = at(y, i);
-    for (int k = 0; k < N * N; k++) Jr[k] = 0.0;
-    const int r = P.jac(yr, Jr);
-    for (int k = 0; k < N * N; k++) at(J, k) = Jr[k];
-    return r;
+    if constexpr (Prob::STRIDED) {
+      return P.jac_s(view(y), view(J));
+    } else {
+      double yr[N], Jr[N * N];
+      for (int i = 0; i < N; i++) yr[i] = at(y, i);
+      for (int k = 0; k < N * N; k++) Jr[k] = 0.0;
+      const int r = P.jac(yr, Jr);
+      for (int k = 0; k < N * N; k++) at(J, k) = Jr[k];
+      return r;
+    }
   }
   // cvLsDenseDQJac (bdf_cell.hpp:1290-1311)
   KX_HD int dq_jac() {
+    if constexpr (Prob::STRIDED) return -1;  // refused by cell_opts
     const double srur = bdfc::rt(DBL_EPSILON);
     const double fn = wrms(ftemp, ewt);
     const double minInc =
@@ -969,6 +1038,15 @@ class Cell {
   // triangular solves and the gamma-ratio scaling (bdf_cell.hpp:435-457,
   // 1134-1153)
   KX_HD void lsolve(double* bv) {
+    if constexpr (Prob::STRIDED) {  // in place in the workspace
+      SView b = view(bv);
+      sp_solve(b);
+      if (gamrat != 1.0) {
+        const double c = 2.0 / (1.0 + gamrat);
+        for (int i = 0; i < N; i++) b[i] = c * b[i];
+      }
+      return;
+    }
     double b[N];
     for (int i = 0; i < N; i++) b[i] = at(bv, i);
     if (o.sparse) {
@@ -1006,20 +1084,20 @@ class Cell {
   KX_HD double inv_dot(double* x) {
     double s = 0.0;
     for (int i = 0; i < N; i++)
-      if (i < o.inv_n && o.inv_c[i] != 0) s += o.inv_c[i] * at(x, i);
+      if (i < o.inv_n && o.invc(i) != 0) s += o.invc(i) * at(x, i);
     return s;
   }
   KX_HD void inv_set(double* x, double target) {
     const int r = o.inv_row;
     double s = 0.0, sa = 0.0, yr = 0.0, cr = 0.0;
     for (int i = 0; i < N; i++) {
-      if (i < o.inv_n && i != r && o.inv_c[i] != 0) {
-        s += o.inv_c[i] * at(x, i);
-        sa += ::fabs(o.inv_c[i] * (at(zn[0], i) + at(acor, i)));
+      if (i < o.inv_n && i != r && o.invc(i) != 0) {
+        s += o.invc(i) * at(x, i);
+        sa += ::fabs(o.invc(i) * (at(zn[0], i) + at(acor, i)));
       }
       if (i == r) {
         yr = at(zn[0], i) + at(acor, i);
-        cr = o.inv_c[i];
+        cr = o.invc(i);
       }
     }
     const double xr = (target - s) / cr;

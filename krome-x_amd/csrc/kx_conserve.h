@@ -17,9 +17,10 @@
 // built without contraction, bit for bit.
 //
 // The species the reference's projections hold fixed are the opaque ones
-// and those of freeze directives (solve_cell.hpp:1434-1445): netgen.py
-// accepts neither, so only the mass-fraction form holds any (species
-// without a mass; none in these networks either).  `fixed` is a bit mask.
+// and those of freeze directives (solve_cell.hpp:1440-1449): netgen.py
+// has no freeze directives; the opaque species (cloud's grains: no atoms,
+// not the electron) are held, and the mass-fraction form also holds the
+// species without a mass.  `fixed` is a bit set over the species.
 //
 // Memory: a cell's row is read once and written once, v[c cs + i ss]
 // (species-major: cs 1, ss ld, coalesced; cell-major: cs NSPEC, ss 1, a
@@ -39,23 +40,43 @@ template <class Net>
 struct Conserve {
   static constexpr int NS = Net::NSPEC;
   static constexpr int NE = Net::NELEM;
-  static_assert(NS <= 64, "the fixed-species mask holds 64 species");
-  using Mask = unsigned long long;
+  // a bit per species
+  struct Mask {
+    unsigned long long w[(NS + 63) / 64] = {};
+    KX_HD bool test(int i) const { return (w[i >> 6] >> (i & 63)) & 1ull; }
+    KX_HD void set(int i) { w[i >> 6] |= 1ull << (i & 63); }
+  };
+  // the opaque species: no atoms of any element, and not the electron
+  KX_HD static bool opaque(int i) {
+    if (Net::HAS_E && i == Net::I_E) return false;
+    for (int e = 0; e < NE; ++e)
+      if (Net::natoms(i, e) != 0) return false;
+    return true;
+  }
+  KX_HD static Mask opaque_mask() {
+    Mask m;
+    for (int i = 0; i < NS; ++i)
+      if (opaque(i)) m.set(i);
+    return m;
+  }
   static_assert(NE >= 1, "a network without elements has nothing to keep");
+  // the species loops unroll fully up to 64 species; a larger network's
+  // (cloud: 452) stay loops
+  static constexpr int UNR = NS <= 64 ? NS : 1;
 
   // conserved row k at species i: atoms of element k (k < NE), charge (NE)
   KX_HD static double row(int k, int i) {
     return k < NE ? double(Net::natoms(i, k)) : double(Net::charge(i));
   }
-  KX_HD static bool is_fixed(Mask fixed, int i) {
-    return (fixed >> i) & Mask(1);
+  KX_HD static bool is_fixed(const Mask& fixed, int i) {
+    return fixed.test(i);
   }
 
   KX_HD static void totals(const double* n, double* tot) {
 #pragma clang fp contract(off)
 #pragma unroll
     for (int e = 0; e < NE; ++e) tot[e] = 0.0;
-#pragma unroll
+#pragma unroll UNR
     for (int i = 0; i < NS; ++i)
 #pragma unroll
       for (int e = 0; e < NE; ++e) tot[e] += double(Net::natoms(i, e)) * n[i];
@@ -65,7 +86,7 @@ struct Conserve {
 #pragma clang fp contract(off)
 #pragma unroll
     for (int e = 0; e < NE; ++e) tot[e] = 0.0;
-#pragma unroll
+#pragma unroll UNR
     for (int i = 0; i < NS; ++i) {
       const double m = Net::mass_g(i);
       if (!(m > 0.0)) continue;
@@ -83,7 +104,7 @@ struct Conserve {
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
       double t = 0.0, sc = ::fabs(target[k]);
-#pragma unroll
+#pragma unroll UNR
       for (int i = 0; i < NS; ++i) {
         const double a = row(k, i) * n[i];
         t += a;
@@ -101,7 +122,7 @@ struct Conserve {
   // clipping negatives, at most 20 times, until the residual is below 1e-13
   template <int NR>
   KX_HD static double project(double* n, const double* target,
-                              Mask fixed) {
+                              const Mask& fixed) {
 #pragma clang fp contract(off)
     for (int it = 0; it < 20; ++it) {
       double r[NR], M[NR][NR], lam[NR];
@@ -109,13 +130,13 @@ struct Conserve {
 #pragma unroll
       for (int k = 0; k < NR; ++k) {
         double t1 = 0.0;
-#pragma unroll
+#pragma unroll UNR
         for (int i = 0; i < NS; ++i) t1 += row(k, i) * n[i];
         r[k] = target[k] - t1;
 #pragma unroll
         for (int l = 0; l < NR; ++l) {
           double s = 0.0;
-#pragma unroll
+#pragma unroll UNR
           for (int i = 0; i < NS; ++i)
             if (!is_fixed(fixed, i)) s += row(k, i) * row(l, i) * n[i];
           M[k][l] = s;
@@ -170,7 +191,7 @@ struct Conserve {
         lam[c] = s / M[c][c];
       }
       bool clipped = false;
-#pragma unroll
+#pragma unroll UNR
       for (int i = 0; i < NS; ++i) {
         if (is_fixed(fixed, i)) continue;
         double d = 0.0;
@@ -193,7 +214,7 @@ struct Conserve {
 #pragma unroll
     for (int k = 0; k <= NE; ++k) {
       target[k] = 0.0;
-#pragma unroll
+#pragma unroll UNR
       for (int i = 0; i < NS; ++i) target[k] += row(k, i) * n0[i];
     }
   }
@@ -203,10 +224,10 @@ struct Conserve {
   // kept), the electrons from neutrality, or the charge projected too when
   // the electrons cannot carry it; 0, FLAG_CONSERVE_UNREACHED (residual
   // above 1e-12) or FLAG_CONSERVE_LARGE (a species moved by over 50 %)
-  KX_HD static int to_n(double* n, const double* ref, Mask fixed) {
+  KX_HD static int to_n(double* n, const double* ref, const Mask& fixed) {
 #pragma clang fp contract(off)
     double n0[NS];
-#pragma unroll
+#pragma unroll UNR
     for (int i = 0; i < NS; ++i) {
       if (!is_fixed(fixed, i) && n[i] < 0.0) n[i] = 0.0;
       n0[i] = n[i];
@@ -220,7 +241,7 @@ struct Conserve {
     if constexpr (Net::HAS_E) {
       if (!is_fixed(fixed, Net::I_E)) {
         double qi = 0.0;
-#pragma unroll
+#pragma unroll UNR
         for (int i = 0; i < NS; ++i)
           if (i != Net::I_E) qi += double(Net::charge(i)) * n[i];
         if (qi >= 0.0) {
@@ -234,7 +255,7 @@ struct Conserve {
     if (!charge_done) project<NE + 1>(n, target, fixed);
     if (residual<NE + 1>(n, target) > 1e-12) return FLAG_CONSERVE_UNREACHED;
     int f = 0;
-#pragma unroll
+#pragma unroll UNR
     for (int i = 0; i < NS; ++i)
       if (n0[i] > 0.0 && ::fabs(n[i] - n0[i]) > 0.5 * n0[i])
         f = FLAG_CONSERVE_LARGE;
@@ -245,16 +266,16 @@ struct Conserve {
   // x = m n; species without a mass held
   KX_HD static int to_x(double* x, const double* ref) {
 #pragma clang fp contract(off)
-    Mask fixed = 0;
+    Mask fixed = opaque_mask();
     double n[NS];
-#pragma unroll
+#pragma unroll UNR
     for (int i = 0; i < NS; ++i) {
       const double m = Net::mass_g(i);
-      if (!(m > 0.0)) fixed |= Mask(1) << i;
+      if (!(m > 0.0)) fixed.set(i);
       n[i] = m > 0.0 ? x[i] / m : 0.0;
     }
     const int f = to_n(n, ref, fixed);
-#pragma unroll
+#pragma unroll UNR
     for (int i = 0; i < NS; ++i)
       if (!is_fixed(fixed, i)) x[i] = Net::mass_g(i) * n[i];
     return f;
@@ -282,7 +303,7 @@ totals_kernel(int ncell, const double* __restrict__ v, long cs, long ss,
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
   double n[C::NS], t[C::NE];
-#pragma unroll
+#pragma unroll C::UNR
   for (int i = 0; i < C::NS; ++i) n[i] = v[c * cs + i * ss];
   if (X) C::totals_x(n, t);
   else C::totals(n, t);
@@ -302,12 +323,12 @@ conserve_kernel(int ncell, double* __restrict__ v, long cs, long ss,
   int f = 0;
   if (c < ncell) {
     double n[C::NS], r[C::NE];
-#pragma unroll
+#pragma unroll C::UNR
     for (int i = 0; i < C::NS; ++i) n[i] = v[c * cs + i * ss];
 #pragma unroll
     for (int e = 0; e < C::NE; ++e) r[e] = ref[c * rs + e];
-    f = X ? C::to_x(n, r) : C::to_n(n, r, 0u);
-#pragma unroll
+    f = X ? C::to_x(n, r) : C::to_n(n, r, C::opaque_mask());
+#pragma unroll C::UNR
     for (int i = 0; i < C::NS; ++i) v[c * cs + i * ss] = n[i];
     flags[c] = f;
   }
@@ -323,7 +344,7 @@ conserve_targets_kernel(CellsArgs a, double* __restrict__ target) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.ncell) return;
   double n[C::NS], t[C::NE + 1];
-#pragma unroll
+#pragma unroll C::UNR
   for (int i = 0; i < C::NS; ++i) n[i] = a.n[c * a.n_cs + i * a.n_ss];
   C::targets(n, t);
 #pragma unroll
@@ -339,12 +360,12 @@ conserve_post_kernel(CellsArgs a, const double* __restrict__ target) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.ncell || !(a.dt[c] > 0.0) || a.flags[c] != 0) return;
   double n[C::NS], t[C::NE + 1];
-#pragma unroll
+#pragma unroll C::UNR
   for (int i = 0; i < C::NS; ++i) n[i] = a.n[c * a.n_cs + i * a.n_ss];
 #pragma unroll
   for (int k = 0; k <= C::NE; ++k) t[k] = target[(long)c * (C::NE + 1) + k];
-  C::template project<C::NE + 1>(n, t, 0u);
-#pragma unroll
+  C::template project<C::NE + 1>(n, t, C::opaque_mask());
+#pragma unroll C::UNR
   for (int i = 0; i < C::NS; ++i) a.n[c * a.n_cs + i * a.n_ss] = n[i];
 }
 

@@ -2024,6 +2024,90 @@ inline bool persist_mixed(const CellsArgs& a, int mixed) {
   return mixed && ET && dq;
 }
 
+// ---- the lane queue: networks above KX_PERSIST_MAX_N equations ----------
+// bytes of one lane slot of the lane-queue launch (the plain solve's
+// interleaved workspace: doubles, then pivots)
+template <class Net, bool ET>
+constexpr size_t lane_slot_bytes() {
+  using Prob = PlainProblem<Net, ET>;
+  return sizeof(double) * cell_doubles<Prob>() + sizeof(int) * Prob::N;
+}
+// resident lanes of a lane-queue launch over ncell cells in workgroups of
+// block lanes: waves_per_cu waves of 64 on each CU (0: 4, one per SIMD at
+// this kernel's register use), no more than the cells rounded up to a
+// workgroup
+inline int lane_queue_lanes(long ncell, int wpc, long& lanes,
+                            int block = 128) {
+  int dev = 0, ncu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return -1;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                            dev) != hipSuccess)
+    return -1;
+  if (wpc <= 0) wpc = 4;
+  lanes = (long)ncu * wpc * 64;
+  const long need = (ncell + block - 1) / block * block;
+  if (lanes > need) lanes = need;
+  lanes = (lanes + block - 1) / block * block;
+  return 0;
+}
+// bytes of a lane-queue launch's workspace for a given number of lanes:
+// the counter's 256 bytes, then the slots
+template <class Net, bool ET>
+constexpr size_t lane_queue_bytes(long lanes) {
+  return 256 + lane_slot_bytes<Net, ET>() * (size_t)lanes;
+}
+// the queue (counter zeroed by a kernel, the densest-first permutation)
+// and the slots' workspace in m, on stream s
+template <class Net, bool ET>
+int lane_queue_prepare(CellsArgs a, PersistMem m, long lanes, hipStream_t s,
+                       LaneQueue& q, double*& work, int*& iwork) {
+  using Prob = PlainProblem<Net, ET>;
+  char* p = static_cast<char*>(m.work);
+  q.counter = reinterpret_cast<int*>(p);
+  p += 256;
+  work = reinterpret_cast<double*>(p);
+  p += sizeof(double) * cell_doubles<Prob>() * (size_t)lanes;
+  iwork = reinterpret_cast<int*>(p);
+  int* hist = static_cast<int*>(m.ord);
+  int* key = hist + ORDER_BUCKETS;
+  int* order = key + a.ncell;
+  const int cb = (a.ncell + 255) / 256;
+  hipLaunchKernelGGL(fill_flags_kernel, dim3(1), dim3(64), 0, s, 1,
+                     q.counter, 0);
+  hipLaunchKernelGGL(fill_flags_kernel, dim3(ORDER_BUCKETS / 256), dim3(256),
+                     0, s, ORDER_BUCKETS, hist, 0);
+  hipLaunchKernelGGL(order_keys_kernel<Net>, dim3(cb), dim3(256), 0, s, a,
+                     key, hist);
+  hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(64), 0, s, hist);
+  hipLaunchKernelGGL(order_scatter_kernel, dim3(cb), dim3(256), 0, s,
+                     a.ncell, key, hist, order);
+  q.order = order;
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+// the lane-queue launch with its memory from the caller (as
+// launch_persistent): FP64 throughout (mixed precision does not apply);
+// block: lanes per workgroup, 64 or 128 (0: 128)
+template <class Net, bool ET, class Mem>
+inline int launch_lane_queue(CellsArgs a, int wpc, hipStream_t s,
+                             Mem&& mem, int block = 128) {
+  if (block <= 0) block = 128;
+  if (block != 64 && block != 128) return -1;
+  long lanes = 0;
+  if (lane_queue_lanes(a.ncell, wpc, lanes, block) != 0) return -1;
+  PersistMem m;
+  const int r = mem(lane_queue_bytes<Net, ET>(lanes),
+                    persist_order_bytes(a.ncell), m);
+  if (r != 0) return r;
+  LaneQueue q;
+  double* work = nullptr;
+  int* iwork = nullptr;
+  if (lane_queue_prepare<Net, ET>(a, m, lanes, s, q, work, iwork) != 0)
+    return -1;
+  hipLaunchKernelGGL((lane_queue_kernel<Net, ET>), dim3(lanes / block),
+                     dim3(block), 0, s, a, q, work, iwork, lanes);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 // bytes of the workspace a launch needs (0: the network is refused)
 template <class Net, bool ET>
 inline size_t persist_need(const CellsArgs& a, int mixed) {
@@ -2043,8 +2127,9 @@ template <class Net, bool ET, class Mem>
 inline int launch_persistent(CellsArgs a, int mixed, int chunk, int wpc,
                              hipStream_t s, Mem&& mem) {
   if constexpr (Problem<Net, ET>::N > KX_PERSIST_MAX_N) {
-    (void)a; (void)mixed; (void)chunk; (void)wpc; (void)s; (void)mem;
-    return FLAG_UNSUPPORTED;
+    // too large for the lockstep kernel's registers: the lane queue
+    (void)mixed; (void)chunk;
+    return launch_lane_queue<Net, ET>(a, wpc, s, mem);
   } else {
     PersistMem m;
     const int r = mem(persist_need<Net, ET>(a, mixed),

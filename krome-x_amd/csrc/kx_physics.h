@@ -33,8 +33,11 @@
 // persistent kernel evaluates the rhs at one program point and keeps the
 // lanes' state in registers, which calls through the stack would defeat
 #define KX_HD __host__ __device__ __attribute__((always_inline))
+// a function compiled once and called (large bodies that run rarely)
+#define KX_HD_CALL __host__ __device__ __attribute__((noinline))
 #else
 #define KX_HD
+#define KX_HD_CALL
 #endif
 
 namespace kx {

@@ -27,41 +27,52 @@ struct CellsArgs {
 
 // the integrator options of a cell (solve_cell.hpp:1662-1705: the
 // tolerances per slot, the DQ Jacobian with evolved Tgas, the charge
-// invariant with the DQ Jacobian and electrons, charge_newton_ 1631-1642)
-template <class Net, bool ET>
-KX_HD inline bool cell_opts(const SolveOpts& o,
-                            CellOpts<Problem<Net, ET>::N>& co) {
+// invariant with the DQ Jacobian and electrons, charge_newton_ 1631-1642);
+// false for what the kernels do not support.  Prob: the problem type the
+// cell is solved with (Problem, or BigProblem for a large network)
+template <class Net, bool ET, class Prob = Problem<Net, ET>>
+KX_HD inline bool cell_opts(const SolveOpts& o, CellOpts<Prob::N>& co) {
   constexpr int NS = Net::NSPEC;
-  constexpr int N = Problem<Net, ET>::N;
+  constexpr int N = Prob::N;
   co.rtol = o.rtol;
-  for (int i = 0; i < NS; ++i) co.atol[i] = o.atol_n;
-  if (ET) co.atol[N - 1] = o.atol_T;
   co.rtol_v = o.tol != nullptr && o.rtol_v != 0;
-  if (o.tol != nullptr)
-    for (int i = 0; i < N; ++i) {
-      co.atol[i] = o.tol[i];
-      co.rtolv[i] = co.rtol_v ? o.tol[N + i] : o.rtol;
-    }
   co.max_steps = o.max_steps;
   co.max_err_fails = o.max_err_fails;
   co.max_conv_fails = o.max_conv_fails;
   co.max_restarts = o.max_restarts;
   co.dq_jac = o.jac == 2 || (o.jac == 0 && ET);
-  // the linear solver (solve_cell.hpp:1601-1620): 3 sparse; 0 (auto)
-  // sparse where its factorization is at most a tenth of the dense one and
-  // the border at most 4 equations; else dense
-  co.sparse = o.linsol == 3 ||
-              (o.linsol == 0 && N - NS <= 4 &&
-               2.0 * NS * NS * NS / 3.0 >= 10.0 * Net::SP_FLOPS);
   co.inv_row = -1;
   co.inv_n = 0;
-  for (int i = 0; i < N; ++i) co.inv_c[i] = i < NS ? Net::charge(i) : 0;
-  if (co.dq_jac && Net::HAS_E) {
-    co.inv_row = Net::I_E;
-    co.inv_n = NS;
+  if constexpr (Prob::STRIDED) {
+    // a large network: Tgas fixed, the analytic Jacobian into the sparse
+    // LU (linsol auto resolves to it: sparse_auto_ picks the sparse LU
+    // for these networks), no LAPACK / dense LU
+    co.atol_n = o.atol_n;
+    co.tol = o.tol;
+    co.sparse = true;
+    return !ET && !co.dq_jac && o.linsol != 1 && o.linsol != 2;
+  } else {
+    for (int i = 0; i < NS; ++i) co.atol[i] = o.atol_n;
+    if (ET) co.atol[N - 1] = o.atol_T;
+    if (o.tol != nullptr)
+      for (int i = 0; i < N; ++i) {
+        co.atol[i] = o.tol[i];
+        co.rtolv[i] = co.rtol_v ? o.tol[N + i] : o.rtol;
+      }
+    // the linear solver (solve_cell.hpp:1601-1620): 3 sparse; 0 (auto)
+    // sparse where its factorization is at most a tenth of the dense one
+    // and the border at most 4 equations; else dense
+    co.sparse = o.linsol == 3 ||
+                (o.linsol == 0 && N - NS <= 4 &&
+                 2.0 * NS * NS * NS / 3.0 >= 10.0 * Net::SP_FLOPS);
+    for (int i = 0; i < N; ++i) co.inv_c[i] = i < NS ? Net::charge(i) : 0;
+    if (co.dq_jac && Net::HAS_E) {
+      co.inv_row = Net::I_E;
+      co.inv_n = NS;
+    }
+    // the analytic Jacobian here is the mass-action block at fixed Tgas
+    return co.dq_jac || !(ET || Net::RATES_DENSITY);
   }
-  // the analytic Jacobian here is the mass-action block at fixed Tgas
-  return co.dq_jac || !(ET || Net::RATES_DENSITY);
 }
 
 // the post-solve of a successful cell: clip, conservation (solve_cell.hpp:
@@ -93,14 +104,14 @@ KX_HD inline void cell_finish(const CellsArgs& a, int c, const double* y) {
 }
 
 #if defined(__HIPCC__)
+// the whole per-cell solve of cell c by one thread, its workspace at work
+// and iwork (this thread's slot of an interleaved workspace, stride S)
 template <class Net, bool ET>
-__global__ void __launch_bounds__(128)
-plain_cells_kernel(CellsArgs a, double* work, int* iwork, long S) {
-  using Prob = Problem<Net, ET>;
+__device__ inline void plain_cell(const CellsArgs& a, int c, double* work,
+                                  int* iwork, long S) {
+  using Prob = PlainProblem<Net, ET>;
   constexpr int NS = Net::NSPEC;
   constexpr int N = Prob::N;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= a.ncell) return;
   long* sc = a.stats ? a.stats + (long)c * 9 : nullptr;
   if (!(a.dt[c] > 0.0)) {  // dt <= 0: untouched (solve_cells.hpp:96-101)
     a.flags[c] = 0;
@@ -113,14 +124,18 @@ plain_cells_kernel(CellsArgs a, double* work, int* iwork, long S) {
   P.Tfix = a.T[c];
   P.hit_T0 = false;
   CellOpts<N> co;
-  if (!cell_opts<Net, ET>(a.o, co)) {
+  if (!cell_opts<Net, ET, Prob>(a.o, co)) {
     a.flags[c] = FLAG_UNSUPPORTED;
     return;
   }
   double y[N];
   for (int i = 0; i < NS; ++i) y[i] = a.n[c * a.n_cs + i * a.n_ss];
   if (ET) y[N - 1] = a.T[c];
-  Cell<N, Prob> cv(P, work + c, iwork + c, S, co);
+  Cell<N, Prob> cv(P, work, iwork, S, co);
+  if constexpr (Prob::STRIDED) {
+    P.k = SView{cv.rates_rows(), S};
+    P.k_set = false;
+  }
   cv.reinit(0.0, y, 0.0);
   CellStats tot;
   tot.zero();
@@ -149,6 +164,49 @@ plain_cells_kernel(CellsArgs a, double* work, int* iwork, long S) {
     sc[8] = restarts;
   }
   if (flag == 0) cell_finish<Net, ET>(a, c, y);
+}
+
+template <class Net, bool ET>
+__global__ void __launch_bounds__(128)
+plain_cells_kernel(CellsArgs a, double* work, int* iwork, long S) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.ncell) return;
+  plain_cell<Net, ET>(a, c, work + c, iwork + c, S);
+}
+
+// The persistent form of the one-thread-per-cell solve, for the networks
+// too large for the lockstep kernel's registers (more than
+// KX_PERSIST_MAX_N equations): every lane takes the next cell of a device
+// work queue (one atomic per cell, over a permutation that hands out the
+// densest, costliest cells first), solves it with plain_cell in its own
+// workspace slot, and takes another until the queue is exhausted.  A lane
+// that finishes early takes the next cell at once, so the lanes do not
+// wait for the slowest cell of their first batch; the launch holds only
+// the resident lanes' workspaces (slot = the lane's global index, stride
+// S = the launch's lanes), whatever the number of cells.
+struct LaneQueue {
+  int* counter;       // the next queue position (zeroed before the launch)
+  const int* order;   // queue position -> cell
+};
+
+template <class Net, bool ET>
+__device__ inline void lane_queue_run(const CellsArgs& a, LaneQueue q,
+                                      double* work, int* iwork, long S,
+                                      long slot) {
+  for (;;) {
+    const int i = atomicAdd(q.counter, 1);
+    if (i >= a.ncell) break;
+    plain_cell<Net, ET>(a, q.order[i], work + slot, iwork + slot, S);
+  }
+}
+
+template <class Net, bool ET>
+__global__ void __launch_bounds__(128)
+lane_queue_kernel(CellsArgs a, LaneQueue q, double* work, int* iwork,
+                  long S) {
+  const long slot = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= S) return;
+  lane_queue_run<Net, ET>(a, q, work, iwork, S, slot);
 }
 #endif
 

@@ -4,6 +4,7 @@
 // jac_cell_ 1122-1132) over a generated network Net, with the state
 // y = [n_0 .. n_{NSPEC-1} | Tgas (when evolved)] in registers.
 #pragma once
+#include "kx_cell.h"
 #include "kx_physics.h"
 
 namespace kx {
@@ -34,6 +35,11 @@ struct Problem {
   using NetT = Net;
   static constexpr int NS = Net::NSPEC;
   static constexpr int N = NS + (ET ? 1 : 0);
+  // the state and the rhs in registers (N values), the saved Jacobian
+  // dense (N x N), the rates evaluated at every rhs
+  static constexpr bool STRIDED = false;
+  static constexpr long J_DOUBLES = (long)N * N;
+  static constexpr long K_DOUBLES = 0;
   // doubles of the Newton matrix's storage: N x N dense, or the sparse
   // factors, their row accumulator, the border's two blocks and its Schur
   // complement (m = N - NS border equations)
@@ -84,5 +90,75 @@ struct Problem {
     return kx::finite(x);
   }
 };
+
+// The problem of a large network (more than KX_BIG_N species: cloud's
+// 452), whose vectors do not fit a lane's registers: the state, the rhs
+// and the Jacobian stay in the cell's interleaved workspace and are read
+// and written through strided views.  Tgas is fixed and no rate reads a
+// density (the networks this form serves), so the rates are evaluated once
+// per solve into the cell's rows k (the reference's kfix cache, runtime/
+// core/solve_cell.hpp:594-607, 1302-1308) and the analytic species
+// Jacobian is written straight into the sparse LU's pattern (the
+// generated jac_species_sparse): J_DOUBLES = the pattern's nonzeros, no
+// dense N x N matrix; the Newton matrix is the sparse factors and their
+// row accumulator.
+template <class Net>
+struct BigProblem {
+  using NetT = Net;
+  static constexpr int NS = Net::NSPEC;
+  static constexpr int N = NS;
+  static constexpr bool STRIDED = true;
+  static constexpr long J_DOUBLES = Net::SP_NNZ;
+  static constexpr long NEWTON_DOUBLES = Net::SP_NNZ + NS;
+  static constexpr long K_DOUBLES = Net::NREA;
+  static_assert(!Net::RATES_DENSITY,
+                "the strided form caches the rates: none may read n");
+  typename Net::Params p;
+  double Tfix;
+  bool hit_T0;
+  SView k;       // the cell's rates rows (set by the kernel)
+  bool k_set;    // evaluated for this solve
+
+  KX_HD double rate_T() const { return ::fmin(::fmax(Tfix, p.Tcmb), 1e9); }
+  // once per solve, so a call rather than a copy at each rhs site: the
+  // thousands of rate expressions are compiled once
+  KX_HD_CALL void rates() {
+    for (int r = 0; r < Net::NREA; ++r) k[r] = 0.0;
+    Net::rates_T(rate_T(), p, k);
+    k_set = true;
+  }
+  // the rhs at y into f; 1 (recoverable) for a non-finite result
+  KX_HD int rhs_s(SView y, SView f) {
+    if (!k_set) rates();
+    Net::rhs_loop_v(y, k, f);
+    bool fin = true;
+    for (int i = 0; i < N; ++i) fin = fin && kx::finite(f[i]);
+    return fin ? 0 : 1;
+  }
+  // the analytic Jacobian at y into the pattern's values Jp
+  KX_HD int jac_s(SView y, SView Jp) {
+    if (!k_set) rates();
+    for (long q = 0; q < J_DOUBLES; ++q) Jp[q] = 0.0;
+    Net::jac_species_sparse(y, k, Jp);
+    return 0;
+  }
+};
+
+// the problem type the plain kernel solves a network with
+template <class Net, bool ET, bool BIG = (Net::NSPEC > KX_BIG_N)>
+struct PlainProblemOf {
+  using type = Problem<Net, ET>;
+};
+template <class Net>
+struct PlainProblemOf<Net, false, true> {
+  using type = BigProblem<Net>;
+};
+// (a large network with Tgas evolved: refused before any launch)
+template <class Net>
+struct PlainProblemOf<Net, true, true> {
+  using type = BigProblem<Net>;
+};
+template <class Net, bool ET>
+using PlainProblem = typename PlainProblemOf<Net, ET>::type;
 
 }  // namespace kx

@@ -35,13 +35,13 @@ _L = ctypes.POINTER(ctypes.c_long)
 _ENUMS = {"jac": ("auto", "analytic", "fd"),
           "linsol": ("auto", "dense", "lapack", "sparse"),
           "path": ("auto", "percell", "device"),
-          "device_kernel": ("plain", "persistent")}
+          "device_kernel": ("plain", "persistent", "lanes")}
 _KEYS = ("evolve_T", "jac", "linsol", "rtol", "atol_n", "atol_T", "atol_u",
          "max_steps", "max_err_fails", "max_conv_fails", "max_restarts",
          "clip_negative", "error_report", "renormalize_x", "conserve",
          "path", "device_cache", "device_memory", "min_device",
          "min_interleave", "device_kernel", "mixed_precision",
-         "queue_chunk", "waves_per_cu", "checkpoint_steps",
+         "queue_chunk", "waves_per_cu", "lane_block", "checkpoint_steps",
          "checkpoint_memory")
 LAYOUT_SPECIES_MAJOR = 0
 LAYOUT_CELL_MAJOR = 1
@@ -134,13 +134,15 @@ class _MultiBatch(ctypes.Structure):
                 ("dt", ctypes.c_void_p), ("par", ctypes.c_void_p),
                 ("flags", ctypes.c_void_p), ("evolve_T", ctypes.c_int),
                 ("rtol", ctypes.c_double), ("atol_n", ctypes.c_double),
-                ("atol_T", ctypes.c_double), ("max_steps", ctypes.c_long)]
+                ("atol_T", ctypes.c_double), ("max_steps", ctypes.c_long),
+                ("jac", ctypes.c_int), ("stats", ctypes.c_void_p)]
 
 
-def load_multi(path=None):
-    """the library of several networks in one persistent launch
-    (libkrome_multi.so, include/krome_multi.h)"""
-    p = Path(path) if path else lib_path("multi")
+def load_multi(name="multi", path=None):
+    """a library of several networks in one persistent launch
+    (libkrome_<name>.so, include/krome_multi.h): "multi" (primordial,
+    primordial3) or "multi_c3" (primordial, primordialZ, cloud)"""
+    p = Path(path) if path else lib_path(name)
     if not p.exists():
         raise KromeError("%s is not built (python3 krome-x_amd/build.py)" % p)
     try:
@@ -160,18 +162,33 @@ class Multi:
         lib.krome_multi_nnet.restype = ctypes.c_int
         lib.krome_multi_network_name.restype = ctypes.c_char_p
         lib.krome_multi_network_name.argtypes = [ctypes.c_int]
+        for f in ("evolve_T", "lockstep", "nspec", "npar"):
+            getattr(lib, "krome_multi_" + f).restype = ctypes.c_int
+            getattr(lib, "krome_multi_" + f).argtypes = [ctypes.c_int]
+        lib.krome_multi_set_launch.restype = ctypes.c_int
+        lib.krome_multi_set_launch.argtypes = [ctypes.c_int, ctypes.c_int]
         lib.krome_multi_solve_device.restype = ctypes.c_int
         lib.krome_multi_solve_device.argtypes = [
             ctypes.POINTER(_MultiBatch), ctypes.c_int, ctypes.c_int, _D]
+        count = lib.krome_multi_nnet()
         self.networks = [lib.krome_multi_network_name(i).decode()
-                         for i in range(lib.krome_multi_nnet())]
+                         for i in range(count)]
+        self.evolve_T = [lib.krome_multi_evolve_T(i) for i in range(count)]
+        self.lockstep = [lib.krome_multi_lockstep(i) for i in range(count)]
+
+    def set_launch(self, waves_per_cu=0, queue_chunk=0):
+        """the launch shape of the following solves (0: the defaults)"""
+        if self._lib.krome_multi_set_launch(int(waves_per_cu),
+                                            int(queue_chunk)) != 0:
+            raise KromeError("launch shape out of range")
 
     def solve_device(self, batches, mixed_precision=1):
         """batches: dicts of network (name), n [nspec, ncell], Tgas, dt
         [ncell], par [ncell, npar], flags (int32 [ncell]), all contiguous
         float64 torch tensors on the GPU (updated in place), and evolve_T,
-        rtol, atol_n, atol_T, max_steps.  Returns (the C function's flag,
-        the launch's kernel ms)"""
+        rtol, atol_n, atol_T, max_steps, jac, and optionally stats (int64
+        [ncell, 9] on the GPU: the cells' counters).  Returns (the C
+        function's flag, the launch's kernel ms)"""
         arr = (_MultiBatch * len(batches))()
         for k, b in enumerate(batches):
             for key in ("n", "Tgas", "dt", "par", "flags"):
@@ -185,7 +202,9 @@ class Multi:
                 b["par"].data_ptr(), b["flags"].data_ptr(),
                 int(b.get("evolve_T", 0)), float(b.get("rtol", 1e-8)),
                 float(b.get("atol_n", 1e-20)), float(b.get("atol_T", 1e-6)),
-                int(b.get("max_steps", 100000)))
+                int(b.get("max_steps", 100000)), int(b.get("jac", 0)),
+                b["stats"].data_ptr() if b.get("stats") is not None
+                else None)
         ms = ctypes.c_double()
         r = self._lib.krome_multi_solve_device(
             arr, len(batches), int(mixed_precision),

@@ -449,6 +449,79 @@ def loop_tables(net):
     return T
 
 
+# networks with more species than this are solved through strided views
+# of the interleaved workspace (their vectors do not fit a lane's
+# registers): the generator adds the forms below
+BIG_N = 64
+
+
+def sparse_positions(net, lt, sp):
+    """for each (Jacobian term, scatter row) pair of the loop tables, in
+    their loop order, the position of its (row, column) in the sparse LU's
+    pattern (L + U in pivot order, sparse_lu_tables)"""
+    pos = {e: k for k, e in enumerate(sp["perm"])}
+    where = {}
+    for k in range(len(sp["perm"])):
+        for p in range(sp["rp"][k], sp["rp"][k + 1]):
+            where[(k, sp["ci"][p])] = p
+    out = []
+    for r in range(len(net.reactions)):
+        for t in range(lt["jt_off"][r], lt["jt_off"][r + 1]):
+            col = pos[lt["jt_col"][t]]
+            for q in range(lt["sc_off"][r], lt["sc_off"][r + 1]):
+                key = (pos[lt["sc_row"][q]], col)
+                if key not in where:
+                    raise KnetError("Jacobian entry %s outside the sparse "
+                                    "pattern" % (key,))
+                out.append(where[key])
+    return out
+
+
+def _emit_big_forms(w, net, lt, sp):
+    """the rhs and the analytic species Jacobian of a large network over
+    any indexable state, rates and outputs (pointers, or strided views of
+    the interleaved workspace); the Jacobian written straight into the
+    sparse LU's pattern positions"""
+    jp = sparse_positions(net, lt, sp)
+    w("  // the rhs over indexable n, k, dn (same operations as rhs_loop)")
+    w("  template <class NV, class KV, class DV>")
+    w("  KX_HD static void rhs_loop_v(const NV& n, const KV& k, "
+      "const DV& dn) {")
+    w("    for (int i = 0; i < NSPEC; ++i) dn[i] = 0.0;")
+    w("    for (int r = 0; r < NREA; ++r) {")
+    w("      double fl = k[r];")
+    w("      for (int p = LT_KF_OFF[r]; p < LT_KF_OFF[r + 1]; ++p)")
+    w("        fl = fl * n[LT_KF_SP[p]];")
+    w("      for (int q = LT_SC_OFF[r]; q < LT_SC_OFF[r + 1]; ++q) {")
+    w("        if (LT_SC_SGN[q] > 0) dn[LT_SC_ROW[q]] += fl;")
+    w("        else dn[LT_SC_ROW[q]] -= fl;")
+    w("      }")
+    w("    }")
+    w("  }")
+    w("  // the species Jacobian at fixed rates added into Jp, indexed by")
+    w("  // the sparse LU's pattern position (LT_JP: one per term and")
+    w("  // scatter row, in the loop order of jac_species_loop, which it")
+    w("  // follows operation for operation)")
+    w("  static constexpr int LT_JP[%d] = {%s};"
+      % (max(len(jp), 1), ", ".join(str(v) for v in jp) or "0"))
+    w("  template <class NV, class KV, class JV>")
+    w("  KX_HD static void jac_species_sparse(const NV& n, const KV& k, "
+      "const JV& Jp) {")
+    w("    int m = 0;")
+    w("    for (int r = 0; r < NREA; ++r)")
+    w("      for (int t = LT_JT_OFF[r]; t < LT_JT_OFF[r + 1]; ++t) {")
+    w("        double d = k[r];")
+    w("        for (int p = LT_JT_FOFF[t]; p < LT_JT_FOFF[t + 1]; ++p)")
+    w("          d = d * n[LT_JF_SP[p]];")
+    w("        if (LT_JT_MULT[t] > 1) d = d * double(LT_JT_MULT[t]);")
+    w("        for (int q = LT_SC_OFF[r]; q < LT_SC_OFF[r + 1]; ++q, ++m) {")
+    w("          if (LT_SC_SGN[q] > 0) Jp[LT_JP[m]] += d;")
+    w("          else Jp[LT_JP[m]] -= d;")
+    w("        }")
+    w("      }")
+    w("  }")
+
+
 def _group(net):
     """tile the ranged lines of identical stoichiometry into piecewise
     groups (network-format.md 3.1); an unranged line is its own group"""
@@ -1319,10 +1392,16 @@ def generate(knet_path, thermal=None, name=None, data_dir=None):
         out.append("%s}" % indent)
         return out
 
+    big = net.nspec > BIG_N
     w("  // rates of the rate temperature Trate (and parameters); the")
-    w("  // caller zeroes k")
-    w("  template <class R>")
-    w("  KX_HD static void rates_T(R Trate, const Params& p, R* k) {")
+    w("  // caller zeroes k%s" % (" (any indexable k: a pointer, or a view of"
+                                " an interleaved workspace)" if big else ""))
+    if big:
+        w("  template <class R, class KV>")
+        w("  KX_HD static void rates_T(R Trate, const Params& p, KV k) {")
+    else:
+        w("  template <class R>")
+        w("  KX_HD static void rates_T(R Trate, const Params& p, R* k) {")
     w("    (void)p;")
     w("    const R Tgas = Trate;")
     w("    (void)Tgas;")
@@ -1402,6 +1481,8 @@ def generate(knet_path, thermal=None, name=None, data_dir=None):
     w("        }")
     w("      }")
     w("  }")
+    if big:
+        _emit_big_forms(w, net, lt, sp)
     if not loop_only:
         w("  template <class R>")
         w("  KX_HD static void rhs_unrolled(const R* n, const R* k, R* dn) {")
@@ -1809,6 +1890,10 @@ CONFIGS = {
     "galaxy_ism_heat": ("galaxy_ism.knet",
                         Thermal(cooling=("H2", "CHEM"),
                                 heating=("CHEM", "PHOTO"))),
+    # the uneven batch's third network: Herbst's molecular-cloud chemistry,
+    # 452 species (GRAIN0 and GRAIN- opaque) and 4429 reactions, the loop
+    # form of the rhs and the static-pivot sparse LU; Tgas fixed
+    "cloud": ("cloud.knet", Thermal()),
 }
 # the other shipped networks this generator covers, at the default thermal
 # options (Tgas fixed), each pinned against the reference's generated code

@@ -48,6 +48,7 @@
 #include "net_xrays.h"
 #include "net_primordialZ_UV.h"
 #include "net_photobin.h"
+#include "net_cloud.h"
 #include "kx_conserve.hpp"
 #include "kx_photo.hpp"
 
@@ -326,12 +327,17 @@ void forms(int ncell, const double* n, const double* T, const double* par,
     const double* y = n + static_cast<size_t>(c) * NS;
     double k[Net::NREA + 1];
     P.rates(P.rate_T(y), y, k);
-    Net::rhs_unrolled(y, k, f + static_cast<size_t>(c) * NS);
-    Net::rhs_loop(y, k, f + nf + static_cast<size_t>(c) * NS);
     double* Ju = J + static_cast<size_t>(c) * NS * NS;
     double* Jl = J + nj + static_cast<size_t>(c) * NS * NS;
     for (int q = 0; q < NS * NS; ++q) Ju[q] = Jl[q] = 0.0;
-    Net::jac_species_unrolled(y, k, Ju, NS);
+    if constexpr (Net::RHS_LOOP) {  // no unrolled form above 500 reactions
+      Net::rhs_loop(y, k, f + static_cast<size_t>(c) * NS);
+      Net::jac_species_loop(y, k, Ju, NS);
+    } else {
+      Net::rhs_unrolled(y, k, f + static_cast<size_t>(c) * NS);
+      Net::jac_species_unrolled(y, k, Ju, NS);
+    }
+    Net::rhs_loop(y, k, f + nf + static_cast<size_t>(c) * NS);
     Net::jac_species_loop(y, k, Jl, NS);
   }
 }
@@ -344,7 +350,7 @@ void forms(int ncell, const double* n, const double* T, const double* par,
   X(9, primordial2) X(10, primordial_wD) X(11, primordial_photo)           \
   X(12, primordial_photoH2) X(13, auto) X(14, chianti) X(15, ions)         \
   X(16, ionsH2) X(17, primordial_UV) X(18, xrays) X(19, primordialZ_UV)  \
-  X(20, photobin)
+  X(20, photobin) X(21, cloud)
 
 // f(Net{}) for network id net, or -1
 template <class F>

@@ -21,7 +21,7 @@ NET_ID = {"primordial": 0, "primordial3": 1, "galaxy_ism": 2,
           "primordial_photo": 11, "primordial_photoH2": 12, "auto": 13,
           "chianti": 14, "ions": 15, "ionsH2": 16, "primordial_UV": 17,
           "xrays": 18, "primordialZ_UV": 19,
-          "photobin": 20}
+          "photobin": 20, "cloud": 21}
 
 
 def pytest_configure(config):

@@ -35,8 +35,10 @@ For each workload it:
      with per-species atol only, then with per-species atol and rtol (some
      entries negative: the scalar) -> <name>_tol.npz.
 
+  9. the uneven batch's cloud network (see make_cloud) -> cloud_*.npz.
+
   python3 tests/golden/make_golden.py [solve] [funcs] [conserve] [sparse]
-  [photo] [extra] [adjoint] [tol] (default: all but extra)
+  [photo] [extra] [adjoint] [tol] [cloud] (default: all but extra)
 
 The generated project and the binaries stay in tests/golden/_build/ (git-
 ignored, not committed).
@@ -74,12 +76,14 @@ TOML = {
                        'outdir = "{out}"\n[thermal]\n'
                        'cooling = ["H2", "CHEM"]\n'
                        'heating = ["CHEM", "PHOTO"]\n',
+    "cloud": '[project]\nnetwork = "{net}/cloud.knet"\noutdir = "{out}"\n',
 }
 # cells of the solve fixtures (the first ones of the seeded workload)
-NSOLVE = {"primordial": 1024, "primordial3": 256, "primordialZ": 256}
+NSOLVE = {"primordial": 1024, "primordial3": 256, "primordialZ": 256,
+          "cloud": 64}
 # the workload of a network's fixtures (cells.WORKLOADS)
 WORKLOAD = {"primordial": "primordial", "primordial3": "primordial3",
-            "primordialZ": "c3_primordialZ"}
+            "primordialZ": "c3_primordialZ", "cloud": "c3_cloud"}
 OPT_KEYS = ("evolve_T", "rtol", "atol_n", "atol_T", "linsol", "jac",
             "max_steps")
 T_GRID = (12.0, 97.0, 314.0, 1.1e3, 5.5e3, 2.3e4, 1.7e5, 3e6, 8e7)
@@ -358,6 +362,54 @@ def make_tol(name, exe, meta):
     np.savez_compressed(HERE / ("%s_tol.npz" % name), **out)
 
 
+def make_cloud(exes, meta):
+    """the uneven batch's cloud network (452 species, 4429 reactions):
+    its sparse-LU tables and element tables, the rates, full rhs and
+    thermal terms at 16 workload cells and 9 cells over the T grid, the
+    analytic Jacobian at the first 4 of them (J is 452^2 per sample), and
+    solve_cells of the first NSOLVE cells (both builds) with linsol auto
+    (sparse here) -> cloud_funcs.npz, cloud_solve.npz, cloud_sparse.npz,
+    cloud_tables.npz"""
+    name = "cloud"
+    gen = cells.WORKLOADS[WORKLOAD[name]]
+    w = gen(16)
+    base = gen(len(T_GRID), first=5000)
+    fw = {"n": np.concatenate([w["n"], base["n"]]),
+          "T": np.concatenate([w["T"], np.array(T_GRID)]),
+          "par": np.concatenate([w["par"], base["par"]]),
+          "opts": w["opts"]}
+    fw["dt"] = np.ones(fw["n"].shape[0])
+    ns = len(meta["species"])
+    k, f, J, th = run_funcs(exes["bitwise"], fw, name + "_funcs",
+                            meta["nrea"], ns)
+    np.savez_compressed(HERE / "cloud_funcs.npz", n=fw["n"], T=fw["T"],
+                        par=fw["par"], k=k, f=f, J=J[:4], thermal=th)
+    print("cloud funcs: %d samples" % k.shape[0])
+    np.savez_compressed(HERE / "cloud_tables.npz",
+                        **ref_tables(exes["bitwise"], name, w))
+    raw, _ = run_mode(exes["bitwise"], "sparse", w, "cloud_sparse_tab", [0.0])
+    nnz = int(raw[0])
+    o = 2
+    tab = {}
+    for key, m in (("perm", ns), ("rp", ns + 1), ("ci", nnz), ("di", ns)):
+        tab["sp_" + key] = raw[o:o + m].astype(np.int64)
+        o += m
+    np.savez_compressed(HERE / "cloud_sparse.npz", sparse_flops=raw[1],
+                        **tab)
+    sw = gen(NSOLVE[name])
+    res = {}
+    for m, exe in exes.items():
+        n, T, flag, stats, secs = run_solve(exe, sw, "cloud_%s" % m)
+        res.update({"n_" + m: n, "T_" + m: T, "flag_" + m: flag,
+                    "stats_" + m: stats, "seconds_" + m: secs})
+        print("cloud %s: %d cells in %.2f s, flags %s, steps max %d" % (
+            m, n.shape[0], secs, np.unique(flag), stats[:, 0].max()))
+    np.savez_compressed(HERE / "cloud_solve.npz", n0=sw["n"], T0=sw["T"],
+                        dt=sw["dt"], par=sw["par"],
+                        opts=np.array([float(sw["opts"][k])
+                                       for k in OPT_KEYS]), **res)
+
+
 # photon bins: cells of the photo solve fixture, the energies at which the
 # cross sections are sampled, the in-bin spectra of the bin tables
 NPHOTO_SOLVE = 256
@@ -592,10 +644,15 @@ def main(argv=None):
     only = {a for a in argv if a in names}
     argv = [a for a in argv if a not in names]
     parts = set(argv if argv else ("solve", "funcs", "conserve", "sparse",
-                                   "photo", "adjoint", "tol"))
+                                   "photo", "adjoint", "tol", "cloud"))
     BUILD.mkdir(parents=True, exist_ok=True)
     if "adjoint" in parts and (not only or "primordial3" in only):
         make_adjoint()
+    if "cloud" in parts:
+        proj = generate("cloud")
+        make_cloud({m: compile_driver(proj, "cloud", m)
+                    for m in ("bitwise", "production")},
+                   cells.network_meta("cloud"))
     for name in ("galaxy_ism", "galaxy_ism_heat"):
         if "photo" not in parts or (only and name not in only):
             continue

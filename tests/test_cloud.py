@@ -1,0 +1,205 @@
+"""The uneven batch's third network, cloud (452 species with the opaque
+GRAIN0 and GRAIN-, 4429 reactions: networks/cloud.knet; opaque species
+per docs/network-format.md:60-66 and kromex/knet.py:98-126), pinned
+against the reference's own generator and solver (tests/golden/
+cloud_*.npz, written by tests/golden/make_golden.py cloud):
+
+- the generated tables: species, charges, element compositions and
+  masses, the static-pivot sparse LU's tables (pivot order, rows of L + U
+  with the fill, diagonals, flops) equal the reference generator's;
+- the position table of the sparse Jacobian (LT_JP): every (term,
+  scatter row) pair of the loop tables lands on the pattern entry of its
+  own (row, column), decoded through the reference's tables;
+- rates within 1e-9 relative (the same zeros), the rhs within 1e-10 of
+  its row scale, the analytic Jacobian within 1e-12 of its scale;
+- whole solves: the oracle against the reference's solve_cells (64 cells
+  of the uneven batch, rtol 1e-6, linsol auto = the sparse LU), equal
+  flags, 3e-4 (the uneven batch's bar, as primordialZ; 1e-4 above 1e-10
+  of the cell total); the oracle's
+  restatement bitwise against the reference's integrator on 8 cells;
+- GPU: the plain kernel's strided form (state, rhs and the sparse
+  Jacobian in the interleaved workspace, the rates cached per solve)
+  against the reference's solves and the oracle.
+"""
+import numpy as np
+import pytest
+
+import cells
+import krome
+from conftest import golden, oracle_opts, rel_err
+
+NET = "cloud"
+# the uneven batch's bar (rtol 1e-6, long integrations: up to 6843 steps),
+# as primordialZ's: last-bit differences of the rate expressions move
+# trace species (one cell's species at 1e-11 of the cell total by 2.7e-4);
+# above 1e-10 of the cell total, 1e-4
+TOL = 3e-4
+TOL_MAJOR = 1e-4
+
+
+def _opts(d):
+    o = d["opts"]
+    return oracle_opts(evolve_T=o[0], rtol=o[1], atol_n=o[2], atol_T=o[3],
+                       linsol=o[4], jac=o[5], max_steps=o[6])
+
+
+def _generated_table(name):
+    """a constexpr int table of the generated header"""
+    from pathlib import Path
+    text = (Path(__file__).resolve().parent.parent / "krome-x_amd" / "csrc"
+            / "generated" / "net_cloud.h").read_text()
+    key = "static constexpr int %s[" % name
+    i = text.index(key)
+    body = text[text.index("{", i) + 1:text.index("}", i)]
+    return np.array([int(v) for v in body.split(",")])
+
+
+def test_tables_equal_the_reference_generator():
+    meta = cells.network_meta(NET)
+    t = golden("cloud_tables.npz")
+    assert list(t["element_names"]) == meta["elements"]
+    assert np.array_equal(t["species_charge"], meta["charge"])
+    assert np.allclose(t["species_mass_g"], meta["mass_g"], rtol=1e-12,
+                       atol=0)
+    assert np.array_equal(t["species_elements"], np.array(meta["natoms"]))
+    # the opaque grains: no atoms, the grain anion one electron's mass
+    sp = meta["species"]
+    g0, gm = sp.index("GRAIN0"), sp.index("GRAIN-")
+    assert not np.array(meta["natoms"])[[g0, gm]].any()
+    assert meta["mass_g"][g0] == 0.0 and meta["mass_g"][gm] > 0.0
+    s = golden("cloud_sparse.npz")
+    for k in ("perm", "rp", "ci", "di"):
+        assert np.array_equal(_generated_table("sp_" + k), s["sp_" + k]), k
+    assert int(s["sparse_flops"]) == meta["sparse"]["flops"]
+
+
+def test_sparse_jacobian_positions():
+    """LT_JP against the reference's pattern: each (term, scatter row)
+    pair of the loop tables, in loop order, names the position whose
+    (row, column) the reference's tables decode to that pair's"""
+    s = golden("cloud_sparse.npz")
+    perm, rp, ci = s["sp_perm"], s["sp_rp"], s["sp_ci"]
+    row_of = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    jp = _generated_table("LT_JP")
+    jt_off, jt_col = _generated_table("LT_JT_OFF"), _generated_table(
+        "LT_JT_COL")
+    sc_off, sc_row = _generated_table("LT_SC_OFF"), _generated_table(
+        "LT_SC_ROW")
+    rows, cols = [], []
+    for r in range(len(jt_off) - 1):
+        for t in range(jt_off[r], jt_off[r + 1]):
+            for q in range(sc_off[r], sc_off[r + 1]):
+                rows.append(sc_row[q])
+                cols.append(jt_col[t])
+    assert len(jp) == len(rows)
+    assert np.array_equal(perm[row_of[jp]], rows)
+    assert np.array_equal(perm[ci[jp]], cols)
+
+
+def test_functions_vs_reference_generated_code(oracle):
+    g = golden("cloud_funcs.npz")
+    ns = g["n"].shape[1]
+    k, f = oracle.funcs(NET, g["n"], g["T"], g["par"], 0)
+    kr = g["k"]
+    assert np.array_equal(k == 0.0, kr == 0.0)
+    nz = kr != 0.0
+    assert np.max(np.abs(k[nz] - kr[nz]) / np.abs(kr[nz])) < 1e-9
+    fr = g["f"][:, :ns]
+    scale = np.abs(fr).max(axis=1, keepdims=True)
+    assert np.max(np.abs(f[:, :ns] - fr) / scale) < 1e-10
+
+
+def test_jacobian_vs_reference(oracle):
+    import ctypes
+    g = golden("cloud_funcs.npz")
+    m = g["J"].shape[0]
+    n, T, par = g["n"][:m], g["T"][:m], g["par"][:m]
+    ns = n.shape[1]
+    J = np.zeros((m, ns * ns))
+    D = ctypes.POINTER(ctypes.c_double)
+    oracle.lib.kxo_jac(21, m, np.ascontiguousarray(n).ctypes.data_as(D),
+                       np.ascontiguousarray(T).ctypes.data_as(D),
+                       np.ascontiguousarray(par).ctypes.data_as(D),
+                       J.ctypes.data_as(D))
+    J = J.reshape(m, ns, ns).transpose(0, 2, 1)
+    Jr = g["J"]
+    scale = np.abs(Jr).max(axis=(1, 2), keepdims=True)
+    assert np.max(np.abs(J - Jr) / scale) < 1e-12
+
+
+def test_oracle_vs_reference_solves(oracle):
+    d = golden("cloud_solve.npz")
+    r, n, T, fl, st = oracle.solve(NET, d["n0"], d["T0"], d["dt"], d["par"],
+                                   _opts(d))
+    assert np.array_equal(fl, d["flag_bitwise"])
+    assert rel_err(n, d["n_bitwise"]) < TOL
+    assert rel_err(n, d["n_bitwise"], 1e-10) < TOL_MAJOR
+    s, sr = st[:, 0].sum(), d["stats_bitwise"][:, 0].sum()
+    assert abs(s - sr) < 0.05 * sr
+
+
+def test_oracle_bitwise_vs_reference_integrator(oracle, oracle_ref):
+    d = golden("cloud_solve.npz")
+    sl = slice(0, 8)
+    o = _opts(d)
+    a = oracle.solve(NET, d["n0"][sl], d["T0"][sl], d["dt"][sl],
+                     d["par"][sl], o)
+    b = oracle_ref.solve(NET, d["n0"][sl], d["T0"][sl], d["dt"][sl],
+                         d["par"][sl], o)
+    assert a[0] == b[0] == 0
+    for x, y in zip(a[1:], b[1:]):
+        assert np.array_equal(x, y)
+
+
+@pytest.mark.gpu
+def test_gpu_plain_kernel_vs_reference(oracle):
+    d = golden("cloud_solve.npz")
+    net = krome.load(NET)
+    o = d["opts"]
+    opts = net.options(evolve_T=0, rtol=o[1], atol_n=o[2], atol_T=o[3],
+                       linsol=0, jac=0, max_steps=int(o[6]),
+                       device_kernel="plain")
+    print("cloud: plain kernel, %d cells" % d["n0"].shape[0], flush=True)
+    res = net.solve(d["n0"], d["T0"], d["dt"], par=d["par"], opts=opts,
+                    check=False)
+    assert res.ierr == 0
+    assert np.array_equal(res.flags, d["flag_bitwise"])
+    assert rel_err(res.n, d["n_bitwise"]) < TOL
+    assert rel_err(res.n, d["n_bitwise"], 1e-10) < TOL_MAJOR
+    _, n, _, _, _ = oracle.solve(NET, d["n0"], d["T0"], d["dt"], d["par"],
+                                 _opts(d))
+    assert rel_err(res.n, n) < TOL
+
+
+@pytest.mark.gpu
+def test_gpu_refusals():
+    """the strided form serves Tgas fixed with the analytic Jacobian and
+    the sparse LU; the rest is refused before any work"""
+    d = golden("cloud_solve.npz")
+    net = krome.load(NET)
+    for kw in ({"jac": 2}, {"linsol": 1}):
+        opts = net.options(rtol=1e-6, **kw)
+        res = net.solve(d["n0"][:4], d["T0"][:4], d["dt"][:4],
+                        par=d["par"][:4], opts=opts, check=False)
+        assert res.ierr == -104 and (res.flags == -104).all(), kw
+        assert np.array_equal(res.n, d["n0"][:4])
+
+
+@pytest.mark.gpu
+def test_gpu_lane_queue_equals_plain():
+    """device_kernel persistent: above 16 equations, the lane queue (each
+    lane solves cells with the plain kernel's code, taking the next from
+    the device queue): bitwise the plain kernel's results"""
+    d = golden("cloud_solve.npz")
+    net = krome.load(NET)
+    o = d["opts"]
+    res = {}
+    for kern in ("plain", "persistent"):
+        opts = net.options(evolve_T=0, rtol=o[1], atol_n=o[2], atol_T=o[3],
+                           max_steps=int(o[6]), device_kernel=kern)
+        print("cloud:", kern, flush=True)
+        res[kern] = net.solve(d["n0"], d["T0"], d["dt"], par=d["par"],
+                              opts=opts, check=False)
+    assert res["persistent"].ierr == 0
+    assert np.array_equal(res["persistent"].flags, res["plain"].flags)
+    assert np.array_equal(res["persistent"].n, res["plain"].n)

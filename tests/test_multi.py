@@ -177,3 +177,88 @@ def test_pair_launch_unset_parameter():
     f = bs[0]["flags"].cpu().numpy()
     assert f[7] == -105 and (np.delete(f, 7) == 0).all()
     assert torch.equal(bs[0]["n"], n0)
+
+
+# ---- the uneven batch's three networks in one launch (libkrome_multi_c3) --
+
+SO_C3 = PKG / "libkrome_multi_c3.so"
+
+
+def lib_c3():
+    if not SO_C3.exists():
+        pytest.skip("%s not built" % SO_C3.name)
+    import krome
+    return krome.load_multi("multi_c3")
+
+
+def test_c3_exports_and_layout():
+    m = lib_c3()
+    text = (REPO / "include" / "krome_multi.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    declared = set(re.findall(r"\b(krome_multi_[a-z_]+)\s*\(", text))
+    out = subprocess.run(["nm", "-D", "--defined-only", str(SO_C3)],
+                         capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (krome_multi_\w+)", out))
+    assert declared <= exported, declared - exported
+    assert m.networks == ["primordial", "primordialZ", "cloud"]
+    # primordial (12 equations) in the lockstep kernel, the larger two in
+    # the lane queue; every network here at fixed Tgas
+    assert m.lockstep == [1, 0, 0] and m.evolve_T == [0, 0, 0]
+    with pytest.raises(Exception):
+        m.set_launch(-1, 0)
+
+
+def _golden_batch(name):
+    import torch
+    from conftest import golden
+    d = golden("%s_solve.npz" % name)
+    o = d["opts"]
+    return d, dict(
+        network=name, evolve_T=int(o[0]), rtol=o[1], atol_n=o[2],
+        atol_T=o[3], max_steps=int(o[6]), jac=int(o[5]),
+        n=torch.tensor(np.ascontiguousarray(d["n0"].T), device="cuda"),
+        Tgas=torch.tensor(d["T0"], device="cuda"),
+        dt=torch.tensor(d["dt"], device="cuda"),
+        par=torch.tensor(np.ascontiguousarray(d["par"]), device="cuda"),
+        flags=torch.zeros(d["n0"].shape[0], dtype=torch.int32,
+                          device="cuda"))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("wpc,chunk", [(0, 0), (2, 64)])
+def test_c3_launch_vs_reference_and_single(wpc, chunk):
+    """the reference's golden cells of the three networks in one launch:
+    each batch within its network's bar of the reference's solves, flags
+    equal, and bitwise the network's own persistent solve (the lane
+    queue's cells are the plain kernel's solve, the lockstep cells the
+    persistent kernel's)"""
+    import torch
+    import krome
+    from conftest import rel_err
+    m = lib_c3()
+    m.set_launch(wpc, chunk)
+    bars = {"primordial": 5e-5, "primordialZ": 3e-4, "cloud": 1e-4}
+    got = [_golden_batch(n) for n in ("primordial", "primordialZ", "cloud")]
+    singles = []
+    for d, b in got:
+        net = krome.load(b["network"])
+        opts = net.options(evolve_T=False, rtol=b["rtol"], atol_n=b["atol_n"],
+                           atol_T=b["atol_T"], jac=b["jac"],
+                           max_steps=b["max_steps"],
+                           device_kernel="persistent", error_report=False)
+        n, T, fl = b["n"].clone(), b["Tgas"].clone(), b["flags"].clone()
+        r = net.solve_device(n, T, b["dt"], b["par"], opts=opts, flags=fl)
+        singles.append((r, n, fl))
+    say("c3 launch", wpc, chunk)
+    r, ms = m.solve_device([b for _, b in got], mixed_precision=1)
+    m.set_launch(0, 0)
+    torch.cuda.synchronize()
+    assert r == 0 and ms > 0.0
+    for (d, b), (r1, n1, f1) in zip(got, singles):
+        name = b["network"]
+        fl = b["flags"].cpu().numpy()
+        assert np.array_equal(fl, d["flag_bitwise"]), name
+        assert rel_err(b["n"].T.cpu().numpy(), d["n_bitwise"]) < bars[name]
+        assert r1 == 0 and torch.equal(b["flags"], f1), name
+        assert torch.equal(b["n"], n1), name
+    m.set_launch(0, 0)
