@@ -447,16 +447,19 @@ def c3_bench(a):
 
 
 def photo_bench(a):
-    """galaxy_ism with photon bins: three kernels, one JSON line"""
+    """galaxy_ism (or galaxy_ism_heat: the same network with heating CHEM
+    + PHOTO and cooling H2 + CHEM, so Tgas moves) with photon bins: three
+    kernels, one JSON line"""
     import torch
     import cells
     import krome
     sys.path.insert(0, str(REPO / "tests"))
 
     dev = torch.device("cuda:0")
-    net = krome.load("galaxy_ism")
+    name = a.photo_network
+    net = krome.load(name)
     nc = a.ncell
-    w = cells.galaxy_ism_cells(nc)
+    w = cells.galaxy_ism_cells(nc, network=name)
     elo, ehi = w["bins"]
     n0 = torch.tensor(np.ascontiguousarray(w["n"].T), device=dev)
     T0 = torch.tensor(w["T"], device=dev)
@@ -495,6 +498,7 @@ def photo_bench(a):
                      "ms_per_step": 1e3 * wall / a.steps,
                      "kernel_ms": float(np.mean(ms)),
                      "n": n.cpu().numpy().T.copy(),
+                     "T": T.cpu().numpy().copy(),
                      "flags": fl.cpu().numpy().copy()}
         print("photo %s: %.1f ms/step" % (key, runs[key]["ms_per_step"]),
               file=sys.stderr, flush=True)
@@ -514,37 +518,44 @@ def photo_bench(a):
             np.array_equal(r["flags"], runs["plain_fp64"]["flags"]))
         if key != "plain_fp64":
             out_runs[key]["max_rel_err_vs_plain_fp64"] = errs(r["n"])
+            Tp = runs["plain_fp64"]["T"]
+            out_runs[key]["max_rel_err_Tgas_vs_plain_fp64"] = float(
+                np.max(np.abs(r["T"] - Tp) / Tp))
+        out_runs[key]["Tgas_moved_max_rel"] = float(
+            np.max(np.abs(r["T"] - w["T"]) / w["T"]))
     # the oracle on a sample: the CPU baseline and the FP64 check
     from conftest import Oracle, oracle_opts
     orc = Oracle(REPO / "oracle" / "_build" / "libkx_oracle_fast.so", "kxo_")
     cores = min(16, os.cpu_count() or 1)
     orc.lib.kxo_set_threads(cores)
     m = min(a.cpu_sample, nc)
-    sig = orc.photo_bins("galaxy_ism", elo, ehi, 1.0)
-    rows = orc.photo_rows("galaxy_ism", w["par"][:m], w["Phi"][:m], sig)
+    sig = orc.photo_tables(name, elo, ehi, 1.0)
+    rows = orc.photo_rows(name, w["par"][:m], w["Phi"][:m], sig)
     t0 = time.perf_counter()
-    ro, no, _, flo, _ = orc.solve("galaxy_ism", w["n"][:m], w["T"][:m],
-                                  w["dt"][:m], rows,
-                                  oracle_opts(**w["opts"]))
+    ro, no, To, flo, _ = orc.solve(name, w["n"][:m], w["T"][:m],
+                                   w["dt"][:m], rows,
+                                   oracle_opts(**w["opts"]))
     el = time.perf_counter() - t0
     bo = no > 1e-12 * no.sum(axis=1, keepdims=True)
     for key, r in runs.items():
         d = np.abs(r["n"][:m] - no) / np.maximum(np.abs(no), 1e-300)
         out_runs[key]["max_rel_err_vs_oracle_sample"] = float(d[bo].max())
+        out_runs[key]["max_rel_err_Tgas_vs_oracle_sample"] = float(
+            np.max(np.abs(r["T"][:m] - To) / To))
         out_runs[key]["flags_equal_to_oracle_sample"] = bool(
             np.array_equal(r["flags"][:m], flo))
     head = runs["persistent_mixed"]
     out = {
-        "metric": "cells/sec to t_end, galaxy_ism with photon bins "
-                  "(persistent mixed precision)",
+        "metric": "cells/sec to t_end, %s with photon bins "
+                  "(persistent mixed precision)" % name,
         "value": head["cells_per_s"], "unit": "cells/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": head["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32",
         "data": "synthetic",
-        "config": {"workload": "%d galaxy_ism cells (seed 4), 32 photon "
+        "config": {"workload": "%d %s cells (seed 4), 32 photon "
                                "bins 0.7..200 eV, a field per cell, Tgas "
-                               "evolved, rtol 1e-8" % nc,
+                               "evolved, rtol 1e-8" % (nc, name),
                    "ncell": nc, "parallelism": "1 GPU"},
         "kernels": out_runs,
         "mixed_vs_plain_fp64": runs["plain_fp64"]["ms_per_step"]
@@ -725,6 +736,8 @@ def main():
                     choices=("solve", "conserve", "multi", "photo", "network",
                              "copy", "adjoint", "c3"))
     ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--photo-network", default="galaxy_ism",
+                    choices=("galaxy_ism", "galaxy_ism_heat"))
     ap.add_argument("--queue-chunk", type=int, default=0)
     ap.add_argument("--network", default="primordialZ")
     ap.add_argument("--cells", default="c3_primordialZ")

@@ -1810,21 +1810,6 @@ persist_cells_kernel(CellsArgs a, PersistWorkOf<Net, ET, MIXED> w0,
   persist_run<Net, ET, MIXED>(a, w0, q, lds_vec);
 }
 
-// two networks in one launch: every wave runs the first network's cells
-// until that queue is exhausted, then the second's, so that the waves that
-// finish the first early fill the launch with the second instead of
-// leaving its tail to a launch of its own
-template <class NetA, bool ETA, bool MA, class NetB, bool ETB, bool MB>
-__global__ void __launch_bounds__(KX_PERSIST_BLOCK, KX_PERSIST_WAVES_PER_EU)
-persist_pair_kernel(CellsArgs a, PersistWorkOf<NetA, ETA, MA> wa,
-                    PersistQueue qa, CellsArgs b,
-                    PersistWorkOf<NetB, ETB, MB> wb, PersistQueue qb) {
-  constexpr long LA = lds_doubles<NetA, ETA>(), LB = lds_doubles<NetB, ETB>();
-  __shared__ double lds_vec[LA > LB ? LA : LB];
-  persist_run<NetA, ETA, MA>(a, wa, qa, lds_vec);
-  persist_run<NetB, ETB, MB>(b, wb, qb, lds_vec);
-}
-
 // ---- the queue's order: the densest cells first ---------------------------
 // A cell's work grows with its density (its reaction rates, and so its
 // stiffness, scale with it; for primordial3 collapse cells the correlation
@@ -1981,27 +1966,6 @@ int launch_persistent_t(CellsArgs a, int wpc, int chunk, hipStream_t s,
   constexpr long B = KX_PERSIST_BLOCK;
   hipLaunchKernelGGL((persist_cells_kernel<Net, ET, MIXED>),
                      dim3(lanes / B), dim3(B), 0, s, a, w, q);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-// two networks' cells in one persistent launch (persist_pair_kernel): the
-// lanes are those of the larger batch; each network keeps its own
-// workspace and queue, and its own Newton-matrix precision
-template <class NetA, bool ETA, bool MA, class NetB, bool ETB, bool MB>
-int launch_persistent_pair_t(CellsArgs a, CellsArgs b, int wpc, int chunk,
-                             hipStream_t s, PersistMem ma, PersistMem mb) {
-  long lanes = 0;
-  if (persist_lanes(a.ncell > b.ncell ? a.ncell : b.ncell, wpc, lanes) != 0)
-    return -1;
-  PersistWorkOf<NetA, ETA, MA> wa;
-  PersistWorkOf<NetB, ETB, MB> wb;
-  PersistQueue qa, qb;
-  int r = persist_prepare<NetA, ETA, MA>(a, ma, chunk, s, wa, qa);
-  if (r == 0) r = persist_prepare<NetB, ETB, MB>(b, mb, chunk, s, wb, qb);
-  if (r != 0) return r;
-  constexpr long B = KX_PERSIST_BLOCK;
-  hipLaunchKernelGGL((persist_pair_kernel<NetA, ETA, MA, NetB, ETB, MB>),
-                     dim3(lanes / B), dim3(B), 0, s, a, wa, qa, b, wb, qb);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -103,12 +103,12 @@ KX_HD inline void cell_finish(const CellsArgs& a, int c, const double* y) {
   if (ET) a.T[c] = y[NS];
 }
 
-#if defined(__HIPCC__)
 // the whole per-cell solve of cell c by one thread, its workspace at work
-// and iwork (this thread's slot of an interleaved workspace, stride S)
+// and iwork (this thread's slot of an interleaved workspace, stride S);
+// host-callable too (the CPU tests run the device's code with it)
 template <class Net, bool ET>
-__device__ inline void plain_cell(const CellsArgs& a, int c, double* work,
-                                  int* iwork, long S) {
+KX_HD inline void plain_cell(const CellsArgs& a, int c, double* work,
+                             int* iwork, long S) {
   using Prob = PlainProblem<Net, ET>;
   constexpr int NS = Net::NSPEC;
   constexpr int N = Prob::N;
@@ -166,6 +166,7 @@ __device__ inline void plain_cell(const CellsArgs& a, int c, double* work,
   if (flag == 0) cell_finish<Net, ET>(a, c, y);
 }
 
+#if defined(__HIPCC__)
 template <class Net, bool ET>
 __global__ void __launch_bounds__(128)
 plain_cells_kernel(CellsArgs a, double* work, int* iwork, long S) {

@@ -30,13 +30,22 @@ struct SolveOpts {
   int rtol_v = 0;    // the weights 1 / fma(rtol_i, |y_i|, atol_i)
 };
 
-template <class Net, bool ET>
+// KC: keep the rates of the rate temperature from one evaluation to the
+// next (the reference's rate cache: rates_fixed once per solve at fixed
+// Tgas, and per Tgas value when Tgas evolves, so that the difference
+// quotients' species columns and the Newton iterations at an unchanged
+// Tgas skip them; runtime/core/solve_cell.hpp:594-607, 1302-1329).  The
+// cached values are the ones the evaluation would compute: the solve is
+// bit for bit the same.  The one-thread-per-cell kernels keep the cache in
+// the thread's private memory; the lockstep kernel, whose lanes share one
+// rhs evaluation, does not
+template <class Net, bool ET, bool KC = false>
 struct Problem {
   using NetT = Net;
   static constexpr int NS = Net::NSPEC;
   static constexpr int N = NS + (ET ? 1 : 0);
   // the state and the rhs in registers (N values), the saved Jacobian
-  // dense (N x N), the rates evaluated at every rhs
+  // dense (N x N)
   static constexpr bool STRIDED = false;
   static constexpr long J_DOUBLES = (long)N * N;
   static constexpr long K_DOUBLES = 0;
@@ -50,10 +59,27 @@ struct Problem {
   typename Net::Params p;
   double Tfix;
   bool hit_T0;
+  double kc[KC ? Net::NREA : 1];  // the rates of kc_T (KC)
+  double kc_T = -1.0;             // their rate temperature; -1: none yet
 
   KX_HD double rate_T(const double* y) const {
     const double T = ET ? y[NS] : Tfix;
     return ::fmin(::fmax(T, p.Tcmb), 1e9);
+  }
+  // the rates at rate temperature Tr and state y into k
+  template <class R>
+  KX_HD void rates_at(R Tr, const R* y, R* k) {
+    if constexpr (KC && std::is_same<R, double>::value) {
+      if (!(Tr == kc_T)) {
+        for (int i = 0; i < Net::NREA; ++i) kc[i] = 0.0;
+        Net::rates_T(Tr, p, kc);
+        kc_T = Tr;
+      }
+      for (int i = 0; i < Net::NREA; ++i) k[i] = kc[i];
+      Net::rates_n(Tr, y, p, k);
+    } else {
+      Net::rates(Tr, y, p, k);
+    }
   }
   // the rhs at y; 1 (recoverable) for a non-positive Tgas or a non-finite
   // result, the integrator then retries with a smaller step
@@ -66,7 +92,7 @@ struct Problem {
     const R T = ET ? y[N - 1] : R(Tfix);
     const R Tr = kx::fmin(kx::fmax(T, R(p.Tcmb)), R(1e9));
     R k[Net::NREA];
-    Net::rates(Tr, y, p, k);
+    rates_at(Tr, y, k);
     Net::rhs(y, k, f);
     if constexpr (ET) {
       const R Tth = kx::fmin(kx::fmax(y[NS], R(2.73)), R(1e9));
@@ -81,7 +107,7 @@ struct Problem {
   // state; J zeroed by the caller)
   KX_HD int jac(const double* y, double* J) {
     double k[Net::NREA];
-    Net::rates(rate_T(y), y, p, k);
+    rates_at(rate_T(y), y, k);
     Net::jac_species(y, k, J, N);
     return 0;
   }
@@ -147,7 +173,7 @@ struct BigProblem {
 // the problem type the plain kernel solves a network with
 template <class Net, bool ET, bool BIG = (Net::NSPEC > KX_BIG_N)>
 struct PlainProblemOf {
-  using type = Problem<Net, ET>;
+  using type = Problem<Net, ET, true>;
 };
 template <class Net>
 struct PlainProblemOf<Net, false, true> {

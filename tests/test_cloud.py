@@ -17,9 +17,13 @@ cloud_*.npz, written by tests/golden/make_golden.py cloud):
   flags, 3e-4 (the uneven batch's bar, as primordialZ; 1e-4 above 1e-10
   of the cell total); the oracle's
   restatement bitwise against the reference's integrator on 8 cells;
+- the same per-cell code the kernels run (kx::plain_cell), compiled for
+  the CPU (tests/plain_host.cpp): the reference's steps exactly on the
+  fixture's cheapest cells;
 - GPU: the plain kernel's strided form (state, rhs and the sparse
   Jacobian in the interleaved workspace, the rates cached per solve)
-  against the reference's solves and the oracle.
+  against the reference's solves and the oracle, and the lane queue
+  bitwise equal to it, on the fixture's cheapest cells.
 """
 import numpy as np
 import pytest
@@ -35,6 +39,18 @@ NET = "cloud"
 # above 1e-10 of the cell total, 1e-4
 TOL = 3e-4
 TOL_MAJOR = 1e-4
+
+
+# GPU tests take the fixture's cheapest cells: a few cells occupy one
+# wave, whose lanes then run each step at the device memory's latency
+# (the strided form reads and writes its ~1.5 MB of state per step in the
+# workspace); the full batch is the bandwidth-bound case the benchmark
+# measures
+NGPU = 8
+
+
+def _cheapest(d, m=NGPU):
+    return np.sort(np.argsort(d["stats_bitwise"][:, 0])[:m])
 
 
 def _opts(d):
@@ -151,23 +167,68 @@ def test_oracle_bitwise_vs_reference_integrator(oracle, oracle_ref):
         assert np.array_equal(x, y)
 
 
+def _host_lib():
+    import subprocess
+    from conftest import REPO
+    out = REPO / "tests" / "_build" / "libplain_host.so"
+    src = REPO / "tests" / "plain_host.cpp"
+    hdr = REPO / "krome-x_amd" / "csrc"
+    if not out.exists() or out.stat().st_mtime < max(
+            f.stat().st_mtime for f in [src, *hdr.glob("*.h"),
+                                        hdr / "generated" / "net_cloud.h"]):
+        out.parent.mkdir(exist_ok=True)
+        subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off",
+                        "-shared", "-fPIC", "-I", str(hdr), "-I",
+                        str(hdr / "generated"), str(src), "-o", str(out)],
+                       check=True)
+    import ctypes
+    return ctypes.CDLL(str(out))
+
+
+def test_device_code_on_host_vs_reference():
+    """the plain kernel's own per-cell code (kx::plain_cell, the strided
+    form for cloud) compiled for the CPU: on the fixture's cheapest
+    cells, the reference's steps exactly and its states to 1e-12"""
+    import ctypes
+    lib = _host_lib()
+    d = golden("cloud_solve.npz")
+    sel = _cheapest(d)
+    n = np.ascontiguousarray(d["n0"][sel])
+    T = np.ascontiguousarray(d["T0"][sel])
+    o = np.array([d["opts"][1], d["opts"][2], d["opts"][3], d["opts"][6],
+                  0, 0], dtype=np.float64)
+    fl = np.zeros(len(sel), dtype=np.intc)
+    st = np.zeros((len(sel), 9), dtype=np.int64)
+    D = ctypes.POINTER(ctypes.c_double)
+    r = lib.plain_host_solve(
+        1, len(sel), n.ctypes.data_as(D), T.ctypes.data_as(D),
+        np.ascontiguousarray(d["dt"][sel]).ctypes.data_as(D),
+        np.ascontiguousarray(d["par"][sel]).ctypes.data_as(D),
+        o.ctypes.data_as(D), fl.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+        st.ctypes.data_as(ctypes.POINTER(ctypes.c_long)))
+    assert r == 0 and (fl == 0).all()
+    assert np.array_equal(st[:, 0], d["stats_bitwise"][sel, 0])
+    assert rel_err(n, d["n_bitwise"][sel]) < 1e-12
+
+
 @pytest.mark.gpu
 def test_gpu_plain_kernel_vs_reference(oracle):
     d = golden("cloud_solve.npz")
+    sel = _cheapest(d)
     net = krome.load(NET)
     o = d["opts"]
     opts = net.options(evolve_T=0, rtol=o[1], atol_n=o[2], atol_T=o[3],
                        linsol=0, jac=0, max_steps=int(o[6]),
                        device_kernel="plain")
-    print("cloud: plain kernel, %d cells" % d["n0"].shape[0], flush=True)
-    res = net.solve(d["n0"], d["T0"], d["dt"], par=d["par"], opts=opts,
-                    check=False)
+    print("cloud: plain kernel, cells %s" % sel.tolist(), flush=True)
+    res = net.solve(d["n0"][sel], d["T0"][sel], d["dt"][sel],
+                    par=d["par"][sel], opts=opts, check=False)
     assert res.ierr == 0
-    assert np.array_equal(res.flags, d["flag_bitwise"])
-    assert rel_err(res.n, d["n_bitwise"]) < TOL
-    assert rel_err(res.n, d["n_bitwise"], 1e-10) < TOL_MAJOR
-    _, n, _, _, _ = oracle.solve(NET, d["n0"], d["T0"], d["dt"], d["par"],
-                                 _opts(d))
+    assert np.array_equal(res.flags, d["flag_bitwise"][sel])
+    assert rel_err(res.n, d["n_bitwise"][sel]) < TOL
+    assert rel_err(res.n, d["n_bitwise"][sel], 1e-10) < TOL_MAJOR
+    _, n, _, _, _ = oracle.solve(NET, d["n0"][sel], d["T0"][sel],
+                                 d["dt"][sel], d["par"][sel], _opts(d))
     assert rel_err(res.n, n) < TOL
 
 
@@ -193,13 +254,14 @@ def test_gpu_lane_queue_equals_plain():
     d = golden("cloud_solve.npz")
     net = krome.load(NET)
     o = d["opts"]
+    sel = _cheapest(d)
     res = {}
     for kern in ("plain", "persistent"):
         opts = net.options(evolve_T=0, rtol=o[1], atol_n=o[2], atol_T=o[3],
                            max_steps=int(o[6]), device_kernel=kern)
         print("cloud:", kern, flush=True)
-        res[kern] = net.solve(d["n0"], d["T0"], d["dt"], par=d["par"],
-                              opts=opts, check=False)
+        res[kern] = net.solve(d["n0"][sel], d["T0"][sel], d["dt"][sel],
+                              par=d["par"][sel], opts=opts, check=False)
     assert res["persistent"].ierr == 0
     assert np.array_equal(res["persistent"].flags, res["plain"].flags)
     assert np.array_equal(res["persistent"].n, res["plain"].n)

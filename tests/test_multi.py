@@ -208,10 +208,16 @@ def test_c3_exports_and_layout():
         m.set_launch(-1, 0)
 
 
-def _golden_batch(name):
+def _golden_batch(name, m=None):
+    """the reference's golden cells of a network as a batch (m: its m
+    cheapest cells only)"""
     import torch
     from conftest import golden
-    d = golden("%s_solve.npz" % name)
+    d = dict(golden("%s_solve.npz" % name))
+    if m is not None:
+        sel = np.sort(np.argsort(d["stats_bitwise"][:, 0])[:m])
+        for k in ("n0", "T0", "dt", "par", "n_bitwise", "flag_bitwise"):
+            d[k] = d[k][sel]
     o = d["opts"]
     return d, dict(
         network=name, evolve_T=int(o[0]), rtol=o[1], atol_n=o[2],
@@ -238,7 +244,10 @@ def test_c3_launch_vs_reference_and_single(wpc, chunk):
     m = lib_c3()
     m.set_launch(wpc, chunk)
     bars = {"primordial": 5e-5, "primordialZ": 3e-4, "cloud": 1e-4}
-    got = [_golden_batch(n) for n in ("primordial", "primordialZ", "cloud")]
+    # cloud's cheapest cells: a few cells of the strided form run at the
+    # device memory's latency (tests/test_cloud.py)
+    got = [_golden_batch("primordial"), _golden_batch("primordialZ"),
+           _golden_batch("cloud", 8)]
     singles = []
     for d, b in got:
         net = krome.load(b["network"])
