@@ -10,8 +10,10 @@
  * MULTI); each batch is solved as krome_solve_cells_device with the
  * persistent kernel (device_kernel 1) would solve it, in a single launch
  * shared by all the batches: networks of up to 16 equations in the
- * lockstep persistent kernel, larger ones in the lane queue (each lane
- * solves whole cells, taking the next from the device work queue).
+ * lockstep persistent kernel, networks above 64 species (Tgas fixed) a
+ * wave per cell, the others in the lane queue (each lane solves whole
+ * cells); every wave and lane takes its next cells from the network's
+ * device work queue.
  */
 #ifndef KROME_MULTI_MI355X_H
 #define KROME_MULTI_MI355X_H
@@ -47,8 +49,12 @@ const char* krome_multi_network_name(int i);  /* NULL out of range */
 int krome_multi_nspec(int i);                 /* -1 out of range */
 int krome_multi_npar(int i);
 int krome_multi_evolve_T(int i);   /* 1: Tgas evolved, 0 fixed, -1 */
-int krome_multi_lockstep(int i);   /* 1: the lockstep kernel, 0: the lane
-                                      queue, -1 out of range */
+int krome_multi_lockstep(int i);   /* 1: the lockstep kernel, 0: not,
+                                      -1 out of range */
+int krome_multi_mode(int i);       /* how network i's cells run: 1 the
+                                      lockstep kernel, 2 a wave per cell
+                                      (above 64 species, Tgas fixed), 0
+                                      the lane queue; -1 out of range */
 
 /* the launch shape of the following calls: waves per CU (0: the
  * default, one wave per SIMD) and the lockstep machine's cells per queue

@@ -6,9 +6,11 @@ persistent kernel, whose lanes share each rhs evaluation with their state
 in registers and LDS; the lane queue, where each lane solves whole cells
 with its state in an interleaved HBM workspace and takes the next from the
 device queue; the plain one-thread-per-cell kernel), how many cells a
-lockstep wave takes per queue grab (queue_chunk), and for the lane queue
-how many lanes a workgroup holds (lane_block, the cells in flight per
-workgroup) and how many waves run per CU (waves_per_cu).  The tuner times
+lockstep wave takes per queue grab (queue_chunk), for the lane queue how
+many lanes a workgroup holds (lane_block, the cells in flight per
+workgroup) and how many waves run per CU (waves_per_cu), and for a wave per
+cell (the networks above 64 species) how many cells a workgroup holds
+(cells_per_workgroup).  The tuner times
 each candidate on a sample of the caller's own cells, resident on the GPU,
 from the kernels' HIP events (krome_device_last_kernel_ms), and keeps the
 fastest whose flags equal the default setting's.
@@ -40,20 +42,40 @@ KERNELS = ("persistent", "lanes", "plain")
 QUEUE_CHUNKS = (0, 64, 256, 1024, 4096)
 LANE_BLOCKS = (64, 128)
 WAVES_PER_CU = (1, 2, 4)
+# a wave per cell: cells (waves) per workgroup, waves per CU
+CELLS_PER_WORKGROUP = (1, 2, 4, 8, 16)
+WAVE_WAVES_PER_CU = (8, 16)
 
 
-def candidates(lockstep):
-    """the settings tried for a network: the lockstep kernel with each
-    queue chunk (when the network fits it), the lane queue with each
-    workgroup size and waves per CU, the plain kernel"""
+def candidates(kind):
+    """the settings tried for a network, by the form its persistent
+    launch takes: "lockstep" (up to 16 equations: the lockstep kernel with
+    each queue chunk), "wave" (above 64 species at fixed Tgas: a wave per
+    cell with each number of cells per workgroup and waves per CU), else
+    "lane"; then the lane queue with each workgroup size and waves per CU,
+    and the plain kernel"""
     out = []
-    if lockstep:
+    if kind == "lockstep":
         out += [{"device_kernel": "persistent", "queue_chunk": c}
                 for c in QUEUE_CHUNKS]
+    if kind == "wave":
+        out += [{"device_kernel": "persistent", "cells_per_workgroup": c,
+                 "waves_per_cu": w}
+                for c, w in itertools.product(CELLS_PER_WORKGROUP,
+                                              WAVE_WAVES_PER_CU)]
     out += [{"device_kernel": "lanes", "lane_block": b, "waves_per_cu": w}
             for b, w in itertools.product(LANE_BLOCKS, WAVES_PER_CU)]
     out.append({"device_kernel": "plain"})
     return out
+
+
+def form_of(nspec, evolve_T):
+    """the persistent launch's form for a network"""
+    if nspec + (1 if evolve_T else 0) <= 16:
+        return "lockstep"
+    if nspec > 64 and not evolve_T:
+        return "wave"
+    return "lane"
 
 
 def sample_indices(ncell, sample):
@@ -78,8 +100,7 @@ def tune(net, w, base, sample=8192, reps=2, space=None, run=None):
     function (default: the GPU through solve_device)"""
     idx = sample_indices(w["n"].shape[0], sample)
     if space is None:
-        neq = net.nspec + (1 if base.get("evolve_T") else 0)
-        space = candidates(neq <= 16)
+        space = candidates(form_of(net.nspec, base.get("evolve_T")))
     if run is None:
         run = _gpu_runner(net, w, idx, base, reps)
     trials = []

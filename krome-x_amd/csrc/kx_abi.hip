@@ -57,6 +57,9 @@ struct Opts {
   int queue_chunk = 0;
   int waves_per_cu = 0;
   int lane_block = 128;
+  // a wave per cell (networks above 64 species, device_kernel 1): cells
+  // per workgroup, 1 to 16 (0: 4)
+  int cells_per_workgroup = 0;
   // the adjoint: accepted steps between checkpoints, the checkpoint
   // pool's bytes (0: room for about 1536 steps per cell of a launch)
   int checkpoint_steps = 32;
@@ -354,6 +357,10 @@ int solve_kernels(kx::CellsArgs a, const Opts& o, const SolveMem& m,
       return kx::launch_lane_queue<Net, false>(a, o.waves_per_cu, s, mem,
                                                o.lane_block);
     }
+    if constexpr (Net::NSPEC > KX_BIG_N)  // a wave per cell
+      if (!a.o.evolve_T)
+        return kx::launch_wave_queue<Net>(a, o.waves_per_cu,
+                                          o.cells_per_workgroup, s, mem);
     if constexpr (Net::THERMAL)
       if (a.o.evolve_T)
         return kx::launch_persistent<Net, true>(a, o.mixed, o.queue_chunk,
@@ -583,6 +590,12 @@ int krome_opts_set(krome_opts* h, const char* key, double v) {
     h->lane_block = (int)v;
     return 0;
   }
+  if (is("cells_per_workgroup")) {
+    if (v != 0 && v != 1 && v != 2 && v != 4 && v != 8 && v != 16)
+      return -1;
+    h->cells_per_workgroup = (int)v;
+    return 0;
+  }
   if (is("mixed_precision")) {
     if (v < 0 || v > 1) return -1;
     h->mixed = (int)v;
@@ -639,6 +652,7 @@ int krome_opts_get(const krome_opts* h, const char* key, double* v) {
   else if (is("queue_chunk")) *v = h->queue_chunk;
   else if (is("waves_per_cu")) *v = h->waves_per_cu;
   else if (is("lane_block")) *v = h->lane_block;
+  else if (is("cells_per_workgroup")) *v = h->cells_per_workgroup;
   else if (is("checkpoint_steps")) *v = h->checkpoint_steps;
   else if (is("checkpoint_memory")) *v = (double)h->checkpoint_memory;
   else return -1;

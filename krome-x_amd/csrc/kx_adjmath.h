@@ -20,6 +20,7 @@ struct AdjProblem {
   static constexpr int N = NS + (ET ? 1 : 0);
   static constexpr long NEWTON_DOUBLES = Problem<Net, ET>::NEWTON_DOUBLES;
   static constexpr bool STRIDED = false;
+  static constexpr int LANES = 1;
   static constexpr long J_DOUBLES = (long)N * N;
   static constexpr long K_DOUBLES = 0;
   typename Net::Params p;

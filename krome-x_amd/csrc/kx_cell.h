@@ -27,6 +27,62 @@ struct SView {
   KX_HD double& operator[](long i) const { return p[i * s]; }
 };
 
+// ---- a wave per cell ------------------------------------------------------
+// A large network's cell can be solved by the 64 lanes of a wave together
+// (Prob::LANES = 64): lane l handles elements l, l + 64, ... of every
+// vector, the reductions (norms, minima, maxima) go through a xor
+// butterfly, and the rhs, the Jacobian, the sparse LU and its solves are
+// split over the lanes (kx_problem.h WaveProblem, Cell's sp_factor /
+// sp_solve).  Every lane of the wave computes the same scalars, so the
+// control flow is the wave's own; data another lane wrote are read after
+// wave_sync().
+KX_HD inline int wave_lane() {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return threadIdx.x & 63;
+#else
+  return 0;
+#endif
+}
+// the sum, minimum and maximum over the wave's 64 lanes: each butterfly
+// step combines two partials in either order (commutative), so every lane
+// ends with the same bits
+KX_HD inline double wave_sum(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+#endif
+  return v;
+}
+KX_HD inline double wave_max(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  for (int o = 32; o > 0; o >>= 1) v = ::fmax(v, __shfl_xor(v, o, 64));
+#endif
+  return v;
+}
+KX_HD inline double wave_min(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  for (int o = 32; o > 0; o >>= 1) v = ::fmin(v, __shfl_xor(v, o, 64));
+#endif
+  return v;
+}
+// what the wave's lanes stored is visible to its other lanes (a wave's
+// memory operations complete in order; this orders the compiler's)
+KX_HD inline void wave_sync() {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+#endif
+}
+// the wave's row accumulator of the sparse LU in LDS: element k at
+// base[(k / 64) stride + k % 64] (the wave's own 64 columns of a block's
+// arena of stride doubles per row)
+struct WaveLds {
+  double* base;
+  long stride;
+  KX_HD double& operator[](long k) const {
+    return base[(k >> 6) * stride + (k & 63)];
+  }
+};
+
 // CVODE's flags (bdf_cell.hpp:107-122)
 enum : int {
   BDF_OK = 0, TOO_MUCH_WORK = -1, TOO_MUCH_ACC = -2, ERR_FAILURE = -3,
@@ -184,6 +240,31 @@ class Cell {
   }
 
   KX_HD double& at(double* v, int i) const { return v[(long)i * S]; }
+  // the elements this thread handles: all (one thread per cell), or every
+  // 64th from its lane (a wave per cell, Prob::LANES 64); reductions of
+  // the per-thread partials over the cell's lanes
+  static constexpr int VS = Prob::LANES;
+  KX_HD int vb() const {
+    if constexpr (VS > 1) return wave_lane();
+    else return 0;
+  }
+  KX_HD double vsum(double v) const {
+    if constexpr (VS > 1) return wave_sum(v);
+    else return v;
+  }
+  KX_HD double vmax(double v) const {
+    if constexpr (VS > 1) return wave_max(v);
+    else return v;
+  }
+  KX_HD double vmin(double v) const {
+    if constexpr (VS > 1) return wave_min(v);
+    else return v;
+  }
+  KX_HD void vsync() const {
+    if constexpr (VS > 1) wave_sync();
+  }
+  // the LU's row accumulator of a wave per cell (set by its kernel)
+  WaveLds lw{nullptr, 0};
   // the view of a workspace vector the strided problems read and write
   KX_HD SView view(const double* v) const {
     return SView{const_cast<double*>(v), S};
@@ -193,7 +274,7 @@ class Cell {
 
   KX_HD void reinit(double t0, const double* y0, double h_init) {
     tn = t0;
-    for (int i = 0; i < N; i++) at(zn[0], i) = y0[i];
+    for (int i = vb(); i < N; i += VS) at(zn[0], i) = y0[i];
     q = 1; L = 2; qwait = L;
     etamax = bdfc::ETA_MAX_FS;
     qu = 0; hu = 0.0; tolsf = 1.0;
@@ -269,7 +350,7 @@ class Cell {
       return ITER_CONTINUE;
     } while (0);
     *tret = tn;
-    for (int i = 0; i < N; i++) yout[i] = at(zn[0], i);
+    for (int i = vb(); i < N; i += VS) yout[i] = at(zn[0], i);
     return istate;
   }
   // the interpolation weights of CVodeGetDky(t, 0): y(t) = sum_j c[j]
@@ -376,11 +457,11 @@ class Cell {
   }
   KX_HD void axpy(double a, const double* x, double* yv) {
     if (a == 1.0) {
-      for (int i = 0; i < N; i++) at(yv, i) += at(const_cast<double*>(x), i);
+      for (int i = vb(); i < N; i += VS) at(yv, i) += at(const_cast<double*>(x), i);
     } else if (a == -1.0) {
-      for (int i = 0; i < N; i++) at(yv, i) -= at(const_cast<double*>(x), i);
+      for (int i = vb(); i < N; i += VS) at(yv, i) -= at(const_cast<double*>(x), i);
     } else {
-      for (int i = 0; i < N; i++)
+      for (int i = vb(); i < N; i += VS)
         at(yv, i) += a * at(const_cast<double*>(x), i);
     }
   }
@@ -391,14 +472,14 @@ class Cell {
     if (b == 1.0 && z == yv) return axpy(a, x, z);
     if (a == 1.0 && z == x) return axpy(b, yv, z);
     if (a == 1.0 && b == 1.0) {
-      for (int i = 0; i < N; i++) at(z, i) = at(x, i) + at(yv, i);
+      for (int i = vb(); i < N; i += VS) at(z, i) = at(x, i) + at(yv, i);
       return;
     }
     if ((a == 1.0 && b == -1.0) || (a == -1.0 && b == 1.0)) {
       const bool t = a == 1.0;
       double* p = t ? x : yv;
       double* m = t ? yv : x;
-      for (int i = 0; i < N; i++) at(z, i) = at(p, i) - at(m, i);
+      for (int i = vb(); i < N; i += VS) at(z, i) = at(p, i) - at(m, i);
       return;
     }
     if (a == 1.0 || b == 1.0) {
@@ -406,7 +487,7 @@ class Cell {
       const double c = t ? b : a;
       double* v1 = t ? yv : x;
       double* v2 = t ? x : yv;
-      for (int i = 0; i < N; i++) at(z, i) = (c * at(v1, i)) + at(v2, i);
+      for (int i = vb(); i < N; i += VS) at(z, i) = (c * at(v1, i)) + at(v2, i);
       return;
     }
     if (a == -1.0 || b == -1.0) {
@@ -414,54 +495,67 @@ class Cell {
       const double c = t ? b : a;
       double* v1 = t ? yv : x;
       double* v2 = t ? x : yv;
-      for (int i = 0; i < N; i++) at(z, i) = (c * at(v1, i)) - at(v2, i);
+      for (int i = vb(); i < N; i += VS) at(z, i) = (c * at(v1, i)) - at(v2, i);
       return;
     }
     if (a == b) {
-      for (int i = 0; i < N; i++) at(z, i) = a * (at(x, i) + at(yv, i));
+      for (int i = vb(); i < N; i += VS) at(z, i) = a * (at(x, i) + at(yv, i));
       return;
     }
     if (a == -b) {
-      for (int i = 0; i < N; i++) at(z, i) = a * (at(x, i) - at(yv, i));
+      for (int i = vb(); i < N; i += VS) at(z, i) = a * (at(x, i) - at(yv, i));
       return;
     }
-    for (int i = 0; i < N; i++) at(z, i) = (a * at(x, i)) + (b * at(yv, i));
+    for (int i = vb(); i < N; i += VS) at(z, i) = (a * at(x, i)) + (b * at(yv, i));
   }
   KX_HD void scal(double c, const double* x, double* z) {
-    for (int i = 0; i < N; i++) at(z, i) = c * at(const_cast<double*>(x), i);
+    for (int i = vb(); i < N; i += VS) at(z, i) = c * at(const_cast<double*>(x), i);
   }
   KX_HD double wrms(const double* x, const double* w) {
     double s = 0.0;
-    for (int i = 0; i < N; i++) {
+    for (int i = vb(); i < N; i += VS) {
       const double p = at(const_cast<double*>(x), i) *
                        at(const_cast<double*>(w), i);
       s += p * p;
     }
-    return bdfc::rt(s / N);
+    return bdfc::rt(vsum(s) / N);
   }
 
   // cvEwtSetSV, or KROME's per-species form (bdf_cell.hpp:689-713)
   KX_HD int ewt_set(const double* yc, double* w) {
     if (o.rtol_v) {
-      for (int i = 0; i < N; i++) {
-        const double d =
-            ::fma(o.rtol_at(i), ::fabs(at(const_cast<double*>(yc), i)),
-                  o.atol_at(i));
-        if (!(d > 0.0)) return -1;
-        at(w, i) = 1.0 / d;
+      if constexpr (VS > 1) {  // every lane's elements, then one decision
+        double bad = 0.0;
+        for (int i = vb(); i < N; i += VS) {
+          const double d =
+              ::fma(o.rtol_at(i), ::fabs(at(const_cast<double*>(yc), i)),
+                    o.atol_at(i));
+          if (!(d > 0.0)) bad = 1.0;
+          at(w, i) = 1.0 / d;
+        }
+        return vmax(bad) > 0.0 ? -1 : 0;
+      } else {
+        for (int i = 0; i < N; i++) {
+          const double d =
+              ::fma(o.rtol_at(i), ::fabs(at(const_cast<double*>(yc), i)),
+                    o.atol_at(i));
+          if (!(d > 0.0)) return -1;
+          at(w, i) = 1.0 / d;
+        }
+        return 0;
       }
-      return 0;
     }
     if constexpr (Prob::STRIDED) {  // the same steps in the workspace
-      for (int i = 0; i < N; i++)
+      for (int i = vb(); i < N; i += VS)
         at(w, i) = (o.rtol * ::fabs(at(const_cast<double*>(yc), i))) +
                    o.atol_at(i);
       if (atolmin0) {
-        double m = at(w, 0);
-        for (int i = 1; i < N; i++) m = at(w, i) < m ? at(w, i) : m;
-        if (m <= 0.0) return -1;
+        double m = at(w, vb());
+        for (int i = vb() + VS; i < N; i += VS)
+          m = at(w, i) < m ? at(w, i) : m;
+        if (vmin(m) <= 0.0) return -1;
       }
-      for (int i = 0; i < N; i++) at(w, i) = 1.0 / at(w, i);
+      for (int i = vb(); i < N; i += VS) at(w, i) = 1.0 / at(w, i);
       return 0;
     } else {
       double v[N];
@@ -523,15 +617,16 @@ class Cell {
   KX_HD double hub_est(double tdist) {
     double* t1 = tempv;
     double* t2 = acor;
-    for (int i = 0; i < N; i++) at(t2, i) = ::fabs(at(zn[0], i));
+    for (int i = vb(); i < N; i += VS) at(t2, i) = ::fabs(at(zn[0], i));
     ewt_set(zn[0], t1);
-    for (int i = 0; i < N; i++) at(t1, i) = 1.0 / at(t1, i);
+    for (int i = vb(); i < N; i += VS) at(t1, i) = 1.0 / at(t1, i);
     lsum(bdfc::HUB_FACTOR, t2, 1.0, t1, t1);
-    for (int i = 0; i < N; i++) at(t2, i) = ::fabs(at(zn[1], i));
-    for (int i = 0; i < N; i++) at(t1, i) = at(t2, i) / at(t1, i);
+    for (int i = vb(); i < N; i += VS) at(t2, i) = ::fabs(at(zn[1], i));
+    for (int i = vb(); i < N; i += VS) at(t1, i) = at(t2, i) / at(t1, i);
     double inv = 0.0;
-    for (int i = 0; i < N; i++)
+    for (int i = vb(); i < N; i += VS)
       if (::fabs(at(t1, i)) > inv) inv = ::fabs(at(t1, i));
+    inv = vmax(inv);
     double hub = bdfc::HUB_FACTOR * tdist;
     if (hub * inv > 1.0) hub = 1.0 / inv;
     return hub;
@@ -701,7 +796,7 @@ class Cell {
     bool setup = nflag == PREV_CONV_FAIL || nflag == PREV_ERR_FAIL ||
                  st.nst == 0 || st.nst >= nstlp + MSBP ||
                  ::fabs(gamrat - 1.0) > DGMAX_LSETUP;
-    for (int i = 0; i < N; i++) at(acor, i) = 0.0;
+    for (int i = vb(); i < N; i += VS) at(acor, i) = 0.0;
     const double tol = tq[4];
     bool jbad = false;
     long niters = 0, nfails = 0;
@@ -735,7 +830,7 @@ class Cell {
         nfails++;
         setup = true;
         jbad = true;
-        for (int i = 0; i < N; i++) at(acor, i) = 0.0;
+        for (int i = vb(); i < N; i += VS) at(acor, i) = 0.0;
         continue;
       }
       break;
@@ -815,6 +910,7 @@ class Cell {
       nstlj = st.nst;
     }
     if (r != 0) return r;
+    if constexpr (VS > 1) return sp_factor_wave(-gamma);
     if (o.sparse || Prob::STRIDED) return sp_factor(-gamma);
     if constexpr (!Prob::STRIDED) return lu() > 0 ? 1 : 0;
     return 1;
@@ -892,6 +988,68 @@ class Cell {
       return sp_border_lu() > 0 ? 1 : 0;
     }
     return 0;
+  }
+  // the same factorization by a wave per cell: the rows in order, each
+  // row's entries and each U row's update split over the lanes (distinct
+  // columns: the updates are independent, each the serial one's
+  // arithmetic), the row accumulator in the wave's LDS (lw); Tgas fixed
+  // (no border)
+  KX_HD int sp_factor_wave(double c) {
+    static_assert(NB == 0, "a wave per cell serves fixed-Tgas networks");
+    vsync();
+    for (int p = vb(); p < Net::SP_NNZ; p += VS) spv(p) = at(J, p) * c;
+    vsync();
+    for (int k = vb(); k < NS; k += VS) spv(Net::sp_di[k]) += 1.0;
+    vsync();
+    for (int i = 0; i < NS; i++) {
+      const int p0 = Net::sp_rp[i], p1 = Net::sp_rp[i + 1];
+      for (int p = p0 + vb(); p < p1; p += VS) lw[Net::sp_ci[p]] = spv(p);
+      vsync();
+      for (int p = p0; p < Net::sp_di[i]; p++) {
+        const int k = Net::sp_ci[p];
+        const double lik = lw[k] / spv(Net::sp_di[k]);
+        vsync();
+        lw[k] = lik;
+        for (int q = Net::sp_di[k] + 1 + vb(); q < Net::sp_rp[k + 1];
+             q += VS)
+          lw[Net::sp_ci[q]] -= lik * spv(q);
+        vsync();
+      }
+      const double pv = lw[i];
+      if (pv == 0.0 || !(::fabs(pv) <= DBL_MAX)) return 1;
+      for (int p = p0 + vb(); p < p1; p += VS) spv(p) = lw[Net::sp_ci[p]];
+      vsync();
+    }
+    return 0;
+  }
+  // b = (c J + I)^-1 b with sp_factor_wave's factors, b in the workspace:
+  // each row's dot product split over the lanes and reduced by the
+  // butterfly (the order of its additions differs from the serial solve)
+  KX_HD void sp_solve_wave(double* bv) {
+    vsync();
+    for (int k = vb(); k < NS; k += VS) lw[k] = at(bv, Net::sp_perm[k]);
+    vsync();
+    for (int i = 0; i < NS; i++) {
+      double part = 0.0;
+      for (int p = Net::sp_rp[i] + vb(); p < Net::sp_di[i]; p += VS)
+        part += spv(p) * lw[Net::sp_ci[p]];
+      const double v = lw[i] - vsum(part);
+      vsync();
+      lw[i] = v;
+      vsync();
+    }
+    for (int i = NS - 1; i >= 0; i--) {
+      double part = 0.0;
+      for (int p = Net::sp_di[i] + 1 + vb(); p < Net::sp_rp[i + 1];
+           p += VS)
+        part += spv(p) * lw[Net::sp_ci[p]];
+      const double v = (lw[i] - vsum(part)) / spv(Net::sp_di[i]);
+      vsync();
+      lw[i] = v;
+      vsync();
+    }
+    for (int k = vb(); k < NS; k += VS) at(bv, Net::sp_perm[k]) = lw[k];
+    vsync();
   }
   // x = L^-1 x (unit lower triangle), x at A + x0 (strided)
   KX_HD void sp_lower(long x0) {
@@ -1040,10 +1198,11 @@ class Cell {
   KX_HD void lsolve(double* bv) {
     if constexpr (Prob::STRIDED) {  // in place in the workspace
       SView b = view(bv);
-      sp_solve(b);
+      if constexpr (VS > 1) sp_solve_wave(bv);
+      else sp_solve(b);
       if (gamrat != 1.0) {
         const double c = 2.0 / (1.0 + gamrat);
-        for (int i = 0; i < N; i++) b[i] = c * b[i];
+        for (int i = vb(); i < N; i += VS) b[i] = c * b[i];
       }
       return;
     }
@@ -1180,7 +1339,7 @@ class Cell {
     qwait--;
     const bool save_q = qwait == 1 && q != QMAX;
     if (save_q) {
-      for (int i = 0; i < N; i++) at(zn[QMAX], i) = at(acor, i);
+      for (int i = vb(); i < N; i += VS) at(zn[QMAX], i) = at(acor, i);
       saved_tq5 = tq[5];
     }
     if constexpr (Tape::ACTIVE) {
@@ -1235,7 +1394,7 @@ class Cell {
     } else {
       eta = etaqp1;
       qprime = q + 1;
-      for (int i = 0; i < N; i++) at(zn[QMAX], i) = at(acor, i);
+      for (int i = vb(); i < N; i += VS) at(zn[QMAX], i) = at(acor, i);
       if constexpr (Tape::ACTIVE) tp.saveq();
     }
     set_eta();
@@ -1258,14 +1417,14 @@ class Cell {
   // CVodeGetDky(t, 0) (bdf_cell.hpp:1492-1500)
   KX_HD void dky(double t, double* out) {
     const double s = (t - tn) / h;
-    for (int i = 0; i < N; i++) out[i] = 0.0;
+    for (int i = vb(); i < N; i += VS) out[i] = 0.0;
     for (int j = q; j >= 0; j--) {
       double c = 1.0;
       for (int i = 0; i < j; i++) c *= s;
       if (j == q)
-        for (int i = 0; i < N; i++) out[i] = c * at(zn[j], i);
+        for (int i = vb(); i < N; i += VS) out[i] = c * at(zn[j], i);
       else
-        for (int i = 0; i < N; i++) out[i] += c * at(zn[j], i);
+        for (int i = vb(); i < N; i += VS) out[i] += c * at(zn[j], i);
     }
   }
 };

@@ -41,8 +41,17 @@ template <class Net, bool ET>
 struct MultiPart {
   static constexpr int N = Problem<Net, ET>::N;
   static constexpr bool LOCKSTEP = N <= KX_PERSIST_MAX_N;
-  // doubles of the block's LDS its lanes use (the lockstep machine's)
-  static constexpr long LDS = LOCKSTEP ? lds_doubles<Net, ET>() : 0;
+  // a large network at fixed Tgas: a wave per cell
+  static constexpr bool WAVE = !LOCKSTEP && Net::NSPEC > KX_BIG_N && !ET;
+  // the C interface's mode: 1 lockstep, 2 a wave per cell, 0 lane queue
+  static constexpr int MODE = LOCKSTEP ? 1 : (WAVE ? 2 : 0);
+  // doubles of the block's LDS its lanes use: the lockstep machine's
+  // arena, or each wave's LU row accumulator (its own 64 columns of the
+  // block's rows)
+  static constexpr long LDS =
+      LOCKSTEP ? lds_doubles<Net, ET>()
+               : (WAVE ? (long)((Net::NSPEC + 63) / 64) * KX_PERSIST_BLOCK
+                       : 0);
   CellsArgs a;
   int on;  // cells in this launch
   // the lockstep machine: its workspace (FP64 or FP32 matrices) and queue
@@ -77,6 +86,10 @@ __device__ __attribute__((always_inline)) inline void multi_run(
       (void)wm;
       persist_run<Net, ET, false>(a, wd, pq, lds);
     }
+  } else if constexpr (MultiPart<Net, ET>::WAVE) {
+    (void)wd; (void)wm; (void)pq; (void)lanes;
+    const WaveLds lw{lds + (threadIdx.x & ~63), (long)KX_PERSIST_BLOCK};
+    wave_queue_run<Net>(a, lq, work, iwork, slot >> 6, lw);
   } else {
     (void)wd; (void)wm; (void)pq; (void)lds;
     lane_queue_run<Net, ET>(a, lq, work, iwork, lanes, slot);
@@ -121,12 +134,12 @@ multi_kernel(MultiArgs g) {
 // the C interface's view of the networks, by index
 struct NetInfo {
   const char* name;
-  int nspec, npar, et, lockstep;
+  int nspec, npar, et, mode;
 };
 const NetInfo kNets[KX_MULTI_COUNT] = {
 #define KX_M_INFO(i, nm, et)                                          \
   {kx::net_##nm::NAME, kx::net_##nm::NSPEC, kx::net_##nm::NPAR, et, \
-   MultiPart<kx::net_##nm, et>::LOCKSTEP ? 1 : 0},
+   MultiPart<kx::net_##nm, et>::MODE},
     KX_MULTI_NETS(KX_M_INFO)
 #undef KX_M_INFO
 };
@@ -240,6 +253,8 @@ int prepare_part(MultiPart<Net, ET>& p, const krome_multi_batch* b, int k,
   size_t wb;
   if constexpr (MultiPart<Net, ET>::LOCKSTEP) {
     wb = kx::persist_need<Net, ET>(p.a, mixed);
+  } else if constexpr (MultiPart<Net, ET>::WAVE) {
+    wb = kx::wave_queue_bytes<Net>(lanes / 64);
   } else {
     wb = kx::lane_queue_bytes<Net, ET>(lanes);
   }
@@ -254,6 +269,9 @@ int prepare_part(MultiPart<Net, ET>& p, const krome_multi_batch* b, int k,
                                                 p.pq);
     return kx::persist_prepare<Net, ET, false>(p.a, m, g_chunk, s, p.wd,
                                                p.pq);
+  } else if constexpr (MultiPart<Net, ET>::WAVE) {
+    return kx::wave_queue_prepare<Net>(p.a, m, lanes / 64, s, p.lq, p.work,
+                                       p.iwork);
   } else {
     return kx::lane_queue_prepare<Net, ET>(p.a, m, lanes, s, p.lq, p.work,
                                            p.iwork);
@@ -278,7 +296,11 @@ int krome_multi_evolve_T(int i) {
   return (i >= 0 && i < KX_MULTI_COUNT) ? kNets[order_of(i)].et : -1;
 }
 int krome_multi_lockstep(int i) {
-  return (i >= 0 && i < KX_MULTI_COUNT) ? kNets[order_of(i)].lockstep : -1;
+  return (i >= 0 && i < KX_MULTI_COUNT) ? (kNets[order_of(i)].mode == 1)
+                                        : -1;
+}
+int krome_multi_mode(int i) {
+  return (i >= 0 && i < KX_MULTI_COUNT) ? kNets[order_of(i)].mode : -1;
 }
 
 int krome_multi_set_launch(int waves_per_cu, int queue_chunk) {

@@ -2072,6 +2072,78 @@ inline int launch_lane_queue(CellsArgs a, int wpc, hipStream_t s,
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// ---- a wave per cell: networks above KX_BIG_N species --------------------
+// wave slots of a launch over ncell cells: waves_per_cu waves on each CU
+// (0: 16), cells per workgroup cpb, no more than the cells rounded up to a
+// workgroup
+inline int wave_queue_slots(long ncell, int wpc, int cpb, long& slots) {
+  int dev = 0, ncu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return -1;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                            dev) != hipSuccess)
+    return -1;
+  if (wpc <= 0) wpc = 16;
+  slots = (long)ncu * wpc;
+  if (slots > ncell) slots = ncell;
+  slots = (slots + cpb - 1) / cpb * cpb;
+  return 0;
+}
+template <class Net>
+constexpr size_t wave_queue_bytes(long slots) {
+  return 256 + (sizeof(double) * wave_slot_doubles<Net>() +
+                sizeof(int) * Net::NSPEC) * (size_t)slots;
+}
+// the queue and the slots' workspace in m (as lane_queue_prepare)
+template <class Net>
+int wave_queue_prepare(CellsArgs a, PersistMem m, long slots, hipStream_t s,
+                       LaneQueue& q, double*& work, int*& iwork) {
+  char* p = static_cast<char*>(m.work);
+  q.counter = reinterpret_cast<int*>(p);
+  p += 256;
+  work = reinterpret_cast<double*>(p);
+  p += sizeof(double) * wave_slot_doubles<Net>() * (size_t)slots;
+  iwork = reinterpret_cast<int*>(p);
+  int* hist = static_cast<int*>(m.ord);
+  int* key = hist + ORDER_BUCKETS;
+  int* order = key + a.ncell;
+  const int cb = (a.ncell + 255) / 256;
+  hipLaunchKernelGGL(fill_flags_kernel, dim3(1), dim3(64), 0, s, 1,
+                     q.counter, 0);
+  hipLaunchKernelGGL(fill_flags_kernel, dim3(ORDER_BUCKETS / 256), dim3(256),
+                     0, s, ORDER_BUCKETS, hist, 0);
+  hipLaunchKernelGGL(order_keys_kernel<Net>, dim3(cb), dim3(256), 0, s, a,
+                     key, hist);
+  hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(64), 0, s, hist);
+  hipLaunchKernelGGL(order_scatter_kernel, dim3(cb), dim3(256), 0, s,
+                     a.ncell, key, hist, order);
+  q.order = order;
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+// the wave-per-cell launch with its memory from the caller: cpb cells per
+// workgroup (1, 2, 4, 8 or 16; 0: 4), waves_per_cu waves per CU (0: 16);
+// FP64 (mixed precision does not apply)
+template <class Net, class Mem>
+inline int launch_wave_queue(CellsArgs a, int wpc, int cpb, hipStream_t s,
+                             Mem&& mem) {
+  if (cpb <= 0) cpb = 4;
+  if (cpb != 1 && cpb != 2 && cpb != 4 && cpb != 8 && cpb != 16) return -1;
+  long slots = 0;
+  if (wave_queue_slots(a.ncell, wpc, cpb, slots) != 0) return -1;
+  PersistMem m;
+  const int r = mem(wave_queue_bytes<Net>(slots),
+                    persist_order_bytes(a.ncell), m);
+  if (r != 0) return r;
+  LaneQueue q;
+  double* work = nullptr;
+  int* iwork = nullptr;
+  if (wave_queue_prepare<Net>(a, m, slots, s, q, work, iwork) != 0)
+    return -1;
+  const size_t lds = sizeof(double) * ((Net::NSPEC + 63) / 64) * 64 * cpb;
+  hipLaunchKernelGGL((wave_queue_kernel<Net>), dim3(slots / cpb),
+                     dim3(64 * cpb), lds, s, a, q, work, iwork, slots);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 // bytes of the workspace a launch needs (0: the network is refused)
 template <class Net, bool ET>
 inline size_t persist_need(const CellsArgs& a, int mixed) {
@@ -2090,7 +2162,11 @@ inline size_t persist_need(const CellsArgs& a, int mixed) {
 template <class Net, bool ET, class Mem>
 inline int launch_persistent(CellsArgs a, int mixed, int chunk, int wpc,
                              hipStream_t s, Mem&& mem) {
-  if constexpr (Problem<Net, ET>::N > KX_PERSIST_MAX_N) {
+  if constexpr (Net::NSPEC > KX_BIG_N && !ET) {
+    // a large network: a wave per cell from the device work queue
+    (void)mixed; (void)chunk;
+    return launch_wave_queue<Net>(a, wpc, 0, s, mem);
+  } else if constexpr (Problem<Net, ET>::N > KX_PERSIST_MAX_N) {
     // too large for the lockstep kernel's registers: the lane queue
     (void)mixed; (void)chunk;
     return launch_lane_queue<Net, ET>(a, wpc, s, mem);

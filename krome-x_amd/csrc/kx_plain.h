@@ -209,6 +209,135 @@ lane_queue_kernel(CellsArgs a, LaneQueue q, double* work, int* iwork,
   if (slot >= S) return;
   lane_queue_run<Net, ET>(a, q, work, iwork, S, slot);
 }
+
+// ---- a wave per cell (networks above KX_BIG_N species) ------------------
+#ifndef KX_WAVE_BLOCK_MAX
+// the largest workgroup of the wave-per-cell kernel: 16 cells of 64 lanes
+#define KX_WAVE_BLOCK_MAX 1024
+#endif
+// doubles of a wave slot: the cell's workspace (contiguous), then its state
+template <class Net>
+constexpr long wave_slot_doubles() {
+  return cell_doubles<WaveProblem<Net>>() + Net::NSPEC;
+}
+
+// the whole per-cell solve of cell c by the calling wave, all 64 lanes
+// (Tgas fixed): its workspace at work (wave_slot_doubles), pivots at iwork,
+// the LU's row accumulator in lw.  Flags and counters from lane 0; the
+// densities read and written by the lane that owns each species
+template <class Net>
+__device__ inline void wave_cell(const CellsArgs& a, int c, double* work,
+                                 int* iwork, WaveLds lw) {
+  using Prob = WaveProblem<Net>;
+  constexpr int NS = Net::NSPEC;
+  constexpr int N = Prob::N;
+  const int l = wave_lane();
+  long* sc = a.stats ? a.stats + (long)c * 9 : nullptr;
+  if (!(a.dt[c] > 0.0)) {  // dt <= 0: untouched (solve_cells.hpp:96-101)
+    if (l == 0) {
+      a.flags[c] = 0;
+      if (sc)
+        for (int i = 0; i < 9; ++i) sc[i] = 0;
+    }
+    return;
+  }
+  Prob P;
+  P.p = Net::params_from_row(a.par + (long)c * a.par_stride);
+  P.Tfix = a.T[c];
+  P.hit_T0 = false;
+  CellOpts<N> co;
+  if (!cell_opts<Net, false, Prob>(a.o, co)) {
+    if (l == 0) a.flags[c] = FLAG_UNSUPPORTED;
+    return;
+  }
+  double* y = work + cell_doubles<Prob>();
+  for (int i = l; i < NS; i += 64) y[i] = a.n[c * a.n_cs + i * a.n_ss];
+  Cell<N, Prob> cv(P, work, iwork, 1, co);
+  cv.lw = lw;
+  P.k = cv.rates_rows();
+  P.k_set = false;
+  cv.reinit(0.0, y, 0.0);
+  CellStats tot;
+  tot.zero();
+  double t = 0.0;
+  int restarts = 0, flag = 0;
+  for (;;) {  // bdf::integrate's restart loop (bdf_cell.hpp:1513-1547)
+    const int f = cv.solve(a.dt[c], y, &t);
+    tot.add(cv.st);
+    if (f >= 0) { flag = 0; break; }
+    const bool again = f == ERR_FAILURE || f == CONV_FAILURE ||
+                       f == REPTD_RHSFUNC_ERR;
+    if (!again || restarts >= co.max_restarts) { flag = f; break; }
+    ++restarts;
+    const double hc = cv.current_step();
+    cv.reinit(t, y, hc > 0.0 ? 0.1 * hc : 0.0);
+  }
+  wave_sync();
+  if (l == 0) {
+    a.flags[c] = flag;
+    if (sc) {
+      sc[0] = tot.nst; sc[1] = tot.nfe; sc[2] = tot.nje; sc[3] = tot.nfeDQ;
+      sc[4] = tot.nsetups; sc[5] = tot.nni; sc[6] = tot.nnf; sc[7] = tot.netf;
+      sc[8] = restarts;
+    }
+  }
+  if (flag != 0) return;
+  // cell_finish split over the lanes (solve_cell.hpp:1458-1490, modes 1
+  // and 3; the charge sums reduced by the butterfly)
+  double q0 = 0.0, q = 0.0, qi = 0.0;
+  for (int i = l; i < NS; i += 64) {
+    const double n0 = a.n[c * a.n_cs + i * a.n_ss];
+    double v = y[i];
+    if (a.o.clip_negative && v < 0.0) v = 0.0;
+    y[i] = v;
+    q0 += Net::charge(i) * n0;
+    q += Net::charge(i) * v;
+    if (!(Net::HAS_E && i == Net::I_E)) qi += Net::charge(i) * v;
+  }
+  q0 = wave_sum(q0);
+  q = wave_sum(q);
+  qi = wave_sum(qi);
+  wave_sync();
+  if constexpr (Net::HAS_E) {
+    if (l == (Net::I_E & 63)) {
+      if (a.o.conserve == 1) y[Net::I_E] = ::fmax(y[Net::I_E] + (q - q0), 0.0);
+      else if (a.o.conserve == 3) y[Net::I_E] = ::fmax(qi, 0.0);
+    }
+  }
+  wave_sync();
+  for (int i = l; i < NS; i += 64) a.n[c * a.n_cs + i * a.n_ss] = y[i];
+}
+
+// each wave takes the next cell of the device work queue (lane 0's
+// atomic, broadcast), solves it with its 64 lanes and takes another until
+// the queue is exhausted; a workgroup holds blockDim / 64 cells at a time
+// ("cells per workgroup"), each wave with its own slot of the workspace
+// and its own 64 columns of the block's LDS (ceil(NSPEC / 64) rows)
+template <class Net>
+__device__ inline void wave_queue_run(const CellsArgs& a, LaneQueue q,
+                                      double* work, int* iwork, long slot,
+                                      WaveLds lw) {
+  constexpr long CD = wave_slot_doubles<Net>();
+  for (;;) {
+    int i = 0;
+    if (wave_lane() == 0) i = atomicAdd(q.counter, 1);
+    i = __shfl(i, 0, 64);
+    if (i >= a.ncell) break;
+    wave_cell<Net>(a, q.order[i], work + slot * CD,
+                   iwork + slot * Net::NSPEC, lw);
+  }
+}
+
+template <class Net>
+__global__ void __launch_bounds__(KX_WAVE_BLOCK_MAX)
+wave_queue_kernel(CellsArgs a, LaneQueue q, double* work, int* iwork,
+                  long slots) {
+  extern __shared__ double wave_lds[];
+  const long slot = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (slot >= slots) return;
+  const WaveLds lw{wave_lds + (threadIdx.x & ~63), (long)blockDim.x};
+  wave_queue_run<Net>(a, q, work, iwork, slot, lw);
+}
 #endif
 
 }  // namespace kx

@@ -45,8 +45,9 @@ struct Problem {
   static constexpr int NS = Net::NSPEC;
   static constexpr int N = NS + (ET ? 1 : 0);
   // the state and the rhs in registers (N values), the saved Jacobian
-  // dense (N x N)
+  // dense (N x N); one thread per cell
   static constexpr bool STRIDED = false;
+  static constexpr int LANES = 1;
   static constexpr long J_DOUBLES = (long)N * N;
   static constexpr long K_DOUBLES = 0;
   // doubles of the Newton matrix's storage: N x N dense, or the sparse
@@ -134,6 +135,7 @@ struct BigProblem {
   static constexpr int NS = Net::NSPEC;
   static constexpr int N = NS;
   static constexpr bool STRIDED = true;
+  static constexpr int LANES = 1;
   static constexpr long J_DOUBLES = Net::SP_NNZ;
   static constexpr long NEWTON_DOUBLES = Net::SP_NNZ + NS;
   static constexpr long K_DOUBLES = Net::NREA;
@@ -166,6 +168,95 @@ struct BigProblem {
     if (!k_set) rates();
     for (long q = 0; q < J_DOUBLES; ++q) Jp[q] = 0.0;
     Net::jac_species_sparse(y, k, Jp);
+    return 0;
+  }
+};
+
+// The same problem solved by the 64 lanes of a wave together (a wave per
+// cell, Cell with LANES 64; the cell's workspace contiguous, S = 1): the
+// rates once per solve (every lane evaluates them and stores the same
+// values), the rhs as the reactions' fluxes split over the lanes, then
+// each species' sum of its fluxes in the loop form's order (the generated
+// LT_ROW_* tables: the same additions in the same order as the scatter,
+// so the same bits), the Jacobian as its terms over the lanes, then each
+// pattern position's sum of its terms in loop order (LT_JPOS_*).  The
+// workspace after the Cell's: the rates, the fluxes and the terms
+template <class Net>
+struct WaveProblem {
+  using NetT = Net;
+  static constexpr int NS = Net::NSPEC;
+  static constexpr int N = NS;
+  static constexpr bool STRIDED = true;
+  static constexpr int LANES = 64;
+  static constexpr long J_DOUBLES = Net::SP_NNZ;
+  static constexpr long NEWTON_DOUBLES = Net::SP_NNZ + NS;
+  static constexpr long K_DOUBLES = 2L * Net::NREA + Net::LT_NTERM;
+  static_assert(!Net::RATES_DENSITY,
+                "the wave form caches the rates: none may read n");
+  typename Net::Params p;
+  double Tfix;
+  bool hit_T0;
+  double* k;     // the cell's rates, then the fluxes, then the terms
+  bool k_set;
+
+  KX_HD double rate_T() const { return ::fmin(::fmax(Tfix, p.Tcmb), 1e9); }
+  KX_HD_CALL void rates() {
+    for (int r = wave_lane(); r < Net::NREA; r += LANES) k[r] = 0.0;
+    wave_sync();
+    Net::rates_T(rate_T(), p, k);
+    wave_sync();
+    k_set = true;
+  }
+  // the rhs at y into f (both in the workspace, contiguous); 1
+  // (recoverable) for a non-finite result, the same on every lane
+  KX_HD int rhs_s(SView y, SView f) {
+    if (!k_set) rates();
+    double* fl = k + Net::NREA;
+    const int l = wave_lane();
+    wave_sync();
+    for (int r = l; r < Net::NREA; r += LANES) {
+      double v = k[r];
+      for (int q = Net::LT_KF_OFF[r]; q < Net::LT_KF_OFF[r + 1]; ++q)
+        v = v * y[Net::LT_KF_SP[q]];
+      fl[r] = v;
+    }
+    wave_sync();
+    double bad = 0.0;
+    for (int i = l; i < NS; i += LANES) {
+      double v = 0.0;
+      for (int e = Net::LT_ROW_OFF[i]; e < Net::LT_ROW_OFF[i + 1]; ++e) {
+        if (Net::LT_ROW_SGN[e] > 0) v += fl[Net::LT_ROW_R[e]];
+        else v -= fl[Net::LT_ROW_R[e]];
+      }
+      f[i] = v;
+      if (!kx::finite(v)) bad = 1.0;
+    }
+    wave_sync();
+    return wave_max(bad) > 0.0 ? 1 : 0;
+  }
+  // the analytic Jacobian at y into the pattern's values Jp
+  KX_HD int jac_s(SView y, SView Jp) {
+    if (!k_set) rates();
+    double* dt = k + 2L * Net::NREA;
+    const int l = wave_lane();
+    wave_sync();
+    for (int t = l; t < Net::LT_NTERM; t += LANES) {
+      double d = k[Net::LT_JT_R[t]];
+      for (int q = Net::LT_JT_FOFF[t]; q < Net::LT_JT_FOFF[t + 1]; ++q)
+        d = d * y[Net::LT_JF_SP[q]];
+      if (Net::LT_JT_MULT[t] > 1) d = d * double(Net::LT_JT_MULT[t]);
+      dt[t] = d;
+    }
+    wave_sync();
+    for (long q = l; q < J_DOUBLES; q += LANES) {
+      double v = 0.0;
+      for (int e = Net::LT_JPOS_OFF[q]; e < Net::LT_JPOS_OFF[q + 1]; ++e) {
+        if (Net::LT_JPOS_SGN[e] > 0) v += dt[Net::LT_JPOS_T[e]];
+        else v -= dt[Net::LT_JPOS_T[e]];
+      }
+      Jp[q] = v;
+    }
+    wave_sync();
     return 0;
   }
 };

@@ -41,8 +41,8 @@ _KEYS = ("evolve_T", "jac", "linsol", "rtol", "atol_n", "atol_T", "atol_u",
          "clip_negative", "error_report", "renormalize_x", "conserve",
          "path", "device_cache", "device_memory", "min_device",
          "min_interleave", "device_kernel", "mixed_precision",
-         "queue_chunk", "waves_per_cu", "lane_block", "checkpoint_steps",
-         "checkpoint_memory")
+         "queue_chunk", "waves_per_cu", "lane_block",
+         "cells_per_workgroup", "checkpoint_steps", "checkpoint_memory")
 LAYOUT_SPECIES_MAJOR = 0
 LAYOUT_CELL_MAJOR = 1
 
@@ -162,7 +162,7 @@ class Multi:
         lib.krome_multi_nnet.restype = ctypes.c_int
         lib.krome_multi_network_name.restype = ctypes.c_char_p
         lib.krome_multi_network_name.argtypes = [ctypes.c_int]
-        for f in ("evolve_T", "lockstep", "nspec", "npar"):
+        for f in ("evolve_T", "lockstep", "mode", "nspec", "npar"):
             getattr(lib, "krome_multi_" + f).restype = ctypes.c_int
             getattr(lib, "krome_multi_" + f).argtypes = [ctypes.c_int]
         lib.krome_multi_set_launch.restype = ctypes.c_int
@@ -175,6 +175,8 @@ class Multi:
                          for i in range(count)]
         self.evolve_T = [lib.krome_multi_evolve_T(i) for i in range(count)]
         self.lockstep = [lib.krome_multi_lockstep(i) for i in range(count)]
+        # 1 the lockstep kernel, 2 a wave per cell, 0 the lane queue
+        self.mode = [lib.krome_multi_mode(i) for i in range(count)]
 
     def set_launch(self, waves_per_cu=0, queue_chunk=0):
         """the launch shape of the following solves (0: the defaults)"""

@@ -504,6 +504,7 @@ def _emit_big_forms(w, net, lt, sp):
     w("  // follows operation for operation)")
     w("  static constexpr int LT_JP[%d] = {%s};"
       % (max(len(jp), 1), ", ".join(str(v) for v in jp) or "0"))
+    _emit_gather_tables(w, net, lt, jp)
     w("  template <class NV, class KV, class JV>")
     w("  KX_HD static void jac_species_sparse(const NV& n, const KV& k, "
       "const JV& Jp) {")
@@ -520,6 +521,56 @@ def _emit_big_forms(w, net, lt, sp):
     w("        }")
     w("      }")
     w("  }")
+
+
+def gather_tables(net, lt, jp):
+    """the scatter of the loop form turned around, for the kernels that
+    split a cell's rhs and Jacobian over the lanes of a wave: per species
+    row, its (reaction, sign) contributions in the loop form's order
+    (row_off / row_r / row_sgn); per term of the Jacobian, its reaction
+    (jt_r); per pattern position, its (term, sign) contributions in loop
+    order (jpos_off / jpos_t / jpos_sgn).  Summed in these orders from 0,
+    each sum takes the same additions in the same order as the scatter"""
+    ns = net.nspec
+    rows = [[] for _ in range(ns)]
+    for r in range(len(net.reactions)):
+        for q in range(lt["sc_off"][r], lt["sc_off"][r + 1]):
+            rows[lt["sc_row"][q]].append((r, lt["sc_sgn"][q]))
+    jt_r = []
+    npos = max(jp) + 1 if jp else 0
+    pos = [[] for _ in range(npos)]
+    m = 0
+    for r in range(len(net.reactions)):
+        for t in range(lt["jt_off"][r], lt["jt_off"][r + 1]):
+            jt_r.append(r)
+            for q in range(lt["sc_off"][r], lt["sc_off"][r + 1]):
+                pos[jp[m]].append((t, lt["sc_sgn"][q]))
+                m += 1
+    out = {"row_off": [0], "row_r": [], "row_sgn": [], "jt_r": jt_r,
+           "jpos_off": [0], "jpos_t": [], "jpos_sgn": []}
+    for lst in rows:
+        out["row_r"] += [r for r, _ in lst]
+        out["row_sgn"] += [g for _, g in lst]
+        out["row_off"].append(len(out["row_r"]))
+    for lst in pos:
+        out["jpos_t"] += [t for t, _ in lst]
+        out["jpos_sgn"] += [g for _, g in lst]
+        out["jpos_off"].append(len(out["jpos_t"]))
+    return out
+
+
+def _emit_gather_tables(w, net, lt, jp):
+    g = gather_tables(net, lt, jp)
+    w("  // the gather form of the scatter (netgen.gather_tables): per")
+    w("  // species row its (reaction, sign) contributions, per Jacobian")
+    w("  // term its reaction, per pattern position its (term, sign)")
+    w("  // contributions, each in the loop form's order")
+    for nm in ("row_off", "row_r", "row_sgn", "jt_r", "jpos_off", "jpos_t",
+               "jpos_sgn"):
+        vals = g[nm] or [0]
+        w("  static constexpr int LT_%s[%d] = {%s};"
+          % (nm.upper(), len(vals), ", ".join(str(v) for v in vals)))
+    w("  static constexpr int LT_NTERM = %d;" % len(g["jt_r"]))
 
 
 def _group(net):

@@ -10,11 +10,17 @@ import autotune
 
 
 def test_space_and_sample():
-    small = autotune.candidates(True)
-    big = autotune.candidates(False)
+    small = autotune.candidates("lockstep")
+    mid = autotune.candidates("lane")
+    big = autotune.candidates("wave")
     kinds = {s["device_kernel"] for s in small}
     assert kinds == {"persistent", "lanes", "plain"}
-    assert {s["device_kernel"] for s in big} == {"lanes", "plain"}
+    assert {s["device_kernel"] for s in mid} == {"lanes", "plain"}
+    assert sorted({s["cells_per_workgroup"] for s in big
+                   if "cells_per_workgroup" in s}) == [1, 2, 4, 8, 16]
+    assert autotune.form_of(9, True) == "lockstep"
+    assert autotune.form_of(27, False) == "lane"
+    assert autotune.form_of(452, False) == "wave"
     assert sorted({s["queue_chunk"] for s in small if "queue_chunk" in s}) \
         == sorted(autotune.QUEUE_CHUNKS)
     idx = autotune.sample_indices(1000, 10)

@@ -248,20 +248,54 @@ def test_gpu_refusals():
 
 @pytest.mark.gpu
 def test_gpu_lane_queue_equals_plain():
-    """device_kernel persistent: above 16 equations, the lane queue (each
-    lane solves cells with the plain kernel's code, taking the next from
-    the device queue): bitwise the plain kernel's results"""
+    """device_kernel lanes: the lane queue (each lane solves cells with
+    the plain kernel's code, taking the next from the device queue):
+    bitwise the plain kernel's results"""
     d = golden("cloud_solve.npz")
     net = krome.load(NET)
     o = d["opts"]
     sel = _cheapest(d)
     res = {}
-    for kern in ("plain", "persistent"):
+    for kern in ("plain", "lanes"):
         opts = net.options(evolve_T=0, rtol=o[1], atol_n=o[2], atol_T=o[3],
                            max_steps=int(o[6]), device_kernel=kern)
         print("cloud:", kern, flush=True)
         res[kern] = net.solve(d["n0"][sel], d["T0"][sel], d["dt"][sel],
                               par=d["par"][sel], opts=opts, check=False)
-    assert res["persistent"].ierr == 0
-    assert np.array_equal(res["persistent"].flags, res["plain"].flags)
-    assert np.array_equal(res["persistent"].n, res["plain"].n)
+    assert res["lanes"].ierr == 0
+    assert np.array_equal(res["lanes"].flags, res["plain"].flags)
+    assert np.array_equal(res["lanes"].n, res["plain"].n)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cpw", [1, 4, 16])
+def test_gpu_wave_per_cell_vs_reference(oracle, cpw):
+    """device_kernel persistent on cloud: a wave per cell (the rhs, the
+    Jacobian, the sparse LU and its solves split over the 64 lanes, the
+    norms by the butterfly), cpw cells per workgroup, every cell of the
+    fixture: the reference's flags, its states within the bar; the same
+    results for every workgroup shape"""
+    d = golden("cloud_solve.npz")
+    net = krome.load(NET)
+    o = d["opts"]
+    opts = net.options(evolve_T=0, rtol=o[1], atol_n=o[2], atol_T=o[3],
+                       max_steps=int(o[6]), device_kernel="persistent",
+                       cells_per_workgroup=cpw)
+    print("cloud: a wave per cell, %d per workgroup" % cpw, flush=True)
+    res = net.solve(d["n0"], d["T0"], d["dt"], par=d["par"], opts=opts,
+                    check=False)
+    assert res.ierr == 0
+    assert np.array_equal(res.flags, d["flag_bitwise"])
+    assert rel_err(res.n, d["n_bitwise"]) < TOL
+    assert rel_err(res.n, d["n_bitwise"], 1e-10) < TOL_MAJOR
+    _, n, _, _, _ = oracle.solve(NET, d["n0"], d["T0"], d["dt"], d["par"],
+                                 _opts(d))
+    assert rel_err(res.n, n) < TOL
+    if cpw != 4:
+        base = net.solve(d["n0"], d["T0"], d["dt"], par=d["par"],
+                         opts=net.options(evolve_T=0, rtol=o[1],
+                                          atol_n=o[2], atol_T=o[3],
+                                          max_steps=int(o[6]),
+                                          device_kernel="persistent"),
+                         check=False)
+        assert np.array_equal(base.n, res.n)

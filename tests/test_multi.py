@@ -204,6 +204,8 @@ def test_c3_exports_and_layout():
     # primordial (12 equations) in the lockstep kernel, the larger two in
     # the lane queue; every network here at fixed Tgas
     assert m.lockstep == [1, 0, 0] and m.evolve_T == [0, 0, 0]
+    # primordialZ (27) in the lane queue, cloud (452) a wave per cell
+    assert m.mode == [1, 0, 2]
     with pytest.raises(Exception):
         m.set_launch(-1, 0)
 
@@ -244,10 +246,7 @@ def test_c3_launch_vs_reference_and_single(wpc, chunk):
     m = lib_c3()
     m.set_launch(wpc, chunk)
     bars = {"primordial": 5e-5, "primordialZ": 3e-4, "cloud": 1e-4}
-    # cloud's cheapest cells: a few cells of the strided form run at the
-    # device memory's latency (tests/test_cloud.py)
-    got = [_golden_batch("primordial"), _golden_batch("primordialZ"),
-           _golden_batch("cloud", 8)]
+    got = [_golden_batch(n) for n in ("primordial", "primordialZ", "cloud")]
     singles = []
     for d, b in got:
         net = krome.load(b["network"])
