@@ -366,6 +366,9 @@ def c3_bench(a):
     parts = (("primordial", cells.c3_primordial_cells, 1 << 19),
              ("primordialZ", cells.c3_primordialZ_cells, 1 << 18),
              ("cloud", cells.c3_cloud_cells, 1 << 17))
+    if a.c3_parts:
+        keep = a.c3_parts.split(",")
+        parts = tuple(q for q in parts if q[0] in keep)
     batches, pristine = [], []
     for name, gen, full in parts:
         nc = max(64, int(full * scale))
@@ -389,7 +392,7 @@ def c3_bench(a):
             b["Tgas"].copy_(T0)
 
     res = {}
-    for mixed in (0, 1):
+    for mixed in ((0,) if a.fp64 else (0, 1)):
         for _ in range(a.warmup):
             restore()
             m.solve_device(batches, mixed)
@@ -416,6 +419,8 @@ def c3_bench(a):
             "steps_max": int(st.max())}
     allst = np.concatenate([b["stats"].cpu().numpy()[:, 0]
                             for b in batches])
+    if 1 not in res:
+        res[1] = res[0]
     wall = res[1][0]
     out = {
         "metric": "cells/sec to t_end, three networks in one persistent "
@@ -586,16 +591,28 @@ def network_bench(a):
     par = torch.tensor(np.ascontiguousarray(w["par"]), device=dev)
     kms = ctypes.c_double()
     kp = ctypes.cast(ctypes.byref(kms), ctypes.POINTER(ctypes.c_double))
-    runs = [("plain_dense", "plain", 1), ("plain_sparse", "plain", 3)]
-    if net.nspec + int(o["evolve_T"]) <= 16:
-        runs.append(("persistent_dense", "persistent", 1))
+    # run names: kernel (plain, lanes, persistent) _ LU (dense, sparse)
+    names = a.kernels.split(",") if a.kernels else (
+        ["plain_dense", "plain_sparse"]
+        + (["persistent_dense"] if net.nspec + int(o["evolve_T"]) <= 16
+           else []))
+    runs = []
+    for key in names:
+        kernel, lu = key.split("_")
+        runs.append((key, kernel, {"dense": 1, "sparse": 3}[lu]))
     out = {}
     for key, kernel, ls in runs:
+        extra = {}
+        if a.waves_per_cu:
+            extra["waves_per_cu"] = a.waves_per_cu
+        if a.cells_per_workgroup:
+            extra["cells_per_workgroup"] = a.cells_per_workgroup
         opts = net.options(evolve_T=bool(o["evolve_T"]), rtol=o["rtol"],
                            atol_n=o["atol_n"], atol_T=o["atol_T"],
                            jac=int(o["jac"]), linsol=ls,
                            max_steps=int(o["max_steps"]),
-                           device_kernel=kernel, error_report=False)
+                           device_kernel=kernel, error_report=False,
+                           **extra)
         n, T = n0.clone(), T0.clone()
         fl = torch.zeros(a.ncell, dtype=torch.int32, device=dev)
         st = torch.zeros((a.ncell, 9), dtype=torch.int64, device=dev)
@@ -741,6 +758,14 @@ def main():
     ap.add_argument("--queue-chunk", type=int, default=0)
     ap.add_argument("--network", default="primordialZ")
     ap.add_argument("--cells", default="c3_primordialZ")
+    ap.add_argument("--kernels", default="",
+                    help="network workload: runs as kernel_lu, comma "
+                         "separated (kernel plain, lanes or persistent; lu "
+                         "dense or sparse)")
+    ap.add_argument("--cells-per-workgroup", type=int, default=0)
+    ap.add_argument("--c3-parts", default="",
+                    help="c3 workload: the networks to include (comma "
+                         "separated; default all three)")
     ap.add_argument("--linsol", default="dense", choices=("dense", "sparse"),
                     help="the Newton matrix's LU (sparse: the plain kernel's "
                          "static-pivot sparse LU, linsol 3)")

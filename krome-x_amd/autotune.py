@@ -42,16 +42,16 @@ KERNELS = ("persistent", "lanes", "plain")
 QUEUE_CHUNKS = (0, 64, 256, 1024, 4096)
 LANE_BLOCKS = (64, 128)
 WAVES_PER_CU = (1, 2, 4)
-# a wave per cell: cells (waves) per workgroup, waves per CU
+# a wave per cell: cells (waves) per workgroup (the waves per CU follow:
+# 8 up to 8 cells a workgroup, 16 at 16)
 CELLS_PER_WORKGROUP = (1, 2, 4, 8, 16)
-WAVE_WAVES_PER_CU = (8, 16)
 
 
 def candidates(kind):
     """the settings tried for a network, by the form its persistent
     launch takes: "lockstep" (up to 16 equations: the lockstep kernel with
     each queue chunk), "wave" (above 64 species at fixed Tgas: a wave per
-    cell with each number of cells per workgroup and waves per CU), else
+    cell with each number of cells per workgroup), else
     "lane"; then the lane queue with each workgroup size and waves per CU,
     and the plain kernel"""
     out = []
@@ -59,10 +59,8 @@ def candidates(kind):
         out += [{"device_kernel": "persistent", "queue_chunk": c}
                 for c in QUEUE_CHUNKS]
     if kind == "wave":
-        out += [{"device_kernel": "persistent", "cells_per_workgroup": c,
-                 "waves_per_cu": w}
-                for c, w in itertools.product(CELLS_PER_WORKGROUP,
-                                              WAVE_WAVES_PER_CU)]
+        out += [{"device_kernel": "persistent", "cells_per_workgroup": c}
+                for c in CELLS_PER_WORKGROUP]
     out += [{"device_kernel": "lanes", "lane_block": b, "waves_per_cu": w}
             for b, w in itertools.product(LANE_BLOCKS, WAVES_PER_CU)]
     out.append({"device_kernel": "plain"})

@@ -43,26 +43,59 @@ KX_HD inline int wave_lane() {
   return 0;
 #endif
 }
-// the sum, minimum and maximum over the wave's 64 lanes: each butterfly
-// step combines two partials in either order (commutative), so every lane
-// ends with the same bits
+// the sum, minimum and maximum over the wave's 64 lanes, the same bits on
+// every lane: a butterfly within each row of 16 lanes by DPP moves (lane
+// pairs 1, 2, then the half-row and row mirrors: each step combines two
+// partials in either order, commutative, so both lanes of a pair agree),
+// then the four row results read as scalars and combined as (r0 + r1) +
+// (r2 + r3).  Called by all 64 lanes together
+#if defined(__HIP_DEVICE_COMPILE__)
+template <int CTRL>
+__device__ inline double wave_dpp(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffu), CTRL,
+                                             0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF,
+                                             0xF, false);
+  return __builtin_bit_cast(
+      double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ inline double wave_read(double v, int lane) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffu), lane);
+  const unsigned hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+template <class Op>
+__device__ inline double wave_reduce(double v, Op op) {
+  v = op(v, wave_dpp<0xB1>(v));   // quad_perm [1, 0, 3, 2]
+  v = op(v, wave_dpp<0x4E>(v));   // quad_perm [2, 3, 0, 1]
+  v = op(v, wave_dpp<0x141>(v));  // row_half_mirror
+  v = op(v, wave_dpp<0x140>(v));  // row_mirror
+  return op(op(wave_read(v, 0), wave_read(v, 16)),
+            op(wave_read(v, 32), wave_read(v, 48)));
+}
+#endif
 KX_HD inline double wave_sum(double v) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-#endif
+  return wave_reduce(v, [](double x, double y) { return x + y; });
+#else
   return v;
+#endif
 }
 KX_HD inline double wave_max(double v) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  for (int o = 32; o > 0; o >>= 1) v = ::fmax(v, __shfl_xor(v, o, 64));
-#endif
+  return wave_reduce(v, [](double x, double y) { return ::fmax(x, y); });
+#else
   return v;
+#endif
 }
 KX_HD inline double wave_min(double v) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  for (int o = 32; o > 0; o >>= 1) v = ::fmin(v, __shfl_xor(v, o, 64));
-#endif
+  return wave_reduce(v, [](double x, double y) { return ::fmin(x, y); });
+#else
   return v;
+#endif
 }
 // what the wave's lanes stored is visible to its other lanes (a wave's
 // memory operations complete in order; this orders the compiler's)
@@ -72,13 +105,40 @@ KX_HD inline void wave_sync() {
   __builtin_amdgcn_wave_barrier();
 #endif
 }
+// pointers the device code may name by address space: LDS (ds_read and
+// ds_write, which wait only on the LDS counter) and global memory
+// (global_load and global_store rather than flat accesses, which make
+// every wait cover both counters); on the host both are plain pointers
+#if defined(__HIP_DEVICE_COMPILE__)
+#define KX_LDS __attribute__((address_space(3)))
+#define KX_GLOBAL __attribute__((address_space(1)))
+#else
+#define KX_LDS
+#define KX_GLOBAL
+#endif
+template <class T>
+KX_HD inline KX_LDS T* lds_ptr(T* p) {
+  return (KX_LDS T*)p;
+}
+template <class T>
+KX_HD inline KX_GLOBAL T* global_ptr(T* p) {
+  return (KX_GLOBAL T*)p;
+}
+
 // the wave's row accumulator of the sparse LU in LDS: element k at
 // base[(k / 64) stride + k % 64] (the wave's own 64 columns of a block's
 // arena of stride doubles per row)
+KX_HD constexpr int imin(int a, int b) { return a < b ? a : b; }
+KX_HD constexpr int imax(int a, int b) { return a > b ? a : b; }
+// rows of 64 doubles a wave's accumulator needs: elements 0 .. NS (NS is
+// the spare slot)
+constexpr long wave_lds_rows(long ns) { return ns / 64 + 1; }
 struct WaveLds {
-  double* base;
+  KX_LDS double* base;
   long stride;
-  KX_HD double& operator[](long k) const {
+  KX_HD WaveLds() : base(nullptr), stride(0) {}
+  KX_HD WaveLds(double* b, long s) : base(lds_ptr(b)), stride(s) {}
+  KX_HD KX_LDS double& operator[](long k) const {
     return base[(k >> 6) * stride + (k & 63)];
   }
 };
@@ -264,7 +324,7 @@ class Cell {
     if constexpr (VS > 1) wave_sync();
   }
   // the LU's row accumulator of a wave per cell (set by its kernel)
-  WaveLds lw{nullptr, 0};
+  WaveLds lw;
   // the view of a workspace vector the strided problems read and write
   KX_HD SView view(const double* v) const {
     return SView{const_cast<double*>(v), S};
@@ -993,62 +1053,185 @@ class Cell {
   // row's entries and each U row's update split over the lanes (distinct
   // columns: the updates are independent, each the serial one's
   // arithmetic), the row accumulator in the wave's LDS (lw); Tgas fixed
-  // (no border)
+  // (no border).  Every value an elimination reads from memory (the pivot
+  // u_kk, the U row of k) is final before the row starts, so a row's
+  // eliminations go WG at a time: all their loads first, then the
+  // eliminations in order.  A lane with no U entry of its own updates the
+  // accumulator's spare slot NS (the wave's LDS holds NS / 64 + 1 rows)
+  static constexpr int WG = 4;
+  KX_HD void wave_elim_group(const KX_GLOBAL double* a, WaveLds lw, int p,
+                             int pd) const {
+    int k[WG], c0[WG], c1[WG];
+    double piv[WG], u0[WG], u1[WG];
+#pragma unroll
+    for (int w = 0; w < WG; ++w) k[w] = Net::sp_ci[imin(p + w, pd - 1)];
+    // every load unconditional (indices clamped into the row: a lane
+    // without the entry loads one it does not use), the selections after
+#pragma unroll
+    for (int w = 0; w < WG; ++w) {
+      const int d = Net::sp_di[k[w]], end = Net::sp_rp[k[w] + 1];
+      const int q0 = d + 1 + vb(), q1 = q0 + VS;
+      const int e0 = imin(q0, end - 1), e1 = imin(q1, end - 1);
+      piv[w] = a[d];
+      const int ci0 = Net::sp_ci[e0], ci1 = Net::sp_ci[e1];
+      u0[w] = a[e0];
+      u1[w] = a[e1];
+      c0[w] = q0 < end ? ci0 : NS;
+      c1[w] = q1 < end ? ci1 : NS;
+    }
+#pragma unroll
+    for (int w = 0; w < WG; ++w) {
+      if (p + w < pd) {
+        // columns k < c0 < c1 (or the spare slot): the three reads first
+        const double xk = lw[k[w]], x0 = lw[c0[w]], x1 = lw[c1[w]];
+        const double lik = xk / piv[w];
+        lw[k[w]] = lik;
+        lw[c0[w]] = x0 - lik * u0[w];
+        lw[c1[w]] = x1 - lik * u1[w];
+        // U rows longer than two lanes' worth (none in most networks)
+        const int end = Net::sp_rp[k[w] + 1];
+        for (int q = Net::sp_di[k[w]] + 1 + 2 * VS + vb(); q < end; q += VS)
+          lw[Net::sp_ci[q]] -= lik * a[q];
+      }
+    }
+  }
   KX_HD int sp_factor_wave(double c) {
     static_assert(NB == 0, "a wave per cell serves fixed-Tgas networks");
+    static_assert(VS == 64, "a wave per cell has 64 lanes");
+    // the wave's workspace is contiguous (S = 1)
+    KX_GLOBAL double* a = global_ptr(A);
+    const KX_GLOBAL double* j = global_ptr(J);
+    const WaveLds lw = this->lw;  // the pointers in registers
     vsync();
-    for (int p = vb(); p < Net::SP_NNZ; p += VS) spv(p) = at(J, p) * c;
+    for (int p = vb(); p < Net::SP_NNZ; p += VS) a[p] = j[p] * c;
     vsync();
-    for (int k = vb(); k < NS; k += VS) spv(Net::sp_di[k]) += 1.0;
+    for (int k = vb(); k < NS; k += VS) a[Net::sp_di[k]] += 1.0;
     vsync();
+    // the lane's first entry of row i (column cn, value vn; none: the
+    // spare slot), loaded a row ahead: a row's values change only when it
+    // is written back
+    int cn = NS;
+    double vn = 0.0;
+    {
+      const int p = Net::sp_rp[0] + vb(), e = imin(p, Net::sp_rp[1] - 1);
+      const int ce = Net::sp_ci[e];
+      const double ve = a[e];
+      cn = p < Net::sp_rp[1] ? ce : NS;
+      vn = ve;
+    }
     for (int i = 0; i < NS; i++) {
-      const int p0 = Net::sp_rp[i], p1 = Net::sp_rp[i + 1];
-      for (int p = p0 + vb(); p < p1; p += VS) lw[Net::sp_ci[p]] = spv(p);
-      vsync();
-      for (int p = p0; p < Net::sp_di[i]; p++) {
-        const int k = Net::sp_ci[p];
-        const double lik = lw[k] / spv(Net::sp_di[k]);
-        vsync();
-        lw[k] = lik;
-        for (int q = Net::sp_di[k] + 1 + vb(); q < Net::sp_rp[k + 1];
-             q += VS)
-          lw[Net::sp_ci[q]] -= lik * spv(q);
-        vsync();
+      const int p0 = Net::sp_rp[i], pd = Net::sp_di[i],
+                p1 = Net::sp_rp[i + 1];
+      lw[cn] = vn;
+      for (int p = p0 + VS + vb(); p < p1; p += VS) lw[Net::sp_ci[p]] = a[p];
+      if (i + 1 < NS) {
+        const int p = p1 + vb(), e = imin(p, Net::sp_rp[i + 2] - 1);
+        const int ce = Net::sp_ci[e];
+        const double ve = a[e];
+        cn = p < Net::sp_rp[i + 2] ? ce : NS;
+        vn = ve;
       }
+      vsync();
+      for (int p = p0; p < pd; p += WG) wave_elim_group(a, lw, p, pd);
+      vsync();
       const double pv = lw[i];
       if (pv == 0.0 || !(::fabs(pv) <= DBL_MAX)) return 1;
-      for (int p = p0 + vb(); p < p1; p += VS) spv(p) = lw[Net::sp_ci[p]];
+      for (int p = p0 + vb(); p < p1; p += VS) a[p] = lw[Net::sp_ci[p]];
       vsync();
     }
     return 0;
   }
+  // this lane's part of the dot product of the factor entries [p, pe) with
+  // the LDS vector lw (lane-strided), the first entry (if any) given: its
+  // value v0 and column c0; the rest four chunks at a time, loads first
+  KX_HD static double wave_row_dot(const KX_GLOBAL double* a, WaveLds lw,
+                                   int p, int pe, double v0, int c0) {
+    double part = v0 * lw[c0];
+    for (int q = p + VS; q < pe; q += 4 * VS) {
+      int cc[4];
+      double vv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int qq = imin(q + u * VS, pe - 1);
+        const int cq = Net::sp_ci[qq];
+        vv[u] = a[qq];
+        cc[u] = q + u * VS < pe ? cq : NS;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (q + u * VS < pe) part += vv[u] * lw[cc[u]];
+    }
+    return part;
+  }
   // b = (c J + I)^-1 b with sp_factor_wave's factors, b in the workspace:
-  // each row's dot product split over the lanes and reduced by the
-  // butterfly (the order of its additions differs from the serial solve)
+  // each row's dot product split over the lanes and reduced over the wave
+  // (the order of its additions differs from the serial solve).  The
+  // factors do not change during the solve, so the lane's first entries
+  // (and the pivots) of WG rows are loaded together ahead of their sums
   KX_HD void sp_solve_wave(double* bv) {
+    static_assert(VS == 64, "a wave per cell has 64 lanes");
+    const KX_GLOBAL double* a = global_ptr(A);
+    KX_GLOBAL double* g = global_ptr(bv);
+    const WaveLds lw = this->lw;
+    const int l = vb();
     vsync();
-    for (int k = vb(); k < NS; k += VS) lw[k] = at(bv, Net::sp_perm[k]);
+    for (int k = l; k < NS; k += VS) lw[k] = g[Net::sp_perm[k]];
+    lw[NS] = 0.0;  // the spare slot: a lane's missing entry reads 0 there
     vsync();
-    for (int i = 0; i < NS; i++) {
-      double part = 0.0;
-      for (int p = Net::sp_rp[i] + vb(); p < Net::sp_di[i]; p += VS)
-        part += spv(p) * lw[Net::sp_ci[p]];
-      const double v = lw[i] - vsum(part);
-      vsync();
-      lw[i] = v;
-      vsync();
+    for (int i0 = 0; i0 < NS; i0 += WG) {  // L y = b
+      int c[WG];
+      double v[WG];
+#pragma unroll
+      for (int w = 0; w < WG; ++w) {
+        const int i = imin(i0 + w, NS - 1);
+        const int p = Net::sp_rp[i] + l, pe = Net::sp_di[i];
+        const int e = imin(p, pe);  // pe: the diagonal, in the row
+        const int ce = Net::sp_ci[e];
+        const double ve = a[e];
+        c[w] = p < pe ? ce : NS;
+        v[w] = p < pe ? ve : 0.0;
+      }
+#pragma unroll
+      for (int w = 0; w < WG; ++w) {
+        const int i = i0 + w;
+        if (i < NS) {
+          const double part = wave_row_dot(a, lw, Net::sp_rp[i] + l,
+                                           Net::sp_di[i], v[w], c[w]);
+          const double y = lw[i] - vsum(part);
+          vsync();
+          lw[i] = y;
+          vsync();
+        }
+      }
     }
-    for (int i = NS - 1; i >= 0; i--) {
-      double part = 0.0;
-      for (int p = Net::sp_di[i] + 1 + vb(); p < Net::sp_rp[i + 1];
-           p += VS)
-        part += spv(p) * lw[Net::sp_ci[p]];
-      const double v = (lw[i] - vsum(part)) / spv(Net::sp_di[i]);
-      vsync();
-      lw[i] = v;
-      vsync();
+    for (int i0 = NS - 1; i0 >= 0; i0 -= WG) {  // U x = y
+      int c[WG];
+      double v[WG], d[WG];
+#pragma unroll
+      for (int w = 0; w < WG; ++w) {
+        const int i = imax(i0 - w, 0);
+        const int p = Net::sp_di[i] + 1 + l, pe = Net::sp_rp[i + 1];
+        const int e = imin(p, pe - 1);  // the diagonal at least
+        const int ce = Net::sp_ci[e];
+        const double ve = a[e];
+        c[w] = p < pe ? ce : NS;
+        v[w] = p < pe ? ve : 0.0;
+        d[w] = a[Net::sp_di[i]];
+      }
+#pragma unroll
+      for (int w = 0; w < WG; ++w) {
+        const int i = i0 - w;
+        if (i >= 0) {
+          const double part = wave_row_dot(a, lw, Net::sp_di[i] + 1 + l,
+                                           Net::sp_rp[i + 1], v[w], c[w]);
+          const double x = (lw[i] - vsum(part)) / d[w];
+          vsync();
+          lw[i] = x;
+          vsync();
+        }
+      }
     }
-    for (int k = vb(); k < NS; k += VS) at(bv, Net::sp_perm[k]) = lw[k];
+    for (int k = l; k < NS; k += VS) g[Net::sp_perm[k]] = lw[k];
     vsync();
   }
   // x = L^-1 x (unit lower triangle), x at A + x0 (strided)

@@ -50,7 +50,7 @@ struct MultiPart {
   // block's rows)
   static constexpr long LDS =
       LOCKSTEP ? lds_doubles<Net, ET>()
-               : (WAVE ? (long)((Net::NSPEC + 63) / 64) * KX_PERSIST_BLOCK
+               : (WAVE ? wave_lds_rows(Net::NSPEC) * KX_PERSIST_BLOCK
                        : 0);
   CellsArgs a;
   int on;  // cells in this launch

@@ -2074,7 +2074,8 @@ inline int launch_lane_queue(CellsArgs a, int wpc, hipStream_t s,
 
 // ---- a wave per cell: networks above KX_BIG_N species --------------------
 // wave slots of a launch over ncell cells: waves_per_cu waves on each CU
-// (0: 16), cells per workgroup cpb, no more than the cells rounded up to a
+// (0: as many as are resident at once, 8 up to 8 cells a workgroup, 16 at
+// 16), cells per workgroup cpb, no more than the cells rounded up to a
 // workgroup
 inline int wave_queue_slots(long ncell, int wpc, int cpb, long& slots) {
   int dev = 0, ncu = 0;
@@ -2082,7 +2083,7 @@ inline int wave_queue_slots(long ncell, int wpc, int cpb, long& slots) {
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
                             dev) != hipSuccess)
     return -1;
-  if (wpc <= 0) wpc = 16;
+  if (wpc <= 0) wpc = cpb > 8 ? 16 : 8;
   slots = (long)ncu * wpc;
   if (slots > ncell) slots = ncell;
   slots = (slots + cpb - 1) / cpb * cpb;
@@ -2120,7 +2121,7 @@ int wave_queue_prepare(CellsArgs a, PersistMem m, long slots, hipStream_t s,
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 // the wave-per-cell launch with its memory from the caller: cpb cells per
-// workgroup (1, 2, 4, 8 or 16; 0: 4), waves_per_cu waves per CU (0: 16);
+// workgroup (1, 2, 4, 8 or 16; 0: 4), waves_per_cu waves per CU (0: 8, or 16 at 16 cells);
 // FP64 (mixed precision does not apply)
 template <class Net, class Mem>
 inline int launch_wave_queue(CellsArgs a, int wpc, int cpb, hipStream_t s,
@@ -2138,9 +2139,13 @@ inline int launch_wave_queue(CellsArgs a, int wpc, int cpb, hipStream_t s,
   int* iwork = nullptr;
   if (wave_queue_prepare<Net>(a, m, slots, s, q, work, iwork) != 0)
     return -1;
-  const size_t lds = sizeof(double) * ((Net::NSPEC + 63) / 64) * 64 * cpb;
-  hipLaunchKernelGGL((wave_queue_kernel<Net>), dim3(slots / cpb),
-                     dim3(64 * cpb), lds, s, a, q, work, iwork, slots);
+  const size_t lds = sizeof(double) * wave_lds_rows(Net::NSPEC) * 64 * cpb;
+  if (cpb > 8)
+    hipLaunchKernelGGL((wave_queue_kernel<Net, 1024>), dim3(slots / cpb),
+                       dim3(64 * cpb), lds, s, a, q, work, iwork, slots);
+  else
+    hipLaunchKernelGGL((wave_queue_kernel<Net, 512>), dim3(slots / cpb),
+                       dim3(64 * cpb), lds, s, a, q, work, iwork, slots);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

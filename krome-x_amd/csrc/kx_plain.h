@@ -211,10 +211,6 @@ lane_queue_kernel(CellsArgs a, LaneQueue q, double* work, int* iwork,
 }
 
 // ---- a wave per cell (networks above KX_BIG_N species) ------------------
-#ifndef KX_WAVE_BLOCK_MAX
-// the largest workgroup of the wave-per-cell kernel: 16 cells of 64 lanes
-#define KX_WAVE_BLOCK_MAX 1024
-#endif
 // doubles of a wave slot: the cell's workspace (contiguous), then its state
 template <class Net>
 constexpr long wave_slot_doubles() {
@@ -226,7 +222,7 @@ constexpr long wave_slot_doubles() {
 // the LU's row accumulator in lw.  Flags and counters from lane 0; the
 // densities read and written by the lane that owns each species
 template <class Net>
-__device__ inline void wave_cell(const CellsArgs& a, int c, double* work,
+__device__ __attribute__((always_inline)) inline void wave_cell(const CellsArgs& a, int c, double* work,
                                  int* iwork, WaveLds lw) {
   using Prob = WaveProblem<Net>;
   constexpr int NS = Net::NSPEC;
@@ -312,9 +308,9 @@ __device__ inline void wave_cell(const CellsArgs& a, int c, double* work,
 // atomic, broadcast), solves it with its 64 lanes and takes another until
 // the queue is exhausted; a workgroup holds blockDim / 64 cells at a time
 // ("cells per workgroup"), each wave with its own slot of the workspace
-// and its own 64 columns of the block's LDS (ceil(NSPEC / 64) rows)
+// and its own 64 columns of the block's LDS (NSPEC / 64 + 1 rows)
 template <class Net>
-__device__ inline void wave_queue_run(const CellsArgs& a, LaneQueue q,
+__device__ __attribute__((always_inline)) inline void wave_queue_run(const CellsArgs& a, LaneQueue q,
                                       double* work, int* iwork, long slot,
                                       WaveLds lw) {
   constexpr long CD = wave_slot_doubles<Net>();
@@ -328,8 +324,11 @@ __device__ inline void wave_queue_run(const CellsArgs& a, LaneQueue q,
   }
 }
 
-template <class Net>
-__global__ void __launch_bounds__(KX_WAVE_BLOCK_MAX)
+// MAXB: the largest workgroup the instantiation serves.  Up to 8 cells
+// (512 lanes) a workgroup its waves may hold 256 VGPRs (two waves per
+// SIMD); 16 cells (1024 lanes) allow 128 (four per SIMD)
+template <class Net, int MAXB>
+__global__ void __launch_bounds__(MAXB)
 wave_queue_kernel(CellsArgs a, LaneQueue q, double* work, int* iwork,
                   long slots) {
   extern __shared__ double wave_lds[];

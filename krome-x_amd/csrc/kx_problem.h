@@ -177,10 +177,12 @@ struct BigProblem {
 // rates once per solve (every lane evaluates them and stores the same
 // values), the rhs as the reactions' fluxes split over the lanes, then
 // each species' sum of its fluxes in the loop form's order (the generated
-// LT_ROW_* tables: the same additions in the same order as the scatter,
-// so the same bits), the Jacobian as its terms over the lanes, then each
-// pattern position's sum of its terms in loop order (LT_JPOS_*).  The
-// workspace after the Cell's: the rates, the fluxes and the terms
+// LT_ROW_* tables; a row of up to 32 contributions takes the scatter's
+// additions in its order, so the same bits; a longer row is summed in
+// runs of 32, LT_SEG_*, whose sums are added in order), the Jacobian as
+// its terms over the lanes, then each pattern position's sum of its terms
+// in loop order (LT_JPOS_*: the scatter's bits).  The workspace after the
+// Cell's: the rates, the fluxes and the terms or runs
 template <class Net>
 struct WaveProblem {
   using NetT = Net;
@@ -190,7 +192,11 @@ struct WaveProblem {
   static constexpr int LANES = 64;
   static constexpr long J_DOUBLES = Net::SP_NNZ;
   static constexpr long NEWTON_DOUBLES = Net::SP_NNZ + NS;
-  static constexpr long K_DOUBLES = 2L * Net::NREA + Net::LT_NTERM;
+  // the rates, the fluxes, then the Jacobian's terms or the rhs runs'
+  // partial sums (never needed together)
+  static constexpr long K_DOUBLES =
+      2L * Net::NREA +
+      (Net::LT_NTERM > Net::LT_NRUN ? Net::LT_NTERM : Net::LT_NRUN);
   static_assert(!Net::RATES_DENSITY,
                 "the wave form caches the rates: none may read n");
   typename Net::Params p;
@@ -207,55 +213,135 @@ struct WaveProblem {
     wave_sync();
     k_set = true;
   }
+  // reactions a lane takes per batch, and contributions per gather chunk:
+  // the loads of a batch or chunk are all issued before its arithmetic, so
+  // they wait on memory once, not once each
+  static constexpr int RB = 4;
+  static constexpr int GC = 8;
+  // sum of the contributions [e, e1) of a signed table (i added, ~i
+  // subtracted) of the values x, from v, in table order
+  KX_HD static double gather(const int* tab, int e, int e1,
+                             const KX_GLOBAL double* x, double v) {
+    for (; e < e1; e += GC) {
+      int ix[GC];
+      double xv[GC];
+#pragma unroll
+      for (int j = 0; j < GC; ++j) ix[j] = tab[imin(e + j, e1 - 1)];
+#pragma unroll
+      for (int j = 0; j < GC; ++j) xv[j] = x[ix[j] >= 0 ? ix[j] : ~ix[j]];
+#pragma unroll
+      for (int j = 0; j < GC; ++j)
+        if (e + j < e1) v = ix[j] >= 0 ? v + xv[j] : v - xv[j];
+    }
+    return v;
+  }
   // the rhs at y into f (both in the workspace, contiguous); 1
-  // (recoverable) for a non-finite result, the same on every lane
+  // (recoverable) for a non-finite result, the same on every lane.  The
+  // fluxes k_r prod y over the lanes; then each species row's sum of its
+  // contributions in the loop form's order: a row of up to 32
+  // contributions is summed whole by one lane (the scatter's arithmetic),
+  // a longer one in runs of 32 on several lanes whose partial sums are
+  // then added in order
   KX_HD int rhs_s(SView y, SView f) {
     if (!k_set) rates();
-    double* fl = k + Net::NREA;
+    const KX_GLOBAL double* kr = global_ptr(k);
+    KX_GLOBAL double* fl = global_ptr(k + Net::NREA);
+    KX_GLOBAL double* part = global_ptr(k + 2L * Net::NREA);
+    const KX_GLOBAL double* yv = global_ptr(y.p);
+    const long ys = y.s;
     const int l = wave_lane();
+    constexpr int W = Net::LT_KF_W;
     wave_sync();
-    for (int r = l; r < Net::NREA; r += LANES) {
-      double v = k[r];
-      for (int q = Net::LT_KF_OFF[r]; q < Net::LT_KF_OFF[r + 1]; ++q)
-        v = v * y[Net::LT_KF_SP[q]];
-      fl[r] = v;
+    for (int r0 = l; r0 < Net::NREA; r0 += RB * LANES) {
+      double kv[RB], xv[RB][W];
+      int sv[RB][W];
+#pragma unroll
+      for (int b = 0; b < RB; ++b) {
+        const int r = imin(r0 + b * LANES, Net::NREA - 1);
+        kv[b] = kr[r];
+#pragma unroll
+        for (int j = 0; j < W; ++j) sv[b][j] = Net::LT_KFW[r * W + j];
+      }
+#pragma unroll
+      for (int b = 0; b < RB; ++b)
+#pragma unroll
+        for (int j = 0; j < W; ++j)
+          xv[b][j] = yv[(sv[b][j] >= 0 ? sv[b][j] : 0) * ys];
+#pragma unroll
+      for (int b = 0; b < RB; ++b) {
+        if (r0 + b * LANES < Net::NREA) {
+          double v = kv[b];
+#pragma unroll
+          for (int j = 0; j < W; ++j)
+            if (sv[b][j] >= 0) v = v * xv[b][j];
+          fl[r0 + b * LANES] = v;
+        }
+      }
+    }
+    wave_sync();
+    for (int s = l; s < Net::LT_NSEG; s += LANES) {
+      const int id = Net::LT_SEG_ID[s];
+      const double v = gather(Net::LT_ROW_RS, Net::LT_SEG_LO[s],
+                              Net::LT_SEG_HI[s], fl, 0.0);
+      if (id >= 0) part[id] = v;
     }
     wave_sync();
     double bad = 0.0;
     for (int i = l; i < NS; i += LANES) {
-      double v = 0.0;
-      for (int e = Net::LT_ROW_OFF[i]; e < Net::LT_ROW_OFF[i + 1]; ++e) {
-        if (Net::LT_ROW_SGN[e] > 0) v += fl[Net::LT_ROW_R[e]];
-        else v -= fl[Net::LT_ROW_R[e]];
-      }
+      const int a = Net::LT_RSEG_OFF[i], b = Net::LT_RSEG_OFF[i + 1];
+      double v = part[a];
+      for (int q = a + 1; q < b; ++q) v += part[q];
       f[i] = v;
       if (!kx::finite(v)) bad = 1.0;
     }
     wave_sync();
     return wave_max(bad) > 0.0 ? 1 : 0;
   }
-  // the analytic Jacobian at y into the pattern's values Jp
+  // the analytic Jacobian at y into the pattern's values Jp: the terms
+  // over the lanes, then each pattern position's sum of its terms in the
+  // loop form's order (the scatter's arithmetic)
   KX_HD int jac_s(SView y, SView Jp) {
     if (!k_set) rates();
-    double* dt = k + 2L * Net::NREA;
+    const KX_GLOBAL double* kr = global_ptr(k);
+    KX_GLOBAL double* dt = global_ptr(k + 2L * Net::NREA);
+    const KX_GLOBAL double* yv = global_ptr(y.p);
+    KX_GLOBAL double* jv = global_ptr(Jp.p);
+    const long ys = y.s, js = Jp.s;
     const int l = wave_lane();
+    constexpr int W = Net::LT_JF_W;
     wave_sync();
-    for (int t = l; t < Net::LT_NTERM; t += LANES) {
-      double d = k[Net::LT_JT_R[t]];
-      for (int q = Net::LT_JT_FOFF[t]; q < Net::LT_JT_FOFF[t + 1]; ++q)
-        d = d * y[Net::LT_JF_SP[q]];
-      if (Net::LT_JT_MULT[t] > 1) d = d * double(Net::LT_JT_MULT[t]);
-      dt[t] = d;
-    }
-    wave_sync();
-    for (long q = l; q < J_DOUBLES; q += LANES) {
-      double v = 0.0;
-      for (int e = Net::LT_JPOS_OFF[q]; e < Net::LT_JPOS_OFF[q + 1]; ++e) {
-        if (Net::LT_JPOS_SGN[e] > 0) v += dt[Net::LT_JPOS_T[e]];
-        else v -= dt[Net::LT_JPOS_T[e]];
+    for (int t0 = l; t0 < Net::LT_NTERM; t0 += RB * LANES) {
+      double kv[RB], xv[RB][W];
+      int sv[RB][W], mv[RB];
+#pragma unroll
+      for (int b = 0; b < RB; ++b) {
+        const int t = imin(t0 + b * LANES, Net::LT_NTERM - 1);
+        kv[b] = kr[Net::LT_JT_R[t]];
+        mv[b] = Net::LT_JT_MULT[t];
+#pragma unroll
+        for (int j = 0; j < W; ++j) sv[b][j] = Net::LT_JFW[t * W + j];
       }
-      Jp[q] = v;
+#pragma unroll
+      for (int b = 0; b < RB; ++b)
+#pragma unroll
+        for (int j = 0; j < W; ++j)
+          xv[b][j] = yv[(sv[b][j] >= 0 ? sv[b][j] : 0) * ys];
+#pragma unroll
+      for (int b = 0; b < RB; ++b) {
+        if (t0 + b * LANES < Net::LT_NTERM) {
+          double d = kv[b];
+#pragma unroll
+          for (int j = 0; j < W; ++j)
+            if (sv[b][j] >= 0) d = d * xv[b][j];
+          if (mv[b] > 1) d = d * double(mv[b]);
+          dt[t0 + b * LANES] = d;
+        }
+      }
     }
+    wave_sync();
+    for (long q = l; q < J_DOUBLES; q += LANES)
+      jv[q * js] = gather(Net::LT_JPOS_TS, Net::LT_JPOS_OFF[q],
+                          Net::LT_JPOS_OFF[q + 1], dt, 0.0);
     wave_sync();
     return 0;
   }

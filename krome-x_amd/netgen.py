@@ -546,31 +546,109 @@ def gather_tables(net, lt, jp):
             for q in range(lt["sc_off"][r], lt["sc_off"][r + 1]):
                 pos[jp[m]].append((t, lt["sc_sgn"][q]))
                 m += 1
-    out = {"row_off": [0], "row_r": [], "row_sgn": [], "jt_r": jt_r,
-           "jpos_off": [0], "jpos_t": [], "jpos_sgn": []}
+    out = {"row_off": [0], "row_rs": [], "jt_r": jt_r,
+           "jpos_off": [0], "jpos_ts": []}
+    # a contribution's index and sign in one entry: r when added, ~r
+    # (= -r - 1) when subtracted
     for lst in rows:
-        out["row_r"] += [r for r, _ in lst]
-        out["row_sgn"] += [g for _, g in lst]
-        out["row_off"].append(len(out["row_r"]))
+        out["row_rs"] += [r if g > 0 else ~r for r, g in lst]
+        out["row_off"].append(len(out["row_rs"]))
     for lst in pos:
-        out["jpos_t"] += [t for t, _ in lst]
-        out["jpos_sgn"] += [g for _, g in lst]
-        out["jpos_off"].append(len(out["jpos_t"]))
+        out["jpos_ts"] += [t if g > 0 else ~t for t, g in lst]
+        out["jpos_off"].append(len(out["jpos_ts"]))
+    # the kinetic reactants of each reaction and the other factors of each
+    # Jacobian term in fixed-width rows (-1 pads), in the loop form's order
+    kw = max([lt["kf_off"][r + 1] - lt["kf_off"][r]
+              for r in range(len(net.reactions))] + [1])
+    out["kf_w"] = kw
+    out["kfw"] = []
+    for r in range(len(net.reactions)):
+        sp = lt["kf_sp"][lt["kf_off"][r]:lt["kf_off"][r + 1]]
+        out["kfw"] += list(sp) + [-1] * (kw - len(sp))
+    nt = len(jt_r)
+    jw = max([lt["jt_foff"][t + 1] - lt["jt_foff"][t] for t in range(nt)]
+             + [1])
+    out["jf_w"] = jw
+    out["jfw"] = []
+    for t in range(nt):
+        sp = lt["jf_sp"][lt["jt_foff"][t]:lt["jt_foff"][t + 1]]
+        out["jfw"] += list(sp) + [-1] * (jw - len(sp))
     return out
+
+
+# the lanes of a wave and the longest run of a row's contributions one
+# lane sums (the wave form's rhs segments)
+WAVE_LANES = 64
+SEG_TERMS = 32
+
+
+def row_segments(row_off, lanes=WAVE_LANES, seg=SEG_TERMS):
+    """the wave form's rhs split: each species row's contributions cut into
+    runs of at most seg (a row of up to seg contributions is one run), the
+    runs dealt to the lanes longest first, each to the least loaded lane
+    (ties: the lowest lane).  Returns (lo, hi, sid): lane l's j-th run at
+    index l + lanes j covers contributions [lo, hi) and is the row-order
+    run sid (empty pads: lo = hi, sid -1), and rseg_off: row i's runs are
+    sid rseg_off[i] .. rseg_off[i + 1] - 1 in row order"""
+    runs, rseg_off = [], [0]
+    for i in range(len(row_off) - 1):
+        a, b = row_off[i], row_off[i + 1]
+        if b == a:
+            runs.append((a, a))
+        while a < b:
+            runs.append((a, min(a + seg, b)))
+            a += seg
+        rseg_off.append(len(runs))
+    load = [0] * lanes
+    per = [[] for _ in range(lanes)]
+    for sid in sorted(range(len(runs)),
+                      key=lambda q: (-(runs[q][1] - runs[q][0]), q)):
+        l = min(range(lanes), key=lambda x: (load[x], x))
+        per[l].append(sid)
+        load[l] += runs[sid][1] - runs[sid][0]
+    depth = max(len(x) for x in per)
+    lo, hi, ids = [], [], []
+    for j in range(depth):
+        for l in range(lanes):
+            if j < len(per[l]):
+                sid = per[l][j]
+                lo.append(runs[sid][0])
+                hi.append(runs[sid][1])
+                ids.append(sid)
+            else:
+                lo.append(0)
+                hi.append(0)
+                ids.append(-1)
+    return lo, hi, ids, rseg_off
 
 
 def _emit_gather_tables(w, net, lt, jp):
     g = gather_tables(net, lt, jp)
+    lo, hi, ids, rseg = row_segments(g["row_off"])
+    w("  // the wave form's rhs runs (netgen.row_segments): lane l's j-th")
+    w("  // run at l + 64 j sums contributions [LT_SEG_LO, LT_SEG_HI) of")
+    w("  // LT_ROW_RS into run LT_SEG_ID (-1: a pad); row i adds its runs")
+    w("  // LT_RSEG_OFF[i] .. LT_RSEG_OFF[i + 1] - 1 in order")
+    for nm, vals in (("seg_lo", lo), ("seg_hi", hi), ("seg_id", ids),
+                     ("rseg_off", rseg)):
+        w("  static constexpr int LT_%s[%d] = {%s};"
+          % (nm.upper(), len(vals), ", ".join(str(v) for v in vals)))
+    w("  static constexpr int LT_NSEG = %d;" % len(lo))
+    w("  static constexpr int LT_NRUN = %d;" % rseg[-1])
     w("  // the gather form of the scatter (netgen.gather_tables): per")
-    w("  // species row its (reaction, sign) contributions, per Jacobian")
-    w("  // term its reaction, per pattern position its (term, sign)")
-    w("  // contributions, each in the loop form's order")
-    for nm in ("row_off", "row_r", "row_sgn", "jt_r", "jpos_off", "jpos_t",
-               "jpos_sgn"):
+    w("  // species row its contributions (reaction r added, ~r subtracted),")
+    w("  // per Jacobian term its reaction, per pattern position its")
+    w("  // contributions (term t or ~t), each in the loop form's order; the")
+    w("  // reactants of each reaction and the other factors of each term in")
+    w("  // rows of LT_KF_W / LT_JF_W entries (-1 pads)")
+    for nm in ("row_off", "row_rs", "jt_r", "jpos_off", "jpos_ts", "kfw",
+               "jfw"):
         vals = g[nm] or [0]
         w("  static constexpr int LT_%s[%d] = {%s};"
           % (nm.upper(), len(vals), ", ".join(str(v) for v in vals)))
     w("  static constexpr int LT_NTERM = %d;" % len(g["jt_r"]))
+    w("  static constexpr int LT_KF_W = %d;" % g["kf_w"])
+    w("  static constexpr int LT_JF_W = %d;" % g["jf_w"])
 
 
 def _group(net):

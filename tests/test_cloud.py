@@ -10,6 +10,9 @@ cloud_*.npz, written by tests/golden/make_golden.py cloud):
 - the position table of the sparse Jacobian (LT_JP): every (term,
   scatter row) pair of the loop tables lands on the pattern entry of its
   own (row, column), decoded through the reference's tables;
+- the wave form's gather tables against the loop form's scatter, bit for
+  bit on random fluxes and terms; its rhs runs cover each contribution
+  once, balanced over the lanes;
 - rates within 1e-9 relative (the same zeros), the rhs within 1e-10 of
   its row scale, the analytic Jacobian within 1e-12 of its scale;
 - whole solves: the oracle against the reference's solve_cells (64 cells
@@ -110,6 +113,93 @@ def test_sparse_jacobian_positions():
     assert len(jp) == len(rows)
     assert np.array_equal(perm[row_of[jp]], rows)
     assert np.array_equal(perm[ci[jp]], cols)
+
+
+def _scatter_rows(vals, off, row, sgn, nrow):
+    """the loop form's scatter: for each source in order, its rows +/- v"""
+    f = [0.0] * nrow
+    for r in range(len(off) - 1):
+        v = float(vals[r])
+        for q in range(off[r], off[r + 1]):
+            f[row[q]] = f[row[q]] + v if sgn[q] > 0 else f[row[q]] - v
+    return f
+
+
+def _signed_sum(tab, e0, e1, vals, v=0.0):
+    for e in range(e0, e1):
+        x = tab[e]
+        v = v + float(vals[x]) if x >= 0 else v - float(vals[~x])
+    return v
+
+
+def test_gather_tables_equal_the_scatter():
+    """the wave form's gather tables (LT_ROW_*, LT_JPOS_*): each species
+    row's and each pattern position's signed sum, in table order from 0,
+    gives the loop form's scatter bit for bit on random fluxes and terms
+    of mixed signs and magnitudes; the rhs runs (LT_SEG_*) cover every
+    contribution once, are balanced over the 64 lanes, and give the
+    scatter's bits on rows of one run and its value within roundoff on
+    the longer rows"""
+    rng = np.random.default_rng(7)
+    sc_off, sc_row = _generated_table("LT_SC_OFF"), _generated_table(
+        "LT_SC_ROW")
+    sc_sgn = _generated_table("LT_SC_SGN")
+    row_off, row_rs = _generated_table("LT_ROW_OFF"), _generated_table(
+        "LT_ROW_RS")
+    ns = len(row_off) - 1
+    fl = rng.standard_normal(len(sc_off) - 1) * 10.0 ** rng.uniform(
+        -30, 5, len(sc_off) - 1)
+    want = _scatter_rows(fl, sc_off, sc_row, sc_sgn, ns)
+    got = [_signed_sum(row_rs, row_off[i], row_off[i + 1], fl)
+           for i in range(ns)]
+    assert np.array_equal(np.array(got), np.array(want))
+    # the runs
+    lo, hi = _generated_table("LT_SEG_LO"), _generated_table("LT_SEG_HI")
+    sid, rseg = _generated_table("LT_SEG_ID"), _generated_table(
+        "LT_RSEG_OFF")
+    assert len(lo) % 64 == 0
+    cover = np.zeros(len(row_rs), dtype=int)
+    part = {}
+    for s_, a, b in zip(sid, lo, hi):
+        if s_ < 0:
+            assert a == b
+            continue
+        cover[a:b] += 1
+        assert b - a <= 32
+        part[int(s_)] = _signed_sum(row_rs, a, b, fl)
+    assert (cover == 1).all() and sorted(part) == list(range(rseg[-1]))
+    load = (hi - lo).reshape(-1, 64).sum(0)
+    assert load.max() <= int(np.ceil(load.mean())) + 32
+    for i in range(ns):
+        runs = [part[q] for q in range(rseg[i], rseg[i + 1])]
+        v = runs[0]
+        for x in runs[1:]:
+            v = v + x
+        if len(runs) == 1:
+            assert v == want[i]
+        else:
+            scale = sum(abs(float(fl[r if r >= 0 else ~r]))
+                        for r in row_rs[row_off[i]:row_off[i + 1]])
+            assert abs(v - want[i]) <= 1e-14 * scale
+    # the Jacobian: terms scattered through LT_JP in loop order
+    jt_off = _generated_table("LT_JT_OFF")
+    jp = _generated_table("LT_JP")
+    nt = jt_off[-1]
+    dt = rng.standard_normal(nt) * 10.0 ** rng.uniform(-30, 5, nt)
+    npos = int(jp.max()) + 1
+    want = [0.0] * npos
+    m = 0
+    for r in range(len(jt_off) - 1):
+        for t in range(jt_off[r], jt_off[r + 1]):
+            for q in range(sc_off[r], sc_off[r + 1]):
+                want[jp[m]] = (want[jp[m]] + float(dt[t]) if sc_sgn[q] > 0
+                               else want[jp[m]] - float(dt[t]))
+                m += 1
+    pos_off, pos_ts = _generated_table("LT_JPOS_OFF"), _generated_table(
+        "LT_JPOS_TS")
+    got = [_signed_sum(pos_ts, pos_off[q], pos_off[q + 1], dt)
+           for q in range(npos)]
+    assert np.array_equal(np.array(got), np.array(want))
 
 
 def test_functions_vs_reference_generated_code(oracle):
