@@ -249,7 +249,8 @@ int kernel_rows(int ncell, const double*& P, int& pst, const double* Phi,
 template <bool ET>
 int launch_plain(kx::CellsArgs a, DevCache* dc, char* work, hipStream_t s) {
   constexpr int N = neq<ET>();
-  const long S = a.ncell;
+  // the workspace in whole blocks of 64 slots (kx::blocked_slot)
+  const long S = kx::blocked_slots(a.ncell);
   const size_t db =
       sizeof(double) * kx::cell_doubles<kx::PlainProblem<Net, ET>>() * S;
   double* wd = reinterpret_cast<double*>(work);
@@ -263,7 +264,7 @@ int launch_plain(kx::CellsArgs a, DevCache* dc, char* work, hipStream_t s) {
   const int tpb = 128;
   const int blocks = (a.ncell + tpb - 1) / tpb;
   hipLaunchKernelGGL((kx::plain_cells_kernel<Net, ET>), dim3(blocks),
-                     dim3(tpb), 0, s, a, wd, wi, S);
+                     dim3(tpb), 0, s, a, wd, wi);
   return hip_ok(hipGetLastError(), "plain kernel launch") ? 0 : -1;
 }
 
@@ -383,8 +384,9 @@ int solve_kernels(kx::CellsArgs a, const Opts& o, const SolveMem& m,
     budget = (size_t)(0.8 * (double)(fr + dc->work.bytes + dc->iwork.bytes));
   }
   const size_t per = plain_cell_bytes(a.o.evolve_T != 0);
-  long chunk = (long)(budget / per);
-  if (chunk < 1) return kx::FLAG_DEVICE_MEMORY;
+  // whole blocks of 64 slots: at least one
+  long chunk = (long)(budget / per) / kx::SLOT_BLOCK * kx::SLOT_BLOCK;
+  if (chunk < kx::SLOT_BLOCK) return kx::FLAG_DEVICE_MEMORY;
   if (chunk > a.ncell) chunk = a.ncell;
   for (long c0 = 0; c0 < a.ncell; c0 += chunk) {
     kx::CellsArgs b = a;

@@ -92,7 +92,7 @@ __device__ __attribute__((always_inline)) inline void multi_run(
     wave_queue_run<Net>(a, lq, work, iwork, slot >> 6, lw);
   } else {
     (void)wd; (void)wm; (void)pq; (void)lds;
-    lane_queue_run<Net, ET>(a, lq, work, iwork, lanes, slot);
+    lane_queue_run<Net, ET>(a, lq, work, iwork, slot);
   }
 }
 

@@ -166,13 +166,30 @@ KX_HD inline void plain_cell(const CellsArgs& a, int c, double* work,
   if (flag == 0) cell_finish<Net, ET>(a, c, y);
 }
 
+// The one-lane-per-cell kernels' workspace in blocks of 64 slots: element
+// i of slot c at ((c / 64) D + i) 64 + c % 64 (D doubles or ints a slot).
+// A wave's 64 lanes still read 64 consecutive values, and one cell's data
+// span D x 512 bytes of its block rather than D x (slots x 8) bytes of the
+// whole workspace, so a lane left alone at a launch's end touches a few
+// pages, not one page per access
+constexpr long SLOT_BLOCK = 64;
+KX_HD constexpr long blocked_slot(long c, long D) {
+  return (c / SLOT_BLOCK) * SLOT_BLOCK * D + c % SLOT_BLOCK;
+}
+// slots a blocked workspace holds for n cells (whole blocks)
+KX_HD constexpr long blocked_slots(long n) {
+  return (n + SLOT_BLOCK - 1) / SLOT_BLOCK * SLOT_BLOCK;
+}
+
 #if defined(__HIPCC__)
 template <class Net, bool ET>
 __global__ void __launch_bounds__(128)
-plain_cells_kernel(CellsArgs a, double* work, int* iwork, long S) {
+plain_cells_kernel(CellsArgs a, double* work, int* iwork) {
+  using Prob = PlainProblem<Net, ET>;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.ncell) return;
-  plain_cell<Net, ET>(a, c, work + c, iwork + c, S);
+  plain_cell<Net, ET>(a, c, work + blocked_slot(c, cell_doubles<Prob>()),
+                      iwork + blocked_slot(c, Prob::N), SLOT_BLOCK);
 }
 
 // The persistent form of the one-thread-per-cell solve, for the networks
@@ -191,23 +208,31 @@ struct LaneQueue {
 };
 
 template <class Net, bool ET>
-__device__ inline void lane_queue_run(const CellsArgs& a, LaneQueue q,
-                                      double* work, int* iwork, long S,
-                                      long slot) {
+__device__ __attribute__((always_inline)) inline void lane_queue_run(
+    const CellsArgs& a, LaneQueue q, double* work, int* iwork, long slot) {
+  using Prob = PlainProblem<Net, ET>;
+  double* w = work + blocked_slot(slot, cell_doubles<Prob>());
+  int* iw = iwork + blocked_slot(slot, Prob::N);
   for (;;) {
     const int i = atomicAdd(q.counter, 1);
     if (i >= a.ncell) break;
-    plain_cell<Net, ET>(a, q.order[i], work + slot, iwork + slot, S);
+    plain_cell<Net, ET>(a, q.order[i], w, iw, SLOT_BLOCK);
   }
 }
 
+// the waves per SIMD the lane queue's build must allow (its registers:
+// 1 leaves the compiler all of them)
+#ifndef KX_LANE_WAVES_PER_EU
+#define KX_LANE_WAVES_PER_EU 1
+#endif
 template <class Net, bool ET>
 __global__ void __launch_bounds__(128)
+__attribute__((amdgpu_waves_per_eu(KX_LANE_WAVES_PER_EU)))
 lane_queue_kernel(CellsArgs a, LaneQueue q, double* work, int* iwork,
                   long S) {
   const long slot = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (slot >= S) return;
-  lane_queue_run<Net, ET>(a, q, work, iwork, S, slot);
+  lane_queue_run<Net, ET>(a, q, work, iwork, slot);
 }
 
 // ---- a wave per cell (networks above KX_BIG_N species) ------------------
