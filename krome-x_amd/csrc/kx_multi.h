@@ -344,10 +344,13 @@ int krome_multi_solve_device(const krome_multi_batch* b, int nbatch,
   long most = 0;
   {
     int k = 0, r = 0;
+  // the lanes the launch needs: a cell per lane, or a wave per cell
 #define KX_M_CHECK(i, nm, et)                                           \
   if (r == 0 && at[k] && at[k]->ncell > 0) {                            \
     r = params_check<kx::net_##nm>(*at[k]);                             \
-    if (at[k]->ncell > most) most = at[k]->ncell;                       \
+    const long need = at[k]->ncell *                                    \
+        (MultiPart<kx::net_##nm, et>::WAVE ? 64L : 1L);                  \
+    if (need > most) most = need;                                       \
   }                                                                     \
   ++k;
     KX_MULTI_NETS(KX_M_CHECK)
