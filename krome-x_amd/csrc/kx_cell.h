@@ -1122,6 +1122,20 @@ class Cell {
     for (int i = 0; i < NS; i++) {
       const int p0 = Net::sp_rp[i], pd = Net::sp_di[i],
                 p1 = Net::sp_rp[i + 1];
+      if (p0 == pd) {
+        // no eliminations: the row is final as scaled, only its pivot is
+        // checked (the accumulator's value of it would be the same)
+        const double pv = a[pd];
+        if (pv == 0.0 || !(::fabs(pv) <= DBL_MAX)) return 1;
+        if (i + 1 < NS) {
+          const int p = p1 + vb(), e = imin(p, Net::sp_rp[i + 2] - 1);
+          const int ce = Net::sp_ci[e];
+          const double ve = a[e];
+          cn = p < Net::sp_rp[i + 2] ? ce : NS;
+          vn = ve;
+        }
+        continue;
+      }
       lw[cn] = vn;
       for (int p = p0 + VS + vb(); p < p1; p += VS) lw[Net::sp_ci[p]] = a[p];
       if (i + 1 < NS) {
@@ -1139,99 +1153,130 @@ class Cell {
       for (int p = p0 + vb(); p < p1; p += VS) a[p] = lw[Net::sp_ci[p]];
       vsync();
     }
+    // the factors' values again in column order, for the solves
+    KX_GLOBAL double* cl = a + SP_WCOL;
+    KX_GLOBAL double* cu = cl + Net::LT_LCP[NS];
+    for (int e = vb(); e < Net::LT_LCP[NS]; e += VS) cl[e] = a[Net::LT_LCQ[e]];
+    for (int e = vb(); e < Net::LT_UCP[NS]; e += VS) cu[e] = a[Net::LT_UCQ[e]];
+    vsync();
     return 0;
   }
-  // this lane's part of the dot product of the factor entries [p, pe) with
-  // the LDS vector lw (lane-strided), the first entry (if any) given: its
-  // value v0 and column c0; the rest four chunks at a time, loads first
-  KX_HD static double wave_row_dot(const KX_GLOBAL double* a, WaveLds lw,
-                                   int p, int pe, double v0, int c0) {
-    double part = v0 * lw[c0];
-    for (int q = p + VS; q < pe; q += 4 * VS) {
-      int cc[4];
-      double vv[4];
+  // where a wave's factorization keeps its factors' values in column
+  // order: after the values and the serial form's row accumulator
+  static constexpr long SP_WCOL = Net::SP_NNZ + NS;
+  // WG columns of a factor: this lane's first two entries of each (rows
+  // r0, r1, values v0, v1; none: the spare slot, 0) and, for U, the pivots
+  struct WaveCols {
+    int r0[WG], r1[WG];
+    double v0[WG], v1[WG], d[WG];
+  };
+  // columns k0, k0 + s, ... (WG of them, clamped to 0 .. NS - 1) of L
+  // (UPPER false) or U; cv: the part's values in column order.  Every load
+  // unconditional (indices clamped into the part), the selections after
+  template <bool UPPER>
+  KX_HD void wave_cols_load(const KX_GLOBAL double* a,
+                            const KX_GLOBAL double* cv, int k0, int s,
+                            WaveCols& g) const {
+    constexpr const int* CP = UPPER ? Net::LT_UCP : Net::LT_LCP;
+    constexpr const int* CR = UPPER ? Net::LT_UCR : Net::LT_LCR;
+    constexpr int NE = CP[NS];
+#pragma unroll
+    for (int w = 0; w < WG; ++w) {
+      const int k = imax(imin(k0 + s * w, NS - 1), 0);
+      const int e1 = CP[k + 1];
+      const int q0 = CP[k] + vb(), q1 = q0 + VS;
+      const int i0 = imax(imin(q0, e1 - 1), 0), i1 = imax(imin(q1, e1 - 1), 0);
+      const int ra = CR[imin(i0, NE - 1)], rb = CR[imin(i1, NE - 1)];
+      const double va = cv[imin(i0, NE - 1)], vb_ = cv[imin(i1, NE - 1)];
+      g.r0[w] = q0 < e1 ? ra : NS;
+      g.v0[w] = q0 < e1 ? va : 0.0;
+      g.r1[w] = q1 < e1 ? rb : NS;
+      g.v1[w] = q1 < e1 ? vb_ : 0.0;
+      if (UPPER) g.d[w] = a[Net::sp_di[k]];
+    }
+  }
+  // column k's updates of the rows below (L) or above (U) it by x_k
+  template <bool UPPER>
+  KX_HD static void wave_col_apply(WaveLds lw, const KX_GLOBAL double* cv,
+                                   const WaveCols& g, int w, int k,
+                                   double xk) {
+    constexpr const int* CP = UPPER ? Net::LT_UCP : Net::LT_LCP;
+    constexpr const int* CR = UPPER ? Net::LT_UCR : Net::LT_LCR;
+    // distinct rows in a column: the lanes' updates are independent
+    const double y0 = lw[g.r0[w]], y1 = lw[g.r1[w]];
+    lw[g.r0[w]] = y0 - g.v0[w] * xk;
+    lw[g.r1[w]] = y1 - g.v1[w] * xk;
+    // longer columns: four more entries a lane at a time, loads first
+    const int end = CP[k + 1];
+    for (int e0 = CP[k] + 2 * VS; e0 < end; e0 += 4 * VS) {
+      int rr[4];
+      double vv[4], yy[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int qq = imin(q + u * VS, pe - 1);
-        const int cq = Net::sp_ci[qq];
-        vv[u] = a[qq];
-        cc[u] = q + u * VS < pe ? cq : NS;
+        const int e = e0 + u * VS + wave_lane(), ee = imin(e, end - 1);
+        const int re = CR[ee];
+        const double ve = cv[ee];
+        rr[u] = e < end ? re : NS;
+        vv[u] = e < end ? ve : 0.0;
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (q + u * VS < pe) part += vv[u] * lw[cc[u]];
+      for (int u = 0; u < 4; ++u) yy[u] = lw[rr[u]];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) lw[rr[u]] = yy[u] - vv[u] * xk;
     }
-    return part;
   }
-  // b = (c J + I)^-1 b with sp_factor_wave's factors, b in the workspace:
-  // each row's dot product split over the lanes and reduced over the wave
-  // (the order of its additions differs from the serial solve).  The
-  // factors do not change during the solve, so the lane's first entries
-  // (and the pivots) of WG rows are loaded together ahead of their sums
+  // b = (c J + I)^-1 b with sp_factor_wave's factors, b in the workspace,
+  // by columns.  L y = b forward: column k's entries update their rows by
+  // y_k, each row taking its updates in increasing column order, which is
+  // the serial row solve's arithmetic.  U x = y backward over the
+  // non-empty columns: x_k = y_k / u_kk, then its entries update the rows
+  // above (each row's updates in decreasing column order: the last bits
+  // differ from the serial solve); the rows of empty columns are divided
+  // at the end.  The columns' operands are loaded WG columns at a time
   KX_HD void sp_solve_wave(double* bv) {
     static_assert(VS == 64, "a wave per cell has 64 lanes");
     const KX_GLOBAL double* a = global_ptr(A);
-    KX_GLOBAL double* g = global_ptr(bv);
+    const KX_GLOBAL double* cl = a + SP_WCOL;
+    const KX_GLOBAL double* cu = cl + Net::LT_LCP[NS];
+    KX_GLOBAL double* gb = global_ptr(bv);
     const WaveLds lw = this->lw;
     const int l = vb();
     vsync();
-    for (int k = l; k < NS; k += VS) lw[k] = g[Net::sp_perm[k]];
-    lw[NS] = 0.0;  // the spare slot: a lane's missing entry reads 0 there
+    for (int k = l; k < NS; k += VS) lw[k] = gb[Net::sp_perm[k]];
     vsync();
-    for (int i0 = 0; i0 < NS; i0 += WG) {  // L y = b
-      int c[WG];
-      double v[WG];
+    for (int k0 = 0; k0 < NS; k0 += WG) {  // L y = b
+      WaveCols g;
+      wave_cols_load<false>(a, cl, k0, 1, g);
 #pragma unroll
       for (int w = 0; w < WG; ++w) {
-        const int i = imin(i0 + w, NS - 1);
-        const int p = Net::sp_rp[i] + l, pe = Net::sp_di[i];
-        const int e = imin(p, pe);  // pe: the diagonal, in the row
-        const int ce = Net::sp_ci[e];
-        const double ve = a[e];
-        c[w] = p < pe ? ce : NS;
-        v[w] = p < pe ? ve : 0.0;
-      }
-#pragma unroll
-      for (int w = 0; w < WG; ++w) {
-        const int i = i0 + w;
-        if (i < NS) {
-          const double part = wave_row_dot(a, lw, Net::sp_rp[i] + l,
-                                           Net::sp_di[i], v[w], c[w]);
-          const double y = lw[i] - vsum(part);
-          vsync();
-          lw[i] = y;
+        const int k = k0 + w;
+        if (k < NS && Net::LT_LCP[k] < Net::LT_LCP[k + 1]) {
+          const double yk = lw[k];
+          wave_col_apply<false>(lw, cl, g, w, k, yk);
           vsync();
         }
       }
     }
-    for (int i0 = NS - 1; i0 >= 0; i0 -= WG) {  // U x = y
-      int c[WG];
-      double v[WG], d[WG];
+    for (int k0 = NS - 1; k0 >= 0; k0 -= WG) {  // U x = y
+      WaveCols g;
+      wave_cols_load<true>(a, cu, k0, -1, g);
 #pragma unroll
       for (int w = 0; w < WG; ++w) {
-        const int i = imax(i0 - w, 0);
-        const int p = Net::sp_di[i] + 1 + l, pe = Net::sp_rp[i + 1];
-        const int e = imin(p, pe - 1);  // the diagonal at least
-        const int ce = Net::sp_ci[e];
-        const double ve = a[e];
-        c[w] = p < pe ? ce : NS;
-        v[w] = p < pe ? ve : 0.0;
-        d[w] = a[Net::sp_di[i]];
-      }
-#pragma unroll
-      for (int w = 0; w < WG; ++w) {
-        const int i = i0 - w;
-        if (i >= 0) {
-          const double part = wave_row_dot(a, lw, Net::sp_di[i] + 1 + l,
-                                           Net::sp_rp[i + 1], v[w], c[w]);
-          const double x = (lw[i] - vsum(part)) / d[w];
+        const int k = k0 - w;
+        if (k >= 0 && Net::LT_UCP[k] < Net::LT_UCP[k + 1]) {
+          const double xk = lw[k] / g.d[w];
           vsync();
-          lw[i] = x;
+          lw[k] = xk;
+          wave_col_apply<true>(lw, cu, g, w, k, xk);
           vsync();
         }
       }
     }
-    for (int k = l; k < NS; k += VS) g[Net::sp_perm[k]] = lw[k];
+    // the rows of the empty columns: their values are final now
+    for (int k = l; k < NS; k += VS)
+      if (Net::LT_UCP[k] == Net::LT_UCP[k + 1]) lw[k] = lw[k] / a[Net::sp_di[k]];
+    vsync();
+    for (int k = l; k < NS; k += VS) gb[Net::sp_perm[k]] = lw[k];
     vsync();
   }
   // x = L^-1 x (unit lower triangle), x at A + x0 (strided)

@@ -191,7 +191,10 @@ struct WaveProblem {
   static constexpr bool STRIDED = true;
   static constexpr int LANES = 64;
   static constexpr long J_DOUBLES = Net::SP_NNZ;
-  static constexpr long NEWTON_DOUBLES = Net::SP_NNZ + NS;
+  // the factors' values, the serial form's row accumulator, then the
+  // factors' strict lower and upper parts again in column order (the
+  // wave's triangular solves)
+  static constexpr long NEWTON_DOUBLES = 2L * Net::SP_NNZ + NS;
   // the rates, the fluxes, then the Jacobian's terms or the rhs runs'
   // partial sums (never needed together)
   static constexpr long K_DOUBLES =

@@ -505,6 +505,7 @@ def _emit_big_forms(w, net, lt, sp):
     w("  static constexpr int LT_JP[%d] = {%s};"
       % (max(len(jp), 1), ", ".join(str(v) for v in jp) or "0"))
     _emit_gather_tables(w, net, lt, jp)
+    _emit_factor_columns(w, sp)
     w("  template <class NV, class KV, class JV>")
     w("  KX_HD static void jac_species_sparse(const NV& n, const KV& k, "
       "const JV& Jp) {")
@@ -574,6 +575,44 @@ def gather_tables(net, lt, jp):
         sp = lt["jf_sp"][lt["jt_foff"][t]:lt["jt_foff"][t + 1]]
         out["jfw"] += list(sp) + [-1] * (jw - len(sp))
     return out
+
+
+def factor_columns(sp):
+    """the static-pivot factors by columns: for the strict lower part L and
+    the strict upper part U, each column k's entries (row, position in the
+    row-ordered factor values), rows increasing; cp[k] .. cp[k + 1] - 1"""
+    ns = len(sp["di"])
+    cols = {"l": [[] for _ in range(ns)], "u": [[] for _ in range(ns)]}
+    for i in range(ns):
+        for q in range(sp["rp"][i], sp["rp"][i + 1]):
+            k = sp["ci"][q]
+            if k < i:
+                cols["l"][k].append((i, q))
+            elif k > i:
+                cols["u"][k].append((i, q))
+    out = {}
+    for part in ("l", "u"):
+        cp, cr, cq = [0], [], []
+        for lst in cols[part]:
+            cr += [r for r, _ in lst]
+            cq += [q for _, q in lst]
+            cp.append(len(cr))
+        out[part] = (cp, cr, cq)
+    return out
+
+
+def _emit_factor_columns(w, sp):
+    fc = factor_columns(sp)
+    w("  // the static-pivot factors by columns (netgen.factor_columns):")
+    w("  // column k of the strict lower part L (LT_LC*) or upper part U")
+    w("  // (LT_UC*) holds entries LT_?CP[k] .. LT_?CP[k + 1] - 1, each a row")
+    w("  // (LT_?CR, increasing) and its position in the factor values")
+    w("  // (LT_?CQ)")
+    for part, tag in (("l", "LC"), ("u", "UC")):
+        for nm, vals in zip(("P", "R", "Q"), fc[part]):
+            vals = vals or [0]
+            w("  static constexpr int LT_%s%s[%d] = {%s};"
+              % (tag, nm, len(vals), ", ".join(str(v) for v in vals)))
 
 
 # the lanes of a wave and the longest run of a row's contributions one
