@@ -1059,17 +1059,40 @@ class Cell {
   // eliminations in order.  A lane with no U entry of its own updates the
   // accumulator's spare slot NS (the wave's LDS holds NS / 64 + 1 rows)
   static constexpr int WG = 4;
+  // the longest U row off the diagonal, and the longest column of the
+  // strict lower (L) and upper (U) factors
+  static constexpr int sp_max_u_row() {
+    int m = 0;
+    for (int k = 0; k < NS; ++k)
+      if (Net::sp_rp[k + 1] - Net::sp_di[k] - 1 > m)
+        m = Net::sp_rp[k + 1] - Net::sp_di[k] - 1;
+    return m;
+  }
+  template <bool UPPER>
+  static constexpr int sp_max_col() {
+    constexpr const int* CP = UPPER ? Net::LT_UCP : Net::LT_LCP;
+    int m = 0;
+    for (int k = 0; k < NS; ++k)
+      if (CP[k + 1] - CP[k] > m) m = CP[k + 1] - CP[k];
+    return m;
+  }
   KX_HD void wave_elim_group(const KX_GLOBAL double* a, WaveLds lw, int p,
                              int pd) const {
     int k[WG], c0[WG], c1[WG];
     double piv[WG], u0[WG], u1[WG];
+    int dd[WG], ee[WG];
 #pragma unroll
-    for (int w = 0; w < WG; ++w) k[w] = Net::sp_ci[imin(p + w, pd - 1)];
+    for (int w = 0; w < WG; ++w) {
+      const int pp = imin(p + w, pd - 1);
+      k[w] = Net::sp_ci[pp];
+      dd[w] = Net::LT_PD[pp];
+      ee[w] = Net::LT_PE[pp];
+    }
     // every load unconditional (indices clamped into the row: a lane
     // without the entry loads one it does not use), the selections after
 #pragma unroll
     for (int w = 0; w < WG; ++w) {
-      const int d = Net::sp_di[k[w]], end = Net::sp_rp[k[w] + 1];
+      const int d = dd[w], end = ee[w];
       const int q0 = d + 1 + vb(), q1 = q0 + VS;
       const int e0 = imin(q0, end - 1), e1 = imin(q1, end - 1);
       piv[w] = a[d];
@@ -1088,10 +1111,14 @@ class Cell {
         lw[k[w]] = lik;
         lw[c0[w]] = x0 - lik * u0[w];
         lw[c1[w]] = x1 - lik * u1[w];
-        // U rows longer than two lanes' worth (none in most networks)
-        const int end = Net::sp_rp[k[w] + 1];
-        for (int q = Net::sp_di[k[w]] + 1 + 2 * VS + vb(); q < end; q += VS)
-          lw[Net::sp_ci[q]] -= lik * a[q];
+        // U rows longer than two lanes' worth (none in cloud; the loop's
+        // table loads would wait on the LDS writes above)
+        if constexpr (sp_max_u_row() > 2 * VS) {
+          const int end = Net::sp_rp[k[w] + 1];
+          for (int q = Net::sp_di[k[w]] + 1 + 2 * VS + vb(); q < end;
+               q += VS)
+            lw[Net::sp_ci[q]] -= lik * a[q];
+        }
       }
     }
   }
@@ -1207,6 +1234,7 @@ class Cell {
     lw[g.r0[w]] = y0 - g.v0[w] * xk;
     lw[g.r1[w]] = y1 - g.v1[w] * xk;
     // longer columns: four more entries a lane at a time, loads first
+    if constexpr (sp_max_col<UPPER>() > 2 * VS) {
     const int end = CP[k + 1];
     for (int e0 = CP[k] + 2 * VS; e0 < end; e0 += 4 * VS) {
       int rr[4];
@@ -1223,6 +1251,7 @@ class Cell {
       for (int u = 0; u < 4; ++u) yy[u] = lw[rr[u]];
 #pragma unroll
       for (int u = 0; u < 4; ++u) lw[rr[u]] = yy[u] - vv[u] * xk;
+    }
     }
   }
   // b = (c J + I)^-1 b with sp_factor_wave's factors, b in the workspace,

@@ -603,6 +603,17 @@ def factor_columns(sp):
 
 def _emit_factor_columns(w, sp):
     fc = factor_columns(sp)
+    # per factor position p (an entry of row i, column k = ci[p]): the
+    # pivot position of column k and the end of U row k, so an elimination's
+    # operands load without a chain through ci
+    pd = [sp["di"][k] for k in sp["ci"]]
+    pe = [sp["rp"][k + 1] for k in sp["ci"]]
+    w("  // per factor position p (column k = sp_ci[p]): sp_di[k] and")
+    w("  // sp_rp[k + 1]")
+    w("  static constexpr int LT_PD[%d] = {%s};"
+      % (len(pd), ", ".join(str(v) for v in pd)))
+    w("  static constexpr int LT_PE[%d] = {%s};"
+      % (len(pe), ", ".join(str(v) for v in pe)))
     w("  // the static-pivot factors by columns (netgen.factor_columns):")
     w("  // column k of the strict lower part L (LT_LC*) or upper part U")
     w("  // (LT_UC*) holds entries LT_?CP[k] .. LT_?CP[k + 1] - 1, each a row")
