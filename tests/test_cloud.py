@@ -12,7 +12,8 @@ cloud_*.npz, written by tests/golden/make_golden.py cloud):
   own (row, column), decoded through the reference's tables;
 - the wave form's gather tables against the loop form's scatter, bit for
   bit on random fluxes and terms; its rhs runs cover each contribution
-  once, balanced over the lanes;
+  once, balanced over the lanes; its column tables of the factors against
+  the reference's pattern;
 - rates within 1e-9 relative (the same zeros), the rhs within 1e-10 of
   its row scale, the analytic Jacobian within 1e-12 of its scale;
 - whole solves: the oracle against the reference's solve_cells (64 cells
@@ -200,6 +201,34 @@ def test_gather_tables_equal_the_scatter():
     got = [_signed_sum(pos_ts, pos_off[q], pos_off[q + 1], dt)
            for q in range(npos)]
     assert np.array_equal(np.array(got), np.array(want))
+
+
+def test_factor_column_tables():
+    """the wave form's column tables of the static-pivot factors (LT_LC*,
+    LT_UC*) against the reference's row-ordered pattern: every strict
+    lower and upper entry appears once, in its own column, rows increasing
+    within a column, at its position; the per-position tables LT_PD and
+    LT_PE are the pivot position and the U-row end of each entry's
+    column"""
+    sp = golden("cloud_sparse.npz")
+    rp, ci, di = sp["sp_rp"], sp["sp_ci"], sp["sp_di"]
+    ns = len(di)
+    row_of = np.repeat(np.arange(ns), np.diff(rp))
+    for tag, lower in (("LC", True), ("UC", False)):
+        cp = _generated_table("LT_%sP" % tag)
+        cr = _generated_table("LT_%sR" % tag)
+        cq = _generated_table("LT_%sQ" % tag)
+        part = np.where(ci < row_of)[0] if lower else \
+            np.where(ci > row_of)[0]
+        assert len(cp) == ns + 1 and cp[-1] == len(part) == len(cq)
+        assert sorted(cq.tolist()) == part.tolist()
+        for k in range(ns):
+            q = cq[cp[k]:cp[k + 1]]
+            assert (ci[q] == k).all()
+            assert np.array_equal(row_of[q], cr[cp[k]:cp[k + 1]])
+            assert (np.diff(cr[cp[k]:cp[k + 1]]) > 0).all()
+    assert np.array_equal(_generated_table("LT_PD"), di[ci])
+    assert np.array_equal(_generated_table("LT_PE"), rp[ci + 1])
 
 
 def test_functions_vs_reference_generated_code(oracle):
