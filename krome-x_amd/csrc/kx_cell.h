@@ -1122,7 +1122,10 @@ class Cell {
       }
     }
   }
-  KX_HD int sp_factor_wave(double c) {
+  // compiled once and called (KX_HD_CALL): the loop then allocates its
+  // registers by itself, with no spill reloads inside it (measured 11 %
+  // faster on cloud than inlined into the step)
+  KX_HD_CALL int sp_factor_wave(double c) {
     static_assert(NB == 0, "a wave per cell serves fixed-Tgas networks");
     static_assert(VS == 64, "a wave per cell has 64 lanes");
     // the wave's workspace is contiguous (S = 1)
@@ -1262,7 +1265,7 @@ class Cell {
   // above (each row's updates in decreasing column order: the last bits
   // differ from the serial solve); the rows of empty columns are divided
   // at the end.  The columns' operands are loaded WG columns at a time
-  KX_HD void sp_solve_wave(double* bv) {
+  KX_HD_CALL void sp_solve_wave(double* bv) {
     static_assert(VS == 64, "a wave per cell has 64 lanes");
     const KX_GLOBAL double* a = global_ptr(A);
     const KX_GLOBAL double* cl = a + SP_WCOL;
