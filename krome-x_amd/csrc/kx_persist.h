@@ -1344,15 +1344,22 @@ __device__ __attribute__((always_inline)) inline void persist_run(
             break;
           }
           ST(jcur) = true;
-          if (!dq_jac) {
-            // the analytic Jacobian at y (fixed Tgas)
-            double yr[N], Jr[N * N];
-            for (int i = 0; i < N; i++) yr[i] = w.at(V_Y, i, s);
-            for (int k = 0; k < N * N; k++) Jr[k] = 0.0;
-            ST(jr) = P.jac(yr, Jr);
-            for (int k = 0; k < N * N; k++) w.Jat(k, s) = typename M::MT(Jr[k]);
-            pc = P_LSETUP_FORM;
-            break;
+          // the analytic Jacobian here is only the mass-action block at
+          // fixed Tgas (the other cases never reach this kernel with jac 1:
+          // `supported` flags them and the host sends them to the lane
+          // queue), so it is compiled only where that block is the whole
+          // Jacobian
+          if constexpr (!(ET || Net::RATES_DENSITY)) {
+            if (!dq_jac) {
+              double yr[N], Jr[N * N];
+              for (int i = 0; i < N; i++) yr[i] = w.at(V_Y, i, s);
+              for (int k = 0; k < N * N; k++) Jr[k] = 0.0;
+              ST(jr) = P.jac(yr, Jr);
+              for (int k = 0; k < N * N; k++)
+                w.Jat(k, s) = typename M::MT(Jr[k]);
+              pc = P_LSETUP_FORM;
+              break;
+            }
           }
           pc = P_DQ_BEGIN;
           break;
@@ -2176,6 +2183,14 @@ inline int launch_persistent(CellsArgs a, int mixed, int chunk, int wpc,
     (void)mixed; (void)chunk;
     return launch_lane_queue<Net, ET>(a, wpc, s, mem);
   } else {
+    // the analytic Jacobian with Tgas evolved or with density-dependent
+    // rates (its difference quotients are per lane, not the lockstep
+    // machine's shared rhs): the lane queue
+    const bool dq = a.o.jac == 2 || (a.o.jac == 0 && ET);
+    if (!dq && (ET || Net::RATES_DENSITY)) {
+      (void)mixed; (void)chunk;
+      return launch_lane_queue<Net, ET>(a, wpc, s, mem);
+    }
     PersistMem m;
     const int r = mem(persist_need<Net, ET>(a, mixed),
                       persist_order_bytes(a.ncell), m);

@@ -70,8 +70,9 @@ KX_HD inline bool cell_opts(const SolveOpts& o, CellOpts<Prob::N>& co) {
       co.inv_row = Net::I_E;
       co.inv_n = NS;
     }
-    // the analytic Jacobian here is the mass-action block at fixed Tgas
-    return co.dq_jac || !(ET || Net::RATES_DENSITY);
+    // the analytic Jacobian: Problem::jac (every network, Tgas fixed or
+    // evolved)
+    return true;
   }
 }
 

@@ -104,13 +104,91 @@ struct Problem {
     return fin ? 0 : 1;
   }
   KX_HD int rhs(const double* y, double* f) { return rhs_r<double>(y, f); }
-  // the analytic Jacobian at fixed Tgas (mass action at the rates of the
-  // state; J zeroed by the caller)
+  // the analytic Jacobian (runtime/core/solve_cell.hpp:883-1118,
+  // JAC_ANALYTIC; J column-major N x N, zeroed by the caller): the
+  // mass-action species block at the state's rates; for rates that read
+  // the densities, their derivatives d k_r / d n_j by difference quotients
+  // of the rates along each species (the step and cut-off of the
+  // reference's fd_dk_, which it uses for table and extern rates; its
+  // other density-dependent rates it differentiates symbolically), added
+  // as the rhs of those derivatives (the rhs is linear in k); with Tgas
+  // evolved, the Tgas column by central differences of the full rhs and
+  // the Tgas row by difference quotients of the thermal rate with the
+  // rates re-evaluated at the perturbed densities (fd_step_n_)
   KX_HD int jac(const double* y, double* J) {
+    const double Tr = rate_T(y);
     double k[Net::NREA];
-    rates_at(rate_T(y), y, k);
+    rates_at(Tr, y, k);
     Net::jac_species(y, k, J, N);
+    if constexpr (Net::RATES_DENSITY) rate_derivs(Tr, y, k, J);
+    if constexpr (ET) tgas_jac(Tr, y, J);
     return 0;
+  }
+  // J += the density-dependent rates' derivatives (a function of its own:
+  // its rate arrays stay out of the caller's frame)
+  KX_HD_CALL void rate_derivs(double Tr, const double* y, const double* k,
+                              double* J) {
+    {
+      double yp[N], kp[Net::NREA], km[Net::NREA], col[NS];
+      for (int i = 0; i < N; ++i) yp[i] = y[i];
+      for (int j = 0; j < NS; ++j) {
+        const double h = ::fmax(1e-7 * ::fabs(y[j]), 1e-30);
+        yp[j] = y[j] + h;
+        rates_at(Tr, yp, kp);
+        double den = h;
+        if (y[j] > h) {
+          yp[j] = y[j] - h;
+          rates_at(Tr, yp, km);
+          den = 2.0 * h;
+        } else {
+          for (int r = 0; r < Net::NREA; ++r) km[r] = k[r];
+        }
+        yp[j] = y[j];
+        bool any = false;
+        for (int r = 0; r < Net::NREA; ++r) {
+          const double d = kp[r] - km[r];
+          kp[r] = ::fabs(d) <= 1e-15 * ::fabs(k[r]) ? 0.0 : d / den;
+          any = any || kp[r] != 0.0;
+        }
+        if (!any) continue;
+        Net::rhs(y, kp, col);
+        for (int i = 0; i < NS; ++i)
+          if (col[i] != 0.0) J[i + j * N] += col[i];
+      }
+    }
+  }
+  // the Tgas column and row (Tgas evolved)
+  KX_HD_CALL void tgas_jac(double Tr, const double* y, double* J) {
+    {
+      double k[Net::NREA];
+      double yp[N], fp[N], fm[N];
+      for (int i = 0; i < N; ++i) yp[i] = y[i];
+      const double h = ::fmax(1e-6 * ::fabs(y[NS]), 1e-8);
+      const bool t0 = hit_T0;
+      yp[NS] = y[NS] + h;
+      rhs(yp, fp);
+      yp[NS] = y[NS] - h;
+      rhs(yp, fm);
+      yp[NS] = y[NS];
+      hit_T0 = t0;
+      for (int i = 0; i < N; ++i)
+        J[i + NS * N] = (fp[i] - fm[i]) / (2.0 * h);
+      const double Tth = ::fmin(::fmax(y[NS], 2.73), 1e9);
+      double kp[Net::NREA];
+      rates_at(Tr, y, k);
+      const double f0 = Net::thermal(Tth, y, k, p);
+      double s = 0.0;
+      for (int i = 0; i < NS; ++i) s += ::fabs(y[i]);
+      for (int j = 0; j < NS; ++j) {
+        const double hj =
+            ::fmax(::fmax(1e-7 * ::fabs(y[j]), 1e-9 * s), 1e-30);
+        yp[j] = y[j] + hj;
+        rates_at(Tr, yp, kp);
+        const double f1 = Net::thermal(Tth, yp, kp, p);
+        yp[j] = y[j];
+        J[NS + j * N] = (f1 - f0) / hj;
+      }
+    }
   }
   template <class R>
   KX_HD static bool kx_isfinite(R x) {

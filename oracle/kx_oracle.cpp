@@ -8,9 +8,13 @@
 //                         min(max(T, Tcmb), 1e9), thermal terms at
 //                         min(max(T, 2.73), 1e9), T <= 0 or a non-finite
 //                         rhs a recoverable failure)
-//   jac_cell_ (fixed T)   solve_cell.hpp:883-1132 (the analytic species
-//                         block; networks with density-dependent rates
-//                         are refused at fixed T)
+//   jac_cell_             solve_cell.hpp:883-1132 (the analytic species
+//                         block; density-dependent rates' derivatives by
+//                         difference quotients of the rates, where the
+//                         reference differentiates expression rates
+//                         symbolically; with Tgas evolved the Tgas column
+//                         and row by difference quotients as the
+//                         reference's)
 //   solve_cells_percell   runtime/core/solve_cells.hpp:47-118, 130-161
 //   conservation          oracle/kx_conserve.hpp (element totals, the
 //                         post-solve mode 2, consistent advection)
@@ -117,11 +121,84 @@ struct Problem {
       if (!std::isfinite(f[i])) return 1;
     return 0;
   }
+  // the analytic Jacobian (solve_cell.hpp:883-1118, JAC_ANALYTIC; J
+  // column-major, leading dimension N, zeroed by the caller): the
+  // mass-action species block at the state's rates; for rates that read
+  // the densities, their derivatives d k_r / d n_j by difference quotients
+  // of the rates along each species (the step and the cut-off of the
+  // reference's fd_dk_, used there for table and extern rates; the
+  // reference takes the other density-dependent rates' derivatives
+  // symbolically through the aggregates, jacobian_split), added as the
+  // rhs of those derivatives (the rhs is linear in k); with Tgas evolved,
+  // the Tgas column by central differences of the full rhs and the Tgas
+  // row by difference quotients of the thermal rate with the rates
+  // re-evaluated at the perturbed densities (fd_step_n_)
   int jac(double, const double* y, const double*, double* J, double*,
           double*, double*) {
+    constexpr int NS = Net::NSPEC;
+    const int N = NS + (evolve_T ? 1 : 0);
+    const double Tr = rate_T(y);
     double k[Net::NREA + 1];
-    rates(rate_T(y), y, k);
-    Net::jac_species(y, k, J, Net::NSPEC);
+    rates(Tr, y, k);
+    Net::jac_species(y, k, J, N);
+    if constexpr (Net::RATES_DENSITY) {
+      double yp[NS + 1], kp[Net::NREA + 1], km[Net::NREA + 1];
+      double dk[Net::NREA + 1], col[NS + 1];
+      for (int i = 0; i < N; ++i) yp[i] = y[i];
+      for (int j = 0; j < NS; ++j) {
+        const double h = std::fmax(1e-7 * std::fabs(y[j]), 1e-30);
+        yp[j] = y[j] + h;
+        rates(Tr, yp, kp);
+        double den = h;
+        if (y[j] > h) {
+          yp[j] = y[j] - h;
+          rates(Tr, yp, km);
+          den = 2.0 * h;
+        } else {
+          for (int r = 0; r < Net::NREA; ++r) km[r] = k[r];
+        }
+        yp[j] = y[j];
+        bool any = false;
+        for (int r = 0; r < Net::NREA; ++r) {
+          const double d = kp[r] - km[r];
+          dk[r] = std::fabs(d) <= 1e-15 * std::fabs(k[r]) ? 0.0 : d / den;
+          any |= dk[r] != 0.0;
+        }
+        if (!any) continue;
+        Net::rhs(y, dk, col);
+        for (int i = 0; i < NS; ++i)
+          if (col[i] != 0.0) J[i + static_cast<long>(j) * N] += col[i];
+      }
+    }
+    if (evolve_T) {
+      double yp[NS + 1], fp[NS + 1], fm[NS + 1];
+      for (int i = 0; i < N; ++i) yp[i] = y[i];
+      const double h = std::fmax(1e-6 * std::fabs(y[NS]), 1e-8);
+      yp[NS] = y[NS] + h;
+      rhs(0.0, yp, fp);
+      yp[NS] = y[NS] - h;
+      rhs(0.0, yp, fm);
+      yp[NS] = y[NS];
+      for (int i = 0; i < N; ++i)
+        J[i + static_cast<long>(NS) * N] = (fp[i] - fm[i]) / (2.0 * h);
+      if constexpr (Net::THERMAL) {
+        const double Tth = std::fmin(std::fmax(y[NS], 2.73), 1e9);
+        double kp[Net::NREA + 1];
+        rates(Tr, y, k);
+        const double f0 = Net::thermal(Tth, y, k, p);
+        double s = 0.0;
+        for (int i = 0; i < NS; ++i) s += std::fabs(y[i]);
+        for (int j = 0; j < NS; ++j) {
+          const double hj = std::fmax(
+              std::fmax(1e-7 * std::fabs(y[j]), 1e-9 * s), 1e-30);
+          yp[j] = y[j] + hj;
+          rates(Tr, yp, kp);
+          const double f1 = Net::thermal(Tth, yp, kp, p);
+          yp[j] = y[j];
+          J[NS + static_cast<long>(j) * N] = (f1 - f0) / hj;
+        }
+      }
+    }
     return 0;
   }
 };
@@ -144,7 +221,7 @@ int solve_one(double* n, double& T, const double* row, double dt,
   if (evolve_T && !Net::THERMAL) return FLAG_UNSUPPORTED;
   const int jac = static_cast<int>(o[O_JAC]);
   const bool dq = jac == 2 || (jac == 0 && evolve_T);
-  if (!dq && (evolve_T || Net::RATES_DENSITY)) return FLAG_UNSUPPORTED;
+
   Problem<Net> P;
   P.p = Net::params_from_row(row);
   P.evolve_T = evolve_T;
@@ -311,6 +388,28 @@ void jacs(int ncell, const double* n, const double* T, const double* par,
   }
 }
 
+// the analytic Jacobian of the solve (JAC_ANALYTIC) at states n, T:
+// J[ncell][NEQ^2] column-major per cell, NEQ = NSPEC (+ 1 with Tgas
+// evolved, the state's last slot)
+template <class Net>
+void jacs_full(int ncell, const double* n, const double* T, const double* par,
+               int evolve_T, double* J) {
+  constexpr int NS = Net::NSPEC;
+  const int N = NS + (evolve_T ? 1 : 0);
+  for (int c = 0; c < ncell; ++c) {
+    Problem<Net> P;
+    P.p = Net::params_from_row(par + static_cast<size_t>(c) * Net::NROW);
+    P.evolve_T = evolve_T != 0;
+    P.Tfix = T[c];
+    double y[NS + 1];
+    for (int i = 0; i < NS; ++i) y[i] = n[static_cast<size_t>(c) * NS + i];
+    y[NS] = T[c];
+    double* Jc = J + static_cast<size_t>(c) * N * N;
+    for (int k = 0; k < N * N; ++k) Jc[k] = 0.0;
+    P.jac(0.0, y, nullptr, Jc, nullptr, nullptr, nullptr);
+  }
+}
+
 // the species rhs and Jacobian at fixed rates in both generated forms,
 // unrolled and the loop form over index tables: f[2][ncell NSPEC],
 // J[2][ncell NSPEC^2] (column-major per cell)
@@ -474,6 +573,17 @@ int KXO_(jac)(int net, int ncell, const double* n, const double* T,
               const double* par, double* J) {
   return with_net(net, [&](auto tag) {
     jacs<decltype(tag)>(ncell, n, T, par, J);
+    return 0;
+  });
+}
+
+// the solve's analytic Jacobian, Tgas evolved or fixed (see jacs_full())
+int KXO_(jac_full)(int net, int ncell, const double* n, const double* T,
+                   const double* par, int evolve_T, double* J) {
+  return with_net(net, [&](auto tag) {
+    using Net = decltype(tag);
+    if (evolve_T && !Net::THERMAL) return FLAG_UNSUPPORTED;
+    jacs_full<Net>(ncell, n, T, par, evolve_T, J);
     return 0;
   });
 }

@@ -362,6 +362,25 @@ def make_tol(name, exe, meta):
     np.savez_compressed(HERE / ("%s_tol.npz" % name), **out)
 
 
+def make_jac1(name, exes):
+    """the network's solve fixture cells with jac 1 (JAC_ANALYTIC: with
+    Tgas evolved the analytic species block, the density-dependent rates'
+    derivatives, and the Tgas column and row by difference quotients)"""
+    w = cells.WORKLOADS[WORKLOAD[name]](NSOLVE[name])
+    opts = np.array([float(w["opts"][k]) for k in OPT_KEYS])
+    opts[OPT_KEYS.index("jac")] = 1.0
+    out = {"n0": w["n"], "T0": w["T"], "dt": w["dt"], "par": w["par"],
+           "opts": opts}
+    for m, exe in exes.items():
+        n, T, flag, stats, secs = run_solve(exe, w, "%s_jac1_%s" % (name, m),
+                                            opts)
+        out.update({"n_" + m: n, "T_" + m: T, "flag_" + m: flag,
+                    "stats_" + m: stats, "seconds_" + m: secs})
+        print("%s jac 1 %s: flags %s, steps %d" % (name, m, np.unique(flag),
+                                                   stats[:, 0].sum()))
+    np.savez_compressed(HERE / ("%s_jac1.npz" % name), **out)
+
+
 def make_cloud(exes, meta):
     """the uneven batch's cloud network (452 species, 4429 reactions):
     its sparse-LU tables and element tables, the rates, full rhs and
@@ -644,7 +663,8 @@ def main(argv=None):
     only = {a for a in argv if a in names}
     argv = [a for a in argv if a not in names]
     parts = set(argv if argv else ("solve", "funcs", "conserve", "sparse",
-                                   "photo", "adjoint", "tol", "cloud"))
+                                   "photo", "adjoint", "tol", "cloud",
+                                   "jac1"))
     BUILD.mkdir(parents=True, exist_ok=True)
     if "adjoint" in parts and (not only or "primordial3" in only):
         make_adjoint()
@@ -672,7 +692,8 @@ def main(argv=None):
         proj = generate(name)
         make_extra(name, compile_driver(proj, name, "bitwise"))
     for name in ("primordial", "primordial3", "primordialZ"):
-        if not parts & {"solve", "funcs", "conserve", "sparse", "tol"}:
+        if not parts & {"solve", "funcs", "conserve", "sparse", "tol",
+                         "jac1"}:
             break
         if only and name not in only:
             continue
@@ -690,6 +711,8 @@ def main(argv=None):
             make_sparse(name, exes["bitwise"])
         if "tol" in parts and name != "primordialZ":
             make_tol(name, exes["bitwise"], meta)
+        if "jac1" in parts and name == "primordial3":
+            make_jac1(name, exes)
         if not parts & {"solve", "funcs"}:
             continue
         w = cells.WORKLOADS[WORKLOAD[name]](NSOLVE[name])
