@@ -1344,22 +1344,18 @@ __device__ __attribute__((always_inline)) inline void persist_run(
             break;
           }
           ST(jcur) = true;
-          // the analytic Jacobian here is only the mass-action block at
-          // fixed Tgas (the other cases never reach this kernel with jac 1:
-          // `supported` flags them and the host sends them to the lane
-          // queue), so it is compiled only where that block is the whole
-          // Jacobian
-          if constexpr (!(ET || Net::RATES_DENSITY)) {
-            if (!dq_jac) {
-              double yr[N], Jr[N * N];
-              for (int i = 0; i < N; i++) yr[i] = w.at(V_Y, i, s);
-              for (int k = 0; k < N * N; k++) Jr[k] = 0.0;
-              ST(jr) = P.jac(yr, Jr);
-              for (int k = 0; k < N * N; k++)
-                w.Jat(k, s) = typename M::MT(Jr[k]);
-              pc = P_LSETUP_FORM;
-              break;
-            }
+          if (!dq_jac) {
+            // the analytic Jacobian at y: the mass-action block at fixed
+            // Tgas (with Tgas evolved or density-dependent rates the cells
+            // are flagged unsupported before they get here; the host sends
+            // those options to the lane queue)
+            double yr[N], Jr[N * N];
+            for (int i = 0; i < N; i++) yr[i] = w.at(V_Y, i, s);
+            for (int k = 0; k < N * N; k++) Jr[k] = 0.0;
+            ST(jr) = P.jac_block(yr, Jr);
+            for (int k = 0; k < N * N; k++) w.Jat(k, s) = typename M::MT(Jr[k]);
+            pc = P_LSETUP_FORM;
+            break;
           }
           pc = P_DQ_BEGIN;
           break;

@@ -124,6 +124,15 @@ struct Problem {
     if constexpr (ET) tgas_jac(Tr, y, J);
     return 0;
   }
+  // the mass-action species block alone at the state's rates (the whole
+  // analytic Jacobian at fixed Tgas for rates that read only Tgas; J
+  // zeroed by the caller)
+  KX_HD int jac_block(const double* y, double* J) {
+    double k[Net::NREA];
+    rates_at(rate_T(y), y, k);
+    Net::jac_species(y, k, J, N);
+    return 0;
+  }
   // J += the density-dependent rates' derivatives (a function of its own:
   // its rate arrays stay out of the caller's frame)
   KX_HD_CALL void rate_derivs(double Tr, const double* y, const double* k,
