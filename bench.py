@@ -391,34 +391,89 @@ def c3_bench(a):
             b["n"].copy_(n0)
             b["Tgas"].copy_(T0)
 
+    # --c3-slices K: the batch in K launches, launch k over the k-th
+    # contiguous slice of every network's cells (views of the same
+    # tensors, copied in and out around each launch)
+    K = max(1, a.c3_slices)
+    k_lo, k_hi = 0, K
+    if a.c3_slice_range:
+        k_lo, k_hi = (int(x) for x in a.c3_slice_range.split(":"))
+        if not 0 <= k_lo < k_hi <= K:
+            raise SystemExit("--c3-slice-range %s outside 0:%d"
+                             % (a.c3_slice_range, K))
+
+    def covered(nc):
+        """the cells of a network of nc cells that the launches cover"""
+        return nc * k_lo // K, nc * k_hi // K
+
+    def launches():
+        if K == 1:
+            yield batches
+            return
+        for k in range(k_lo, k_hi):
+            sub = []
+            for b in batches:
+                nc = b["Tgas"].shape[0]
+                lo, hi = nc * k // K, nc * (k + 1) // K
+                d = dict(b)
+                for f in ("Tgas", "dt", "par", "flags", "stats"):
+                    d[f] = b[f][lo:hi].contiguous()
+                d["n"] = b["n"][:, lo:hi].contiguous()
+                d["_span"] = (lo, hi)
+                sub.append(d)
+            yield sub
+
+    def solve(mixed):
+        """one pass over the whole batch: (return, kernel ms)"""
+        tot, ret = 0.0, 0
+        for k, sub in enumerate(launches()):
+            r, ms = m.solve_device(sub, mixed)
+            tot += ms
+            ret = ret or r
+            if K > 1:
+                for d, b in zip(sub, batches):
+                    lo, hi = d["_span"]
+                    for f in ("Tgas", "flags", "stats"):
+                        b[f][lo:hi].copy_(d[f])
+                    b["n"][:, lo:hi].copy_(d["n"])
+                print("c3 slice %d/%d mixed=%d: %.1f ms, return %d"
+                      % (k + 1, K, mixed, ms, r), file=sys.stderr,
+                      flush=True)
+        return ret, tot
+
     res = {}
     for mixed in ((0,) if a.fp64 else (0, 1)):
         for _ in range(a.warmup):
             restore()
-            m.solve_device(batches, mixed)
+            solve(mixed)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         kms = []
         for _ in range(a.steps):
             restore()
-            r, ms = m.solve_device(batches, mixed)
+            r, ms = solve(mixed)
             print("c3 launch mixed=%d: %.1f ms, return %d" % (mixed, ms, r),
                   file=sys.stderr, flush=True)
             kms.append(ms)
         torch.cuda.synchronize()
         res[mixed] = (time.perf_counter() - t0, float(np.mean(kms)))
-    ncell = sum(b["Tgas"].shape[0] for b in batches)
-    per = {}
+    ncell, per, saved = 0, {}, {}
     for b in batches:
-        st = b["stats"].cpu().numpy()[:, 0]
-        fl = b["flags"].cpu().numpy()
+        lo, hi = covered(b["Tgas"].shape[0])
+        ncell += hi - lo
+        st = b["stats"].cpu().numpy()[lo:hi, 0]
+        fl = b["flags"].cpu().numpy()[lo:hi]
+        saved[b["network"] + "_steps"] = st
+        saved[b["network"] + "_flags"] = fl
         per[b["network"]] = {
             "cells": int(st.size), "flags_nonzero": int((fl != 0).sum()),
             "steps_p50": float(np.percentile(st, 50)),
             "steps_p99": float(np.percentile(st, 99)),
             "steps_max": int(st.max())}
-    allst = np.concatenate([b["stats"].cpu().numpy()[:, 0]
+    allst = np.concatenate([saved[b["network"] + "_steps"]
                             for b in batches])
+    if a.c3_save:
+        np.savez(a.c3_save, kernel_ms=np.array([res[0][1]]), **saved)
     if 1 not in res:
         res[1] = res[0]
     wall = res[1][0]
@@ -434,6 +489,8 @@ def c3_bench(a):
                                "cloud cells (seed 2, density 1..1e12, dt "
                                "1e8..1e14 s, Tgas fixed) in one launch",
                    "ncell": ncell, "parallelism": "1 GPU",
+                   "launches_per_step": k_hi - k_lo,
+                   "slices": "%d:%d of %d" % (k_lo, k_hi, K),
                    "waves_per_cu": a.waves_per_cu,
                    "queue_chunk": a.queue_chunk},
         "fp64_cells_per_s": ncell * a.steps / res[0][0],
@@ -776,6 +833,16 @@ def main():
     ap.add_argument("--c3-parts", default="",
                     help="c3 workload: the networks to include (comma "
                          "separated; default all three)")
+    ap.add_argument("--c3-slices", type=int, default=1,
+                    help="c3 workload: launches per step, each over a "
+                         "contiguous slice of every network's cells "
+                         "(default 1: the whole batch in one launch)")
+    ap.add_argument("--c3-slice-range", default="",
+                    help="c3 workload: LO:HI, run only slices LO..HI-1 of "
+                         "--c3-slices (the value then counts their cells)")
+    ap.add_argument("--c3-save", default="",
+                    help="c3 workload: write the covered cells' steps and "
+                         "flags per network to this .npz")
     ap.add_argument("--linsol", default="dense", choices=("dense", "sparse"),
                     help="the Newton matrix's LU (sparse: the plain kernel's "
                          "static-pivot sparse LU, linsol 3)")
