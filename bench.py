@@ -87,6 +87,8 @@ METRIC = ("cells/sec to t_end, mixed-prec persistent vs plain FP64 GPU, "
 PEAK_FP64 = 78.6e12       # MI355X FP64 vector (= FP64 matrix), FLOP/s
 PEAK_FP32 = 157.3e12      # MI355X FP32 vector
 PEAK_HBM = 8.0e12         # bytes/s
+# the device the workloads run on (one GPU)
+DEVICE = "cuda:0"
 # algorithmic bytes per cell: read n (9), Tgas, dt, 3 parameters; write n
 # (9), Tgas (8 B each) and the flag (4 B)
 BYTES_PER_CELL = 8 * (9 + 1 + 1 + 3) + 8 * (9 + 1) + 4
@@ -359,7 +361,7 @@ def c3_bench(a):
     import cells
     import krome
 
-    dev = torch.device("cuda:0")
+    dev = torch.device(DEVICE)
     m = krome.load_multi("multi_c3")
     m.set_launch(a.waves_per_cu, a.queue_chunk)
     scale = a.ncell / float(1 << 20)
