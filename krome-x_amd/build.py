@@ -37,10 +37,13 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-result"]
 # a library's optimisation level where it differs from FLAGS' -O3 (the
-# later flag wins): primordial3's lockstep mixed kernel on the 2^20
-# collapse cells of the bench ran 934.9 ms at -O2 against 965.9 ms at -O3,
-# results bitwise equal (the other libraries are not measured at -O2)
-OPT = {"primordial3": "-O2"}
+# later flag wins), each measured with results bitwise equal:
+# primordial3's lockstep mixed kernel on the 2^20 collapse cells of the
+# bench ran 934.9 ms at -O2 against 965.9 ms at -O3; primordial's lockstep
+# kernel 291 against 297-308 ms on 2^18 fixed-Tgas cells and 110.8 against
+# 113.5-114.4 ms on 4096 (its plain kernel unchanged); the other libraries
+# are not measured at -O2
+OPT = {"primordial3": "-O2", "primordial": "-O2"}
 
 
 def lib_path(net):
